@@ -1,0 +1,167 @@
+"""Benchmark: Newton timesteps/s of the device pipeline on the 2000-body mixed
+pile at ~30k contacts per step.
+
+A step is one full timestep on the device-resident scene: grid-SDF contact
+generation, the primal-dual Newton solve (FP64 outer loop, FP32 block-Jacobi
+PCG inner solve, FP64 merit line search) and integration.  The timed region
+starts from the settled state of the seeded pile (the state after the
+workload's 1000 steps), with every input already in HBM.
+
+Prints one JSON line.  The CPU baseline is the FP64 restatement of the same
+algorithm (oracle/), run on the host cores on a bounded sample of steps from
+the same state.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from primal_dual_contacts_miror_amd import pdc, scenes  # noqa: E402
+
+METRIC = "Newton timesteps/sec at 30k contacts; batched scene-steps/sec + p50/p99 latency"
+PEAK_HBM = 8.0e12
+KT_NAMES = {"pcg": pdc.KT_PCG, "contact_terms": pdc.KT_CONTACT_TERMS, "contacts": pdc.KT_CONTACTS}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def settled_state(sc, name, settle_steps=None):
+    path = os.path.join(ROOT, "tests", "golden", "%s_settled.npz" % name)
+    if settle_steps is None and os.path.exists(path):
+        z = np.load(path)
+        return (z["pos"], z["quat"], z["vel"], z["angvel"]), "fixture"
+    n = sc.steps if settle_steps is None else settle_steps
+    log("settling %s on the device (%d steps)" % (name, n))
+    ds = pdc.DynamicSystem(sc)
+    for k in range(n):
+        ds.step(1)
+        if k % 100 == 0:
+            log("settle step", k)
+    st = ds.state()
+    ds.ctx.close()
+    return st, "device-settled"
+
+
+def cpu_baseline(sc, state, budget_s, threads):
+    from oracle import orc
+    p = sc.params
+    p.pcg_tol = 1e-8   # AMGCL's default relative tolerance on the reference's PCG branch
+    o = orc.OracleScene(sc, p)
+    o.set_threads(threads)
+    o.set_state(*state)
+    t0 = time.perf_counter()
+    n = 0
+    contacts = 0
+    while True:
+        st = o.step(1)
+        n += 1
+        contacts += st.n_contacts
+        el = time.perf_counter() - t0
+        log("cpu baseline step %d: %d contacts, %.2fs" % (n, st.n_contacts, el))
+        if el > budget_s or n >= 50:
+            break
+    return {
+        "value": n / el,
+        "unit": "timesteps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d timestep(s) of the FP64 CPU restatement from the same settled state "
+                  "(%d contacts/step avg; block-Jacobi PCG branch, tol 1e-8)" % (n, contacts // n),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="pile_2000")
+    ap.add_argument("--kernel", default="pcg", choices=sorted(KT_NAMES))
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--settle-steps", type=int, default=None,
+                    help="settle on the device for this many steps instead of loading the settled state")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(scene-level sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    sc = scenes.WORKLOADS[args.workload]()
+    state, origin = settled_state(sc, args.workload, args.settle_steps)
+    ds = pdc.DynamicSystem(sc)
+    ds.set_state(*state)
+    log("warmup %d steps" % args.warmup)
+    for _ in range(args.warmup):
+        st = ds.step(1)
+        log("warmup: %d contacts, %d Newton it, %d PCG it" % (st.n_contacts, st.newton_iterations,
+                                                              st.pcg_iterations))
+    ds.ctx.enable_kernel_timing(KT_NAMES[args.kernel])
+    ds.ctx.reset_timings()
+    contacts = []
+    newton = []
+    pcg_it = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = ds.step(1)
+        contacts.append(st.n_contacts)
+        newton.append(st.newton_iterations)
+        pcg_it.append(st.pcg_iterations)
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    ms, launches, nbytes = ds.ctx.kernel_timing()
+    log("timed %d steps in %.3fs: contacts %d..%d, Newton it %.1f, PCG it %.1f" % (
+        args.steps, elapsed, min(contacts), max(contacts), np.mean(newton), np.mean(pcg_it)))
+    achieved = nbytes / (ms * 1e-3) if ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (Newton, residuals, line search) + f32 (PCG inner solve)",
+        "data": "synthetic: seeded 2000-body mixed pile, %s state after %d steps" % (origin, sc.steps),
+        "config": {
+            "workload": "pile_2000: 1000 spheres + 1000 boxes in a 20x20x40 container, "
+                        "96^3 SDF grids, mu 0.6, dt 0.005, settled",
+            "contacts_per_step": float(np.mean(contacts)),
+            "newton_iterations_per_step": float(np.mean(newton)),
+            "pcg_iterations_per_step": float(np.mean(pcg_it)),
+            "bodies": int(sc.n),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": args.kernel,
+            "achieved": achieved / 1e9,
+            "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM,
+            "traffic": None,
+            "launches": launches,
+            "avg_launch_us": 1e3 * ms / max(launches, 1),
+            "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
+        },
+    }
+    ds.ctx.close()
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(sc, state, args.cpu_budget, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

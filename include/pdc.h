@@ -1,0 +1,202 @@
+/*
+ * pdc.h — C ABI of the MI355X per-timestep rigid-body contact pipeline.
+ *
+ * The pipeline is: grid-SDF contact generation, per-contact primal-dual
+ * friction terms, assembly of the 6-DOF block-sparse Newton system, a
+ * block-Jacobi preconditioned CG on 6x6 block-CSR, and the merit line search.
+ * Every entry point is plain C: integer status codes, plain pointers and
+ * sizes, no exceptions cross the boundary.  All host arrays are caller-owned;
+ * the library owns device buffers per context.  A context is not thread-safe:
+ * use one context per host thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   - Solver plugin  ForceSolver::fillForces   src/Solver/ForceSolverBase.h:78-82
+ *                    PrimalDualForceSolver     src/Solver/PrimalDual/PrimalDualForceSolver.cpp:79-130
+ *   - configuration  fillFromYAML              src/Solver/PrimalDual/PrimalDualForceSolver.cpp:650-675,
+ *                                              src/Solver/PrimalDual/NonSmoothForces/NonSmoothContactForce.cpp:320-334
+ *   - contacts       DynamicSystem::contactDetection  src/Dynamics/DynamicSystem.cpp:106-122
+ *                    ContactGeneratorBase::operator() src/Contact/ContactGenerator/ContactGeneratorBase.h:85-95
+ *   - timestep       DynamicSystem::step       src/Dynamics/DynamicSystem.cpp:130-218
+ *   - timings        printTimings              src/Solver/PrimalDual/PrimalDualForceSolver.cpp:636-648
+ */
+#ifndef PDC_H
+#define PDC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+	PDC_OK = 0,
+	PDC_ERR_INVALID_ARGUMENT = 1, /* null pointer, negative size, bad enum */
+	PDC_ERR_CONFIG = 2,           /* configuration rejected (the reference throws std::runtime_error) */
+	PDC_ERR_DEVICE = 3,           /* a HIP call failed */
+	PDC_ERR_NO_DEVICE = 4,        /* no MI355X (gfx950) device visible */
+	PDC_ERR_CAPACITY = 5,         /* request exceeds a fixed capacity or the HBM budget */
+	PDC_ERR_STATE = 6             /* call order violated (e.g. step before bodies) */
+};
+
+/* ---- enums --------------------------------------------------------------- */
+enum { PDC_SHAPE_BOX = 0, PDC_SHAPE_SPHERE = 1 };
+
+/* non_smooth_force.name: "non_smooth_contact" | "only_normal"
+ * (PrimalDualForceSolver.cpp:664-673) */
+enum { PDC_NON_SMOOTH_CONTACT = 0, PDC_ONLY_NORMAL = 1 };
+
+/* Arithmetic of the inner PCG solve.  The Newton outer loop, residuals,
+ * barrier terms, centering parameter and line-search merit are FP64 always. */
+enum { PDC_PCG_FP64 = 0, PDC_PCG_FP32 = 1 };
+
+/* ---- parameters ---------------------------------------------------------- */
+/* Defaults (pdc_default_params) follow the reference's code defaults:
+ *   simulation.timestep 1e-2, gravity (0,-9.81,0)      DynamicSystem.cpp:386, DynamicSystem.h:241
+ *   solver.tolerance 1e-6, solver.iterations 100        PrimalDualForceSolver.cpp:651-652
+ *   static_coeff 0.5, restitution 0.0 (0.5 only_normal) NonSmoothContactForce.cpp:329-333, OnlyNormalForce.cpp:227
+ *   spring_k 1e6, spring_damp 0.01 when spring based   NonSmoothContactForce.cpp:324-327
+ */
+typedef struct pdc_params {
+	double timestep;
+	double gravity[3];
+	int32_t force_model;        /* PDC_NON_SMOOTH_CONTACT | PDC_ONLY_NORMAL */
+	int32_t spring_based_force; /* 0 | 1 */
+	double friction_coeff;      /* global mu (static_coeff) */
+	double restitution;
+	double spring_k;
+	double spring_damp;
+	double tolerance;           /* Newton exit tolerance on exitErr */
+	int32_t iterations;         /* Newton iteration cap */
+	int32_t pcg_precision;      /* PDC_PCG_FP64 | PDC_PCG_FP32 */
+	double pcg_tol;             /* relative residual |r|/|b| of the scaled system */
+	int32_t pcg_max_iter;       /* the reference's maxCGIt = 1000 (PrimalDualForceSolver.cpp:381) */
+	int32_t grid_n;             /* SDF grid nodes per axis (64, 96, ...) */
+	double grid_pad_frac;       /* grid box = shape half-extent * (1 + grid_pad_frac) */
+	double sample_spacing;      /* target spacing of surface samples */
+} pdc_params;
+
+/* Per-call solver statistics (the reference logs these through Corrade::Debug
+ * and accumulates them into its .info timings). */
+typedef struct pdc_solve_stats {
+	int32_t n_contacts;
+	int32_t n_local_bodies;     /* bodies in contact (the local solve set) */
+	int32_t newton_iterations;
+	int32_t pcg_iterations;     /* summed over Newton iterations */
+	int32_t pcg_maxiter_hits;   /* solves that reached pcg_max_iter (Hu *= 1.3 rule) */
+	int32_t gd_fallbacks;       /* line-search failures that fell back to gradient descent */
+	int32_t line_search_trials; /* trial points evaluated */
+	int32_t ties;               /* contact samples within the tie band of the threshold */
+	int32_t n_blocks;           /* 6x6 blocks of the Newton matrix */
+	double exit_error;
+	double char_mass;
+	double char_speed;
+} pdc_solve_stats;
+
+/* Timings in seconds accumulated since the last pdc_reset_timings, measured
+ * with HIP events on the context's stream. */
+typedef struct pdc_timings {
+	double contact_detection;   /* (a) */
+	double force_process;       /* whole solve, like m_forceProcessTimer */
+	double build_system;        /* (b)+(c), like m_systemBuildTimer */
+	double solve_system;        /* (d),      like m_systemSolveTimer */
+	double line_search;         /* (e),      like m_lineSearchTimer */
+	int64_t steps;
+	int64_t newton_iterations;
+	int64_t contacts;
+} pdc_timings;
+
+/* ---- library ------------------------------------------------------------- */
+typedef struct pdc_context pdc_context;
+
+/* Fills *p with the reference defaults listed above. */
+void pdc_default_params(pdc_params* p);
+
+/* Number of usable gfx950 devices (0 when none); never fails. */
+int pdc_device_count(void);
+
+/* Creates a context on HIP device `device`.  Returns PDC_ERR_NO_DEVICE when
+ * no gfx950 device is visible — there is no CPU fallback. */
+int pdc_create(int device, const pdc_params* params, pdc_context** out);
+void pdc_destroy(pdc_context* ctx);
+
+/* Replaces the configuration.  Rejects what the reference rejects
+ * (PrimalDualForceSolver.cpp:655-661) plus out-of-range numbers. */
+int pdc_configure(pdc_context* ctx, const pdc_params* params);
+
+/* Message of the last failing call on this context ("" if none). */
+const char* pdc_last_error(const pdc_context* ctx);
+
+/* ---- bodies (RigidObjectBase inputs: RigidObjectBase.cpp:45-119) ---------- */
+/* Uploads the body table: shape type and dimensions (box half extents, or
+ * sphere radius in dims[3i]), mass, principal moments of inertia in the body
+ * frame, static flags, and whether the body's surface samples are tested
+ * against other bodies.  Builds the FP32 SDF grid and the surface samples of
+ * every body on the device. */
+int pdc_upload_bodies(pdc_context* ctx, int32_t n_bodies,
+                      const int32_t* shape_type, const double* dims,
+                      const double* mass, const double* inertia_diag,
+                      const int32_t* is_static, const int32_t* emit_samples);
+
+/* Body state: pos (3N), quat (4N, x y z w as in the reference's NPZ output),
+ * linear and angular velocity (3N each).  Kept device-resident. */
+int pdc_set_state(pdc_context* ctx, const double* pos, const double* quat_xyzw,
+                  const double* vel, const double* angvel);
+int pdc_get_state(pdc_context* ctx, double* pos, double* quat_xyzw,
+                  double* vel, double* angvel);
+
+/* ---- (a) contact generation ----------------------------------------------- */
+/* Runs broadphase, grid-SDF sample tests, compaction, the reference's
+ * per-pair position merge (DynamicSystem.cpp:341-381) and frame construction
+ * (ContactGeneratorBase.h:21-49) on the device-resident state.  The contact
+ * list stays on the device; *n_contacts receives its length and *n_ties the
+ * number of samples whose signed distance lies within the tie band. */
+int pdc_detect_contacts(pdc_context* ctx, int32_t* n_contacts, int32_t* n_ties);
+
+/* Copies the current contact list to the host in the reference's
+ * ContactInformation layout (ContactInfo.h:138-144): obj_ids (2C, ordered
+ * objId[0] < objId[1]), pos (3C), depth (C), transformation (36C: per contact
+ * transformation[0] then transformation[1], each 6x3 column-major), and
+ * local_force (3C) from the last solve.  Any pointer may be NULL. */
+int pdc_get_contacts(pdc_context* ctx, int32_t capacity, int32_t* obj_ids,
+                     double* pos, double* depth, double* transformation,
+                     double* local_force);
+
+/* ---- (b)-(e) solver: ForceSolver::fillForces drop-in ---------------------- */
+/* Computes contact impulses for the given contacts and state, exactly the
+ * contract of PrimalDualForceSolver::fillForces (PrimalDualForceSolver.cpp:79-130):
+ * out_acc (6N) receives every body's (acceleration, angularAcc) velocity
+ * increment — g*dt and 0 for bodies without contacts — and out_local_force
+ * (3C) receives each contact's force in its local frame times charSpeed *
+ * charMass.  Contact ids use the global numbering of the uploaded bodies. */
+int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts,
+                    const int32_t* obj_ids, const double* transformation,
+                    const double* depth, const double* quat_xyzw,
+                    const double* vel, const double* angvel, double* out_acc,
+                    double* out_local_force, pdc_solve_stats* stats);
+
+/* ---- whole timestep on the device-resident state ---------------------------
+ * DynamicSystem::step without adaptive substeps (DynamicSystem.cpp:130-161,
+ * 207-210): contact detection, fillForces, then semi-implicit integration.
+ * `stats` (may be NULL) receives the statistics of the last step. */
+int pdc_step(pdc_context* ctx, int32_t n_steps, pdc_solve_stats* stats);
+
+int pdc_get_timings(const pdc_context* ctx, pdc_timings* out);
+void pdc_reset_timings(pdc_context* ctx);
+
+/* Kernel-level timing used by the benchmark: when enabled, every launch of
+ * the kernel class `which` is bracketed by HIP events on the context's stream.
+ * pdc_kernel_timing returns the summed milliseconds, the launch count and the
+ * summed algorithmic bytes (compulsory traffic: each input read once, each
+ * output written once) of those launches. */
+enum { PDC_KT_NONE = 0, PDC_KT_SPMV = 1, PDC_KT_CONTACT_TERMS = 2,
+       PDC_KT_ASSEMBLY = 3, PDC_KT_LINE_SEARCH = 4, PDC_KT_CONTACTS = 5,
+       PDC_KT_PCG = 6 };
+int pdc_enable_kernel_timing(pdc_context* ctx, int32_t which);
+int pdc_kernel_timing(pdc_context* ctx, double* total_ms, int64_t* launches,
+                      double* algorithmic_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDC_H */

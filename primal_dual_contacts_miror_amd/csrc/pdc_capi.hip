@@ -1,0 +1,435 @@
+// pdc_capi.hip — the extern "C" boundary of libpdc (include/pdc.h).
+//
+// No exception crosses the boundary: every entry point catches, records the
+// message for pdc_last_error and returns a status code.  Configuration errors
+// are the ones the reference throws std::runtime_error for
+// (PrimalDualForceSolver.cpp:655-661, DynamicSystem.cpp:394-400).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "pdc_internal.h"
+
+using pdc::HipError;
+
+namespace {
+
+struct ConfigError : std::runtime_error {
+	using std::runtime_error::runtime_error;
+};
+
+bool is_gfx950(int dev) {
+	hipDeviceProp_t prop;
+	if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+	return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+void validate(const pdc_params* p) {
+	if (!p) throw ConfigError("null parameters");
+	if (p->force_model != PDC_NON_SMOOTH_CONTACT && p->force_model != PDC_ONLY_NORMAL)
+		throw ConfigError("non_smooth_force.name must be non_smooth_contact or only_normal");
+	if (!(p->timestep > 0) || !std::isfinite(p->timestep)) throw ConfigError("timestep must be > 0");
+	if (p->iterations < 0) throw ConfigError("iterations must be >= 0");
+	if (!(p->tolerance >= 0)) throw ConfigError("tolerance must be >= 0");
+	if (p->pcg_precision != PDC_PCG_FP32 && p->pcg_precision != PDC_PCG_FP64)
+		throw ConfigError("pcg_precision must be PDC_PCG_FP32 or PDC_PCG_FP64");
+	if (!(p->pcg_tol > 0)) throw ConfigError("pcg_tol must be > 0");
+	if (p->pcg_max_iter < 1) throw ConfigError("pcg_max_iter must be >= 1");
+	if (p->grid_n < 4 || p->grid_n > 512) throw ConfigError("grid_n must be in [4, 512]");
+	if (!(p->grid_pad_frac > 0) || p->grid_pad_frac > 10) throw ConfigError("grid_pad_frac must be in (0, 10]");
+	if (!(p->sample_spacing > 0)) throw ConfigError("sample_spacing must be > 0");
+	if (!(p->friction_coeff >= 0)) throw ConfigError("static_coeff must be >= 0");
+	if (p->spring_based_force && !(p->spring_k > 0)) throw ConfigError("spring_k must be > 0");
+}
+
+template <class F>
+int guard(pdc_context* ctx, F fn) {
+	if (!ctx) return PDC_ERR_INVALID_ARGUMENT;
+	try {
+		ctx->err.clear();
+		PDC_HIP(hipSetDevice(ctx->device));
+		fn();
+		return PDC_OK;
+	} catch (const ConfigError& e) {
+		ctx->err = e.what();
+		return PDC_ERR_CONFIG;
+	} catch (const HipError& e) {
+		ctx->err = e.what();
+		return e.code;
+	} catch (const std::invalid_argument& e) {
+		ctx->err = e.what();
+		return PDC_ERR_INVALID_ARGUMENT;
+	} catch (const std::logic_error& e) {
+		ctx->err = e.what();
+		return PDC_ERR_STATE;
+	} catch (const std::bad_alloc&) {
+		ctx->err = "host allocation failed";
+		return PDC_ERR_CAPACITY;
+	} catch (const std::exception& e) {
+		ctx->err = e.what();
+		return PDC_ERR_DEVICE;
+	}
+}
+
+double now_s() {
+	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void free_all(pdc_context* c) {
+	c->b_shape.release(); c->b_static.release(); c->b_emit.release(); c->b_dims.release();
+	c->b_mass.release(); c->b_I0.release(); c->b_pos.release(); c->b_quat.release();
+	c->b_vel.release(); c->b_ang.release(); c->b_acc.release(); c->b_R.release(); c->b_aabb.release();
+	c->b_grid.release(); c->grid_pool.release(); c->b_soff.release(); c->samples.release();
+	c->bp_cnt.release(); c->bp_off.release(); c->pr_a.release(); c->pr_b.release();
+	c->pr_soff.release(); c->pr_nh.release(); c->pr_nc.release(); c->pr_coff.release();
+	c->pr_tmp.release(); c->stage_raw.release(); c->stage_m.release();
+	c->c_id0.release(); c->c_id1.release(); c->c_pos.release(); c->c_depth.release();
+	c->c_T.release(); c->c_lf.release();
+	c->t_key_flag.release(); c->t_pid.release(); c->ap_a.release(); c->ap_b.release();
+	c->ap_start.release(); c->g_flag.release(); c->g2l.release(); c->l2g.release();
+	c->li0.release(); c->li1.release(); c->adj_cnt.release(); c->adj_ptr.release();
+	c->adj_cur.release(); c->adj_nb.release(); c->adj_pair.release(); c->a_cnt.release();
+	c->a_ptr.release(); c->a_col.release(); c->a_pair.release(); c->a_diag.release();
+	c->l_static.release(); c->l_v.release(); c->l_vold.release(); c->l_Ms.release();
+	c->l_Minv.release(); c->l_vphys.release(); c->l_speed.release(); c->l_ru.release();
+	c->l_rhs.release(); c->l_D.release(); c->l_du.release(); c->l_gdu.release();
+	c->s_K.release(); c->s_f.release(); c->s_lam.release(); c->s_c.release(); c->s_g.release();
+	c->s_H.release(); c->s_rf.release(); c->s_rl.release(); c->s_sf.release(); c->s_invC.release();
+	c->s_W.release(); c->s_kf.release(); c->s_df.release(); c->s_dl.release(); c->s_ft.release();
+	c->s_lt.release(); c->partials.release();
+	c->A32.release(); c->P32.release(); c->b32.release(); c->x32.release(); c->pcg32.release();
+	c->A64.release(); c->P64.release(); c->b64.release(); c->x64.release(); c->pcg64.release();
+	c->pcg_part.release(); c->d_sc.release();
+	c->c_stride = 0;
+}
+
+void require_bodies(pdc_context* ctx) {
+	if (ctx->N <= 0) throw std::logic_error("no bodies uploaded (call pdc_upload_bodies first)");
+}
+
+}  // namespace
+
+extern "C" {
+
+void pdc_default_params(pdc_params* p) {
+	if (!p) return;
+	std::memset(p, 0, sizeof(*p));
+	p->timestep = 1e-2;
+	p->gravity[0] = 0.0; p->gravity[1] = -9.81; p->gravity[2] = 0.0;
+	p->force_model = PDC_NON_SMOOTH_CONTACT;
+	p->spring_based_force = 0;
+	p->friction_coeff = 0.5;
+	p->restitution = 0.0;
+	p->spring_k = 1e6;
+	p->spring_damp = 0.01;
+	p->tolerance = 1e-6;
+	p->iterations = 100;
+	p->pcg_precision = PDC_PCG_FP64;
+	p->pcg_tol = 1e-8;
+	p->pcg_max_iter = 1000;
+	p->grid_n = 64;
+	p->grid_pad_frac = 0.1;
+	p->sample_spacing = 0.1;
+}
+
+int pdc_device_count(void) {
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+	int ok = 0;
+	for (int d = 0; d < n; ++d) ok += is_gfx950(d) ? 1 : 0;
+	return ok;
+}
+
+int pdc_create(int device, const pdc_params* params, pdc_context** out) {
+	if (!out) return PDC_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n || !is_gfx950(device))
+		return PDC_ERR_NO_DEVICE;
+	pdc_context* ctx = new pdc_context();
+	ctx->device = device;
+	pdc_params p;
+	if (params) p = *params;
+	else pdc_default_params(&p);
+	int rc = guard(ctx, [&] {
+		validate(&p);
+		ctx->p = p;
+		PDC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+		ctx->d_sc.ensure(1);
+		PDC_HIP(hipHostMalloc((void**)&ctx->h_sc, sizeof(pdc::Scalars), hipHostMallocDefault));
+		std::memset(ctx->h_sc, 0, sizeof(pdc::Scalars));
+		PDC_HIP(hipMemsetAsync(ctx->d_sc.p, 0, sizeof(pdc::Scalars), ctx->stream));
+		hipDeviceProp_t prop;
+		PDC_HIP(hipGetDeviceProperties(&prop, device));
+		ctx->cus = prop.multiProcessorCount;
+		int coop = 0;
+		PDC_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
+		if (!coop) throw HipError("device does not support cooperative launches", PDC_ERR_NO_DEVICE);
+		PDC_HIP(hipStreamSynchronize(ctx->stream));
+	});
+	if (rc != PDC_OK) {
+		pdc_destroy(ctx);
+		return rc;
+	}
+	*out = ctx;
+	return PDC_OK;
+}
+
+void pdc_destroy(pdc_context* ctx) {
+	if (!ctx) return;
+	(void)hipSetDevice(ctx->device);
+	if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+	free_all(ctx);
+	for (auto e : ctx->kt.pool) (void)hipEventDestroy(e);
+	if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
+	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+	delete ctx;
+}
+
+int pdc_configure(pdc_context* ctx, const pdc_params* params) {
+	return guard(ctx, [&] {
+		validate(params);
+		const bool regrid = ctx->N > 0 && (params->grid_n != ctx->p.grid_n ||
+		                                   params->grid_pad_frac != ctx->p.grid_pad_frac ||
+		                                   params->sample_spacing != ctx->p.sample_spacing);
+		if (regrid)
+			throw ConfigError("grid_n, grid_pad_frac and sample_spacing are fixed once bodies are uploaded");
+		ctx->p = *params;
+	});
+}
+
+const char* pdc_last_error(const pdc_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pdc_upload_bodies(pdc_context* ctx, int32_t n, const int32_t* shape_type, const double* dims,
+                      const double* mass, const double* inertia_diag, const int32_t* is_static,
+                      const int32_t* emit_samples) {
+	return guard(ctx, [&] {
+		if (n < 0 || (n > 0 && (!shape_type || !dims || !mass || !inertia_diag || !is_static)))
+			throw std::invalid_argument("pdc_upload_bodies: null array or negative count");
+		for (int i = 0; i < n; ++i) {
+			if (shape_type[i] != PDC_SHAPE_BOX && shape_type[i] != PDC_SHAPE_SPHERE)
+				throw std::invalid_argument("pdc_upload_bodies: unknown shape type");
+			for (int a = 0; a < 3; ++a)
+				if (!(dims[3 * i + a] > 0) && !(shape_type[i] == PDC_SHAPE_SPHERE && a > 0))
+					throw std::invalid_argument("pdc_upload_bodies: dimensions must be > 0");
+			if (!(mass[i] > 0)) throw std::invalid_argument("pdc_upload_bodies: mass must be > 0");
+			for (int a = 0; a < 3; ++a)
+				if (!(inertia_diag[3 * i + a] > 0))
+					throw std::invalid_argument("pdc_upload_bodies: inertia must be > 0");
+		}
+		pdc::upload_bodies(ctx, n, shape_type, dims, mass, inertia_diag, is_static, emit_samples);
+		// identity state until the caller sets one
+		std::vector<double> z3(3 * (size_t)n, 0.0), q(4 * (size_t)n, 0.0);
+		for (int i = 0; i < n; ++i) q[4 * i + 3] = 1.0;
+		if (n > 0) {
+			PDC_HIP(hipMemcpyAsync(ctx->b_pos.p, z3.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+			PDC_HIP(hipMemcpyAsync(ctx->b_quat.p, q.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+			PDC_HIP(hipMemcpyAsync(ctx->b_vel.p, z3.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+			PDC_HIP(hipMemcpyAsync(ctx->b_ang.p, z3.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+			PDC_HIP(hipStreamSynchronize(ctx->stream));
+		}
+		ctx->C = 0;
+	});
+}
+
+int pdc_set_state(pdc_context* ctx, const double* pos, const double* quat_xyzw, const double* vel,
+                  const double* angvel) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		const size_t n = (size_t)ctx->N;
+		hipStream_t s = ctx->stream;
+		if (pos) PDC_HIP(hipMemcpyAsync(ctx->b_pos.p, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+		if (quat_xyzw) PDC_HIP(hipMemcpyAsync(ctx->b_quat.p, quat_xyzw, 4 * n * sizeof(double), hipMemcpyHostToDevice, s));
+		if (vel) PDC_HIP(hipMemcpyAsync(ctx->b_vel.p, vel, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+		if (angvel) PDC_HIP(hipMemcpyAsync(ctx->b_ang.p, angvel, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+		PDC_HIP(hipStreamSynchronize(s));
+	});
+}
+
+int pdc_get_state(pdc_context* ctx, double* pos, double* quat_xyzw, double* vel, double* angvel) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		const size_t n = (size_t)ctx->N;
+		hipStream_t s = ctx->stream;
+		if (pos) PDC_HIP(hipMemcpyAsync(pos, ctx->b_pos.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+		if (quat_xyzw) PDC_HIP(hipMemcpyAsync(quat_xyzw, ctx->b_quat.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+		if (vel) PDC_HIP(hipMemcpyAsync(vel, ctx->b_vel.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+		if (angvel) PDC_HIP(hipMemcpyAsync(angvel, ctx->b_ang.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+		PDC_HIP(hipStreamSynchronize(s));
+	});
+}
+
+int pdc_detect_contacts(pdc_context* ctx, int32_t* n_contacts, int32_t* n_ties) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		const double t0 = now_s();
+		pdc::detect_contacts(ctx);
+		PDC_HIP(hipStreamSynchronize(ctx->stream));
+		ctx->timings.contact_detection += now_s() - t0;
+		if (n_contacts) *n_contacts = ctx->C;
+		if (n_ties) *n_ties = ctx->n_ties;
+	});
+}
+
+int pdc_get_contacts(pdc_context* ctx, int32_t capacity, int32_t* obj_ids, double* pos, double* depth,
+                     double* transformation, double* local_force) {
+	return guard(ctx, [&] {
+		const int C = ctx->C;
+		if (capacity < C) throw std::invalid_argument("pdc_get_contacts: capacity smaller than the contact count");
+		if (C == 0) return;
+		const size_t cs = (size_t)ctx->c_stride;
+		hipStream_t s = ctx->stream;
+		std::vector<int> i0(C), i1(C);
+		std::vector<double> P(3 * (size_t)C), Dp(C), T(36 * (size_t)C), L(3 * (size_t)C);
+		PDC_HIP(hipMemcpyAsync(i0.data(), ctx->c_id0.p, C * sizeof(int), hipMemcpyDeviceToHost, s));
+		PDC_HIP(hipMemcpyAsync(i1.data(), ctx->c_id1.p, C * sizeof(int), hipMemcpyDeviceToHost, s));
+		for (int k = 0; k < 3; ++k) {
+			PDC_HIP(hipMemcpyAsync(&P[k * (size_t)C], ctx->c_pos.p + k * cs, C * sizeof(double), hipMemcpyDeviceToHost, s));
+			PDC_HIP(hipMemcpyAsync(&L[k * (size_t)C], ctx->c_lf.p + k * cs, C * sizeof(double), hipMemcpyDeviceToHost, s));
+		}
+		PDC_HIP(hipMemcpyAsync(Dp.data(), ctx->c_depth.p, C * sizeof(double), hipMemcpyDeviceToHost, s));
+		for (int k = 0; k < 36; ++k)
+			PDC_HIP(hipMemcpyAsync(&T[k * (size_t)C], ctx->c_T.p + k * cs, C * sizeof(double), hipMemcpyDeviceToHost, s));
+		PDC_HIP(hipStreamSynchronize(s));
+		for (int i = 0; i < C; ++i) {
+			if (obj_ids) { obj_ids[2 * i] = i0[i]; obj_ids[2 * i + 1] = i1[i]; }
+			for (int k = 0; k < 3; ++k) {
+				if (pos) pos[3 * i + k] = P[k * (size_t)C + i];
+				if (local_force) local_force[3 * i + k] = L[k * (size_t)C + i];
+			}
+			if (depth) depth[i] = Dp[i];
+			if (transformation)
+				for (int k = 0; k < 36; ++k) transformation[36 * (size_t)i + k] = T[k * (size_t)C + i];
+		}
+	});
+}
+
+int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids,
+                    const double* transformation, const double* depth, const double* quat_xyzw,
+                    const double* vel, const double* angvel, double* out_acc, double* out_local_force,
+                    pdc_solve_stats* stats) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		const int C = n_contacts;
+		const int N = ctx->N;
+		if (C < 0 || (C > 0 && (!obj_ids || !transformation)) || !quat_xyzw || !vel || !angvel)
+			throw std::invalid_argument("pdc_fill_forces: null array or negative count");
+		for (int i = 0; i < 2 * C; ++i)
+			if (obj_ids[i] < 0 || obj_ids[i] >= N)
+				throw std::invalid_argument("pdc_fill_forces: contact body id out of range");
+		for (int i = 0; i < C; ++i)
+			if (obj_ids[2 * i] == obj_ids[2 * i + 1])
+				throw std::invalid_argument("pdc_fill_forces: contact between a body and itself");
+		const double t0 = now_s();
+		hipStream_t s = ctx->stream;
+		PDC_HIP(hipMemcpyAsync(ctx->b_quat.p, quat_xyzw, 4 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
+		PDC_HIP(hipMemcpyAsync(ctx->b_vel.p, vel, 3 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
+		PDC_HIP(hipMemcpyAsync(ctx->b_ang.p, angvel, 3 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
+		// contacts grouped by unordered body pair (stable), as the topology needs
+		std::vector<int> perm(C);
+		std::iota(perm.begin(), perm.end(), 0);
+		auto key = [&](int i) {
+			const int a = std::min(obj_ids[2 * i], obj_ids[2 * i + 1]);
+			const int b = std::max(obj_ids[2 * i], obj_ids[2 * i + 1]);
+			return std::make_pair(a, b);
+		};
+		std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return key(x) < key(y); });
+		ctx->C = 0;
+		if (C > 0) {
+			pdc::ensure_contact_capacity(ctx, C);
+			const size_t cs = (size_t)ctx->c_stride;
+			std::vector<int> i0(C), i1(C);
+			std::vector<double> Dp(C), T(36 * (size_t)C);
+			for (int k = 0; k < C; ++k) {
+				const int i = perm[k];
+				i0[k] = obj_ids[2 * i];
+				i1[k] = obj_ids[2 * i + 1];
+				Dp[k] = depth ? depth[i] : 0.0;
+				for (int c = 0; c < 36; ++c) T[c * (size_t)C + k] = transformation[36 * (size_t)i + c];
+			}
+			PDC_HIP(hipMemcpyAsync(ctx->c_id0.p, i0.data(), C * sizeof(int), hipMemcpyHostToDevice, s));
+			PDC_HIP(hipMemcpyAsync(ctx->c_id1.p, i1.data(), C * sizeof(int), hipMemcpyHostToDevice, s));
+			PDC_HIP(hipMemcpyAsync(ctx->c_depth.p, Dp.data(), C * sizeof(double), hipMemcpyHostToDevice, s));
+			for (int c = 0; c < 36; ++c)
+				PDC_HIP(hipMemcpyAsync(ctx->c_T.p + c * cs, &T[c * (size_t)C], C * sizeof(double),
+				                       hipMemcpyHostToDevice, s));
+			PDC_HIP(hipStreamSynchronize(s));
+			ctx->C = C;
+		}
+		pdc_solve_stats st{};
+		pdc::solve(ctx, ctx->p.timestep, ctx->p.gravity, &st);
+		if (out_acc)
+			PDC_HIP(hipMemcpyAsync(out_acc, ctx->b_acc.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s));
+		std::vector<double> L(3 * (size_t)C);
+		if (C > 0 && out_local_force) {
+			const size_t cs = (size_t)ctx->c_stride;
+			for (int k = 0; k < 3; ++k)
+				PDC_HIP(hipMemcpyAsync(&L[k * (size_t)C], ctx->c_lf.p + k * cs, C * sizeof(double),
+				                       hipMemcpyDeviceToHost, s));
+		}
+		PDC_HIP(hipStreamSynchronize(s));
+		if (C > 0 && out_local_force)
+			for (int k = 0; k < C; ++k)
+				for (int c = 0; c < 3; ++c) out_local_force[3 * perm[k] + c] = L[c * (size_t)C + k];
+		ctx->last = st;
+		if (stats) *stats = st;
+		ctx->timings.force_process += now_s() - t0;
+		ctx->timings.steps += 1;
+	});
+}
+
+int pdc_step(pdc_context* ctx, int32_t n_steps, pdc_solve_stats* stats) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		if (n_steps < 0) throw std::invalid_argument("pdc_step: negative step count");
+		for (int k = 0; k < n_steps; ++k) {
+			const double t0 = now_s();
+			pdc::detect_contacts(ctx);
+			const double t1 = now_s();
+			pdc_solve_stats st{};
+			pdc::solve(ctx, ctx->p.timestep, ctx->p.gravity, &st);
+			pdc::integrate(ctx, ctx->p.timestep);
+			PDC_HIP(hipStreamSynchronize(ctx->stream));
+			const double t2 = now_s();
+			ctx->timings.contact_detection += t1 - t0;
+			ctx->timings.force_process += t2 - t1;
+			ctx->timings.steps += 1;
+			ctx->last = st;
+		}
+		if (stats) *stats = ctx->last;
+	});
+}
+
+int pdc_get_timings(const pdc_context* ctx, pdc_timings* out) {
+	if (!ctx || !out) return PDC_ERR_INVALID_ARGUMENT;
+	*out = ctx->timings;
+	return PDC_OK;
+}
+
+void pdc_reset_timings(pdc_context* ctx) {
+	if (ctx) std::memset(&ctx->timings, 0, sizeof(ctx->timings));
+}
+
+int pdc_enable_kernel_timing(pdc_context* ctx, int32_t which) {
+	return guard(ctx, [&] {
+		if (which < PDC_KT_NONE || which > PDC_KT_PCG) throw std::invalid_argument("unknown kernel class");
+		pdc::kt_collect(ctx);
+		ctx->kt.which = which;
+		ctx->kt.total_ms = 0;
+		ctx->kt.launches = 0;
+		ctx->kt.bytes = 0;
+	});
+}
+
+int pdc_kernel_timing(pdc_context* ctx, double* total_ms, int64_t* launches, double* algorithmic_bytes) {
+	return guard(ctx, [&] {
+		pdc::kt_collect(ctx);
+		if (total_ms) *total_ms = ctx->kt.total_ms;
+		if (launches) *launches = ctx->kt.launches;
+		if (algorithmic_bytes) *algorithmic_bytes = ctx->kt.bytes;
+	});
+}
+
+}  // extern "C"

@@ -1,0 +1,555 @@
+// pdc_contacts.hip — (a) grid-SDF contact generation on the device.
+//
+// Replaces the reference's FCL broadphase + narrowphase and its per-pair merge
+// (src/Dynamics/DynamicSystem.cpp:219-228, 247-298, 317-383;
+// src/Contact/ContactGenerator/ContactGeneratorBase.h:21-68).  Per step:
+//   1. per body: rotation and world AABB of its grid box;
+//   2. broadphase: every pair a < b (not both static) whose AABBs overlap, in
+//      (a, b) order (two passes + scan, so the order is deterministic);
+//   3. one workgroup per pair: the surface samples of a against the grid of b
+//      and of b against the grid of a; hits (signed distance < 0) are compacted
+//      in sample order with wavefront ballots; the hits are sorted by position
+//      (lexicographic) in LDS and chain-merged (|dp|^2 <= 1e-8: summed normal,
+//      minimum depth), the reference's merge rule;
+//   4. scan of per-pair counts; one workgroup per pair writes the final
+//      contact list (ids, position, depth, both 6x3 frames) in pair order.
+// Normal convention (MeshMeshContactGenerator.cpp:43-55): a sample of body a
+// inside b gives normal -grad(sd_b); a sample of b inside a gives +grad(sd_a).
+#include <cmath>
+
+#include "pdc_internal.h"
+#include "pdc_reduce.h"
+
+namespace pdc {
+
+// ---------------------------------------------------------------------------
+// host: surface samples (body frame)
+//   box:    each face (axis a, side -/+) carries a cell-centred nu x nw grid,
+//           n = max(1, round(edge / spacing)), in the order axis, side, iu, iw
+//   sphere: Fibonacci lattice of max(8, round(4 pi r^2 / spacing^2)) points
+// ---------------------------------------------------------------------------
+static void make_samples(int type, const double* dims, double spacing, std::vector<double>& out) {
+	if (type == PDC_SHAPE_SPHERE) {
+		const double r = dims[0];
+		const double PI = 3.14159265358979323846;
+		long n = std::lround(4.0 * PI * r * r / (spacing * spacing));
+		if (n < 8) n = 8;
+		const double ga = PI * (3.0 - std::sqrt(5.0));
+		for (long i = 0; i < n; ++i) {
+			const double y = 1.0 - (2.0 * (double)i + 1.0) / (double)n;
+			const double rho = std::sqrt(std::max(0.0, 1.0 - y * y));
+			const double phi = (double)i * ga;
+			out.push_back(r * (rho * std::cos(phi)));
+			out.push_back(r * y);
+			out.push_back(r * (rho * std::sin(phi)));
+		}
+		return;
+	}
+	for (int a = 0; a < 3; ++a) {
+		const int u = (a + 1) % 3, w = (a + 2) % 3;
+		long nu = std::lround(2.0 * dims[u] / spacing);
+		long nw = std::lround(2.0 * dims[w] / spacing);
+		if (nu < 1) nu = 1;
+		if (nw < 1) nw = 1;
+		const double du = 2.0 * dims[u] / (double)nu;
+		const double dw = 2.0 * dims[w] / (double)nw;
+		for (int s = -1; s <= 1; s += 2)
+			for (long iu = 0; iu < nu; ++iu)
+				for (long iw = 0; iw < nw; ++iw) {
+					double p[3];
+					p[a] = (double)s * dims[a];
+					p[u] = -dims[u] + ((double)iu + 0.5) * du;
+					p[w] = -dims[w] + ((double)iw + 0.5) * dw;
+					out.push_back(p[0]);
+					out.push_back(p[1]);
+					out.push_back(p[2]);
+				}
+	}
+}
+
+// One thread per grid node; blockIdx.y = body.
+__global__ __launch_bounds__(256) void k_build_grids(const GridDesc* __restrict__ grids, const int* __restrict__ shape,
+                              const double* __restrict__ dims, float* __restrict__ pool) {
+	const int b = blockIdx.y;
+	const GridDesc g = grids[b];
+	const int64_t n3 = (int64_t)g.n * g.n * g.n;
+	const double d[3] = {dims[3 * b], dims[3 * b + 1], dims[3 * b + 2]};
+	const int type = shape[b];
+	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n3;
+	     t += (int64_t)gridDim.x * blockDim.x) {
+		const int i = (int)(t % g.n);
+		const int j = (int)((t / g.n) % g.n);
+		const int k = (int)(t / ((int64_t)g.n * g.n));
+		const double px = g.lo[0] + (double)i * g.sp[0];
+		const double py = g.lo[1] + (double)j * g.sp[1];
+		const double pz = g.lo[2] + (double)k * g.sp[2];
+		pool[g.offset + t] = (float)analytic_sdf(type, d, px, py, pz);
+	}
+}
+
+void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,
+                   const double* inertia, const int* is_static, const int* emit) {
+	ctx->N = n;
+	const size_t N = (size_t)n;
+	ctx->b_shape.ensure(N); ctx->b_static.ensure(N); ctx->b_emit.ensure(N);
+	ctx->b_dims.ensure(3 * N); ctx->b_mass.ensure(N); ctx->b_I0.ensure(3 * N);
+	ctx->b_pos.ensure(3 * N); ctx->b_quat.ensure(4 * N); ctx->b_vel.ensure(3 * N);
+	ctx->b_ang.ensure(3 * N); ctx->b_acc.ensure(6 * N);
+	ctx->b_R.ensure(9 * N); ctx->b_aabb.ensure(6 * N);
+	ctx->b_grid.ensure(N); ctx->b_soff.ensure(N + 1);
+	std::vector<int> em(N);
+	for (size_t i = 0; i < N; ++i) em[i] = emit ? (emit[i] != 0) : 1;
+	hipStream_t s = ctx->stream;
+	PDC_HIP(hipMemcpyAsync(ctx->b_shape.p, shape, N * sizeof(int), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_static.p, is_static, N * sizeof(int), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_emit.p, em.data(), N * sizeof(int), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_dims.p, dims, 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_mass.p, mass, N * sizeof(double), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_I0.p, inertia, 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+	ctx->h_static.assign(is_static, is_static + N);
+	// grids
+	std::vector<GridDesc> gd(N);
+	int64_t off = 0;
+	const int gn = ctx->p.grid_n;
+	for (size_t b = 0; b < N; ++b) {
+		double e[3];
+		if (shape[b] == PDC_SHAPE_SPHERE) e[0] = e[1] = e[2] = dims[3 * b];
+		else { e[0] = dims[3 * b]; e[1] = dims[3 * b + 1]; e[2] = dims[3 * b + 2]; }
+		for (int a = 0; a < 3; ++a) {
+			const double h = e[a] * (1.0 + ctx->p.grid_pad_frac);
+			gd[b].half[a] = h;
+			gd[b].lo[a] = -h;
+			gd[b].sp[a] = (2.0 * h) / (double)(gn - 1);
+		}
+		gd[b].n = gn;
+		gd[b].offset = off;
+		off += (int64_t)gn * gn * gn;
+	}
+	ctx->grid_pool.ensure((size_t)off);
+	PDC_HIP(hipMemcpyAsync(ctx->b_grid.p, gd.data(), N * sizeof(GridDesc), hipMemcpyHostToDevice, s));
+	// samples
+	std::vector<double> smp;
+	std::vector<int> soff(N + 1, 0);
+	for (size_t b = 0; b < N; ++b) {
+		soff[b] = (int)(smp.size() / 3);
+		make_samples(shape[b], &dims[3 * b], ctx->p.sample_spacing, smp);
+	}
+	soff[N] = (int)(smp.size() / 3);
+	ctx->n_samples = soff[N];
+	ctx->h_soff = soff;
+	ctx->samples.ensure(std::max<size_t>(smp.size(), 3));
+	// tie band: 1e-7 of the finest grid spacing of the scene
+	double finest = 1e300;
+	for (size_t b = 0; b < N; ++b)
+		for (int a = 0; a < 3; ++a) finest = std::min(finest, gd[b].sp[a]);
+	ctx->tie_band = N ? 1e-7 * finest : 0.0;
+	if (!smp.empty())
+		PDC_HIP(hipMemcpyAsync(ctx->samples.p, smp.data(), smp.size() * sizeof(double),
+		                       hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_soff.p, soff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+	if (N > 0) {
+		const int64_t n3 = (int64_t)gn * gn * gn;
+		int gx = (int)std::min<int64_t>((n3 + kBlock - 1) / kBlock, 4096);
+		for (size_t b0 = 0; b0 < N; b0 += 65535) {
+			const int nb = (int)std::min<size_t>(65535, N - b0);
+			hipLaunchKernelGGL(k_build_grids, dim3(gx, nb), dim3(kBlock), 0, s, ctx->b_grid.p + b0,
+			                   ctx->b_shape.p + b0, ctx->b_dims.p + 3 * b0, ctx->grid_pool.p);
+			PDC_CHECK_LAUNCH();
+		}
+	}
+	PDC_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// per body: rotation and world AABB of the grid box
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_body_frames(int N, const double* __restrict__ quat, const double* __restrict__ pos,
+                              const GridDesc* __restrict__ grids, double* __restrict__ Rout,
+                              double* __restrict__ aabb) {
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= N) return;
+	double q[4] = {quat[4 * b], quat[4 * b + 1], quat[4 * b + 2], quat[4 * b + 3]};
+	double R[9];
+	quat_rot(q, R);
+	for (int k = 0; k < 9; ++k) Rout[9 * b + k] = R[k];
+	const GridDesc g = grids[b];
+	for (int r = 0; r < 3; ++r) {
+		const double h = fabs(R[r * 3 + 0]) * g.half[0] + fabs(R[r * 3 + 1]) * g.half[1] +
+		                 fabs(R[r * 3 + 2]) * g.half[2];
+		aabb[6 * b + r] = pos[3 * b + r] - h;
+		aabb[6 * b + 3 + r] = pos[3 * b + r] + h;
+	}
+}
+
+__device__ __forceinline__ bool aabb_overlap(const double* A, const double* B) {
+	for (int r = 0; r < 3; ++r)
+		if (A[r] > B[3 + r] || B[r] > A[3 + r]) return false;
+	return true;
+}
+
+// Broadphase, pass 1: candidates b > a per body a.
+__global__ __launch_bounds__(256) void k_bp_count(int N, const double* __restrict__ aabb, const int* __restrict__ stat,
+                           int* __restrict__ cnt) {
+	const int a = blockIdx.x * blockDim.x + threadIdx.x;
+	if (a >= N) return;
+	double A[6];
+	for (int k = 0; k < 6; ++k) A[k] = aabb[6 * a + k];
+	const int sa = stat[a];
+	int c = 0;
+	for (int b = a + 1; b < N; ++b) {
+		if (sa && stat[b]) continue;
+		if (aabb_overlap(A, &aabb[6 * b])) ++c;
+	}
+	cnt[a] = c;
+}
+
+// Broadphase, pass 2: write the candidates in (a, b) order.
+__global__ __launch_bounds__(256) void k_bp_write(int N, const double* __restrict__ aabb, const int* __restrict__ stat,
+                           const int* __restrict__ off, int* __restrict__ pa, int* __restrict__ pb,
+                           const int* __restrict__ soff, const int* __restrict__ emit,
+                           int* __restrict__ stage_cnt) {
+	const int a = blockIdx.x * blockDim.x + threadIdx.x;
+	if (a >= N) return;
+	double A[6];
+	for (int k = 0; k < 6; ++k) A[k] = aabb[6 * a + k];
+	const int sa = stat[a];
+	const int na = emit[a] ? soff[a + 1] - soff[a] : 0;
+	int o = off[a];
+	for (int b = a + 1; b < N; ++b) {
+		if (sa && stat[b]) continue;
+		if (aabb_overlap(A, &aabb[6 * b])) {
+			pa[o] = a;
+			pb[o] = b;
+			stage_cnt[o] = na + (emit[b] ? soff[b + 1] - soff[b] : 0);
+			++o;
+		}
+	}
+}
+
+struct PairArgs {
+	const int* pa;
+	const int* pb;
+	const int* soff;       // per body sample offsets
+	const int* emit;
+	const double* samples;
+	const double* R;
+	const double* pos;
+	const GridDesc* grids;
+	const float* pool;
+	const int* stage_off;  // per pair
+	double* stage_raw;     // 7 doubles per slot
+	double* stage_m;
+	int* pair_nh;
+	int* pair_nc;
+	Scalars* sc;
+	double tie_band;
+};
+
+// Tests samples [s0, s0 + S) of body q against the grid of body g; appends hits.
+__device__ __forceinline__ bool sample_hit(const PairArgs& A, const double* Rq,
+                                           const double* xq, const double* Rg, const double* xg,
+                                           const GridDesc& G, int s, double sign, double* out7,
+                                           int* tie) {
+	const double* l = &A.samples[3 * (size_t)s];
+	const double l0 = l[0], l1 = l[1], l2 = l[2];
+	double p[3];
+	for (int r = 0; r < 3; ++r) p[r] = Rq[r * 3 + 0] * l0 + Rq[r * 3 + 1] * l1 + Rq[r * 3 + 2] * l2 + xq[r];
+	const double d[3] = {p[0] - xg[0], p[1] - xg[1], p[2] - xg[2]};
+	double q[3];
+	for (int c = 0; c < 3; ++c) q[c] = Rg[0 * 3 + c] * d[0] + Rg[1 * 3 + c] * d[1] + Rg[2 * 3 + c] * d[2];
+	const float* pool = A.pool + G.offset;
+	const int n = G.n;
+	auto fetch = [&](int i, int j, int k) -> double { return (double)pool[grid_index(n, i, j, k)]; };
+	double sd, g[3];
+	if (!grid_query(G, q, fetch, &sd, g)) return false;
+	if (fabs(sd) <= A.tie_band) *tie += 1;
+	if (!(sd < 0.0)) return false;
+	double gw[3];
+	for (int r = 0; r < 3; ++r) gw[r] = Rg[r * 3 + 0] * g[0] + Rg[r * 3 + 1] * g[1] + Rg[r * 3 + 2] * g[2];
+	const double gg = gw[0] * gw[0] + gw[1] * gw[1] + gw[2] * gw[2];
+	if (!(gg > 0.0)) return false;
+	const double gl = sqrt(gg);
+	out7[0] = p[0]; out7[1] = p[1]; out7[2] = p[2];
+	out7[3] = sign * (gw[0] / gl);
+	out7[4] = sign * (gw[1] / gl);
+	out7[5] = sign * (gw[2] / gl);
+	out7[6] = -sd;
+	return true;
+}
+
+__device__ __forceinline__ bool pos_less(double ax, double ay, double az, double bx, double by,
+                                         double bz) {
+	if (ax != bx) return ax < bx;
+	if (ay != by) return ay < by;
+	return az < bz;
+}
+
+// Sample pass: one 256-thread workgroup per candidate pair; hits are compacted
+// in sample order (a's samples first, then b's) into the pair's staging slots.
+__global__ __launch_bounds__(256) void k_pair_hits(PairArgs A) {
+	__shared__ int wcnt[4];
+	__shared__ int s_nh, s_ties;
+	const int p = blockIdx.x;
+	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	const int a = A.pa[p], b = A.pb[p];
+	double Ra[9], Rb[9], xa[3], xb[3];
+	for (int k = 0; k < 9; ++k) { Ra[k] = A.R[9 * a + k]; Rb[k] = A.R[9 * b + k]; }
+	for (int k = 0; k < 3; ++k) { xa[k] = A.pos[3 * a + k]; xb[k] = A.pos[3 * b + k]; }
+	const GridDesc Ga = A.grids[a], Gb = A.grids[b];
+	const int base = A.stage_off[p];
+	if (tid == 0) { s_nh = 0; s_ties = 0; }
+	__syncthreads();
+	int myties = 0;
+	for (int pass = 0; pass < 2; ++pass) {
+		const int qb = pass == 0 ? a : b;
+		if (!A.emit[qb]) continue;
+		const int s0 = A.soff[qb], S = A.soff[qb + 1] - s0;
+		for (int c0 = 0; c0 < S; c0 += 256) {
+			const int s = c0 + tid;
+			double h7[7];
+			bool hit = false;
+			if (s < S) {
+				if (pass == 0) hit = sample_hit(A, Ra, xa, Rb, xb, Gb, s0 + s, -1.0, h7, &myties);
+				else hit = sample_hit(A, Rb, xb, Ra, xa, Ga, s0 + s, +1.0, h7, &myties);
+			}
+			const unsigned long long m = __ballot(hit);
+			const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+			const int rank = __popcll(m & lt);
+			if (lane == 0) wcnt[wid] = __popcll(m);
+			__syncthreads();
+			int before = 0;
+			for (int w = 0; w < wid; ++w) before += wcnt[w];
+			const int total = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+			const int nh0 = s_nh;
+			if (hit) {
+				double* o = &A.stage_raw[7 * (size_t)(base + nh0 + before + rank)];
+				for (int k = 0; k < 7; ++k) o[k] = h7[k];
+			}
+			__syncthreads();
+			if (tid == 0) s_nh = nh0 + total;
+			__syncthreads();
+		}
+	}
+	if (myties) atomicAdd(&s_ties, myties);
+	__syncthreads();
+	if (tid == 0) {
+		const int nh = s_nh;
+		A.pair_nh[p] = nh;
+		if (s_ties) atomicAdd(&A.sc->ties, s_ties);
+		if (nh) atomicAdd(&A.sc->n_raw, nh);
+		if (nh > kSortCap) atomicOr(&A.sc->overflow, 1);
+	}
+}
+
+// Merge pass: one 256-thread workgroup per pair with hits; sorts the hits by
+// position (bitonic in LDS) and chain-merges them.
+__global__ __launch_bounds__(256) void k_pair_merge(PairArgs A) {
+	__shared__ double kx[kSortCap], ky[kSortCap], kz[kSortCap];
+	__shared__ short kidx[kSortCap];
+	__shared__ unsigned char brk[kSortCap];
+	__shared__ int gidx[kSortCap];
+	__shared__ int scan_part[256];
+	const int p = blockIdx.x;
+	const int tid = threadIdx.x;
+	const int nh = A.pair_nh[p];
+	if (nh == 0 || nh > kSortCap) {
+		if (tid == 0) A.pair_nc[p] = 0;
+		return;
+	}
+	const int base = A.stage_off[p];
+	if (nh == 1) {
+		if (tid < 7) A.stage_m[7 * (size_t)base + tid] = A.stage_raw[7 * (size_t)base + tid];
+		if (tid == 0) A.pair_nc[p] = 1;
+		return;
+	}
+	int M = 1;
+	while (M < nh) M <<= 1;
+	for (int i = tid; i < M; i += 256) {
+		if (i < nh) {
+			const double* o = &A.stage_raw[7 * (size_t)(base + i)];
+			kx[i] = o[0]; ky[i] = o[1]; kz[i] = o[2];
+		} else {
+			kx[i] = INFINITY; ky[i] = INFINITY; kz[i] = INFINITY;
+		}
+		kidx[i] = (short)i;
+	}
+	__syncthreads();
+	for (int k = 2; k <= M; k <<= 1) {
+		for (int j = k >> 1; j > 0; j >>= 1) {
+			for (int i = tid; i < M; i += 256) {
+				const int l = i ^ j;
+				if (l > i) {
+					const bool up = (i & k) == 0;
+					const bool sw = up ? pos_less(kx[l], ky[l], kz[l], kx[i], ky[i], kz[i])
+					                   : pos_less(kx[i], ky[i], kz[i], kx[l], ky[l], kz[l]);
+					if (sw) {
+						double t;
+						t = kx[i]; kx[i] = kx[l]; kx[l] = t;
+						t = ky[i]; ky[i] = ky[l]; ky[l] = t;
+						t = kz[i]; kz[i] = kz[l]; kz[l] = t;
+						const short ti = kidx[i]; kidx[i] = kidx[l]; kidx[l] = ti;
+					}
+				}
+			}
+			__syncthreads();
+		}
+	}
+	// a group starts where |p_i - p_{i-1}|^2 > 1e-8 (DynamicSystem.cpp:356-363)
+	for (int i = tid; i < nh; i += 256) {
+		unsigned char f = 1;
+		if (i > 0) {
+			const double d0 = kx[i] - kx[i - 1], d1 = ky[i] - ky[i - 1], d2 = kz[i] - kz[i - 1];
+			f = (d0 * d0 + d1 * d1 + d2 * d2 > 1e-8) ? 1 : 0;
+		}
+		brk[i] = f;
+	}
+	__syncthreads();
+	{
+		int loc = 0;
+		const int b0 = tid * 8;
+		for (int k = 0; k < 8; ++k)
+			if (b0 + k < nh) loc += brk[b0 + k];
+		scan_part[tid] = loc;
+		__syncthreads();
+		if (tid == 0) {
+			int run = 0;
+			for (int t = 0; t < 256; ++t) { const int v = scan_part[t]; scan_part[t] = run; run += v; }
+		}
+		__syncthreads();
+		int run = scan_part[tid];
+		for (int k = 0; k < 8; ++k)
+			if (b0 + k < nh) { gidx[b0 + k] = run; run += brk[b0 + k]; }
+	}
+	__syncthreads();
+	// group = first contact of the run; normal = normalize(sum of the members'
+	// frame normals), depth = min (DynamicSystem.cpp:364-381)
+	for (int i = tid; i < nh; i += 256) {
+		if (!brk[i]) continue;
+		int j = i + 1;
+		while (j < nh && !brk[j]) ++j;
+		const double* o = &A.stage_raw[7 * (size_t)(base + kidx[i])];
+		double out[7] = {o[0], o[1], o[2], o[3], o[4], o[5], o[6]};
+		if (j - i > 1) {
+			double Rr[9];
+			frame_rotation(&o[3], Rr);
+			double n[3] = {Rr[0], Rr[3], Rr[6]};
+			double dep = o[6];
+			for (int k = i + 1; k < j; ++k) {
+				const double* ok = &A.stage_raw[7 * (size_t)(base + kidx[k])];
+				frame_rotation(&ok[3], Rr);
+				n[0] += Rr[0]; n[1] += Rr[3]; n[2] += Rr[6];
+				dep = ok[6] < dep ? ok[6] : dep;
+			}
+			const double nn = sqrt(dot3(n, n));
+			if (nn > 0) { n[0] /= nn; n[1] /= nn; n[2] /= nn; }
+			out[3] = n[0]; out[4] = n[1]; out[5] = n[2];
+			out[6] = dep;
+		}
+		double* w = &A.stage_m[7 * (size_t)(base + gidx[i])];
+		for (int k = 0; k < 7; ++k) w[k] = out[k];
+	}
+	if (tid == 0) A.pair_nc[p] = gidx[nh - 1] + brk[nh - 1];
+}
+
+// One workgroup per pair: final contact records in pair order.
+__global__ __launch_bounds__(256) void k_pair_write(const int* __restrict__ pa, const int* __restrict__ pb,
+                             const int* __restrict__ stage_off, const int* __restrict__ pair_nc,
+                             const int* __restrict__ coff, const double* __restrict__ stage_m,
+                             const double* __restrict__ pos, int C, int* __restrict__ id0,
+                             int* __restrict__ id1, double* __restrict__ cpos,
+                             double* __restrict__ cdepth, double* __restrict__ cT,
+                             double* __restrict__ clf) {
+	const int p = blockIdx.x;
+	const int n = pair_nc[p];
+	if (n == 0) return;
+	const int a = pa[p], b = pb[p];
+	const double xa[3] = {pos[3 * a], pos[3 * a + 1], pos[3 * a + 2]};
+	const double xb[3] = {pos[3 * b], pos[3 * b + 1], pos[3 * b + 2]};
+	const int base = stage_off[p];
+	const int o0 = coff[p];
+	for (int g = threadIdx.x; g < n; g += blockDim.x) {
+		const double* s = &stage_m[7 * (size_t)(base + g)];
+		const int i = o0 + g;
+		double R[9], T[18];
+		frame_rotation(&s[3], R);
+		id0[i] = a;
+		id1[i] = b;
+		for (int k = 0; k < 3; ++k) cpos[(size_t)k * C + i] = s[k];
+		cdepth[i] = fabs(s[6]);
+		frame_T(R, s, xa, 0, T);
+		for (int k = 0; k < 18; ++k) cT[(size_t)k * C + i] = T[k];
+		frame_T(R, s, xb, 1, T);
+		for (int k = 0; k < 18; ++k) cT[(size_t)(18 + k) * C + i] = T[k];
+		for (int k = 0; k < 3; ++k) clf[(size_t)k * C + i] = 0.0;
+	}
+}
+
+void detect_contacts(pdc_context* ctx) {
+	hipStream_t s = ctx->stream;
+	const int N = ctx->N;
+	ctx->C = 0;
+	PDC_HIP(hipMemsetAsync(ctx->d_sc.p, 0, sizeof(Scalars), s));
+	if (N < 2) return;
+	hipLaunchKernelGGL(k_body_frames, dim3(grid_for(N)), dim3(kBlock), 0, s, N, ctx->b_quat.p,
+	                   ctx->b_pos.p, ctx->b_grid.p, ctx->b_R.p, ctx->b_aabb.p);
+	PDC_CHECK_LAUNCH();
+	ctx->bp_cnt.ensure(N + 1);
+	ctx->bp_off.ensure(N + 1);
+	hipLaunchKernelGGL(k_bp_count, dim3(grid_for(N, 64)), dim3(64), 0, s, N, ctx->b_aabb.p,
+	                   ctx->b_static.p, ctx->bp_cnt.p);
+	PDC_CHECK_LAUNCH();
+	scan_exclusive(ctx, ctx->bp_cnt.p, ctx->bp_off.p, N);
+	const int P = read_int(ctx, ctx->bp_off.p + N);
+	if (P == 0) return;
+	ctx->pr_a.ensure(P); ctx->pr_b.ensure(P); ctx->pr_tmp.ensure(P + 1);
+	ctx->pr_soff.ensure(P + 1); ctx->pr_nc.ensure(P + 1); ctx->pr_coff.ensure(P + 1);
+	ctx->pr_nh.ensure(P + 1);
+	hipLaunchKernelGGL(k_bp_write, dim3(grid_for(N, 64)), dim3(64), 0, s, N, ctx->b_aabb.p,
+	                   ctx->b_static.p, ctx->bp_off.p, ctx->pr_a.p, ctx->pr_b.p, ctx->b_soff.p,
+	                   ctx->b_emit.p, ctx->pr_tmp.p);
+	PDC_CHECK_LAUNCH();
+	scan_exclusive(ctx, ctx->pr_tmp.p, ctx->pr_soff.p, P);
+	const int S = read_int(ctx, ctx->pr_soff.p + P);
+	ctx->stage_raw.ensure(7 * (size_t)std::max(S, 1));
+	ctx->stage_m.ensure(7 * (size_t)std::max(S, 1));
+	PairArgs A;
+	A.pa = ctx->pr_a.p; A.pb = ctx->pr_b.p; A.soff = ctx->b_soff.p; A.emit = ctx->b_emit.p;
+	A.samples = ctx->samples.p; A.R = ctx->b_R.p; A.pos = ctx->b_pos.p; A.grids = ctx->b_grid.p;
+	A.pool = ctx->grid_pool.p; A.stage_off = ctx->pr_soff.p; A.stage_raw = ctx->stage_raw.p;
+	A.stage_m = ctx->stage_m.p; A.pair_nc = ctx->pr_nc.p; A.sc = ctx->d_sc.p;
+	A.tie_band = ctx->tie_band;
+	A.pair_nh = ctx->pr_nh.p;
+	hipEvent_t e0;
+	kt_begin(ctx, PDC_KT_CONTACTS, &e0);
+	hipLaunchKernelGGL(k_pair_hits, dim3(P), dim3(256), 0, s, A);
+	PDC_CHECK_LAUNCH();
+	kt_end(ctx, PDC_KT_CONTACTS, e0);
+	// per tested sample: its body-frame position (24 B) and the 8 FP32 nodes of
+	// its grid cell (32 B); per hit: position, normal, depth written (56 B)
+	if (e0) ctx->kt.pending_contact_slots = S;
+	hipLaunchKernelGGL(k_pair_merge, dim3(P), dim3(256), 0, s, A);
+	PDC_CHECK_LAUNCH();
+	scan_exclusive(ctx, ctx->pr_nc.p, ctx->pr_coff.p, P);
+	sync_scalars(ctx);
+	if (ctx->h_sc->overflow)
+		throw HipError("a body pair produced more than 2048 sample hits; use a coarser sample_spacing",
+		               PDC_ERR_CAPACITY);
+	const int C = read_int(ctx, ctx->pr_coff.p + P);
+	ctx->n_pairs = P;
+	ctx->n_raw = ctx->h_sc->n_raw;
+	if (ctx->kt.pending_contact_slots) {
+		ctx->kt.bytes += 56.0 * ctx->kt.pending_contact_slots + 56.0 * ctx->n_raw;
+		ctx->kt.pending_contact_slots = 0;
+	}
+	ctx->n_ties = ctx->h_sc->ties;
+	if (C == 0) return;
+	ensure_contact_capacity(ctx, C);
+	ctx->C = C;
+	hipLaunchKernelGGL(k_pair_write, dim3(P), dim3(64), 0, s, ctx->pr_a.p, ctx->pr_b.p,
+	                   ctx->pr_soff.p, ctx->pr_nc.p, ctx->pr_coff.p, ctx->stage_m.p, ctx->b_pos.p,
+	                   (int)ctx->c_stride, ctx->c_id0.p, ctx->c_id1.p, ctx->c_pos.p, ctx->c_depth.p,
+	                   ctx->c_T.p, ctx->c_lf.p);
+	PDC_CHECK_LAUNCH();
+}
+
+}  // namespace pdc

@@ -1,0 +1,193 @@
+// pdc_internal.h — context, device buffers and launch helpers of libpdc.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pdc.h"
+#include "pdc_math.h"
+
+namespace pdc {
+
+constexpr int kBlock = 256;            // threads per workgroup of the 1-D kernels
+constexpr int kMaxPartialBlocks = 4096;  // workgroups whose partial sums a reduction reads
+constexpr int kSortCap = 2048;         // raw sample hits one pair can sort in LDS
+
+struct HipError : std::runtime_error {
+	int code;
+	HipError(const std::string& m, int c) : std::runtime_error(m), code(c) {}
+};
+
+#define PDC_HIP(call)                                                                   \
+	do {                                                                                \
+		hipError_t e_ = (call);                                                         \
+		if (e_ != hipSuccess)                                                           \
+			throw ::pdc::HipError(std::string(#call) + ": " + hipGetErrorString(e_),    \
+			                      PDC_ERR_DEVICE);                                      \
+	} while (0)
+
+#define PDC_CHECK_LAUNCH() PDC_HIP(hipGetLastError())
+
+inline int grid_for(int64_t n, int block = kBlock) {
+	int64_t g = (n + block - 1) / block;
+	return (int)(g < 1 ? 1 : g);
+}
+
+// Growable device buffer (reallocates only when the capacity grows).
+template <class T>
+struct DBuf {
+	T* p = nullptr;
+	size_t cap = 0;
+	void ensure(size_t n) {
+		if (n <= cap) return;
+		size_t nc = cap ? cap : 256;
+		while (nc < n) nc = nc + nc / 2 + 1;
+		if (p) PDC_HIP(hipFree(p));
+		p = nullptr;
+		PDC_HIP(hipMalloc(&p, nc * sizeof(T)));
+		cap = nc;
+	}
+	void release() {
+		if (p) (void)hipFree(p);
+		p = nullptr;
+		cap = 0;
+	}
+	operator T*() const { return p; }
+};
+
+// Scalars shared between the reduction kernels and the host (one device struct).
+struct Scalars {
+	double char_mass, char_speed;
+	double gap, muc;
+	double ru2, wrf2, rl2, ac2;
+	double err, sErr, sacErr, eps, exitErr;
+	double Hu;
+	// line search trial
+	double t_ru2, t_wrf2, t_rl2, t_ac2, t_gErr_u, t_gErr_f, t_gErr_l;
+	double t_inErr, t_sacIn, t_gErr;
+	int pcg_iters;
+	int overflow;        // capacity overflow flag set by kernels
+	int n_pairs;         // broadphase candidate pairs
+	int n_contacts;      // contacts after merge
+	int n_raw;           // raw sample hits
+	int ties;
+	int n_local;         // bodies in contact
+	int n_apairs;        // pairs with contacts
+	int nnzb;            // blocks of the Newton matrix
+	int pad;
+};
+
+struct KernelTimer {
+	int which = PDC_KT_NONE;
+	double total_ms = 0;
+	int64_t launches = 0;
+	double bytes = 0;              // algorithmic bytes of the timed launches
+	int64_t pending_contact_slots = 0;
+	std::vector<hipEvent_t> pool;  // start/stop pairs awaiting read-back
+	size_t used = 0;
+};
+
+}  // namespace pdc
+
+struct pdc_context {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	pdc_params p{};
+	std::string err;
+	int cus = 0;
+
+	// ---- bodies (global numbering) ----
+	int N = 0;
+	int64_t n_samples = 0;
+	pdc::DBuf<int> b_shape, b_static, b_emit;
+	pdc::DBuf<double> b_dims, b_mass, b_I0;
+	pdc::DBuf<double> b_pos, b_quat, b_vel, b_ang, b_acc;  // acc: 6 per body
+	pdc::DBuf<double> b_R, b_aabb;                         // 9 and 6 per body
+	pdc::DBuf<pdc::GridDesc> b_grid;
+	pdc::DBuf<float> grid_pool;
+	pdc::DBuf<int> b_soff;                                 // N+1 sample offsets
+	pdc::DBuf<double> samples;                             // 3 per sample
+	std::vector<int> h_soff;                               // host copy of sample offsets
+	std::vector<int> h_static;
+
+	// ---- broadphase / pairs ----
+	pdc::DBuf<int> bp_cnt, bp_off;                          // per body a: candidates b > a
+	pdc::DBuf<int> pr_a, pr_b;                              // candidate pairs
+	pdc::DBuf<int> pr_soff;                                 // staging offsets (P+1)
+	pdc::DBuf<int> pr_nh;                                   // raw hits per pair
+	pdc::DBuf<int> pr_nc, pr_coff;                          // merged contacts per pair, offsets (P+1)
+	pdc::DBuf<int> pr_tmp;
+	pdc::DBuf<double> stage_raw;                            // 7 per slot: pos, n, depth
+	pdc::DBuf<double> stage_m;                              // 7 per slot: merged
+	int64_t stage_cap = 0;
+
+	// ---- contacts (SoA) ----
+	int C = 0;
+	pdc::DBuf<int> c_id0, c_id1;
+	pdc::DBuf<double> c_pos;    // [3][C]
+	pdc::DBuf<double> c_depth;  // [C]
+	pdc::DBuf<double> c_T;      // [36][C]: (side*18 + col*6 + row)
+	pdc::DBuf<double> c_lf;     // [3][C]
+	int c_stride = 0;           // SoA row stride (the contact capacity)
+	int n_pairs = 0, n_raw = 0, n_ties = 0;
+	double tie_band = 0;
+
+	// ---- topology (local numbering) ----
+	int NL = 0, NP = 0, NNZB = 0;
+	pdc::DBuf<int> t_key_flag, t_pid;                       // per contact
+	pdc::DBuf<int> ap_a, ap_b, ap_start;                    // active pairs (global ids), start (NP+1)
+	pdc::DBuf<int> g_flag, g2l, l2g;                        // per global body
+	pdc::DBuf<int> li0, li1;                                // per contact local ids
+	pdc::DBuf<int> adj_cnt, adj_ptr, adj_cur, adj_nb, adj_pair;
+	pdc::DBuf<int> a_cnt, a_ptr, a_col, a_pair, a_diag;
+
+	// ---- solver (local numbering) ----
+	pdc::DBuf<int> l_static;
+	pdc::DBuf<double> l_v, l_vold, l_Ms, l_Minv, l_vphys;   // 6,6,7,7,6 per body
+	pdc::DBuf<unsigned long long> l_speed;                   // objSpeed as ordered bits
+	pdc::DBuf<double> l_ru, l_rhs, l_D, l_du, l_gdu;        // 6 per body
+	pdc::DBuf<double> s_K, s_f, s_lam, s_c, s_g, s_H;        // per contact
+	pdc::DBuf<double> s_rf, s_rl, s_sf, s_invC, s_W, s_kf, s_df, s_dl;
+	pdc::DBuf<double> s_ft, s_lt;                            // trial force / multiplier
+	pdc::DBuf<double> partials;                              // [k][kMaxPartialBlocks]
+	pdc::DBuf<float> A32, P32, b32, x32, pcg32;              // FP32 PCG
+	pdc::DBuf<double> A64, P64, b64, x64, pcg64;             // FP64 PCG
+	pdc::DBuf<double> pcg_part;
+	pdc::DBuf<pdc::Scalars> d_sc;
+	pdc::Scalars* h_sc = nullptr;                            // pinned mirror
+
+	// ---- statistics ----
+	pdc_timings timings{};
+	pdc_solve_stats last{};
+	pdc::KernelTimer kt;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace pdc {
+
+// pdc_util.hip
+void scan_exclusive(pdc_context* ctx, const int* in, int* out, int n);  // out has n+1 entries
+int read_int(pdc_context* ctx, const int* dptr);
+void sync_scalars(pdc_context* ctx);  // device Scalars -> ctx->h_sc (blocking)
+void kt_begin(pdc_context* ctx, int which, hipEvent_t* ev);
+void kt_end(pdc_context* ctx, int which, hipEvent_t ev0);
+void kt_collect(pdc_context* ctx);
+
+// pdc_contacts.hip
+void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,
+                   const double* inertia, const int* is_static, const int* emit);
+void detect_contacts(pdc_context* ctx);  // fills c_* and ctx->C
+
+// pdc_solver.hip
+void build_topology(pdc_context* ctx);   // from c_id0/c_id1 (sorted by pair key)
+void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st);
+void write_acc_default(pdc_context* ctx, double dt, const double* g);
+void integrate(pdc_context* ctx, double dt);
+void ensure_contact_capacity(pdc_context* ctx, size_t C);
+
+}  // namespace pdc

@@ -1,0 +1,228 @@
+// pdc_math.h — device/host math shared by the HIP kernels.
+//
+// The grid-SDF functions define this project's contact generator and must give
+// bit-identical FP64 results on host and device: the library is compiled with
+// -ffp-contract=off and every expression below fixes its evaluation order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define PDC_HD __host__ __device__ __forceinline__
+
+namespace pdc {
+
+PDC_HD double dot3(const double* a, const double* b) {
+	return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+PDC_HD void cross3(const double* a, const double* b, double* o) {
+	o[0] = a[1] * b[2] - a[2] * b[1];
+	o[1] = a[2] * b[0] - a[0] * b[2];
+	o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Rotation (row-major) of a unit quaternion stored (x, y, z, w); the standard
+// formula used by Eigen's Quaternion::toRotationMatrix.
+PDC_HD void quat_rot(const double* q, double* R) {
+	const double x = q[0], y = q[1], z = q[2], w = q[3];
+	const double tx = 2 * x, ty = 2 * y, tz = 2 * z;
+	const double twx = tx * w, twy = ty * w, twz = tz * w;
+	const double txx = tx * x, txy = ty * x, txz = tz * x;
+	const double tyy = ty * y, tyz = tz * y, tzz = tz * z;
+	R[0] = 1 - (tyy + tzz); R[1] = txy - twz;       R[2] = txz + twy;
+	R[3] = txy + twz;       R[4] = 1 - (txx + tzz); R[5] = tyz - twx;
+	R[6] = txz - twy;       R[7] = tyz + twx;       R[8] = 1 - (txx + tyy);
+}
+
+// Hamilton product o = a * b, quaternions stored (x, y, z, w).
+PDC_HD void quat_mul(const double* a, const double* b, double* o) {
+	double r[4];
+	r[3] = a[3] * b[3] - a[0] * b[0] - a[1] * b[1] - a[2] * b[2];
+	r[0] = a[3] * b[0] + a[0] * b[3] + a[1] * b[2] - a[2] * b[1];
+	r[1] = a[3] * b[1] + a[1] * b[3] + a[2] * b[0] - a[0] * b[2];
+	r[2] = a[3] * b[2] + a[2] * b[3] + a[0] * b[1] - a[1] * b[0];
+	o[0] = r[0]; o[1] = r[1]; o[2] = r[2]; o[3] = r[3];
+}
+
+// o = R diag(d) R^T, stored as the 6 unique entries (00 01 02 11 12 22).
+PDC_HD void rot_diag_rot_sym(const double* R, const double* d, double* o) {
+	int k = 0;
+	for (int r = 0; r < 3; ++r)
+		for (int c = r; c < 3; ++c)
+			o[k++] = R[r * 3 + 0] * d[0] * R[c * 3 + 0] + R[r * 3 + 1] * d[1] * R[c * 3 + 1] +
+			         R[r * 3 + 2] * d[2] * R[c * 3 + 2];
+}
+PDC_HD void sym3_mv(const double* s, const double* x, double* y) {
+	y[0] = s[0] * x[0] + s[1] * x[1] + s[2] * x[2];
+	y[1] = s[1] * x[0] + s[3] * x[1] + s[4] * x[2];
+	y[2] = s[2] * x[0] + s[4] * x[1] + s[5] * x[2];
+}
+
+// ---------------------------------------------------------------------------
+// Lorentz-circle constraint (reference LorentzCircleConstraint.cpp:5-55):
+//   c  = sqrt(|f_t|^2 + P) - mu f_n - sqrt(P) - SH,  P = 1e-16, SH = 1e-8
+//   g  = f_t / sqrt(|f_t|^2 + P)
+//   H  = (z z^T + p I) (|f_t|^2 + p)^(-3/2), z = (f_2, -f_1), p = max(P, 1e-6)
+//   radius(f_n) = sqrt(k^2 - P), k = mu f_n + sqrt(P) + SH
+// ---------------------------------------------------------------------------
+struct Lorentz {
+	static constexpr double P = 1e-16;
+	static constexpr double SQP = 1e-8;  // sqrt(P), exactly representable as computed
+	static constexpr double SH = 1e-8;
+	PDC_HD static double constraint(double mu, double f0, double f1, double f2) {
+		const double r = f1 * f1 + f2 * f2 + P;
+		return sqrt(r) - mu * f0 - sqrt(P) - SH;
+	}
+	PDC_HD static void gradient(double f1, double f2, double* g) {
+		const double r = sqrt(f1 * f1 + f2 * f2 + P);
+		g[0] = f1 / r;
+		g[1] = f2 / r;
+	}
+	// symmetric 2x2: H00, H01, H11
+	PDC_HD static void hessian(double f1, double f2, double* H) {
+		const double p = 1e-6;
+		const double r2 = f1 * f1 + f2 * f2 + p;
+		const double r3 = 1.0 / (r2 * sqrt(r2));
+		const double z0 = f2, z1 = -f1;
+		H[0] = (z0 * z0 + p) * r3;
+		H[1] = (z0 * z1) * r3;
+		H[2] = (z1 * z1 + p) * r3;
+	}
+	PDC_HD static double radius(double mu, double f0) {
+		const double k = mu * f0 + sqrt(P) + SH;
+		return sqrt(k * k - P);
+	}
+	// project(in, start = in, target = 0) (LorentzCircleConstraint.cpp:36-49)
+	PDC_HD static void project(double mu, const double* in, double* out) {
+		out[0] = in[0]; out[1] = in[1]; out[2] = in[2];
+		if (in[0] < 0) out[0] = 0;
+		if (constraint(mu, out[0], out[1], out[2]) > 0) {
+			double r = radius(mu, in[0]);
+			r = r > 0.0 ? r : 0.0;
+			const double nt = sqrt(out[1] * out[1] + out[2] * out[2]);
+			const double s = r / nt;
+			out[1] *= s;
+			out[2] *= s;
+		}
+	}
+};
+
+// ---------------------------------------------------------------------------
+// Contact frame (reference ContactGeneratorBase.h:21-49): rot maps x^ to n by
+// the angle-axis rotation (atan2(|x^ x n|, x^.n), normalize(x^ x n)); T_j =
+// [rot_j; c_j x rot_j(:,k)], c_j = pos - x_j, rot_1 = -rot_0.  Output: the
+// rotation (row-major) of side 0.  n = -x^ gives rot = -I.
+// ---------------------------------------------------------------------------
+PDC_HD void frame_rotation(const double* n, double* R) {
+	const double axis[3] = {0.0, -n[2], n[1]};
+	const double sq = dot3(axis, axis);
+	const double an = sqrt(sq);
+	const double angle = atan2(an, n[0]);
+	double a[3] = {axis[0], axis[1], axis[2]};
+	if (sq > 0) {
+		const double inv = sqrt(sq);
+		a[0] = axis[0] / inv; a[1] = axis[1] / inv; a[2] = axis[2] / inv;
+	}
+	const double s = sin(angle), c = cos(angle);
+	const double sa[3] = {s * a[0], s * a[1], s * a[2]};
+	const double c1 = 1 - c;
+	const double ca[3] = {c1 * a[0], c1 * a[1], c1 * a[2]};
+	R[0] = ca[0] * a[0] + c;
+	R[4] = ca[1] * a[1] + c;
+	R[8] = ca[2] * a[2] + c;
+	double t = ca[0] * a[1];
+	R[1] = t - sa[2]; R[3] = t + sa[2];
+	t = ca[0] * a[2];
+	R[2] = t + sa[1]; R[6] = t - sa[1];
+	t = ca[1] * a[2];
+	R[5] = t - sa[0]; R[7] = t + sa[0];
+}
+// T (6x3 column-major, T[col*6 + row]) of side j from the side-0 rotation.
+PDC_HD void frame_T(const double* R0, const double* pos, const double* xj, int j, double* T) {
+	const double cj[3] = {pos[0] - xj[0], pos[1] - xj[1], pos[2] - xj[2]};
+	const double sg = j ? -1.0 : 1.0;
+	for (int col = 0; col < 3; ++col) {
+		const double rc[3] = {sg * R0[0 * 3 + col], sg * R0[1 * 3 + col], sg * R0[2 * 3 + col]};
+		double cr[3];
+		cross3(cj, rc, cr);
+		T[col * 6 + 0] = rc[0]; T[col * 6 + 1] = rc[1]; T[col * 6 + 2] = rc[2];
+		T[col * 6 + 3] = cr[0]; T[col * 6 + 4] = cr[1]; T[col * 6 + 5] = cr[2];
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Grid SDF of a primitive.  Grid: n^3 FP32 nodes over [-h, h] per axis,
+// h = half extent * (1 + pad), spacing 2h/(n-1); node (i,j,k) at
+// lo + (i,j,k)*spacing holds (float) of the analytic FP64 distance.
+// ---------------------------------------------------------------------------
+PDC_HD double analytic_sdf(int type, const double* dims, double px, double py, double pz) {
+	if (type == 1) {
+		return sqrt(px * px + py * py + pz * pz) - dims[0];
+	}
+	const double qx = fabs(px) - dims[0];
+	const double qy = fabs(py) - dims[1];
+	const double qz = fabs(pz) - dims[2];
+	const double ox = qx > 0.0 ? qx : 0.0;
+	const double oy = qy > 0.0 ? qy : 0.0;
+	const double oz = qz > 0.0 ? qz : 0.0;
+	const double outside = sqrt(ox * ox + oy * oy + oz * oz);
+	const double m = qx > (qy > qz ? qy : qz) ? qx : (qy > qz ? qy : qz);
+	const double inside = m < 0.0 ? m : 0.0;
+	return outside + inside;
+}
+
+struct GridDesc {
+	double lo[3];
+	double sp[3];
+	double half[3];
+	int n;
+	int64_t offset;  // into the float node pool
+};
+
+// Trilinear value and gradient at local point q; false when outside the grid box.
+template <class Fetch>
+PDC_HD bool grid_query(const GridDesc& g, const double* q, Fetch fetch, double* sd, double* grad) {
+	double t[3];
+	int i[3];
+	const double top = (double)(g.n - 1);
+	for (int a = 0; a < 3; ++a) {
+		const double gg = (q[a] - g.lo[a]) / g.sp[a];
+		if (!(gg >= 0.0 && gg <= top)) return false;
+		int ia = (int)gg;
+		if (ia > g.n - 2) ia = g.n - 2;
+		i[a] = ia;
+		t[a] = gg - (double)ia;
+	}
+	const double v000 = fetch(i[0], i[1], i[2]);
+	const double v100 = fetch(i[0] + 1, i[1], i[2]);
+	const double v010 = fetch(i[0], i[1] + 1, i[2]);
+	const double v110 = fetch(i[0] + 1, i[1] + 1, i[2]);
+	const double v001 = fetch(i[0], i[1], i[2] + 1);
+	const double v101 = fetch(i[0] + 1, i[1], i[2] + 1);
+	const double v011 = fetch(i[0], i[1] + 1, i[2] + 1);
+	const double v111 = fetch(i[0] + 1, i[1] + 1, i[2] + 1);
+	const double tx = t[0], ty = t[1], tz = t[2];
+	const double c00 = v000 + tx * (v100 - v000);
+	const double c10 = v010 + tx * (v110 - v010);
+	const double c01 = v001 + tx * (v101 - v001);
+	const double c11 = v011 + tx * (v111 - v011);
+	const double c0 = c00 + ty * (c10 - c00);
+	const double c1 = c01 + ty * (c11 - c01);
+	*sd = c0 + tz * (c1 - c0);
+	const double ux = 1.0 - tx, uy = 1.0 - ty, uz = 1.0 - tz;
+	grad[0] = (uy * uz * (v100 - v000) + ty * uz * (v110 - v010) + uy * tz * (v101 - v001) +
+	           ty * tz * (v111 - v011)) / g.sp[0];
+	grad[1] = (ux * uz * (v010 - v000) + tx * uz * (v110 - v100) + ux * tz * (v011 - v001) +
+	           tx * tz * (v111 - v101)) / g.sp[1];
+	grad[2] = (ux * uy * (v001 - v000) + tx * uy * (v101 - v100) + ux * ty * (v011 - v010) +
+	           tx * ty * (v111 - v110)) / g.sp[2];
+	return true;
+}
+
+// Node layout inside one grid: x fastest, then y, then z.
+PDC_HD int64_t grid_index(int n, int i, int j, int k) {
+	return ((int64_t)k * n + j) * n + i;
+}
+
+}  // namespace pdc

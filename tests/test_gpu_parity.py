@@ -1,0 +1,254 @@
+"""GPU: the HIP path (through the C ABI) against the oracle on identical inputs.
+
+Bars (stated per test):
+  contacts      identical index sets (ids, position and depth bit-exact); frames
+                within 1e-12; samples inside the tie band of the threshold are
+                counted and may differ
+  fillForces    FP64 PCG vs the oracle's direct solve: velocity increments and
+                local forces within 1e-6 of their scale
+  trajectories  FP64: positions within 1e-6 of the body size, orientations within
+                1e-6 rad
+  FP32 PCG      the FP64 Newton loop refines the FP32 inner solve: same 1e-6 bar
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from primal_dual_contacts_miror_amd import pdc, scenes
+from primal_dual_contacts_miror_amd.params import ONLY_NORMAL, PCG_FP32, PCG_FP64, default_params
+from primal_dual_contacts_miror_amd.scenes import _Builder
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _gpu_scene(sc, params=None):
+    return pdc.DynamicSystem(sc, params)
+
+
+def _contacts_gpu(ds):
+    c, ties = ds.ctx.detect_contacts()
+    ids, pos, depth, T, lf = ds.ctx.get_contacts(c)
+    return orc.Contacts(ids, pos, depth, T, lf), ties
+
+
+def assert_same_contacts(cg, co, ties):
+    """Identical sets modulo samples in the tie band; matched records must agree."""
+    kg = {(int(a), int(b), p.tobytes()): i for i, (a, b, p) in enumerate(zip(cg.ids[:, 0], cg.ids[:, 1], cg.pos))}
+    ko = {(int(a), int(b), p.tobytes()): i for i, (a, b, p) in enumerate(zip(co.ids[:, 0], co.ids[:, 1], co.pos))}
+    only_g = set(kg) - set(ko)
+    only_o = set(ko) - set(kg)
+    if only_g or only_o:
+        print("contact-set differences: gpu-only %d oracle-only %d ties %d" % (len(only_g), len(only_o), ties))
+    assert len(only_g) + len(only_o) <= ties
+    for k, ig in kg.items():
+        if k not in ko:
+            continue
+        io = ko[k]
+        assert cg.depth[ig] == co.depth[io]
+        np.testing.assert_allclose(cg.T[ig], co.T[io], atol=1e-12)
+    # same order when the sets agree
+    if not only_g and not only_o:
+        np.testing.assert_array_equal(cg.ids, co.ids)
+        np.testing.assert_array_equal(cg.pos, co.pos)
+
+
+def _advance_oracle(sc, steps):
+    o = orc.OracleScene(sc)
+    o.set_threads(8)
+    if steps:
+        o.step(steps)
+    return o
+
+
+def test_contacts_stack(require_gpu):
+    sc = scenes.stack_10()
+    o = _advance_oracle(sc, 5)
+    pos, quat, vel, ang = o.get_state()
+    ds = _gpu_scene(sc)
+    ds.set_state(pos, quat, vel, ang)
+    cg, ties = _contacts_gpu(ds)
+    n, t2 = o.detect()
+    co = o.contacts()
+    assert cg.n == n == 500
+    assert_same_contacts(cg, co, max(ties, t2))
+
+
+def test_contacts_rotated_pile(require_gpu):
+    sc = scenes.pile(60, seed=3, half=2.5, height=8.0, name="pile_60", grid_n=48, sample_spacing=0.1)
+    o = _advance_oracle(sc, 220)
+    pos, quat, vel, ang = o.get_state()
+    ds = _gpu_scene(sc)
+    ds.set_state(pos, quat, vel, ang)
+    cg, ties = _contacts_gpu(ds)
+    n, t2 = o.detect()
+    co = o.contacts()
+    print("pile_60 contacts", n, "gpu", cg.n, "ties", ties, t2)
+    assert n > 100
+    assert_same_contacts(cg, co, max(ties, t2))
+
+
+def test_contacts_empty_and_far_apart(require_gpu):
+    b = _Builder()
+    b.sphere(0.5, (0, 10, 0))
+    b.box((0.5, 0.5, 0.5), (5, 20, 0))
+    sc = b.build("ff", default_params(), 1)
+    ds = _gpu_scene(sc)
+    c, ties = ds.ctx.detect_contacts()
+    assert c == 0 and ties == 0
+    st = ds.step(10)
+    assert st.n_contacts == 0
+    pos, quat, vel, ang = ds.state()
+    np.testing.assert_allclose(vel[0], [0, -9.81 * 0.1, 0], atol=1e-13)
+    assert pos[0, 1] == pytest.approx(10 - 9.81 * 1e-4 * 55, abs=1e-12)
+
+
+def _fill_parity(sc, steps_before, params_gpu, tol_rel=1e-6):
+    o = _advance_oracle(sc, steps_before)
+    pos, quat, vel, ang = o.get_state()
+    n, _ = o.detect()
+    co = o.contacts()
+    o.set_linear_solver(orc.SOLVER_DIRECT)
+    acc_o, lf_o, st_o = o.fill_forces(co, quat, vel, ang)
+    solver = pdc.PrimalDualForceSolver(sc, params_gpu)
+    acc_g, lf_g, st_g = solver.fill_forces(co, quat, vel, ang)
+    print("C", n, "oracle it", st_o.newton_iterations, "gpu it", st_g.newton_iterations,
+          "pcg", st_g.pcg_iterations, "exit", st_o.exit_error, st_g.exit_error)
+    sa = np.max(np.abs(acc_o))
+    sf = np.max(np.abs(lf_o))
+    err_a = np.max(np.abs(acc_g - acc_o)) / sa
+    err_f = np.max(np.abs(lf_g - lf_o)) / sf
+    print("rel err acc %.3e force %.3e" % (err_a, err_f))
+    assert st_g.char_mass == pytest.approx(st_o.char_mass, rel=1e-13)
+    assert st_g.char_speed == pytest.approx(st_o.char_speed, rel=1e-13)
+    assert err_a < tol_rel
+    assert err_f < tol_rel
+    return err_a, err_f
+
+
+def test_fill_forces_stack_fp64(require_gpu):
+    sc = scenes.stack_10(tolerance=1e-10)
+    p = sc.params
+    p.pcg_tol = 1e-13
+    _fill_parity(sc, 30, p)
+
+
+def test_fill_forces_pyramid_fp64(require_gpu):
+    sc = scenes.pyramid_210(tolerance=1e-10)
+    p = sc.params
+    p.pcg_tol = 1e-13
+    _fill_parity(sc, 25, p)
+
+
+def test_fill_forces_pyramid_fp32_inner(require_gpu):
+    """FP32 PCG inner solve, FP64 Newton outer loop: same 1e-6 bar."""
+    sc = scenes.pyramid_210(tolerance=1e-10)
+    p = sc.params
+    p.pcg_precision = PCG_FP32
+    p.pcg_tol = 1e-6
+    _fill_parity(sc, 25, p)
+
+
+def test_fill_forces_only_normal(require_gpu):
+    sc = scenes.stack_10(tolerance=1e-10, force_model=ONLY_NORMAL)
+    p = sc.params
+    p.pcg_tol = 1e-13
+    _fill_parity(sc, 20, p)
+
+
+def test_fill_forces_spring_and_restitution(require_gpu):
+    sc = scenes.pile(40, seed=5, half=2.0, height=6.0, name="pile_40", grid_n=48, sample_spacing=0.12,
+                     tolerance=1e-10, spring_based_force=1, spring_k=1e4, spring_damp=0.1)
+    p = sc.params
+    p.pcg_precision = PCG_FP64
+    p.pcg_tol = 1e-13
+    _fill_parity(sc, 150, p, tol_rel=1e-5)
+    sc2 = scenes.pile(40, seed=5, half=2.0, height=6.0, name="pile_40", grid_n=48, sample_spacing=0.12,
+                      tolerance=1e-10, restitution=0.3)
+    p2 = sc2.params
+    p2.pcg_precision = PCG_FP64
+    p2.pcg_tol = 1e-13
+    _fill_parity(sc2, 150, p2, tol_rel=1e-5)
+
+
+def _traj_errors(sc, o_state, g_state):
+    pos_o, q_o = o_state[0], o_state[1]
+    pos_g, q_g = g_state[0], g_state[1]
+    scale = np.max(sc.dims, axis=1)
+    dyn = sc.is_static == 0
+    ep = np.max(np.linalg.norm(pos_g - pos_o, axis=1)[dyn] / scale[dyn])
+    dots = np.clip(np.abs(np.sum(q_g * q_o, axis=1)), 0, 1)
+    ea = np.max(2 * np.arccos(dots)[dyn])
+    return ep, ea
+
+
+def test_trajectory_stack_fp64(require_gpu):
+    sc = scenes.stack_10(tolerance=1e-10)
+    p = sc.params
+    p.pcg_tol = 1e-13
+    o = orc.OracleScene(sc)
+    o.set_linear_solver(orc.SOLVER_DIRECT)
+    ds = _gpu_scene(sc, p)
+    for k in range(40):
+        so = o.step(1)
+        sg = ds.step(1)
+        assert so.n_contacts == sg.n_contacts
+    ep, ea = _traj_errors(sc, o.get_state(), ds.state())
+    print("stack 40 steps: position err %.3e (of body size), angle err %.3e rad" % (ep, ea))
+    assert ep < 1e-6 and ea < 1e-6
+
+
+def test_trajectory_pyramid_fp32_inner(require_gpu):
+    sc = scenes.pyramid_210(tolerance=1e-10)
+    p = sc.params
+    p.pcg_precision = PCG_FP32
+    p.pcg_tol = 1e-6
+    o = orc.OracleScene(sc)
+    o.set_threads(8)
+    ds = _gpu_scene(sc, p)
+    for k in range(30):
+        so = o.step(1)
+        sg = ds.step(1)
+    ep, ea = _traj_errors(sc, o.get_state(), ds.state())
+    print("pyramid 30 steps FP32-PCG: position err %.3e, angle err %.3e" % (ep, ea))
+    assert ep < 1e-6 and ea < 1e-6
+
+
+def test_step_is_deterministic(require_gpu):
+    sc = scenes.pile(60, seed=3, half=2.5, height=8.0, name="pile_60", grid_n=48, sample_spacing=0.1)
+    o = _advance_oracle(sc, 200)
+    state = o.get_state()
+    res = []
+    for _ in range(2):
+        ds = _gpu_scene(sc)
+        ds.set_state(*state)
+        ds.step(5)
+        res.append(ds.state())
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_settled_pile_one_step(require_gpu):
+    """Per-step parity at ~30k contacts from an identical input state (the
+    settled seeded pile): contact sets identical modulo ties; the FP32-PCG /
+    FP64-Newton step within the 1e-6 bar of the FP64 oracle."""
+    path = os.path.join(HERE, "golden", "pile_2000_settled.npz")
+    if not os.path.exists(path):
+        pytest.skip("settled pile fixture not generated")
+    z = np.load(path)
+    sc = scenes.pile_2000()
+    state = (z["pos"], z["quat"], z["vel"], z["angvel"])
+    ds = _gpu_scene(sc)
+    ds.set_state(*state)
+    cg, ties = _contacts_gpu(ds)
+    o = orc.OracleScene(sc)
+    o.set_threads(16)
+    o.set_state(*state)
+    n, t2 = o.detect()
+    co = o.contacts()
+    print("settled pile: contacts %d (gpu %d), ties %d/%d" % (n, cg.n, ties, t2))
+    assert n > 25000
+    assert_same_contacts(cg, co, max(ties, t2))
