@@ -26,15 +26,21 @@ from primal_dual_contacts_miror_amd import pdc, scenes  # noqa: E402
 
 METRIC = "Newton timesteps/sec at 30k contacts; batched scene-steps/sec + p50/p99 latency"
 PEAK_HBM = 8.0e12
-KT_NAMES = {"pcg": pdc.KT_PCG, "contact_terms": pdc.KT_CONTACT_TERMS, "contacts": pdc.KT_CONTACTS}
+# kernel classes timed with HIP events (include/pdc.h PDC_KT_*)
+KT_NAMES = {"spmv": pdc.KT_SPMV, "pcg_update": pdc.KT_PCG, "contact_terms": pdc.KT_CONTACT_TERMS,
+            "contacts": pdc.KT_CONTACTS}
+KT_KERNEL = {"spmv": "k_pcg_a (block-CSR SpMV q = A z + beta q, p update, p.q partials)",
+             "pcg_update": "k_pcg_b (x, r updates, block-Jacobi z = P r, partials)",
+             "contact_terms": "k_constraints + k_contact_res", "contacts": "k_pair_hits"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def settled_state(sc, name, settle_steps=None):
-    path = os.path.join(ROOT, "tests", "golden", "%s_settled.npz" % name)
+def settled_state(sc, name, settle_steps=None, path=None):
+    if path is None:
+        path = os.path.join(ROOT, "tests", "golden", "%s_settled.npz" % name)
     if settle_steps is None and os.path.exists(path):
         z = np.load(path)
         return (z["pos"], z["quat"], z["vel"], z["angvel"]), "fixture"
@@ -84,10 +90,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pile_2000")
-    ap.add_argument("--kernel", default="pcg", choices=sorted(KT_NAMES))
+    ap.add_argument("--kernel", default="spmv", choices=sorted(KT_NAMES))
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--state", default=None, help="npz with pos, quat, vel, angvel to start from")
     ap.add_argument("--settle-steps", type=int, default=None,
                     help="settle on the device for this many steps instead of loading the settled state")
     args = ap.parse_args()
@@ -96,7 +103,7 @@ def main():
               "(scene-level sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
     sc = scenes.WORKLOADS[args.workload]()
-    state, origin = settled_state(sc, args.workload, args.settle_steps)
+    state, origin = settled_state(sc, args.workload, args.settle_steps, args.state)
     ds = pdc.DynamicSystem(sc)
     ds.set_state(*state)
     log("warmup %d steps" % args.warmup)
@@ -145,7 +152,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": args.kernel,
+            "kernel": KT_KERNEL[args.kernel],
             "achieved": achieved / 1e9,
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",

@@ -189,6 +189,11 @@ void pdc_reset_timings(pdc_context* ctx);
  * pdc_kernel_timing returns the summed milliseconds, the launch count and the
  * summed algorithmic bytes (compulsory traffic: each input read once, each
  * output written once) of those launches. */
+/* Classes: SPMV = the PCG kernel holding the block-CSR SpMV (k_pcg_a);
+ * PCG = the PCG vector-update / preconditioner kernel (k_pcg_b);
+ * CONTACT_TERMS = constraint + contact-residual kernels of a Newton
+ * iteration; ASSEMBLY = the assembly kernels; LINE_SEARCH = the contact
+ * kernel of a line-search trial; CONTACTS = the sample-test kernel. */
 enum { PDC_KT_NONE = 0, PDC_KT_SPMV = 1, PDC_KT_CONTACT_TERMS = 2,
        PDC_KT_ASSEMBLY = 3, PDC_KT_LINE_SEARCH = 4, PDC_KT_CONTACTS = 5,
        PDC_KT_PCG = 6 };

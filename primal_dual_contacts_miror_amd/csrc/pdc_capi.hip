@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <numeric>
 
 #include "pdc_internal.h"
@@ -91,17 +92,19 @@ void free_all(pdc_context* c) {
 	c->ap_start.release(); c->g_flag.release(); c->g2l.release(); c->l2g.release();
 	c->li0.release(); c->li1.release(); c->adj_cnt.release(); c->adj_ptr.release();
 	c->adj_cur.release(); c->adj_nb.release(); c->adj_pair.release(); c->a_cnt.release();
-	c->a_ptr.release(); c->a_col.release(); c->a_pair.release(); c->a_diag.release();
+	c->a_ptr.release(); c->a_col.release(); c->a_diag.release();
+	c->adj_pos.release(); c->adj_blk.release(); c->adj_nb_u.release(); c->adj_pair_u.release();
+	c->adj_row_u.release(); c->pair_adj.release(); c->inc_cnt.release(); c->inc_ptr.release();
+	c->inc.release(); c->c_pid.release();
 	c->l_static.release(); c->l_v.release(); c->l_vold.release(); c->l_Ms.release();
 	c->l_Minv.release(); c->l_vphys.release(); c->l_speed.release(); c->l_ru.release();
 	c->l_rhs.release(); c->l_D.release(); c->l_du.release(); c->l_gdu.release();
 	c->s_K.release(); c->s_f.release(); c->s_lam.release(); c->s_c.release(); c->s_g.release();
 	c->s_H.release(); c->s_rf.release(); c->s_rl.release(); c->s_sf.release(); c->s_invC.release();
-	c->s_W.release(); c->s_kf.release(); c->s_df.release(); c->s_dl.release(); c->s_ft.release();
-	c->s_lt.release(); c->partials.release();
+	c->s_W.release(); c->s_kf.release(); c->s_df.release(); c->s_dl.release(); c->partials.release();
 	c->A32.release(); c->P32.release(); c->b32.release(); c->x32.release(); c->pcg32.release();
 	c->A64.release(); c->P64.release(); c->b64.release(); c->x64.release(); c->pcg64.release();
-	c->pcg_part.release(); c->d_sc.release();
+	c->d_sc.release();
 	c->c_stride = 0;
 }
 
@@ -159,6 +162,8 @@ int pdc_create(int device, const pdc_params* params, pdc_context** out) {
 		PDC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
 		ctx->d_sc.ensure(1);
 		PDC_HIP(hipHostMalloc((void**)&ctx->h_sc, sizeof(pdc::Scalars), hipHostMallocDefault));
+		PDC_HIP(hipHostMalloc(&ctx->h_pcg_args, 512, hipHostMallocDefault));
+		if (const char* g = std::getenv("PDC_NO_GRAPHS")) ctx->use_graphs = g[0] == '0';
 		std::memset(ctx->h_sc, 0, sizeof(pdc::Scalars));
 		PDC_HIP(hipMemsetAsync(ctx->d_sc.p, 0, sizeof(pdc::Scalars), ctx->stream));
 		hipDeviceProp_t prop;
@@ -181,7 +186,15 @@ void pdc_destroy(pdc_context* ctx) {
 	if (!ctx) return;
 	(void)hipSetDevice(ctx->device);
 	if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+	for (auto& G : ctx->pcg_graph) {
+		if (G.exec) (void)hipGraphExecDestroy(G.exec);
+		if (G.graph) (void)hipGraphDestroy(G.graph);
+		for (auto e : G.ev) (void)hipEventDestroy(e);
+	}
+	for (auto e : ctx->kt_direct) (void)hipEventDestroy(e);
 	free_all(ctx);
+	ctx->pcg_args.release();
+	if (ctx->h_pcg_args) (void)hipHostFree(ctx->h_pcg_args);
 	for (auto e : ctx->kt.pool) (void)hipEventDestroy(e);
 	if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

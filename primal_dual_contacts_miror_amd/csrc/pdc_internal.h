@@ -70,7 +70,11 @@ struct Scalars {
 	// line search trial
 	double t_ru2, t_wrf2, t_rl2, t_ac2, t_gErr_u, t_gErr_f, t_gErr_l;
 	double t_inErr, t_sacIn, t_gErr;
+	double ls_step;
+	int ls_decided, ls_trials;
 	int pcg_iters;
+	int pcg_done;
+	int pcg_itc[2];
 	int overflow;        // capacity overflow flag set by kernels
 	int n_pairs;         // broadphase candidate pairs
 	int n_contacts;      // contacts after merge
@@ -79,7 +83,15 @@ struct Scalars {
 	int n_local;         // bodies in contact
 	int n_apairs;        // pairs with contacts
 	int nnzb;            // blocks of the Newton matrix
-	int pad;
+};
+
+struct PcgGraph {
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	int nblk = 0;
+	const void* args = nullptr;
+	int kt_which = 0;
+	std::vector<hipEvent_t> ev;
 };
 
 struct KernelTimer {
@@ -143,21 +155,36 @@ struct pdc_context {
 	pdc::DBuf<int> ap_a, ap_b, ap_start;                    // active pairs (global ids), start (NP+1)
 	pdc::DBuf<int> g_flag, g2l, l2g;                        // per global body
 	pdc::DBuf<int> li0, li1;                                // per contact local ids
-	pdc::DBuf<int> adj_cnt, adj_ptr, adj_cur, adj_nb, adj_pair;
-	pdc::DBuf<int> a_cnt, a_ptr, a_col, a_pair, a_diag;
+	pdc::DBuf<int> adj_cnt, adj_ptr, adj_cur, adj_nb, adj_pair, adj_pos, adj_blk;
+	pdc::DBuf<int> adj_nb_u, adj_pair_u, adj_row_u, pair_adj;
+	pdc::DBuf<int> inc_cnt, inc_ptr, inc, c_pid;
+	pdc::DBuf<int> a_cnt, a_ptr, a_col, a_diag;
+	int pcg_last_iters = 0;
+	pdc::DBuf<char> pcg_args;                                // PCG arguments (device)
+	void* h_pcg_args = nullptr;                               // pinned staging copy
+	pdc::PcgGraph pcg_graph[2];                               // recorded chunks (FP32, FP64)
+	bool use_graphs = true;
+	bool graph_events_ok = true;
+	std::vector<hipEvent_t> kt_direct;                        // events for direct-launch timing
+	hipEvent_t* kt_direct_ev(size_t n) {
+		while (kt_direct.size() < n) {
+			hipEvent_t e;
+			if (hipEventCreate(&e) != hipSuccess) break;
+			kt_direct.push_back(e);
+		}
+		return kt_direct.data();
+	}
 
 	// ---- solver (local numbering) ----
 	pdc::DBuf<int> l_static;
 	pdc::DBuf<double> l_v, l_vold, l_Ms, l_Minv, l_vphys;   // 6,6,7,7,6 per body
-	pdc::DBuf<unsigned long long> l_speed;                   // objSpeed as ordered bits
+	pdc::DBuf<double> l_speed;                               // objSpeed per body
 	pdc::DBuf<double> l_ru, l_rhs, l_D, l_du, l_gdu;        // 6 per body
 	pdc::DBuf<double> s_K, s_f, s_lam, s_c, s_g, s_H;        // per contact
 	pdc::DBuf<double> s_rf, s_rl, s_sf, s_invC, s_W, s_kf, s_df, s_dl;
-	pdc::DBuf<double> s_ft, s_lt;                            // trial force / multiplier
 	pdc::DBuf<double> partials;                              // [k][kMaxPartialBlocks]
 	pdc::DBuf<float> A32, P32, b32, x32, pcg32;              // FP32 PCG
 	pdc::DBuf<double> A64, P64, b64, x64, pcg64;             // FP64 PCG
-	pdc::DBuf<double> pcg_part;
 	pdc::DBuf<pdc::Scalars> d_sc;
 	pdc::Scalars* h_sc = nullptr;                            // pinned mirror
 
@@ -177,6 +204,7 @@ void sync_scalars(pdc_context* ctx);  // device Scalars -> ctx->h_sc (blocking)
 void kt_begin(pdc_context* ctx, int which, hipEvent_t* ev);
 void kt_end(pdc_context* ctx, int which, hipEvent_t ev0);
 void kt_collect(pdc_context* ctx);
+void kt_end_nocollect(pdc_context* ctx, hipEvent_t ev0);
 
 // pdc_contacts.hip
 void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,

@@ -24,10 +24,11 @@ namespace cg = cooperative_groups;
 namespace pdc {
 
 constexpr int kMaxWG = 1024;  // workgroups of the reducing 1-D kernels
+constexpr int kTrials = 10;   // line-search trial steps evaluated per fused pass
 enum {
-	P_GAP = 0, P_WRF, P_RL, P_AC, P_RU,
-	P_T_WRF, P_T_RL, P_T_AC, P_T_GF, P_T_GL, P_T_RU, P_T_GU,
-	P_ND_TRACE, P_ND_CNT, P_COUNT
+	P_GAP = 0, P_WRF, P_RL, P_AC, P_RU, P_ND_TRACE, P_ND_CNT,
+	P_LS,                                  // 7 slots per trial: wrf2 rl2 ac2 gf gl ru2 gu
+	P_COUNT = P_LS + 7 * kTrials
 };
 
 inline int wg_for(int64_t n) { return std::min(grid_for(n), kMaxWG); }
@@ -89,83 +90,131 @@ __global__ __launch_bounds__(256) void k_adj_count(int NP, const int* __restrict
 	atomicAdd(&cnt[g2l[pb[p]]], 1);
 }
 
+// Unsorted adjacency (the slot order depends on the atomics; k_adj_rank
+// reorders it deterministically).
 __global__ __launch_bounds__(256) void k_adj_fill(int NP, const int* __restrict__ pa, const int* __restrict__ pb,
                            const int* __restrict__ g2l, int* __restrict__ cur, int* __restrict__ nb,
-                           int* __restrict__ pair) {
+                           int* __restrict__ pair, int* __restrict__ row) {
 	const int p = blockIdx.x * blockDim.x + threadIdx.x;
 	if (p >= NP) return;
 	const int la = g2l[pa[p]], lb = g2l[pb[p]];
 	int k = atomicAdd(&cur[la], 1);
 	nb[k] = lb;
 	pair[k] = p;
+	row[k] = la;
 	k = atomicAdd(&cur[lb], 1);
 	nb[k] = la;
 	pair[k] = p;
+	row[k] = lb;
 }
 
-// Sorts each adjacency row by partner (partners are unique per row), which
-// makes the row order independent of the atomic fill order.
-__global__ __launch_bounds__(256) void k_adj_sort(int NL, const int* __restrict__ ptr, int* __restrict__ nb,
-                           int* __restrict__ pair) {
-	const int r = blockIdx.x * blockDim.x + threadIdx.x;
-	if (r >= NL) return;
-	const int s = ptr[r], e = ptr[r + 1];
-	for (int i = s + 1; i < e; ++i) {
-		const int kn = nb[i], kp = pair[i];
-		int j = i - 1;
-		while (j >= s && nb[j] > kn) {
-			nb[j + 1] = nb[j];
-			pair[j + 1] = pair[j];
-			--j;
+// Sorted position of each adjacency entry = its rank by partner within the
+// row (partners are unique per row).  Also records, per pair, the sorted
+// adjacency entry of each of its two ends.
+__global__ __launch_bounds__(256) void k_adj_rank(int E, const int* __restrict__ ptr, const int* __restrict__ nb_u,
+                           const int* __restrict__ pair_u, const int* __restrict__ row_u,
+                           const int* __restrict__ pa, const int* __restrict__ g2l,
+                           int* __restrict__ nb, int* __restrict__ pair, int* __restrict__ pair_adj) {
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= E) return;
+	const int r = row_u[k], me = nb_u[k];
+	int rank = 0;
+	for (int q = ptr[r]; q < ptr[r + 1]; ++q) rank += nb_u[q] < me ? 1 : 0;
+	const int dst = ptr[r] + rank;
+	const int p = pair_u[k];
+	nb[dst] = me;
+	pair[dst] = p;
+	pair_adj[2 * p + (g2l[pa[p]] == r ? 0 : 1)] = dst;
+}
+
+// One wavefront per local body: exclusive prefix of the pair sizes along the
+// sorted adjacency row (the body's incidence offsets), the row's contact
+// total, the static flag and the block count of its Newton-matrix row.
+__global__ __launch_bounds__(256) void k_adj_pos(int NL, const int* __restrict__ ptr, const int* __restrict__ nb,
+                          const int* __restrict__ pair, const int* __restrict__ pstart,
+                          const int* __restrict__ l2g, const int* __restrict__ bstat,
+                          int* __restrict__ adj_pos, int* __restrict__ inc_cnt, int* __restrict__ a_cnt,
+                          int* __restrict__ lstat) {
+	const int lane = threadIdx.x & 63;
+	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const int nw = (gridDim.x * blockDim.x) >> 6;
+	for (int r = wave; r < NL; r += nw) {
+		const int st = bstat[l2g[r]];
+		int run = 0, act = 0;
+		for (int base = ptr[r]; base < ptr[r + 1]; base += 64) {
+			const int k = base + lane;
+			int sz = 0, ac = 0;
+			if (k < ptr[r + 1]) {
+				const int p = pair[k];
+				sz = pstart[p + 1] - pstart[p];
+				ac = bstat[l2g[nb[k]]] ? 0 : 1;
+			}
+			int x = sz;
+#pragma unroll
+			for (int off = 1; off < 64; off <<= 1) {
+				const int y = __shfl_up(x, off, 64);
+				if (lane >= off) x += y;
+			}
+			if (k < ptr[r + 1]) adj_pos[k] = run + x - sz;
+			run += __shfl(x, 63, 64);
+			act += wave_sum(ac);
 		}
-		nb[j + 1] = kn;
-		pair[j + 1] = kp;
+		if (lane == 0) {
+			inc_cnt[r] = run;
+			a_cnt[r] = 1 + (st ? 0 : act);
+			lstat[r] = st;
+		}
 	}
 }
 
-__global__ __launch_bounds__(256) void k_a_count(int NL, const int* __restrict__ l2g, const int* __restrict__ bstat,
-                          const int* __restrict__ ptr, const int* __restrict__ nb,
-                          int* __restrict__ lstat, int* __restrict__ cnt) {
-	const int r = blockIdx.x * blockDim.x + threadIdx.x;
-	if (r >= NL) return;
-	const int st = bstat[l2g[r]];
-	lstat[r] = st;
-	int c = 1;
-	if (!st)
-		for (int k = ptr[r]; k < ptr[r + 1]; ++k) c += bstat[l2g[nb[k]]] ? 0 : 1;
-	cnt[r] = c;
+// Incidence list: entry (contact i, side j) of body r at
+// inc_ptr[r] + adj_pos[entry of (r, pair)] + (i - pair start).
+__global__ __launch_bounds__(256) void k_inc_fill(int C, const int* __restrict__ flag, const int* __restrict__ pex,
+                           const int* __restrict__ li0, const int* __restrict__ li1,
+                           const int* __restrict__ pa, const int* __restrict__ g2l,
+                           const int* __restrict__ pstart, const int* __restrict__ pair_adj,
+                           const int* __restrict__ adj_pos, const int* __restrict__ inc_ptr,
+                           int* __restrict__ pid, int* __restrict__ inc) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= C) return;
+	const int p = pex[i] + flag[i] - 1;
+	pid[i] = p;
+	const int ra = g2l[pa[p]];
+	const int off = i - pstart[p];
+	for (int j = 0; j < 2; ++j) {
+		const int r = j ? li1[i] : li0[i];
+		const int k = pair_adj[2 * p + (r == ra ? 0 : 1)];
+		inc[inc_ptr[r] + adj_pos[k] + off] = 2 * i + j;
+	}
 }
 
 // Block-CSR pattern of the Newton matrix (linsysReserve, NonSmoothForceBase.h:151-187):
-// the diagonal block of every body plus one block per active-active pair.
+// the diagonal block of every body plus one block per active-active pair,
+// columns ascending; adj_blk maps adjacency entries to their block (-1: none).
 __global__ __launch_bounds__(256) void k_a_fill(int NL, const int* __restrict__ lstat, const int* __restrict__ ptr,
-                         const int* __restrict__ nb, const int* __restrict__ pair,
-                         const int* __restrict__ aptr, int* __restrict__ acol,
-                         int* __restrict__ apair, int* __restrict__ adiag) {
+                         const int* __restrict__ nb, const int* __restrict__ aptr,
+                         int* __restrict__ acol, int* __restrict__ adiag, int* __restrict__ adj_blk) {
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= NL) return;
 	int o = aptr[r];
 	bool diag_done = false;
-	if (!lstat[r]) {
-		for (int k = ptr[r]; k < ptr[r + 1]; ++k) {
-			const int c = nb[k];
-			if (lstat[c]) continue;
-			if (!diag_done && c > r) {
-				adiag[r] = o;
-				acol[o] = r;
-				apair[o] = -1;
-				++o;
-				diag_done = true;
-			}
-			acol[o] = c;
-			apair[o] = pair[k];
-			++o;
+	for (int k = ptr[r]; k < ptr[r + 1]; ++k) {
+		const int c = nb[k];
+		if (lstat[r] || lstat[c]) {
+			adj_blk[k] = -1;
+			continue;
 		}
+		if (!diag_done && c > r) {
+			adiag[r] = o;
+			acol[o++] = r;
+			diag_done = true;
+		}
+		adj_blk[k] = o;
+		acol[o++] = c;
 	}
 	if (!diag_done) {
 		adiag[r] = o;
 		acol[o] = r;
-		apair[o] = -1;
 	}
 }
 
@@ -183,8 +232,7 @@ void ensure_contact_capacity(pdc_context* ctx, size_t C) {
 	re(ctx->li0, 1); re(ctx->li1, 1);
 	re(ctx->s_K, 1); re(ctx->s_f, 3); re(ctx->s_lam, 2); re(ctx->s_c, 2); re(ctx->s_g, 2);
 	re(ctx->s_H, 3); re(ctx->s_rf, 3); re(ctx->s_rl, 2); re(ctx->s_sf, 3); re(ctx->s_invC, 2);
-	re(ctx->s_W, 4); re(ctx->s_kf, 3); re(ctx->s_df, 3); re(ctx->s_dl, 2); re(ctx->s_ft, 3);
-	re(ctx->s_lt, 2);
+	re(ctx->s_W, 4); re(ctx->s_kf, 3); re(ctx->s_df, 3); re(ctx->s_dl, 2);
 	ctx->c_stride = (int)cs;
 }
 
@@ -213,8 +261,11 @@ void build_topology(pdc_context* ctx) {
 	hipLaunchKernelGGL(k_contact_local, dim3(grid_for(C)), dim3(kBlock), 0, s, C, ctx->c_id0.p,
 	                   ctx->c_id1.p, ctx->g2l.p, ctx->li0.p, ctx->li1.p);
 	PDC_CHECK_LAUNCH();
+	const size_t E = 2 * (size_t)NP;
 	ctx->adj_cnt.ensure(NL + 1); ctx->adj_ptr.ensure(NL + 1); ctx->adj_cur.ensure(NL + 1);
-	ctx->adj_nb.ensure(2 * (size_t)NP); ctx->adj_pair.ensure(2 * (size_t)NP);
+	ctx->adj_nb.ensure(E); ctx->adj_pair.ensure(E); ctx->adj_pos.ensure(E); ctx->adj_blk.ensure(E);
+	ctx->adj_nb_u.ensure(E); ctx->adj_pair_u.ensure(E); ctx->adj_row_u.ensure(E);
+	ctx->pair_adj.ensure(E);
 	PDC_HIP(hipMemsetAsync(ctx->adj_cnt.p, 0, (NL + 1) * sizeof(int), s));
 	hipLaunchKernelGGL(k_adj_count, dim3(grid_for(NP)), dim3(kBlock), 0, s, NP, ctx->ap_a.p,
 	                   ctx->ap_b.p, ctx->g2l.p, ctx->adj_cnt.p);
@@ -223,23 +274,33 @@ void build_topology(pdc_context* ctx) {
 	PDC_HIP(hipMemcpyAsync(ctx->adj_cur.p, ctx->adj_ptr.p, (NL + 1) * sizeof(int),
 	                       hipMemcpyDeviceToDevice, s));
 	hipLaunchKernelGGL(k_adj_fill, dim3(grid_for(NP)), dim3(kBlock), 0, s, NP, ctx->ap_a.p,
-	                   ctx->ap_b.p, ctx->g2l.p, ctx->adj_cur.p, ctx->adj_nb.p, ctx->adj_pair.p);
+	                   ctx->ap_b.p, ctx->g2l.p, ctx->adj_cur.p, ctx->adj_nb_u.p, ctx->adj_pair_u.p,
+	                   ctx->adj_row_u.p);
 	PDC_CHECK_LAUNCH();
-	hipLaunchKernelGGL(k_adj_sort, dim3(grid_for(NL)), dim3(kBlock), 0, s, NL, ctx->adj_ptr.p,
-	                   ctx->adj_nb.p, ctx->adj_pair.p);
+	hipLaunchKernelGGL(k_adj_rank, dim3(grid_for((int64_t)E)), dim3(kBlock), 0, s, (int)E, ctx->adj_ptr.p,
+	                   ctx->adj_nb_u.p, ctx->adj_pair_u.p, ctx->adj_row_u.p, ctx->ap_a.p, ctx->g2l.p,
+	                   ctx->adj_nb.p, ctx->adj_pair.p, ctx->pair_adj.p);
 	PDC_CHECK_LAUNCH();
 	ctx->l_static.ensure(NL + 1); ctx->a_cnt.ensure(NL + 1); ctx->a_ptr.ensure(NL + 1);
-	ctx->a_diag.ensure(NL + 1);
-	hipLaunchKernelGGL(k_a_count, dim3(grid_for(NL)), dim3(kBlock), 0, s, NL, ctx->l2g.p,
-	                   ctx->b_static.p, ctx->adj_ptr.p, ctx->adj_nb.p, ctx->l_static.p, ctx->a_cnt.p);
+	ctx->a_diag.ensure(NL + 1); ctx->inc_cnt.ensure(NL + 1); ctx->inc_ptr.ensure(NL + 1);
+	hipLaunchKernelGGL(k_adj_pos, dim3(std::max(1, std::min((NL + 3) / 4, 1024))), dim3(kBlock), 0, s, NL,
+	                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->adj_pair.p, ctx->ap_start.p, ctx->l2g.p,
+	                   ctx->b_static.p, ctx->adj_pos.p, ctx->inc_cnt.p, ctx->a_cnt.p, ctx->l_static.p);
 	PDC_CHECK_LAUNCH();
+	scan_exclusive(ctx, ctx->inc_cnt.p, ctx->inc_ptr.p, NL);
 	scan_exclusive(ctx, ctx->a_cnt.p, ctx->a_ptr.p, NL);
+	ctx->inc.ensure(2 * (size_t)C);
+	ctx->c_pid.ensure((size_t)C);
+	hipLaunchKernelGGL(k_inc_fill, dim3(grid_for(C)), dim3(kBlock), 0, s, C, ctx->t_key_flag.p, ctx->t_pid.p,
+	                   ctx->li0.p, ctx->li1.p, ctx->ap_a.p, ctx->g2l.p, ctx->ap_start.p, ctx->pair_adj.p,
+	                   ctx->adj_pos.p, ctx->inc_ptr.p, ctx->c_pid.p, ctx->inc.p);
+	PDC_CHECK_LAUNCH();
 	const int NNZB = read_int(ctx, ctx->a_ptr.p + NL);
 	ctx->NNZB = NNZB;
-	ctx->a_col.ensure(NNZB); ctx->a_pair.ensure(NNZB);
+	ctx->a_col.ensure(NNZB);
 	hipLaunchKernelGGL(k_a_fill, dim3(grid_for(NL)), dim3(kBlock), 0, s, NL, ctx->l_static.p,
-	                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->adj_pair.p, ctx->a_ptr.p, ctx->a_col.p,
-	                   ctx->a_pair.p, ctx->a_diag.p);
+	                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->a_ptr.p, ctx->a_col.p, ctx->a_diag.p,
+	                   ctx->adj_blk.p);
 	PDC_CHECK_LAUNCH();
 }
 
@@ -256,18 +317,22 @@ struct SArgs {
 	const double* T;        // [36][cs]
 	const double* depth;
 	// topology
-	const int* adj_ptr;
-	const int* adj_pair;
-	const int* pstart;
+	const int* pstart;    // per pair: first contact (NP+1)
+	const int* pa;        // per pair: global id of the first body
+	const int* g2l;
 	const int* lstat;
 	const int* l2g;
+	const int* inc_ptr;   // per body: incidence offsets (NL+1)
+	const int* inc;       // incidence entries 2*contact + side
+	const int* pair_adj;  // per pair: sorted adjacency entry of each end
+	const int* adj_blk;   // per adjacency entry: Newton-matrix block or -1
 	// bodies
 	double* v;
 	double* vold;
 	double* vphys;
 	double* Ms;
 	double* Minv;
-	unsigned long long* speed;
+	double* speed;
 	double* ru;
 	double* rhs;
 	double* D;
@@ -287,8 +352,6 @@ struct SArgs {
 	double* kf;
 	double* df;
 	double* dl;
-	double* ft;
-	double* lt;
 	double* part;
 	Scalars* sc;
 };
@@ -349,7 +412,6 @@ __global__ __launch_bounds__(256) void k_setup_bodies(SArgs a, const double* __r
 			a.Ms[7 * r + 1 + k] = Iw[k];
 			a.Minv[7 * r + 1 + k] = Ii[k];
 		}
-		a.speed[r] = 0ull;
 		if (!st) {
 			acc[0] += 3 * m + (Iw[0] + Iw[3] + Iw[5]);
 			acc[1] += 1.0;
@@ -359,35 +421,51 @@ __global__ __launch_bounds__(256) void k_setup_bodies(SArgs a, const double* __r
 	write_partials<2>(acc, a.part, slots);
 }
 
-// objSpeed per body (PrimalDualForceSolver.cpp:38-63); max via ordered bits.
-__global__ __launch_bounds__(256) void k_nondim_contacts(SArgs a) {
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
-		const int l[2] = {a.li0[i], a.li1[i]};
-		bool hasStatic = false;
-		double rel[3] = {0, 0, 0};
-		for (int j = 0; j < 2; ++j) {
-			double ve[6];
-			for (int k = 0; k < 6; ++k) ve[k] = a.vphys[6 * l[j] + k];
-			if (!a.lstat[l[j]]) {
-				for (int k = 0; k < 3; ++k) ve[k] += a.gdt[k];
-			} else {
-				hasStatic = true;
-			}
-			double half[3];
-			for (int col = 0; col < 3; ++col) {
-				double s = 0;
-				for (int r = 0; r < 6; ++r) s += Tc(a, i, j, col * 6 + r) * ve[r];
-				half[col] = -s;
-			}
-			for (int k = 0; k < 3; ++k) rel[k] += half[k];
-			const double hn = sqrt(dot3(half, half));
-			atomicMax(&a.speed[l[j]], (unsigned long long)__double_as_longlong(hn));
+// objSpeed per body (PrimalDualForceSolver.cpp:38-63): one wavefront per
+// body takes the maximum over its incident contacts of the norm of its own
+// half relative velocity and, when the contact involves a static body, of
+// the full relative velocity.
+__global__ __launch_bounds__(256) void k_nondim_speed(SArgs a) {
+	const int lane = threadIdx.x & 63;
+	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const int nw = (gridDim.x * blockDim.x) >> 6;
+	for (int r = wave; r < a.NL; r += nw) {
+		double m = 0;
+		if (a.lstat[r]) {
+			if (lane == 0) a.speed[r] = 0;
+			continue;  // charSpeed averages over active bodies only
 		}
-		if (hasStatic) {
-			const double rn = sqrt(dot3(rel, rel));
-			for (int j = 0; j < 2; ++j)
-				atomicMax(&a.speed[l[j]], (unsigned long long)__double_as_longlong(rn));
+		for (int k = a.inc_ptr[r] + lane; k < a.inc_ptr[r + 1]; k += 64) {
+			const int e = a.inc[k];
+			const int i = e >> 1, js = e & 1;
+			const int l[2] = {a.li0[i], a.li1[i]};
+			bool hasStatic = false;
+			double rel[3] = {0, 0, 0}, own = 0;
+			for (int j = 0; j < 2; ++j) {
+				double ve[6];
+				for (int q = 0; q < 6; ++q) ve[q] = a.vphys[6 * l[j] + q];
+				if (!a.lstat[l[j]]) {
+					for (int q = 0; q < 3; ++q) ve[q] += a.gdt[q];
+				} else {
+					hasStatic = true;
+				}
+				double half[3];
+				for (int col = 0; col < 3; ++col) {
+					double t = 0;
+					for (int q = 0; q < 6; ++q) t += Tc(a, i, j, col * 6 + q) * ve[q];
+					half[col] = -t;
+				}
+				for (int q = 0; q < 3; ++q) rel[q] += half[q];
+				if (j == js) own = sqrt(dot3(half, half));
+			}
+			m = own > m ? own : m;
+			if (hasStatic) {
+				const double rn = sqrt(dot3(rel, rel));
+				m = rn > m ? rn : m;
+			}
 		}
+		m = wave_max(m);
+		if (lane == 0) a.speed[r] = m;
 	}
 }
 
@@ -402,7 +480,7 @@ __global__ __launch_bounds__(1024) void k_nondim_final(SArgs a, int nwg, const d
 		if (a.lstat[r]) continue;
 		const double m = mass[a.l2g[r]];
 		v[0] += m;
-		v[1] += __longlong_as_double((long long)a.speed[r]) * m;
+		v[1] += a.speed[r] * m;
 	}
 	double out[2];
 	block_sum<2, double>(v, lds, out);
@@ -570,55 +648,48 @@ __global__ __launch_bounds__(256) void k_contact_res(SArgs a, int nwg_gap, doubl
 	write_partials<3>(acc, a.part, slots);
 }
 
-// Iterates the contacts incident to local body r: fn(i, side).
-template <class F>
-__device__ __forceinline__ void for_incident(const SArgs& a, int r, F fn) {
-	for (int k = a.adj_ptr[r]; k < a.adj_ptr[r + 1]; ++k) {
-		const int p = a.adj_pair[k];
-		for (int i = a.pstart[p]; i < a.pstart[p + 1]; ++i) fn(i, a.li0[i] == r ? 0 : 1);
-	}
-}
-
 // ru = M (v - vold) + sum_j T_j f (static rows 0)  (PrimalDualForceSolver.cpp:151-163).
-// TRIAL: at v + 0.99 step du and the trial forces; also partial du_in . ru.
-template <bool TRIAL>
-__global__ __launch_bounds__(256) void k_body_res(SArgs a, const double* __restrict__ dudir, double step) {
-	double acc[2] = {0, 0};
-	const double* F = TRIAL ? a.ft : a.f;
+// One wavefront per body: lanes take the body's incident contacts with stride
+// 64, the six sums are reduced in a fixed butterfly order.
+__global__ __launch_bounds__(256) void k_body_res(SArgs a) {
+	const int lane = threadIdx.x & 63;
+	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const int nw = (gridDim.x * blockDim.x) >> 6;
+	double acc[1] = {0};
+	const double* F = a.f;
 	const int fd = a.nsc ? 3 : 1;
-	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.NL; r += gridDim.x * blockDim.x) {
+	for (int r = wave; r < a.NL; r += nw) {
 		double ru[6] = {0, 0, 0, 0, 0, 0};
 		if (!a.lstat[r]) {
-			double d[6];
-			for (int k = 0; k < 6; ++k) {
-				double vk = a.v[6 * r + k];
-				if (TRIAL) vk = vk + 0.99 * (step * dudir[6 * r + k]);
-				d[k] = vk - a.vold[6 * r + k];
+			double add[6] = {0, 0, 0, 0, 0, 0};
+			for (int k = a.inc_ptr[r] + lane; k < a.inc_ptr[r + 1]; k += 64) {
+				const int e = a.inc[k];
+				const int i = e >> 1, j = e & 1;
+				for (int row = 0; row < 6; ++row) {
+					double t = 0;
+					for (int col = 0; col < fd; ++col) t += Tc(a, i, j, col * 6 + row) * F[col * a.cs + i];
+					add[row] += t;
+				}
 			}
+			for (int q = 0; q < 6; ++q) add[q] = wave_sum(add[q]);
+			double d[6];
+			for (int q = 0; q < 6; ++q) d[q] = a.v[6 * r + q] - a.vold[6 * r + q];
 			const double* M = &a.Ms[7 * r];
-			for (int k = 0; k < 3; ++k) ru[k] = M[0] * d[k];
+			for (int q = 0; q < 3; ++q) ru[q] = M[0] * d[q];
 			double t[3];
 			sym3_mv(M + 1, d + 3, t);
 			ru[3] = t[0]; ru[4] = t[1]; ru[5] = t[2];
-			double add[6] = {0, 0, 0, 0, 0, 0};
-			for_incident(a, r, [&](int i, int j) {
-				for (int row = 0; row < 6; ++row) {
-					double s = 0;
-					for (int col = 0; col < fd; ++col) s += Tc(a, i, j, col * 6 + row) * F[col * a.cs + i];
-					add[row] += s;
-				}
-			});
-			for (int k = 0; k < 6; ++k) ru[k] += add[k];
+			for (int q = 0; q < 6; ++q) ru[q] += add[q];
 		}
-		if (!TRIAL)
-			for (int k = 0; k < 6; ++k) a.ru[6 * r + k] = ru[k];
-		for (int k = 0; k < 6; ++k) {
-			acc[0] += ru[k] * ru[k];
-			if (TRIAL) acc[1] += (step * dudir[6 * r + k]) * ru[k];
+		if (lane == 0) {
+			for (int q = 0; q < 6; ++q) {
+				a.ru[6 * r + q] = ru[q];
+				acc[0] += ru[q] * ru[q];
+			}
 		}
 	}
-	const int slots[2] = {TRIAL ? P_T_RU : P_RU, P_T_GU};
-	write_partials<2>(acc, a.part, slots);
+	const int slots[1] = {P_RU};
+	write_partials<1>(acc, a.part, slots);
 }
 
 // err, eps and the exit test (PrimalDualForceSolver.cpp:295-320); one workgroup.
@@ -748,80 +819,136 @@ __device__ __forceinline__ bool spd_inverse6(const double* A, double* Ai) {
 }
 
 // Diagonal blocks, rhs, Jacobi scaling D, block-Jacobi preconditioner, scaled b.
+// One wavefront per body: lanes accumulate T_j W T_j^T (upper triangle) and
+// y = -sum T_j Kinv sf over the body's incident contacts; lane 0 finishes.
 template <class T>
 __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict__ adiag, T* __restrict__ Aval,
                            T* __restrict__ Pval, T* __restrict__ bvec) {
+	const int lane = threadIdx.x & 63;
+	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const int nw = (gridDim.x * blockDim.x) >> 6;
 	const double eps = a.sc->eps;
 	const int fd = a.nsc ? 3 : 1;
-	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.NL; r += gridDim.x * blockDim.x) {
-		double B[36];
-		for (int k = 0; k < 36; ++k) B[k] = 0;
-		double y[6] = {0, 0, 0, 0, 0, 0};
+	for (int r = wave; r < a.NL; r += nw) {
 		const bool st = a.lstat[r] != 0;
-		if (st) {
-			for (int k = 0; k < 6; ++k) B[k * 6 + k] = 1.0 + eps;
-		} else {
-			const double* M = &a.Ms[7 * r];
-			for (int k = 0; k < 3; ++k) B[k * 6 + k] = M[0];
-			B[3 * 6 + 3] = M[1]; B[3 * 6 + 4] = M[2]; B[3 * 6 + 5] = M[3];
-			B[4 * 6 + 3] = M[2]; B[4 * 6 + 4] = M[4]; B[4 * 6 + 5] = M[5];
-			B[5 * 6 + 3] = M[3]; B[5 * 6 + 4] = M[5]; B[5 * 6 + 5] = M[6];
-			for (int k = 0; k < 6; ++k) B[k * 6 + k] += eps;
-			for_incident(a, r, [&](int i, int j) {
-				// y += getTrans(j) kInvSf = -T_j kInvSf
-				for (int row = 0; row < 6; ++row) {
-					double s = 0;
-					for (int col = 0; col < fd; ++col) s += Tc(a, i, j, col * 6 + row) * a.kf[col * a.cs + i];
-					y[row] -= s;
+		double U[21], y[6];
+		for (int q = 0; q < 21; ++q) U[q] = 0;
+		for (int q = 0; q < 6; ++q) y[q] = 0;
+		if (!st) {
+			for (int k = a.inc_ptr[r] + lane; k < a.inc_ptr[r + 1]; k += 64) {
+				const int e = a.inc[k];
+				const int i = e >> 1, j = e & 1;
+				double tj0[6], tj1[6], tj2[6];
+				for (int q = 0; q < 6; ++q) {
+					tj0[q] = Tc(a, i, j, q);
+					tj1[q] = Tc(a, i, j, 6 + q);
+					tj2[q] = Tc(a, i, j, 12 + q);
 				}
-				add_tWt(a, i, j, j, B);
-			});
+				const double kf0 = a.kf[i];
+				const double kf1 = fd == 3 ? a.kf[a.cs + i] : 0.0, kf2 = fd == 3 ? a.kf[2 * a.cs + i] : 0.0;
+				for (int q = 0; q < 6; ++q) y[q] -= tj0[q] * kf0 + tj1[q] * kf1 + tj2[q] * kf2;
+				const double kn = a.W[i];
+				const double w00 = a.W[a.cs + i], w01 = a.W[2 * a.cs + i], w11 = a.W[3 * a.cs + i];
+				int u = 0;
+				for (int row = 0; row < 6; ++row) {
+					const double a0 = tj0[row] * kn;
+					const double at0 = tj1[row] * w00 + tj2[row] * w01;
+					const double at1 = tj1[row] * w01 + tj2[row] * w11;
+					for (int col = row; col < 6; ++col) U[u++] += a0 * tj0[col] + at0 * tj1[col] + at1 * tj2[col];
+				}
+			}
+			for (int q = 0; q < 21; ++q) U[q] = wave_sum(U[q]);
+			for (int q = 0; q < 6; ++q) y[q] = wave_sum(y[q]);
 		}
-		double Dr[6], rhs[6];
-		for (int k = 0; k < 6; ++k) {
-			Dr[k] = 1 / sqrt(B[k * 6 + k]);
-			rhs[k] = st ? 0.0 : y[k] - a.ru[6 * r + k];
-			a.D[6 * r + k] = Dr[k];
-			a.rhs[6 * r + k] = rhs[k];
-			bvec[6 * r + k] = (T)(rhs[k] * Dr[k]);
+		if (lane == 0) {
+			double B[36];
+			for (int q = 0; q < 36; ++q) B[q] = 0;
+			if (st) {
+				for (int q = 0; q < 6; ++q) B[q * 6 + q] = 1.0 + eps;
+			} else {
+				const double* M = &a.Ms[7 * r];
+				int u = 0;
+				for (int row = 0; row < 6; ++row)
+					for (int col = row; col < 6; ++col) {
+						B[row * 6 + col] = U[u];
+						B[col * 6 + row] = U[u];
+						++u;
+					}
+				for (int q = 0; q < 3; ++q) B[q * 6 + q] += M[0];
+				B[3 * 6 + 3] += M[1]; B[3 * 6 + 4] += M[2]; B[3 * 6 + 5] += M[3];
+				B[4 * 6 + 3] += M[2]; B[4 * 6 + 4] += M[4]; B[4 * 6 + 5] += M[5];
+				B[5 * 6 + 3] += M[3]; B[5 * 6 + 4] += M[5]; B[5 * 6 + 5] += M[6];
+				for (int q = 0; q < 6; ++q) B[q * 6 + q] += eps;
+			}
+			double Dr[6];
+			for (int q = 0; q < 6; ++q) {
+				Dr[q] = 1 / sqrt(B[q * 6 + q]);
+				const double rhs = st ? 0.0 : y[q] - a.ru[6 * r + q];
+				a.D[6 * r + q] = Dr[q];
+				a.rhs[6 * r + q] = rhs;
+				bvec[6 * r + q] = (T)(rhs * Dr[q]);
+			}
+			double S[36], Si[36];
+			for (int ii = 0; ii < 6; ++ii)
+				for (int jj = 0; jj < 6; ++jj) S[ii * 6 + jj] = B[ii * 6 + jj] * (Dr[ii] * Dr[jj]);
+			const int d = adiag[r];
+			for (int q = 0; q < 36; ++q) Aval[36 * (size_t)d + q] = (T)S[q];
+			if (!spd_inverse6(S, Si)) {
+				for (int q = 0; q < 36; ++q) Si[q] = 0;
+				for (int q = 0; q < 6; ++q) Si[q * 6 + q] = 1.0 / S[q * 6 + q];
+			}
+			for (int q = 0; q < 36; ++q) Pval[36 * (size_t)r + q] = (T)Si[q];
 		}
-		double S[36], Si[36];
-		for (int i = 0; i < 6; ++i)
-			for (int j = 0; j < 6; ++j) S[i * 6 + j] = B[i * 6 + j] * (Dr[i] * Dr[j]);
-		const int d = adiag[r];
-		for (int k = 0; k < 36; ++k) Aval[36 * (size_t)d + k] = (T)S[k];
-		if (!spd_inverse6(S, Si)) {
-			for (int k = 0; k < 36; ++k) Si[k] = 0;
-			for (int k = 0; k < 6; ++k) Si[k * 6 + k] = 1.0 / S[k * 6 + k];
-		}
-		for (int k = 0; k < 36; ++k) Pval[36 * (size_t)r + k] = (T)Si[k];
 	}
 }
 
-// Off-diagonal blocks: sum over the pair's contacts of T_side(r) W T_side(c)^T, scaled.
+// Off-diagonal blocks, one wavefront per body pair: B = sum over the pair's
+// contacts of T_side(a) W T_side(b)^T; block (a, b) = D_a B D_b and block
+// (b, a) = its transpose (the Newton matrix is symmetric).
 template <class T>
-__global__ __launch_bounds__(256) void k_asm_off(SArgs a, int NL, const int* __restrict__ aptr, const int* __restrict__ acol,
-                          const int* __restrict__ apair, T* __restrict__ Aval) {
-	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < NL; r += gridDim.x * blockDim.x) {
-		for (int blk = aptr[r]; blk < aptr[r + 1]; ++blk) {
-			const int p = apair[blk];
-			if (p < 0) continue;
-			const int c = acol[blk];
-			double B[36];
-			for (int k = 0; k < 36; ++k) B[k] = 0;
-			for (int i = a.pstart[p]; i < a.pstart[p + 1]; ++i) {
-				const int j = a.li0[i] == r ? 0 : 1;
-				add_tWt(a, i, j, 1 - j, B);
-			}
-			for (int ii = 0; ii < 6; ++ii)
-				for (int jj = 0; jj < 6; ++jj)
-					Aval[36 * (size_t)blk + ii * 6 + jj] =
-						(T)(B[ii * 6 + jj] * (a.D[6 * r + ii] * a.D[6 * c + jj]));
+__global__ __launch_bounds__(256) void k_asm_off(SArgs a, int NP, T* __restrict__ Aval) {
+	const int lane = threadIdx.x & 63;
+	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const int nw = (gridDim.x * blockDim.x) >> 6;
+	for (int p = wave; p < NP; p += nw) {
+		const int ka = a.pair_adj[2 * p], kb = a.pair_adj[2 * p + 1];
+		const int blk_ab = a.adj_blk[ka], blk_ba = a.adj_blk[kb];
+		if (blk_ab < 0) continue;  // a static end: no coupling block
+		const int ra = a.g2l[a.pa[p]];
+		double B[36];
+		for (int q = 0; q < 36; ++q) B[q] = 0;
+		for (int i = a.pstart[p] + lane; i < a.pstart[p + 1]; i += 64) {
+			const int j = a.li0[i] == ra ? 0 : 1;
+			add_tWt(a, i, j, 1 - j, B);
+		}
+		for (int q = 0; q < 36; ++q) B[q] = wave_sum(B[q]);
+		const int rb = a.li0[a.pstart[p]] == ra ? a.li1[a.pstart[p]] : a.li0[a.pstart[p]];
+		if (lane < 36) {
+			const int ii = lane / 6, jj = lane - 6 * (lane / 6);
+			double bl = 0;  // B[lane] without a dynamically indexed array
+#pragma unroll
+			for (int q = 0; q < 36; ++q) bl = q == lane ? B[q] : bl;
+			const double v = bl * (a.D[6 * ra + ii] * a.D[6 * rb + jj]);
+			Aval[36 * (size_t)blk_ab + ii * 6 + jj] = (T)v;
+			Aval[36 * (size_t)blk_ba + jj * 6 + ii] = (T)v;
 		}
 	}
 }
 
 // ---- (d) PCG ----------------------------------------------------------------
+// Block-Jacobi PCG on the scaled block-CSR system, zero initial guess, as two
+// ordinary kernels per iteration (no grid-wide barrier inside a kernel):
+//   A(it): convergence test on |r|^2 (stop when |r|^2 <= tol^2 |b|^2),
+//          beta = rz_it / rz_{it-1}, p = z + beta p, q = A z + beta q  (= A p),
+//          partial p.q
+//   B(it): alpha = rz_it / p.q, x += alpha p, r' = r - alpha q, z = P r',
+//          partial r'.z and r'.r'
+// Every workgroup re-reduces the previous kernel's partial sums in the same
+// fixed order, so all of them take the same decisions.  Partials and r are
+// double-buffered by iteration parity.  The arguments and the iteration
+// counter live in device memory (the counter alternates between two slots),
+// so a chunk of iterations is a fixed sequence of launches that is recorded
+// once as a hipGraph and replayed; kernels after convergence return at once.
 template <class T>
 struct PcgArgs {
 	int nb;
@@ -831,16 +958,18 @@ struct PcgArgs {
 	const T* P;
 	const T* b;
 	T* x;
-	T* r0;
-	T* r1;
+	T* r[2];
 	T* z;
 	T* p;
 	T* q;
-	T* part;  // 3 * npart
-	int npart;
-	int maxit;
+	T* rz[2];   // partials, one per workgroup
+	T* rr[2];
+	T* pq;
+	T* bb;
 	T tol2;
+	int* done;
 	int* iters;
+	int* itc;   // two iteration-counter slots
 };
 
 template <class T>
@@ -856,93 +985,124 @@ __device__ __forceinline__ T spmv_row(const PcgArgs<T>& a, const T* __restrict__
 	return s;
 }
 
-// Block-Jacobi PCG (zero initial guess) in one cooperative launch; iterates
-// while it < maxit and |r|^2 > tol^2 |b|^2.  Uses q = A p = A z + beta q_old so
-// the SpMV reads z, which is final at the grid barrier.
+// Sums of N partial arrays (one value per workgroup) in a fixed order, all
+// loads issued before the reductions; every thread gets the results.
+template <int N, class T>
+__device__ __forceinline__ void reduce_partials(const T* const (&src)[N], int W, T (&out)[N], T* lds) {
+	if (threadIdx.x < 64) {
+		T v[N];
+#pragma unroll
+		for (int k = 0; k < N; ++k) v[k] = 0;
+		for (int w = threadIdx.x; w < W; w += 64)
+#pragma unroll
+			for (int k = 0; k < N; ++k) v[k] += src[k][w];
+#pragma unroll
+		for (int k = 0; k < N; ++k) {
+			v[k] = wave_sum(v[k]);
+			if (threadIdx.x == 0) lds[k] = v[k];
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int k = 0; k < N; ++k) out[k] = lds[k];
+}
+
 template <class T>
-__global__ __launch_bounds__(256) void k_pcg(PcgArgs<T> a) {
-	cg::grid_group grid = cg::this_grid();
+__global__ __launch_bounds__(256) void k_pcg_init(const PcgArgs<T>* __restrict__ pa) {
 	__shared__ T lds[3 * 16];
-	__shared__ T slot;
+	const PcgArgs<T> a = *pa;
 	const int n = 6 * a.nb;
-	const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-	const int gs = gridDim.x * blockDim.x;
-	const int W = gridDim.x;
-	T v2[2] = {0, 0};
-	for (int t = gt; t < n; t += gs) {
+	T v[2] = {0, 0};
+	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
 		const int br = t / 6, i = t - 6 * br;
 		const T bt = a.b[t];
-		a.r0[t] = bt;
+		a.r[0][t] = bt;
 		a.x[t] = 0;
 		a.p[t] = 0;
 		a.q[t] = 0;
 		T zt = 0;
 		for (int c = 0; c < 6; ++c) zt += a.P[36 * (size_t)br + 6 * i + c] * a.b[6 * br + c];
 		a.z[t] = zt;
-		v2[0] += bt * zt;
-		v2[1] += bt * bt;
+		v[0] += bt * zt;
+		v[1] += bt * bt;
 	}
-	{
-		T out[2];
-		block_sum<2, T>(v2, lds, out);
-		if (threadIdx.x == 0) { a.part[1 * a.npart + blockIdx.x] = out[0]; a.part[2 * a.npart + blockIdx.x] = out[1]; }
+	T out[2];
+	block_sum<2, T>(v, lds, out);
+	if (threadIdx.x == 0) {
+		a.rz[0][blockIdx.x] = out[0];
+		a.rr[0][blockIdx.x] = out[1];
+		a.bb[blockIdx.x] = out[1];
+		if (blockIdx.x == 0) { *a.done = 0; *a.iters = 0; a.itc[0] = 0; a.itc[1] = 0; }
 	}
-	grid.sync();
-	T rz = block_reduce_array(&a.part[1 * a.npart], W, &slot);
-	const T bb = block_reduce_array(&a.part[2 * a.npart], W, &slot);
-	const T eps2 = a.tol2 * bb;
-	T rr = bb;
-	T beta = 0;
-	T* rc = a.r0;
-	T* rn = a.r1;
-	int it = 0;
-	if (rr > eps2) {
-		while (it < a.maxit) {
-			T v1[1] = {0};
-			for (int t = gt; t < n; t += gs) {
-				const T pt = a.z[t] + beta * a.p[t];
-				const T qt = spmv_row(a, a.z, t) + beta * a.q[t];
-				a.p[t] = pt;
-				a.q[t] = qt;
-				v1[0] += pt * qt;
-			}
-			{
-				T out[1];
-				block_sum<1, T>(v1, lds, out);
-				if (threadIdx.x == 0) a.part[blockIdx.x] = out[0];
-			}
-			grid.sync();
-			const T pq = block_reduce_array(&a.part[0], W, &slot);
-			const T alpha = rz / pq;
-			T v3[2] = {0, 0};
-			for (int t = gt; t < n; t += gs) {
-				const int br = t / 6, i = t - 6 * br;
-				a.x[t] += alpha * a.p[t];
-				T rv[6];
-				for (int c = 0; c < 6; ++c) rv[c] = rc[6 * br + c] - alpha * a.q[6 * br + c];
-				rn[t] = rv[i];
-				T zt = 0;
-				for (int c = 0; c < 6; ++c) zt += a.P[36 * (size_t)br + 6 * i + c] * rv[c];
-				a.z[t] = zt;
-				v3[0] += rv[i] * zt;
-				v3[1] += rv[i] * rv[i];
-			}
-			{
-				T out[2];
-				block_sum<2, T>(v3, lds, out);
-				if (threadIdx.x == 0) { a.part[1 * a.npart + blockIdx.x] = out[0]; a.part[2 * a.npart + blockIdx.x] = out[1]; }
-			}
-			grid.sync();
-			const T rzn = block_reduce_array(&a.part[1 * a.npart], W, &slot);
-			rr = block_reduce_array(&a.part[2 * a.npart], W, &slot);
-			++it;
-			if (!(rr > eps2)) break;
-			beta = rzn / rz;
-			rz = rzn;
-			T* tmp = rc; rc = rn; rn = tmp;
-		}
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pcg_a(const PcgArgs<T>* __restrict__ pa, int slot) {
+	__shared__ T lds[16];
+	const PcgArgs<T> a = *pa;
+	if (*a.done) return;
+	const int it = a.itc[slot];
+	const int W = gridDim.x;
+	const T* src[4] = {a.rr[it & 1], a.bb, a.rz[it & 1], a.rz[(it + 1) & 1]};
+	T red[4];
+	reduce_partials<4, T>(src, W, red, lds);
+	const T rr = red[0], bb = red[1], rz = red[2];
+	if (!(rr > a.tol2 * bb)) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) { *a.done = 1; *a.iters = it; }
+		return;
 	}
-	if (gt == 0) *a.iters = it;
+	// rz[(it+1)&1] holds rz_{it-1} until B(it) overwrites it
+	const T beta = it > 0 ? rz / red[3] : T(0);
+	const int n = 6 * a.nb;
+	T v[1] = {0};
+	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+		const T pt = a.z[t] + beta * a.p[t];
+		const T qt = spmv_row(a, a.z, t) + beta * a.q[t];
+		a.p[t] = pt;
+		a.q[t] = qt;
+		v[0] += pt * qt;
+	}
+	__syncthreads();
+	T out[1];
+	block_sum<1, T>(v, lds, out);
+	if (threadIdx.x == 0) a.pq[blockIdx.x] = out[0];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pcg_b(const PcgArgs<T>* __restrict__ pa, int slot) {
+	__shared__ T lds[2 * 16];
+	const PcgArgs<T> a = *pa;
+	if (*a.done) return;
+	const int it = a.itc[slot];
+	const int W = gridDim.x;
+	const T* src[2] = {a.rz[it & 1], a.pq};
+	T red[2];
+	reduce_partials<2, T>(src, W, red, lds);
+	const T alpha = red[0] / red[1];
+	const T* rc = a.r[it & 1];
+	T* rn = a.r[(it + 1) & 1];
+	const int n = 6 * a.nb;
+	T v[2] = {0, 0};
+	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+		const int br = t / 6, i = t - 6 * br;
+		a.x[t] += alpha * a.p[t];
+		T rv[6];
+		for (int c = 0; c < 6; ++c) rv[c] = rc[6 * br + c] - alpha * a.q[6 * br + c];
+		rn[t] = rv[i];
+		T zt = 0;
+		for (int c = 0; c < 6; ++c) zt += a.P[36 * (size_t)br + 6 * i + c] * rv[c];
+		a.z[t] = zt;
+		v[0] += rv[i] * zt;
+		v[1] += rv[i] * rv[i];
+	}
+	__syncthreads();
+	T out[2];
+	block_sum<2, T>(v, lds, out);
+	if (threadIdx.x == 0) {
+		a.rz[(it + 1) & 1][blockIdx.x] = out[0];
+		a.rr[(it + 1) & 1][blockIdx.x] = out[1];
+		if (blockIdx.x == 0) a.itc[slot ^ 1] = it + 1;
+	}
 }
 
 template <class T>
@@ -981,78 +1141,213 @@ __global__ __launch_bounds__(256) void k_neg(int n, const double* __restrict__ x
 	if (t < n) y[t] = -x[t];
 }
 
-// ---- (e) line search trial (PrimalDualForceSolver.cpp:588-613) --------------
-// Filter (NonSmoothContactForce.cpp:281-319), trial point, contact residuals
-// with fresh constraints; partial |wrf|^2, |rl|^2, |ac|^2, df.rf, (dl/l).rl.
-__global__ __launch_bounds__(256) void k_ls_contact(SArgs a, const double* __restrict__ du, const double* __restrict__ dF,
-                             const double* __restrict__ dL, double step) {
-	const double muc = a.sc->muc;
-	double acc[5] = {0, 0, 0, 0, 0};
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
-		const double cf[3] = {a.f[i], a.nsc ? a.f[a.cs + i] : 0.0, a.nsc ? a.f[2 * a.cs + i] : 0.0};
-		const double lam[2] = {a.lam[i], a.nsc ? a.lam[a.cs + i] : 0.0};
-		double dfi[3] = {step * dF[i], a.nsc ? step * dF[a.cs + i] : 0.0, a.nsc ? step * dF[2 * a.cs + i] : 0.0};
-		double dli[2] = {step * dL[i], a.nsc ? step * dL[a.cs + i] : 0.0};
-		if (a.nsc) {
-			double tf[3] = {cf[0] + dfi[0], cf[1] + dfi[1], cf[2] + dfi[2]};
-			if (tf[0] < 0) {
-				tf[0] = 0;
-				for (int k = 0; k < 3; ++k) dfi[k] = tf[k] - cf[k];
-			}
-			if (Lorentz::constraint(a.mu, tf[0], tf[1], tf[2]) > 0) {
-				double pf[3];
-				Lorentz::project(a.mu, tf, pf);
-				for (int k = 0; k < 3; ++k) dfi[k] = pf[k] - cf[k];
-			}
-		} else {
-			const double tf = cf[0] + dfi[0];
-			if (tf < 0) dfi[0] = 0 - cf[0];
+// ---- (e) line search as fused reductions (PrimalDualForceSolver.cpp:562-635) --
+// The trials of one search share the direction, so they are independent:
+// one pass evaluates kTrials step sizes 2^-k0 ... 2^-(k0+kTrials-1) at once
+// and k_ls_final applies the reference's sequential acceptance rule to them,
+// which selects exactly the step the one-trial-at-a-time loop would.
+
+// Filtered trial point of contact i at `step` (NonSmoothContactForce.cpp:281-319,
+// OnlyNormalForce.cpp:172-201): f' = f + 0.99 df_in, lambda' = lambda + 0.99 dl_in.
+__device__ __forceinline__ void trial_point(const SArgs& a, int i, const double* __restrict__ dF,
+                                            const double* __restrict__ dL, double step, double* fI,
+                                            double* lI, double* dfi, double* dli) {
+	const double cf[3] = {a.f[i], a.nsc ? a.f[a.cs + i] : 0.0, a.nsc ? a.f[2 * a.cs + i] : 0.0};
+	const double lam[2] = {a.lam[i], a.nsc ? a.lam[a.cs + i] : 0.0};
+	dfi[0] = step * dF[i];
+	dfi[1] = a.nsc ? step * dF[a.cs + i] : 0.0;
+	dfi[2] = a.nsc ? step * dF[2 * a.cs + i] : 0.0;
+	dli[0] = step * dL[i];
+	dli[1] = a.nsc ? step * dL[a.cs + i] : 0.0;
+	if (a.nsc) {
+		double tf[3] = {cf[0] + dfi[0], cf[1] + dfi[1], cf[2] + dfi[2]};
+		if (tf[0] < 0) {
+			tf[0] = 0;
+			for (int k = 0; k < 3; ++k) dfi[k] = tf[k] - cf[k];
 		}
-		const int nl = a.nsc ? 2 : 1;
-		for (int k = 0; k < nl; ++k)
-			if (dli[k] < 0 && lam[k] + dli[k] < 0) dli[k] = -lam[k];
-		double fI[3], lI[2];
-		for (int k = 0; k < 3; ++k) fI[k] = cf[k] + 0.99 * dfi[k];
-		for (int k = 0; k < 2; ++k) lI[k] = lam[k] + 0.99 * dli[k];
-		const int l0 = a.li0[i], l1 = a.li1[i];
-		double va[6], vb[6];
-		for (int k = 0; k < 6; ++k) {
-			va[k] = a.v[6 * l0 + k] + 0.99 * (step * du[6 * l0 + k]);
-			vb[k] = a.v[6 * l1 + k] + 0.99 * (step * du[6 * l1 + k]);
+		if (Lorentz::constraint(a.mu, tf[0], tf[1], tf[2]) > 0) {
+			double pf[3];
+			Lorentz::project(a.mu, tf, pf);
+			for (int k = 0; k < 3; ++k) dfi[k] = pf[k] - cf[k];
 		}
-		double rf[3], rl[2], wrf[3], ac[3];
-		contact_terms(a, i, va, vb, fI, lI, muc, rf, rl, wrf, ac);
-		a.ft[i] = fI[0];
-		a.lt[i] = lI[0];
-		if (a.nsc) {
-			a.ft[a.cs + i] = fI[1];
-			a.ft[2 * a.cs + i] = fI[2];
-			a.lt[a.cs + i] = lI[1];
-		}
-		acc[0] += wrf[0] * wrf[0] + wrf[1] * wrf[1] + wrf[2] * wrf[2];
-		acc[1] += rl[0] * rl[0] + (a.nsc ? rl[1] * rl[1] : 0.0);
-		acc[2] += ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
-		acc[3] += dfi[0] * rf[0] + (a.nsc ? dfi[1] * rf[1] + dfi[2] * rf[2] : 0.0);
-		acc[4] += dli[0] / lI[0] * rl[0] + (a.nsc ? dli[1] / lI[1] * rl[1] : 0.0);
+	} else {
+		const double tf = cf[0] + dfi[0];
+		if (tf < 0) dfi[0] = 0 - cf[0];
 	}
-	const int slots[5] = {P_T_WRF, P_T_RL, P_T_AC, P_T_GF, P_T_GL};
-	write_partials<5>(acc, a.part, slots);
+#pragma unroll
+	for (int k = 0; k < 2; ++k)
+		if ((k == 0 || a.nsc) && dli[k] < 0 && lam[k] + dli[k] < 0) dli[k] = -lam[k];
+#pragma unroll
+	for (int k = 0; k < 3; ++k) fI[k] = cf[k] + 0.99 * dfi[k];
+#pragma unroll
+	for (int k = 0; k < 2; ++k) lI[k] = lam[k] + 0.99 * dli[k];
 }
 
-__global__ __launch_bounds__(256) void k_ls_final(SArgs a, int nwg_c, int nwg_b) {
-	__shared__ double slot;
-	const double wrf2 = block_reduce_array(&a.part[P_T_WRF * kMaxWG], nwg_c, &slot);
-	const double rl2 = block_reduce_array(&a.part[P_T_RL * kMaxWG], nwg_c, &slot);
-	const double ac2 = block_reduce_array(&a.part[P_T_AC * kMaxWG], nwg_c, &slot);
-	const double gf = block_reduce_array(&a.part[P_T_GF * kMaxWG], nwg_c, &slot);
-	const double gl = block_reduce_array(&a.part[P_T_GL * kMaxWG], nwg_c, &slot);
-	const double ru2 = block_reduce_array(&a.part[P_T_RU * kMaxWG], nwg_b, &slot);
-	const double gu = block_reduce_array(&a.part[P_T_GU * kMaxWG], nwg_b, &slot);
+// Contact part of the trials: partial |wrf'|^2, |rl'|^2, |ac'|^2, df_in.rf'
+// and sum (dl_in / lambda') rl' per trial (fresh constraint values).
+__global__ __launch_bounds__(256) void k_ls_contact(SArgs a, const double* __restrict__ du,
+                                                    const double* __restrict__ dF,
+                                                    const double* __restrict__ dL, int k0) {
+	const double muc = a.sc->muc;
+	for (int t = 0; t < kTrials; ++t) {
+		const double step = ldexp(1.0, -(k0 + t));
+		double acc[5] = {0, 0, 0, 0, 0};
+		for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
+			double fI[3], lI[2], dfi[3], dli[2];
+			trial_point(a, i, dF, dL, step, fI, lI, dfi, dli);
+			const int l0 = a.li0[i], l1 = a.li1[i];
+			double va[6], vb[6];
+			for (int k = 0; k < 6; ++k) {
+				va[k] = a.v[6 * l0 + k] + 0.99 * (step * du[6 * l0 + k]);
+				vb[k] = a.v[6 * l1 + k] + 0.99 * (step * du[6 * l1 + k]);
+			}
+			double rf[3], rl[2], wrf[3], ac[3];
+			contact_terms(a, i, va, vb, fI, lI, muc, rf, rl, wrf, ac);
+			acc[0] += wrf[0] * wrf[0] + wrf[1] * wrf[1] + wrf[2] * wrf[2];
+			acc[1] += rl[0] * rl[0] + (a.nsc ? rl[1] * rl[1] : 0.0);
+			acc[2] += ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
+			acc[3] += dfi[0] * rf[0] + (a.nsc ? dfi[1] * rf[1] + dfi[2] * rf[2] : 0.0);
+			acc[4] += dli[0] / lI[0] * rl[0] + (a.nsc ? dli[1] / lI[1] * rl[1] : 0.0);
+		}
+		const int b = P_LS + 7 * t;
+		const int slots[5] = {b, b + 1, b + 2, b + 3, b + 4};
+		write_partials<5>(acc, a.part, slots);
+	}
+}
+
+// Body part of the trials, one wavefront per body: ru' = M (v' - vold) +
+// sum_j T_j f' at the trial steps of group G (kGroup trials per launch, to
+// bound register use); partial |ru'|^2 and du_in . ru'.
+constexpr int kGroup = 5;
+template <int G>
+__global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restrict__ du,
+                                                 const double* __restrict__ dF,
+                                                 const double* __restrict__ dL, int k0) {
+	const int lane = threadIdx.x & 63;
+	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const int nw = (gridDim.x * blockDim.x) >> 6;
+	double acc[2 * kGroup];
+#pragma unroll
+	for (int q = 0; q < 2 * kGroup; ++q) acc[q] = 0;
+	for (int r = wave; r < a.NL; r += nw) {
+		if (a.lstat[r]) continue;
+		double add[kGroup][6];
+#pragma unroll
+		for (int t = 0; t < kGroup; ++t)
+#pragma unroll
+			for (int q = 0; q < 6; ++q) add[t][q] = 0;
+		for (int k = a.inc_ptr[r] + lane; k < a.inc_ptr[r + 1]; k += 64) {
+			const int e = a.inc[k];
+			const int i = e >> 1, j = e & 1;
+			double Tj[18];
+#pragma unroll
+			for (int q = 0; q < 18; ++q) Tj[q] = Tc(a, i, j, q);
+#pragma unroll
+			for (int t = 0; t < kGroup; ++t) {
+				double fI[3], lI[2], dfi[3], dli[2];
+				trial_point(a, i, dF, dL, ldexp(1.0, -(k0 + G * kGroup + t)), fI, lI, dfi, dli);
+				// fI[1] = fI[2] = 0 for the frictionless model
+#pragma unroll
+				for (int row = 0; row < 6; ++row)
+					add[t][row] += Tj[row] * fI[0] + Tj[6 + row] * fI[1] + Tj[12 + row] * fI[2];
+			}
+		}
+#pragma unroll
+		for (int t = 0; t < kGroup; ++t)
+#pragma unroll
+			for (int q = 0; q < 6; ++q) add[t][q] = wave_sum(add[t][q]);
+		if (lane == 0) {
+			const double* M = &a.Ms[7 * r];
+#pragma unroll
+			for (int t = 0; t < kGroup; ++t) {
+				const double step = ldexp(1.0, -(k0 + G * kGroup + t));
+				double d[6], dd[6];
+				for (int q = 0; q < 6; ++q) {
+					dd[q] = step * du[6 * r + q];
+					d[q] = (a.v[6 * r + q] + 0.99 * dd[q]) - a.vold[6 * r + q];
+				}
+				double ru[6];
+				for (int q = 0; q < 3; ++q) ru[q] = M[0] * d[q];
+				double tt[3];
+				sym3_mv(M + 1, d + 3, tt);
+				ru[3] = tt[0]; ru[4] = tt[1]; ru[5] = tt[2];
+				for (int q = 0; q < 6; ++q) {
+					ru[q] += add[t][q];
+					acc[2 * t] += ru[q] * ru[q];
+					acc[2 * t + 1] += dd[q] * ru[q];
+				}
+			}
+		}
+	}
+	int slots[2 * kGroup];
+#pragma unroll
+	for (int t = 0; t < kGroup; ++t) {
+		slots[2 * t] = P_LS + 7 * (G * kGroup + t) + 5;
+		slots[2 * t + 1] = P_LS + 7 * (G * kGroup + t) + 6;
+	}
+	write_partials<2 * kGroup>(acc, a.part, slots);
+}
+static_assert(kTrials == 2 * kGroup, "k_ls_body covers the trials in two groups");
+
+// Acceptance (PrimalDualForceSolver.cpp:605-631) over the trials of the pass,
+// in order: the first trial with (accept and gErr > 0), or the search's last
+// trial, decides.  ls_step = accepted step, or 0 when the decision is a
+// failure; ls_decided = 0 when no trial of this pass decided.
+__global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int nwg_c, int nwg_b, int k0, int maxLS,
+                                                   double sErr, double sacErr) {
+	__shared__ double red[7 * kTrials];
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	for (int q = wid; q < 7 * kTrials; q += 16) {
+		const int slot = P_LS + q;
+		const int t = q / 7, c = q - 7 * (q / 7);
+		(void)t;
+		const int n = c < 5 ? nwg_c : nwg_b;
+		double v = 0;
+		for (int w = lane; w < n; w += 64) v += a.part[slot * kMaxWG + w];
+		v = wave_sum(v);
+		if (lane == 0) red[q] = v;
+	}
+	__syncthreads();
 	if (threadIdx.x == 0) {
 		Scalars* S = a.sc;
-		S->t_inErr = ru2 + wrf2 + rl2;
-		S->t_sacIn = ru2 + ac2 + rl2;
-		S->t_gErr = -gu + gf + gl;
+		S->ls_decided = 0;
+		S->ls_step = 0;
+		S->ls_trials = 0;
+		for (int t = 0; t < kTrials && k0 + t < maxLS; ++t) {
+			const double* R = &red[7 * t];
+			const double inErr = R[5] + R[0] + R[1];
+			const double sacIn = R[5] + R[2] + R[1];
+			const double gErr = -R[6] + R[3] + R[4];
+			const bool acpt = (inErr < sErr) | (sacIn < sacErr);
+			S->ls_trials = t + 1;
+			if ((acpt && gErr > 0) || k0 + t == maxLS - 1) {
+				S->ls_decided = 1;
+				if (acpt) {
+					S->ls_step = ldexp(1.0, -(k0 + t));
+					S->t_inErr = inErr;
+					S->t_sacIn = sacIn;
+					S->t_gErr = gErr;
+				}
+				break;
+			}
+		}
+	}
+}
+
+// Commit the accepted trial: f, lambda in place (the same trial point the
+// pass evaluated), then v (k_commit_v).
+__global__ __launch_bounds__(256) void k_ls_commit(SArgs a, const double* __restrict__ dF,
+                                                   const double* __restrict__ dL, double step) {
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
+		double fI[3], lI[2], dfi[3], dli[2];
+		trial_point(a, i, dF, dL, step, fI, lI, dfi, dli);
+		a.f[i] = fI[0];
+		a.lam[i] = lI[0];
+		if (a.nsc) {
+			a.f[a.cs + i] = fI[1];
+			a.f[2 * a.cs + i] = fI[2];
+			a.lam[a.cs + i] = lI[1];
+		}
 	}
 }
 
@@ -1117,6 +1412,11 @@ __global__ __launch_bounds__(256) void k_integrate(int N, const int* __restrict_
 	for (int k = 0; k < 3; ++k) { vel[3 * b + k] = v[k]; ang[3 * b + k] = w[k]; }
 }
 
+// Iteration count when the solve ran to the iteration cap.
+__global__ void k_pcg_final(const int* done, int* iters, int maxit) {
+	if (threadIdx.x == 0 && !*done) *iters = maxit;
+}
+
 // ===========================================================================
 // host orchestration
 // ===========================================================================
@@ -1134,33 +1434,49 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	a.dt = dt;
 	for (int k = 0; k < 3; ++k) a.gdt[k] = g[k] * dt;
 	a.li0 = ctx->li0.p; a.li1 = ctx->li1.p; a.T = ctx->c_T.p; a.depth = ctx->c_depth.p;
-	a.adj_ptr = ctx->adj_ptr.p; a.adj_pair = ctx->adj_pair.p; a.pstart = ctx->ap_start.p;
+	a.pstart = ctx->ap_start.p; a.pa = ctx->ap_a.p; a.g2l = ctx->g2l.p;
 	a.lstat = ctx->l_static.p; a.l2g = ctx->l2g.p;
+	a.inc_ptr = ctx->inc_ptr.p; a.inc = ctx->inc.p; a.pair_adj = ctx->pair_adj.p; a.adj_blk = ctx->adj_blk.p;
 	a.v = ctx->l_v.p; a.vold = ctx->l_vold.p; a.vphys = ctx->l_vphys.p; a.Ms = ctx->l_Ms.p;
 	a.Minv = ctx->l_Minv.p; a.speed = ctx->l_speed.p; a.ru = ctx->l_ru.p; a.rhs = ctx->l_rhs.p;
 	a.D = ctx->l_D.p; a.du = ctx->l_du.p;
 	a.K = ctx->s_K.p; a.f = ctx->s_f.p; a.lam = ctx->s_lam.p; a.c = ctx->s_c.p; a.g = ctx->s_g.p;
 	a.H = ctx->s_H.p; a.rf = ctx->s_rf.p; a.rl = ctx->s_rl.p; a.sf = ctx->s_sf.p;
 	a.invC = ctx->s_invC.p; a.W = ctx->s_W.p; a.kf = ctx->s_kf.p; a.df = ctx->s_df.p;
-	a.dl = ctx->s_dl.p; a.ft = ctx->s_ft.p; a.lt = ctx->s_lt.p;
+	a.dl = ctx->s_dl.p;
 	a.part = ctx->partials.p; a.sc = ctx->d_sc.p;
 	return a;
+}
+
+// One recorded chunk of kPcgChunk PCG iterations (kernel pairs alternating
+// the counter slots 0, 1).  With event timing on, an event pair brackets each
+// SpMV kernel (or each update kernel) inside the graph.
+constexpr int kPcgChunk = 16;
+
+template <class T>
+static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>* dargs, int nblk, int kt_which,
+                               hipEvent_t* ev) {
+	hipStream_t s = ctx->stream;
+	for (int k = 0; k < kPcgChunk; ++k) {
+		const int slot = k & 1;
+		if (ev && kt_which == PDC_KT_SPMV) PDC_HIP(hipEventRecord(ev[2 * k], s));
+		hipLaunchKernelGGL(k_pcg_a<T>, dim3(nblk), dim3(256), 0, s, dargs, slot);
+		if (ev && kt_which == PDC_KT_SPMV) PDC_HIP(hipEventRecord(ev[2 * k + 1], s));
+		if (ev && kt_which == PDC_KT_PCG) PDC_HIP(hipEventRecord(ev[2 * k], s));
+		hipLaunchKernelGGL(k_pcg_b<T>, dim3(nblk), dim3(256), 0, s, dargs, slot);
+		if (ev && kt_which == PDC_KT_PCG) PDC_HIP(hipEventRecord(ev[2 * k + 1], s));
+	}
+	PDC_CHECK_LAUNCH();
 }
 
 template <class T>
 static int run_pcg(pdc_context* ctx, DBuf<T>& A, DBuf<T>& P, DBuf<T>& b, DBuf<T>& x, DBuf<T>& work) {
 	const int NL = ctx->NL;
 	const int n = 6 * NL;
-	static int max_blocks_cache[2] = {0, 0};
-	int& maxb = max_blocks_cache[sizeof(T) == 8];
-	if (maxb == 0) {
-		int per_cu = 0;
-		PDC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<T>, 256, 0));
-		maxb = std::max(1, per_cu) * ctx->cus;
-	}
-	const int nblk = std::max(1, std::min({grid_for(n, 256), maxb, 1024}));
-	work.ensure(5 * (size_t)n + 8);
-	ctx->pcg_part.ensure(4096);
+	// grid fixed per scene (rows of all bodies) so the recorded chunk stays valid
+	const int nblk = std::max(1, std::min(grid_for(6 * (int64_t)std::max(ctx->N, 1), 256), 2048));
+	const size_t ncap = 6 * (size_t)std::max(ctx->N, 1);
+	work.ensure(5 * ncap + 6 * 2048 + 64);
 	PcgArgs<T> pa;
 	pa.nb = NL;
 	pa.ptr = ctx->a_ptr.p;
@@ -1169,39 +1485,115 @@ static int run_pcg(pdc_context* ctx, DBuf<T>& A, DBuf<T>& P, DBuf<T>& b, DBuf<T>
 	pa.P = P.p;
 	pa.b = b.p;
 	pa.x = x.p;
-	pa.r0 = work.p;
-	pa.r1 = work.p + n;
-	pa.z = work.p + 2 * n;
-	pa.p = work.p + 3 * n;
-	pa.q = work.p + 4 * n;
-	pa.part = reinterpret_cast<T*>(ctx->pcg_part.p);
-	pa.npart = 1024;
-	pa.maxit = ctx->p.pcg_max_iter;
+	T* w = work.p;
+	pa.r[0] = w; pa.r[1] = w + ncap; pa.z = w + 2 * ncap; pa.p = w + 3 * ncap; pa.q = w + 4 * ncap;
+	T* parts = w + 5 * ncap;
+	pa.rz[0] = parts; pa.rz[1] = parts + 2048; pa.rr[0] = parts + 2 * 2048; pa.rr[1] = parts + 3 * 2048;
+	pa.pq = parts + 4 * 2048; pa.bb = parts + 5 * 2048;
 	pa.tol2 = (T)(ctx->p.pcg_tol * ctx->p.pcg_tol);
+	pa.done = &ctx->d_sc.p->pcg_done;
 	pa.iters = &ctx->d_sc.p->pcg_iters;
-	void* args[] = {&pa};
-	hipEvent_t e0;
-	kt_begin(ctx, PDC_KT_PCG, &e0);
-	PDC_HIP(hipLaunchCooperativeKernel((const void*)k_pcg<T>, dim3(nblk), dim3(256), args, 0,
-	                                   ctx->stream));
-	kt_end(ctx, PDC_KT_PCG, e0);
-	hipLaunchKernelGGL(k_du<T>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, NL, ctx->l_D.p,
-	                   x.p, ctx->l_du.p);
+	pa.itc = ctx->d_sc.p->pcg_itc;
+	ctx->pcg_args.ensure(sizeof(PcgArgs<double>));
+	const PcgArgs<T>* dargs = reinterpret_cast<const PcgArgs<T>*>(ctx->pcg_args.p);
+	hipStream_t s = ctx->stream;
+	*reinterpret_cast<PcgArgs<T>*>(ctx->h_pcg_args) = pa;
+	PDC_HIP(hipMemcpyAsync(ctx->pcg_args.p, ctx->h_pcg_args, sizeof(PcgArgs<T>), hipMemcpyHostToDevice, s));
+	hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(256), 0, s, dargs);
+	PDC_CHECK_LAUNCH();
+
+	const int kt_which = ctx->kt.which;
+	const bool kt_on = kt_which == PDC_KT_SPMV || kt_which == PDC_KT_PCG;
+	// graphs carry the event nodes only where the runtime records events in a capture
+	const bool graph = ctx->use_graphs && !(kt_on && !ctx->graph_events_ok);
+	PcgGraph& G = ctx->pcg_graph[sizeof(T) == 8 ? 1 : 0];
+	const int want_kt = kt_on ? kt_which : 0;
+	if (graph && (G.exec == nullptr || G.nblk != nblk || G.args != (const void*)dargs || G.kt_which != want_kt)) {
+		if (G.exec) { (void)hipGraphExecDestroy(G.exec); G.exec = nullptr; }
+		if (G.graph) { (void)hipGraphDestroy(G.graph); G.graph = nullptr; }
+		if (kt_on && G.ev.empty()) {
+			G.ev.resize(2 * kPcgChunk);
+			for (auto& e : G.ev) PDC_HIP(hipEventCreate(&e));
+		}
+		PDC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+		bool ok = true;
+		try {
+			pcg_chunk_launches<T>(ctx, dargs, nblk, kt_which, kt_on ? G.ev.data() : nullptr);
+		} catch (const HipError&) {
+			ok = false;
+		}
+		hipGraph_t gr = nullptr;
+		const hipError_t e1 = hipStreamEndCapture(s, &gr);
+		if (!ok || e1 != hipSuccess || gr == nullptr) {
+			if (gr) (void)hipGraphDestroy(gr);
+			(void)hipGetLastError();
+			if (!kt_on) throw HipError("PCG chunk capture failed", PDC_ERR_DEVICE);
+			ctx->graph_events_ok = false;  // time with direct launches from now on
+		} else {
+			G.graph = gr;
+			PDC_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+			G.nblk = nblk;
+			G.args = dargs;
+			G.kt_which = want_kt;
+		}
+	}
+	const bool use_graph = graph && G.exec != nullptr && G.kt_which == want_kt;
+	// chunks: as many as the previous solve needed, then one at a time
+	const int maxit = ctx->p.pcg_max_iter;
+	int launched = 0;
+	int want = std::max(1, (std::min(ctx->pcg_last_iters + 2, maxit) + kPcgChunk - 1) / kPcgChunk);
+	int flags[2] = {0, 0};
+	while (launched < maxit) {
+		for (int c = 0; c < want && launched < maxit; ++c) {
+			if (use_graph) {
+				PDC_HIP(hipGraphLaunch(G.exec, s));
+			} else if (kt_on) {
+				pcg_chunk_launches<T>(ctx, dargs, nblk, kt_which, ctx->kt_direct_ev(2 * kPcgChunk));
+			} else {
+				pcg_chunk_launches<T>(ctx, dargs, nblk, 0, nullptr);
+			}
+			launched += kPcgChunk;
+			if (kt_on) {
+				// the chunk's events are reused by the next chunk: read them now
+				PDC_HIP(hipMemcpyAsync(flags, &ctx->d_sc.p->pcg_iters, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+				PDC_HIP(hipStreamSynchronize(s));
+				const int done_at = flags[1] ? flags[0] : launched;
+				const int active = std::max(0, std::min(kPcgChunk, done_at - (launched - kPcgChunk)));
+				hipEvent_t* ev = use_graph ? G.ev.data() : ctx->kt_direct_ev(2 * kPcgChunk);
+				for (int k = 0; k < active; ++k) {
+					float e = 0;
+					PDC_HIP(hipEventElapsedTime(&e, ev[2 * k], ev[2 * k + 1]));
+					ctx->kt.total_ms += e;
+					ctx->kt.launches += 1;
+				}
+				if (flags[1]) break;
+			}
+		}
+		PDC_HIP(hipMemcpyAsync(flags, &ctx->d_sc.p->pcg_iters, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+		PDC_HIP(hipStreamSynchronize(s));
+		if (flags[1]) break;  // pcg_done
+		want = 1;
+	}
+	const int iters = flags[1] ? flags[0] : maxit;
+	if (kt_on) {
+		const double Tb = sizeof(T), n6 = 6.0 * NL, nnzb = ctx->NNZB;
+		if (kt_which == PDC_KT_SPMV)
+			ctx->kt.bytes += iters * (nnzb * (36.0 * Tb + 4.0) + 4.0 * (NL + 1) + 5.0 * Tb * n6);
+		else
+			ctx->kt.bytes += iters * (7.0 * Tb * n6 + 36.0 * Tb * NL);
+	}
+	hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(64), 0, s, &ctx->d_sc.p->pcg_done, &ctx->d_sc.p->pcg_iters, maxit);
+	hipLaunchKernelGGL(k_du<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, NL, ctx->l_D.p, x.p, ctx->l_du.p);
 	PDC_CHECK_LAUNCH();
 	return nblk;
 }
 
-// Algorithmic bytes (each input read once, each output written once).
-// PCG launch with `it` iterations, scalar size T, n6 = 6 NL:
-//   set-up   b read, r x p q z written (6 T n6) + block-Jacobi blocks (36 T NL)
-//   per it   A blocks and column indices nnzb (36 T + 4), row pointers 4 (NL+1),
-//            block-Jacobi blocks 36 T NL, twelve n6-vector reads/writes (12 T n6)
-double pcg_bytes(int NL, int nnzb, int it, int T) {
-	const double n6 = 6.0 * NL;
-	const double init = 6.0 * T * n6 + 36.0 * T * NL;
-	const double per = (double)nnzb * (36.0 * T + 4.0) + 4.0 * (NL + 1) + 36.0 * T * NL + 12.0 * T * n6;
-	return init + (double)it * per;
-}
+// Algorithmic bytes (each input read once, each output written once), per
+// launch, scalar size T, n6 = 6 NL:
+//   k_pcg_a  A blocks and column indices nnzb (36 T + 4), row pointers
+//            4 (NL+1), z read T n6, p and q read and written 4 T n6
+//   k_pcg_b  x read/written 2 T n6; p, r, q read 3 T n6; r', z written 2 T n6;
+//            block-Jacobi blocks 36 T NL
 // Constraint + contact-residual kernels of one Newton iteration, per contact:
 //   f 24, lambda 16 in; c 16, g 16, H 24 out (constraints);
 //   T 288, local ids 8, f 24, lambda 16, K 8 in; rf 24, rl 16 out (residuals);
@@ -1249,17 +1641,17 @@ void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st) {
 	}
 	SArgs a = make_args(ctx, dt, g);
 	const int wc = wg_for(C), wb = wg_for(NL);
+	const int wv = std::max(1, std::min((NL + 3) / 4, kMaxWG));      // one wavefront per body
+	const int wp = std::max(1, std::min((ctx->NP + 3) / 4, 4096));   // one wavefront per pair
 	const dim3 B(kBlock);
 	const int nsc = a.nsc;
 	const int fd = nsc ? 3 : 1, cd = nsc ? 2 : 1;
 	const double nR = 6.0 * NL, nF = (double)fd * C, nLc = (double)cd * C;
 
-	hipEvent_t t0;
-	kt_begin(ctx, PDC_KT_NONE, &t0);
 	hipLaunchKernelGGL(k_setup_bodies, dim3(wb), B, 0, s, a, ctx->b_vel.p, ctx->b_ang.p, ctx->b_quat.p,
 	                   ctx->b_mass.p, ctx->b_I0.p);
 	PDC_CHECK_LAUNCH();
-	hipLaunchKernelGGL(k_nondim_contacts, dim3(wc), B, 0, s, a);
+	hipLaunchKernelGGL(k_nondim_speed, dim3(wv), B, 0, s, a);
 	PDC_CHECK_LAUNCH();
 	const double gn = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
 	hipLaunchKernelGGL(k_nondim_final, dim3(1), dim3(1024), 0, s, a, wb, ctx->b_mass.p, gn * dt);
@@ -1282,9 +1674,9 @@ void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st) {
 		PDC_CHECK_LAUNCH();
 		kt_end(ctx, PDC_KT_CONTACT_TERMS, e0);
 		if (e0) ctx->kt.bytes += contact_terms_bytes(C, NL);
-		hipLaunchKernelGGL(k_body_res<false>, dim3(wb), B, 0, s, a, (const double*)nullptr, 0.0);
+		hipLaunchKernelGGL(k_body_res, dim3(wv), B, 0, s, a);
 		PDC_CHECK_LAUNCH();
-		hipLaunchKernelGGL(k_err_final, dim3(1), dim3(256), 0, s, a, wc, wb, Hu, nR, nF, nLc, (double)C);
+		hipLaunchKernelGGL(k_err_final, dim3(1), dim3(256), 0, s, a, wc, wv, Hu, nR, nF, nLc, (double)C);
 		PDC_CHECK_LAUNCH();
 		sync_scalars(ctx);
 		exitErr = H->exitErr;
@@ -1295,17 +1687,15 @@ void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st) {
 		hipLaunchKernelGGL(k_asm_prep, dim3(wc), B, 0, s, a);
 		PDC_CHECK_LAUNCH();
 		if (fp32) {
-			hipLaunchKernelGGL(k_asm_diag<float>, dim3(wb), B, 0, s, a, ctx->a_diag.p, ctx->A32.p,
+			hipLaunchKernelGGL(k_asm_diag<float>, dim3(wv), B, 0, s, a, ctx->a_diag.p, ctx->A32.p,
 			                   ctx->P32.p, ctx->b32.p);
 			PDC_CHECK_LAUNCH();
-			hipLaunchKernelGGL(k_asm_off<float>, dim3(wb), B, 0, s, a, NL, ctx->a_ptr.p, ctx->a_col.p,
-			                   ctx->a_pair.p, ctx->A32.p);
+			hipLaunchKernelGGL(k_asm_off<float>, dim3(wp), B, 0, s, a, ctx->NP, ctx->A32.p);
 		} else {
-			hipLaunchKernelGGL(k_asm_diag<double>, dim3(wb), B, 0, s, a, ctx->a_diag.p, ctx->A64.p,
+			hipLaunchKernelGGL(k_asm_diag<double>, dim3(wv), B, 0, s, a, ctx->a_diag.p, ctx->A64.p,
 			                   ctx->P64.p, ctx->b64.p);
 			PDC_CHECK_LAUNCH();
-			hipLaunchKernelGGL(k_asm_off<double>, dim3(wb), B, 0, s, a, NL, ctx->a_ptr.p, ctx->a_col.p,
-			                   ctx->a_pair.p, ctx->A64.p);
+			hipLaunchKernelGGL(k_asm_off<double>, dim3(wp), B, 0, s, a, ctx->NP, ctx->A64.p);
 		}
 		PDC_CHECK_LAUNCH();
 		kt_end(ctx, PDC_KT_ASSEMBLY, e0);
@@ -1316,38 +1706,33 @@ void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st) {
 		PDC_CHECK_LAUNCH();
 		sync_scalars(ctx);
 		const int iters = H->pcg_iters;
+		ctx->pcg_last_iters = iters;
 		pcg_total += iters;
-		if (ctx->kt.which == PDC_KT_PCG) ctx->kt.bytes += pcg_bytes(NL, ctx->NNZB, iters, fp32 ? 4 : 8);
 		if (iters == ctx->p.pcg_max_iter) pcg_hits++;
-		// line search
+		// line search: fused passes of kTrials steps each
 		auto line_search = [&](const double* du, const double* dF, const double* dL, int maxLS) {
-			double step = 1.0;
-			for (int k = 0; k < maxLS; ++k) {
+			for (int k0 = 0; k0 < maxLS; k0 += kTrials) {
 				hipEvent_t e1;
 				kt_begin(ctx, PDC_KT_LINE_SEARCH, &e1);
-				hipLaunchKernelGGL(k_ls_contact, dim3(wc), B, 0, s, a, du, dF, dL, step);
+				hipLaunchKernelGGL(k_ls_contact, dim3(wc), B, 0, s, a, du, dF, dL, k0);
 				PDC_CHECK_LAUNCH();
 				kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
-				hipLaunchKernelGGL(k_body_res<true>, dim3(wb), B, 0, s, a, du, step);
+				hipLaunchKernelGGL(k_ls_body<0>, dim3(wv), B, 0, s, a, du, dF, dL, k0);
+				hipLaunchKernelGGL(k_ls_body<1>, dim3(wv), B, 0, s, a, du, dF, dL, k0);
 				PDC_CHECK_LAUNCH();
-				hipLaunchKernelGGL(k_ls_final, dim3(1), dim3(256), 0, s, a, wc, wb);
+				hipLaunchKernelGGL(k_ls_final, dim3(1), dim3(1024), 0, s, a, wc, wv, k0, maxLS, sErr, sacErr);
 				PDC_CHECK_LAUNCH();
 				sync_scalars(ctx);
-				trials++;
-				const bool acpt = (H->t_inErr < sErr) | (H->t_sacIn < sacErr);
-				if ((acpt && H->t_gErr > 0) || k == maxLS - 1) {
-					if (acpt) {
-						hipLaunchKernelGGL(k_commit_v, dim3(grid_for(6 * NL)), B, 0, s, 6 * NL, ctx->l_v.p,
-						                   du, step);
-						PDC_CHECK_LAUNCH();
-						std::swap(ctx->s_f, ctx->s_ft);
-						std::swap(ctx->s_lam, ctx->s_lt);
-						a = make_args(ctx, dt, g);
-						return step;
-					}
-					break;
+				trials += H->ls_trials;
+				if (!H->ls_decided) continue;
+				const double step = H->ls_step;
+				if (step > 0) {
+					hipLaunchKernelGGL(k_ls_commit, dim3(wc), B, 0, s, a, dF, dL, step);
+					PDC_CHECK_LAUNCH();
+					hipLaunchKernelGGL(k_commit_v, dim3(grid_for(6 * NL)), B, 0, s, 6 * NL, ctx->l_v.p, du, step);
+					PDC_CHECK_LAUNCH();
 				}
-				step *= 0.5;
+				return step;
 			}
 			return 0.0;
 		};

@@ -64,7 +64,7 @@ void kt_begin(pdc_context* ctx, int which, hipEvent_t* ev) {
 	if (ctx->kt.which != which || which == PDC_KT_NONE) return;
 	auto& kt = ctx->kt;
 	if (kt.used + 2 > kt.pool.size()) {
-		for (int k = 0; k < 64; ++k) {
+		for (int k = 0; k < 512; ++k) {
 			hipEvent_t e;
 			PDC_HIP(hipEventCreate(&e));
 			kt.pool.push_back(e);
@@ -81,6 +81,13 @@ void kt_end(pdc_context* ctx, int which, hipEvent_t ev0) {
 	PDC_HIP(hipEventRecord(e1, ctx->stream));
 	kt.used += 2;
 	if (kt.used + 2 > 1024) kt_collect(ctx);
+}
+
+void kt_end_nocollect(pdc_context* ctx, hipEvent_t ev0) {
+	if (!ev0) return;
+	auto& kt = ctx->kt;
+	PDC_HIP(hipEventRecord(kt.pool[kt.used + 1], ctx->stream));
+	kt.used += 2;
 }
 
 void kt_collect(pdc_context* ctx) {
