@@ -30,7 +30,8 @@ PEAK_HBM = 8.0e12
 KT_NAMES = {"spmv": pdc.KT_SPMV, "pcg_update": pdc.KT_PCG, "contact_terms": pdc.KT_CONTACT_TERMS,
             "contacts": pdc.KT_CONTACTS}
 KT_KERNEL = {"spmv": "k_pcg_a (block-CSR SpMV q = A z + beta q, p update, p.q partials)",
-             "pcg_update": "k_pcg_b (x, r updates, block-Jacobi z = P r, partials)",
+             "pcg_update": "one PCG iteration = k_pcg_a + k_pcg_b (SpMV, vector updates, block-Jacobi "
+                           "z = P r), graph-replayed",
              "contact_terms": "k_constraints + k_contact_res", "contacts": "k_pair_hits"}
 
 

@@ -162,7 +162,6 @@ int pdc_create(int device, const pdc_params* params, pdc_context** out) {
 		PDC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
 		ctx->d_sc.ensure(1);
 		PDC_HIP(hipHostMalloc((void**)&ctx->h_sc, sizeof(pdc::Scalars), hipHostMallocDefault));
-		PDC_HIP(hipHostMalloc(&ctx->h_pcg_args, 512, hipHostMallocDefault));
 		if (const char* g = std::getenv("PDC_NO_GRAPHS")) ctx->use_graphs = g[0] == '0';
 		std::memset(ctx->h_sc, 0, sizeof(pdc::Scalars));
 		PDC_HIP(hipMemsetAsync(ctx->d_sc.p, 0, sizeof(pdc::Scalars), ctx->stream));
@@ -189,12 +188,9 @@ void pdc_destroy(pdc_context* ctx) {
 	for (auto& G : ctx->pcg_graph) {
 		if (G.exec) (void)hipGraphExecDestroy(G.exec);
 		if (G.graph) (void)hipGraphDestroy(G.graph);
-		for (auto e : G.ev) (void)hipEventDestroy(e);
 	}
 	for (auto e : ctx->kt_direct) (void)hipEventDestroy(e);
 	free_all(ctx);
-	ctx->pcg_args.release();
-	if (ctx->h_pcg_args) (void)hipHostFree(ctx->h_pcg_args);
 	for (auto e : ctx->kt.pool) (void)hipEventDestroy(e);
 	if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

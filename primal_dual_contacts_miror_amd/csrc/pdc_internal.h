@@ -60,8 +60,20 @@ struct DBuf {
 	operator T*() const { return p; }
 };
 
+// PCG control block: read by every PCG kernel, written by the first
+// workgroup of the kernel that changes it.
+struct PcgCtl {
+	int nb;        // block rows (bodies in contact)
+	int iters;     // iterations of the last solve
+	int done;      // converged flag
+	int itc[2];    // iteration-counter slots
+	int pad;
+	double tol2;   // squared relative tolerance
+};
+
 // Scalars shared between the reduction kernels and the host (one device struct).
 struct Scalars {
+	PcgCtl pcg;
 	double char_mass, char_speed;
 	double gap, muc;
 	double ru2, wrf2, rl2, ac2;
@@ -72,9 +84,6 @@ struct Scalars {
 	double t_inErr, t_sacIn, t_gErr;
 	double ls_step;
 	int ls_decided, ls_trials;
-	int pcg_iters;
-	int pcg_done;
-	int pcg_itc[2];
 	int overflow;        // capacity overflow flag set by kernels
 	int n_pairs;         // broadphase candidate pairs
 	int n_contacts;      // contacts after merge
@@ -89,9 +98,7 @@ struct PcgGraph {
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	int nblk = 0;
-	const void* args = nullptr;
-	int kt_which = 0;
-	std::vector<hipEvent_t> ev;
+	unsigned char args[512] = {};  // the kernel arguments the chunk was recorded with
 };
 
 struct KernelTimer {
@@ -160,11 +167,8 @@ struct pdc_context {
 	pdc::DBuf<int> inc_cnt, inc_ptr, inc, c_pid;
 	pdc::DBuf<int> a_cnt, a_ptr, a_col, a_diag;
 	int pcg_last_iters = 0;
-	pdc::DBuf<char> pcg_args;                                // PCG arguments (device)
-	void* h_pcg_args = nullptr;                               // pinned staging copy
 	pdc::PcgGraph pcg_graph[2];                               // recorded chunks (FP32, FP64)
 	bool use_graphs = true;
-	bool graph_events_ok = true;
 	std::vector<hipEvent_t> kt_direct;                        // events for direct-launch timing
 	hipEvent_t* kt_direct_ev(size_t n) {
 		while (kt_direct.size() < n) {

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "pdc_internal.h"
 #include "pdc_reduce.h"
@@ -947,8 +948,9 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, int NP, T* __restrict_
 // fixed order, so all of them take the same decisions.  Partials and r are
 // double-buffered by iteration parity.  The arguments and the iteration
 // counter live in device memory (the counter alternates between two slots),
-// so a chunk of iterations is a fixed sequence of launches that is recorded
-// once as a hipGraph and replayed; kernels after convergence return at once.
+// the buffer pointers are kernel arguments: a chunk of iterations is a fixed
+// sequence of launches, recorded once as a hipGraph and replayed (recorded
+// again only when a buffer moves); kernels after convergence return at once.
 template <class T>
 struct PcgArgs {
 	int nb;
@@ -966,10 +968,7 @@ struct PcgArgs {
 	T* rr[2];
 	T* pq;
 	T* bb;
-	T tol2;
-	int* done;
-	int* iters;
-	int* itc;   // two iteration-counter slots
+	PcgCtl* ctl;  // body count, tolerance, done flag, iteration counters (device)
 };
 
 template <class T>
@@ -1008,10 +1007,9 @@ __device__ __forceinline__ void reduce_partials(const T* const (&src)[N], int W,
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_pcg_init(const PcgArgs<T>* __restrict__ pa) {
+__global__ __launch_bounds__(256) void k_pcg_init(PcgArgs<T> a) {
 	__shared__ T lds[3 * 16];
-	const PcgArgs<T> a = *pa;
-	const int n = 6 * a.nb;
+	const int n = 6 * a.ctl->nb;
 	T v[2] = {0, 0};
 	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
 		const int br = t / 6, i = t - 6 * br;
@@ -1032,28 +1030,28 @@ __global__ __launch_bounds__(256) void k_pcg_init(const PcgArgs<T>* __restrict__
 		a.rz[0][blockIdx.x] = out[0];
 		a.rr[0][blockIdx.x] = out[1];
 		a.bb[blockIdx.x] = out[1];
-		if (blockIdx.x == 0) { *a.done = 0; *a.iters = 0; a.itc[0] = 0; a.itc[1] = 0; }
+		if (blockIdx.x == 0) { a.ctl->done = 0; a.ctl->iters = 0; a.ctl->itc[0] = 0; a.ctl->itc[1] = 0; }
 	}
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_pcg_a(const PcgArgs<T>* __restrict__ pa, int slot) {
+__global__ __launch_bounds__(256) void k_pcg_a(PcgArgs<T> a, int slot) {
 	__shared__ T lds[16];
-	const PcgArgs<T> a = *pa;
-	if (*a.done) return;
-	const int it = a.itc[slot];
+	const PcgCtl c = *a.ctl;
+	if (c.done) return;
+	const int it = c.itc[slot];
 	const int W = gridDim.x;
 	const T* src[4] = {a.rr[it & 1], a.bb, a.rz[it & 1], a.rz[(it + 1) & 1]};
 	T red[4];
 	reduce_partials<4, T>(src, W, red, lds);
 	const T rr = red[0], bb = red[1], rz = red[2];
-	if (!(rr > a.tol2 * bb)) {
-		if (blockIdx.x == 0 && threadIdx.x == 0) { *a.done = 1; *a.iters = it; }
+	if (!(rr > (T)c.tol2 * bb)) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) { a.ctl->done = 1; a.ctl->iters = it; }
 		return;
 	}
 	// rz[(it+1)&1] holds rz_{it-1} until B(it) overwrites it
 	const T beta = it > 0 ? rz / red[3] : T(0);
-	const int n = 6 * a.nb;
+	const int n = 6 * c.nb;
 	T v[1] = {0};
 	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
 		const T pt = a.z[t] + beta * a.p[t];
@@ -1069,11 +1067,11 @@ __global__ __launch_bounds__(256) void k_pcg_a(const PcgArgs<T>* __restrict__ pa
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_pcg_b(const PcgArgs<T>* __restrict__ pa, int slot) {
+__global__ __launch_bounds__(256) void k_pcg_b(PcgArgs<T> a, int slot) {
 	__shared__ T lds[2 * 16];
-	const PcgArgs<T> a = *pa;
-	if (*a.done) return;
-	const int it = a.itc[slot];
+	const PcgCtl c = *a.ctl;
+	if (c.done) return;
+	const int it = c.itc[slot];
 	const int W = gridDim.x;
 	const T* src[2] = {a.rz[it & 1], a.pq};
 	T red[2];
@@ -1081,7 +1079,7 @@ __global__ __launch_bounds__(256) void k_pcg_b(const PcgArgs<T>* __restrict__ pa
 	const T alpha = red[0] / red[1];
 	const T* rc = a.r[it & 1];
 	T* rn = a.r[(it + 1) & 1];
-	const int n = 6 * a.nb;
+	const int n = 6 * c.nb;
 	T v[2] = {0, 0};
 	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
 		const int br = t / 6, i = t - 6 * br;
@@ -1101,7 +1099,7 @@ __global__ __launch_bounds__(256) void k_pcg_b(const PcgArgs<T>* __restrict__ pa
 	if (threadIdx.x == 0) {
 		a.rz[(it + 1) & 1][blockIdx.x] = out[0];
 		a.rr[(it + 1) & 1][blockIdx.x] = out[1];
-		if (blockIdx.x == 0) a.itc[slot ^ 1] = it + 1;
+		if (blockIdx.x == 0) a.ctl->itc[slot ^ 1] = it + 1;
 	}
 }
 
@@ -1413,8 +1411,8 @@ __global__ __launch_bounds__(256) void k_integrate(int N, const int* __restrict_
 }
 
 // Iteration count when the solve ran to the iteration cap.
-__global__ void k_pcg_final(const int* done, int* iters, int maxit) {
-	if (threadIdx.x == 0 && !*done) *iters = maxit;
+__global__ void k_pcg_final(PcgCtl* ctl, int maxit) {
+	if (threadIdx.x == 0 && !ctl->done) ctl->iters = maxit;
 }
 
 // ===========================================================================
@@ -1449,22 +1447,17 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 }
 
 // One recorded chunk of kPcgChunk PCG iterations (kernel pairs alternating
-// the counter slots 0, 1).  With event timing on, an event pair brackets each
-// SpMV kernel (or each update kernel) inside the graph.
+// the counter slots 0, 1).
 constexpr int kPcgChunk = 16;
 
 template <class T>
-static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>* dargs, int nblk, int kt_which,
-                               hipEvent_t* ev) {
+static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk, hipEvent_t* ev) {
 	hipStream_t s = ctx->stream;
 	for (int k = 0; k < kPcgChunk; ++k) {
-		const int slot = k & 1;
-		if (ev && kt_which == PDC_KT_SPMV) PDC_HIP(hipEventRecord(ev[2 * k], s));
-		hipLaunchKernelGGL(k_pcg_a<T>, dim3(nblk), dim3(256), 0, s, dargs, slot);
-		if (ev && kt_which == PDC_KT_SPMV) PDC_HIP(hipEventRecord(ev[2 * k + 1], s));
-		if (ev && kt_which == PDC_KT_PCG) PDC_HIP(hipEventRecord(ev[2 * k], s));
-		hipLaunchKernelGGL(k_pcg_b<T>, dim3(nblk), dim3(256), 0, s, dargs, slot);
-		if (ev && kt_which == PDC_KT_PCG) PDC_HIP(hipEventRecord(ev[2 * k + 1], s));
+		if (ev) PDC_HIP(hipEventRecord(ev[2 * k], s));
+		hipLaunchKernelGGL(k_pcg_a<T>, dim3(nblk), dim3(256), 0, s, pa, k & 1);
+		if (ev) PDC_HIP(hipEventRecord(ev[2 * k + 1], s));
+		hipLaunchKernelGGL(k_pcg_b<T>, dim3(nblk), dim3(256), 0, s, pa, k & 1);
 	}
 	PDC_CHECK_LAUNCH();
 }
@@ -1478,7 +1471,6 @@ static int run_pcg(pdc_context* ctx, DBuf<T>& A, DBuf<T>& P, DBuf<T>& b, DBuf<T>
 	const size_t ncap = 6 * (size_t)std::max(ctx->N, 1);
 	work.ensure(5 * ncap + 6 * 2048 + 64);
 	PcgArgs<T> pa;
-	pa.nb = NL;
 	pa.ptr = ctx->a_ptr.p;
 	pa.col = ctx->a_col.p;
 	pa.A = A.p;
@@ -1490,110 +1482,96 @@ static int run_pcg(pdc_context* ctx, DBuf<T>& A, DBuf<T>& P, DBuf<T>& b, DBuf<T>
 	T* parts = w + 5 * ncap;
 	pa.rz[0] = parts; pa.rz[1] = parts + 2048; pa.rr[0] = parts + 2 * 2048; pa.rr[1] = parts + 3 * 2048;
 	pa.pq = parts + 4 * 2048; pa.bb = parts + 5 * 2048;
-	pa.tol2 = (T)(ctx->p.pcg_tol * ctx->p.pcg_tol);
-	pa.done = &ctx->d_sc.p->pcg_done;
-	pa.iters = &ctx->d_sc.p->pcg_iters;
-	pa.itc = ctx->d_sc.p->pcg_itc;
-	ctx->pcg_args.ensure(sizeof(PcgArgs<double>));
-	const PcgArgs<T>* dargs = reinterpret_cast<const PcgArgs<T>*>(ctx->pcg_args.p);
+	pa.ctl = &ctx->d_sc.p->pcg;
 	hipStream_t s = ctx->stream;
-	*reinterpret_cast<PcgArgs<T>*>(ctx->h_pcg_args) = pa;
-	PDC_HIP(hipMemcpyAsync(ctx->pcg_args.p, ctx->h_pcg_args, sizeof(PcgArgs<T>), hipMemcpyHostToDevice, s));
-	hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(256), 0, s, dargs);
+	PcgCtl* hc = &ctx->h_sc->pcg;
+	hc->nb = NL;
+	hc->tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
+	PDC_HIP(hipMemcpyAsync(pa.ctl, hc, sizeof(PcgCtl), hipMemcpyHostToDevice, s));
+	hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(256), 0, s, pa);
 	PDC_CHECK_LAUNCH();
-
-	const int kt_which = ctx->kt.which;
-	const bool kt_on = kt_which == PDC_KT_SPMV || kt_which == PDC_KT_PCG;
-	// graphs carry the event nodes only where the runtime records events in a capture
-	const bool graph = ctx->use_graphs && !(kt_on && !ctx->graph_events_ok);
+	// the chunk is recorded again when a buffer moved or the grid changed
 	PcgGraph& G = ctx->pcg_graph[sizeof(T) == 8 ? 1 : 0];
-	const int want_kt = kt_on ? kt_which : 0;
-	if (graph && (G.exec == nullptr || G.nblk != nblk || G.args != (const void*)dargs || G.kt_which != want_kt)) {
+	if (ctx->use_graphs && (G.exec == nullptr || G.nblk != nblk ||
+	                        std::memcmp(G.args, &pa, sizeof(pa)) != 0)) {
 		if (G.exec) { (void)hipGraphExecDestroy(G.exec); G.exec = nullptr; }
 		if (G.graph) { (void)hipGraphDestroy(G.graph); G.graph = nullptr; }
-		if (kt_on && G.ev.empty()) {
-			G.ev.resize(2 * kPcgChunk);
-			for (auto& e : G.ev) PDC_HIP(hipEventCreate(&e));
-		}
 		PDC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-		bool ok = true;
-		try {
-			pcg_chunk_launches<T>(ctx, dargs, nblk, kt_which, kt_on ? G.ev.data() : nullptr);
-		} catch (const HipError&) {
-			ok = false;
-		}
-		hipGraph_t gr = nullptr;
-		const hipError_t e1 = hipStreamEndCapture(s, &gr);
-		if (!ok || e1 != hipSuccess || gr == nullptr) {
-			if (gr) (void)hipGraphDestroy(gr);
-			(void)hipGetLastError();
-			if (!kt_on) throw HipError("PCG chunk capture failed", PDC_ERR_DEVICE);
-			ctx->graph_events_ok = false;  // time with direct launches from now on
-		} else {
-			G.graph = gr;
-			PDC_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
-			G.nblk = nblk;
-			G.args = dargs;
-			G.kt_which = want_kt;
-		}
+		pcg_chunk_launches<T>(ctx, pa, nblk, nullptr);
+		PDC_HIP(hipStreamEndCapture(s, &G.graph));
+		PDC_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+		G.nblk = nblk;
+		std::memcpy(G.args, &pa, sizeof(pa));
 	}
-	const bool use_graph = graph && G.exec != nullptr && G.kt_which == want_kt;
-	// chunks: as many as the previous solve needed, then one at a time
+	// Kernel timing.  SPMV: k_pcg_a alone, launched directly between an event
+	// pair per launch (events cannot be timed inside a replayed graph); only
+	// launches of active iterations are counted.  PCG: the whole iteration
+	// (k_pcg_a + k_pcg_b) from an event pair around every group of chunks,
+	// charged to the active iterations (the idle launches after convergence
+	// are included in the time).
+	const bool kt_spmv = ctx->kt.which == PDC_KT_SPMV;
+	const bool kt_iter = ctx->kt.which == PDC_KT_PCG;
+	const bool graph = ctx->use_graphs && !kt_spmv;
+	hipEvent_t* ev = kt_spmv ? ctx->kt_direct_ev(2 * kPcgChunk) : kt_iter ? ctx->kt_direct_ev(2) : nullptr;
 	const int maxit = ctx->p.pcg_max_iter;
 	int launched = 0;
 	int want = std::max(1, (std::min(ctx->pcg_last_iters + 2, maxit) + kPcgChunk - 1) / kPcgChunk);
 	int flags[2] = {0, 0};
+	double kt_ms = 0;
 	while (launched < maxit) {
+		if (kt_spmv) want = 1;  // the event pool holds one chunk
+		if (kt_iter) PDC_HIP(hipEventRecord(ev[0], s));
+		const int before = launched;
 		for (int c = 0; c < want && launched < maxit; ++c) {
-			if (use_graph) {
-				PDC_HIP(hipGraphLaunch(G.exec, s));
-			} else if (kt_on) {
-				pcg_chunk_launches<T>(ctx, dargs, nblk, kt_which, ctx->kt_direct_ev(2 * kPcgChunk));
-			} else {
-				pcg_chunk_launches<T>(ctx, dargs, nblk, 0, nullptr);
-			}
+			if (graph) PDC_HIP(hipGraphLaunch(G.exec, s));
+			else pcg_chunk_launches<T>(ctx, pa, nblk, kt_spmv ? ev : nullptr);
 			launched += kPcgChunk;
-			if (kt_on) {
-				// the chunk's events are reused by the next chunk: read them now
-				PDC_HIP(hipMemcpyAsync(flags, &ctx->d_sc.p->pcg_iters, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-				PDC_HIP(hipStreamSynchronize(s));
-				const int done_at = flags[1] ? flags[0] : launched;
-				const int active = std::max(0, std::min(kPcgChunk, done_at - (launched - kPcgChunk)));
-				hipEvent_t* ev = use_graph ? G.ev.data() : ctx->kt_direct_ev(2 * kPcgChunk);
-				for (int k = 0; k < active; ++k) {
-					float e = 0;
-					PDC_HIP(hipEventElapsedTime(&e, ev[2 * k], ev[2 * k + 1]));
-					ctx->kt.total_ms += e;
-					ctx->kt.launches += 1;
-				}
-				if (flags[1]) break;
-			}
 		}
-		PDC_HIP(hipMemcpyAsync(flags, &ctx->d_sc.p->pcg_iters, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+		if (kt_iter) PDC_HIP(hipEventRecord(ev[1], s));
+		PDC_HIP(hipMemcpyAsync(flags, &ctx->d_sc.p->pcg.iters, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
 		PDC_HIP(hipStreamSynchronize(s));
-		if (flags[1]) break;  // pcg_done
+		if (kt_iter) {
+			float e = 0;
+			PDC_HIP(hipEventElapsedTime(&e, ev[0], ev[1]));
+			kt_ms += e;
+		} else if (kt_spmv) {
+			// iterations k < iters reduced and updated; iteration `iters` ran
+			// its SpMV launch only to the convergence test
+			const int active = flags[1] ? std::min(kPcgChunk, flags[0] - before) : kPcgChunk;
+			for (int k = 0; k < active; ++k) {
+				float e = 0;
+				PDC_HIP(hipEventElapsedTime(&e, ev[2 * k], ev[2 * k + 1]));
+				kt_ms += e;
+			}
+			ctx->kt.launches += std::max(active, 0);
+		}
+		if (flags[1]) break;  // done
 		want = 1;
 	}
 	const int iters = flags[1] ? flags[0] : maxit;
-	if (kt_on) {
+	if (kt_spmv || kt_iter) {
+		// Algorithmic bytes, each input read once and each output written once:
+		//   k_pcg_a  A blocks and column indices nnzb (36 T + 4), row pointers
+		//            4 (NL + 1), z read T n6, p and q read and written 4 T n6
+		//   k_pcg_b  x read and written 2 T n6, p, r, q read 3 T n6, r' and z
+		//            written 2 T n6; block-Jacobi blocks 36 T NL
 		const double Tb = sizeof(T), n6 = 6.0 * NL, nnzb = ctx->NNZB;
-		if (kt_which == PDC_KT_SPMV)
-			ctx->kt.bytes += iters * (nnzb * (36.0 * Tb + 4.0) + 4.0 * (NL + 1) + 5.0 * Tb * n6);
-		else
-			ctx->kt.bytes += iters * (7.0 * Tb * n6 + 36.0 * Tb * NL);
+		const double spmv = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NL + 1) + 5.0 * Tb * n6;
+		ctx->kt.total_ms += kt_ms;
+		if (kt_iter) {
+			ctx->kt.launches += iters;
+			ctx->kt.bytes += iters * (spmv + 7.0 * Tb * n6 + 36.0 * Tb * NL);
+		} else {
+			ctx->kt.bytes += iters * spmv;
+		}
 	}
-	hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(64), 0, s, &ctx->d_sc.p->pcg_done, &ctx->d_sc.p->pcg_iters, maxit);
+	hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(64), 0, s, pa.ctl, maxit);
 	hipLaunchKernelGGL(k_du<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, NL, ctx->l_D.p, x.p, ctx->l_du.p);
 	PDC_CHECK_LAUNCH();
 	return nblk;
 }
 
-// Algorithmic bytes (each input read once, each output written once), per
-// launch, scalar size T, n6 = 6 NL:
-//   k_pcg_a  A blocks and column indices nnzb (36 T + 4), row pointers
-//            4 (NL+1), z read T n6, p and q read and written 4 T n6
-//   k_pcg_b  x read/written 2 T n6; p, r, q read 3 T n6; r', z written 2 T n6;
-//            block-Jacobi blocks 36 T NL
+// Algorithmic bytes (each input read once, each output written once).
 // Constraint + contact-residual kernels of one Newton iteration, per contact:
 //   f 24, lambda 16 in; c 16, g 16, H 24 out (constraints);
 //   T 288, local ids 8, f 24, lambda 16, K 8 in; rf 24, rl 16 out (residuals);
@@ -1705,7 +1683,7 @@ void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st) {
 		hipLaunchKernelGGL(k_increments, dim3(wc), B, 0, s, a);
 		PDC_CHECK_LAUNCH();
 		sync_scalars(ctx);
-		const int iters = H->pcg_iters;
+		const int iters = H->pcg.iters;
 		ctx->pcg_last_iters = iters;
 		pcg_total += iters;
 		if (iters == ctx->p.pcg_max_iter) pcg_hits++;
