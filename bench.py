@@ -27,11 +27,11 @@ from primal_dual_contacts_miror_amd import pdc, scenes  # noqa: E402
 METRIC = "Newton timesteps/sec at 30k contacts; batched scene-steps/sec + p50/p99 latency"
 PEAK_HBM = 8.0e12
 # kernel classes timed with HIP events (include/pdc.h PDC_KT_*)
-KT_NAMES = {"spmv": pdc.KT_SPMV, "pcg_update": pdc.KT_PCG, "contact_terms": pdc.KT_CONTACT_TERMS,
+KT_NAMES = {"pcg": pdc.KT_PCG, "pcg_per_launch": pdc.KT_SPMV, "contact_terms": pdc.KT_CONTACT_TERMS,
             "contacts": pdc.KT_CONTACTS}
-KT_KERNEL = {"spmv": "k_pcg_a (block-CSR SpMV q = A z + beta q, p update, p.q partials)",
-             "pcg_update": "one PCG iteration = k_pcg_a + k_pcg_b (SpMV, vector updates, block-Jacobi "
-                           "z = P r), graph-replayed",
+KT_KERNEL = {"pcg": "k_pcg_iter (one PCG iteration: block-CSR SpMV with A P, block-Jacobi apply, vector "
+                    "updates, partial dot products), timed per replayed graph of 16 launches",
+             "pcg_per_launch": "k_pcg_iter, launched without graph replay, timed per launch",
              "contact_terms": "k_constraints + k_contact_res", "contacts": "k_pair_hits"}
 
 
@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pile_2000")
-    ap.add_argument("--kernel", default="spmv", choices=sorted(KT_NAMES))
+    ap.add_argument("--kernel", default="pcg", choices=sorted(KT_NAMES))
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")

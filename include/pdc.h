@@ -189,10 +189,11 @@ void pdc_reset_timings(pdc_context* ctx);
  * pdc_kernel_timing returns the summed milliseconds, the launch count and the
  * summed algorithmic bytes (compulsory traffic: each input read once, each
  * output written once) of those launches. */
-/* Classes: SPMV = the PCG kernel holding the block-CSR SpMV (k_pcg_a),
- * timed per launch with the PCG iterations launched directly (no graph
- * replay) while enabled; PCG = one whole PCG iteration (k_pcg_a + k_pcg_b),
- * timed per replayed group of iterations and counted per active iteration;
+/* Classes: SPMV = the PCG iteration kernel (k_pcg_iter: block-CSR SpMV with
+ * the preconditioned operator, block-Jacobi apply, vector updates, partial
+ * dot products), timed per launch with the iterations launched directly (no
+ * graph replay) while enabled, active iterations only; PCG = the same kernel
+ * timed per replayed group of iterations, every launch counted;
  * CONTACT_TERMS = constraint + contact-residual kernels of a Newton
  * iteration; ASSEMBLY = the assembly kernels; LINE_SEARCH = the contact
  * kernel of a line-search trial; CONTACTS = the sample-test kernel. */

@@ -102,7 +102,7 @@ void free_all(pdc_context* c) {
 	c->s_K.release(); c->s_f.release(); c->s_lam.release(); c->s_c.release(); c->s_g.release();
 	c->s_H.release(); c->s_rf.release(); c->s_rl.release(); c->s_sf.release(); c->s_invC.release();
 	c->s_W.release(); c->s_kf.release(); c->s_df.release(); c->s_dl.release(); c->partials.release();
-	c->A32.release(); c->P32.release(); c->b32.release(); c->x32.release(); c->pcg32.release();
+	c->A32.release(); c->P32.release(); c->b32.release();
 	c->A64.release(); c->P64.release(); c->b64.release(); c->x64.release(); c->pcg64.release();
 	c->d_sc.release();
 	c->c_stride = 0;

@@ -167,7 +167,7 @@ struct pdc_context {
 	pdc::DBuf<int> inc_cnt, inc_ptr, inc, c_pid;
 	pdc::DBuf<int> a_cnt, a_ptr, a_col, a_diag;
 	int pcg_last_iters = 0;
-	pdc::PcgGraph pcg_graph[2];                               // recorded chunks (FP32, FP64)
+	pdc::PcgGraph pcg_graph[4];                               // recorded chunks (FP32, FP64; x2 forms)
 	bool use_graphs = true;
 	std::vector<hipEvent_t> kt_direct;                        // events for direct-launch timing
 	hipEvent_t* kt_direct_ev(size_t n) {
@@ -187,7 +187,7 @@ struct pdc_context {
 	pdc::DBuf<double> s_K, s_f, s_lam, s_c, s_g, s_H;        // per contact
 	pdc::DBuf<double> s_rf, s_rl, s_sf, s_invC, s_W, s_kf, s_df, s_dl;
 	pdc::DBuf<double> partials;                              // [k][kMaxPartialBlocks]
-	pdc::DBuf<float> A32, P32, b32, x32, pcg32;              // FP32 PCG
+	pdc::DBuf<float> A32, P32, b32;                           // FP32 PCG operator, preconditioner, rhs
 	pdc::DBuf<double> A64, P64, b64, x64, pcg64;             // FP64 PCG
 	pdc::DBuf<pdc::Scalars> d_sc;
 	pdc::Scalars* h_sc = nullptr;                            // pinned mirror

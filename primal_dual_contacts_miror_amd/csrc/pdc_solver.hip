@@ -282,7 +282,7 @@ void build_topology(pdc_context* ctx) {
 	                   ctx->adj_nb_u.p, ctx->adj_pair_u.p, ctx->adj_row_u.p, ctx->ap_a.p, ctx->g2l.p,
 	                   ctx->adj_nb.p, ctx->adj_pair.p, ctx->pair_adj.p);
 	PDC_CHECK_LAUNCH();
-	ctx->l_static.ensure(NL + 1); ctx->a_cnt.ensure(NL + 1); ctx->a_ptr.ensure(NL + 1);
+	ctx->l_static.ensure(NL + 1); ctx->a_cnt.ensure(NL + 1); ctx->a_ptr.ensure((size_t)ctx->N + 1);
 	ctx->a_diag.ensure(NL + 1); ctx->inc_cnt.ensure(NL + 1); ctx->inc_ptr.ensure(NL + 1);
 	hipLaunchKernelGGL(k_adj_pos, dim3(std::max(1, std::min((NL + 3) / 4, 1024))), dim3(kBlock), 0, s, NL,
 	                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->adj_pair.p, ctx->ap_start.p, ctx->l2g.p,
@@ -819,8 +819,9 @@ __device__ __forceinline__ bool spd_inverse6(const double* A, double* Ai) {
 	return true;
 }
 
-// Diagonal blocks, rhs, Jacobi scaling D, block-Jacobi preconditioner, scaled b.
-// One wavefront per body: lanes accumulate T_j W T_j^T (upper triangle) and
+// Diagonal blocks, rhs, Jacobi scaling D, block-Jacobi preconditioner P, scaled
+// b; the stored matrix is the preconditioned operator A P (block (a, a) =
+// A_aa P_a, the identity up to rounding).  One wavefront per body: lanes accumulate T_j W T_j^T (upper triangle) and
 // y = -sum T_j Kinv sf over the body's incident contacts; lane 0 finishes.
 template <class T>
 __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict__ adiag, T* __restrict__ Aval,
@@ -892,25 +893,34 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 			double S[36], Si[36];
 			for (int ii = 0; ii < 6; ++ii)
 				for (int jj = 0; jj < 6; ++jj) S[ii * 6 + jj] = B[ii * 6 + jj] * (Dr[ii] * Dr[jj]);
-			const int d = adiag[r];
-			for (int q = 0; q < 36; ++q) Aval[36 * (size_t)d + q] = (T)S[q];
 			if (!spd_inverse6(S, Si)) {
 				for (int q = 0; q < 36; ++q) Si[q] = 0;
 				for (int q = 0; q < 6; ++q) Si[q * 6 + q] = 1.0 / S[q * 6 + q];
 			}
+			// the preconditioner as stored, and the diagonal block of A P with it
+			for (int q = 0; q < 36; ++q) Si[q] = (double)(T)Si[q];
 			for (int q = 0; q < 36; ++q) Pval[36 * (size_t)r + q] = (T)Si[q];
+			const int d = adiag[r];
+			for (int ii = 0; ii < 6; ++ii)
+				for (int jj = 0; jj < 6; ++jj) {
+					double acc = 0;
+					for (int k = 0; k < 6; ++k) acc += S[ii * 6 + k] * Si[k * 6 + jj];
+					Aval[36 * (size_t)d + ii * 6 + jj] = (T)acc;
+				}
 		}
 	}
 }
 
-// Off-diagonal blocks, one wavefront per body pair: B = sum over the pair's
-// contacts of T_side(a) W T_side(b)^T; block (a, b) = D_a B D_b and block
-// (b, a) = its transpose (the Newton matrix is symmetric).
+// Off-diagonal blocks, one wavefront per body pair: S = sum over the pair's
+// contacts of T_side(a) W T_side(b)^T; A_ab = D_a S D_b and A_ba = A_ab^T (the
+// Newton matrix is symmetric).  Stored are the blocks of the preconditioned
+// operator, A_ab P_b and A_ba P_a; P comes from k_asm_diag, which runs first.
 template <class T>
-__global__ __launch_bounds__(256) void k_asm_off(SArgs a, int NP, T* __restrict__ Aval) {
+__global__ __launch_bounds__(256) void k_asm_off(SArgs a, int NP, const T* __restrict__ Pval, T* __restrict__ Aval) {
 	const int lane = threadIdx.x & 63;
 	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
 	const int nw = (gridDim.x * blockDim.x) >> 6;
+	const int ii = lane < 36 ? lane / 6 : 0, jj = lane < 36 ? lane - 6 * (lane / 6) : 0;
 	for (int p = wave; p < NP; p += nw) {
 		const int ka = a.pair_adj[2 * p], kb = a.pair_adj[2 * p + 1];
 		const int blk_ab = a.adj_blk[ka], blk_ba = a.adj_blk[kb];
@@ -924,82 +934,143 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, int NP, T* __restrict_
 		}
 		for (int q = 0; q < 36; ++q) B[q] = wave_sum(B[q]);
 		const int rb = a.li0[a.pstart[p]] == ra ? a.li1[a.pstart[p]] : a.li0[a.pstart[p]];
-		if (lane < 36) {
-			const int ii = lane / 6, jj = lane - 6 * (lane / 6);
-			double bl = 0;  // B[lane] without a dynamically indexed array
+		double bl = 0;  // B[lane] without a dynamically indexed array
 #pragma unroll
-			for (int q = 0; q < 36; ++q) bl = q == lane ? B[q] : bl;
-			const double v = bl * (a.D[6 * ra + ii] * a.D[6 * rb + jj]);
-			Aval[36 * (size_t)blk_ab + ii * 6 + jj] = (T)v;
-			Aval[36 * (size_t)blk_ba + jj * 6 + ii] = (T)v;
+		for (int q = 0; q < 36; ++q) bl = q == lane ? B[q] : bl;
+		// lane (ii, jj) < 36 holds A_ab[ii][jj]; rows and columns by shuffles
+		const double v = bl * (a.D[6 * ra + ii] * a.D[6 * rb + jj]);
+		double bab = 0, bba = 0;
+#pragma unroll
+		for (int k = 0; k < 6; ++k) {
+			const double a_ik = __shfl(v, 6 * ii + k, 64);  // A_ab[ii][k]
+			const double a_ki = __shfl(v, 6 * k + ii, 64);  // A_ab[k][ii] = A_ba[ii][k]
+			bab += a_ik * (double)Pval[36 * (size_t)rb + 6 * k + jj];
+			bba += a_ki * (double)Pval[36 * (size_t)ra + 6 * k + jj];
+		}
+		if (lane < 36) {
+			Aval[36 * (size_t)blk_ab + ii * 6 + jj] = (T)bab;
+			Aval[36 * (size_t)blk_ba + ii * 6 + jj] = (T)bba;
 		}
 	}
 }
 
 // ---- (d) PCG ----------------------------------------------------------------
-// Block-Jacobi PCG on the scaled block-CSR system, zero initial guess, as two
-// ordinary kernels per iteration (no grid-wide barrier inside a kernel):
-//   A(it): convergence test on |r|^2 (stop when |r|^2 <= tol^2 |b|^2),
-//          beta = rz_it / rz_{it-1}, p = z + beta p, q = A z + beta q  (= A p),
-//          partial p.q
-//   B(it): alpha = rz_it / p.q, x += alpha p, r' = r - alpha q, z = P r',
-//          partial r'.z and r'.r'
-// Every workgroup re-reduces the previous kernel's partial sums in the same
-// fixed order, so all of them take the same decisions.  Partials and r are
-// double-buffered by iteration parity.  The arguments and the iteration
-// counter live in device memory (the counter alternates between two slots),
-// the buffer pointers are kernel arguments: a chunk of iterations is a fixed
-// sequence of launches, recorded once as a hipGraph and replayed (recorded
-// again only when a buffer moves); kernels after convergence return at once.
+// Block-Jacobi preconditioned CG on the scaled block-CSR system with a zero
+// initial guess, in the single-reduction form of Chronopoulos and Gear
+// ("s-step iterative methods for symmetric linear systems", J. Comput. Appl.
+// Math. 25 (1989); the preconditioned recurrences as listed by Ghysels and
+// Vanroose, Parallel Computing 40 (2014), Alg. 3).  With M = P the
+// block-Jacobi inverse and the assembled operator B = A M (B_ij = A_ij P_j):
+//   init: r = b, u = M b, w = A u = B b, z = q = s = p = 0
+//   iteration i:  gamma = (r, u), delta = (w, u), |r|^2   (from the partials)
+//     stop when |r|^2 <= tol^2 |b|^2
+//     beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
+//     (i = 0: beta = 0, alpha = gamma / delta)
+//     m = M w, n = A m = B w
+//     z = n + beta z, q = m + beta q, s = w + beta s, p = u + beta p
+//     x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z
+//     partials of (r, u), (w, u), (r, r) for iteration i + 1
+// One kernel per iteration, so the kernel boundary is the only grid-wide
+// synchronisation.  Every workgroup re-reduces the partial sums in the same
+// fixed order and takes the same decisions.  w (read by neighbouring block
+// rows) and the partials are double-buffered by iteration parity; gamma and
+// alpha of the previous iteration pass through a two-slot history written by
+// workgroup 0.  The iteration counter alternates between two slots of the
+// control block, so a chunk of iterations is a fixed sequence of launches
+// (recorded once as a hipGraph and replayed); after convergence every launch
+// returns at once.
+//
+// The recursively updated u and w carry absolute rounding errors of order
+// eps |u_0|, so their relative accuracy falls as the residual falls; the
+// vectors, the reductions and the scalars are therefore FP64 on both paths,
+// while B and P are stored as T (FP32 on the FP32 path, which then solves the
+// FP32-rounded operator).
+//
+// Layout: one wavefront per block row; lane = 8 * row + slot for the six
+// scalar rows of the block row, the eight slot lanes of a row stride over the
+// row's blocks and are summed with three xor shuffles.
 template <class T>
 struct PcgArgs {
-	int nb;
 	const int* ptr;
 	const int* col;
-	const T* A;
-	const T* P;
+	const T* B;     // A M, block-CSR, row-major 6x6 blocks
+	const T* P;     // block-Jacobi inverse, one 6x6 block per block row
 	const T* b;
-	T* x;
-	T* r[2];
-	T* z;
-	T* p;
-	T* q;
-	T* rz[2];   // partials, one per workgroup
-	T* rr[2];
-	T* pq;
-	T* bb;
-	PcgCtl* ctl;  // body count, tolerance, done flag, iteration counters (device)
+	double* x;
+	double* r;
+	double* u;
+	double* w[2];
+	double* z;
+	double* q;
+	double* s;
+	double* p;
+	double* gam[2];  // partials, one per workgroup, by iteration parity
+	double* del[2];
+	double* rr[2];
+	double* bb;
+	double* hist;    // gamma, alpha of iteration i at [2 (i & 1)], [2 (i & 1) + 1]
+	PcgCtl* ctl;     // block rows, tolerance, done flag, iteration counters (device)
 };
 
-template <class T>
-__device__ __forceinline__ T spmv_row(const PcgArgs<T>& a, const T* __restrict__ xv, int t) {
-	const int br = t / 6, i = t - 6 * br;
-	T s = 0;
-	for (int k = a.ptr[br]; k < a.ptr[br + 1]; ++k) {
-		const T* blk = a.A + 36 * (size_t)k + 6 * i;
-		const T* xc = xv + 6 * a.col[k];
+// Element k (0 or 1) of a two-slot kernel argument, selected without a runtime
+// index into the argument struct (which would give it a per-thread copy).
+template <class P>
+__device__ __forceinline__ P pick(const P (&slots)[2], int k) {
+	return k ? slots[1] : slots[0];
+}
+
+// Row `i` (0..5) of block row `br` of B times v; every lane of the row's
+// eight-lane group returns the sum.  Lanes of rows 6, 7 return 0.
+template <class T, class V>
+__device__ __forceinline__ double spmv_lane(const PcgArgs<T>& a, const V* __restrict__ v, int br, int i, int sl) {
+	double acc = 0;
+	if (i < 6) {
+		const int k0 = a.ptr[br], k1 = a.ptr[br + 1];
+		for (int k = k0 + sl; k < k1; k += 8) {
+			const T* blk = a.B + 36 * (size_t)k + 6 * i;
+			const V* vc = v + 6 * (size_t)a.col[k];
 #pragma unroll
-		for (int c = 0; c < 6; ++c) s += blk[c] * xc[c];
+			for (int c = 0; c < 6; ++c) acc += (double)blk[c] * (double)vc[c];
+		}
 	}
+	acc += __shfl_xor(acc, 4, 64);
+	acc += __shfl_xor(acc, 2, 64);
+	acc += __shfl_xor(acc, 1, 64);
+	return acc;
+}
+
+constexpr int kPcgRowsPerWG = 8;            // block rows (wavefronts) per workgroup
+constexpr int kPcgThreads = 64 * kPcgRowsPerWG;
+constexpr int kPartLoads = kMaxWG / 64;     // partial values per lane
+
+// Lane share of one partial array (one value per workgroup of the previous
+// launch, W <= kMaxWG of them): all loads issued at once, summed in a fixed
+// order.  The wavefront sum of the returned values is the array total.
+__device__ __forceinline__ double partial_lane_sum(const double* __restrict__ src, int W) {
+	const int lane = threadIdx.x & 63;
+	double v[kPartLoads];
+	// unconditional loads (indices past W clamped to the last value, which the
+	// sum then drops) so that all of them are in flight together
+#pragma unroll
+	for (int k = 0; k < kPartLoads; ++k) v[k] = src[min(lane + 64 * k, W - 1)];
+	double s = 0;
+#pragma unroll
+	for (int k = 0; k < kPartLoads; ++k) s += lane + 64 * k < W ? v[k] : 0.0;
 	return s;
 }
 
-// Sums of N partial arrays (one value per workgroup) in a fixed order, all
-// loads issued before the reductions; every thread gets the results.
-template <int N, class T>
-__device__ __forceinline__ void reduce_partials(const T* const (&src)[N], int W, T (&out)[N], T* lds) {
-	if (threadIdx.x < 64) {
-		T v[N];
-#pragma unroll
-		for (int k = 0; k < N; ++k) v[k] = 0;
-		for (int w = threadIdx.x; w < W; w += 64)
-#pragma unroll
-			for (int k = 0; k < N; ++k) v[k] += src[k][w];
-#pragma unroll
-		for (int k = 0; k < N; ++k) {
-			v[k] = wave_sum(v[k]);
-			if (threadIdx.x == 0) lds[k] = v[k];
-		}
+// Totals of N <= 4 partial arrays p0..p{N-1}: wavefront k sums array k; every
+// thread gets all totals.  (Separate pointer arguments, not an array of
+// pointers: a runtime-indexed private array would be placed in LDS.)
+template <int N>
+__device__ __forceinline__ void reduce_partials(const double* p0, const double* p1, const double* p2,
+                                                const double* p3, int W, double (&out)[N], double* lds) {
+	static_assert(N <= 4 && N <= kPcgRowsPerWG, "one partial array per wavefront");
+	const int wid = threadIdx.x >> 6;
+	if (wid < N) {
+		const double* p = wid == 0 ? p0 : wid == 1 ? p1 : wid == 2 ? p2 : p3;
+		const double t = wave_sum(partial_lane_sum(p, W));
+		if ((threadIdx.x & 63) == 0) lds[wid] = t;
 	}
 	__syncthreads();
 #pragma unroll
@@ -1007,27 +1078,191 @@ __device__ __forceinline__ void reduce_partials(const T* const (&src)[N], int W,
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_pcg_init(PcgArgs<T> a) {
-	__shared__ T lds[3 * 16];
-	const int n = 6 * a.ctl->nb;
-	T v[2] = {0, 0};
-	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-		const int br = t / 6, i = t - 6 * br;
-		const T bt = a.b[t];
-		a.r[0][t] = bt;
-		a.x[t] = 0;
-		a.p[t] = 0;
-		a.q[t] = 0;
-		T zt = 0;
-		for (int c = 0; c < 6; ++c) zt += a.P[36 * (size_t)br + 6 * i + c] * a.b[6 * br + c];
-		a.z[t] = zt;
-		v[0] += bt * zt;
-		v[1] += bt * bt;
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
+	__shared__ double lds[4 * 16];
+	const int nb = a.ctl->nb;
+	const int lane = threadIdx.x & 63;
+	const int i = lane >> 3, sl = lane & 7;
+	double v[4] = {0, 0, 0, 0};
+	for (int br = blockIdx.x * kPcgRowsPerWG + (threadIdx.x >> 6); br < nb; br += gridDim.x * kPcgRowsPerWG) {
+		const double w0 = spmv_lane(a, a.b, br, i, sl);
+		if (i < 6 && sl == 0) {
+			const int t = 6 * br + i;
+			const double bt = a.b[t];
+			double ut = 0;
+#pragma unroll
+			for (int c = 0; c < 6; ++c) ut += (double)a.P[36 * (size_t)br + 6 * i + c] * (double)a.b[6 * br + c];
+			a.x[t] = 0;
+			a.r[t] = bt;
+			a.u[t] = ut;
+			a.w[0][t] = w0;
+			a.z[t] = 0;
+			a.q[t] = 0;
+			a.s[t] = 0;
+			a.p[t] = 0;
+			v[0] += bt * ut;
+			v[1] += w0 * ut;
+			v[2] += bt * bt;
+		}
 	}
-	T out[2];
-	block_sum<2, T>(v, lds, out);
+	double out[4];
+	block_sum<4, double>(v, lds, out);
 	if (threadIdx.x == 0) {
-		a.rz[0][blockIdx.x] = out[0];
+		a.gam[0][blockIdx.x] = out[0];
+		a.del[0][blockIdx.x] = out[1];
+		a.rr[0][blockIdx.x] = out[2];
+		a.bb[blockIdx.x] = out[2];
+		if (blockIdx.x == 0) {
+			a.ctl->done = 0; a.ctl->iters = 0; a.ctl->itc[0] = 0; a.ctl->itc[1] = 0;
+			// history read by iteration 0: beta = gamma / inf = 0, alpha = gamma / delta
+			a.hist[2] = __builtin_huge_val();
+			a.hist[3] = 1.0;
+		}
+	}
+}
+
+// Operator row and row vectors of block row `br` as read by k_pcg_iter
+// (meaningful in the lanes with i < 6, sl == 0).
+struct PcgRow {
+	double n, m, w, z, q, s, p, u, x, r;
+};
+
+template <class T>
+__device__ __forceinline__ PcgRow pcg_row_load(const PcgArgs<T>& a, const double* __restrict__ wc, int br, int i, int sl) {
+	PcgRow R;
+	R.n = spmv_lane(a, wc, br, i, sl);
+	R.m = R.w = R.z = R.q = R.s = R.p = R.u = R.x = R.r = 0;
+	if (i < 6 && sl == 0) {
+		const int t = 6 * br + i;
+		double m = 0;
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) m += (double)a.P[36 * (size_t)br + 6 * i + cc] * wc[6 * br + cc];
+		R.m = m;
+		R.w = wc[t];
+		R.z = a.z[t]; R.q = a.q[t]; R.s = a.s[t]; R.p = a.p[t];
+		R.u = a.u[t]; R.x = a.x[t]; R.r = a.r[t];
+	}
+	return R;
+}
+
+// Iteration parity equals the launch slot: every chunk has an even number of
+// launches and each active launch advances the counter by one.  The operator
+// row and row vectors of the wavefront's first block row are loaded before the
+// partial sums are reduced, so the two memory round trips overlap.  `nrows`
+// (the scene's body count) bounds those early loads: row pointers, P and the
+// vectors are sized for every body, rows past the solve's block rows are
+// empty.
+template <class T>
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot, int nrows) {
+	__shared__ double lds[4 * 16];
+	static_assert(kPcgRowsPerWG >= 4, "four partial arrays, one per wavefront");
+	const int cur = slot, nxt = slot ^ 1;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const int i = lane >> 3, sl = lane & 7;
+	const PcgCtl c = *a.ctl;
+	// (a runtime index into the private copy would place it in LDS, one slot per thread)
+	const int it = slot ? c.itc[1] : c.itc[0];
+	const double* __restrict__ wc = pick(a.w, cur);
+	double part = 0;
+	if (wid < 4) {
+		const double* src = wid == 0 ? pick(a.gam, cur) : wid == 1 ? pick(a.del, cur) : wid == 2 ? pick(a.rr, cur) : a.bb;
+		part = partial_lane_sum(src, gridDim.x);
+	}
+	const int br0 = blockIdx.x * kPcgRowsPerWG + wid;
+	PcgRow R{};
+	if (br0 < nrows) R = pcg_row_load(a, wc, br0, i, sl);
+	if (wid < 4) {
+		part = wave_sum(part);
+		if (lane == 0) lds[wid] = part;
+	}
+	__syncthreads();
+	if (c.done) return;
+	const double gamma = lds[0], delta = lds[1], rr = lds[2], bb = lds[3];
+	if (!(rr > c.tol2 * bb)) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) { a.ctl->done = 1; a.ctl->iters = it; }
+		return;
+	}
+	const double gprev = a.hist[2 * nxt], aprev = a.hist[2 * nxt + 1];
+	const double beta = gamma / gprev;
+	const double alpha = gamma / (delta - beta * gamma / aprev);
+	double* __restrict__ wn = pick(a.w, nxt);
+	double v[4] = {0, 0, 0, 0};
+	for (int br = br0; br < c.nb; br += gridDim.x * kPcgRowsPerWG) {
+		if (br != br0) R = pcg_row_load(a, wc, br, i, sl);
+		if (i < 6 && sl == 0) {
+			const int t = 6 * br + i;
+			const double zt = R.n + beta * R.z;
+			const double qt = R.m + beta * R.q;
+			const double st = R.w + beta * R.s;
+			const double pt = R.u + beta * R.p;
+			a.z[t] = zt;
+			a.q[t] = qt;
+			a.s[t] = st;
+			a.p[t] = pt;
+			a.x[t] = R.x + alpha * pt;
+			const double rt = R.r - alpha * st;
+			const double ut = R.u - alpha * qt;
+			const double wnt = R.w - alpha * zt;
+			a.r[t] = rt;
+			a.u[t] = ut;
+			wn[t] = wnt;
+			v[0] += rt * ut;
+			v[1] += wnt * ut;
+			v[2] += rt * rt;
+		}
+	}
+	double out[4];
+	block_sum<4, double>(v, lds, out);
+	if (threadIdx.x == 0) {
+		pick(a.gam, nxt)[blockIdx.x] = out[0];
+		pick(a.del, nxt)[blockIdx.x] = out[1];
+		pick(a.rr, nxt)[blockIdx.x] = out[2];
+		if (blockIdx.x == 0) {
+			a.hist[2 * cur] = gamma;
+			a.hist[2 * cur + 1] = alpha;
+			a.ctl->itc[slot ^ 1] = it + 1;
+		}
+	}
+}
+
+// Two-reduction form (standard PCG), used for tolerances below the attainable
+// accuracy of the single-reduction recurrences.  It runs on the same stored
+// operator B = A M through y with p = M y, so that A p = B y:
+//   init: r = b, x = y = q = 0; gamma = (r, M r), |r|^2
+//   A(i): stop when |r|^2 <= tol^2 |b|^2; beta = gamma / gamma_prev (0 at i = 0)
+//         y = r + beta y, q = B r + beta q (= A p), partial (M y, q)
+//   B(i): alpha = gamma / (p, q); x += alpha M y, r -= alpha q,
+//         partials of (r, M r) and |r|^2 for iteration i + 1
+// gamma partials are double-buffered by parity (slot i & 1 holds gamma_i, the
+// other slot gamma_{i-1} until B(i) overwrites it).  The six lanes of a block
+// row read the whole 6-vector of their block before any of them writes, which
+// a single wavefront executes in program order.
+template <class T>
+__global__ __launch_bounds__(kPcgThreads) void k_pcg2_init(PcgArgs<T> a) {
+	__shared__ double lds[4 * 16];
+	const int nb = a.ctl->nb;
+	const int lane = threadIdx.x & 63;
+	const int i = lane >> 3, sl = lane & 7;
+	double v[2] = {0, 0};
+	for (int br = blockIdx.x * kPcgRowsPerWG + (threadIdx.x >> 6); br < nb; br += gridDim.x * kPcgRowsPerWG) {
+		if (i < 6 && sl == 0) {
+			const int t = 6 * br + i;
+			double zt = 0;
+#pragma unroll
+			for (int c = 0; c < 6; ++c) zt += (double)a.P[36 * (size_t)br + 6 * i + c] * (double)a.b[6 * br + c];
+			const double bt = a.b[t];
+			a.x[t] = 0;
+			a.r[t] = bt;
+			a.p[t] = 0;  // y
+			a.q[t] = 0;
+			v[0] += bt * zt;
+			v[1] += bt * bt;
+		}
+	}
+	double out[2];
+	block_sum<2, double>(v, lds, out);
+	if (threadIdx.x == 0) {
+		a.gam[0][blockIdx.x] = out[0];
 		a.rr[0][blockIdx.x] = out[1];
 		a.bb[blockIdx.x] = out[1];
 		if (blockIdx.x == 0) { a.ctl->done = 0; a.ctl->iters = 0; a.ctl->itc[0] = 0; a.ctl->itc[1] = 0; }
@@ -1035,78 +1270,95 @@ __global__ __launch_bounds__(256) void k_pcg_init(PcgArgs<T> a) {
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_pcg_a(PcgArgs<T> a, int slot) {
-	__shared__ T lds[16];
+__global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) {
+	__shared__ double lds[4 * 16];
 	const PcgCtl c = *a.ctl;
 	if (c.done) return;
-	const int it = c.itc[slot];
-	const int W = gridDim.x;
-	const T* src[4] = {a.rr[it & 1], a.bb, a.rz[it & 1], a.rz[(it + 1) & 1]};
-	T red[4];
-	reduce_partials<4, T>(src, W, red, lds);
-	const T rr = red[0], bb = red[1], rz = red[2];
-	if (!(rr > (T)c.tol2 * bb)) {
+	const int it = slot ? c.itc[1] : c.itc[0];  // no dynamic index into a private copy
+	const int cur = it & 1, nxt = cur ^ 1;
+	double red[4];
+	reduce_partials<4>(pick(a.rr, cur), a.bb, pick(a.gam, cur), pick(a.gam, nxt), gridDim.x, red, lds);
+	if (!(red[0] > c.tol2 * red[1])) {
 		if (blockIdx.x == 0 && threadIdx.x == 0) { a.ctl->done = 1; a.ctl->iters = it; }
 		return;
 	}
-	// rz[(it+1)&1] holds rz_{it-1} until B(it) overwrites it
-	const T beta = it > 0 ? rz / red[3] : T(0);
-	const int n = 6 * c.nb;
-	T v[1] = {0};
-	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-		const T pt = a.z[t] + beta * a.p[t];
-		const T qt = spmv_row(a, a.z, t) + beta * a.q[t];
-		a.p[t] = pt;
-		a.q[t] = qt;
-		v[0] += pt * qt;
+	const double beta = it > 0 ? red[2] / red[3] : 0.0;
+	double* __restrict__ y = a.p;
+	const int lane = threadIdx.x & 63;
+	const int i = lane >> 3, sl = lane & 7;
+	double v[1] = {0};
+	for (int br = blockIdx.x * kPcgRowsPerWG + (threadIdx.x >> 6); br < c.nb; br += gridDim.x * kPcgRowsPerWG) {
+		const double n = spmv_lane(a, a.r, br, i, sl);
+		if (i < 6 && sl == 0) {
+			const int t = 6 * br + i;
+			double yb[6];
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) yb[cc] = a.r[6 * br + cc] + beta * y[6 * br + cc];
+			double pt = 0;
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) pt += (double)a.P[36 * (size_t)br + 6 * i + cc] * yb[cc];
+			double yt = yb[0];
+#pragma unroll
+			for (int cc = 1; cc < 6; ++cc) yt = cc == i ? yb[cc] : yt;
+			const double qt = n + beta * a.q[t];
+			y[t] = yt;
+			a.q[t] = qt;
+			v[0] += pt * qt;
+		}
 	}
-	__syncthreads();
-	T out[1];
-	block_sum<1, T>(v, lds, out);
-	if (threadIdx.x == 0) a.pq[blockIdx.x] = out[0];
+	double out[1];
+	block_sum<1, double>(v, lds, out);
+	if (threadIdx.x == 0) a.del[0][blockIdx.x] = out[0];
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_pcg_b(PcgArgs<T> a, int slot) {
-	__shared__ T lds[2 * 16];
+__global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) {
+	__shared__ double lds[4 * 16];
 	const PcgCtl c = *a.ctl;
 	if (c.done) return;
-	const int it = c.itc[slot];
-	const int W = gridDim.x;
-	const T* src[2] = {a.rz[it & 1], a.pq};
-	T red[2];
-	reduce_partials<2, T>(src, W, red, lds);
-	const T alpha = red[0] / red[1];
-	const T* rc = a.r[it & 1];
-	T* rn = a.r[(it + 1) & 1];
-	const int n = 6 * c.nb;
-	T v[2] = {0, 0};
-	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-		const int br = t / 6, i = t - 6 * br;
-		a.x[t] += alpha * a.p[t];
-		T rv[6];
-		for (int c = 0; c < 6; ++c) rv[c] = rc[6 * br + c] - alpha * a.q[6 * br + c];
-		rn[t] = rv[i];
-		T zt = 0;
-		for (int c = 0; c < 6; ++c) zt += a.P[36 * (size_t)br + 6 * i + c] * rv[c];
-		a.z[t] = zt;
-		v[0] += rv[i] * zt;
-		v[1] += rv[i] * rv[i];
+	const int it = slot ? c.itc[1] : c.itc[0];  // no dynamic index into a private copy
+	const int cur = it & 1, nxt = cur ^ 1;
+	double red[2];
+	reduce_partials<2>(pick(a.gam, cur), a.del[0], nullptr, nullptr, gridDim.x, red, lds);
+	const double alpha = red[0] / red[1];
+	const double* __restrict__ y = a.p;
+	const int lane = threadIdx.x & 63;
+	const int i = lane >> 3, sl = lane & 7;
+	double v[2] = {0, 0};
+	for (int br = blockIdx.x * kPcgRowsPerWG + (threadIdx.x >> 6); br < c.nb; br += gridDim.x * kPcgRowsPerWG) {
+		if (i < 6 && sl == 0) {
+			const int t = 6 * br + i;
+			double rb[6], pt = 0;
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) {
+				const double pc = (double)a.P[36 * (size_t)br + 6 * i + cc];
+				pt += pc * y[6 * br + cc];
+				rb[cc] = a.r[6 * br + cc] - alpha * a.q[6 * br + cc];
+			}
+			double zt = 0;
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) zt += (double)a.P[36 * (size_t)br + 6 * i + cc] * rb[cc];
+			double rt = rb[0];
+#pragma unroll
+			for (int cc = 1; cc < 6; ++cc) rt = cc == i ? rb[cc] : rt;
+			a.x[t] += alpha * pt;
+			a.r[t] = rt;
+			v[0] += rt * zt;
+			v[1] += rt * rt;
+		}
 	}
-	__syncthreads();
-	T out[2];
-	block_sum<2, T>(v, lds, out);
+	double out[2];
+	block_sum<2, double>(v, lds, out);
 	if (threadIdx.x == 0) {
-		a.rz[(it + 1) & 1][blockIdx.x] = out[0];
-		a.rr[(it + 1) & 1][blockIdx.x] = out[1];
+		pick(a.gam, nxt)[blockIdx.x] = out[0];
+		pick(a.rr, nxt)[blockIdx.x] = out[1];
 		if (blockIdx.x == 0) a.ctl->itc[slot ^ 1] = it + 1;
 	}
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void k_du(int NL, const double* __restrict__ D, const T* __restrict__ x, double* __restrict__ du) {
+__global__ __launch_bounds__(256) void k_du(int NL, const double* __restrict__ D, const double* __restrict__ x, double* __restrict__ du) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t < 6 * NL) du[t] = (double)x[t] * D[t];
+	if (t < 6 * NL) du[t] = x[t] * D[t];
 }
 
 // ---- increments (NonSmoothContactForce.cpp:254-280) -------------------------
@@ -1450,65 +1702,89 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 // the counter slots 0, 1).
 constexpr int kPcgChunk = 16;
 
+static_assert(kPcgChunk % 2 == 0, "iteration parity is the launch slot");
+
 template <class T>
-static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk, hipEvent_t* ev) {
+static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk, bool two, hipEvent_t* ev) {
 	hipStream_t s = ctx->stream;
+	const dim3 B(kPcgThreads);
 	for (int k = 0; k < kPcgChunk; ++k) {
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k], s));
-		hipLaunchKernelGGL(k_pcg_a<T>, dim3(nblk), dim3(256), 0, s, pa, k & 1);
+		if (two) hipLaunchKernelGGL(k_pcg2_a<T>, dim3(nblk), B, 0, s, pa, k & 1);
+		else hipLaunchKernelGGL(k_pcg_iter<T>, dim3(nblk), B, 0, s, pa, k & 1, ctx->N);
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k + 1], s));
-		hipLaunchKernelGGL(k_pcg_b<T>, dim3(nblk), dim3(256), 0, s, pa, k & 1);
+		if (two) hipLaunchKernelGGL(k_pcg2_b<T>, dim3(nblk), B, 0, s, pa, k & 1);
 	}
 	PDC_CHECK_LAUNCH();
 }
 
+__global__ void k_ptr_pad(int* __restrict__ ptr, int NL, int N) {
+	const int r = NL + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+	if (r <= N) ptr[r] = ptr[NL];
+}
+
+// Below this relative tolerance the two-reduction form is used: the
+// single-reduction recurrences stall near 1e-10 on ill-conditioned systems.
+constexpr double kSingleReductionMinTol = 1e-9;
+
 template <class T>
-static int run_pcg(pdc_context* ctx, DBuf<T>& A, DBuf<T>& P, DBuf<T>& b, DBuf<T>& x, DBuf<T>& work) {
+static int run_pcg(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b) {
 	const int NL = ctx->NL;
 	const int n = 6 * NL;
-	// grid fixed per scene (rows of all bodies) so the recorded chunk stays valid
-	const int nblk = std::max(1, std::min(grid_for(6 * (int64_t)std::max(ctx->N, 1), 256), 2048));
-	const size_t ncap = 6 * (size_t)std::max(ctx->N, 1);
-	work.ensure(5 * ncap + 6 * 2048 + 64);
+	// grid fixed per scene (block rows of all bodies) so the recorded chunk stays valid
+	const int rows = std::max(ctx->N, 1);
+	const int nblk = std::max(1, std::min((rows + kPcgRowsPerWG - 1) / kPcgRowsPerWG, kMaxWG));
+	const size_t ncap = 6 * (size_t)rows;
+	DBuf<double>& work = ctx->pcg64;
+	work.ensure(10 * ncap + 7 * (size_t)kMaxWG + 64);
+	ctx->x64.ensure(ncap);
 	PcgArgs<T> pa;
 	pa.ptr = ctx->a_ptr.p;
 	pa.col = ctx->a_col.p;
-	pa.A = A.p;
+	pa.B = Bm.p;
 	pa.P = P.p;
 	pa.b = b.p;
-	pa.x = x.p;
-	T* w = work.p;
-	pa.r[0] = w; pa.r[1] = w + ncap; pa.z = w + 2 * ncap; pa.p = w + 3 * ncap; pa.q = w + 4 * ncap;
-	T* parts = w + 5 * ncap;
-	pa.rz[0] = parts; pa.rz[1] = parts + 2048; pa.rr[0] = parts + 2 * 2048; pa.rr[1] = parts + 3 * 2048;
-	pa.pq = parts + 4 * 2048; pa.bb = parts + 5 * 2048;
+	pa.x = ctx->x64.p;
+	double* w = work.p;
+	pa.r = w; pa.u = w + ncap; pa.w[0] = w + 2 * ncap; pa.w[1] = w + 3 * ncap;
+	pa.z = w + 4 * ncap; pa.q = w + 5 * ncap; pa.s = w + 6 * ncap; pa.p = w + 7 * ncap;
+	double* parts = w + 10 * ncap;
+	pa.gam[0] = parts; pa.gam[1] = parts + kMaxWG;
+	pa.del[0] = parts + 2 * kMaxWG; pa.del[1] = parts + 3 * kMaxWG;
+	pa.rr[0] = parts + 4 * kMaxWG; pa.rr[1] = parts + 5 * kMaxWG;
+	pa.bb = parts + 6 * kMaxWG;
+	pa.hist = parts + 7 * kMaxWG;
 	pa.ctl = &ctx->d_sc.p->pcg;
 	hipStream_t s = ctx->stream;
 	PcgCtl* hc = &ctx->h_sc->pcg;
 	hc->nb = NL;
 	hc->tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
 	PDC_HIP(hipMemcpyAsync(pa.ctl, hc, sizeof(PcgCtl), hipMemcpyHostToDevice, s));
-	hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(256), 0, s, pa);
+	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
+	// rows NL..N of the row pointers: empty rows (read by the early loads)
+	if (ctx->N > NL)
+		hipLaunchKernelGGL(k_ptr_pad, dim3(grid_for(ctx->N - NL)), dim3(kBlock), 0, s, ctx->a_ptr.p, NL, ctx->N);
+	if (two) hipLaunchKernelGGL(k_pcg2_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
+	else hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
 	PDC_CHECK_LAUNCH();
 	// the chunk is recorded again when a buffer moved or the grid changed
-	PcgGraph& G = ctx->pcg_graph[sizeof(T) == 8 ? 1 : 0];
+	PcgGraph& G = ctx->pcg_graph[(sizeof(T) == 8 ? 1 : 0) + (two ? 2 : 0)];
 	if (ctx->use_graphs && (G.exec == nullptr || G.nblk != nblk ||
 	                        std::memcmp(G.args, &pa, sizeof(pa)) != 0)) {
 		if (G.exec) { (void)hipGraphExecDestroy(G.exec); G.exec = nullptr; }
 		if (G.graph) { (void)hipGraphDestroy(G.graph); G.graph = nullptr; }
 		PDC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-		pcg_chunk_launches<T>(ctx, pa, nblk, nullptr);
+		pcg_chunk_launches<T>(ctx, pa, nblk, two, nullptr);
 		PDC_HIP(hipStreamEndCapture(s, &G.graph));
 		PDC_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
 		G.nblk = nblk;
 		std::memcpy(G.args, &pa, sizeof(pa));
 	}
-	// Kernel timing.  SPMV: k_pcg_a alone, launched directly between an event
-	// pair per launch (events cannot be timed inside a replayed graph); only
-	// launches of active iterations are counted.  PCG: the whole iteration
-	// (k_pcg_a + k_pcg_b) from an event pair around every group of chunks,
-	// charged to the active iterations (the idle launches after convergence
-	// are included in the time).
+	// Kernel timing of k_pcg_iter.  SPMV: launched directly between an event
+	// pair per launch (events cannot be timed inside a replayed graph), only
+	// launches of active iterations counted.  PCG: an event pair around every
+	// group of replayed chunks, every launch counted (the launches after
+	// convergence, which return at once, included).
 	const bool kt_spmv = ctx->kt.which == PDC_KT_SPMV;
 	const bool kt_iter = ctx->kt.which == PDC_KT_PCG;
 	const bool graph = ctx->use_graphs && !kt_spmv;
@@ -1524,7 +1800,7 @@ static int run_pcg(pdc_context* ctx, DBuf<T>& A, DBuf<T>& P, DBuf<T>& b, DBuf<T>
 		const int before = launched;
 		for (int c = 0; c < want && launched < maxit; ++c) {
 			if (graph) PDC_HIP(hipGraphLaunch(G.exec, s));
-			else pcg_chunk_launches<T>(ctx, pa, nblk, kt_spmv ? ev : nullptr);
+			else pcg_chunk_launches<T>(ctx, pa, nblk, two, kt_spmv ? ev : nullptr);
 			launched += kPcgChunk;
 		}
 		if (kt_iter) PDC_HIP(hipEventRecord(ev[1], s));
@@ -1534,6 +1810,7 @@ static int run_pcg(pdc_context* ctx, DBuf<T>& A, DBuf<T>& P, DBuf<T>& b, DBuf<T>
 			float e = 0;
 			PDC_HIP(hipEventElapsedTime(&e, ev[0], ev[1]));
 			kt_ms += e;
+			ctx->kt.launches += launched - before;
 		} else if (kt_spmv) {
 			// iterations k < iters reduced and updated; iteration `iters` ran
 			// its SpMV launch only to the convergence test
@@ -1550,23 +1827,21 @@ static int run_pcg(pdc_context* ctx, DBuf<T>& A, DBuf<T>& P, DBuf<T>& b, DBuf<T>
 	}
 	const int iters = flags[1] ? flags[0] : maxit;
 	if (kt_spmv || kt_iter) {
-		// Algorithmic bytes, each input read once and each output written once:
-		//   k_pcg_a  A blocks and column indices nnzb (36 T + 4), row pointers
-		//            4 (NL + 1), z read T n6, p and q read and written 4 T n6
-		//   k_pcg_b  x read and written 2 T n6, p, r, q read 3 T n6, r' and z
-		//            written 2 T n6; block-Jacobi blocks 36 T NL
+		// Algorithmic bytes of one active iteration (of k_pcg_iter, or of
+		// k_pcg2_a for the two-reduction form), each input read once and each
+		// output written once, FP64 vectors:
+		//   k_pcg_iter  B blocks and column indices nnzb (36 T + 4), row pointers
+		//               4 (NL + 1), P blocks 36 T NL, w, z, q, s, p, u, x, r read
+		//               8 x 8 n6, z, q, s, p, x, r, u, w' written 8 x 8 n6
+		//   k_pcg2_a    B, column indices, row pointers as above, P 36 T NL,
+		//               r, y, q read 3 x 8 n6, y, q written 2 x 8 n6
 		const double Tb = sizeof(T), n6 = 6.0 * NL, nnzb = ctx->NNZB;
-		const double spmv = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NL + 1) + 5.0 * Tb * n6;
+		const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NL + 1) + 36.0 * Tb * NL;
 		ctx->kt.total_ms += kt_ms;
-		if (kt_iter) {
-			ctx->kt.launches += iters;
-			ctx->kt.bytes += iters * (spmv + 7.0 * Tb * n6 + 36.0 * Tb * NL);
-		} else {
-			ctx->kt.bytes += iters * spmv;
-		}
+		ctx->kt.bytes += iters * (mat + (two ? 40.0 : 128.0) * n6);
 	}
 	hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(64), 0, s, pa.ctl, maxit);
-	hipLaunchKernelGGL(k_du<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, NL, ctx->l_D.p, x.p, ctx->l_du.p);
+	hipLaunchKernelGGL(k_du, dim3(grid_for(n)), dim3(kBlock), 0, s, NL, ctx->l_D.p, ctx->x64.p, ctx->l_du.p);
 	PDC_CHECK_LAUNCH();
 	return nblk;
 }
@@ -1613,9 +1888,9 @@ void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st) {
 	const bool fp32 = ctx->p.pcg_precision == PDC_PCG_FP32;
 	const size_t nnzb = (size_t)ctx->NNZB;
 	if (fp32) {
-		ctx->A32.ensure(36 * nnzb); ctx->P32.ensure(36 * nb); ctx->b32.ensure(6 * nb); ctx->x32.ensure(6 * nb);
+		ctx->A32.ensure(36 * nnzb); ctx->P32.ensure(36 * (size_t)ctx->N); ctx->b32.ensure(6 * (size_t)ctx->N);
 	} else {
-		ctx->A64.ensure(36 * nnzb); ctx->P64.ensure(36 * nb); ctx->b64.ensure(6 * nb); ctx->x64.ensure(6 * nb);
+		ctx->A64.ensure(36 * nnzb); ctx->P64.ensure(36 * (size_t)ctx->N); ctx->b64.ensure(6 * (size_t)ctx->N);
 	}
 	SArgs a = make_args(ctx, dt, g);
 	const int wc = wg_for(C), wb = wg_for(NL);
@@ -1668,18 +1943,18 @@ void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st) {
 			hipLaunchKernelGGL(k_asm_diag<float>, dim3(wv), B, 0, s, a, ctx->a_diag.p, ctx->A32.p,
 			                   ctx->P32.p, ctx->b32.p);
 			PDC_CHECK_LAUNCH();
-			hipLaunchKernelGGL(k_asm_off<float>, dim3(wp), B, 0, s, a, ctx->NP, ctx->A32.p);
+			hipLaunchKernelGGL(k_asm_off<float>, dim3(wp), B, 0, s, a, ctx->NP, ctx->P32.p, ctx->A32.p);
 		} else {
 			hipLaunchKernelGGL(k_asm_diag<double>, dim3(wv), B, 0, s, a, ctx->a_diag.p, ctx->A64.p,
 			                   ctx->P64.p, ctx->b64.p);
 			PDC_CHECK_LAUNCH();
-			hipLaunchKernelGGL(k_asm_off<double>, dim3(wp), B, 0, s, a, ctx->NP, ctx->A64.p);
+			hipLaunchKernelGGL(k_asm_off<double>, dim3(wp), B, 0, s, a, ctx->NP, ctx->P64.p, ctx->A64.p);
 		}
 		PDC_CHECK_LAUNCH();
 		kt_end(ctx, PDC_KT_ASSEMBLY, e0);
 		// solve
-		if (fp32) run_pcg<float>(ctx, ctx->A32, ctx->P32, ctx->b32, ctx->x32, ctx->pcg32);
-		else run_pcg<double>(ctx, ctx->A64, ctx->P64, ctx->b64, ctx->x64, ctx->pcg64);
+		if (fp32) run_pcg<float>(ctx, ctx->A32, ctx->P32, ctx->b32);
+		else run_pcg<double>(ctx, ctx->A64, ctx->P64, ctx->b64);
 		hipLaunchKernelGGL(k_increments, dim3(wc), B, 0, s, a);
 		PDC_CHECK_LAUNCH();
 		sync_scalars(ctx);

@@ -134,14 +134,17 @@ def _fill_parity(sc, steps_before, params_gpu, tol_rel=1e-6, forces_unique=False
     print("velocity-increment err %.3e of |g|dt; local-force err %.3e of max" % (err_a, err_f))
     assert st_g.char_mass == pytest.approx(st_o.char_mass, rel=1e-13)
     assert st_g.char_speed == pytest.approx(st_o.char_speed, rel=1e-12)
+    assert err_a < tol_rel
     if st_o.exit_error < params_gpu.tolerance:
         assert st_g.exit_error < params_gpu.tolerance
     else:
         # the reference algorithm stalls on this input (iteration cap reached):
-        # the device must stall the same way
+        # the device must stall the same way, at the same error level.  The
+        # error at the cap of a stalled run depends on the rounding of every
+        # inner solve (iterative on the device, direct on the oracle), so only
+        # its level is compared.
         assert st_g.newton_iterations == st_o.newton_iterations
-        assert st_g.exit_error == pytest.approx(st_o.exit_error, rel=1e-3)
-    assert err_a < tol_rel
+        assert st_g.exit_error == pytest.approx(st_o.exit_error, rel=2e-2)
     if forces_unique:
         assert err_f < tol_rel
     return err_a, err_f
