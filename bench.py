@@ -59,9 +59,8 @@ def settled_state(sc, name, settle_steps=None, path=None):
 
 def cpu_baseline(sc, state, budget_s, threads):
     from oracle import orc
-    p = sc.params
-    p.pcg_tol = 1e-8   # AMGCL's default relative tolerance on the reference's PCG branch
-    o = orc.OracleScene(sc, p)
+    # same parameters as the device run (tolerances, PCG branch and its tolerance), so both do the same work
+    o = orc.OracleScene(sc, sc.params)
     o.set_threads(threads)
     o.set_state(*state)
     t0 = time.perf_counter()
@@ -80,9 +79,22 @@ def cpu_baseline(sc, state, budget_s, threads):
         "unit": "timesteps/s",
         "cores": threads,
         "kind": "port",
-        "sample": "%d timestep(s) of the FP64 CPU restatement from the same settled state "
-                  "(%d contacts/step avg; block-Jacobi PCG branch, tol 1e-8)" % (n, contacts // n),
+        "sample": "%d timestep(s) of the FP64 CPU restatement (OpenMP) from the same settled state, same "
+                  "parameters (%d contacts/step avg; block-Jacobi PCG branch, pcg_tol %g)" % (
+                      n, contacts // n, sc.params.pcg_tol),
     }
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of the timed kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (profiles/<kernel>_<workload>_traffic.json), or None when there is none."""
+    if kernel != "pcg":
+        return None
+    path = os.path.join(ROOT, "profiles", "pcg_iter_%s_traffic.json" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return float(json.load(f)["traffic_bytes_per_launch"])
 
 
 def main():
@@ -158,7 +170,7 @@ def main():
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM,
-            "traffic": None,
+            "traffic": measured_traffic(args.kernel, args.workload),
             "launches": launches,
             "avg_launch_us": 1e3 * ms / max(launches, 1),
             "algorithmic_bytes_per_launch": nbytes / max(launches, 1),

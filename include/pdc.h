@@ -46,8 +46,10 @@ enum { PDC_SHAPE_BOX = 0, PDC_SHAPE_SPHERE = 1 };
  * (PrimalDualForceSolver.cpp:664-673) */
 enum { PDC_NON_SMOOTH_CONTACT = 0, PDC_ONLY_NORMAL = 1 };
 
-/* Arithmetic of the inner PCG solve.  The Newton outer loop, residuals,
- * barrier terms, centering parameter and line-search merit are FP64 always. */
+/* Storage of the inner PCG's operator: FP32 6x6 blocks (PDC_PCG_FP32) or FP64
+ * blocks.  The PCG vectors, dot products and scalars are FP64 on both, and the
+ * Newton outer loop, residuals, barrier terms, centering parameter and
+ * line-search merit are FP64 always. */
 enum { PDC_PCG_FP64 = 0, PDC_PCG_FP32 = 1 };
 
 /* ---- parameters ---------------------------------------------------------- */
@@ -69,7 +71,9 @@ typedef struct pdc_params {
 	double tolerance;           /* Newton exit tolerance on exitErr */
 	int32_t iterations;         /* Newton iteration cap */
 	int32_t pcg_precision;      /* PDC_PCG_FP64 | PDC_PCG_FP32 */
-	double pcg_tol;             /* relative residual |r|/|b| of the scaled system */
+	double pcg_tol;             /* relative residual |r|/|b| of the scaled system; at or above
+	                               1e-9 the single-reduction PCG form is used (one launch per
+	                               iteration), below it the two-reduction form */
 	int32_t pcg_max_iter;       /* the reference's maxCGIt = 1000 (PrimalDualForceSolver.cpp:381) */
 	int32_t grid_n;             /* SDF grid nodes per axis (64, 96, ...) */
 	double grid_pad_frac;       /* grid box = shape half-extent * (1 + grid_pad_frac) */

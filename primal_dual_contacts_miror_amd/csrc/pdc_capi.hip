@@ -168,9 +168,6 @@ int pdc_create(int device, const pdc_params* params, pdc_context** out) {
 		hipDeviceProp_t prop;
 		PDC_HIP(hipGetDeviceProperties(&prop, device));
 		ctx->cus = prop.multiProcessorCount;
-		int coop = 0;
-		PDC_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
-		if (!coop) throw HipError("device does not support cooperative launches", PDC_ERR_NO_DEVICE);
 		PDC_HIP(hipStreamSynchronize(ctx->stream));
 	});
 	if (rc != PDC_OK) {

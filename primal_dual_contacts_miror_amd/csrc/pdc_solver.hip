@@ -4,14 +4,13 @@
 // src/Solver/PrimalDual/PrimalDualForceSolver.cpp:167-561) with the
 // NonSmoothContactForce / OnlyNormalForce plugins
 // (src/Solver/PrimalDual/NonSmoothForces/*.cpp).  Everything is FP64 except the
-// inner PCG, which runs in FP32 or FP64 (pdc_params.pcg_precision).
+// inner PCG's stored operator, which is FP32 or FP64 (pdc_params.pcg_precision).
 //
 // Layout: bodies in the local solve set are numbered by ascending global id;
 // per-body vectors are AoS (6 doubles per body), per-contact arrays are SoA
 // with row stride ctx->c_stride.  Contact-to-body sums are gathers over a
 // per-body pair adjacency (each pair's contacts are contiguous), so no
 // floating-point atomics are used and every sum has a fixed order.
-#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 #include <cmath>
@@ -20,7 +19,6 @@
 #include "pdc_internal.h"
 #include "pdc_reduce.h"
 
-namespace cg = cooperative_groups;
 
 namespace pdc {
 

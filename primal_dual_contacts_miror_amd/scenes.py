@@ -210,7 +210,7 @@ def pile(n_bodies=2000, seed=0, half=10.0, height=40.0, sphere_r=(0.2, 0.5),
             b.box(tuple(0.5 * v), tuple(p), quat=_random_quat(rng))
     # tolerance 1e-4 / 50 iterations: the solver settings of the reference's own
     # pile scenes (scenes/funnel/funnel.yaml:10-11, scenes/hourglass/hourglass_1k7grains.yaml:11-12)
-    p = default_params(timestep=0.005, friction_coeff=0.6, grid_n=96, sample_spacing=0.055,
+    p = default_params(timestep=0.005, friction_coeff=0.6, grid_n=96, sample_spacing=0.05,
                        tolerance=1e-4, iterations=50, pcg_precision=PCG_FP32, pcg_tol=1e-5)
     for k, v in param_overrides.items():
         setattr(p, k, v)
