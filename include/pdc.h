@@ -182,8 +182,32 @@ int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts,
 /* ---- whole timestep on the device-resident state ---------------------------
  * DynamicSystem::step without adaptive substeps (DynamicSystem.cpp:130-161,
  * 207-210): contact detection, fillForces, then semi-implicit integration.
- * `stats` (may be NULL) receives the statistics of the last step. */
+ * `stats` (may be NULL) receives the statistics of the last step (summed over
+ * scenes when the bodies are partitioned into several). */
 int pdc_step(pdc_context* ctx, int32_t n_steps, pdc_solve_stats* stats);
+
+/* ---- independent scenes: batched and served stepping ----------------------
+ * No counterpart in the reference, which simulates one scene per process
+ * (DynamicSystem.cpp:130-218).  The uploaded body table is partitioned into
+ * n_scenes independent scenes of consecutive bodies: scene k holds bodies
+ * [body_begin[k], body_begin[k+1]), body_begin[0] = 0, body_begin[n] = N.
+ * Bodies of different scenes never touch.  Every scene runs the same
+ * per-timestep pipeline as a single scene, with its own Newton iteration,
+ * PCG and line search; all scenes advance in the same kernel launches, and a
+ * scene's result does not depend on which other scenes are stepped with it.
+ * pdc_upload_bodies resets the partition to one scene. */
+int pdc_set_scenes(pdc_context* ctx, int32_t n_scenes, const int32_t* body_begin);
+int pdc_scene_count(const pdc_context* ctx);
+/* Per-scene friction coefficient (n_scenes entries); pdc_configure with a new
+ * friction_coeff resets every scene to it. */
+int pdc_set_scene_friction(pdc_context* ctx, const double* mu);
+/* n_steps timesteps of the selected scenes (all scenes when n_selected is 0);
+ * the other scenes keep their state.  stats (may be NULL) receives each
+ * selected scene's statistics of the last step (n_scenes entries when
+ * n_selected is 0).  pdc_step is pdc_step_scenes over all scenes, its stats
+ * summed over scenes (Newton iterations and exit error: the maximum). */
+int pdc_step_scenes(pdc_context* ctx, int32_t n_steps, int32_t n_selected,
+                    const int32_t* scene_ids, pdc_solve_stats* stats);
 
 int pdc_get_timings(const pdc_context* ctx, pdc_timings* out);
 void pdc_reset_timings(pdc_context* ctx);

@@ -105,6 +105,11 @@ void free_all(pdc_context* c) {
 	c->A32.release(); c->P32.release(); c->b32.release();
 	c->A64.release(); c->P64.release(); c->b64.release(); c->x64.release(); c->pcg64.release();
 	c->d_sc.release();
+	c->b_scene.release(); c->d_sbeg.release(); c->d_step.release();
+	c->r_cbeg.release(); c->r_lbeg.release(); c->r_pbeg.release(); c->r_nnzb.release();
+	c->t_c.release(); c->t_b.release(); c->t_w.release(); c->t_p.release(); c->t_r.release();
+	c->tb_c.release(); c->tb_b.release(); c->tb_w.release(); c->tb_p.release(); c->tb_r.release();
+	c->d_ss.release(); c->d_sctl.release(); c->d_ps.release();
 	c->c_stride = 0;
 }
 
@@ -190,6 +195,7 @@ void pdc_destroy(pdc_context* ctx) {
 	free_all(ctx);
 	for (auto e : ctx->kt.pool) (void)hipEventDestroy(e);
 	if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
+	if (ctx->h_ss) (void)hipHostFree(ctx->h_ss);
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 	delete ctx;
 }
@@ -202,6 +208,7 @@ int pdc_configure(pdc_context* ctx, const pdc_params* params) {
 		                                   params->sample_spacing != ctx->p.sample_spacing);
 		if (regrid)
 			throw ConfigError("grid_n, grid_pad_frac and sample_spacing are fixed once bodies are uploaded");
+		if (params->friction_coeff != ctx->p.friction_coeff) ctx->h_mu.assign(ctx->S, params->friction_coeff);
 		ctx->p = *params;
 	});
 }
@@ -270,6 +277,7 @@ int pdc_get_state(pdc_context* ctx, double* pos, double* quat_xyzw, double* vel,
 int pdc_detect_contacts(pdc_context* ctx, int32_t* n_contacts, int32_t* n_ties) {
 	return guard(ctx, [&] {
 		require_bodies(ctx);
+		if (std::count(ctx->h_step.begin(), ctx->h_step.end(), 0) > 0) pdc::set_stepped(ctx, 0, nullptr);
 		const double t0 = now_s();
 		pdc::detect_contacts(ctx);
 		PDC_HIP(hipStreamSynchronize(ctx->stream));
@@ -328,6 +336,14 @@ int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids
 		for (int i = 0; i < C; ++i)
 			if (obj_ids[2 * i] == obj_ids[2 * i + 1])
 				throw std::invalid_argument("pdc_fill_forces: contact between a body and itself");
+		if (ctx->S > 1) {
+			auto scene_of = [&](int b) {
+				return (int)(std::upper_bound(ctx->h_sbeg.begin(), ctx->h_sbeg.end(), b) - ctx->h_sbeg.begin()) - 1;
+			};
+			for (int i = 0; i < C; ++i)
+				if (scene_of(obj_ids[2 * i]) != scene_of(obj_ids[2 * i + 1]))
+					throw std::invalid_argument("pdc_fill_forces: contact between bodies of different scenes");
+		}
 		const double t0 = now_s();
 		hipStream_t s = ctx->stream;
 		PDC_HIP(hipMemcpyAsync(ctx->b_quat.p, quat_xyzw, 4 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
@@ -364,8 +380,8 @@ int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids
 			PDC_HIP(hipStreamSynchronize(s));
 			ctx->C = C;
 		}
-		pdc_solve_stats st{};
-		pdc::solve(ctx, ctx->p.timestep, ctx->p.gravity, &st);
+		pdc::solve(ctx, ctx->p.timestep, ctx->p.gravity);
+		const pdc_solve_stats st = pdc::aggregate_stats(ctx);
 		if (out_acc)
 			PDC_HIP(hipMemcpyAsync(out_acc, ctx->b_acc.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s));
 		std::vector<double> L(3 * (size_t)C);
@@ -386,25 +402,66 @@ int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids
 	});
 }
 
+static void step_stepped_scenes(pdc_context* ctx, int n_steps) {
+	for (int k = 0; k < n_steps; ++k) {
+		const double t0 = now_s();
+		pdc::detect_contacts(ctx);
+		const double t1 = now_s();
+		pdc::solve(ctx, ctx->p.timestep, ctx->p.gravity);
+		pdc::integrate(ctx, ctx->p.timestep);
+		PDC_HIP(hipStreamSynchronize(ctx->stream));
+		const double t2 = now_s();
+		ctx->timings.contact_detection += t1 - t0;
+		ctx->timings.force_process += t2 - t1;
+		ctx->timings.steps += 1;
+		ctx->last = pdc::aggregate_stats(ctx);
+	}
+}
+
 int pdc_step(pdc_context* ctx, int32_t n_steps, pdc_solve_stats* stats) {
 	return guard(ctx, [&] {
 		require_bodies(ctx);
 		if (n_steps < 0) throw std::invalid_argument("pdc_step: negative step count");
-		for (int k = 0; k < n_steps; ++k) {
-			const double t0 = now_s();
-			pdc::detect_contacts(ctx);
-			const double t1 = now_s();
-			pdc_solve_stats st{};
-			pdc::solve(ctx, ctx->p.timestep, ctx->p.gravity, &st);
-			pdc::integrate(ctx, ctx->p.timestep);
-			PDC_HIP(hipStreamSynchronize(ctx->stream));
-			const double t2 = now_s();
-			ctx->timings.contact_detection += t1 - t0;
-			ctx->timings.force_process += t2 - t1;
-			ctx->timings.steps += 1;
-			ctx->last = st;
-		}
+		if (std::count(ctx->h_step.begin(), ctx->h_step.end(), 0) > 0) pdc::set_stepped(ctx, 0, nullptr);
+		step_stepped_scenes(ctx, n_steps);
 		if (stats) *stats = ctx->last;
+	});
+}
+
+int pdc_set_scenes(pdc_context* ctx, int32_t n_scenes, const int32_t* body_begin) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		if (n_scenes < 1 || !body_begin) throw std::invalid_argument("pdc_set_scenes: null offsets or no scene");
+		pdc::set_scenes(ctx, n_scenes, body_begin);
+	});
+}
+
+int pdc_scene_count(const pdc_context* ctx) { return ctx ? ctx->S : 0; }
+
+int pdc_set_scene_friction(pdc_context* ctx, const double* mu) {
+	return guard(ctx, [&] {
+		if (!mu) throw std::invalid_argument("pdc_set_scene_friction: null array");
+		for (int k = 0; k < ctx->S; ++k)
+			if (!(mu[k] >= 0)) throw ConfigError("static_coeff must be >= 0");
+		ctx->h_mu.assign(mu, mu + ctx->S);
+	});
+}
+
+int pdc_step_scenes(pdc_context* ctx, int32_t n_steps, int32_t n_selected, const int32_t* scene_ids,
+                    pdc_solve_stats* stats) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		if (n_steps < 0 || n_selected < 0 || (n_selected > 0 && !scene_ids))
+			throw std::invalid_argument("pdc_step_scenes: negative count or null scene ids");
+		if (n_selected == 0) pdc::set_stepped(ctx, 0, nullptr);
+		else pdc::set_stepped(ctx, n_selected, scene_ids);
+		step_stepped_scenes(ctx, n_steps);
+		if (stats) {
+			if (n_selected == 0)
+				for (int k = 0; k < ctx->S; ++k) stats[k] = ctx->scene_stats[k];
+			else
+				for (int k = 0; k < n_selected; ++k) stats[k] = ctx->scene_stats[scene_ids[k]];
+		}
 	});
 }
 

@@ -158,6 +158,60 @@ void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims
 		}
 	}
 	PDC_HIP(hipStreamSynchronize(s));
+	const int one[2] = {0, (int)N};
+	set_scenes(ctx, 1, one);
+}
+
+void set_scenes(pdc_context* ctx, int S, const int* sbeg) {
+	const int N = ctx->N;
+	if (S < 1) throw std::invalid_argument("the scene count must be at least 1");
+	if (sbeg[0] != 0 || sbeg[S] != N)
+		throw std::invalid_argument("scene offsets must start at 0 and end at the body count");
+	for (int k = 0; k < S; ++k)
+		if (sbeg[k + 1] < sbeg[k]) throw std::invalid_argument("scene offsets must be non-decreasing");
+	hipStream_t s = ctx->stream;
+	PDC_HIP(hipStreamSynchronize(s));
+	ctx->S = S;
+	ctx->h_sbeg.assign(sbeg, sbeg + S + 1);
+	std::vector<int> bs(std::max(N, 1), 0);
+	int cap = 0;
+	for (int k = 0; k < S; ++k) {
+		for (int b = sbeg[k]; b < sbeg[k + 1]; ++b) bs[b] = k;
+		cap += (sbeg[k + 1] - sbeg[k] + kPcgRowsPerWG - 1) / kPcgRowsPerWG;  // PCG tiles of the scene at most
+	}
+	ctx->cap_tr = std::max(cap, 1);
+	ctx->b_scene.ensure(bs.size());
+	PDC_HIP(hipMemcpyAsync(ctx->b_scene.p, bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+	ctx->d_sbeg.ensure(S + 1);
+	PDC_HIP(hipMemcpyAsync(ctx->d_sbeg.p, sbeg, (S + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+	ctx->h_mu.assign(S, ctx->p.friction_coeff);
+	ctx->d_ss.ensure(S);
+	ctx->d_sctl.ensure(S);
+	ctx->h_sctl.assign(S, SceneCtl{});
+	ctx->d_ps.ensure(S);
+	PDC_HIP(hipMemsetAsync(ctx->d_ps.p, 0, S * sizeof(PcgScene), s));
+	if ((size_t)S > ctx->h_ss_cap) {
+		if (ctx->h_ss) PDC_HIP(hipHostFree(ctx->h_ss));
+		ctx->h_ss = nullptr;
+		ctx->h_ss_cap = 0;
+		PDC_HIP(hipHostMalloc((void**)&ctx->h_ss, S * sizeof(SceneSc), hipHostMallocDefault));
+		ctx->h_ss_cap = S;
+	}
+	ctx->scene_stats.assign(S, pdc_solve_stats{});
+	set_stepped(ctx, 0, nullptr);
+}
+
+void set_stepped(pdc_context* ctx, int n, const int* ids) {
+	const int S = ctx->S;
+	ctx->h_step.assign(S, ids ? 0 : 1);
+	if (ids)
+		for (int k = 0; k < n; ++k) {
+			if (ids[k] < 0 || ids[k] >= S) throw std::invalid_argument("scene id out of range");
+			ctx->h_step[ids[k]] = 1;
+		}
+	ctx->d_step.ensure(S);
+	PDC_HIP(hipMemcpyAsync(ctx->d_step.p, ctx->h_step.data(), S * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+	PDC_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 // ---------------------------------------------------------------------------
@@ -187,16 +241,21 @@ __device__ __forceinline__ bool aabb_overlap(const double* A, const double* B) {
 	return true;
 }
 
-// Broadphase, pass 1: candidates b > a per body a.
+// Broadphase, pass 1: candidates b > a per body a, within a's scene, for the
+// scenes stepped by this call.
 __global__ __launch_bounds__(256) void k_bp_count(int N, const double* __restrict__ aabb, const int* __restrict__ stat,
-                           int* __restrict__ cnt) {
+                           const int* __restrict__ bscene, const int* __restrict__ sbeg,
+                           const int* __restrict__ stepped, int* __restrict__ cnt) {
 	const int a = blockIdx.x * blockDim.x + threadIdx.x;
 	if (a >= N) return;
+	const int sc = bscene[a];
+	if (!stepped[sc]) { cnt[a] = 0; return; }
+	const int end = sbeg[sc + 1];
 	double A[6];
 	for (int k = 0; k < 6; ++k) A[k] = aabb[6 * a + k];
 	const int sa = stat[a];
 	int c = 0;
-	for (int b = a + 1; b < N; ++b) {
+	for (int b = a + 1; b < end; ++b) {
 		if (sa && stat[b]) continue;
 		if (aabb_overlap(A, &aabb[6 * b])) ++c;
 	}
@@ -205,17 +264,21 @@ __global__ __launch_bounds__(256) void k_bp_count(int N, const double* __restric
 
 // Broadphase, pass 2: write the candidates in (a, b) order.
 __global__ __launch_bounds__(256) void k_bp_write(int N, const double* __restrict__ aabb, const int* __restrict__ stat,
-                           const int* __restrict__ off, int* __restrict__ pa, int* __restrict__ pb,
-                           const int* __restrict__ soff, const int* __restrict__ emit,
-                           int* __restrict__ stage_cnt) {
+                           const int* __restrict__ bscene, const int* __restrict__ sbeg,
+                           const int* __restrict__ stepped, const int* __restrict__ off,
+                           int* __restrict__ pa, int* __restrict__ pb, const int* __restrict__ soff,
+                           const int* __restrict__ emit, int* __restrict__ stage_cnt) {
 	const int a = blockIdx.x * blockDim.x + threadIdx.x;
 	if (a >= N) return;
+	const int sc = bscene[a];
+	if (!stepped[sc]) return;
+	const int end = sbeg[sc + 1];
 	double A[6];
 	for (int k = 0; k < 6; ++k) A[k] = aabb[6 * a + k];
 	const int sa = stat[a];
 	const int na = emit[a] ? soff[a + 1] - soff[a] : 0;
 	int o = off[a];
-	for (int b = a + 1; b < N; ++b) {
+	for (int b = a + 1; b < end; ++b) {
 		if (sa && stat[b]) continue;
 		if (aabb_overlap(A, &aabb[6 * b])) {
 			pa[o] = a;
@@ -496,7 +559,7 @@ void detect_contacts(pdc_context* ctx) {
 	ctx->bp_cnt.ensure(N + 1);
 	ctx->bp_off.ensure(N + 1);
 	hipLaunchKernelGGL(k_bp_count, dim3(grid_for(N, 64)), dim3(64), 0, s, N, ctx->b_aabb.p,
-	                   ctx->b_static.p, ctx->bp_cnt.p);
+	                   ctx->b_static.p, ctx->b_scene.p, ctx->d_sbeg.p, ctx->d_step.p, ctx->bp_cnt.p);
 	PDC_CHECK_LAUNCH();
 	scan_exclusive(ctx, ctx->bp_cnt.p, ctx->bp_off.p, N);
 	const int P = read_int(ctx, ctx->bp_off.p + N);
@@ -505,8 +568,8 @@ void detect_contacts(pdc_context* ctx) {
 	ctx->pr_soff.ensure(P + 1); ctx->pr_nc.ensure(P + 1); ctx->pr_coff.ensure(P + 1);
 	ctx->pr_nh.ensure(P + 1);
 	hipLaunchKernelGGL(k_bp_write, dim3(grid_for(N, 64)), dim3(64), 0, s, N, ctx->b_aabb.p,
-	                   ctx->b_static.p, ctx->bp_off.p, ctx->pr_a.p, ctx->pr_b.p, ctx->b_soff.p,
-	                   ctx->b_emit.p, ctx->pr_tmp.p);
+	                   ctx->b_static.p, ctx->b_scene.p, ctx->d_sbeg.p, ctx->d_step.p, ctx->bp_off.p,
+	                   ctx->pr_a.p, ctx->pr_b.p, ctx->b_soff.p, ctx->b_emit.p, ctx->pr_tmp.p);
 	PDC_CHECK_LAUNCH();
 	scan_exclusive(ctx, ctx->pr_tmp.p, ctx->pr_soff.p, P);
 	const int S = read_int(ctx, ctx->pr_soff.p + P);

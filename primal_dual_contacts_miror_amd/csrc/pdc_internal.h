@@ -60,38 +60,53 @@ struct DBuf {
 	operator T*() const { return p; }
 };
 
-// PCG control block: read by every PCG kernel, written by the first
-// workgroup of the kernel that changes it.
-struct PcgCtl {
-	int nb;        // block rows (bodies in contact)
-	int iters;     // iterations of the last solve
-	int done;      // converged flag
-	int itc[2];    // iteration-counter slots
-	int pad;
-	double tol2;   // squared relative tolerance
+constexpr int kPcgRowsPerWG = 8;  // PCG block rows (wavefronts) per workgroup
+
+// Work tile: a run of consecutive elements (contacts, bodies, pairs or PCG
+// block rows) of one scene, processed by one workgroup.  Tiles never cross a
+// scene boundary, so per-workgroup partial sums are per-scene sums.
+struct Tile {
+	int scene;   // -1 for padding tiles
+	int first;   // first element
+	int count;   // elements in the tile
 };
 
-// Scalars shared between the reduction kernels and the host (one device struct).
-struct Scalars {
-	PcgCtl pcg;
+// Per-scene scalars of the solve, written by the reduction kernels and read
+// by the host (one entry per scene).
+struct SceneSc {
 	double char_mass, char_speed;
 	double gap, muc;
 	double ru2, wrf2, rl2, ac2;
 	double err, sErr, sacErr, eps, exitErr;
-	double Hu;
-	// line search trial
-	double t_ru2, t_wrf2, t_rl2, t_ac2, t_gErr_u, t_gErr_f, t_gErr_l;
 	double t_inErr, t_sacIn, t_gErr;
-	double ls_step;
+	double ls_step;      // accepted step of the last line-search pass (0: failure)
 	int ls_decided, ls_trials;
+};
+
+// Per-scene control written by the host before a phase of the solve.
+struct SceneCtl {
+	int act;     // the scene takes part in the phase
+	int pad;
+	double Hu;   // the scene's Hu (eps = Hu err^0.333335)
+	double mu;   // the scene's friction coefficient
+};
+
+// Per-scene PCG control block: read by every PCG kernel of the scene, written
+// by the scene's first tile.
+struct PcgScene {
+	int iters;   // iterations of the last solve
+	int done;    // converged flag
+	int itc[2];  // iteration-counter slots
+};
+
+// Detection counters (one device struct).
+struct Scalars {
 	int overflow;        // capacity overflow flag set by kernels
 	int n_pairs;         // broadphase candidate pairs
 	int n_contacts;      // contacts after merge
 	int n_raw;           // raw sample hits
 	int ties;
-	int n_local;         // bodies in contact
-	int n_apairs;        // pairs with contacts
-	int nnzb;            // blocks of the Newton matrix
+	int pad[3];
 };
 
 struct PcgGraph {
@@ -192,6 +207,30 @@ struct pdc_context {
 	pdc::DBuf<pdc::Scalars> d_sc;
 	pdc::Scalars* h_sc = nullptr;                            // pinned mirror
 
+	// ---- scenes ----
+	int S = 1;                                                // independent scenes
+	std::vector<int> h_sbeg{0, 0};                           // per scene: first body (S + 1)
+	std::vector<int> h_step;                                  // per scene: stepped by the current call
+	std::vector<double> h_mu;                                 // per scene friction coefficient
+	pdc::DBuf<int> b_scene;                                   // per body: scene
+	pdc::DBuf<int> d_sbeg;                                    // per scene: first body (S + 1)
+	pdc::DBuf<int> d_step;                                    // per scene: stepped (broadphase, integration)
+	pdc::DBuf<int> r_cbeg, r_lbeg, r_pbeg;                    // per scene: first contact, local body, pair
+	pdc::DBuf<int> r_nnzb;                                    // per scene: Newton-matrix blocks (+ total)
+	std::vector<int> h_cbeg, h_lbeg, h_pbeg, h_nnzb;
+	pdc::DBuf<pdc::Tile> t_c, t_b, t_w, t_p, t_r;             // tiles: contacts, bodies, body waves, pair waves, PCG rows
+	pdc::DBuf<int> tb_c, tb_b, tb_w, tb_p, tb_r;              // per scene: first tile (S + 1)
+	int n_tc = 0, n_tb = 0, n_tw = 0, n_tp = 0, n_tr = 0;
+	int cap_tr = 0;                                           // PCG tiles launched (fixed per scene set)
+	pdc::DBuf<pdc::SceneSc> d_ss;                             // per-scene scalars
+	pdc::SceneSc* h_ss = nullptr;                             // pinned mirror
+	size_t h_ss_cap = 0;
+	pdc::DBuf<pdc::SceneCtl> d_sctl;                          // per-scene control
+	std::vector<pdc::SceneCtl> h_sctl;
+	pdc::DBuf<pdc::PcgScene> d_ps;                            // per-scene PCG control
+	std::vector<pdc::PcgScene> h_ps;
+	std::vector<pdc_solve_stats> scene_stats;                 // per scene, last solve
+
 	// ---- statistics ----
 	pdc_timings timings{};
 	pdc_solve_stats last{};
@@ -217,7 +256,16 @@ void detect_contacts(pdc_context* ctx);  // fills c_* and ctx->C
 
 // pdc_solver.hip
 void build_topology(pdc_context* ctx);   // from c_id0/c_id1 (sorted by pair key)
-void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st);
+// Newton solve of every scene with contacts; per-scene statistics in
+// ctx->scene_stats, their aggregate from aggregate_stats().
+void solve(pdc_context* ctx, double dt, const double* g);
+pdc_solve_stats aggregate_stats(const pdc_context* ctx);
+// Scene table: S scenes over the uploaded bodies (sbeg: S + 1 body offsets);
+// allocates the per-scene buffers and resets friction to the configured one.
+void set_scenes(pdc_context* ctx, int S, const int* sbeg);
+// Marks the scenes stepped by the next detection / integration (all when
+// ids == nullptr).
+void set_stepped(pdc_context* ctx, int n, const int* ids);
 void write_acc_default(pdc_context* ctx, double dt, const double* g);
 void integrate(pdc_context* ctx, double dt);
 void ensure_contact_capacity(pdc_context* ctx, size_t C);

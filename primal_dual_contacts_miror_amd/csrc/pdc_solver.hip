@@ -235,6 +235,86 @@ void ensure_contact_capacity(pdc_context* ctx, size_t C) {
 	ctx->c_stride = (int)cs;
 }
 
+// First element of every scene in a list grouped by scene: element k belongs
+// to the scene of global body ids[k]; beg[s] = first k of scene >= s, beg[S] = n.
+__global__ void k_scene_bounds(int n, int S, const int* __restrict__ ids, const int* __restrict__ bscene,
+                               int* __restrict__ beg) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i > n) return;
+	const int hi = i < n ? bscene[ids[i]] : S;
+	const int lo = i > 0 ? bscene[ids[i - 1]] : -1;
+	for (int s = lo + 1; s <= hi; ++s) beg[s] = i;
+}
+
+// Newton-matrix blocks per scene (block rows of a scene are contiguous).
+__global__ void k_scene_nnzb(int S, const int* __restrict__ lbeg, const int* __restrict__ aptr, int* __restrict__ out) {
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s < S) out[s] = aptr[lbeg[s + 1]] - aptr[lbeg[s]];
+	if (s == S) out[S] = aptr[lbeg[S]];
+}
+
+// Tiles of `per` consecutive elements inside each scene; padded with empty
+// tiles (scene -1) up to `pad_to`.
+static void make_tiles(const std::vector<int>& beg, int S, int per, std::vector<Tile>& t, std::vector<int>& tb,
+                       int pad_to = 0) {
+	t.clear();
+	tb.assign(S + 1, 0);
+	for (int sc = 0; sc < S; ++sc) {
+		tb[sc] = (int)t.size();
+		for (int e = beg[sc]; e < beg[sc + 1]; e += per) t.push_back(Tile{sc, e, std::min(per, beg[sc + 1] - e)});
+	}
+	tb[S] = (int)t.size();
+	while ((int)t.size() < pad_to) t.push_back(Tile{-1, 0, 0});
+}
+
+static int upload_tiles(pdc_context* ctx, const std::vector<Tile>& t, const std::vector<int>& tb, DBuf<Tile>& dt,
+                        DBuf<int>& dtb) {
+	hipStream_t s = ctx->stream;
+	dt.ensure(std::max<size_t>(t.size(), 1));
+	dtb.ensure(tb.size());
+	if (!t.empty()) PDC_HIP(hipMemcpyAsync(dt.p, t.data(), t.size() * sizeof(Tile), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(dtb.p, tb.data(), tb.size() * sizeof(int), hipMemcpyHostToDevice, s));
+	return (int)t.size();
+}
+
+// Per-scene contact, local-body and pair ranges, the Newton-matrix size, and
+// the work tiles of every kernel kind (one read-back).
+static void scene_ranges(pdc_context* ctx) {
+	hipStream_t s = ctx->stream;
+	const int S = ctx->S, C = ctx->C, NL = ctx->NL, NP = ctx->NP;
+	ctx->r_cbeg.ensure(S + 1); ctx->r_lbeg.ensure(S + 1); ctx->r_pbeg.ensure(S + 1);
+	ctx->r_nnzb.ensure(S + 1);
+	hipLaunchKernelGGL(k_scene_bounds, dim3(grid_for(C + 1)), dim3(kBlock), 0, s, C, S, ctx->c_id0.p,
+	                   ctx->b_scene.p, ctx->r_cbeg.p);
+	hipLaunchKernelGGL(k_scene_bounds, dim3(grid_for(NL + 1)), dim3(kBlock), 0, s, NL, S, ctx->l2g.p,
+	                   ctx->b_scene.p, ctx->r_lbeg.p);
+	hipLaunchKernelGGL(k_scene_bounds, dim3(grid_for(NP + 1)), dim3(kBlock), 0, s, NP, S, ctx->ap_a.p,
+	                   ctx->b_scene.p, ctx->r_pbeg.p);
+	hipLaunchKernelGGL(k_scene_nnzb, dim3(grid_for(S + 1)), dim3(kBlock), 0, s, S, ctx->r_lbeg.p, ctx->a_ptr.p,
+	                   ctx->r_nnzb.p);
+	PDC_CHECK_LAUNCH();
+	ctx->h_cbeg.resize(S + 1); ctx->h_lbeg.resize(S + 1); ctx->h_pbeg.resize(S + 1); ctx->h_nnzb.resize(S + 1);
+	PDC_HIP(hipMemcpyAsync(ctx->h_cbeg.data(), ctx->r_cbeg.p, (S + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+	PDC_HIP(hipMemcpyAsync(ctx->h_lbeg.data(), ctx->r_lbeg.p, (S + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+	PDC_HIP(hipMemcpyAsync(ctx->h_pbeg.data(), ctx->r_pbeg.p, (S + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+	PDC_HIP(hipMemcpyAsync(ctx->h_nnzb.data(), ctx->r_nnzb.p, (S + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+	PDC_HIP(hipStreamSynchronize(s));
+	ctx->NNZB = ctx->h_nnzb[S];
+	std::vector<Tile> t;
+	std::vector<int> tb;
+	make_tiles(ctx->h_cbeg, S, kBlock, t, tb);
+	ctx->n_tc = upload_tiles(ctx, t, tb, ctx->t_c, ctx->tb_c);
+	make_tiles(ctx->h_lbeg, S, kBlock, t, tb);
+	ctx->n_tb = upload_tiles(ctx, t, tb, ctx->t_b, ctx->tb_b);
+	make_tiles(ctx->h_lbeg, S, 4, t, tb);
+	ctx->n_tw = upload_tiles(ctx, t, tb, ctx->t_w, ctx->tb_w);
+	make_tiles(ctx->h_pbeg, S, 4, t, tb);
+	ctx->n_tp = upload_tiles(ctx, t, tb, ctx->t_p, ctx->tb_p);
+	make_tiles(ctx->h_lbeg, S, kPcgRowsPerWG, t, tb, ctx->cap_tr);
+	if ((int)t.size() > ctx->cap_tr) throw HipError("PCG tile table exceeds its capacity", PDC_ERR_STATE);
+	ctx->n_tr = upload_tiles(ctx, t, tb, ctx->t_r, ctx->tb_r);
+}
+
 void build_topology(pdc_context* ctx) {
 	hipStream_t s = ctx->stream;
 	const int C = ctx->C, N = ctx->N;
@@ -294,8 +374,8 @@ void build_topology(pdc_context* ctx) {
 	                   ctx->li0.p, ctx->li1.p, ctx->ap_a.p, ctx->g2l.p, ctx->ap_start.p, ctx->pair_adj.p,
 	                   ctx->adj_pos.p, ctx->inc_ptr.p, ctx->c_pid.p, ctx->inc.p);
 	PDC_CHECK_LAUNCH();
-	const int NNZB = read_int(ctx, ctx->a_ptr.p + NL);
-	ctx->NNZB = NNZB;
+	scene_ranges(ctx);
+	const int NNZB = ctx->NNZB;
 	ctx->a_col.ensure(NNZB);
 	hipLaunchKernelGGL(k_a_fill, dim3(grid_for(NL)), dim3(kBlock), 0, s, NL, ctx->l_static.p,
 	                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->a_ptr.p, ctx->a_col.p, ctx->a_diag.p,
@@ -307,9 +387,22 @@ void build_topology(pdc_context* ctx) {
 // solver kernels
 // ===========================================================================
 struct SArgs {
-	int C, cs, NL, nsc;
-	double mu, restitution, compliance, reduction, dt, gdt[3];
+	int C, cs, NL, nsc, S;
+	int pstride;          // partial-sum slot stride (>= every tile count)
+	double restitution, compliance, reduction, dt, gdt[3];
 	int spring;
+	// scenes and work tiles (a workgroup's tile lies inside one scene)
+	const Tile* tc;       // contacts, kBlock per tile
+	const Tile* tb;       // local bodies, kBlock per tile
+	const Tile* tw;       // local bodies, one wavefront each, 4 per tile
+	const Tile* tp;       // active pairs, one wavefront each, 4 per tile
+	const int* tbc;       // per scene: first tile of each kind (S + 1)
+	const int* tbb;
+	const int* tbw;
+	const int* cbeg;      // per scene: first contact, first local body (S + 1)
+	const int* lbeg;
+	SceneSc* ss;
+	const SceneCtl* ctl;
 	// contacts
 	const int* li0;
 	const int* li1;
@@ -352,7 +445,6 @@ struct SArgs {
 	double* df;
 	double* dl;
 	double* part;
-	Scalars* sc;
 };
 
 __device__ __forceinline__ double Tc(const SArgs& a, int i, int j, int k) {
@@ -373,20 +465,36 @@ __device__ __forceinline__ void to_local(const SArgs& a, int i, const double* va
 }
 
 template <int K>
-__device__ __forceinline__ void write_partials(double (&v)[K], double* part, const int* slots) {
+__device__ __forceinline__ void write_partials(double (&v)[K], double* part, int stride, const int* slots) {
 	__shared__ double lds[K * 16];
 	double out[K];
 	block_sum<K, double>(v, lds, out);
 	if (threadIdx.x == 0)
-		for (int k = 0; k < K; ++k) part[slots[k] * kMaxWG + blockIdx.x] = out[k];
+		for (int k = 0; k < K; ++k) part[slots[k] * (size_t)stride + blockIdx.x] = out[k];
 }
+
+// Sum of partial slot `slot` over tiles [t0, t1) by the first wavefront, in a
+// fixed order; every thread of the workgroup gets it (`tmp`: an LDS scalar).
+__device__ __forceinline__ double tile_sum(const double* part, int stride, int slot, int t0, int t1,
+                                           double* tmp) {
+	return block_reduce_array(part + slot * (size_t)stride + t0, t1 - t0, tmp);
+}
+
+// The workgroup's tile, or nullptr when its scene sits out the phase.
+#define PDC_TILE(TILES)                                  \
+	const Tile tile = (TILES)[blockIdx.x];               \
+	if (tile.scene < 0 || !a.ctl[tile.scene].act) return; \
+	const int scn = tile.scene;
 
 // ---- setup ----------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_setup_bodies(SArgs a, const double* __restrict__ vel, const double* __restrict__ ang,
                                const double* __restrict__ quat, const double* __restrict__ mass,
                                const double* __restrict__ I0) {
+	PDC_TILE(a.tb);
+	(void)scn;
 	double acc[2] = {0, 0};
-	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.NL; r += gridDim.x * blockDim.x) {
+	if ((int)threadIdx.x < tile.count) {
+		const int r = tile.first + threadIdx.x;
 		const int g = a.l2g[r];
 		const int st = a.lstat[r];
 		double v[6] = {vel[3 * g], vel[3 * g + 1], vel[3 * g + 2], ang[3 * g], ang[3 * g + 1], ang[3 * g + 2]};
@@ -417,7 +525,7 @@ __global__ __launch_bounds__(256) void k_setup_bodies(SArgs a, const double* __r
 		}
 	}
 	const int slots[2] = {P_ND_TRACE, P_ND_CNT};
-	write_partials<2>(acc, a.part, slots);
+	write_partials<2>(acc, a.part, a.pstride, slots);
 }
 
 // objSpeed per body (PrimalDualForceSolver.cpp:38-63): one wavefront per
@@ -425,14 +533,15 @@ __global__ __launch_bounds__(256) void k_setup_bodies(SArgs a, const double* __r
 // half relative velocity and, when the contact involves a static body, of
 // the full relative velocity.
 __global__ __launch_bounds__(256) void k_nondim_speed(SArgs a) {
-	const int lane = threadIdx.x & 63;
-	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-	const int nw = (gridDim.x * blockDim.x) >> 6;
-	for (int r = wave; r < a.NL; r += nw) {
+	PDC_TILE(a.tw);
+	(void)scn;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	if (wid < tile.count) {
+		const int r = tile.first + wid;
 		double m = 0;
 		if (a.lstat[r]) {
 			if (lane == 0) a.speed[r] = 0;
-			continue;  // charSpeed averages over active bodies only
+			return;  // charSpeed averages over active bodies only
 		}
 		for (int k = a.inc_ptr[r] + lane; k < a.inc_ptr[r + 1]; k += 64) {
 			const int e = a.inc[k];
@@ -468,14 +577,16 @@ __global__ __launch_bounds__(256) void k_nondim_speed(SArgs a) {
 	}
 }
 
-// charMass, charSpeed (PrimalDualForceSolver.cpp:23-78); one workgroup.
-__global__ __launch_bounds__(1024) void k_nondim_final(SArgs a, int nwg, const double* __restrict__ mass, double gnorm_dt) {
+// charMass, charSpeed (PrimalDualForceSolver.cpp:23-78); one workgroup per scene.
+__global__ __launch_bounds__(1024) void k_nondim_final(SArgs a, const double* __restrict__ mass, double gnorm_dt) {
 	__shared__ double slot;
 	__shared__ double lds[2 * 16];
-	const double tr = block_reduce_array(&a.part[P_ND_TRACE * kMaxWG], nwg, &slot);
-	const double cnt = block_reduce_array(&a.part[P_ND_CNT * kMaxWG], nwg, &slot);
+	const int sc = blockIdx.x;
+	if (!a.ctl[sc].act) return;
+	const double tr = tile_sum(a.part, a.pstride, P_ND_TRACE, a.tbb[sc], a.tbb[sc + 1], &slot);
+	const double cnt = tile_sum(a.part, a.pstride, P_ND_CNT, a.tbb[sc], a.tbb[sc + 1], &slot);
 	double v[2] = {0, 0};
-	for (int r = threadIdx.x; r < a.NL; r += blockDim.x) {
+	for (int r = a.lbeg[sc] + threadIdx.x; r < a.lbeg[sc + 1]; r += blockDim.x) {
 		if (a.lstat[r]) continue;
 		const double m = mass[a.l2g[r]];
 		v[0] += m;
@@ -488,16 +599,18 @@ __global__ __launch_bounds__(1024) void k_nondim_final(SArgs a, int nwg, const d
 		double s = out[1] / out[0];
 		s = s > gnorm_dt * 0.25 ? s : gnorm_dt * 0.25;
 		s = s > 1e-4 ? s : 1e-4;
-		a.sc->char_mass = avg;
-		a.sc->char_speed = s;
+		a.ss[sc].char_mass = avg;
+		a.ss[sc].char_speed = s;
 	}
 }
 
 // Scale velocities by charSpeed and masses by charMass; K, f, lambda init
 // (NonSmoothContactForce.cpp:8-52, PrimalDualForceSolver.cpp:216-252).
 __global__ __launch_bounds__(256) void k_scale_bodies(SArgs a) {
-	const double s = a.sc->char_speed, m = a.sc->char_mass;
-	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.NL; r += gridDim.x * blockDim.x) {
+	PDC_TILE(a.tb);
+	const double s = a.ss[scn].char_speed, m = a.ss[scn].char_mass;
+	if ((int)threadIdx.x < tile.count) {
+		const int r = tile.first + threadIdx.x;
 		for (int k = 0; k < 6; ++k) {
 			a.v[6 * r + k] /= s;
 			a.vold[6 * r + k] /= s;
@@ -508,8 +621,10 @@ __global__ __launch_bounds__(256) void k_scale_bodies(SArgs a) {
 }
 
 __global__ __launch_bounds__(256) void k_preprocess(SArgs a) {
-	const double s = a.sc->char_speed;
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
+	PDC_TILE(a.tc);
+	const double s = a.ss[scn].char_speed;
+	if ((int)threadIdx.x < tile.count) {
+		const int i = tile.first + threadIdx.x;
 		double Ki = 0;
 		if (a.spring) {
 			Ki += -a.reduction * a.depth[i] / a.dt / s;
@@ -531,12 +646,15 @@ __global__ __launch_bounds__(256) void k_preprocess(SArgs a) {
 // Constraint values, gradients, Hessians; partial sums of lambda.c for the
 // surrogate duality gap (NonSmoothContactForce.cpp:105-118, NonSmoothForceBase.h:82-86).
 __global__ __launch_bounds__(256) void k_constraints(SArgs a) {
+	PDC_TILE(a.tc);
+	const double mu = a.ctl[scn].mu;
 	double acc[1] = {0};
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
+	if ((int)threadIdx.x < tile.count) {
+		const int i = tile.first + threadIdx.x;
 		const double f0 = a.f[i];
 		if (a.nsc) {
 			const double f1 = a.f[a.cs + i], f2 = a.f[2 * a.cs + i];
-			const double c0 = -f0, c1 = Lorentz::constraint(a.mu, f0, f1, f2);
+			const double c0 = -f0, c1 = Lorentz::constraint(mu, f0, f1, f2);
 			double g[2], H[3];
 			Lorentz::gradient(f1, f2, g);
 			Lorentz::hessian(f1, f2, H);
@@ -550,18 +668,18 @@ __global__ __launch_bounds__(256) void k_constraints(SArgs a) {
 		}
 	}
 	const int slots[1] = {P_GAP};
-	write_partials<1>(acc, a.part, slots);
+	write_partials<1>(acc, a.part, a.pstride, slots);
 }
 
 // Residual terms of one contact at (v', f', lambda') (NonSmoothContactForce.cpp:53-172).
 // Returns rf (3), rl (2), wrf (3), ac (3) with fresh constraint values.
 __device__ __forceinline__ void contact_terms(const SArgs& a, int i, const double* va, const double* vb,
-                                              const double* f, const double* lam, double muc,
-                                              double* rf, double* rl, double* wrf, double* ac) {
+                                              const double* f, const double* lam, double muc, double mu,
+                                              double cm, double* rf, double* rl, double* wrf, double* ac) {
 	double u[3];
 	to_local(a, i, va, vb, u);
 	const double Ki = a.K[i];
-	const double spring_term = a.spring ? a.compliance * a.sc->char_mass * f[0] : 0.0;
+	const double spring_term = a.spring ? a.compliance * cm * f[0] : 0.0;
 	if (a.nsc) {
 		double r0 = -u[0], r1 = -u[1], r2 = -u[2];
 		if (a.spring) r0 -= spring_term;
@@ -573,7 +691,7 @@ __device__ __forceinline__ void contact_terms(const SArgs& a, int i, const doubl
 		r0 -= Ki;
 		rf[0] = r0; rf[1] = r1; rf[2] = r2;
 		rl[0] = lam[0] * -f[0] + muc;
-		rl[1] = lam[1] * Lorentz::constraint(a.mu, f[0], f[1], f[2]) + muc;
+		rl[1] = lam[1] * Lorentz::constraint(mu, f[0], f[1], f[2]) + muc;
 		// tangential weight (NonSmoothContactForce.cpp:74-104)
 		const double tsq = u[1] * u[1] + u[2] * u[2];
 		double tn[2] = {u[1], u[2]};
@@ -593,9 +711,9 @@ __device__ __forceinline__ void contact_terms(const SArgs& a, int i, const doubl
 			sym3_mv(Mi + 1, hh + 3, t);
 			const double q = Mi[0] * (hh[0] * hh[0] + hh[1] * hh[1] + hh[2] * hh[2]) +
 			                 (hh[3] * t[0] + hh[4] * t[1] + hh[5] * t[2]);
-			fw += q * a.sc->char_mass;
+			fw += q * cm;
 		}
-		const double rad = Lorentz::radius(a.mu, f[0]);
+		const double rad = Lorentz::radius(mu, f[0]);
 		double w = rad / (sqrt(tsq) + 1e-8) * fw;
 		w = w < 1.0 ? w : 1.0;
 		wrf[0] = r0; wrf[1] = r1 * w; wrf[2] = r2 * w;
@@ -624,18 +742,22 @@ __device__ __forceinline__ void contact_terms(const SArgs& a, int i, const doubl
 }
 
 // Main-iteration residuals of the contacts: stores rf, rl; partial |wrf|^2, |rl|^2, |ac|^2.
-__global__ __launch_bounds__(256) void k_contact_res(SArgs a, int nwg_gap, double inv_nL) {
+__global__ __launch_bounds__(256) void k_contact_res(SArgs a) {
 	__shared__ double slot;
-	const double gapsum = block_reduce_array(&a.part[P_GAP * kMaxWG], nwg_gap, &slot);
-	const double gap = -gapsum * inv_nL;
+	PDC_TILE(a.tc);
+	const double gapsum = tile_sum(a.part, a.pstride, P_GAP, a.tbc[scn], a.tbc[scn + 1], &slot);
+	const int Cs = a.cbeg[scn + 1] - a.cbeg[scn];
+	const double gap = -gapsum / ((double)(a.nsc ? 2 : 1) * Cs);
 	const double muc = gap * 0.1;
-	if (blockIdx.x == 0 && threadIdx.x == 0) { a.sc->gap = gap; a.sc->muc = muc; }
+	if ((int)blockIdx.x == a.tbc[scn] && threadIdx.x == 0) { a.ss[scn].gap = gap; a.ss[scn].muc = muc; }
+	const double mu = a.ctl[scn].mu, cm = a.ss[scn].char_mass;
 	double acc[3] = {0, 0, 0};
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
+	if ((int)threadIdx.x < tile.count) {
+		const int i = tile.first + threadIdx.x;
 		const double f[3] = {a.f[i], a.f[a.cs + i], a.f[2 * a.cs + i]};
 		const double lam[2] = {a.lam[i], a.lam[a.cs + i]};
 		double rf[3], rl[2], wrf[3], ac[3];
-		contact_terms(a, i, &a.v[6 * a.li0[i]], &a.v[6 * a.li1[i]], f, lam, muc, rf, rl, wrf, ac);
+		contact_terms(a, i, &a.v[6 * a.li0[i]], &a.v[6 * a.li1[i]], f, lam, muc, mu, cm, rf, rl, wrf, ac);
 		for (int k = 0; k < 3; ++k) a.rf[k * a.cs + i] = rf[k];
 		a.rl[i] = rl[0];
 		a.rl[a.cs + i] = rl[1];
@@ -644,20 +766,21 @@ __global__ __launch_bounds__(256) void k_contact_res(SArgs a, int nwg_gap, doubl
 		acc[2] += ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
 	}
 	const int slots[3] = {P_WRF, P_RL, P_AC};
-	write_partials<3>(acc, a.part, slots);
+	write_partials<3>(acc, a.part, a.pstride, slots);
 }
 
 // ru = M (v - vold) + sum_j T_j f (static rows 0)  (PrimalDualForceSolver.cpp:151-163).
 // One wavefront per body: lanes take the body's incident contacts with stride
 // 64, the six sums are reduced in a fixed butterfly order.
 __global__ __launch_bounds__(256) void k_body_res(SArgs a) {
-	const int lane = threadIdx.x & 63;
-	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-	const int nw = (gridDim.x * blockDim.x) >> 6;
+	PDC_TILE(a.tw);
+	(void)scn;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	double acc[1] = {0};
 	const double* F = a.f;
 	const int fd = a.nsc ? 3 : 1;
-	for (int r = wave; r < a.NL; r += nw) {
+	if (wid < tile.count) {
+		const int r = tile.first + wid;
 		double ru[6] = {0, 0, 0, 0, 0, 0};
 		if (!a.lstat[r]) {
 			double add[6] = {0, 0, 0, 0, 0, 0};
@@ -688,25 +811,30 @@ __global__ __launch_bounds__(256) void k_body_res(SArgs a) {
 		}
 	}
 	const int slots[1] = {P_RU};
-	write_partials<1>(acc, a.part, slots);
+	write_partials<1>(acc, a.part, a.pstride, slots);
 }
 
-// err, eps and the exit test (PrimalDualForceSolver.cpp:295-320); one workgroup.
-__global__ __launch_bounds__(256) void k_err_final(SArgs a, int nwg_c, int nwg_b, double Hu, double nR, double nF,
-                            double nLc, double nC) {
+// err, eps and the exit test (PrimalDualForceSolver.cpp:295-320); one workgroup per scene.
+__global__ __launch_bounds__(256) void k_err_final(SArgs a) {
 	__shared__ double slot;
-	const double wrf2 = block_reduce_array(&a.part[P_WRF * kMaxWG], nwg_c, &slot);
-	const double rl2 = block_reduce_array(&a.part[P_RL * kMaxWG], nwg_c, &slot);
-	const double ac2 = block_reduce_array(&a.part[P_AC * kMaxWG], nwg_c, &slot);
-	const double ru2 = block_reduce_array(&a.part[P_RU * kMaxWG], nwg_b, &slot);
+	const int sc = blockIdx.x;
+	if (!a.ctl[sc].act) return;
+	const int c0 = a.tbc[sc], c1 = a.tbc[sc + 1], w0 = a.tbw[sc], w1 = a.tbw[sc + 1];
+	const double wrf2 = tile_sum(a.part, a.pstride, P_WRF, c0, c1, &slot);
+	const double rl2 = tile_sum(a.part, a.pstride, P_RL, c0, c1, &slot);
+	const double ac2 = tile_sum(a.part, a.pstride, P_AC, c0, c1, &slot);
+	const double ru2 = tile_sum(a.part, a.pstride, P_RU, w0, w1, &slot);
 	if (threadIdx.x == 0) {
-		Scalars* S = a.sc;
+		const double nC = a.cbeg[sc + 1] - a.cbeg[sc];
+		const double nR = 6.0 * (a.lbeg[sc + 1] - a.lbeg[sc]);
+		const double nF = (a.nsc ? 3.0 : 1.0) * nC, nLc = (a.nsc ? 2.0 : 1.0) * nC;
+		const double Hu = a.ctl[sc].Hu;
+		SceneSc* S = &a.ss[sc];
 		S->ru2 = ru2; S->wrf2 = wrf2; S->rl2 = rl2; S->ac2 = ac2;
 		const double err = ru2 + wrf2 + rl2;
 		S->err = err;
 		S->sErr = err;
 		S->sacErr = ru2 + ac2 + rl2;
-		S->Hu = Hu;
 		S->eps = Hu * pow(err, 0.66667 * 0.5);
 		const double momen = sqrt(ru2) / nR;
 		double e = fmax(momen, sqrt(wrf2) / nF);
@@ -720,8 +848,10 @@ __global__ __launch_bounds__(256) void k_err_final(SArgs a, int nwg_c, int nwg_b
 // ---- (c) assembly ----------------------------------------------------------
 // Per contact: invC, sf, Kn^-1, Kt^-1, Kt^-1 sf (NonSmoothContactForce.cpp:184-219).
 __global__ __launch_bounds__(256) void k_asm_prep(SArgs a) {
-	const double eps = a.sc->eps;
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
+	PDC_TILE(a.tc);
+	const double eps = a.ss[scn].eps, cm = a.ss[scn].char_mass;
+	if ((int)threadIdx.x < tile.count) {
+		const int i = tile.first + threadIdx.x;
 		const double l0 = a.lam[i];
 		if (a.nsc) {
 			const double l1 = a.lam[a.cs + i];
@@ -729,7 +859,7 @@ __global__ __launch_bounds__(256) void k_asm_prep(SArgs a) {
 			const double ic1 = 1. / (a.c[a.cs + i] - eps * l1);
 			const double g0 = a.g[i], g1 = a.g[a.cs + i];
 			double lN = -l0 * ic0;
-			if (a.spring) lN += a.compliance * a.sc->char_mass;
+			if (a.spring) lN += a.compliance * cm;
 			const double lm = -l1 * ic1;
 			const double L00 = lm * g0 * g0, L01 = lm * g0 * g1, L11 = lm * g1 * g1;
 			const double rl0 = a.rl[i], rl1 = a.rl[a.cs + i];
@@ -751,7 +881,7 @@ __global__ __launch_bounds__(256) void k_asm_prep(SArgs a) {
 		} else {
 			const double ic = 1. / (a.c[i] - eps);
 			double reg = -l0 * ic;
-			if (a.spring) reg += a.compliance * a.sc->char_mass;
+			if (a.spring) reg += a.compliance * cm;
 			const double sf0 = a.rf[i] + (-a.rl[i] * ic);
 			const double kn = 1. / (eps + reg);
 			a.invC[i] = ic;
@@ -824,12 +954,12 @@ __device__ __forceinline__ bool spd_inverse6(const double* A, double* Ai) {
 template <class T>
 __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict__ adiag, T* __restrict__ Aval,
                            T* __restrict__ Pval, T* __restrict__ bvec) {
-	const int lane = threadIdx.x & 63;
-	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-	const int nw = (gridDim.x * blockDim.x) >> 6;
-	const double eps = a.sc->eps;
+	PDC_TILE(a.tw);
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const double eps = a.ss[scn].eps;
 	const int fd = a.nsc ? 3 : 1;
-	for (int r = wave; r < a.NL; r += nw) {
+	if (wid < tile.count) {
+		const int r = tile.first + wid;
 		const bool st = a.lstat[r] != 0;
 		double U[21], y[6];
 		for (int q = 0; q < 21; ++q) U[q] = 0;
@@ -914,15 +1044,16 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 // Newton matrix is symmetric).  Stored are the blocks of the preconditioned
 // operator, A_ab P_b and A_ba P_a; P comes from k_asm_diag, which runs first.
 template <class T>
-__global__ __launch_bounds__(256) void k_asm_off(SArgs a, int NP, const T* __restrict__ Pval, T* __restrict__ Aval) {
-	const int lane = threadIdx.x & 63;
-	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-	const int nw = (gridDim.x * blockDim.x) >> 6;
+__global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ Pval, T* __restrict__ Aval) {
+	PDC_TILE(a.tp);
+	(void)scn;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int ii = lane < 36 ? lane / 6 : 0, jj = lane < 36 ? lane - 6 * (lane / 6) : 0;
-	for (int p = wave; p < NP; p += nw) {
+	if (wid < tile.count) {
+		const int p = tile.first + wid;
 		const int ka = a.pair_adj[2 * p], kb = a.pair_adj[2 * p + 1];
 		const int blk_ab = a.adj_blk[ka], blk_ba = a.adj_blk[kb];
-		if (blk_ab < 0) continue;  // a static end: no coupling block
+		if (blk_ab < 0) return;  // a static end: no coupling block
 		const int ra = a.g2l[a.pa[p]];
 		double B[36];
 		for (int q = 0; q < 36; ++q) B[q] = 0;
@@ -1002,12 +1133,16 @@ struct PcgArgs {
 	double* q;
 	double* s;
 	double* p;
-	double* gam[2];  // partials, one per workgroup, by iteration parity
+	double* gam[2];  // partials, one per tile, by iteration parity
 	double* del[2];
 	double* rr[2];
 	double* bb;
-	double* hist;    // gamma, alpha of iteration i at [2 (i & 1)], [2 (i & 1) + 1]
-	PcgCtl* ctl;     // block rows, tolerance, done flag, iteration counters (device)
+	double* hist;    // per scene: gamma, alpha of iteration i at [4 s + 2 (i & 1)], [+ 1]
+	const Tile* tiles;      // kPcgRowsPerWG block rows per tile, one tile per workgroup
+	const int* tbeg;        // per scene: first tile (S + 1)
+	const SceneCtl* ctl;    // per scene: takes part in this solve
+	PcgScene* ps;           // per scene: done flag, iteration counters
+	double tol2;            // squared relative tolerance
 };
 
 // Element k (0 or 1) of a two-slot kernel argument, selected without a runtime
@@ -1037,37 +1172,39 @@ __device__ __forceinline__ double spmv_lane(const PcgArgs<T>& a, const V* __rest
 	return acc;
 }
 
-constexpr int kPcgRowsPerWG = 8;            // block rows (wavefronts) per workgroup
 constexpr int kPcgThreads = 64 * kPcgRowsPerWG;
-constexpr int kPartLoads = kMaxWG / 64;     // partial values per lane
+constexpr int kPartLoads = 16;              // partial values per lane and pass
 
-// Lane share of one partial array (one value per workgroup of the previous
-// launch, W <= kMaxWG of them): all loads issued at once, summed in a fixed
-// order.  The wavefront sum of the returned values is the array total.
+// Lane share of one scene's partial array (one value per tile of the previous
+// launch, W of them): the loads of a pass are unconditional (indices past W
+// clamped to the last value, which the sum then drops) so that all of them
+// are in flight together; one pass covers 1024 tiles.  The wavefront sum of
+// the returned values is the array total, in a fixed order.
 __device__ __forceinline__ double partial_lane_sum(const double* __restrict__ src, int W) {
 	const int lane = threadIdx.x & 63;
-	double v[kPartLoads];
-	// unconditional loads (indices past W clamped to the last value, which the
-	// sum then drops) so that all of them are in flight together
-#pragma unroll
-	for (int k = 0; k < kPartLoads; ++k) v[k] = src[min(lane + 64 * k, W - 1)];
 	double s = 0;
+	for (int base = 0; base < W; base += 64 * kPartLoads) {
+		double v[kPartLoads];
 #pragma unroll
-	for (int k = 0; k < kPartLoads; ++k) s += lane + 64 * k < W ? v[k] : 0.0;
+		for (int k = 0; k < kPartLoads; ++k) v[k] = src[min(base + lane + 64 * k, W - 1)];
+#pragma unroll
+		for (int k = 0; k < kPartLoads; ++k) s += base + lane + 64 * k < W ? v[k] : 0.0;
+	}
 	return s;
 }
 
-// Totals of N <= 4 partial arrays p0..p{N-1}: wavefront k sums array k; every
-// thread gets all totals.  (Separate pointer arguments, not an array of
-// pointers: a runtime-indexed private array would be placed in LDS.)
+// Totals of N <= 4 partial arrays p0..p{N-1} over the scene's tiles
+// [t0, t0 + W): wavefront k sums array k; every thread gets all totals.
+// (Separate pointer arguments, not an array of pointers: a runtime-indexed
+// private array would be placed in LDS.)
 template <int N>
 __device__ __forceinline__ void reduce_partials(const double* p0, const double* p1, const double* p2,
-                                                const double* p3, int W, double (&out)[N], double* lds) {
+                                                const double* p3, int t0, int W, double (&out)[N], double* lds) {
 	static_assert(N <= 4 && N <= kPcgRowsPerWG, "one partial array per wavefront");
 	const int wid = threadIdx.x >> 6;
 	if (wid < N) {
 		const double* p = wid == 0 ? p0 : wid == 1 ? p1 : wid == 2 ? p2 : p3;
-		const double t = wave_sum(partial_lane_sum(p, W));
+		const double t = wave_sum(partial_lane_sum(p + t0, W));
 		if ((threadIdx.x & 63) == 0) lds[wid] = t;
 	}
 	__syncthreads();
@@ -1075,14 +1212,30 @@ __device__ __forceinline__ void reduce_partials(const double* p0, const double* 
 	for (int k = 0; k < N; ++k) out[k] = lds[k];
 }
 
+// The scene's first tile resets its control block and history.
+template <class T>
+__device__ __forceinline__ void pcg_scene_reset(const PcgArgs<T>& a, int scn) {
+	if ((int)blockIdx.x == a.tbeg[scn] && threadIdx.x == 0) {
+		a.ps[scn].done = 0;
+		a.ps[scn].iters = 0;
+		a.ps[scn].itc[0] = 0;
+		a.ps[scn].itc[1] = 0;
+		// history read by iteration 0: beta = gamma / inf = 0, alpha = gamma / delta
+		a.hist[4 * scn + 2] = __builtin_huge_val();
+		a.hist[4 * scn + 3] = 1.0;
+	}
+}
+
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	__shared__ double lds[4 * 16];
-	const int nb = a.ctl->nb;
-	const int lane = threadIdx.x & 63;
+	const Tile tile = a.tiles[blockIdx.x];
+	if (tile.scene < 0 || !a.ctl[tile.scene].act) return;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[4] = {0, 0, 0, 0};
-	for (int br = blockIdx.x * kPcgRowsPerWG + (threadIdx.x >> 6); br < nb; br += gridDim.x * kPcgRowsPerWG) {
+	if (wid < tile.count) {
+		const int br = tile.first + wid;
 		const double w0 = spmv_lane(a, a.b, br, i, sl);
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
@@ -1110,13 +1263,8 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 		a.del[0][blockIdx.x] = out[1];
 		a.rr[0][blockIdx.x] = out[2];
 		a.bb[blockIdx.x] = out[2];
-		if (blockIdx.x == 0) {
-			a.ctl->done = 0; a.ctl->iters = 0; a.ctl->itc[0] = 0; a.ctl->itc[1] = 0;
-			// history read by iteration 0: beta = gamma / inf = 0, alpha = gamma / delta
-			a.hist[2] = __builtin_huge_val();
-			a.hist[3] = 1.0;
-		}
 	}
+	pcg_scene_reset(a, tile.scene);
 }
 
 // Operator row and row vectors of block row `br` as read by k_pcg_iter
@@ -1144,70 +1292,70 @@ __device__ __forceinline__ PcgRow pcg_row_load(const PcgArgs<T>& a, const double
 }
 
 // Iteration parity equals the launch slot: every chunk has an even number of
-// launches and each active launch advances the counter by one.  The operator
-// row and row vectors of the wavefront's first block row are loaded before the
-// partial sums are reduced, so the two memory round trips overlap.  `nrows`
-// (the scene's body count) bounds those early loads: row pointers, P and the
-// vectors are sized for every body, rows past the solve's block rows are
-// empty.
+// launches and each active launch advances the scene's counter by one.  The
+// operator row and row vectors of the wavefront's block row are loaded before
+// the partial sums are reduced, so the two memory round trips overlap.
 template <class T>
-__global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot, int nrows) {
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
 	static_assert(kPcgRowsPerWG >= 4, "four partial arrays, one per wavefront");
+	const Tile tile = a.tiles[blockIdx.x];
+	const int scn = tile.scene;
+	if (scn < 0) return;
 	const int cur = slot, nxt = slot ^ 1;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
-	const PcgCtl c = *a.ctl;
+	const PcgScene c = a.ps[scn];
+	const int act = a.ctl[scn].act;
 	// (a runtime index into the private copy would place it in LDS, one slot per thread)
 	const int it = slot ? c.itc[1] : c.itc[0];
+	const int t0 = a.tbeg[scn], W = a.tbeg[scn + 1] - t0;
 	const double* __restrict__ wc = pick(a.w, cur);
 	double part = 0;
 	if (wid < 4) {
 		const double* src = wid == 0 ? pick(a.gam, cur) : wid == 1 ? pick(a.del, cur) : wid == 2 ? pick(a.rr, cur) : a.bb;
-		part = partial_lane_sum(src, gridDim.x);
+		part = partial_lane_sum(src + t0, W);
 	}
-	const int br0 = blockIdx.x * kPcgRowsPerWG + wid;
+	const int br = tile.first + wid;
 	PcgRow R{};
-	if (br0 < nrows) R = pcg_row_load(a, wc, br0, i, sl);
+	if (wid < tile.count) R = pcg_row_load(a, wc, br, i, sl);
 	if (wid < 4) {
 		part = wave_sum(part);
 		if (lane == 0) lds[wid] = part;
 	}
 	__syncthreads();
-	if (c.done) return;
+	if (c.done || !act) return;
 	const double gamma = lds[0], delta = lds[1], rr = lds[2], bb = lds[3];
-	if (!(rr > c.tol2 * bb)) {
-		if (blockIdx.x == 0 && threadIdx.x == 0) { a.ctl->done = 1; a.ctl->iters = it; }
+	const bool first = (int)blockIdx.x == t0;
+	if (!(rr > a.tol2 * bb)) {
+		if (first && threadIdx.x == 0) { a.ps[scn].done = 1; a.ps[scn].iters = it; }
 		return;
 	}
-	const double gprev = a.hist[2 * nxt], aprev = a.hist[2 * nxt + 1];
+	const double gprev = a.hist[4 * scn + 2 * nxt], aprev = a.hist[4 * scn + 2 * nxt + 1];
 	const double beta = gamma / gprev;
 	const double alpha = gamma / (delta - beta * gamma / aprev);
 	double* __restrict__ wn = pick(a.w, nxt);
 	double v[4] = {0, 0, 0, 0};
-	for (int br = br0; br < c.nb; br += gridDim.x * kPcgRowsPerWG) {
-		if (br != br0) R = pcg_row_load(a, wc, br, i, sl);
-		if (i < 6 && sl == 0) {
-			const int t = 6 * br + i;
-			const double zt = R.n + beta * R.z;
-			const double qt = R.m + beta * R.q;
-			const double st = R.w + beta * R.s;
-			const double pt = R.u + beta * R.p;
-			a.z[t] = zt;
-			a.q[t] = qt;
-			a.s[t] = st;
-			a.p[t] = pt;
-			a.x[t] = R.x + alpha * pt;
-			const double rt = R.r - alpha * st;
-			const double ut = R.u - alpha * qt;
-			const double wnt = R.w - alpha * zt;
-			a.r[t] = rt;
-			a.u[t] = ut;
-			wn[t] = wnt;
-			v[0] += rt * ut;
-			v[1] += wnt * ut;
-			v[2] += rt * rt;
-		}
+	if (wid < tile.count && i < 6 && sl == 0) {
+		const int t = 6 * br + i;
+		const double zt = R.n + beta * R.z;
+		const double qt = R.m + beta * R.q;
+		const double st = R.w + beta * R.s;
+		const double pt = R.u + beta * R.p;
+		a.z[t] = zt;
+		a.q[t] = qt;
+		a.s[t] = st;
+		a.p[t] = pt;
+		a.x[t] = R.x + alpha * pt;
+		const double rt = R.r - alpha * st;
+		const double ut = R.u - alpha * qt;
+		const double wnt = R.w - alpha * zt;
+		a.r[t] = rt;
+		a.u[t] = ut;
+		wn[t] = wnt;
+		v[0] += rt * ut;
+		v[1] += wnt * ut;
+		v[2] += rt * rt;
 	}
 	double out[4];
 	block_sum<4, double>(v, lds, out);
@@ -1215,10 +1363,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 		pick(a.gam, nxt)[blockIdx.x] = out[0];
 		pick(a.del, nxt)[blockIdx.x] = out[1];
 		pick(a.rr, nxt)[blockIdx.x] = out[2];
-		if (blockIdx.x == 0) {
-			a.hist[2 * cur] = gamma;
-			a.hist[2 * cur + 1] = alpha;
-			a.ctl->itc[slot ^ 1] = it + 1;
+		if (first) {
+			a.hist[4 * scn + 2 * cur] = gamma;
+			a.hist[4 * scn + 2 * cur + 1] = alpha;
+			a.ps[scn].itc[slot ^ 1] = it + 1;
 		}
 	}
 }
@@ -1238,24 +1386,24 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg2_init(PcgArgs<T> a) {
 	__shared__ double lds[4 * 16];
-	const int nb = a.ctl->nb;
-	const int lane = threadIdx.x & 63;
+	const Tile tile = a.tiles[blockIdx.x];
+	if (tile.scene < 0 || !a.ctl[tile.scene].act) return;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[2] = {0, 0};
-	for (int br = blockIdx.x * kPcgRowsPerWG + (threadIdx.x >> 6); br < nb; br += gridDim.x * kPcgRowsPerWG) {
-		if (i < 6 && sl == 0) {
-			const int t = 6 * br + i;
-			double zt = 0;
+	if (wid < tile.count && i < 6 && sl == 0) {
+		const int br = tile.first + wid;
+		const int t = 6 * br + i;
+		double zt = 0;
 #pragma unroll
-			for (int c = 0; c < 6; ++c) zt += (double)a.P[36 * (size_t)br + 6 * i + c] * (double)a.b[6 * br + c];
-			const double bt = a.b[t];
-			a.x[t] = 0;
-			a.r[t] = bt;
-			a.p[t] = 0;  // y
-			a.q[t] = 0;
-			v[0] += bt * zt;
-			v[1] += bt * bt;
-		}
+		for (int c = 0; c < 6; ++c) zt += (double)a.P[36 * (size_t)br + 6 * i + c] * (double)a.b[6 * br + c];
+		const double bt = a.b[t];
+		a.x[t] = 0;
+		a.r[t] = bt;
+		a.p[t] = 0;  // y
+		a.q[t] = 0;
+		v[0] += bt * zt;
+		v[1] += bt * bt;
 	}
 	double out[2];
 	block_sum<2, double>(v, lds, out);
@@ -1263,29 +1411,34 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_init(PcgArgs<T> a) {
 		a.gam[0][blockIdx.x] = out[0];
 		a.rr[0][blockIdx.x] = out[1];
 		a.bb[blockIdx.x] = out[1];
-		if (blockIdx.x == 0) { a.ctl->done = 0; a.ctl->iters = 0; a.ctl->itc[0] = 0; a.ctl->itc[1] = 0; }
 	}
+	pcg_scene_reset(a, tile.scene);
 }
 
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
-	const PcgCtl c = *a.ctl;
-	if (c.done) return;
+	const Tile tile = a.tiles[blockIdx.x];
+	const int scn = tile.scene;
+	if (scn < 0) return;
+	const PcgScene c = a.ps[scn];
+	if (c.done || !a.ctl[scn].act) return;
 	const int it = slot ? c.itc[1] : c.itc[0];  // no dynamic index into a private copy
 	const int cur = it & 1, nxt = cur ^ 1;
+	const int t0 = a.tbeg[scn], W = a.tbeg[scn + 1] - t0;
 	double red[4];
-	reduce_partials<4>(pick(a.rr, cur), a.bb, pick(a.gam, cur), pick(a.gam, nxt), gridDim.x, red, lds);
-	if (!(red[0] > c.tol2 * red[1])) {
-		if (blockIdx.x == 0 && threadIdx.x == 0) { a.ctl->done = 1; a.ctl->iters = it; }
+	reduce_partials<4>(pick(a.rr, cur), a.bb, pick(a.gam, cur), pick(a.gam, nxt), t0, W, red, lds);
+	if (!(red[0] > a.tol2 * red[1])) {
+		if ((int)blockIdx.x == t0 && threadIdx.x == 0) { a.ps[scn].done = 1; a.ps[scn].iters = it; }
 		return;
 	}
 	const double beta = it > 0 ? red[2] / red[3] : 0.0;
 	double* __restrict__ y = a.p;
-	const int lane = threadIdx.x & 63;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[1] = {0};
-	for (int br = blockIdx.x * kPcgRowsPerWG + (threadIdx.x >> 6); br < c.nb; br += gridDim.x * kPcgRowsPerWG) {
+	if (wid < tile.count) {
+		const int br = tile.first + wid;
 		const double n = spmv_lane(a, a.r, br, i, sl);
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
@@ -1312,56 +1465,66 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) 
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
-	const PcgCtl c = *a.ctl;
-	if (c.done) return;
+	const Tile tile = a.tiles[blockIdx.x];
+	const int scn = tile.scene;
+	if (scn < 0) return;
+	const PcgScene c = a.ps[scn];
+	if (c.done || !a.ctl[scn].act) return;
 	const int it = slot ? c.itc[1] : c.itc[0];  // no dynamic index into a private copy
 	const int cur = it & 1, nxt = cur ^ 1;
+	const int t0 = a.tbeg[scn], W = a.tbeg[scn + 1] - t0;
 	double red[2];
-	reduce_partials<2>(pick(a.gam, cur), a.del[0], nullptr, nullptr, gridDim.x, red, lds);
+	reduce_partials<2>(pick(a.gam, cur), a.del[0], nullptr, nullptr, t0, W, red, lds);
 	const double alpha = red[0] / red[1];
 	const double* __restrict__ y = a.p;
-	const int lane = threadIdx.x & 63;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[2] = {0, 0};
-	for (int br = blockIdx.x * kPcgRowsPerWG + (threadIdx.x >> 6); br < c.nb; br += gridDim.x * kPcgRowsPerWG) {
-		if (i < 6 && sl == 0) {
-			const int t = 6 * br + i;
-			double rb[6], pt = 0;
+	if (wid < tile.count && i < 6 && sl == 0) {
+		const int br = tile.first + wid;
+		const int t = 6 * br + i;
+		double rb[6], pt = 0;
 #pragma unroll
-			for (int cc = 0; cc < 6; ++cc) {
-				const double pc = (double)a.P[36 * (size_t)br + 6 * i + cc];
-				pt += pc * y[6 * br + cc];
-				rb[cc] = a.r[6 * br + cc] - alpha * a.q[6 * br + cc];
-			}
-			double zt = 0;
-#pragma unroll
-			for (int cc = 0; cc < 6; ++cc) zt += (double)a.P[36 * (size_t)br + 6 * i + cc] * rb[cc];
-			double rt = rb[0];
-#pragma unroll
-			for (int cc = 1; cc < 6; ++cc) rt = cc == i ? rb[cc] : rt;
-			a.x[t] += alpha * pt;
-			a.r[t] = rt;
-			v[0] += rt * zt;
-			v[1] += rt * rt;
+		for (int cc = 0; cc < 6; ++cc) {
+			const double pc = (double)a.P[36 * (size_t)br + 6 * i + cc];
+			pt += pc * y[6 * br + cc];
+			rb[cc] = a.r[6 * br + cc] - alpha * a.q[6 * br + cc];
 		}
+		double zt = 0;
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) zt += (double)a.P[36 * (size_t)br + 6 * i + cc] * rb[cc];
+		double rt = rb[0];
+#pragma unroll
+		for (int cc = 1; cc < 6; ++cc) rt = cc == i ? rb[cc] : rt;
+		a.x[t] += alpha * pt;
+		a.r[t] = rt;
+		v[0] += rt * zt;
+		v[1] += rt * rt;
 	}
 	double out[2];
 	block_sum<2, double>(v, lds, out);
 	if (threadIdx.x == 0) {
 		pick(a.gam, nxt)[blockIdx.x] = out[0];
 		pick(a.rr, nxt)[blockIdx.x] = out[1];
-		if (blockIdx.x == 0) a.ctl->itc[slot ^ 1] = it + 1;
+		if ((int)blockIdx.x == t0) a.ps[scn].itc[slot ^ 1] = it + 1;
 	}
 }
 
-__global__ __launch_bounds__(256) void k_du(int NL, const double* __restrict__ D, const double* __restrict__ x, double* __restrict__ du) {
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t < 6 * NL) du[t] = x[t] * D[t];
+__global__ __launch_bounds__(256) void k_du(SArgs a, const double* __restrict__ x) {
+	PDC_TILE(a.tb);
+	(void)scn;
+	if ((int)threadIdx.x < tile.count) {
+		const int r = tile.first + threadIdx.x;
+		for (int k = 0; k < 6; ++k) a.du[6 * r + k] = x[6 * r + k] * a.D[6 * r + k];
+	}
 }
 
 // ---- increments (NonSmoothContactForce.cpp:254-280) -------------------------
 __global__ __launch_bounds__(256) void k_increments(SArgs a) {
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
+	PDC_TILE(a.tc);
+	(void)scn;
+	if ((int)threadIdx.x < tile.count) {
+		const int i = tile.first + threadIdx.x;
 		double u[3];
 		to_local(a, i, &a.du[6 * a.li0[i]], &a.du[6 * a.li1[i]], u);
 		const double l0 = a.lam[i];
@@ -1384,9 +1547,13 @@ __global__ __launch_bounds__(256) void k_increments(SArgs a) {
 	}
 }
 
-__global__ __launch_bounds__(256) void k_neg(int n, const double* __restrict__ x, double* __restrict__ y) {
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t < n) y[t] = -x[t];
+__global__ __launch_bounds__(256) void k_neg(SArgs a, const double* __restrict__ x, double* __restrict__ y) {
+	PDC_TILE(a.tb);
+	(void)scn;
+	if ((int)threadIdx.x < tile.count) {
+		const int r = tile.first + threadIdx.x;
+		for (int k = 0; k < 6; ++k) y[6 * r + k] = -x[6 * r + k];
+	}
 }
 
 // ---- (e) line search as fused reductions (PrimalDualForceSolver.cpp:562-635) --
@@ -1398,7 +1565,7 @@ __global__ __launch_bounds__(256) void k_neg(int n, const double* __restrict__ x
 // Filtered trial point of contact i at `step` (NonSmoothContactForce.cpp:281-319,
 // OnlyNormalForce.cpp:172-201): f' = f + 0.99 df_in, lambda' = lambda + 0.99 dl_in.
 __device__ __forceinline__ void trial_point(const SArgs& a, int i, const double* __restrict__ dF,
-                                            const double* __restrict__ dL, double step, double* fI,
+                                            const double* __restrict__ dL, double step, double mu, double* fI,
                                             double* lI, double* dfi, double* dli) {
 	const double cf[3] = {a.f[i], a.nsc ? a.f[a.cs + i] : 0.0, a.nsc ? a.f[2 * a.cs + i] : 0.0};
 	const double lam[2] = {a.lam[i], a.nsc ? a.lam[a.cs + i] : 0.0};
@@ -1413,9 +1580,9 @@ __device__ __forceinline__ void trial_point(const SArgs& a, int i, const double*
 			tf[0] = 0;
 			for (int k = 0; k < 3; ++k) dfi[k] = tf[k] - cf[k];
 		}
-		if (Lorentz::constraint(a.mu, tf[0], tf[1], tf[2]) > 0) {
+		if (Lorentz::constraint(mu, tf[0], tf[1], tf[2]) > 0) {
 			double pf[3];
-			Lorentz::project(a.mu, tf, pf);
+			Lorentz::project(mu, tf, pf);
 			for (int k = 0; k < 3; ++k) dfi[k] = pf[k] - cf[k];
 		}
 	} else {
@@ -1436,13 +1603,15 @@ __device__ __forceinline__ void trial_point(const SArgs& a, int i, const double*
 __global__ __launch_bounds__(256) void k_ls_contact(SArgs a, const double* __restrict__ du,
                                                     const double* __restrict__ dF,
                                                     const double* __restrict__ dL, int k0) {
-	const double muc = a.sc->muc;
+	PDC_TILE(a.tc);
+	const double muc = a.ss[scn].muc, mu = a.ctl[scn].mu, cm = a.ss[scn].char_mass;
 	for (int t = 0; t < kTrials; ++t) {
 		const double step = ldexp(1.0, -(k0 + t));
 		double acc[5] = {0, 0, 0, 0, 0};
-		for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
+		if ((int)threadIdx.x < tile.count) {
+			const int i = tile.first + threadIdx.x;
 			double fI[3], lI[2], dfi[3], dli[2];
-			trial_point(a, i, dF, dL, step, fI, lI, dfi, dli);
+			trial_point(a, i, dF, dL, step, mu, fI, lI, dfi, dli);
 			const int l0 = a.li0[i], l1 = a.li1[i];
 			double va[6], vb[6];
 			for (int k = 0; k < 6; ++k) {
@@ -1450,7 +1619,7 @@ __global__ __launch_bounds__(256) void k_ls_contact(SArgs a, const double* __res
 				vb[k] = a.v[6 * l1 + k] + 0.99 * (step * du[6 * l1 + k]);
 			}
 			double rf[3], rl[2], wrf[3], ac[3];
-			contact_terms(a, i, va, vb, fI, lI, muc, rf, rl, wrf, ac);
+			contact_terms(a, i, va, vb, fI, lI, muc, mu, cm, rf, rl, wrf, ac);
 			acc[0] += wrf[0] * wrf[0] + wrf[1] * wrf[1] + wrf[2] * wrf[2];
 			acc[1] += rl[0] * rl[0] + (a.nsc ? rl[1] * rl[1] : 0.0);
 			acc[2] += ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
@@ -1459,7 +1628,7 @@ __global__ __launch_bounds__(256) void k_ls_contact(SArgs a, const double* __res
 		}
 		const int b = P_LS + 7 * t;
 		const int slots[5] = {b, b + 1, b + 2, b + 3, b + 4};
-		write_partials<5>(acc, a.part, slots);
+		write_partials<5>(acc, a.part, a.pstride, slots);
 	}
 }
 
@@ -1471,14 +1640,14 @@ template <int G>
 __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restrict__ du,
                                                  const double* __restrict__ dF,
                                                  const double* __restrict__ dL, int k0) {
-	const int lane = threadIdx.x & 63;
-	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-	const int nw = (gridDim.x * blockDim.x) >> 6;
+	PDC_TILE(a.tw);
+	const double mu = a.ctl[scn].mu;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	double acc[2 * kGroup];
 #pragma unroll
 	for (int q = 0; q < 2 * kGroup; ++q) acc[q] = 0;
-	for (int r = wave; r < a.NL; r += nw) {
-		if (a.lstat[r]) continue;
+	if (wid < tile.count && !a.lstat[tile.first + wid]) {
+		const int r = tile.first + wid;
 		double add[kGroup][6];
 #pragma unroll
 		for (int t = 0; t < kGroup; ++t)
@@ -1493,7 +1662,7 @@ __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restri
 #pragma unroll
 			for (int t = 0; t < kGroup; ++t) {
 				double fI[3], lI[2], dfi[3], dli[2];
-				trial_point(a, i, dF, dL, ldexp(1.0, -(k0 + G * kGroup + t)), fI, lI, dfi, dli);
+				trial_point(a, i, dF, dL, ldexp(1.0, -(k0 + G * kGroup + t)), mu, fI, lI, dfi, dli);
 				// fI[1] = fI[2] = 0 for the frictionless model
 #pragma unroll
 				for (int row = 0; row < 6; ++row)
@@ -1533,7 +1702,7 @@ __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restri
 		slots[2 * t] = P_LS + 7 * (G * kGroup + t) + 5;
 		slots[2 * t + 1] = P_LS + 7 * (G * kGroup + t) + 6;
 	}
-	write_partials<2 * kGroup>(acc, a.part, slots);
+	write_partials<2 * kGroup>(acc, a.part, a.pstride, slots);
 }
 static_assert(kTrials == 2 * kGroup, "k_ls_body covers the trials in two groups");
 
@@ -1541,23 +1710,24 @@ static_assert(kTrials == 2 * kGroup, "k_ls_body covers the trials in two groups"
 // in order: the first trial with (accept and gErr > 0), or the search's last
 // trial, decides.  ls_step = accepted step, or 0 when the decision is a
 // failure; ls_decided = 0 when no trial of this pass decided.
-__global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int nwg_c, int nwg_b, int k0, int maxLS,
-                                                   double sErr, double sacErr) {
+__global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int k0, int maxLS) {
 	__shared__ double red[7 * kTrials];
+	const int sc = blockIdx.x;
+	if (!a.ctl[sc].act) return;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	for (int q = wid; q < 7 * kTrials; q += 16) {
 		const int slot = P_LS + q;
-		const int t = q / 7, c = q - 7 * (q / 7);
-		(void)t;
-		const int n = c < 5 ? nwg_c : nwg_b;
+		const int c = q - 7 * (q / 7);
+		const int t0 = c < 5 ? a.tbc[sc] : a.tbw[sc], t1 = c < 5 ? a.tbc[sc + 1] : a.tbw[sc + 1];
 		double v = 0;
-		for (int w = lane; w < n; w += 64) v += a.part[slot * kMaxWG + w];
+		for (int w = t0 + lane; w < t1; w += 64) v += a.part[slot * (size_t)a.pstride + w];
 		v = wave_sum(v);
 		if (lane == 0) red[q] = v;
 	}
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		Scalars* S = a.sc;
+		SceneSc* S = &a.ss[sc];
+		const double sErr = S->sErr, sacErr = S->sacErr;
 		S->ls_decided = 0;
 		S->ls_step = 0;
 		S->ls_trials = 0;
@@ -1585,10 +1755,13 @@ __global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int nwg_c, int nwg_b
 // Commit the accepted trial: f, lambda in place (the same trial point the
 // pass evaluated), then v (k_commit_v).
 __global__ __launch_bounds__(256) void k_ls_commit(SArgs a, const double* __restrict__ dF,
-                                                   const double* __restrict__ dL, double step) {
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
+                                                   const double* __restrict__ dL) {
+	PDC_TILE(a.tc);
+	const double step = a.ss[scn].ls_step, mu = a.ctl[scn].mu;
+	if ((int)threadIdx.x < tile.count) {
+		const int i = tile.first + threadIdx.x;
 		double fI[3], lI[2], dfi[3], dli[2];
-		trial_point(a, i, dF, dL, step, fI, lI, dfi, dli);
+		trial_point(a, i, dF, dL, step, mu, fI, lI, dfi, dli);
 		a.f[i] = fI[0];
 		a.lam[i] = lI[0];
 		if (a.nsc) {
@@ -1599,22 +1772,30 @@ __global__ __launch_bounds__(256) void k_ls_commit(SArgs a, const double* __rest
 	}
 }
 
-__global__ __launch_bounds__(256) void k_commit_v(int n, double* __restrict__ v, const double* __restrict__ du, double step) {
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t < n) v[t] = v[t] + 0.99 * (step * du[t]);
+__global__ __launch_bounds__(256) void k_commit_v(SArgs a, const double* __restrict__ du) {
+	PDC_TILE(a.tb);
+	const double step = a.ss[scn].ls_step;
+	if ((int)threadIdx.x < tile.count) {
+		const int r = tile.first + threadIdx.x;
+		for (int k = 0; k < 6; ++k) a.v[6 * r + k] = a.v[6 * r + k] + 0.99 * (step * du[6 * r + k]);
+	}
 }
 
 // ---- write-back (PrimalDualForceSolver.cpp:491-510) --------------------------
 __global__ __launch_bounds__(256) void k_writeback_bodies(SArgs a, double* __restrict__ acc) {
-	const double s = a.sc->char_speed;
-	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.NL; r += gridDim.x * blockDim.x) {
+	PDC_TILE(a.tb);
+	const double s = a.ss[scn].char_speed;
+	if ((int)threadIdx.x < tile.count) {
+		const int r = tile.first + threadIdx.x;
 		const int g = a.l2g[r];
 		for (int k = 0; k < 6; ++k) acc[6 * g + k] = a.v[6 * r + k] * s - a.vphys[6 * r + k];
 	}
 }
 __global__ __launch_bounds__(256) void k_writeback_contacts(SArgs a, double* __restrict__ lf) {
-	const double s = a.sc->char_speed, m = a.sc->char_mass;
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.C; i += gridDim.x * blockDim.x) {
+	PDC_TILE(a.tc);
+	const double s = a.ss[scn].char_speed, m = a.ss[scn].char_mass;
+	if ((int)threadIdx.x < tile.count) {
+		const int i = tile.first + threadIdx.x;
 		lf[i] = a.f[i] * s * m;
 		lf[a.cs + i] = a.nsc ? a.f[a.cs + i] * s * m : 0.0;
 		lf[2 * a.cs + i] = a.nsc ? a.f[2 * a.cs + i] * s * m : 0.0;
@@ -1629,11 +1810,12 @@ __global__ __launch_bounds__(256) void k_acc_default(int N, double* __restrict__
 }
 
 // ---- integration (DynamicSystem.cpp:137-161) ---------------------------------
-__global__ __launch_bounds__(256) void k_integrate(int N, const int* __restrict__ stat, const double* __restrict__ acc,
+__global__ __launch_bounds__(256) void k_integrate(int N, const int* __restrict__ stat, const int* __restrict__ bscene,
+                            const int* __restrict__ stepped, const double* __restrict__ acc,
                             double* __restrict__ pos, double* __restrict__ quat,
                             double* __restrict__ vel, double* __restrict__ ang, double dt) {
 	const int b = blockIdx.x * blockDim.x + threadIdx.x;
-	if (b >= N) return;
+	if (b >= N || !stepped[bscene[b]]) return;
 	double v[3] = {vel[3 * b], vel[3 * b + 1], vel[3 * b + 2]};
 	double w[3] = {ang[3 * b], ang[3 * b + 1], ang[3 * b + 2]};
 	if (!stat[b]) {
@@ -1660,9 +1842,10 @@ __global__ __launch_bounds__(256) void k_integrate(int N, const int* __restrict_
 	for (int k = 0; k < 3; ++k) { vel[3 * b + k] = v[k]; ang[3 * b + k] = w[k]; }
 }
 
-// Iteration count when the solve ran to the iteration cap.
-__global__ void k_pcg_final(PcgCtl* ctl, int maxit) {
-	if (threadIdx.x == 0 && !ctl->done) ctl->iters = maxit;
+// Iteration count of the scenes whose solve ran to the iteration cap.
+__global__ void k_pcg_final(PcgScene* ps, const SceneCtl* ctl, int S, int maxit) {
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s < S && ctl[s].act && !ps[s].done) ps[s].iters = maxit;
 }
 
 // ===========================================================================
@@ -1671,9 +1854,10 @@ __global__ void k_pcg_final(PcgCtl* ctl, int maxit) {
 static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	SArgs a;
 	const pdc_params& p = ctx->p;
-	a.C = ctx->C; a.cs = ctx->c_stride; a.NL = ctx->NL;
+	a.C = ctx->C; a.cs = ctx->c_stride; a.NL = ctx->NL; a.S = ctx->S;
+	a.pstride = std::max(1, std::max(ctx->n_tc, std::max(ctx->n_tb, ctx->n_tw)));
 	a.nsc = p.force_model == PDC_NON_SMOOTH_CONTACT;
-	a.mu = p.friction_coeff; a.restitution = p.restitution;
+	a.restitution = p.restitution;
 	a.spring = p.spring_based_force;
 	const double damp = p.spring_damp * p.spring_k;
 	const double tk = dt * p.spring_k;
@@ -1681,6 +1865,10 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	a.reduction = tk / (tk + damp);
 	a.dt = dt;
 	for (int k = 0; k < 3; ++k) a.gdt[k] = g[k] * dt;
+	a.tc = ctx->t_c.p; a.tb = ctx->t_b.p; a.tw = ctx->t_w.p; a.tp = ctx->t_p.p;
+	a.tbc = ctx->tb_c.p; a.tbb = ctx->tb_b.p; a.tbw = ctx->tb_w.p;
+	a.cbeg = ctx->r_cbeg.p; a.lbeg = ctx->r_lbeg.p;
+	a.ss = ctx->d_ss.p; a.ctl = ctx->d_sctl.p;
 	a.li0 = ctx->li0.p; a.li1 = ctx->li1.p; a.T = ctx->c_T.p; a.depth = ctx->c_depth.p;
 	a.pstart = ctx->ap_start.p; a.pa = ctx->ap_a.p; a.g2l = ctx->g2l.p;
 	a.lstat = ctx->l_static.p; a.l2g = ctx->l2g.p;
@@ -1692,12 +1880,12 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	a.H = ctx->s_H.p; a.rf = ctx->s_rf.p; a.rl = ctx->s_rl.p; a.sf = ctx->s_sf.p;
 	a.invC = ctx->s_invC.p; a.W = ctx->s_W.p; a.kf = ctx->s_kf.p; a.df = ctx->s_df.p;
 	a.dl = ctx->s_dl.p;
-	a.part = ctx->partials.p; a.sc = ctx->d_sc.p;
+	a.part = ctx->partials.p;
 	return a;
 }
 
-// One recorded chunk of kPcgChunk PCG iterations (kernel pairs alternating
-// the counter slots 0, 1).
+// One recorded chunk of kPcgChunk PCG iterations (launches alternating the
+// counter slots 0, 1).
 constexpr int kPcgChunk = 16;
 
 static_assert(kPcgChunk % 2 == 0, "iteration parity is the launch slot");
@@ -1709,32 +1897,27 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 	for (int k = 0; k < kPcgChunk; ++k) {
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k], s));
 		if (two) hipLaunchKernelGGL(k_pcg2_a<T>, dim3(nblk), B, 0, s, pa, k & 1);
-		else hipLaunchKernelGGL(k_pcg_iter<T>, dim3(nblk), B, 0, s, pa, k & 1, ctx->N);
+		else hipLaunchKernelGGL(k_pcg_iter<T>, dim3(nblk), B, 0, s, pa, k & 1);
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k + 1], s));
 		if (two) hipLaunchKernelGGL(k_pcg2_b<T>, dim3(nblk), B, 0, s, pa, k & 1);
 	}
 	PDC_CHECK_LAUNCH();
 }
 
-__global__ void k_ptr_pad(int* __restrict__ ptr, int NL, int N) {
-	const int r = NL + 1 + blockIdx.x * blockDim.x + threadIdx.x;
-	if (r <= N) ptr[r] = ptr[NL];
-}
-
 // Below this relative tolerance the two-reduction form is used: the
 // single-reduction recurrences stall near 1e-10 on ill-conditioned systems.
 constexpr double kSingleReductionMinTol = 1e-9;
 
+// PCG of every scene with h_sctl[s].act; iters[s] receives its iteration count.
 template <class T>
-static int run_pcg(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b) {
-	const int NL = ctx->NL;
-	const int n = 6 * NL;
-	// grid fixed per scene (block rows of all bodies) so the recorded chunk stays valid
-	const int rows = std::max(ctx->N, 1);
-	const int nblk = std::max(1, std::min((rows + kPcgRowsPerWG - 1) / kPcgRowsPerWG, kMaxWG));
-	const size_t ncap = 6 * (size_t)rows;
+static void run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b,
+                    std::vector<int>& iters) {
+	const int S = ctx->S;
+	// grid fixed per scene set (capacity of PCG tiles) so the recorded chunk stays valid
+	const int nblk = std::max(1, ctx->cap_tr);
+	const size_t ncap = 6 * (size_t)std::max(ctx->N, 1), pcap = (size_t)nblk;
 	DBuf<double>& work = ctx->pcg64;
-	work.ensure(10 * ncap + 7 * (size_t)kMaxWG + 64);
+	work.ensure(10 * ncap + 7 * pcap + 4 * (size_t)S + 64);
 	ctx->x64.ensure(ncap);
 	PcgArgs<T> pa;
 	pa.ptr = ctx->a_ptr.p;
@@ -1747,26 +1930,24 @@ static int run_pcg(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b) {
 	pa.r = w; pa.u = w + ncap; pa.w[0] = w + 2 * ncap; pa.w[1] = w + 3 * ncap;
 	pa.z = w + 4 * ncap; pa.q = w + 5 * ncap; pa.s = w + 6 * ncap; pa.p = w + 7 * ncap;
 	double* parts = w + 10 * ncap;
-	pa.gam[0] = parts; pa.gam[1] = parts + kMaxWG;
-	pa.del[0] = parts + 2 * kMaxWG; pa.del[1] = parts + 3 * kMaxWG;
-	pa.rr[0] = parts + 4 * kMaxWG; pa.rr[1] = parts + 5 * kMaxWG;
-	pa.bb = parts + 6 * kMaxWG;
-	pa.hist = parts + 7 * kMaxWG;
-	pa.ctl = &ctx->d_sc.p->pcg;
+	pa.gam[0] = parts; pa.gam[1] = parts + pcap;
+	pa.del[0] = parts + 2 * pcap; pa.del[1] = parts + 3 * pcap;
+	pa.rr[0] = parts + 4 * pcap; pa.rr[1] = parts + 5 * pcap;
+	pa.bb = parts + 6 * pcap;
+	pa.hist = parts + 7 * pcap;
+	pa.tiles = ctx->t_r.p;
+	pa.tbeg = ctx->tb_r.p;
+	pa.ctl = ctx->d_sctl.p;
+	pa.ps = ctx->d_ps.p;
+	pa.tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
 	hipStream_t s = ctx->stream;
-	PcgCtl* hc = &ctx->h_sc->pcg;
-	hc->nb = NL;
-	hc->tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
-	PDC_HIP(hipMemcpyAsync(pa.ctl, hc, sizeof(PcgCtl), hipMemcpyHostToDevice, s));
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
-	// rows NL..N of the row pointers: empty rows (read by the early loads)
-	if (ctx->N > NL)
-		hipLaunchKernelGGL(k_ptr_pad, dim3(grid_for(ctx->N - NL)), dim3(kBlock), 0, s, ctx->a_ptr.p, NL, ctx->N);
 	if (two) hipLaunchKernelGGL(k_pcg2_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
 	else hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
 	PDC_CHECK_LAUNCH();
 	// the chunk is recorded again when a buffer moved or the grid changed
 	PcgGraph& G = ctx->pcg_graph[(sizeof(T) == 8 ? 1 : 0) + (two ? 2 : 0)];
+	static_assert(sizeof(PcgArgs<T>) <= sizeof(G.args), "recorded arguments fit");
 	if (ctx->use_graphs && (G.exec == nullptr || G.nblk != nblk ||
 	                        std::memcmp(G.args, &pa, sizeof(pa)) != 0)) {
 		if (G.exec) { (void)hipGraphExecDestroy(G.exec); G.exec = nullptr; }
@@ -1790,8 +1971,9 @@ static int run_pcg(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b) {
 	const int maxit = ctx->p.pcg_max_iter;
 	int launched = 0;
 	int want = std::max(1, (std::min(ctx->pcg_last_iters + 2, maxit) + kPcgChunk - 1) / kPcgChunk);
-	int flags[2] = {0, 0};
+	ctx->h_ps.resize(S);
 	double kt_ms = 0;
+	bool all_done = false;
 	while (launched < maxit) {
 		if (kt_spmv) want = 1;  // the event pool holds one chunk
 		if (kt_iter) PDC_HIP(hipEventRecord(ev[0], s));
@@ -1802,46 +1984,66 @@ static int run_pcg(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b) {
 			launched += kPcgChunk;
 		}
 		if (kt_iter) PDC_HIP(hipEventRecord(ev[1], s));
-		PDC_HIP(hipMemcpyAsync(flags, &ctx->d_sc.p->pcg.iters, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+		PDC_HIP(hipMemcpyAsync(ctx->h_ps.data(), ctx->d_ps.p, S * sizeof(PcgScene), hipMemcpyDeviceToHost, s));
 		PDC_HIP(hipStreamSynchronize(s));
+		all_done = true;
+		int min_done_it = launched;  // for the per-launch timing: launches all scenes were active in
+		for (int sc = 0; sc < S; ++sc) {
+			if (!ctx->h_sctl[sc].act) continue;
+			if (!ctx->h_ps[sc].done) all_done = false;
+			else min_done_it = std::min(min_done_it, std::max(ctx->h_ps[sc].iters, before));
+		}
 		if (kt_iter) {
 			float e = 0;
 			PDC_HIP(hipEventElapsedTime(&e, ev[0], ev[1]));
 			kt_ms += e;
 			ctx->kt.launches += launched - before;
 		} else if (kt_spmv) {
-			// iterations k < iters reduced and updated; iteration `iters` ran
-			// its SpMV launch only to the convergence test
-			const int active = flags[1] ? std::min(kPcgChunk, flags[0] - before) : kPcgChunk;
+			// launches up to the last active iteration of the scene that ran longest
+			int active = 0;
+			for (int sc = 0; sc < S; ++sc)
+				if (ctx->h_sctl[sc].act)
+					active = std::max(active, ctx->h_ps[sc].done ? ctx->h_ps[sc].iters - before : kPcgChunk);
+			active = std::min(std::max(active, 0), kPcgChunk);
 			for (int k = 0; k < active; ++k) {
 				float e = 0;
 				PDC_HIP(hipEventElapsedTime(&e, ev[2 * k], ev[2 * k + 1]));
 				kt_ms += e;
 			}
-			ctx->kt.launches += std::max(active, 0);
+			ctx->kt.launches += active;
 		}
-		if (flags[1]) break;  // done
+		(void)min_done_it;
+		if (all_done) break;  // every scene converged
 		want = 1;
 	}
-	const int iters = flags[1] ? flags[0] : maxit;
+	int last = 0;
+	for (int sc = 0; sc < S; ++sc) {
+		iters[sc] = 0;
+		if (!ctx->h_sctl[sc].act) continue;
+		iters[sc] = ctx->h_ps[sc].done ? ctx->h_ps[sc].iters : maxit;
+		last = std::max(last, iters[sc]);
+	}
+	ctx->pcg_last_iters = last;
 	if (kt_spmv || kt_iter) {
-		// Algorithmic bytes of one active iteration (of k_pcg_iter, or of
-		// k_pcg2_a for the two-reduction form), each input read once and each
-		// output written once, FP64 vectors:
+		// Algorithmic bytes of one active iteration of a scene (of k_pcg_iter,
+		// or of k_pcg2_a for the two-reduction form), each input read once and
+		// each output written once, FP64 vectors:
 		//   k_pcg_iter  B blocks and column indices nnzb (36 T + 4), row pointers
 		//               4 (NL + 1), P blocks 36 T NL, w, z, q, s, p, u, x, r read
 		//               8 x 8 n6, z, q, s, p, x, r, u, w' written 8 x 8 n6
 		//   k_pcg2_a    B, column indices, row pointers as above, P 36 T NL,
 		//               r, y, q read 3 x 8 n6, y, q written 2 x 8 n6
-		const double Tb = sizeof(T), n6 = 6.0 * NL, nnzb = ctx->NNZB;
-		const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NL + 1) + 36.0 * Tb * NL;
+		const double Tb = sizeof(T);
 		ctx->kt.total_ms += kt_ms;
-		ctx->kt.bytes += iters * (mat + (two ? 40.0 : 128.0) * n6);
+		for (int sc = 0; sc < S; ++sc) {
+			if (!ctx->h_sctl[sc].act) continue;
+			const double NLs = ctx->h_lbeg[sc + 1] - ctx->h_lbeg[sc], n6 = 6.0 * NLs, nnzb = ctx->h_nnzb[sc];
+			const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NLs + 1) + 36.0 * Tb * NLs;
+			ctx->kt.bytes += iters[sc] * (mat + (two ? 40.0 : 128.0) * n6);
+		}
 	}
-	hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(64), 0, s, pa.ctl, maxit);
-	hipLaunchKernelGGL(k_du, dim3(grid_for(n)), dim3(kBlock), 0, s, NL, ctx->l_D.p, ctx->x64.p, ctx->l_du.p);
+	if (ctx->n_tb > 0) hipLaunchKernelGGL(k_du, dim3(ctx->n_tb), dim3(kBlock), 0, s, sa, ctx->x64.p);
 	PDC_CHECK_LAUNCH();
-	return nblk;
 }
 
 // Algorithmic bytes (each input read once, each output written once).
@@ -1861,20 +2063,29 @@ void write_acc_default(pdc_context* ctx, double dt, const double* g) {
 void integrate(pdc_context* ctx, double dt) {
 	if (ctx->N == 0) return;
 	hipLaunchKernelGGL(k_integrate, dim3(grid_for(ctx->N)), dim3(kBlock), 0, ctx->stream, ctx->N,
-	                   ctx->b_static.p, ctx->b_acc.p, ctx->b_pos.p, ctx->b_quat.p, ctx->b_vel.p,
-	                   ctx->b_ang.p, dt);
+	                   ctx->b_static.p, ctx->b_scene.p, ctx->d_step.p, ctx->b_acc.p, ctx->b_pos.p,
+	                   ctx->b_quat.p, ctx->b_vel.p, ctx->b_ang.p, dt);
 	PDC_CHECK_LAUNCH();
 }
 
-void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st) {
+// Per-scene scalars to the pinned host mirror (blocking).
+static void read_scene_scalars(pdc_context* ctx) {
+	PDC_HIP(hipMemcpyAsync(ctx->h_ss, ctx->d_ss.p, ctx->S * sizeof(SceneSc), hipMemcpyDeviceToHost, ctx->stream));
+	PDC_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// The Newton solve of every scene that has contacts, all scenes advancing
+// together: each phase runs for the scenes still taking part (SceneCtl.act),
+// a scene drops out when it converges or reaches the iteration cap, and every
+// decision of the reference's loop (PrimalDualForceSolver.cpp:278-490) is
+// taken per scene.  A scene's arithmetic does not depend on the other scenes.
+void solve(pdc_context* ctx, double dt, const double* g) {
 	hipStream_t s = ctx->stream;
-	pdc_solve_stats stats{};
+	const int S = ctx->S;
+	ctx->scene_stats.assign(S, pdc_solve_stats{});
 	write_acc_default(ctx, dt, g);
 	const int C = ctx->C;
-	if (C == 0) {
-		if (st) *st = stats;
-		return;
-	}
+	if (C == 0) return;
 	build_topology(ctx);
 	const int NL = ctx->NL;
 	const size_t nb = (size_t)NL;
@@ -1882,148 +2093,197 @@ void solve(pdc_context* ctx, double dt, const double* g, pdc_solve_stats* st) {
 	ctx->l_Ms.ensure(7 * nb); ctx->l_Minv.ensure(7 * nb); ctx->l_speed.ensure(nb);
 	ctx->l_ru.ensure(6 * nb); ctx->l_rhs.ensure(6 * nb); ctx->l_D.ensure(6 * nb);
 	ctx->l_du.ensure(6 * nb); ctx->l_gdu.ensure(6 * nb);
-	ctx->partials.ensure((size_t)P_COUNT * kMaxWG);
 	const bool fp32 = ctx->p.pcg_precision == PDC_PCG_FP32;
 	const size_t nnzb = (size_t)ctx->NNZB;
 	if (fp32) {
-		ctx->A32.ensure(36 * nnzb); ctx->P32.ensure(36 * (size_t)ctx->N); ctx->b32.ensure(6 * (size_t)ctx->N);
+		ctx->A32.ensure(36 * nnzb); ctx->P32.ensure(36 * nb); ctx->b32.ensure(6 * nb);
 	} else {
-		ctx->A64.ensure(36 * nnzb); ctx->P64.ensure(36 * (size_t)ctx->N); ctx->b64.ensure(6 * (size_t)ctx->N);
+		ctx->A64.ensure(36 * nnzb); ctx->P64.ensure(36 * nb); ctx->b64.ensure(6 * nb);
 	}
+	SArgs a0 = make_args(ctx, dt, g);
+	ctx->partials.ensure((size_t)P_COUNT * a0.pstride);
 	SArgs a = make_args(ctx, dt, g);
-	const int wc = wg_for(C), wb = wg_for(NL);
-	const int wv = std::max(1, std::min((NL + 3) / 4, kMaxWG));      // one wavefront per body
-	const int wp = std::max(1, std::min((ctx->NP + 3) / 4, 4096));   // one wavefront per pair
-	const dim3 B(kBlock);
-	const int nsc = a.nsc;
-	const int fd = nsc ? 3 : 1, cd = nsc ? 2 : 1;
-	const double nR = 6.0 * NL, nF = (double)fd * C, nLc = (double)cd * C;
+	const int ntc = ctx->n_tc, ntb = ctx->n_tb, ntw = ctx->n_tw, ntp = ctx->n_tp;
+	const int maxit = ctx->p.pcg_max_iter;
 
-	hipLaunchKernelGGL(k_setup_bodies, dim3(wb), B, 0, s, a, ctx->b_vel.p, ctx->b_ang.p, ctx->b_quat.p,
-	                   ctx->b_mass.p, ctx->b_I0.p);
-	PDC_CHECK_LAUNCH();
-	hipLaunchKernelGGL(k_nondim_speed, dim3(wv), B, 0, s, a);
-	PDC_CHECK_LAUNCH();
+	std::vector<char> in(S), run(S);
+	for (int sc = 0; sc < S; ++sc) in[sc] = run[sc] = ctx->h_cbeg[sc + 1] > ctx->h_cbeg[sc];
+	std::vector<double> Hu(S, 1e-4), exitErr(S, 0.0), step(S, 0.0);
+	std::vector<int> it(S, 0), pcg_total(S, 0), pcg_hits(S, 0), gd(S, 0), trials(S, 0), piters(S, 0);
+	ctx->h_sctl.resize(S);
+	auto upload_ctl = [&](const std::vector<char>& on) {
+		bool any = false;
+		for (int sc = 0; sc < S; ++sc) {
+			ctx->h_sctl[sc].act = on[sc] ? 1 : 0;
+			ctx->h_sctl[sc].Hu = Hu[sc];
+			ctx->h_sctl[sc].mu = ctx->h_mu[sc];
+			any |= on[sc] != 0;
+		}
+		PDC_HIP(hipMemcpyAsync(ctx->d_sctl.p, ctx->h_sctl.data(), S * sizeof(SceneCtl), hipMemcpyHostToDevice, s));
+		return any;
+	};
+	auto launch = [&](auto kernel, int grid, int threads, auto... args) {
+		if (grid > 0) hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, s, args...);
+	};
+
+	upload_ctl(in);
+	launch(k_setup_bodies, ntb, kBlock, a, ctx->b_vel.p, ctx->b_ang.p, ctx->b_quat.p, ctx->b_mass.p, ctx->b_I0.p);
+	launch(k_nondim_speed, ntw, kBlock, a);
 	const double gn = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-	hipLaunchKernelGGL(k_nondim_final, dim3(1), dim3(1024), 0, s, a, wb, ctx->b_mass.p, gn * dt);
-	PDC_CHECK_LAUNCH();
-	hipLaunchKernelGGL(k_scale_bodies, dim3(wb), B, 0, s, a);
-	PDC_CHECK_LAUNCH();
-	hipLaunchKernelGGL(k_preprocess, dim3(wc), B, 0, s, a);
+	launch(k_nondim_final, S, 1024, a, (const double*)ctx->b_mass.p, gn * dt);
+	launch(k_scale_bodies, ntb, kBlock, a);
+	launch(k_preprocess, ntc, kBlock, a);
 	PDC_CHECK_LAUNCH();
 
-	double Hu = 1e-4;
-	int it = 0, pcg_total = 0, pcg_hits = 0, gd = 0, trials = 0;
-	double exitErr = 0;
-	Scalars* H = ctx->h_sc;
-	for (it = 0; it < ctx->p.iterations; ++it) {
+	for (;;) {
+		for (int sc = 0; sc < S; ++sc)
+			if (run[sc] && it[sc] >= ctx->p.iterations) run[sc] = 0;
+		if (!upload_ctl(run)) break;
 		hipEvent_t e0;
 		kt_begin(ctx, PDC_KT_CONTACT_TERMS, &e0);
-		hipLaunchKernelGGL(k_constraints, dim3(wc), B, 0, s, a);
-		PDC_CHECK_LAUNCH();
-		hipLaunchKernelGGL(k_contact_res, dim3(wc), B, 0, s, a, wc, 1.0 / nLc);
-		PDC_CHECK_LAUNCH();
+		launch(k_constraints, ntc, kBlock, a);
+		launch(k_contact_res, ntc, kBlock, a);
 		kt_end(ctx, PDC_KT_CONTACT_TERMS, e0);
 		if (e0) ctx->kt.bytes += contact_terms_bytes(C, NL);
-		hipLaunchKernelGGL(k_body_res, dim3(wv), B, 0, s, a);
+		launch(k_body_res, ntw, kBlock, a);
+		launch(k_err_final, S, 256, a);
 		PDC_CHECK_LAUNCH();
-		hipLaunchKernelGGL(k_err_final, dim3(1), dim3(256), 0, s, a, wc, wv, Hu, nR, nF, nLc, (double)C);
-		PDC_CHECK_LAUNCH();
-		sync_scalars(ctx);
-		exitErr = H->exitErr;
-		if (exitErr < ctx->p.tolerance) break;
-		const double sErr = H->sErr, sacErr = H->sacErr;
+		read_scene_scalars(ctx);
+		for (int sc = 0; sc < S; ++sc) {
+			if (!run[sc]) continue;
+			exitErr[sc] = ctx->h_ss[sc].exitErr;
+			if (exitErr[sc] < ctx->p.tolerance) run[sc] = 0;
+		}
+		if (!upload_ctl(run)) break;
 		// assembly
 		kt_begin(ctx, PDC_KT_ASSEMBLY, &e0);
-		hipLaunchKernelGGL(k_asm_prep, dim3(wc), B, 0, s, a);
-		PDC_CHECK_LAUNCH();
+		launch(k_asm_prep, ntc, kBlock, a);
 		if (fp32) {
-			hipLaunchKernelGGL(k_asm_diag<float>, dim3(wv), B, 0, s, a, ctx->a_diag.p, ctx->A32.p,
-			                   ctx->P32.p, ctx->b32.p);
-			PDC_CHECK_LAUNCH();
-			hipLaunchKernelGGL(k_asm_off<float>, dim3(wp), B, 0, s, a, ctx->NP, ctx->P32.p, ctx->A32.p);
+			launch(k_asm_diag<float>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A32.p, ctx->P32.p, ctx->b32.p);
+			launch(k_asm_off<float>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
 		} else {
-			hipLaunchKernelGGL(k_asm_diag<double>, dim3(wv), B, 0, s, a, ctx->a_diag.p, ctx->A64.p,
-			                   ctx->P64.p, ctx->b64.p);
-			PDC_CHECK_LAUNCH();
-			hipLaunchKernelGGL(k_asm_off<double>, dim3(wp), B, 0, s, a, ctx->NP, ctx->P64.p, ctx->A64.p);
+			launch(k_asm_diag<double>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A64.p, ctx->P64.p, ctx->b64.p);
+			launch(k_asm_off<double>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
 		}
 		PDC_CHECK_LAUNCH();
 		kt_end(ctx, PDC_KT_ASSEMBLY, e0);
 		// solve
-		if (fp32) run_pcg<float>(ctx, ctx->A32, ctx->P32, ctx->b32);
-		else run_pcg<double>(ctx, ctx->A64, ctx->P64, ctx->b64);
-		hipLaunchKernelGGL(k_increments, dim3(wc), B, 0, s, a);
+		if (fp32) run_pcg<float>(ctx, a, ctx->A32, ctx->P32, ctx->b32, piters);
+		else run_pcg<double>(ctx, a, ctx->A64, ctx->P64, ctx->b64, piters);
+		launch(k_increments, ntc, kBlock, a);
 		PDC_CHECK_LAUNCH();
-		sync_scalars(ctx);
-		const int iters = H->pcg.iters;
-		ctx->pcg_last_iters = iters;
-		pcg_total += iters;
-		if (iters == ctx->p.pcg_max_iter) pcg_hits++;
-		// line search: fused passes of kTrials steps each
-		auto line_search = [&](const double* du, const double* dF, const double* dL, int maxLS) {
+		for (int sc = 0; sc < S; ++sc) {
+			if (!run[sc]) continue;
+			pcg_total[sc] += piters[sc];
+			if (piters[sc] == maxit) pcg_hits[sc]++;
+			step[sc] = 0;
+		}
+		// line search: fused passes of kTrials steps each, for the scenes still searching
+		auto line_search = [&](const double* du, const double* dF, const double* dL, int maxLS,
+		                       std::vector<char> pend) {
 			for (int k0 = 0; k0 < maxLS; k0 += kTrials) {
+				if (!upload_ctl(pend)) return;
 				hipEvent_t e1;
 				kt_begin(ctx, PDC_KT_LINE_SEARCH, &e1);
-				hipLaunchKernelGGL(k_ls_contact, dim3(wc), B, 0, s, a, du, dF, dL, k0);
-				PDC_CHECK_LAUNCH();
+				launch(k_ls_contact, ntc, kBlock, a, du, dF, dL, k0);
 				kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
-				hipLaunchKernelGGL(k_ls_body<0>, dim3(wv), B, 0, s, a, du, dF, dL, k0);
-				hipLaunchKernelGGL(k_ls_body<1>, dim3(wv), B, 0, s, a, du, dF, dL, k0);
+				launch(k_ls_body<0>, ntw, kBlock, a, du, dF, dL, k0);
+				launch(k_ls_body<1>, ntw, kBlock, a, du, dF, dL, k0);
+				launch(k_ls_final, S, 1024, a, k0, maxLS);
 				PDC_CHECK_LAUNCH();
-				hipLaunchKernelGGL(k_ls_final, dim3(1), dim3(1024), 0, s, a, wc, wv, k0, maxLS, sErr, sacErr);
-				PDC_CHECK_LAUNCH();
-				sync_scalars(ctx);
-				trials += H->ls_trials;
-				if (!H->ls_decided) continue;
-				const double step = H->ls_step;
-				if (step > 0) {
-					hipLaunchKernelGGL(k_ls_commit, dim3(wc), B, 0, s, a, dF, dL, step);
-					PDC_CHECK_LAUNCH();
-					hipLaunchKernelGGL(k_commit_v, dim3(grid_for(6 * NL)), B, 0, s, 6 * NL, ctx->l_v.p, du, step);
+				read_scene_scalars(ctx);
+				std::vector<char> commit(S, 0);
+				bool any = false;
+				for (int sc = 0; sc < S; ++sc) {
+					if (!pend[sc]) continue;
+					trials[sc] += ctx->h_ss[sc].ls_trials;
+					if (!ctx->h_ss[sc].ls_decided) continue;
+					pend[sc] = 0;
+					step[sc] = ctx->h_ss[sc].ls_step;
+					if (step[sc] > 0) { commit[sc] = 1; any = true; }
+				}
+				if (any) {
+					upload_ctl(commit);
+					launch(k_ls_commit, ntc, kBlock, a, dF, dL);
+					launch(k_commit_v, ntb, kBlock, a, du);
 					PDC_CHECK_LAUNCH();
 				}
-				return step;
 			}
-			return 0.0;
 		};
-		double step = line_search(ctx->l_du.p, ctx->s_df.p, ctx->s_dl.p, 10);
-		bool useGD = false;
-		if (step == 0) {
-			useGD = true;
-			gd++;
-			hipLaunchKernelGGL(k_neg, dim3(grid_for(6 * NL)), B, 0, s, 6 * NL, ctx->l_ru.p, ctx->l_gdu.p);
+		line_search(ctx->l_du.p, ctx->s_df.p, ctx->s_dl.p, 10, run);
+		std::vector<char> useGD(S, 0);
+		bool anyGD = false;
+		for (int sc = 0; sc < S; ++sc)
+			if (run[sc] && step[sc] == 0) { useGD[sc] = 1; anyGD = true; gd[sc]++; }
+		if (anyGD) {
+			upload_ctl(useGD);
+			launch(k_neg, ntb, kBlock, a, (const double*)ctx->l_ru.p, ctx->l_gdu.p);
 			PDC_CHECK_LAUNCH();
-			step = line_search(ctx->l_gdu.p, ctx->s_rf.p, ctx->s_rl.p, 20);
+			line_search(ctx->l_gdu.p, ctx->s_rf.p, ctx->s_rl.p, 20, useGD);
 		}
-		if (!useGD && step == 1) {
-			if (iters == ctx->p.pcg_max_iter) Hu *= 1.3;
-			else Hu /= 4;
-		} else {
-			Hu *= 4;
+		for (int sc = 0; sc < S; ++sc) {
+			if (!run[sc]) continue;
+			if (!useGD[sc] && step[sc] == 1) {
+				if (piters[sc] == maxit) Hu[sc] *= 1.3;
+				else Hu[sc] /= 4;
+			} else {
+				Hu[sc] *= 4;
+			}
+			Hu[sc] = std::max(Hu[sc], 1e-6);
+			it[sc]++;
 		}
-		Hu = std::max(Hu, 1e-6);
 	}
-	hipLaunchKernelGGL(k_writeback_bodies, dim3(wb), B, 0, s, a, ctx->b_acc.p);
+	upload_ctl(in);
+	launch(k_writeback_bodies, ntb, kBlock, a, ctx->b_acc.p);
+	launch(k_writeback_contacts, ntc, kBlock, a, ctx->c_lf.p);
 	PDC_CHECK_LAUNCH();
-	hipLaunchKernelGGL(k_writeback_contacts, dim3(wc), B, 0, s, a, ctx->c_lf.p);
-	PDC_CHECK_LAUNCH();
-	sync_scalars(ctx);
-	stats.n_contacts = C;
-	stats.n_local_bodies = NL;
-	stats.newton_iterations = it;
-	stats.pcg_iterations = pcg_total;
-	stats.pcg_maxiter_hits = pcg_hits;
-	stats.gd_fallbacks = gd;
-	stats.line_search_trials = trials;
-	stats.exit_error = exitErr;
-	stats.char_mass = H->char_mass;
-	stats.char_speed = H->char_speed;
-	stats.ties = ctx->n_ties;
-	stats.n_blocks = ctx->NNZB;
-	if (st) *st = stats;
-	ctx->timings.newton_iterations += it;
+	read_scene_scalars(ctx);
+	int itmax = 0;
+	for (int sc = 0; sc < S; ++sc) {
+		pdc_solve_stats& st = ctx->scene_stats[sc];
+		st.n_contacts = ctx->h_cbeg[sc + 1] - ctx->h_cbeg[sc];
+		if (!in[sc]) continue;
+		st.n_local_bodies = ctx->h_lbeg[sc + 1] - ctx->h_lbeg[sc];
+		st.newton_iterations = it[sc];
+		st.pcg_iterations = pcg_total[sc];
+		st.pcg_maxiter_hits = pcg_hits[sc];
+		st.gd_fallbacks = gd[sc];
+		st.line_search_trials = trials[sc];
+		st.exit_error = exitErr[sc];
+		st.char_mass = ctx->h_ss[sc].char_mass;
+		st.char_speed = ctx->h_ss[sc].char_speed;
+		st.n_blocks = ctx->h_nnzb[sc];
+		itmax = std::max(itmax, it[sc]);
+	}
+	if (S == 1) ctx->scene_stats[0].ties = ctx->n_ties;
+	ctx->timings.newton_iterations += itmax;
 	ctx->timings.contacts += C;
+}
+
+// Statistics of the last solve over the scenes: counts summed, Newton
+// iterations and exit error the maximum over scenes, characteristic scales of
+// the first scene with contacts.
+pdc_solve_stats aggregate_stats(const pdc_context* ctx) {
+	pdc_solve_stats t{};
+	bool first = true;
+	for (const pdc_solve_stats& st : ctx->scene_stats) {
+		t.n_contacts += st.n_contacts;
+		t.n_local_bodies += st.n_local_bodies;
+		t.newton_iterations = std::max(t.newton_iterations, st.newton_iterations);
+		t.pcg_iterations += st.pcg_iterations;
+		t.pcg_maxiter_hits += st.pcg_maxiter_hits;
+		t.gd_fallbacks += st.gd_fallbacks;
+		t.line_search_trials += st.line_search_trials;
+		t.n_blocks += st.n_blocks;
+		t.exit_error = std::max(t.exit_error, st.exit_error);
+		if (first && st.n_contacts > 0) {
+			t.char_mass = st.char_mass;
+			t.char_speed = st.char_speed;
+			first = false;
+		}
+	}
+	t.ties = ctx->n_ties;
+	return t;
 }
 
 }  // namespace pdc

@@ -36,6 +36,7 @@ EXPORTS = [
     "pdc_last_error", "pdc_upload_bodies", "pdc_set_state", "pdc_get_state",
     "pdc_detect_contacts", "pdc_get_contacts", "pdc_fill_forces", "pdc_step",
     "pdc_get_timings", "pdc_reset_timings", "pdc_enable_kernel_timing", "pdc_kernel_timing",
+    "pdc_set_scenes", "pdc_scene_count", "pdc_set_scene_friction", "pdc_step_scenes",
 ]
 
 
@@ -82,6 +83,10 @@ def load_library(path=LIB_PATH):
     L.pdc_fill_forces.argtypes = [V, ctypes.c_int32, _I, _D, _D, _D, _D, _D, _D, _D,
                                   ctypes.POINTER(PdcSolveStats)]
     L.pdc_step.argtypes = [V, ctypes.c_int32, ctypes.POINTER(PdcSolveStats)]
+    L.pdc_set_scenes.argtypes = [V, ctypes.c_int32, _I]
+    L.pdc_scene_count.argtypes = [V]
+    L.pdc_set_scene_friction.argtypes = [V, _D]
+    L.pdc_step_scenes.argtypes = [V, ctypes.c_int32, ctypes.c_int32, _I, ctypes.POINTER(PdcSolveStats)]
     L.pdc_get_timings.argtypes = [V, ctypes.POINTER(PdcTimings)]
     L.pdc_reset_timings.argtypes = [V]
     L.pdc_reset_timings.restype = None
@@ -196,6 +201,31 @@ class Context:
         self._check(self._L.pdc_step(self.h, int(n), ctypes.byref(st)))
         return st
 
+    def set_scenes(self, body_begin):
+        """Partitions the bodies into independent scenes: scene k holds bodies
+        [body_begin[k], body_begin[k+1])."""
+        off = np.ascontiguousarray(body_begin, dtype=np.int32)
+        self._check(self._L.pdc_set_scenes(self.h, len(off) - 1, _i(off)))
+
+    def scene_count(self):
+        return int(self._L.pdc_scene_count(self.h))
+
+    def set_scene_friction(self, mu):
+        self._check(self._L.pdc_set_scene_friction(self.h, _d(_f64(mu))))
+
+    def step_scenes(self, n=1, scene_ids=None):
+        """n timesteps of the given scenes (all when None); per-scene statistics
+        of the last step, one per selected scene."""
+        if scene_ids is None:
+            k = self.scene_count()
+            st = (PdcSolveStats * max(k, 1))()
+            self._check(self._L.pdc_step_scenes(self.h, int(n), 0, None, st))
+            return [st[i] for i in range(k)]
+        ids = np.ascontiguousarray(scene_ids, dtype=np.int32)
+        st = (PdcSolveStats * max(len(ids), 1))()
+        self._check(self._L.pdc_step_scenes(self.h, int(n), len(ids), _i(ids), st))
+        return [st[i] for i in range(len(ids))]
+
     def timings(self):
         t = PdcTimings()
         self._check(self._L.pdc_get_timings(self.h, ctypes.byref(t)))
@@ -253,6 +283,35 @@ class DynamicSystem:
 
     def state(self):
         return self.ctx.get_state()
+
+    def set_state(self, pos, quat, vel, angvel):
+        self.ctx.set_state(pos, quat, vel, angvel)
+
+
+class BatchedSystem:
+    """Many independent scenes resident in one context and stepped together
+    (or any subset of them) in the same kernel launches."""
+
+    def __init__(self, scenes, params=None, device=0):
+        from .scenes import concat
+        self.scenes = list(scenes)
+        merged, self.offsets, mu = concat(self.scenes)
+        self.ctx = Context(params if params is not None else merged.params, device)
+        self.ctx.upload_bodies(merged.shape_type, merged.dims, merged.mass, merged.inertia,
+                               merged.is_static, merged.emit)
+        self.ctx.set_scenes(self.offsets)
+        self.ctx.set_scene_friction(mu)
+        self.ctx.set_state(merged.pos, merged.quat, merged.vel, merged.angvel)
+
+    def step(self, n=1, scene_ids=None):
+        return self.ctx.step_scenes(n, scene_ids)
+
+    def state(self, k=None):
+        st = self.ctx.get_state()
+        if k is None:
+            return st
+        a, b = self.offsets[k], self.offsets[k + 1]
+        return tuple(x[a:b] for x in st)
 
     def set_state(self, pos, quat, vel, angvel):
         self.ctx.set_state(pos, quat, vel, angvel)

@@ -231,6 +231,23 @@ def random_pile(seed, n_min=10, n_max=500, **kw):
                 grid_n=kw.pop("grid_n", 32), **kw)
 
 
+def concat(scenes):
+    """Body tables and states of independent scenes stacked into one table.
+
+    Returns (merged scene with the first scene's parameters, body offsets
+    (len(scenes) + 1,) int32, per-scene friction coefficients).  Scene k holds
+    bodies [offsets[k], offsets[k+1]).
+    """
+    fields = ["shape_type", "dims", "mass", "inertia", "is_static", "emit", "pos", "quat", "vel", "angvel"]
+    merged = Scene(name="batch_%d" % len(scenes),
+                   **{f: np.concatenate([getattr(s, f) for s in scenes]) for f in fields},
+                   params=scenes[0].params, steps=scenes[0].steps)
+    offsets = np.zeros(len(scenes) + 1, dtype=np.int32)
+    offsets[1:] = np.cumsum([s.n for s in scenes])
+    mu = np.array([s.params.friction_coeff for s in scenes], dtype=np.float64)
+    return merged, offsets, mu
+
+
 WORKLOADS = {
     "stack_10": stack_10,
     "pyramid_210": pyramid_210,
