@@ -22,7 +22,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from primal_dual_contacts_miror_amd import pdc, scenes  # noqa: E402
+from primal_dual_contacts_miror_amd import pdc, scenes, serve  # noqa: E402
 
 METRIC = "Newton timesteps/sec at 30k contacts; batched scene-steps/sec + p50/p99 latency"
 PEAK_HBM = 8.0e12
@@ -85,6 +85,74 @@ def cpu_baseline(sc, state, budget_s, threads):
     }
 
 
+def batched_leg(n_scenes, steps, plain_scenes, plain_steps, seed_base=1000):
+    """Many independent seeded random piles (10-500 bodies, log-uniform) in
+    one context, all stepped together from their seeded drop; then, at the
+    same (contact-rich) state, the batched rate over a few steps against the
+    plain per-scene path (one scene per step call, one after another)."""
+    scs = [scenes.random_pile(seed_base + k) for k in range(n_scenes)]
+    bs = pdc.BatchedSystem(scs)
+    log("batched: %d scenes, %d bodies, %d steps" % (n_scenes, sum(s.n for s in scs), steps))
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st = bs.step(1)
+        if k % 50 == 0:
+            log("batched step %d: %d contacts" % (k, sum(x.n_contacts for x in st)))
+    el = time.perf_counter() - t0
+    contacts = [x.n_contacts for x in st]
+    tail = 10
+    t0 = time.perf_counter()
+    for _ in range(tail):
+        bs.step(1)
+    el_tail = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k in range(plain_scenes):
+        for _ in range(plain_steps):
+            bs.step(1, [k])
+    el_plain = time.perf_counter() - t0
+    batched_tail = n_scenes * tail / el_tail
+    plain = plain_scenes * plain_steps / el_plain
+    out = {
+        "workload": "%d seeded random piles, 10-500 bodies (log-uniform), spheres + boxes, dt 0.005, 32^3 grids, "
+                    "FP32 PCG / FP64 Newton, stepped together from the seeded drop" % n_scenes,
+        "steps": steps,
+        "scene_steps_per_s": n_scenes * steps / el,
+        "contacts_last_step": {"total": int(sum(contacts)), "max_per_scene": int(max(contacts))},
+        "contact_phase": {
+            "batched_scene_steps_per_s": batched_tail,
+            "plain_scene_steps_per_s": plain,
+            "speedup_vs_plain": batched_tail / plain,
+            "sample": "%d batched steps of all scenes vs %d steps of each of %d scenes alone, same state" % (
+                tail, plain_steps, plain_scenes),
+        },
+    }
+    return bs, out
+
+
+def served_leg(bs, rates, duration, plain_rate, seed=7):
+    """Continuous-batching server over the batched leg's resident scenes
+    (their contact-rich state): Poisson request streams of one-step requests."""
+    srv = serve.SceneServer.over(bs)
+    n = len(bs.scenes)
+    out = {"resident_scenes": n, "duration_s": duration, "rates": []}
+    for rate in rates:
+        reqs = serve.poisson_requests(rate, duration, n, seed=seed)
+        served = srv.run(reqs, batched=True, time_limit=duration * 4)
+        summ = serve.latency_summary(served)
+        summ["rate_req_per_s"] = rate
+        summ["dropped"] = len(reqs) - summ["requests"]
+        log("served %g req/s: %s" % (rate, summ))
+        out["rates"].append(summ)
+    reqs = serve.poisson_requests(plain_rate, duration, n, seed=seed)
+    served = srv.run(reqs, batched=False, time_limit=duration * 4)
+    summ = serve.latency_summary(served)
+    summ["rate_req_per_s"] = plain_rate
+    summ["dropped"] = len(reqs) - summ["requests"]
+    log("plain %g req/s: %s" % (plain_rate, summ))
+    out["plain_per_scene"] = summ
+    return out
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of the timed kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (profiles/<kernel>_<workload>_traffic.json), or None when there is none."""
@@ -110,6 +178,12 @@ def main():
     ap.add_argument("--state", default=None, help="npz with pos, quat, vel, angvel to start from")
     ap.add_argument("--settle-steps", type=int, default=None,
                     help="settle on the device for this many steps instead of loading the settled state")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched and served legs")
+    ap.add_argument("--batched-scenes", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-rates", default="200,1000,5000")
+    ap.add_argument("--served-duration", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -177,6 +251,12 @@ def main():
         },
     }
     ds.ctx.close()
+    if not args.no_batched:
+        bs, out["batched"] = batched_leg(args.batched_scenes, args.batched_steps, 16, 10)
+        if not args.no_served:
+            rates = [float(r) for r in args.served_rates.split(",") if r]
+            out["served"] = served_leg(bs, rates, args.served_duration, plain_rate=40.0)
+        bs.ctx.close()
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sc, state, args.cpu_budget, threads)
