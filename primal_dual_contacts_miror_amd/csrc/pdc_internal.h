@@ -71,32 +71,44 @@ struct Tile {
 	int count;   // elements in the tile
 };
 
-// Per-scene scalars of the solve, written by the reduction kernels and read
-// by the host (one entry per scene).
+// Per-scene state of the solve, advanced on the device (the Newton loop's
+// decisions are taken per scene by the kernels) and read by the host once per
+// Newton iteration and at the end of the solve.
 struct SceneSc {
 	double char_mass, char_speed;
 	double gap, muc;
 	double ru2, wrf2, rl2, ac2;
 	double err, sErr, sacErr, eps, exitErr;
 	double t_inErr, t_sacIn, t_gErr;
-	double ls_step;      // accepted step of the last line-search pass (0: failure)
-	int ls_decided, ls_trials;
-};
-
-// Per-scene control written by the host before a phase of the solve.
-struct SceneCtl {
-	int act;     // the scene takes part in the phase
+	double Hu;           // eps = Hu err^0.333335
+	double ls_step;      // accepted step of this iteration's line search (0: none)
+	int run;             // the scene's Newton loop continues
+	int it;              // completed Newton iterations
+	int ls_state;        // LS_* below
+	int gd_count;        // line searches that fell back to gradient descent
+	int trials;          // line-search trial points evaluated
+	int pcg_total;       // PCG iterations, summed over Newton iterations
+	int pcg_hits;        // PCG solves that reached the iteration cap
 	int pad;
-	double Hu;   // the scene's Hu (eps = Hu err^0.333335)
+};
+enum { LS_PENDING = 0, LS_ACCEPTED = 1, LS_GD_PENDING = 2, LS_GD_DECIDED = 3 };
+
+// Per-scene control written by the host once per solve.
+struct SceneCtl {
+	int act;     // the scene has contacts and takes part in the solve
+	int pad;
 	double mu;   // the scene's friction coefficient
 };
 
 // Per-scene PCG control block: read by every PCG kernel of the scene, written
-// by the scene's first tile.
+// by the scene's first tile; `run` mirrors the scene's Newton state for the
+// host's one read per Newton iteration.
 struct PcgScene {
 	int iters;   // iterations of the last solve
 	int done;    // converged flag
 	int itc[2];  // iteration-counter slots
+	int run;     // the scene's Newton loop continues (copied by k_err_final)
+	int pad;
 };
 
 // Detection counters (one device struct).

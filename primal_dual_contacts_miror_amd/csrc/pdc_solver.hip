@@ -403,6 +403,7 @@ struct SArgs {
 	const int* lbeg;
 	SceneSc* ss;
 	const SceneCtl* ctl;
+	PcgScene* ps;
 	// contacts
 	const int* li0;
 	const int* li1;
@@ -480,11 +481,18 @@ __device__ __forceinline__ double tile_sum(const double* part, int stride, int s
 	return block_reduce_array(part + slot * (size_t)stride + t0, t1 - t0, tmp);
 }
 
-// The workgroup's tile, or nullptr when its scene sits out the phase.
-#define PDC_TILE(TILES)                                  \
-	const Tile tile = (TILES)[blockIdx.x];               \
-	if (tile.scene < 0 || !a.ctl[tile.scene].act) return; \
+// The workgroup's tile and its scene; the workgroup returns at once when the
+// scene is not in the solve (PDC_TILE) or has left the Newton loop (PDC_TILE_RUN).
+#define PDC_TILE_BEGIN(TILES)                \
+	const Tile tile = (TILES)[blockIdx.x];   \
+	if (tile.scene < 0) return;              \
 	const int scn = tile.scene;
+#define PDC_TILE(TILES)      \
+	PDC_TILE_BEGIN(TILES)    \
+	if (!a.ctl[scn].act) return;
+#define PDC_TILE_RUN(TILES)  \
+	PDC_TILE_BEGIN(TILES)    \
+	if (!a.ss[scn].run) return;
 
 // ---- setup ----------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_setup_bodies(SArgs a, const double* __restrict__ vel, const double* __restrict__ ang,
@@ -646,7 +654,7 @@ __global__ __launch_bounds__(256) void k_preprocess(SArgs a) {
 // Constraint values, gradients, Hessians; partial sums of lambda.c for the
 // surrogate duality gap (NonSmoothContactForce.cpp:105-118, NonSmoothForceBase.h:82-86).
 __global__ __launch_bounds__(256) void k_constraints(SArgs a) {
-	PDC_TILE(a.tc);
+	PDC_TILE_RUN(a.tc);
 	const double mu = a.ctl[scn].mu;
 	double acc[1] = {0};
 	if ((int)threadIdx.x < tile.count) {
@@ -744,7 +752,7 @@ __device__ __forceinline__ void contact_terms(const SArgs& a, int i, const doubl
 // Main-iteration residuals of the contacts: stores rf, rl; partial |wrf|^2, |rl|^2, |ac|^2.
 __global__ __launch_bounds__(256) void k_contact_res(SArgs a) {
 	__shared__ double slot;
-	PDC_TILE(a.tc);
+	PDC_TILE_RUN(a.tc);
 	const double gapsum = tile_sum(a.part, a.pstride, P_GAP, a.tbc[scn], a.tbc[scn + 1], &slot);
 	const int Cs = a.cbeg[scn + 1] - a.cbeg[scn];
 	const double gap = -gapsum / ((double)(a.nsc ? 2 : 1) * Cs);
@@ -773,7 +781,7 @@ __global__ __launch_bounds__(256) void k_contact_res(SArgs a) {
 // One wavefront per body: lanes take the body's incident contacts with stride
 // 64, the six sums are reduced in a fixed butterfly order.
 __global__ __launch_bounds__(256) void k_body_res(SArgs a) {
-	PDC_TILE(a.tw);
+	PDC_TILE_RUN(a.tw);
 	(void)scn;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	double acc[1] = {0};
@@ -815,10 +823,10 @@ __global__ __launch_bounds__(256) void k_body_res(SArgs a) {
 }
 
 // err, eps and the exit test (PrimalDualForceSolver.cpp:295-320); one workgroup per scene.
-__global__ __launch_bounds__(256) void k_err_final(SArgs a) {
+__global__ __launch_bounds__(256) void k_err_final(SArgs a, double tol) {
 	__shared__ double slot;
 	const int sc = blockIdx.x;
-	if (!a.ctl[sc].act) return;
+	if (!a.ctl[sc].act || !a.ss[sc].run) return;
 	const int c0 = a.tbc[sc], c1 = a.tbc[sc + 1], w0 = a.tbw[sc], w1 = a.tbw[sc + 1];
 	const double wrf2 = tile_sum(a.part, a.pstride, P_WRF, c0, c1, &slot);
 	const double rl2 = tile_sum(a.part, a.pstride, P_RL, c0, c1, &slot);
@@ -828,8 +836,8 @@ __global__ __launch_bounds__(256) void k_err_final(SArgs a) {
 		const double nC = a.cbeg[sc + 1] - a.cbeg[sc];
 		const double nR = 6.0 * (a.lbeg[sc + 1] - a.lbeg[sc]);
 		const double nF = (a.nsc ? 3.0 : 1.0) * nC, nLc = (a.nsc ? 2.0 : 1.0) * nC;
-		const double Hu = a.ctl[sc].Hu;
 		SceneSc* S = &a.ss[sc];
+		const double Hu = S->Hu;
 		S->ru2 = ru2; S->wrf2 = wrf2; S->rl2 = rl2; S->ac2 = ac2;
 		const double err = ru2 + wrf2 + rl2;
 		S->err = err;
@@ -842,13 +850,18 @@ __global__ __launch_bounds__(256) void k_err_final(SArgs a) {
 		e = fmax(e, S->gap);
 		e = fmin(e, fmax(momen, sqrt(ac2) / nC));
 		S->exitErr = e;
+		// the exit test; a scene that stays starts this iteration's line search afresh
+		if (e < tol) S->run = 0;
+		S->ls_state = LS_PENDING;
+		S->ls_step = 0;
+		a.ps[sc].run = S->run;
 	}
 }
 
 // ---- (c) assembly ----------------------------------------------------------
 // Per contact: invC, sf, Kn^-1, Kt^-1, Kt^-1 sf (NonSmoothContactForce.cpp:184-219).
 __global__ __launch_bounds__(256) void k_asm_prep(SArgs a) {
-	PDC_TILE(a.tc);
+	PDC_TILE_RUN(a.tc);
 	const double eps = a.ss[scn].eps, cm = a.ss[scn].char_mass;
 	if ((int)threadIdx.x < tile.count) {
 		const int i = tile.first + threadIdx.x;
@@ -954,7 +967,7 @@ __device__ __forceinline__ bool spd_inverse6(const double* A, double* Ai) {
 template <class T>
 __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict__ adiag, T* __restrict__ Aval,
                            T* __restrict__ Pval, T* __restrict__ bvec) {
-	PDC_TILE(a.tw);
+	PDC_TILE_RUN(a.tw);
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const double eps = a.ss[scn].eps;
 	const int fd = a.nsc ? 3 : 1;
@@ -1045,7 +1058,7 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 // operator, A_ab P_b and A_ba P_a; P comes from k_asm_diag, which runs first.
 template <class T>
 __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ Pval, T* __restrict__ Aval) {
-	PDC_TILE(a.tp);
+	PDC_TILE_RUN(a.tp);
 	(void)scn;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int ii = lane < 36 ? lane / 6 : 0, jj = lane < 36 ? lane - 6 * (lane / 6) : 0;
@@ -1140,7 +1153,7 @@ struct PcgArgs {
 	double* hist;    // per scene: gamma, alpha of iteration i at [4 s + 2 (i & 1)], [+ 1]
 	const Tile* tiles;      // kPcgRowsPerWG block rows per tile, one tile per workgroup
 	const int* tbeg;        // per scene: first tile (S + 1)
-	const SceneCtl* ctl;    // per scene: takes part in this solve
+	const SceneSc* ss;      // per scene: `run`, the scene is still in its Newton loop
 	PcgScene* ps;           // per scene: done flag, iteration counters
 	double tol2;            // squared relative tolerance
 };
@@ -1230,7 +1243,7 @@ template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	__shared__ double lds[4 * 16];
 	const Tile tile = a.tiles[blockIdx.x];
-	if (tile.scene < 0 || !a.ctl[tile.scene].act) return;
+	if (tile.scene < 0 || !a.ss[tile.scene].run) return;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[4] = {0, 0, 0, 0};
@@ -1306,7 +1319,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	const PcgScene c = a.ps[scn];
-	const int act = a.ctl[scn].act;
+	const int act = a.ss[scn].run;
 	// (a runtime index into the private copy would place it in LDS, one slot per thread)
 	const int it = slot ? c.itc[1] : c.itc[0];
 	const int t0 = a.tbeg[scn], W = a.tbeg[scn + 1] - t0;
@@ -1387,7 +1400,7 @@ template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg2_init(PcgArgs<T> a) {
 	__shared__ double lds[4 * 16];
 	const Tile tile = a.tiles[blockIdx.x];
-	if (tile.scene < 0 || !a.ctl[tile.scene].act) return;
+	if (tile.scene < 0 || !a.ss[tile.scene].run) return;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[2] = {0, 0};
@@ -1422,7 +1435,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) 
 	const int scn = tile.scene;
 	if (scn < 0) return;
 	const PcgScene c = a.ps[scn];
-	if (c.done || !a.ctl[scn].act) return;
+	if (c.done || !a.ss[scn].run) return;
 	const int it = slot ? c.itc[1] : c.itc[0];  // no dynamic index into a private copy
 	const int cur = it & 1, nxt = cur ^ 1;
 	const int t0 = a.tbeg[scn], W = a.tbeg[scn + 1] - t0;
@@ -1469,7 +1482,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) 
 	const int scn = tile.scene;
 	if (scn < 0) return;
 	const PcgScene c = a.ps[scn];
-	if (c.done || !a.ctl[scn].act) return;
+	if (c.done || !a.ss[scn].run) return;
 	const int it = slot ? c.itc[1] : c.itc[0];  // no dynamic index into a private copy
 	const int cur = it & 1, nxt = cur ^ 1;
 	const int t0 = a.tbeg[scn], W = a.tbeg[scn + 1] - t0;
@@ -1511,7 +1524,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) 
 }
 
 __global__ __launch_bounds__(256) void k_du(SArgs a, const double* __restrict__ x) {
-	PDC_TILE(a.tb);
+	PDC_TILE_RUN(a.tb);
 	(void)scn;
 	if ((int)threadIdx.x < tile.count) {
 		const int r = tile.first + threadIdx.x;
@@ -1521,7 +1534,7 @@ __global__ __launch_bounds__(256) void k_du(SArgs a, const double* __restrict__ 
 
 // ---- increments (NonSmoothContactForce.cpp:254-280) -------------------------
 __global__ __launch_bounds__(256) void k_increments(SArgs a) {
-	PDC_TILE(a.tc);
+	PDC_TILE_RUN(a.tc);
 	(void)scn;
 	if ((int)threadIdx.x < tile.count) {
 		const int i = tile.first + threadIdx.x;
@@ -1548,8 +1561,8 @@ __global__ __launch_bounds__(256) void k_increments(SArgs a) {
 }
 
 __global__ __launch_bounds__(256) void k_neg(SArgs a, const double* __restrict__ x, double* __restrict__ y) {
-	PDC_TILE(a.tb);
-	(void)scn;
+	PDC_TILE_RUN(a.tb);
+	if (a.ss[scn].ls_state != LS_GD_PENDING) return;
 	if ((int)threadIdx.x < tile.count) {
 		const int r = tile.first + threadIdx.x;
 		for (int k = 0; k < 6; ++k) y[6 * r + k] = -x[6 * r + k];
@@ -1598,14 +1611,18 @@ __device__ __forceinline__ void trial_point(const SArgs& a, int i, const double*
 	for (int k = 0; k < 2; ++k) lI[k] = lam[k] + 0.99 * dli[k];
 }
 
-// Contact part of the trials: partial |wrf'|^2, |rl'|^2, |ac'|^2, df_in.rf'
-// and sum (dl_in / lambda') rl' per trial (fresh constraint values).
+// Contact part of NT trials k0 .. k0 + NT - 1 of a search in state `want`:
+// partial |wrf'|^2, |rl'|^2, |ac'|^2, df_in.rf' and sum (dl_in / lambda') rl'
+// per trial (fresh constraint values).
+template <int NT>
 __global__ __launch_bounds__(256) void k_ls_contact(SArgs a, const double* __restrict__ du,
                                                     const double* __restrict__ dF,
-                                                    const double* __restrict__ dL, int k0) {
-	PDC_TILE(a.tc);
+                                                    const double* __restrict__ dL, int k0, int want) {
+	static_assert(NT <= kTrials, "partial slots");
+	PDC_TILE_RUN(a.tc);
+	if (a.ss[scn].ls_state != want) return;
 	const double muc = a.ss[scn].muc, mu = a.ctl[scn].mu, cm = a.ss[scn].char_mass;
-	for (int t = 0; t < kTrials; ++t) {
+	for (int t = 0; t < NT; ++t) {
 		const double step = ldexp(1.0, -(k0 + t));
 		double acc[5] = {0, 0, 0, 0, 0};
 		if ((int)threadIdx.x < tile.count) {
@@ -1639,8 +1656,9 @@ constexpr int kGroup = 5;
 template <int G>
 __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restrict__ du,
                                                  const double* __restrict__ dF,
-                                                 const double* __restrict__ dL, int k0) {
-	PDC_TILE(a.tw);
+                                                 const double* __restrict__ dL, int k0, int want) {
+	PDC_TILE_RUN(a.tw);
+	if (a.ss[scn].ls_state != want) return;
 	const double mu = a.ctl[scn].mu;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	double acc[2 * kGroup];
@@ -1706,16 +1724,20 @@ __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restri
 }
 static_assert(kTrials == 2 * kGroup, "k_ls_body covers the trials in two groups");
 
-// Acceptance (PrimalDualForceSolver.cpp:605-631) over the trials of the pass,
-// in order: the first trial with (accept and gErr > 0), or the search's last
-// trial, decides.  ls_step = accepted step, or 0 when the decision is a
-// failure; ls_decided = 0 when no trial of this pass decided.
-__global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int k0, int maxLS) {
+// Acceptance (PrimalDualForceSolver.cpp:605-631) over the nt trials of a pass
+// (trials k0 .. k0 + nt - 1 of a search of maxLS trials), in order: the first
+// trial with (accept and gErr > 0), or the search's last trial, decides.  The
+// trials of one search share the direction, so evaluating a pass at once
+// selects exactly the step the one-trial-at-a-time loop would.  A decided
+// regular search is accepted (LS_ACCEPTED, ls_step) or falls back to gradient
+// descent (LS_GD_PENDING, PrimalDualForceSolver.cpp:441-467); a decided
+// gradient-descent search ends in LS_GD_DECIDED with ls_step 0 on failure.
+__global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int k0, int nt, int maxLS, int gd) {
 	__shared__ double red[7 * kTrials];
 	const int sc = blockIdx.x;
-	if (!a.ctl[sc].act) return;
+	if (!a.ctl[sc].act || !a.ss[sc].run || a.ss[sc].ls_state != (gd ? LS_GD_PENDING : LS_PENDING)) return;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-	for (int q = wid; q < 7 * kTrials; q += 16) {
+	for (int q = wid; q < 7 * nt; q += 16) {
 		const int slot = P_LS + q;
 		const int c = q - 7 * (q / 7);
 		const int t0 = c < 5 ? a.tbc[sc] : a.tbw[sc], t1 = c < 5 ? a.tbc[sc + 1] : a.tbw[sc + 1];
@@ -1728,23 +1750,29 @@ __global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int k0, int maxLS) {
 	if (threadIdx.x == 0) {
 		SceneSc* S = &a.ss[sc];
 		const double sErr = S->sErr, sacErr = S->sacErr;
-		S->ls_decided = 0;
-		S->ls_step = 0;
-		S->ls_trials = 0;
-		for (int t = 0; t < kTrials && k0 + t < maxLS; ++t) {
+		for (int t = 0; t < nt && k0 + t < maxLS; ++t) {
 			const double* R = &red[7 * t];
 			const double inErr = R[5] + R[0] + R[1];
 			const double sacIn = R[5] + R[2] + R[1];
 			const double gErr = -R[6] + R[3] + R[4];
 			const bool acpt = (inErr < sErr) | (sacIn < sacErr);
-			S->ls_trials = t + 1;
+			S->trials += 1;
 			if ((acpt && gErr > 0) || k0 + t == maxLS - 1) {
-				S->ls_decided = 1;
 				if (acpt) {
 					S->ls_step = ldexp(1.0, -(k0 + t));
 					S->t_inErr = inErr;
 					S->t_sacIn = sacIn;
 					S->t_gErr = gErr;
+				} else {
+					S->ls_step = 0;
+				}
+				if (gd) {
+					S->ls_state = LS_GD_DECIDED;
+				} else if (acpt) {
+					S->ls_state = LS_ACCEPTED;
+				} else {
+					S->ls_state = LS_GD_PENDING;
+					S->gd_count += 1;
 				}
 				break;
 			}
@@ -1752,16 +1780,21 @@ __global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int k0, int maxLS) {
 	}
 }
 
-// Commit the accepted trial: f, lambda in place (the same trial point the
-// pass evaluated), then v (k_commit_v).
+// Commit the accepted trial point of the scene's search (the regular
+// direction, or the gradient-descent one after a fallback): f, lambda in
+// place, then v (k_commit_v).
 __global__ __launch_bounds__(256) void k_ls_commit(SArgs a, const double* __restrict__ dF,
-                                                   const double* __restrict__ dL) {
-	PDC_TILE(a.tc);
+                                                   const double* __restrict__ dL,
+                                                   const double* __restrict__ gF,
+                                                   const double* __restrict__ gL) {
+	PDC_TILE_RUN(a.tc);
 	const double step = a.ss[scn].ls_step, mu = a.ctl[scn].mu;
+	if (!(step > 0)) return;
+	const bool g = a.ss[scn].ls_state == LS_GD_DECIDED;
 	if ((int)threadIdx.x < tile.count) {
 		const int i = tile.first + threadIdx.x;
 		double fI[3], lI[2], dfi[3], dli[2];
-		trial_point(a, i, dF, dL, step, mu, fI, lI, dfi, dli);
+		trial_point(a, i, g ? gF : dF, g ? gL : dL, step, mu, fI, lI, dfi, dli);
 		a.f[i] = fI[0];
 		a.lam[i] = lI[0];
 		if (a.nsc) {
@@ -1772,13 +1805,54 @@ __global__ __launch_bounds__(256) void k_ls_commit(SArgs a, const double* __rest
 	}
 }
 
-__global__ __launch_bounds__(256) void k_commit_v(SArgs a, const double* __restrict__ du) {
-	PDC_TILE(a.tb);
+__global__ __launch_bounds__(256) void k_commit_v(SArgs a, const double* __restrict__ du,
+                                                  const double* __restrict__ gdu) {
+	PDC_TILE_RUN(a.tb);
 	const double step = a.ss[scn].ls_step;
+	if (!(step > 0)) return;
+	const double* d = a.ss[scn].ls_state == LS_GD_DECIDED ? gdu : du;
 	if ((int)threadIdx.x < tile.count) {
 		const int r = tile.first + threadIdx.x;
-		for (int k = 0; k < 6; ++k) a.v[6 * r + k] = a.v[6 * r + k] + 0.99 * (step * du[6 * r + k]);
+		for (int k = 0; k < 6; ++k) a.v[6 * r + k] = a.v[6 * r + k] + 0.99 * (step * d[6 * r + k]);
 	}
+}
+
+// End of a Newton iteration per scene (PrimalDualForceSolver.cpp:468-490):
+// the Hu schedule, the PCG counters, the iteration count and the cap.
+__global__ void k_newton_end(SArgs a, int maxit, int iterations) {
+	const int sc = blockIdx.x * blockDim.x + threadIdx.x;
+	if (sc >= a.S || !a.ctl[sc].act || !a.ss[sc].run) return;
+	SceneSc* S = &a.ss[sc];
+	const PcgScene P = a.ps[sc];
+	const int iters = P.done ? P.iters : maxit;
+	S->pcg_total += iters;
+	if (iters == maxit) S->pcg_hits += 1;
+	double Hu = S->Hu;
+	const bool useGD = S->ls_state == LS_GD_DECIDED;
+	if (!useGD && S->ls_step == 1.0) Hu = iters == maxit ? Hu * 1.3 : Hu / 4;
+	else Hu *= 4;
+	S->Hu = fmax(Hu, 1e-6);
+	S->it += 1;
+	if (S->it >= iterations) S->run = 0;
+	a.ps[sc].run = S->run;
+}
+
+// Start of a solve per scene: the Newton state of the scenes with contacts.
+__global__ void k_scene_init(SArgs a) {
+	const int sc = blockIdx.x * blockDim.x + threadIdx.x;
+	if (sc >= a.S) return;
+	SceneSc* S = &a.ss[sc];
+	S->run = a.ctl[sc].act;
+	S->it = 0;
+	S->Hu = 1e-4;
+	S->ls_state = LS_PENDING;
+	S->ls_step = 0;
+	S->gd_count = 0;
+	S->trials = 0;
+	S->pcg_total = 0;
+	S->pcg_hits = 0;
+	S->exitErr = 0;
+	a.ps[sc].run = S->run;
 }
 
 // ---- write-back (PrimalDualForceSolver.cpp:491-510) --------------------------
@@ -1842,11 +1916,6 @@ __global__ __launch_bounds__(256) void k_integrate(int N, const int* __restrict_
 	for (int k = 0; k < 3; ++k) { vel[3 * b + k] = v[k]; ang[3 * b + k] = w[k]; }
 }
 
-// Iteration count of the scenes whose solve ran to the iteration cap.
-__global__ void k_pcg_final(PcgScene* ps, const SceneCtl* ctl, int S, int maxit) {
-	const int s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s < S && ctl[s].act && !ps[s].done) ps[s].iters = maxit;
-}
 
 // ===========================================================================
 // host orchestration
@@ -1868,7 +1937,7 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	a.tc = ctx->t_c.p; a.tb = ctx->t_b.p; a.tw = ctx->t_w.p; a.tp = ctx->t_p.p;
 	a.tbc = ctx->tb_c.p; a.tbb = ctx->tb_b.p; a.tbw = ctx->tb_w.p;
 	a.cbeg = ctx->r_cbeg.p; a.lbeg = ctx->r_lbeg.p;
-	a.ss = ctx->d_ss.p; a.ctl = ctx->d_sctl.p;
+	a.ss = ctx->d_ss.p; a.ctl = ctx->d_sctl.p; a.ps = ctx->d_ps.p;
 	a.li0 = ctx->li0.p; a.li1 = ctx->li1.p; a.T = ctx->c_T.p; a.depth = ctx->c_depth.p;
 	a.pstart = ctx->ap_start.p; a.pa = ctx->ap_a.p; a.g2l = ctx->g2l.p;
 	a.lstat = ctx->l_static.p; a.l2g = ctx->l2g.p;
@@ -1908,10 +1977,10 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 // single-reduction recurrences stall near 1e-10 on ill-conditioned systems.
 constexpr double kSingleReductionMinTol = 1e-9;
 
-// PCG of every scene with h_sctl[s].act; iters[s] receives its iteration count.
+// PCG of every scene still in its Newton loop; returns whether any scene is
+// (the scenes' run flags, set by k_err_final, arrive with the PCG done flags).
 template <class T>
-static void run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b,
-                    std::vector<int>& iters) {
+static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b) {
 	const int S = ctx->S;
 	// grid fixed per scene set (capacity of PCG tiles) so the recorded chunk stays valid
 	const int nblk = std::max(1, ctx->cap_tr);
@@ -1937,7 +2006,7 @@ static void run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 	pa.hist = parts + 7 * pcap;
 	pa.tiles = ctx->t_r.p;
 	pa.tbeg = ctx->tb_r.p;
-	pa.ctl = ctx->d_sctl.p;
+	pa.ss = ctx->d_ss.p;
 	pa.ps = ctx->d_ps.p;
 	pa.tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
 	hipStream_t s = ctx->stream;
@@ -1987,12 +2056,8 @@ static void run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 		PDC_HIP(hipMemcpyAsync(ctx->h_ps.data(), ctx->d_ps.p, S * sizeof(PcgScene), hipMemcpyDeviceToHost, s));
 		PDC_HIP(hipStreamSynchronize(s));
 		all_done = true;
-		int min_done_it = launched;  // for the per-launch timing: launches all scenes were active in
-		for (int sc = 0; sc < S; ++sc) {
-			if (!ctx->h_sctl[sc].act) continue;
-			if (!ctx->h_ps[sc].done) all_done = false;
-			else min_done_it = std::min(min_done_it, std::max(ctx->h_ps[sc].iters, before));
-		}
+		for (int sc = 0; sc < S; ++sc)
+			if (ctx->h_ps[sc].run && !ctx->h_ps[sc].done) all_done = false;
 		if (kt_iter) {
 			float e = 0;
 			PDC_HIP(hipEventElapsedTime(&e, ev[0], ev[1]));
@@ -2002,7 +2067,7 @@ static void run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 			// launches up to the last active iteration of the scene that ran longest
 			int active = 0;
 			for (int sc = 0; sc < S; ++sc)
-				if (ctx->h_sctl[sc].act)
+				if (ctx->h_ps[sc].run)
 					active = std::max(active, ctx->h_ps[sc].done ? ctx->h_ps[sc].iters - before : kPcgChunk);
 			active = std::min(std::max(active, 0), kPcgChunk);
 			for (int k = 0; k < active; ++k) {
@@ -2012,18 +2077,19 @@ static void run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 			}
 			ctx->kt.launches += active;
 		}
-		(void)min_done_it;
-		if (all_done) break;  // every scene converged
+		if (all_done) break;  // every running scene converged
 		want = 1;
 	}
+	std::vector<int> iters(S, 0);
 	int last = 0;
+	bool any = false;
 	for (int sc = 0; sc < S; ++sc) {
-		iters[sc] = 0;
-		if (!ctx->h_sctl[sc].act) continue;
+		if (!ctx->h_ps[sc].run) continue;
+		any = true;
 		iters[sc] = ctx->h_ps[sc].done ? ctx->h_ps[sc].iters : maxit;
 		last = std::max(last, iters[sc]);
 	}
-	ctx->pcg_last_iters = last;
+	if (any) ctx->pcg_last_iters = last;
 	if (kt_spmv || kt_iter) {
 		// Algorithmic bytes of one active iteration of a scene (of k_pcg_iter,
 		// or of k_pcg2_a for the two-reduction form), each input read once and
@@ -2036,7 +2102,7 @@ static void run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 		const double Tb = sizeof(T);
 		ctx->kt.total_ms += kt_ms;
 		for (int sc = 0; sc < S; ++sc) {
-			if (!ctx->h_sctl[sc].act) continue;
+			if (!ctx->h_ps[sc].run) continue;
 			const double NLs = ctx->h_lbeg[sc + 1] - ctx->h_lbeg[sc], n6 = 6.0 * NLs, nnzb = ctx->h_nnzb[sc];
 			const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NLs + 1) + 36.0 * Tb * NLs;
 			ctx->kt.bytes += iters[sc] * (mat + (two ? 40.0 : 128.0) * n6);
@@ -2044,6 +2110,7 @@ static void run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 	}
 	if (ctx->n_tb > 0) hipLaunchKernelGGL(k_du, dim3(ctx->n_tb), dim3(kBlock), 0, s, sa, ctx->x64.p);
 	PDC_CHECK_LAUNCH();
+	return any;
 }
 
 // Algorithmic bytes (each input read once, each output written once).
@@ -2104,29 +2171,21 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	ctx->partials.ensure((size_t)P_COUNT * a0.pstride);
 	SArgs a = make_args(ctx, dt, g);
 	const int ntc = ctx->n_tc, ntb = ctx->n_tb, ntw = ctx->n_tw, ntp = ctx->n_tp;
-	const int maxit = ctx->p.pcg_max_iter;
 
-	std::vector<char> in(S), run(S);
-	for (int sc = 0; sc < S; ++sc) in[sc] = run[sc] = ctx->h_cbeg[sc + 1] > ctx->h_cbeg[sc];
-	std::vector<double> Hu(S, 1e-4), exitErr(S, 0.0), step(S, 0.0);
-	std::vector<int> it(S, 0), pcg_total(S, 0), pcg_hits(S, 0), gd(S, 0), trials(S, 0), piters(S, 0);
+	std::vector<char> in(S);
+	for (int sc = 0; sc < S; ++sc) in[sc] = ctx->h_cbeg[sc + 1] > ctx->h_cbeg[sc];
 	ctx->h_sctl.resize(S);
-	auto upload_ctl = [&](const std::vector<char>& on) {
-		bool any = false;
-		for (int sc = 0; sc < S; ++sc) {
-			ctx->h_sctl[sc].act = on[sc] ? 1 : 0;
-			ctx->h_sctl[sc].Hu = Hu[sc];
-			ctx->h_sctl[sc].mu = ctx->h_mu[sc];
-			any |= on[sc] != 0;
-		}
-		PDC_HIP(hipMemcpyAsync(ctx->d_sctl.p, ctx->h_sctl.data(), S * sizeof(SceneCtl), hipMemcpyHostToDevice, s));
-		return any;
-	};
+	for (int sc = 0; sc < S; ++sc) {
+		ctx->h_sctl[sc].act = in[sc] ? 1 : 0;
+		ctx->h_sctl[sc].mu = ctx->h_mu[sc];
+	}
+	PDC_HIP(hipMemcpyAsync(ctx->d_sctl.p, ctx->h_sctl.data(), S * sizeof(SceneCtl), hipMemcpyHostToDevice, s));
 	auto launch = [&](auto kernel, int grid, int threads, auto... args) {
 		if (grid > 0) hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, s, args...);
 	};
+	const int gS = (S + kBlock - 1) / kBlock;
 
-	upload_ctl(in);
+	launch(k_scene_init, gS, kBlock, a);
 	launch(k_setup_bodies, ntb, kBlock, a, ctx->b_vel.p, ctx->b_ang.p, ctx->b_quat.p, ctx->b_mass.p, ctx->b_I0.p);
 	launch(k_nondim_speed, ntw, kBlock, a);
 	const double gn = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
@@ -2135,10 +2194,15 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	launch(k_preprocess, ntc, kBlock, a);
 	PDC_CHECK_LAUNCH();
 
-	for (;;) {
-		for (int sc = 0; sc < S; ++sc)
-			if (run[sc] && it[sc] >= ctx->p.iterations) run[sc] = 0;
-		if (!upload_ctl(run)) break;
+	// The Newton loop of every scene (PrimalDualForceSolver.cpp:278-490) as a
+	// fixed sequence of launches per iteration: each scene's decisions (exit
+	// test, line-search acceptance, gradient-descent fallback, Hu schedule,
+	// iteration cap) are taken on the device, and the launches of a scene that
+	// is in another state return at once.  The host waits once per iteration,
+	// for the PCG, and learns there which scenes are still iterating.
+	const double* du = ctx->l_du.p;
+	const double* gdu = ctx->l_gdu.p;
+	for (int iter = 0; iter < ctx->p.iterations; ++iter) {
 		hipEvent_t e0;
 		kt_begin(ctx, PDC_KT_CONTACT_TERMS, &e0);
 		launch(k_constraints, ntc, kBlock, a);
@@ -2146,15 +2210,8 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		kt_end(ctx, PDC_KT_CONTACT_TERMS, e0);
 		if (e0) ctx->kt.bytes += contact_terms_bytes(C, NL);
 		launch(k_body_res, ntw, kBlock, a);
-		launch(k_err_final, S, 256, a);
+		launch(k_err_final, S, 256, a, ctx->p.tolerance);
 		PDC_CHECK_LAUNCH();
-		read_scene_scalars(ctx);
-		for (int sc = 0; sc < S; ++sc) {
-			if (!run[sc]) continue;
-			exitErr[sc] = ctx->h_ss[sc].exitErr;
-			if (exitErr[sc] < ctx->p.tolerance) run[sc] = 0;
-		}
-		if (!upload_ctl(run)) break;
 		// assembly
 		kt_begin(ctx, PDC_KT_ASSEMBLY, &e0);
 		launch(k_asm_prep, ntc, kBlock, a);
@@ -2167,73 +2224,40 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		}
 		PDC_CHECK_LAUNCH();
 		kt_end(ctx, PDC_KT_ASSEMBLY, e0);
-		// solve
-		if (fp32) run_pcg<float>(ctx, a, ctx->A32, ctx->P32, ctx->b32, piters);
-		else run_pcg<double>(ctx, a, ctx->A64, ctx->P64, ctx->b64, piters);
+		// solve (the one wait of the iteration)
+		const bool any = fp32 ? run_pcg<float>(ctx, a, ctx->A32, ctx->P32, ctx->b32)
+		                      : run_pcg<double>(ctx, a, ctx->A64, ctx->P64, ctx->b64);
+		if (!any) break;  // every scene converged or reached the cap
 		launch(k_increments, ntc, kBlock, a);
+		// line search: two passes of kGroup trials in the Newton direction,
+		// then the gradient-descent fallback (PrimalDualForceSolver.cpp:441-467)
+		// in two passes of kTrials; each pass runs for the scenes still searching
+		hipEvent_t e1;
+		for (int k0 = 0; k0 < 10; k0 += kGroup) {
+			kt_begin(ctx, PDC_KT_LINE_SEARCH, &e1);
+			launch(k_ls_contact<kGroup>, ntc, kBlock, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
+			       (int)LS_PENDING);
+			kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
+			launch(k_ls_body<0>, ntw, kBlock, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
+			       (int)LS_PENDING);
+			launch(k_ls_final, S, 1024, a, k0, kGroup, 10, 0);
+		}
+		launch(k_neg, ntb, kBlock, a, (const double*)ctx->l_ru.p, ctx->l_gdu.p);
+		for (int k0 = 0; k0 < 20; k0 += kTrials) {
+			launch(k_ls_contact<kTrials>, ntc, kBlock, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p,
+			       k0, (int)LS_GD_PENDING);
+			launch(k_ls_body<0>, ntw, kBlock, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, k0,
+			       (int)LS_GD_PENDING);
+			launch(k_ls_body<1>, ntw, kBlock, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, k0,
+			       (int)LS_GD_PENDING);
+			launch(k_ls_final, S, 1024, a, k0, kTrials, 20, 1);
+		}
+		launch(k_ls_commit, ntc, kBlock, a, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
+		       (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p);
+		launch(k_commit_v, ntb, kBlock, a, du, gdu);
+		launch(k_newton_end, gS, kBlock, a, ctx->p.pcg_max_iter, ctx->p.iterations);
 		PDC_CHECK_LAUNCH();
-		for (int sc = 0; sc < S; ++sc) {
-			if (!run[sc]) continue;
-			pcg_total[sc] += piters[sc];
-			if (piters[sc] == maxit) pcg_hits[sc]++;
-			step[sc] = 0;
-		}
-		// line search: fused passes of kTrials steps each, for the scenes still searching
-		auto line_search = [&](const double* du, const double* dF, const double* dL, int maxLS,
-		                       std::vector<char> pend) {
-			for (int k0 = 0; k0 < maxLS; k0 += kTrials) {
-				if (!upload_ctl(pend)) return;
-				hipEvent_t e1;
-				kt_begin(ctx, PDC_KT_LINE_SEARCH, &e1);
-				launch(k_ls_contact, ntc, kBlock, a, du, dF, dL, k0);
-				kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
-				launch(k_ls_body<0>, ntw, kBlock, a, du, dF, dL, k0);
-				launch(k_ls_body<1>, ntw, kBlock, a, du, dF, dL, k0);
-				launch(k_ls_final, S, 1024, a, k0, maxLS);
-				PDC_CHECK_LAUNCH();
-				read_scene_scalars(ctx);
-				std::vector<char> commit(S, 0);
-				bool any = false;
-				for (int sc = 0; sc < S; ++sc) {
-					if (!pend[sc]) continue;
-					trials[sc] += ctx->h_ss[sc].ls_trials;
-					if (!ctx->h_ss[sc].ls_decided) continue;
-					pend[sc] = 0;
-					step[sc] = ctx->h_ss[sc].ls_step;
-					if (step[sc] > 0) { commit[sc] = 1; any = true; }
-				}
-				if (any) {
-					upload_ctl(commit);
-					launch(k_ls_commit, ntc, kBlock, a, dF, dL);
-					launch(k_commit_v, ntb, kBlock, a, du);
-					PDC_CHECK_LAUNCH();
-				}
-			}
-		};
-		line_search(ctx->l_du.p, ctx->s_df.p, ctx->s_dl.p, 10, run);
-		std::vector<char> useGD(S, 0);
-		bool anyGD = false;
-		for (int sc = 0; sc < S; ++sc)
-			if (run[sc] && step[sc] == 0) { useGD[sc] = 1; anyGD = true; gd[sc]++; }
-		if (anyGD) {
-			upload_ctl(useGD);
-			launch(k_neg, ntb, kBlock, a, (const double*)ctx->l_ru.p, ctx->l_gdu.p);
-			PDC_CHECK_LAUNCH();
-			line_search(ctx->l_gdu.p, ctx->s_rf.p, ctx->s_rl.p, 20, useGD);
-		}
-		for (int sc = 0; sc < S; ++sc) {
-			if (!run[sc]) continue;
-			if (!useGD[sc] && step[sc] == 1) {
-				if (piters[sc] == maxit) Hu[sc] *= 1.3;
-				else Hu[sc] /= 4;
-			} else {
-				Hu[sc] *= 4;
-			}
-			Hu[sc] = std::max(Hu[sc], 1e-6);
-			it[sc]++;
-		}
 	}
-	upload_ctl(in);
 	launch(k_writeback_bodies, ntb, kBlock, a, ctx->b_acc.p);
 	launch(k_writeback_contacts, ntc, kBlock, a, ctx->c_lf.p);
 	PDC_CHECK_LAUNCH();
@@ -2243,17 +2267,18 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		pdc_solve_stats& st = ctx->scene_stats[sc];
 		st.n_contacts = ctx->h_cbeg[sc + 1] - ctx->h_cbeg[sc];
 		if (!in[sc]) continue;
+		const SceneSc& H = ctx->h_ss[sc];
 		st.n_local_bodies = ctx->h_lbeg[sc + 1] - ctx->h_lbeg[sc];
-		st.newton_iterations = it[sc];
-		st.pcg_iterations = pcg_total[sc];
-		st.pcg_maxiter_hits = pcg_hits[sc];
-		st.gd_fallbacks = gd[sc];
-		st.line_search_trials = trials[sc];
-		st.exit_error = exitErr[sc];
-		st.char_mass = ctx->h_ss[sc].char_mass;
-		st.char_speed = ctx->h_ss[sc].char_speed;
+		st.newton_iterations = H.it;
+		st.pcg_iterations = H.pcg_total;
+		st.pcg_maxiter_hits = H.pcg_hits;
+		st.gd_fallbacks = H.gd_count;
+		st.line_search_trials = H.trials;
+		st.exit_error = H.exitErr;
+		st.char_mass = H.char_mass;
+		st.char_speed = H.char_speed;
 		st.n_blocks = ctx->h_nnzb[sc];
-		itmax = std::max(itmax, it[sc]);
+		itmax = std::max(itmax, H.it);
 	}
 	if (S == 1) ctx->scene_stats[0].ties = ctx->n_ties;
 	ctx->timings.newton_iterations += itmax;
