@@ -107,8 +107,8 @@ void free_all(pdc_context* c) {
 	c->d_sc.release();
 	c->b_scene.release(); c->d_sbeg.release(); c->d_step.release();
 	c->r_cbeg.release(); c->r_lbeg.release(); c->r_pbeg.release(); c->r_nnzb.release();
-	c->t_c.release(); c->t_b.release(); c->t_w.release(); c->t_p.release(); c->t_r.release();
-	c->tb_c.release(); c->tb_b.release(); c->tb_w.release(); c->tb_p.release(); c->tb_r.release();
+	c->t_c.release(); c->t_b.release(); c->t_w.release(); c->t_p.release(); c->t_pcg.release();
+	c->tb_c.release(); c->tb_b.release(); c->tb_w.release(); c->tb_p.release();
 	c->d_ss.release(); c->d_sctl.release(); c->d_ps.release();
 	c->c_stride = 0;
 }

@@ -71,6 +71,22 @@ struct Tile {
 	int count;   // elements in the tile
 };
 
+// PCG workgroup record, 64 bytes, read at the start of every PCG launch: the
+// tile's block rows, its scene's tile range, the row pointers of its rows and
+// the tile's own copy of the scene's PCG state.  Every workgroup of a scene
+// takes the same decisions, so each one keeps and updates its own copy and no
+// workgroup reads another's record.
+struct PcgTile {
+	int scene;    // -1 for padding tiles
+	int first;    // first block row
+	int count;    // block rows (<= kPcgRowsPerWG)
+	int t0, W;    // the scene's tiles [t0, t0 + W)
+	int done;     // the scene's PCG has converged, or the scene is not solving
+	int it;       // iterations completed
+	int ptr[kPcgRowsPerWG + 1];  // row pointers of block rows first .. first + count
+};
+static_assert(sizeof(PcgTile) == 64, "one 64-byte record per PCG workgroup");
+
 // Per-scene state of the solve, advanced on the device (the Newton loop's
 // decisions are taken per scene by the kernels) and read by the host once per
 // Newton iteration and at the end of the solve.
@@ -100,13 +116,12 @@ struct SceneCtl {
 	double mu;   // the scene's friction coefficient
 };
 
-// Per-scene PCG control block: read by every PCG kernel of the scene, written
-// by the scene's first tile; `run` mirrors the scene's Newton state for the
-// host's one read per Newton iteration.
+// Per-scene PCG outcome for the host (written by the scene's first tile) and
+// `run`, the scene's Newton state, mirrored here for the host's one read per
+// Newton iteration.
 struct PcgScene {
 	int iters;   // iterations of the last solve
 	int done;    // converged flag
-	int itc[2];  // iteration-counter slots
 	int run;     // the scene's Newton loop continues (copied by k_err_final)
 	int pad;
 };
@@ -230,10 +245,11 @@ struct pdc_context {
 	pdc::DBuf<int> r_cbeg, r_lbeg, r_pbeg;                    // per scene: first contact, local body, pair
 	pdc::DBuf<int> r_nnzb;                                    // per scene: Newton-matrix blocks (+ total)
 	std::vector<int> h_cbeg, h_lbeg, h_pbeg, h_nnzb;
-	pdc::DBuf<pdc::Tile> t_c, t_b, t_w, t_p, t_r;             // tiles: contacts, bodies, body waves, pair waves, PCG rows
-	pdc::DBuf<int> tb_c, tb_b, tb_w, tb_p, tb_r;              // per scene: first tile (S + 1)
+	pdc::DBuf<pdc::Tile> t_c, t_b, t_w, t_p;                  // tiles: contacts, bodies, body waves, pair waves
+	pdc::DBuf<int> tb_c, tb_b, tb_w, tb_p;                    // per scene: first tile (S + 1)
 	int n_tc = 0, n_tb = 0, n_tw = 0, n_tp = 0, n_tr = 0;
 	int cap_tr = 0;                                           // PCG tiles launched (fixed per scene set)
+	pdc::DBuf<pdc::PcgTile> t_pcg;                            // PCG workgroup records (cap_tr)
 	pdc::DBuf<pdc::SceneSc> d_ss;                             // per-scene scalars
 	pdc::SceneSc* h_ss = nullptr;                             // pinned mirror
 	size_t h_ss_cap = 0;

@@ -312,7 +312,20 @@ static void scene_ranges(pdc_context* ctx) {
 	ctx->n_tp = upload_tiles(ctx, t, tb, ctx->t_p, ctx->tb_p);
 	make_tiles(ctx->h_lbeg, S, kPcgRowsPerWG, t, tb, ctx->cap_tr);
 	if ((int)t.size() > ctx->cap_tr) throw HipError("PCG tile table exceeds its capacity", PDC_ERR_STATE);
-	ctx->n_tr = upload_tiles(ctx, t, tb, ctx->t_r, ctx->tb_r);
+	std::vector<PcgTile> pt(t.size());
+	for (size_t k = 0; k < t.size(); ++k) {
+		PcgTile r{};
+		r.scene = t[k].scene;
+		r.first = t[k].first;
+		r.count = t[k].count;
+		r.t0 = r.scene >= 0 ? tb[r.scene] : 0;
+		r.W = r.scene >= 0 ? tb[r.scene + 1] - tb[r.scene] : 0;
+		r.done = 1;
+		pt[k] = r;
+	}
+	ctx->n_tr = (int)t.size();
+	ctx->t_pcg.ensure(std::max<size_t>(pt.size(), 1));
+	PDC_HIP(hipMemcpyAsync(ctx->t_pcg.p, pt.data(), pt.size() * sizeof(PcgTile), hipMemcpyHostToDevice, ctx->stream));
 }
 
 void build_topology(pdc_context* ctx) {
@@ -1151,8 +1164,7 @@ struct PcgArgs {
 	double* rr[2];
 	double* bb;
 	double* hist;    // per scene: gamma, alpha of iteration i at [4 s + 2 (i & 1)], [+ 1]
-	const Tile* tiles;      // kPcgRowsPerWG block rows per tile, one tile per workgroup
-	const int* tbeg;        // per scene: first tile (S + 1)
+	PcgTile* tiles;         // one record per workgroup (kPcgRowsPerWG block rows)
 	const SceneSc* ss;      // per scene: `run`, the scene is still in its Newton loop
 	PcgScene* ps;           // per scene: done flag, iteration counters
 	double tol2;            // squared relative tolerance
@@ -1179,6 +1191,25 @@ __device__ __forceinline__ double spmv_lane(const PcgArgs<T>& a, const V* __rest
 			for (int c = 0; c < 6; ++c) acc += (double)blk[c] * (double)vc[c];
 		}
 	}
+	acc += __shfl_xor(acc, 4, 64);
+	acc += __shfl_xor(acc, 2, 64);
+	acc += __shfl_xor(acc, 1, 64);
+	return acc;
+}
+
+// Row `i` (0..5) of the block row whose blocks are [k0, k1) of B times v;
+// every lane of the row's eight-lane group returns the sum.
+template <class T, class V>
+__device__ __forceinline__ double spmv_lane_range(const PcgArgs<T>& a, const V* __restrict__ v, int k0, int k1,
+                                                  int i, int sl) {
+	double acc = 0;
+	if (i < 6)
+		for (int k = k0 + sl; k < k1; k += 8) {
+			const T* blk = a.B + 36 * (size_t)k + 6 * i;
+			const V* vc = v + 6 * (size_t)a.col[k];
+#pragma unroll
+			for (int c = 0; c < 6; ++c) acc += (double)blk[c] * (double)vc[c];
+		}
 	acc += __shfl_xor(acc, 4, 64);
 	acc += __shfl_xor(acc, 2, 64);
 	acc += __shfl_xor(acc, 1, 64);
@@ -1225,30 +1256,47 @@ __device__ __forceinline__ void reduce_partials(const double* p0, const double* 
 	for (int k = 0; k < N; ++k) out[k] = lds[k];
 }
 
-// The scene's first tile resets its control block and history.
+// Start of a solve for one tile: its record's row pointers and state; the
+// scene's first tile also resets the scene's outcome and history.
 template <class T>
-__device__ __forceinline__ void pcg_scene_reset(const PcgArgs<T>& a, int scn) {
-	if ((int)blockIdx.x == a.tbeg[scn] && threadIdx.x == 0) {
-		a.ps[scn].done = 0;
-		a.ps[scn].iters = 0;
-		a.ps[scn].itc[0] = 0;
-		a.ps[scn].itc[1] = 0;
-		// history read by iteration 0: beta = gamma / inf = 0, alpha = gamma / delta
-		a.hist[4 * scn + 2] = __builtin_huge_val();
-		a.hist[4 * scn + 3] = 1.0;
+__device__ __forceinline__ void pcg_tile_reset(const PcgArgs<T>& a, PcgTile* Tp, int scn) {
+	if ((int)threadIdx.x <= Tp->count) Tp->ptr[threadIdx.x] = a.ptr[Tp->first + threadIdx.x];
+	if (threadIdx.x == 0) {
+		Tp->done = 0;
+		Tp->it = 0;
+		if ((int)blockIdx.x == Tp->t0) {
+			a.ps[scn].done = 0;
+			a.ps[scn].iters = 0;
+			// history read by iteration 0: beta = gamma / inf = 0, alpha = gamma / delta
+			a.hist[4 * scn + 2] = __builtin_huge_val();
+			a.hist[4 * scn + 3] = 1.0;
+		}
 	}
+}
+
+// A tile whose scene is not solving marks itself done.
+template <class T>
+__device__ __forceinline__ bool pcg_tile_skip(const PcgArgs<T>& a, PcgTile* Tp) {
+	const int scn = Tp->scene;
+	if (scn < 0) return true;
+	if (!a.ss[scn].run) {
+		if (threadIdx.x == 0) Tp->done = 1;
+		return true;
+	}
+	return false;
 }
 
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	__shared__ double lds[4 * 16];
-	const Tile tile = a.tiles[blockIdx.x];
-	if (tile.scene < 0 || !a.ss[tile.scene].run) return;
+	PcgTile* Tp = a.tiles + blockIdx.x;
+	if (pcg_tile_skip(a, Tp)) return;
+	const int scn = Tp->scene, cnt = Tp->count, first = Tp->first;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[4] = {0, 0, 0, 0};
-	if (wid < tile.count) {
-		const int br = tile.first + wid;
+	if (wid < cnt) {
+		const int br = first + wid;
 		const double w0 = spmv_lane(a, a.b, br, i, sl);
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
@@ -1277,71 +1325,90 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 		a.rr[0][blockIdx.x] = out[2];
 		a.bb[blockIdx.x] = out[2];
 	}
-	pcg_scene_reset(a, tile.scene);
+	pcg_tile_reset(a, Tp, scn);
 }
 
-// Operator row and row vectors of block row `br` as read by k_pcg_iter
+// Operator row and row vectors of one block row as read by k_pcg_iter
 // (meaningful in the lanes with i < 6, sl == 0).
 struct PcgRow {
 	double n, m, w, z, q, s, p, u, x, r;
 };
 
-template <class T>
-__device__ __forceinline__ PcgRow pcg_row_load(const PcgArgs<T>& a, const double* __restrict__ wc, int br, int i, int sl) {
-	PcgRow R;
-	R.n = spmv_lane(a, wc, br, i, sl);
-	R.m = R.w = R.z = R.q = R.s = R.p = R.u = R.x = R.r = 0;
-	if (i < 6 && sl == 0) {
-		const int t = 6 * br + i;
-		double m = 0;
-#pragma unroll
-		for (int cc = 0; cc < 6; ++cc) m += (double)a.P[36 * (size_t)br + 6 * i + cc] * wc[6 * br + cc];
-		R.m = m;
-		R.w = wc[t];
-		R.z = a.z[t]; R.q = a.q[t]; R.s = a.s[t]; R.p = a.p[t];
-		R.u = a.u[t]; R.x = a.x[t]; R.r = a.r[t];
-	}
-	return R;
-}
-
 // Iteration parity equals the launch slot: every chunk has an even number of
 // launches and each active launch advances the scene's counter by one.  The
-// operator row and row vectors of the wavefront's block row are loaded before
-// the partial sums are reduced, so the two memory round trips overlap.
+// workgroup's record (one scalar load) gives the rows, the scene's partial
+// range and the row pointers, so the partial sums, the operator rows and the
+// row vectors are all requested in the second memory round trip and the
+// gathered neighbour values in the third.  A tile whose scene has converged
+// returns after reading its record.
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
 	static_assert(kPcgRowsPerWG >= 4, "four partial arrays, one per wavefront");
-	const Tile tile = a.tiles[blockIdx.x];
-	const int scn = tile.scene;
-	if (scn < 0) return;
+	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
+	// the whole record in one batch of scalar loads; the wavefront's row
+	// pointers are selected from it with the wave-uniform index (no runtime
+	// index into the private copy), and the early exit depends only on fields
+	// every wavefront shares
+	const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+	int rec[16];
+	const int* __restrict__ rp = reinterpret_cast<const int*>(Tp);
+#pragma unroll
+	for (int k = 0; k < 16; ++k) rec[k] = rp[k];
+	const int scn = rec[0], first = rec[1], cnt = rec[2], t0 = rec[3], W = rec[4], done = rec[5], it = rec[6];
+	int k0 = rec[7], k1 = rec[8];
+#pragma unroll
+	for (int q = 1; q < kPcgRowsPerWG; ++q)
+		if (wid == q) {
+			k0 = rec[7 + q];
+			k1 = rec[8 + q];
+		}
+	if (scn < 0 || done) return;
 	const int cur = slot, nxt = slot ^ 1;
-	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
 	const int i = lane >> 3, sl = lane & 7;
-	const PcgScene c = a.ps[scn];
-	const int act = a.ss[scn].run;
-	// (a runtime index into the private copy would place it in LDS, one slot per thread)
-	const int it = slot ? c.itc[1] : c.itc[0];
-	const int t0 = a.tbeg[scn], W = a.tbeg[scn + 1] - t0;
 	const double* __restrict__ wc = pick(a.w, cur);
-	double part = 0;
+	// partial sums of the previous launch: loads issued now, summed after the
+	// operator-row loads are in flight (a scene of more than 1024 tiles sums
+	// the rest afterwards)
+	const double* psrc = wid == 0 ? pick(a.gam, cur) : wid == 1 ? pick(a.del, cur) : wid == 2 ? pick(a.rr, cur) : a.bb;
+	psrc += t0;
+	double pv[kPartLoads];
 	if (wid < 4) {
-		const double* src = wid == 0 ? pick(a.gam, cur) : wid == 1 ? pick(a.del, cur) : wid == 2 ? pick(a.rr, cur) : a.bb;
-		part = partial_lane_sum(src + t0, W);
+#pragma unroll
+		for (int k = 0; k < kPartLoads; ++k) pv[k] = psrc[min(lane + 64 * k, W - 1)];
 	}
-	const int br = tile.first + wid;
+	const int br = first + wid;
 	PcgRow R{};
-	if (wid < tile.count) R = pcg_row_load(a, wc, br, i, sl);
+	if (wid < cnt) {
+		R.n = spmv_lane_range(a, wc, k0, k1, i, sl);
+		if (i < 6 && sl == 0) {
+			const int t = 6 * br + i;
+			double m = 0;
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) m += (double)a.P[36 * (size_t)br + 6 * i + cc] * wc[6 * br + cc];
+			R.m = m;
+			R.w = wc[t];
+			R.z = a.z[t]; R.q = a.q[t]; R.s = a.s[t]; R.p = a.p[t];
+			R.u = a.u[t]; R.x = a.x[t]; R.r = a.r[t];
+		}
+	}
 	if (wid < 4) {
+		double part = 0;
+#pragma unroll
+		for (int k = 0; k < kPartLoads; ++k) part += lane + 64 * k < W ? pv[k] : 0.0;
+		if (W > 64 * kPartLoads) part += partial_lane_sum(psrc + 64 * kPartLoads, W - 64 * kPartLoads);
 		part = wave_sum(part);
 		if (lane == 0) lds[wid] = part;
 	}
 	__syncthreads();
-	if (c.done || !act) return;
 	const double gamma = lds[0], delta = lds[1], rr = lds[2], bb = lds[3];
-	const bool first = (int)blockIdx.x == t0;
+	const bool first_tile = (int)blockIdx.x == t0;
 	if (!(rr > a.tol2 * bb)) {
-		if (first && threadIdx.x == 0) { a.ps[scn].done = 1; a.ps[scn].iters = it; }
+		if (threadIdx.x == 0) {
+			Tp->done = 1;
+			if (first_tile) { a.ps[scn].done = 1; a.ps[scn].iters = it; }
+		}
 		return;
 	}
 	const double gprev = a.hist[4 * scn + 2 * nxt], aprev = a.hist[4 * scn + 2 * nxt + 1];
@@ -1349,7 +1416,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	const double alpha = gamma / (delta - beta * gamma / aprev);
 	double* __restrict__ wn = pick(a.w, nxt);
 	double v[4] = {0, 0, 0, 0};
-	if (wid < tile.count && i < 6 && sl == 0) {
+	if (wid < cnt && i < 6 && sl == 0) {
 		const int t = 6 * br + i;
 		const double zt = R.n + beta * R.z;
 		const double qt = R.m + beta * R.q;
@@ -1376,10 +1443,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 		pick(a.gam, nxt)[blockIdx.x] = out[0];
 		pick(a.del, nxt)[blockIdx.x] = out[1];
 		pick(a.rr, nxt)[blockIdx.x] = out[2];
-		if (first) {
+		Tp->it = it + 1;
+		if (first_tile) {
 			a.hist[4 * scn + 2 * cur] = gamma;
 			a.hist[4 * scn + 2 * cur + 1] = alpha;
-			a.ps[scn].itc[slot ^ 1] = it + 1;
 		}
 	}
 }
@@ -1399,13 +1466,14 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg2_init(PcgArgs<T> a) {
 	__shared__ double lds[4 * 16];
-	const Tile tile = a.tiles[blockIdx.x];
-	if (tile.scene < 0 || !a.ss[tile.scene].run) return;
+	PcgTile* Tp = a.tiles + blockIdx.x;
+	if (pcg_tile_skip(a, Tp)) return;
+	const int scn = Tp->scene, cnt = Tp->count, first = Tp->first;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[2] = {0, 0};
-	if (wid < tile.count && i < 6 && sl == 0) {
-		const int br = tile.first + wid;
+	if (wid < cnt && i < 6 && sl == 0) {
+		const int br = first + wid;
 		const int t = 6 * br + i;
 		double zt = 0;
 #pragma unroll
@@ -1425,24 +1493,25 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_init(PcgArgs<T> a) {
 		a.rr[0][blockIdx.x] = out[1];
 		a.bb[blockIdx.x] = out[1];
 	}
-	pcg_scene_reset(a, tile.scene);
+	pcg_tile_reset(a, Tp, scn);
 }
 
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
-	const Tile tile = a.tiles[blockIdx.x];
-	const int scn = tile.scene;
-	if (scn < 0) return;
-	const PcgScene c = a.ps[scn];
-	if (c.done || !a.ss[scn].run) return;
-	const int it = slot ? c.itc[1] : c.itc[0];  // no dynamic index into a private copy
+	(void)slot;
+	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
+	const int scn = Tp->scene;
+	if (scn < 0 || Tp->done) return;
+	const int it = Tp->it, t0 = Tp->t0, W = Tp->W, cnt = Tp->count, first = Tp->first;
 	const int cur = it & 1, nxt = cur ^ 1;
-	const int t0 = a.tbeg[scn], W = a.tbeg[scn + 1] - t0;
 	double red[4];
 	reduce_partials<4>(pick(a.rr, cur), a.bb, pick(a.gam, cur), pick(a.gam, nxt), t0, W, red, lds);
 	if (!(red[0] > a.tol2 * red[1])) {
-		if ((int)blockIdx.x == t0 && threadIdx.x == 0) { a.ps[scn].done = 1; a.ps[scn].iters = it; }
+		if (threadIdx.x == 0) {
+			Tp->done = 1;
+			if ((int)blockIdx.x == t0) { a.ps[scn].done = 1; a.ps[scn].iters = it; }
+		}
 		return;
 	}
 	const double beta = it > 0 ? red[2] / red[3] : 0.0;
@@ -1450,9 +1519,9 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) 
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[1] = {0};
-	if (wid < tile.count) {
-		const int br = tile.first + wid;
-		const double n = spmv_lane(a, a.r, br, i, sl);
+	if (wid < cnt) {
+		const int br = first + wid;
+		const double n = spmv_lane_range(a, a.r, Tp->ptr[wid], Tp->ptr[wid + 1], i, sl);
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
 			double yb[6];
@@ -1478,14 +1547,12 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) 
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
-	const Tile tile = a.tiles[blockIdx.x];
-	const int scn = tile.scene;
-	if (scn < 0) return;
-	const PcgScene c = a.ps[scn];
-	if (c.done || !a.ss[scn].run) return;
-	const int it = slot ? c.itc[1] : c.itc[0];  // no dynamic index into a private copy
+	(void)slot;
+	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
+	const int scn = Tp->scene;
+	if (scn < 0 || Tp->done) return;
+	const int it = Tp->it, t0 = Tp->t0, W = Tp->W, cnt = Tp->count, first = Tp->first;
 	const int cur = it & 1, nxt = cur ^ 1;
-	const int t0 = a.tbeg[scn], W = a.tbeg[scn + 1] - t0;
 	double red[2];
 	reduce_partials<2>(pick(a.gam, cur), a.del[0], nullptr, nullptr, t0, W, red, lds);
 	const double alpha = red[0] / red[1];
@@ -1493,8 +1560,8 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) 
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[2] = {0, 0};
-	if (wid < tile.count && i < 6 && sl == 0) {
-		const int br = tile.first + wid;
+	if (wid < cnt && i < 6 && sl == 0) {
+		const int br = first + wid;
 		const int t = 6 * br + i;
 		double rb[6], pt = 0;
 #pragma unroll
@@ -1519,7 +1586,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) 
 	if (threadIdx.x == 0) {
 		pick(a.gam, nxt)[blockIdx.x] = out[0];
 		pick(a.rr, nxt)[blockIdx.x] = out[1];
-		if ((int)blockIdx.x == t0) a.ps[scn].itc[slot ^ 1] = it + 1;
+		Tp->it = it + 1;
 	}
 }
 
@@ -2004,8 +2071,7 @@ static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 	pa.rr[0] = parts + 4 * pcap; pa.rr[1] = parts + 5 * pcap;
 	pa.bb = parts + 6 * pcap;
 	pa.hist = parts + 7 * pcap;
-	pa.tiles = ctx->t_r.p;
-	pa.tbeg = ctx->tb_r.p;
+	pa.tiles = ctx->t_pcg.p;
 	pa.ss = ctx->d_ss.p;
 	pa.ps = ctx->d_ps.p;
 	pa.tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
