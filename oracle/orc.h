@@ -91,8 +91,10 @@ int32_t orc_first_system(orc_scene* s, int32_t C, const int32_t* ids,
  * A is n x n row-major, n a multiple of 6.  Returns 0 on success. */
 int32_t orc_block_cholesky_solve(int32_t n, const double* A, const double* b,
                                  double* x);
+/* Block-Jacobi PCG on a dense SPD matrix (for tests): from x as given when
+ * `warm` is nonzero, else from zero.  Returns the iteration count. */
 int32_t orc_pcg_solve(int32_t n, const double* A, const double* b, double tol,
-                      int32_t maxit, double* x);
+                      int32_t maxit, double* x, int32_t warm);
 
 #ifdef __cplusplus
 }

@@ -67,7 +67,7 @@ def lib():
         L.orc_block_cholesky_solve.restype = ctypes.c_int32
         L.orc_block_cholesky_solve.argtypes = [ctypes.c_int32, _D, _D, _D]
         L.orc_pcg_solve.restype = ctypes.c_int32
-        L.orc_pcg_solve.argtypes = [ctypes.c_int32, _D, _D, ctypes.c_double, ctypes.c_int32, _D]
+        L.orc_pcg_solve.argtypes = [ctypes.c_int32, _D, _D, ctypes.c_double, ctypes.c_int32, _D, ctypes.c_int32]
         _lib = L
     return _lib
 
@@ -251,9 +251,10 @@ def block_cholesky_solve(A, b):
     return x
 
 
-def pcg_solve(A, b, tol=1e-12, maxit=1000):
+def pcg_solve(A, b, tol=1e-12, maxit=1000, x0=None):
+    """Block-Jacobi PCG from x0 (zero when None); returns (x, iterations)."""
     A = _f64(A)
     b = _f64(b)
-    x = np.zeros_like(b)
-    it = lib().orc_pcg_solve(A.shape[0], _d(A), _d(b), float(tol), int(maxit), _d(x))
+    x = np.zeros_like(b) if x0 is None else _f64(x0).copy()
+    it = lib().orc_pcg_solve(A.shape[0], _d(A), _d(b), float(tol), int(maxit), _d(x), int(x0 is not None))
     return x, it

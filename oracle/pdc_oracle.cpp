@@ -548,10 +548,11 @@ void bcsr_mv(const BCSR& A, const double* x, double* y) {
 	}
 }
 
-/* Block-Jacobi PCG, zero initial guess.  Iterates while it < maxit and
- * |r| > tol |b| (the convergence test of a textbook CG, as in AMGCL's cg).
- * Returns the iteration count. */
-int bcsr_pcg(const BCSR& A, const double* b, double tol, int maxit, double* x) {
+/* Block-Jacobi PCG.  Starts from x as given when `warm` (r = b - A x), else
+ * from zero.  Iterates while it < maxit and |r| > tol |b| (the convergence
+ * test of a textbook CG, as in AMGCL's cg, whose initial guess is the caller's
+ * X: PrimalDualForceSolver.cpp:406-414).  Returns the iteration count. */
+int bcsr_pcg(const BCSR& A, const double* b, double tol, int maxit, double* x, bool warm) {
 	const int n = A.n;
 	const size_t m = 6 * (size_t)n;
 	std::vector<Blk> Pinv(n);
@@ -573,7 +574,12 @@ int bcsr_pcg(const BCSR& A, const double* b, double tol, int maxit, double* x) {
 		}
 	}
 	std::vector<double> r(b, b + m), z(m), p(m), q(m);
-	std::fill(x, x + m, 0.0);
+	if (warm) {
+		bcsr_mv(A, x, q.data());
+		for (size_t i = 0; i < m; ++i) r[i] = b[i] - q[i];
+	} else {
+		std::fill(x, x + m, 0.0);
+	}
 	auto prec = [&](const std::vector<double>& in, std::vector<double>& out) {
 #pragma omp parallel for schedule(static)
 		for (int k = 0; k < n; ++k)
@@ -1389,7 +1395,12 @@ struct Solver {
 				if (chol.factorize(A)) chol.solve(rhs.data(), z.data());
 				else std::fill(z.begin(), z.end(), 0.0);
 			} else {
-				iters = bcsr_pcg(A, rhs.data(), p.pcg_tol, p.pcg_max_iter, z.data());
+				/* initial guess: the previous iteration's du (the reference hands
+				 * AMGCL X = du, the unscaled increment, or -ru after a gradient-
+				 * descent fallback; :407-414, :451); zero in the first iteration,
+				 * where the reference's du is freshly allocated */
+				if (it > 0) z = du;
+				iters = bcsr_pcg(A, rhs.data(), p.pcg_tol, p.pcg_max_iter, z.data(), it > 0);
 				pcgTotal += iters;
 				if (iters == p.pcg_max_iter) pcgHits++;
 			}
@@ -1401,9 +1412,8 @@ struct Solver {
 			if (ls.step == 0) {
 				useGD = true;
 				gdCount++;
-				std::vector<double> gdu(nR);
-				for (size_t k = 0; k < nR; ++k) gdu[k] = -R.ru[k];
-				ls = line_search(v, f, lam, gdu, R.rf, R.rl, vold, sErr, sacErr, muc, 20);
+				for (size_t k = 0; k < nR; ++k) du[k] = -R.ru[k];
+				ls = line_search(v, f, lam, du, R.rf, R.rl, vold, sErr, sacErr, muc, 20);
 				trials += ls.trials;
 			}
 			v = ls.v; f = ls.f; lam = ls.l;
@@ -1726,11 +1736,11 @@ int32_t orc_block_cholesky_solve(int32_t n, const double* A, const double* b, do
 	return 0;
 }
 int32_t orc_pcg_solve(int32_t n, const double* A, const double* b, double tol, int32_t maxit,
-                      double* x) {
+                      double* x, int32_t warm) {
 	if (n % 6) return -1;
 	BCSR B;
 	dense_to_bcsr(n, A, B);
-	return bcsr_pcg(B, b, tol, maxit, x);
+	return bcsr_pcg(B, b, tol, maxit, x, warm != 0);
 }
 
 }  // extern "C"

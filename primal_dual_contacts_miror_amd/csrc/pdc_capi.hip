@@ -103,7 +103,7 @@ void free_all(pdc_context* c) {
 	c->s_H.release(); c->s_rf.release(); c->s_rl.release(); c->s_sf.release(); c->s_invC.release();
 	c->s_W.release(); c->s_kf.release(); c->s_df.release(); c->s_dl.release(); c->partials.release();
 	c->A32.release(); c->P32.release(); c->b32.release();
-	c->A64.release(); c->P64.release(); c->b64.release(); c->x64.release(); c->pcg64.release();
+	c->A64.release(); c->P64.release(); c->b64.release(); c->x64.release(); c->y64.release(); c->pcg64.release();
 	c->d_sc.release();
 	c->b_scene.release(); c->d_sbeg.release(); c->d_step.release();
 	c->r_cbeg.release(); c->r_lbeg.release(); c->r_pbeg.release(); c->r_nnzb.release();

@@ -241,51 +241,72 @@ __device__ __forceinline__ bool aabb_overlap(const double* A, const double* B) {
 	return true;
 }
 
-// Broadphase, pass 1: candidates b > a per body a, within a's scene, for the
-// scenes stepped by this call.
-__global__ __launch_bounds__(256) void k_bp_count(int N, const double* __restrict__ aabb, const int* __restrict__ stat,
-                           const int* __restrict__ bscene, const int* __restrict__ sbeg,
-                           const int* __restrict__ stepped, int* __restrict__ cnt) {
-	const int a = blockIdx.x * blockDim.x + threadIdx.x;
-	if (a >= N) return;
-	const int sc = bscene[a];
-	if (!stepped[sc]) { cnt[a] = 0; return; }
-	const int end = sbeg[sc + 1];
-	double A[6];
-	for (int k = 0; k < 6; ++k) A[k] = aabb[6 * a + k];
-	const int sa = stat[a];
-	int c = 0;
-	for (int b = a + 1; b < end; ++b) {
-		if (sa && stat[b]) continue;
-		if (aabb_overlap(A, &aabb[6 * b])) ++c;
-	}
-	cnt[a] = c;
+// Broadphase over the scene's bodies, one wavefront per body a: the lanes
+// test the candidates b > a of a's scene 64 at a time, and a ballot gives the
+// count (pass 1) or each hit's place in the (a, b)-ordered output (pass 2).
+// Static-static pairs are skipped (DynamicSystem.cpp:326-331).  Only the
+// scenes stepped by this call take part.
+constexpr int kBpWaves = 4;
+
+__device__ __forceinline__ bool bp_candidate(const double* __restrict__ aabb, const int* __restrict__ stat,
+                                             const double (&A)[6], int sa, int b, int end) {
+	if (b >= end || (sa && stat[b])) return false;
+	const double* B = aabb + 6 * (size_t)b;
+	return aabb_overlap(A, B);
 }
 
-// Broadphase, pass 2: write the candidates in (a, b) order.
-__global__ __launch_bounds__(256) void k_bp_write(int N, const double* __restrict__ aabb, const int* __restrict__ stat,
-                           const int* __restrict__ bscene, const int* __restrict__ sbeg,
-                           const int* __restrict__ stepped, const int* __restrict__ off,
-                           int* __restrict__ pa, int* __restrict__ pb, const int* __restrict__ soff,
-                           const int* __restrict__ emit, int* __restrict__ stage_cnt) {
-	const int a = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64 * kBpWaves) void k_bp_count(int N, const double* __restrict__ aabb,
+                                                          const int* __restrict__ stat,
+                                                          const int* __restrict__ bscene,
+                                                          const int* __restrict__ sbeg,
+                                                          const int* __restrict__ stepped, int* __restrict__ cnt) {
+	const int a = blockIdx.x * kBpWaves + (int)(threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (a >= N) return;
+	const int sc = bscene[a];
+	int c = 0;
+	if (stepped[sc]) {
+		const int end = sbeg[sc + 1];
+		double A[6];
+		for (int k = 0; k < 6; ++k) A[k] = aabb[6 * (size_t)a + k];
+		const int sa = stat[a];
+		for (int base = a + 1; base < end; base += 64)
+			c += __popcll(__ballot(bp_candidate(aabb, stat, A, sa, base + lane, end)));
+	}
+	if (lane == 0) cnt[a] = c;
+}
+
+__global__ __launch_bounds__(64 * kBpWaves) void k_bp_write(int N, const double* __restrict__ aabb,
+                                                          const int* __restrict__ stat,
+                                                          const int* __restrict__ bscene,
+                                                          const int* __restrict__ sbeg,
+                                                          const int* __restrict__ stepped,
+                                                          const int* __restrict__ off, int* __restrict__ pa,
+                                                          int* __restrict__ pb, const int* __restrict__ soff,
+                                                          const int* __restrict__ emit, int* __restrict__ stage_cnt) {
+	const int a = blockIdx.x * kBpWaves + (int)(threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
 	if (a >= N) return;
 	const int sc = bscene[a];
 	if (!stepped[sc]) return;
 	const int end = sbeg[sc + 1];
 	double A[6];
-	for (int k = 0; k < 6; ++k) A[k] = aabb[6 * a + k];
+	for (int k = 0; k < 6; ++k) A[k] = aabb[6 * (size_t)a + k];
 	const int sa = stat[a];
 	const int na = emit[a] ? soff[a + 1] - soff[a] : 0;
+	const unsigned long long below = (1ull << lane) - 1;
 	int o = off[a];
-	for (int b = a + 1; b < end; ++b) {
-		if (sa && stat[b]) continue;
-		if (aabb_overlap(A, &aabb[6 * b])) {
-			pa[o] = a;
-			pb[o] = b;
-			stage_cnt[o] = na + (emit[b] ? soff[b + 1] - soff[b] : 0);
-			++o;
+	for (int base = a + 1; base < end; base += 64) {
+		const int b = base + lane;
+		const bool hit = bp_candidate(aabb, stat, A, sa, b, end);
+		const unsigned long long m = __ballot(hit);
+		if (hit) {
+			const int k = o + __popcll(m & below);
+			pa[k] = a;
+			pb[k] = b;
+			stage_cnt[k] = na + (emit[b] ? soff[b + 1] - soff[b] : 0);
 		}
+		o += __popcll(m);
 	}
 }
 
@@ -558,7 +579,8 @@ void detect_contacts(pdc_context* ctx) {
 	PDC_CHECK_LAUNCH();
 	ctx->bp_cnt.ensure(N + 1);
 	ctx->bp_off.ensure(N + 1);
-	hipLaunchKernelGGL(k_bp_count, dim3(grid_for(N, 64)), dim3(64), 0, s, N, ctx->b_aabb.p,
+	const int bp_grid = (N + kBpWaves - 1) / kBpWaves;
+	hipLaunchKernelGGL(k_bp_count, dim3(bp_grid), dim3(64 * kBpWaves), 0, s, N, ctx->b_aabb.p,
 	                   ctx->b_static.p, ctx->b_scene.p, ctx->d_sbeg.p, ctx->d_step.p, ctx->bp_cnt.p);
 	PDC_CHECK_LAUNCH();
 	scan_exclusive(ctx, ctx->bp_cnt.p, ctx->bp_off.p, N);
@@ -567,7 +589,7 @@ void detect_contacts(pdc_context* ctx) {
 	ctx->pr_a.ensure(P); ctx->pr_b.ensure(P); ctx->pr_tmp.ensure(P + 1);
 	ctx->pr_soff.ensure(P + 1); ctx->pr_nc.ensure(P + 1); ctx->pr_coff.ensure(P + 1);
 	ctx->pr_nh.ensure(P + 1);
-	hipLaunchKernelGGL(k_bp_write, dim3(grid_for(N, 64)), dim3(64), 0, s, N, ctx->b_aabb.p,
+	hipLaunchKernelGGL(k_bp_write, dim3(bp_grid), dim3(64 * kBpWaves), 0, s, N, ctx->b_aabb.p,
 	                   ctx->b_static.p, ctx->b_scene.p, ctx->d_sbeg.p, ctx->d_step.p, ctx->bp_off.p,
 	                   ctx->pr_a.p, ctx->pr_b.p, ctx->b_soff.p, ctx->b_emit.p, ctx->pr_tmp.p);
 	PDC_CHECK_LAUNCH();

@@ -230,7 +230,8 @@ struct pdc_context {
 	pdc::DBuf<double> s_rf, s_rl, s_sf, s_invC, s_W, s_kf, s_df, s_dl;
 	pdc::DBuf<double> partials;                              // [k][kMaxPartialBlocks]
 	pdc::DBuf<float> A32, P32, b32;                           // FP32 PCG operator, preconditioner, rhs
-	pdc::DBuf<double> A64, P64, b64, x64, pcg64;             // FP64 PCG
+	pdc::DBuf<double> A64, P64, b64, pcg64;                  // FP64 PCG
+	pdc::DBuf<double> x64, y64;                              // PCG solution (from its start x0) and M^-1 x0
 	pdc::DBuf<pdc::Scalars> d_sc;
 	pdc::Scalars* h_sc = nullptr;                            // pinned mirror
 

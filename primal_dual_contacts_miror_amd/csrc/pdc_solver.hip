@@ -442,7 +442,9 @@ struct SArgs {
 	double* ru;
 	double* rhs;
 	double* D;
-	double* du;
+	double* du;           // Newton increment; after the commit, the next PCG's start
+	double* xs;           // PCG start x0 (then its solution) and S x0 (S: diagonal block)
+	double* ys;
 	// contact state
 	double* K;
 	double* f;
@@ -1051,6 +1053,18 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 				for (int q = 0; q < 36; ++q) Si[q] = 0;
 				for (int q = 0; q < 6; ++q) Si[q * 6 + q] = 1.0 / S[q * 6 + q];
 			}
+			// PCG start: the previous iteration's increment, zero in the first
+			// (PrimalDualForceSolver.cpp:407-414 hands AMGCL X = du), and S x0,
+			// which the stored operator A P maps to A x0
+			const bool warm = a.ss[scn].it > 0;
+			double x0[6];
+			for (int q = 0; q < 6; ++q) x0[q] = warm ? a.du[6 * r + q] : 0.0;
+			for (int ii = 0; ii < 6; ++ii) {
+				double acc = 0;
+				for (int k = 0; k < 6; ++k) acc += S[ii * 6 + k] * x0[k];
+				a.xs[6 * r + ii] = x0[ii];
+				a.ys[6 * r + ii] = acc;
+			}
 			// the preconditioner as stored, and the diagonal block of A P with it
 			for (int q = 0; q < 36; ++q) Si[q] = (double)(T)Si[q];
 			for (int q = 0; q < 36; ++q) Pval[36 * (size_t)r + q] = (T)Si[q];
@@ -1110,13 +1124,15 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ 
 }
 
 // ---- (d) PCG ----------------------------------------------------------------
-// Block-Jacobi preconditioned CG on the scaled block-CSR system with a zero
-// initial guess, in the single-reduction form of Chronopoulos and Gear
+// Block-Jacobi preconditioned CG on the scaled block-CSR system from the
+// initial guess x0 (the previous Newton increment, zero in the first
+// iteration), in the single-reduction form of Chronopoulos and Gear
 // ("s-step iterative methods for symmetric linear systems", J. Comput. Appl.
 // Math. 25 (1989); the preconditioned recurrences as listed by Ghysels and
 // Vanroose, Parallel Computing 40 (2014), Alg. 3).  With M = P the
 // block-Jacobi inverse and the assembled operator B = A M (B_ij = A_ij P_j):
-//   init: r = b, u = M b, w = A u = B b, z = q = s = p = 0
+//   start: x = x0, r = b - A x0 = b - B (M^-1 x0)
+//   init: u = M r, w = A u = B r, z = q = s = p = 0
 //   iteration i:  gamma = (r, u), delta = (w, u), |r|^2   (from the partials)
 //     stop when |r|^2 <= tol^2 |b|^2
 //     beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
@@ -1286,6 +1302,22 @@ __device__ __forceinline__ bool pcg_tile_skip(const PcgArgs<T>& a, PcgTile* Tp) 
 	return false;
 }
 
+// Initial residual of both forms, r = b - A x0 = b - B y with y = M^-1 x0
+// (x0 itself is already in x).  A zero start has y = 0 and gives r = b exactly.
+template <class T>
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_start(PcgArgs<T> a, const double* __restrict__ y) {
+	PcgTile* Tp = a.tiles + blockIdx.x;
+	if (pcg_tile_skip(a, Tp)) return;
+	const int cnt = Tp->count, first = Tp->first;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const int i = lane >> 3, sl = lane & 7;
+	if (wid < cnt) {
+		const int br = first + wid;
+		const double ay = spmv_lane(a, y, br, i, sl);
+		if (i < 6 && sl == 0) a.r[6 * br + i] = (double)a.b[6 * br + i] - ay;
+	}
+}
+
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	__shared__ double lds[4 * 16];
@@ -1297,24 +1329,23 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	double v[4] = {0, 0, 0, 0};
 	if (wid < cnt) {
 		const int br = first + wid;
-		const double w0 = spmv_lane(a, a.b, br, i, sl);
+		const double w0 = spmv_lane(a, a.r, br, i, sl);
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
-			const double bt = a.b[t];
+			const double rt = a.r[t], bt = a.b[t];
 			double ut = 0;
 #pragma unroll
-			for (int c = 0; c < 6; ++c) ut += (double)a.P[36 * (size_t)br + 6 * i + c] * (double)a.b[6 * br + c];
-			a.x[t] = 0;
-			a.r[t] = bt;
+			for (int c = 0; c < 6; ++c) ut += (double)a.P[36 * (size_t)br + 6 * i + c] * a.r[6 * br + c];
 			a.u[t] = ut;
 			a.w[0][t] = w0;
 			a.z[t] = 0;
 			a.q[t] = 0;
 			a.s[t] = 0;
 			a.p[t] = 0;
-			v[0] += bt * ut;
+			v[0] += rt * ut;
 			v[1] += w0 * ut;
-			v[2] += bt * bt;
+			v[2] += rt * rt;
+			v[3] += bt * bt;
 		}
 	}
 	double out[4];
@@ -1323,7 +1354,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 		a.gam[0][blockIdx.x] = out[0];
 		a.del[0][blockIdx.x] = out[1];
 		a.rr[0][blockIdx.x] = out[2];
-		a.bb[blockIdx.x] = out[2];
+		a.bb[blockIdx.x] = out[3];
 	}
 	pcg_tile_reset(a, Tp, scn);
 }
@@ -1454,7 +1485,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 // Two-reduction form (standard PCG), used for tolerances below the attainable
 // accuracy of the single-reduction recurrences.  It runs on the same stored
 // operator B = A M through y with p = M y, so that A p = B y:
-//   init: r = b, x = y = q = 0; gamma = (r, M r), |r|^2
+//   init: x = x0, r = b - A x0 (as above), y = q = 0; gamma = (r, M r), |r|^2
 //   A(i): stop when |r|^2 <= tol^2 |b|^2; beta = gamma / gamma_prev (0 at i = 0)
 //         y = r + beta y, q = B r + beta q (= A p), partial (M y, q)
 //   B(i): alpha = gamma / (p, q); x += alpha M y, r -= alpha q,
@@ -1471,27 +1502,26 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_init(PcgArgs<T> a) {
 	const int scn = Tp->scene, cnt = Tp->count, first = Tp->first;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
-	double v[2] = {0, 0};
+	double v[3] = {0, 0, 0};
 	if (wid < cnt && i < 6 && sl == 0) {
 		const int br = first + wid;
 		const int t = 6 * br + i;
 		double zt = 0;
 #pragma unroll
-		for (int c = 0; c < 6; ++c) zt += (double)a.P[36 * (size_t)br + 6 * i + c] * (double)a.b[6 * br + c];
-		const double bt = a.b[t];
-		a.x[t] = 0;
-		a.r[t] = bt;
+		for (int c = 0; c < 6; ++c) zt += (double)a.P[36 * (size_t)br + 6 * i + c] * a.r[6 * br + c];
+		const double rt = a.r[t], bt = a.b[t];
 		a.p[t] = 0;  // y
 		a.q[t] = 0;
-		v[0] += bt * zt;
-		v[1] += bt * bt;
+		v[0] += rt * zt;
+		v[1] += rt * rt;
+		v[2] += bt * bt;
 	}
-	double out[2];
-	block_sum<2, double>(v, lds, out);
+	double out[3];
+	block_sum<3, double>(v, lds, out);
 	if (threadIdx.x == 0) {
 		a.gam[0][blockIdx.x] = out[0];
 		a.rr[0][blockIdx.x] = out[1];
-		a.bb[blockIdx.x] = out[1];
+		a.bb[blockIdx.x] = out[2];
 	}
 	pcg_tile_reset(a, Tp, scn);
 }
@@ -1872,15 +1902,19 @@ __global__ __launch_bounds__(256) void k_ls_commit(SArgs a, const double* __rest
 	}
 }
 
-__global__ __launch_bounds__(256) void k_commit_v(SArgs a, const double* __restrict__ du,
-                                                  const double* __restrict__ gdu) {
+// After a gradient-descent fallback the increment becomes -ru, as in the
+// reference (:451), and so does the next PCG's start.
+__global__ __launch_bounds__(256) void k_commit_v(SArgs a, const double* __restrict__ gdu) {
 	PDC_TILE_RUN(a.tb);
 	const double step = a.ss[scn].ls_step;
-	if (!(step > 0)) return;
-	const double* d = a.ss[scn].ls_state == LS_GD_DECIDED ? gdu : du;
+	const bool gd = a.ss[scn].ls_state == LS_GD_DECIDED;
 	if ((int)threadIdx.x < tile.count) {
 		const int r = tile.first + threadIdx.x;
-		for (int k = 0; k < 6; ++k) a.v[6 * r + k] = a.v[6 * r + k] + 0.99 * (step * d[6 * r + k]);
+		for (int k = 0; k < 6; ++k) {
+			const double d = gd ? gdu[6 * r + k] : a.du[6 * r + k];
+			if (step > 0) a.v[6 * r + k] = a.v[6 * r + k] + 0.99 * (step * d);
+			if (gd) a.du[6 * r + k] = d;
+		}
 	}
 }
 
@@ -2011,7 +2045,7 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	a.inc_ptr = ctx->inc_ptr.p; a.inc = ctx->inc.p; a.pair_adj = ctx->pair_adj.p; a.adj_blk = ctx->adj_blk.p;
 	a.v = ctx->l_v.p; a.vold = ctx->l_vold.p; a.vphys = ctx->l_vphys.p; a.Ms = ctx->l_Ms.p;
 	a.Minv = ctx->l_Minv.p; a.speed = ctx->l_speed.p; a.ru = ctx->l_ru.p; a.rhs = ctx->l_rhs.p;
-	a.D = ctx->l_D.p; a.du = ctx->l_du.p;
+	a.D = ctx->l_D.p; a.du = ctx->l_du.p; a.xs = ctx->x64.p; a.ys = ctx->y64.p;
 	a.K = ctx->s_K.p; a.f = ctx->s_f.p; a.lam = ctx->s_lam.p; a.c = ctx->s_c.p; a.g = ctx->s_g.p;
 	a.H = ctx->s_H.p; a.rf = ctx->s_rf.p; a.rl = ctx->s_rl.p; a.sf = ctx->s_sf.p;
 	a.invC = ctx->s_invC.p; a.W = ctx->s_W.p; a.kf = ctx->s_kf.p; a.df = ctx->s_df.p;
@@ -2077,6 +2111,7 @@ static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 	pa.tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
 	hipStream_t s = ctx->stream;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
+	hipLaunchKernelGGL(k_pcg_start<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa, (const double*)ctx->y64.p);
 	if (two) hipLaunchKernelGGL(k_pcg2_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
 	else hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
 	PDC_CHECK_LAUNCH();
@@ -2226,6 +2261,7 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	ctx->l_Ms.ensure(7 * nb); ctx->l_Minv.ensure(7 * nb); ctx->l_speed.ensure(nb);
 	ctx->l_ru.ensure(6 * nb); ctx->l_rhs.ensure(6 * nb); ctx->l_D.ensure(6 * nb);
 	ctx->l_du.ensure(6 * nb); ctx->l_gdu.ensure(6 * nb);
+	ctx->x64.ensure(6 * (size_t)std::max(ctx->N, 1)); ctx->y64.ensure(6 * (size_t)std::max(ctx->N, 1));
 	const bool fp32 = ctx->p.pcg_precision == PDC_PCG_FP32;
 	const size_t nnzb = (size_t)ctx->NNZB;
 	if (fp32) {
@@ -2320,7 +2356,7 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		}
 		launch(k_ls_commit, ntc, kBlock, a, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
 		       (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p);
-		launch(k_commit_v, ntb, kBlock, a, du, gdu);
+		launch(k_commit_v, ntb, kBlock, a, gdu);
 		launch(k_newton_end, gS, kBlock, a, ctx->p.pcg_max_iter, ctx->p.iterations);
 		PDC_CHECK_LAUNCH();
 	}

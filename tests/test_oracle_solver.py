@@ -173,6 +173,16 @@ def test_linear_solvers_agree_with_dense():
     # zero right-hand side: no iteration, zero solution
     x0, it0 = orc.pcg_solve(A, np.zeros(n))
     assert it0 == 0 and np.all(x0 == 0)
+    # initial guess: the exact solution needs no iteration; a perturbed one
+    # converges to the same solution, and the test is still relative to |b|
+    xs, its = orc.pcg_solve(A, b, tol=1e-9, x0=x)
+    assert its == 0 and np.array_equal(xs, x)
+    xw, itw = orc.pcg_solve(A, b, tol=1e-13, maxit=500, x0=x + 1e-3 * rng.normal(size=n))
+    np.testing.assert_allclose(xw, x, rtol=1e-9, atol=1e-11)
+    assert 0 < itw < it
+    # a start far from the solution still converges (the residual is b - A x0)
+    xf, _ = orc.pcg_solve(A, b, tol=1e-13, maxit=500, x0=1e3 * rng.normal(size=n))
+    np.testing.assert_allclose(xf, x, rtol=1e-8, atol=1e-9)
 
 
 def test_direct_and_pcg_branches_agree():
