@@ -304,8 +304,7 @@ int pdc_get_contacts(pdc_context* ctx, int32_t capacity, int32_t* obj_ids, doubl
 			PDC_HIP(hipMemcpyAsync(&L[k * (size_t)C], ctx->c_lf.p + k * cs, C * sizeof(double), hipMemcpyDeviceToHost, s));
 		}
 		PDC_HIP(hipMemcpyAsync(Dp.data(), ctx->c_depth.p, C * sizeof(double), hipMemcpyDeviceToHost, s));
-		for (int k = 0; k < 36; ++k)
-			PDC_HIP(hipMemcpyAsync(&T[k * (size_t)C], ctx->c_T.p + k * cs, C * sizeof(double), hipMemcpyDeviceToHost, s));
+		PDC_HIP(hipMemcpyAsync(T.data(), ctx->c_T.p, 36 * (size_t)C * sizeof(double), hipMemcpyDeviceToHost, s));
 		PDC_HIP(hipStreamSynchronize(s));
 		for (int i = 0; i < C; ++i) {
 			if (obj_ids) { obj_ids[2 * i] = i0[i]; obj_ids[2 * i + 1] = i1[i]; }
@@ -315,7 +314,7 @@ int pdc_get_contacts(pdc_context* ctx, int32_t capacity, int32_t* obj_ids, doubl
 			}
 			if (depth) depth[i] = Dp[i];
 			if (transformation)
-				for (int k = 0; k < 36; ++k) transformation[36 * (size_t)i + k] = T[k * (size_t)C + i];
+				for (int k = 0; k < 36; ++k) transformation[36 * (size_t)i + k] = T[36 * (size_t)i + k];
 		}
 	});
 }
@@ -361,7 +360,6 @@ int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids
 		ctx->C = 0;
 		if (C > 0) {
 			pdc::ensure_contact_capacity(ctx, C);
-			const size_t cs = (size_t)ctx->c_stride;
 			std::vector<int> i0(C), i1(C);
 			std::vector<double> Dp(C), T(36 * (size_t)C);
 			for (int k = 0; k < C; ++k) {
@@ -369,14 +367,12 @@ int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids
 				i0[k] = obj_ids[2 * i];
 				i1[k] = obj_ids[2 * i + 1];
 				Dp[k] = depth ? depth[i] : 0.0;
-				for (int c = 0; c < 36; ++c) T[c * (size_t)C + k] = transformation[36 * (size_t)i + c];
+				for (int c = 0; c < 36; ++c) T[36 * (size_t)k + c] = transformation[36 * (size_t)i + c];
 			}
 			PDC_HIP(hipMemcpyAsync(ctx->c_id0.p, i0.data(), C * sizeof(int), hipMemcpyHostToDevice, s));
 			PDC_HIP(hipMemcpyAsync(ctx->c_id1.p, i1.data(), C * sizeof(int), hipMemcpyHostToDevice, s));
 			PDC_HIP(hipMemcpyAsync(ctx->c_depth.p, Dp.data(), C * sizeof(double), hipMemcpyHostToDevice, s));
-			for (int c = 0; c < 36; ++c)
-				PDC_HIP(hipMemcpyAsync(ctx->c_T.p + c * cs, &T[c * (size_t)C], C * sizeof(double),
-				                       hipMemcpyHostToDevice, s));
+			PDC_HIP(hipMemcpyAsync(ctx->c_T.p, T.data(), 36 * (size_t)C * sizeof(double), hipMemcpyHostToDevice, s));
 			PDC_HIP(hipStreamSynchronize(s));
 			ctx->C = C;
 		}

@@ -561,9 +561,9 @@ __global__ __launch_bounds__(256) void k_pair_write(const int* __restrict__ pa, 
 		for (int k = 0; k < 3; ++k) cpos[(size_t)k * C + i] = s[k];
 		cdepth[i] = fabs(s[6]);
 		frame_T(R, s, xa, 0, T);
-		for (int k = 0; k < 18; ++k) cT[(size_t)k * C + i] = T[k];
+		for (int k = 0; k < 18; ++k) cT[36 * (size_t)i + k] = T[k];
 		frame_T(R, s, xb, 1, T);
-		for (int k = 0; k < 18; ++k) cT[(size_t)(18 + k) * C + i] = T[k];
+		for (int k = 0; k < 18; ++k) cT[36 * (size_t)i + 18 + k] = T[k];
 		for (int k = 0; k < 3; ++k) clf[(size_t)k * C + i] = 0.0;
 	}
 }

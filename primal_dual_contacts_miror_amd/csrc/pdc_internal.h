@@ -192,7 +192,7 @@ struct pdc_context {
 	pdc::DBuf<int> c_id0, c_id1;
 	pdc::DBuf<double> c_pos;    // [3][C]
 	pdc::DBuf<double> c_depth;  // [C]
-	pdc::DBuf<double> c_T;      // [36][C]: (side*18 + col*6 + row)
+	pdc::DBuf<double> c_T;      // [C][36]: (side*18 + col*6 + row), one contact's 36 values together
 	pdc::DBuf<double> c_lf;     // [3][C]
 	int c_stride = 0;           // SoA row stride (the contact capacity)
 	int n_pairs = 0, n_raw = 0, n_ties = 0;

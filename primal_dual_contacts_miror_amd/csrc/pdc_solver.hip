@@ -23,7 +23,8 @@
 namespace pdc {
 
 constexpr int kMaxWG = 1024;  // workgroups of the reducing 1-D kernels
-constexpr int kTrials = 10;   // line-search trial steps evaluated per fused pass
+constexpr int kTrials = 10;   // line-search trials per gradient-descent pass (the most per pass)
+constexpr int kGroup = 5;     // line-search trials per pass in the Newton direction
 enum {
 	P_GAP = 0, P_WRF, P_RL, P_AC, P_RU, P_ND_TRACE, P_ND_CNT,
 	P_LS,                                  // 7 slots per trial: wrf2 rl2 ac2 gf gl ru2 gu
@@ -420,7 +421,7 @@ struct SArgs {
 	// contacts
 	const int* li0;
 	const int* li1;
-	const double* T;        // [36][cs]
+	const double* T;        // [C][36]
 	const double* depth;
 	// topology
 	const int* pstart;    // per pair: first contact (NP+1)
@@ -464,7 +465,7 @@ struct SArgs {
 };
 
 __device__ __forceinline__ double Tc(const SArgs& a, int i, int j, int k) {
-	return a.T[(size_t)(j * 18 + k) * a.cs + i];
+	return a.T[36 * (size_t)i + j * 18 + k];
 }
 
 // u = toLocal(v) = -T0^T v_a - T1^T v_b (ContactInfo.h:46-51); v given per side.
@@ -1140,6 +1141,8 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ 
 //     m = M w, n = A m = B w
 //     z = n + beta z, q = m + beta q, s = w + beta s, p = u + beta p
 //     x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z
+//   u = M r and q = M s hold throughout; neither is stored: a block row
+//   forms M r and M s from its own r and s and the recurrences above from them
 //     partials of (r, u), (w, u), (r, r) for iteration i + 1
 // One kernel per iteration, so the kernel boundary is the only grid-wide
 // synchronisation.  Every workgroup re-reduces the partial sums in the same
@@ -1336,10 +1339,8 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 			double ut = 0;
 #pragma unroll
 			for (int c = 0; c < 6; ++c) ut += (double)a.P[36 * (size_t)br + 6 * i + c] * a.r[6 * br + c];
-			a.u[t] = ut;
 			a.w[0][t] = w0;
 			a.z[t] = 0;
-			a.q[t] = 0;
 			a.s[t] = 0;
 			a.p[t] = 0;
 			v[0] += rt * ut;
@@ -1360,9 +1361,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 }
 
 // Operator row and row vectors of one block row as read by k_pcg_iter
-// (meaningful in the lanes with i < 6, sl == 0).
+// (meaningful in the lanes with i < 6, sl == 0).  u = M r and q = M s are
+// not stored: each lane forms its entry from the block's r and s.
 struct PcgRow {
-	double n, m, w, z, q, s, p, u, x, r;
+	double n, m, w, z, s, p, x, r, u, ms;
 };
 
 // Iteration parity equals the launch slot: every chunk has an even number of
@@ -1415,13 +1417,20 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 		R.n = spmv_lane_range(a, wc, k0, k1, i, sl);
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
-			double m = 0;
+			double m = 0, u = 0, ms = 0;
 #pragma unroll
-			for (int cc = 0; cc < 6; ++cc) m += (double)a.P[36 * (size_t)br + 6 * i + cc] * wc[6 * br + cc];
+			for (int cc = 0; cc < 6; ++cc) {
+				const double pc = (double)a.P[36 * (size_t)br + 6 * i + cc];
+				m += pc * wc[6 * br + cc];
+				u += pc * a.r[6 * br + cc];
+				ms += pc * a.s[6 * br + cc];
+			}
 			R.m = m;
+			R.u = u;
+			R.ms = ms;
 			R.w = wc[t];
-			R.z = a.z[t]; R.q = a.q[t]; R.s = a.s[t]; R.p = a.p[t];
-			R.u = a.u[t]; R.x = a.x[t]; R.r = a.r[t];
+			R.z = a.z[t]; R.s = a.s[t]; R.p = a.p[t];
+			R.x = a.x[t]; R.r = a.r[t];
 		}
 	}
 	if (wid < 4) {
@@ -1450,19 +1459,17 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	if (wid < cnt && i < 6 && sl == 0) {
 		const int t = 6 * br + i;
 		const double zt = R.n + beta * R.z;
-		const double qt = R.m + beta * R.q;
+		const double qt = R.m + beta * R.ms;   // M s of the updated s
 		const double st = R.w + beta * R.s;
 		const double pt = R.u + beta * R.p;
 		a.z[t] = zt;
-		a.q[t] = qt;
 		a.s[t] = st;
 		a.p[t] = pt;
 		a.x[t] = R.x + alpha * pt;
 		const double rt = R.r - alpha * st;
-		const double ut = R.u - alpha * qt;
+		const double ut = R.u - alpha * qt;    // M r of the updated r
 		const double wnt = R.w - alpha * zt;
 		a.r[t] = rt;
-		a.u[t] = ut;
 		wn[t] = wnt;
 		v[0] += rt * ut;
 		v[1] += wnt * ut;
@@ -1668,8 +1675,9 @@ __global__ __launch_bounds__(256) void k_neg(SArgs a, const double* __restrict__
 
 // ---- (e) line search as fused reductions (PrimalDualForceSolver.cpp:562-635) --
 // The trials of one search share the direction, so they are independent:
-// one pass evaluates kTrials step sizes 2^-k0 ... 2^-(k0+kTrials-1) at once
-// and k_ls_final applies the reference's sequential acceptance rule to them,
+// one pass evaluates nt step sizes 2^-k0 ... 2^-(k0+nt-1) at once (kGroup
+// in the Newton direction, kTrials in the gradient-descent one) and
+// k_ls_final applies the reference's sequential acceptance rule to them,
 // which selects exactly the step the one-trial-at-a-time loop would.
 
 // Filtered trial point of contact i at `step` (NonSmoothContactForce.cpp:281-319,
@@ -1708,49 +1716,46 @@ __device__ __forceinline__ void trial_point(const SArgs& a, int i, const double*
 	for (int k = 0; k < 2; ++k) lI[k] = lam[k] + 0.99 * dli[k];
 }
 
-// Contact part of NT trials k0 .. k0 + NT - 1 of a search in state `want`:
-// partial |wrf'|^2, |rl'|^2, |ac'|^2, df_in.rf' and sum (dl_in / lambda') rl'
-// per trial (fresh constraint values).
-template <int NT>
+// Line-search trials of one pass, a trial per grid row: trial t = blockIdx.y
+// of the pass is the point at step 2^-(k0 + t); its partial sums go to slots
+// P_LS + 7 t.  A pass runs only for the scenes whose search is in state `want`.
+//
+// Contact part: partial |wrf'|^2, |rl'|^2, |ac'|^2, df_in.rf' and
+// sum (dl_in / lambda') rl' (fresh constraint values).
 __global__ __launch_bounds__(256) void k_ls_contact(SArgs a, const double* __restrict__ du,
                                                     const double* __restrict__ dF,
                                                     const double* __restrict__ dL, int k0, int want) {
-	static_assert(NT <= kTrials, "partial slots");
 	PDC_TILE_RUN(a.tc);
 	if (a.ss[scn].ls_state != want) return;
 	const double muc = a.ss[scn].muc, mu = a.ctl[scn].mu, cm = a.ss[scn].char_mass;
-	for (int t = 0; t < NT; ++t) {
-		const double step = ldexp(1.0, -(k0 + t));
-		double acc[5] = {0, 0, 0, 0, 0};
-		if ((int)threadIdx.x < tile.count) {
-			const int i = tile.first + threadIdx.x;
-			double fI[3], lI[2], dfi[3], dli[2];
-			trial_point(a, i, dF, dL, step, mu, fI, lI, dfi, dli);
-			const int l0 = a.li0[i], l1 = a.li1[i];
-			double va[6], vb[6];
-			for (int k = 0; k < 6; ++k) {
-				va[k] = a.v[6 * l0 + k] + 0.99 * (step * du[6 * l0 + k]);
-				vb[k] = a.v[6 * l1 + k] + 0.99 * (step * du[6 * l1 + k]);
-			}
-			double rf[3], rl[2], wrf[3], ac[3];
-			contact_terms(a, i, va, vb, fI, lI, muc, mu, cm, rf, rl, wrf, ac);
-			acc[0] += wrf[0] * wrf[0] + wrf[1] * wrf[1] + wrf[2] * wrf[2];
-			acc[1] += rl[0] * rl[0] + (a.nsc ? rl[1] * rl[1] : 0.0);
-			acc[2] += ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
-			acc[3] += dfi[0] * rf[0] + (a.nsc ? dfi[1] * rf[1] + dfi[2] * rf[2] : 0.0);
-			acc[4] += dli[0] / lI[0] * rl[0] + (a.nsc ? dli[1] / lI[1] * rl[1] : 0.0);
+	const int t = blockIdx.y;
+	const double step = ldexp(1.0, -(k0 + t));
+	double acc[5] = {0, 0, 0, 0, 0};
+	if ((int)threadIdx.x < tile.count) {
+		const int i = tile.first + threadIdx.x;
+		double fI[3], lI[2], dfi[3], dli[2];
+		trial_point(a, i, dF, dL, step, mu, fI, lI, dfi, dli);
+		const int l0 = a.li0[i], l1 = a.li1[i];
+		double va[6], vb[6];
+		for (int k = 0; k < 6; ++k) {
+			va[k] = a.v[6 * l0 + k] + 0.99 * (step * du[6 * l0 + k]);
+			vb[k] = a.v[6 * l1 + k] + 0.99 * (step * du[6 * l1 + k]);
 		}
-		const int b = P_LS + 7 * t;
-		const int slots[5] = {b, b + 1, b + 2, b + 3, b + 4};
-		write_partials<5>(acc, a.part, a.pstride, slots);
+		double rf[3], rl[2], wrf[3], ac[3];
+		contact_terms(a, i, va, vb, fI, lI, muc, mu, cm, rf, rl, wrf, ac);
+		acc[0] = wrf[0] * wrf[0] + wrf[1] * wrf[1] + wrf[2] * wrf[2];
+		acc[1] = rl[0] * rl[0] + (a.nsc ? rl[1] * rl[1] : 0.0);
+		acc[2] = ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
+		acc[3] = dfi[0] * rf[0] + (a.nsc ? dfi[1] * rf[1] + dfi[2] * rf[2] : 0.0);
+		acc[4] = dli[0] / lI[0] * rl[0] + (a.nsc ? dli[1] / lI[1] * rl[1] : 0.0);
 	}
+	const int b = P_LS + 7 * t;
+	const int slots[5] = {b, b + 1, b + 2, b + 3, b + 4};
+	write_partials<5>(acc, a.part, a.pstride, slots);
 }
 
-// Body part of the trials, one wavefront per body: ru' = M (v' - vold) +
-// sum_j T_j f' at the trial steps of group G (kGroup trials per launch, to
-// bound register use); partial |ru'|^2 and du_in . ru'.
-constexpr int kGroup = 5;
-template <int G>
+// Body part, one wavefront per body: ru' = M (v' - vold) + sum_j T_j f';
+// partial |ru'|^2 and du_in . ru'.
 __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restrict__ du,
                                                  const double* __restrict__ dF,
                                                  const double* __restrict__ dL, int k0, int want) {
@@ -1758,68 +1763,46 @@ __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restri
 	if (a.ss[scn].ls_state != want) return;
 	const double mu = a.ctl[scn].mu;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-	double acc[2 * kGroup];
-#pragma unroll
-	for (int q = 0; q < 2 * kGroup; ++q) acc[q] = 0;
+	const int t = blockIdx.y;
+	const double step = ldexp(1.0, -(k0 + t));
+	double acc[2] = {0, 0};
 	if (wid < tile.count && !a.lstat[tile.first + wid]) {
 		const int r = tile.first + wid;
-		double add[kGroup][6];
-#pragma unroll
-		for (int t = 0; t < kGroup; ++t)
-#pragma unroll
-			for (int q = 0; q < 6; ++q) add[t][q] = 0;
+		double add[6] = {0, 0, 0, 0, 0, 0};
 		for (int k = a.inc_ptr[r] + lane; k < a.inc_ptr[r + 1]; k += 64) {
 			const int e = a.inc[k];
 			const int i = e >> 1, j = e & 1;
-			double Tj[18];
+			double fI[3], lI[2], dfi[3], dli[2];
+			trial_point(a, i, dF, dL, step, mu, fI, lI, dfi, dli);
+			// fI[1] = fI[2] = 0 for the frictionless model
 #pragma unroll
-			for (int q = 0; q < 18; ++q) Tj[q] = Tc(a, i, j, q);
-#pragma unroll
-			for (int t = 0; t < kGroup; ++t) {
-				double fI[3], lI[2], dfi[3], dli[2];
-				trial_point(a, i, dF, dL, ldexp(1.0, -(k0 + G * kGroup + t)), mu, fI, lI, dfi, dli);
-				// fI[1] = fI[2] = 0 for the frictionless model
-#pragma unroll
-				for (int row = 0; row < 6; ++row)
-					add[t][row] += Tj[row] * fI[0] + Tj[6 + row] * fI[1] + Tj[12 + row] * fI[2];
-			}
+			for (int row = 0; row < 6; ++row)
+				add[row] += Tc(a, i, j, row) * fI[0] + Tc(a, i, j, 6 + row) * fI[1] + Tc(a, i, j, 12 + row) * fI[2];
 		}
 #pragma unroll
-		for (int t = 0; t < kGroup; ++t)
-#pragma unroll
-			for (int q = 0; q < 6; ++q) add[t][q] = wave_sum(add[t][q]);
+		for (int q = 0; q < 6; ++q) add[q] = wave_sum(add[q]);
 		if (lane == 0) {
 			const double* M = &a.Ms[7 * r];
-#pragma unroll
-			for (int t = 0; t < kGroup; ++t) {
-				const double step = ldexp(1.0, -(k0 + G * kGroup + t));
-				double d[6], dd[6];
-				for (int q = 0; q < 6; ++q) {
-					dd[q] = step * du[6 * r + q];
-					d[q] = (a.v[6 * r + q] + 0.99 * dd[q]) - a.vold[6 * r + q];
-				}
-				double ru[6];
-				for (int q = 0; q < 3; ++q) ru[q] = M[0] * d[q];
-				double tt[3];
-				sym3_mv(M + 1, d + 3, tt);
-				ru[3] = tt[0]; ru[4] = tt[1]; ru[5] = tt[2];
-				for (int q = 0; q < 6; ++q) {
-					ru[q] += add[t][q];
-					acc[2 * t] += ru[q] * ru[q];
-					acc[2 * t + 1] += dd[q] * ru[q];
-				}
+			double d[6], dd[6];
+			for (int q = 0; q < 6; ++q) {
+				dd[q] = step * du[6 * r + q];
+				d[q] = (a.v[6 * r + q] + 0.99 * dd[q]) - a.vold[6 * r + q];
+			}
+			double ru[6];
+			for (int q = 0; q < 3; ++q) ru[q] = M[0] * d[q];
+			double tt[3];
+			sym3_mv(M + 1, d + 3, tt);
+			ru[3] = tt[0]; ru[4] = tt[1]; ru[5] = tt[2];
+			for (int q = 0; q < 6; ++q) {
+				ru[q] += add[q];
+				acc[0] += ru[q] * ru[q];
+				acc[1] += dd[q] * ru[q];
 			}
 		}
 	}
-	int slots[2 * kGroup];
-#pragma unroll
-	for (int t = 0; t < kGroup; ++t) {
-		slots[2 * t] = P_LS + 7 * (G * kGroup + t) + 5;
-		slots[2 * t + 1] = P_LS + 7 * (G * kGroup + t) + 6;
-	}
-	write_partials<2 * kGroup>(acc, a.part, a.pstride, slots);
+	const int slots[2] = {P_LS + 7 * t + 5, P_LS + 7 * t + 6};
+	write_partials<2>(acc, a.part, a.pstride, slots);
 }
-static_assert(kTrials == 2 * kGroup, "k_ls_body covers the trials in two groups");
 
 // Acceptance (PrimalDualForceSolver.cpp:605-631) over the nt trials of a pass
 // (trials k0 .. k0 + nt - 1 of a search of maxLS trials), in order: the first
@@ -2140,7 +2123,12 @@ static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 	hipEvent_t* ev = kt_spmv ? ctx->kt_direct_ev(2 * kPcgChunk) : kt_iter ? ctx->kt_direct_ev(2) : nullptr;
 	const int maxit = ctx->p.pcg_max_iter;
 	int launched = 0;
-	int want = std::max(1, (std::min(ctx->pcg_last_iters + 2, maxit) + kPcgChunk - 1) / kPcgChunk);
+	// The iteration count of a solve scatters widely around the previous
+	// one's: the first group covers 0.8 of it and every further group a
+	// quarter of what was launched, which balances the launches that find
+	// the solve converged (a few us each) against the host round trips.
+	auto chunks_for = [](double launches) { return std::max(1, (int)std::ceil(launches / kPcgChunk)); };
+	int want = chunks_for(0.8 * std::min(ctx->pcg_last_iters, maxit));
 	ctx->h_ps.resize(S);
 	double kt_ms = 0;
 	bool all_done = false;
@@ -2179,7 +2167,7 @@ static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 			ctx->kt.launches += active;
 		}
 		if (all_done) break;  // every running scene converged
-		want = 1;
+		want = chunks_for(0.25 * launched);
 	}
 	std::vector<int> iters(S, 0);
 	int last = 0;
@@ -2196,8 +2184,8 @@ static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 		// or of k_pcg2_a for the two-reduction form), each input read once and
 		// each output written once, FP64 vectors:
 		//   k_pcg_iter  B blocks and column indices nnzb (36 T + 4), row pointers
-		//               4 (NL + 1), P blocks 36 T NL, w, z, q, s, p, u, x, r read
-		//               8 x 8 n6, z, q, s, p, x, r, u, w' written 8 x 8 n6
+		//               4 (NL + 1), P blocks 36 T NL, w, z, s, p, x, r read
+		//               6 x 8 n6, z, s, p, x, r, w' written 6 x 8 n6
 		//   k_pcg2_a    B, column indices, row pointers as above, P 36 T NL,
 		//               r, y, q read 3 x 8 n6, y, q written 2 x 8 n6
 		const double Tb = sizeof(T);
@@ -2206,7 +2194,7 @@ static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 			if (!ctx->h_ps[sc].run) continue;
 			const double NLs = ctx->h_lbeg[sc + 1] - ctx->h_lbeg[sc], n6 = 6.0 * NLs, nnzb = ctx->h_nnzb[sc];
 			const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NLs + 1) + 36.0 * Tb * NLs;
-			ctx->kt.bytes += iters[sc] * (mat + (two ? 40.0 : 128.0) * n6);
+			ctx->kt.bytes += iters[sc] * (mat + (two ? 40.0 : 96.0) * n6);
 		}
 	}
 	if (ctx->n_tb > 0) hipLaunchKernelGGL(k_du, dim3(ctx->n_tb), dim3(kBlock), 0, s, sa, ctx->x64.p);
@@ -2285,6 +2273,10 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	auto launch = [&](auto kernel, int grid, int threads, auto... args) {
 		if (grid > 0) hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, s, args...);
 	};
+	// a line-search pass: one grid row per trial
+	auto launch_trials = [&](auto kernel, int tiles, int trials, auto... args) {
+		if (tiles > 0) hipLaunchKernelGGL(kernel, dim3(tiles, trials), dim3(kBlock), 0, s, args...);
+	};
 	const int gS = (S + kBlock - 1) / kBlock;
 
 	launch(k_scene_init, gS, kBlock, a);
@@ -2337,21 +2329,19 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		hipEvent_t e1;
 		for (int k0 = 0; k0 < 10; k0 += kGroup) {
 			kt_begin(ctx, PDC_KT_LINE_SEARCH, &e1);
-			launch(k_ls_contact<kGroup>, ntc, kBlock, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
-			       (int)LS_PENDING);
+			launch_trials(k_ls_contact, ntc, kGroup, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
+			              (int)LS_PENDING);
 			kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
-			launch(k_ls_body<0>, ntw, kBlock, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
-			       (int)LS_PENDING);
+			launch_trials(k_ls_body, ntw, kGroup, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
+			              (int)LS_PENDING);
 			launch(k_ls_final, S, 1024, a, k0, kGroup, 10, 0);
 		}
 		launch(k_neg, ntb, kBlock, a, (const double*)ctx->l_ru.p, ctx->l_gdu.p);
 		for (int k0 = 0; k0 < 20; k0 += kTrials) {
-			launch(k_ls_contact<kTrials>, ntc, kBlock, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p,
-			       k0, (int)LS_GD_PENDING);
-			launch(k_ls_body<0>, ntw, kBlock, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, k0,
-			       (int)LS_GD_PENDING);
-			launch(k_ls_body<1>, ntw, kBlock, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, k0,
-			       (int)LS_GD_PENDING);
+			launch_trials(k_ls_contact, ntc, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p,
+			              k0, (int)LS_GD_PENDING);
+			launch_trials(k_ls_body, ntw, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, k0,
+			              (int)LS_GD_PENDING);
 			launch(k_ls_final, S, 1024, a, k0, kTrials, 20, 1);
 		}
 		launch(k_ls_commit, ntc, kBlock, a, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
