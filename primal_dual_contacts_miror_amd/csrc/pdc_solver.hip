@@ -807,20 +807,22 @@ __global__ __launch_bounds__(256) void k_body_res(SArgs a) {
 		const int r = tile.first + wid;
 		double ru[6] = {0, 0, 0, 0, 0, 0};
 		if (!a.lstat[r]) {
+			// the body's rows first (every lane, same address), then the gathers
+			double d[6], M[7];
+			for (int q = 0; q < 6; ++q) d[q] = a.v[6 * r + q] - a.vold[6 * r + q];
+			for (int q = 0; q < 7; ++q) M[q] = a.Ms[7 * r + q];
 			double add[6] = {0, 0, 0, 0, 0, 0};
 			for (int k = a.inc_ptr[r] + lane; k < a.inc_ptr[r + 1]; k += 64) {
 				const int e = a.inc[k];
 				const int i = e >> 1, j = e & 1;
-				for (int row = 0; row < 6; ++row) {
-					double t = 0;
-					for (int col = 0; col < fd; ++col) t += Tc(a, i, j, col * 6 + row) * F[col * a.cs + i];
-					add[row] += t;
-				}
+				// every load of the contact issued before the first use
+				const double f0 = F[i];
+				const double f1 = fd == 3 ? F[a.cs + i] : 0.0, f2 = fd == 3 ? F[2 * a.cs + i] : 0.0;
+#pragma unroll
+				for (int row = 0; row < 6; ++row)
+					add[row] += Tc(a, i, j, row) * f0 + Tc(a, i, j, 6 + row) * f1 + Tc(a, i, j, 12 + row) * f2;
 			}
 			for (int q = 0; q < 6; ++q) add[q] = wave_sum(add[q]);
-			double d[6];
-			for (int q = 0; q < 6; ++q) d[q] = a.v[6 * r + q] - a.vold[6 * r + q];
-			const double* M = &a.Ms[7 * r];
 			for (int q = 0; q < 3; ++q) ru[q] = M[0] * d[q];
 			double t[3];
 			sym3_mv(M + 1, d + 3, t);
@@ -923,160 +925,177 @@ __global__ __launch_bounds__(256) void k_asm_prep(SArgs a) {
 	}
 }
 
-// acc(6x6) += T_j W T_k^T for contact i (upper triangle only when SYM).
-__device__ __forceinline__ void add_tWt(const SArgs& a, int i, int j, int k, double* B) {
-	const double kn = a.W[i];
-	const double w00 = a.W[a.cs + i], w01 = a.W[2 * a.cs + i], w11 = a.W[3 * a.cs + i];
-	double tk0[6], tk1[6], tk2[6];
-	for (int c = 0; c < 6; ++c) {
-		tk0[c] = Tc(a, i, k, c);
-		tk1[c] = Tc(a, i, k, 6 + c);
-		tk2[c] = Tc(a, i, k, 12 + c);
-	}
-	for (int r = 0; r < 6; ++r) {
-		const double a0 = Tc(a, i, j, r) * kn;
-		const double t1 = Tc(a, i, j, 6 + r), t2 = Tc(a, i, j, 12 + r);
-		const double at0 = t1 * w00 + t2 * w01;
-		const double at1 = t1 * w01 + t2 * w11;
-		for (int c = 0; c < 6; ++c) B[r * 6 + c] += a0 * tk0[c] + at0 * tk1[c] + at1 * tk2[c];
-	}
+// Value q (lane dependent) of six, as a select chain: a runtime index into a
+// private array would place the array in scratch or LDS.
+__device__ __forceinline__ double sel6(int q, double v0, double v1, double v2, double v3, double v4, double v5) {
+	return q == 0 ? v0 : q == 1 ? v1 : q == 2 ? v2 : q == 3 ? v3 : q == 4 ? v4 : v5;
 }
-
-// In-place inverse of an SPD 6x6 through its Cholesky factor; false if not SPD.
-__device__ __forceinline__ bool spd_inverse6(const double* A, double* Ai) {
-	double L[36];
-	for (int k = 0; k < 36; ++k) L[k] = 0;
-	for (int j = 0; j < 6; ++j) {
-		double d = A[j * 6 + j];
-		for (int k = 0; k < j; ++k) d -= L[j * 6 + k] * L[j * 6 + k];
-		if (!(d > 0)) return false;
-		const double ljj = sqrt(d);
-		L[j * 6 + j] = ljj;
-		for (int i = j + 1; i < 6; ++i) {
-			double s = A[i * 6 + j];
-			for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[j * 6 + k];
-			L[i * 6 + j] = s / ljj;
-		}
-	}
-	for (int c = 0; c < 6; ++c) {
-		double e[6] = {0, 0, 0, 0, 0, 0};
-		e[c] = 1;
-		for (int i = 0; i < 6; ++i) {
-			double s = e[i];
-			for (int k = 0; k < i; ++k) s -= L[i * 6 + k] * e[k];
-			e[i] = s / L[i * 6 + i];
-		}
-		for (int i = 5; i >= 0; --i) {
-			double s = e[i];
-			for (int k = i + 1; k < 6; ++k) s -= L[k * 6 + i] * e[k];
-			e[i] = s / L[i * 6 + i];
-		}
-		for (int i = 0; i < 6; ++i) Ai[i * 6 + c] = e[i];
-	}
-	return true;
-}
+#define PDC_PICK6(v, q) sel6((q), (v)[0], (v)[1], (v)[2], (v)[3], (v)[4], (v)[5])
+// element (row, k) of a row-major 6x6 array, row lane dependent, k constant
+#define PDC_ROW6(m, row, k) sel6((row), (m)[k], (m)[6 + (k)], (m)[12 + (k)], (m)[18 + (k)], (m)[24 + (k)], (m)[30 + (k)])
 
 // Diagonal blocks, rhs, Jacobi scaling D, block-Jacobi preconditioner P, scaled
 // b; the stored matrix is the preconditioned operator A P (block (a, a) =
-// A_aa P_a, the identity up to rounding).  One wavefront per body: lanes accumulate T_j W T_j^T (upper triangle) and
-// y = -sum T_j Kinv sf over the body's incident contacts; lane 0 finishes.
+// A_aa P_a, the identity up to rounding).  One wavefront per body: lanes
+// accumulate T_j W T_j^T (upper triangle) and y = -sum T_j Kinv sf over the
+// body's incident contacts; after the wavefront sums every lane holds the
+// block.  Every lane factors it (Cholesky), lane c < 6 solves for column c of
+// the inverse, and lane 6 i + j < 36 forms entry (i, j) of the stored blocks.
 template <class T>
 __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict__ adiag, T* __restrict__ Aval,
                            T* __restrict__ Pval, T* __restrict__ bvec) {
 	PDC_TILE_RUN(a.tw);
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const double eps = a.ss[scn].eps;
+	const bool warm = a.ss[scn].it > 0;
 	const int fd = a.nsc ? 3 : 1;
-	if (wid < tile.count) {
-		const int r = tile.first + wid;
-		const bool st = a.lstat[r] != 0;
-		double U[21], y[6];
-		for (int q = 0; q < 21; ++q) U[q] = 0;
-		for (int q = 0; q < 6; ++q) y[q] = 0;
-		if (!st) {
-			for (int k = a.inc_ptr[r] + lane; k < a.inc_ptr[r + 1]; k += 64) {
-				const int e = a.inc[k];
-				const int i = e >> 1, j = e & 1;
-				double tj0[6], tj1[6], tj2[6];
-				for (int q = 0; q < 6; ++q) {
-					tj0[q] = Tc(a, i, j, q);
-					tj1[q] = Tc(a, i, j, 6 + q);
-					tj2[q] = Tc(a, i, j, 12 + q);
-				}
-				const double kf0 = a.kf[i];
-				const double kf1 = fd == 3 ? a.kf[a.cs + i] : 0.0, kf2 = fd == 3 ? a.kf[2 * a.cs + i] : 0.0;
-				for (int q = 0; q < 6; ++q) y[q] -= tj0[q] * kf0 + tj1[q] * kf1 + tj2[q] * kf2;
-				const double kn = a.W[i];
-				const double w00 = a.W[a.cs + i], w01 = a.W[2 * a.cs + i], w11 = a.W[3 * a.cs + i];
-				int u = 0;
-				for (int row = 0; row < 6; ++row) {
-					const double a0 = tj0[row] * kn;
-					const double at0 = tj1[row] * w00 + tj2[row] * w01;
-					const double at1 = tj1[row] * w01 + tj2[row] * w11;
-					for (int col = row; col < 6; ++col) U[u++] += a0 * tj0[col] + at0 * tj1[col] + at1 * tj2[col];
-				}
-			}
-			for (int q = 0; q < 21; ++q) U[q] = wave_sum(U[q]);
-			for (int q = 0; q < 6; ++q) y[q] = wave_sum(y[q]);
-		}
-		if (lane == 0) {
-			double B[36];
-			for (int q = 0; q < 36; ++q) B[q] = 0;
-			if (st) {
-				for (int q = 0; q < 6; ++q) B[q * 6 + q] = 1.0 + eps;
-			} else {
-				const double* M = &a.Ms[7 * r];
-				int u = 0;
-				for (int row = 0; row < 6; ++row)
-					for (int col = row; col < 6; ++col) {
-						B[row * 6 + col] = U[u];
-						B[col * 6 + row] = U[u];
-						++u;
-					}
-				for (int q = 0; q < 3; ++q) B[q * 6 + q] += M[0];
-				B[3 * 6 + 3] += M[1]; B[3 * 6 + 4] += M[2]; B[3 * 6 + 5] += M[3];
-				B[4 * 6 + 3] += M[2]; B[4 * 6 + 4] += M[4]; B[4 * 6 + 5] += M[5];
-				B[5 * 6 + 3] += M[3]; B[5 * 6 + 4] += M[5]; B[5 * 6 + 5] += M[6];
-				for (int q = 0; q < 6; ++q) B[q * 6 + q] += eps;
-			}
-			double Dr[6];
+	if (wid >= tile.count) return;
+	const int r = tile.first + wid;
+	const bool st = a.lstat[r] != 0;
+	// body rows, requested before the contact gathers (every lane, same address)
+	double M[7], ru[6], x0[6];
+	for (int q = 0; q < 7; ++q) M[q] = a.Ms[7 * r + q];
+	for (int q = 0; q < 6; ++q) {
+		ru[q] = a.ru[6 * r + q];
+		// PCG start: the previous iteration's increment, zero in the first
+		// (PrimalDualForceSolver.cpp:407-414 hands AMGCL X = du)
+		x0[q] = warm ? a.du[6 * r + q] : 0.0;
+	}
+	double U[21], y[6];
+	for (int q = 0; q < 21; ++q) U[q] = 0;
+	for (int q = 0; q < 6; ++q) y[q] = 0;
+	if (!st) {
+		for (int k = a.inc_ptr[r] + lane; k < a.inc_ptr[r + 1]; k += 64) {
+			const int e = a.inc[k];
+			const int i = e >> 1, j = e & 1;
+			double tj0[6], tj1[6], tj2[6];
 			for (int q = 0; q < 6; ++q) {
-				Dr[q] = 1 / sqrt(B[q * 6 + q]);
-				const double rhs = st ? 0.0 : y[q] - a.ru[6 * r + q];
-				a.D[6 * r + q] = Dr[q];
-				a.rhs[6 * r + q] = rhs;
-				bvec[6 * r + q] = (T)(rhs * Dr[q]);
+				tj0[q] = Tc(a, i, j, q);
+				tj1[q] = Tc(a, i, j, 6 + q);
+				tj2[q] = Tc(a, i, j, 12 + q);
 			}
-			double S[36], Si[36];
-			for (int ii = 0; ii < 6; ++ii)
-				for (int jj = 0; jj < 6; ++jj) S[ii * 6 + jj] = B[ii * 6 + jj] * (Dr[ii] * Dr[jj]);
-			if (!spd_inverse6(S, Si)) {
-				for (int q = 0; q < 36; ++q) Si[q] = 0;
-				for (int q = 0; q < 6; ++q) Si[q * 6 + q] = 1.0 / S[q * 6 + q];
+			const double kf0 = a.kf[i];
+			const double kf1 = fd == 3 ? a.kf[a.cs + i] : 0.0, kf2 = fd == 3 ? a.kf[2 * a.cs + i] : 0.0;
+			for (int q = 0; q < 6; ++q) y[q] -= tj0[q] * kf0 + tj1[q] * kf1 + tj2[q] * kf2;
+			const double kn = a.W[i];
+			const double w00 = a.W[a.cs + i], w01 = a.W[2 * a.cs + i], w11 = a.W[3 * a.cs + i];
+			int u = 0;
+			for (int row = 0; row < 6; ++row) {
+				const double a0 = tj0[row] * kn;
+				const double at0 = tj1[row] * w00 + tj2[row] * w01;
+				const double at1 = tj1[row] * w01 + tj2[row] * w11;
+				for (int col = row; col < 6; ++col) U[u++] += a0 * tj0[col] + at0 * tj1[col] + at1 * tj2[col];
 			}
-			// PCG start: the previous iteration's increment, zero in the first
-			// (PrimalDualForceSolver.cpp:407-414 hands AMGCL X = du), and S x0,
-			// which the stored operator A P maps to A x0
-			const bool warm = a.ss[scn].it > 0;
-			double x0[6];
-			for (int q = 0; q < 6; ++q) x0[q] = warm ? a.du[6 * r + q] : 0.0;
-			for (int ii = 0; ii < 6; ++ii) {
-				double acc = 0;
-				for (int k = 0; k < 6; ++k) acc += S[ii * 6 + k] * x0[k];
-				a.xs[6 * r + ii] = x0[ii];
-				a.ys[6 * r + ii] = acc;
-			}
-			// the preconditioner as stored, and the diagonal block of A P with it
-			for (int q = 0; q < 36; ++q) Si[q] = (double)(T)Si[q];
-			for (int q = 0; q < 36; ++q) Pval[36 * (size_t)r + q] = (T)Si[q];
-			const int d = adiag[r];
-			for (int ii = 0; ii < 6; ++ii)
-				for (int jj = 0; jj < 6; ++jj) {
-					double acc = 0;
-					for (int k = 0; k < 6; ++k) acc += S[ii * 6 + k] * Si[k * 6 + jj];
-					Aval[36 * (size_t)d + ii * 6 + jj] = (T)acc;
-				}
 		}
+		for (int q = 0; q < 21; ++q) U[q] = wave_sum(U[q]);
+		for (int q = 0; q < 6; ++q) y[q] = wave_sum(y[q]);
+	}
+	// the block B (every lane)
+	double S[36];
+#pragma unroll
+	for (int q = 0; q < 36; ++q) S[q] = 0;
+	if (st) {
+#pragma unroll
+		for (int q = 0; q < 6; ++q) S[q * 6 + q] = 1.0 + eps;
+	} else {
+		int u = 0;
+#pragma unroll
+		for (int row = 0; row < 6; ++row)
+#pragma unroll
+			for (int col = row; col < 6; ++col) {
+				S[row * 6 + col] = U[u];
+				S[col * 6 + row] = U[u];
+				++u;
+			}
+#pragma unroll
+		for (int q = 0; q < 3; ++q) S[q * 6 + q] += M[0];
+		S[3 * 6 + 3] += M[1]; S[3 * 6 + 4] += M[2]; S[3 * 6 + 5] += M[3];
+		S[4 * 6 + 3] += M[2]; S[4 * 6 + 4] += M[4]; S[4 * 6 + 5] += M[5];
+		S[5 * 6 + 3] += M[3]; S[5 * 6 + 4] += M[5]; S[5 * 6 + 5] += M[6];
+#pragma unroll
+		for (int q = 0; q < 6; ++q) S[q * 6 + q] += eps;
+	}
+	double Dr[6];
+#pragma unroll
+	for (int q = 0; q < 6; ++q) Dr[q] = 1 / sqrt(S[q * 6 + q]);
+	if (lane < 6) {
+		const double rhs = st ? 0.0 : PDC_PICK6(y, lane) - PDC_PICK6(ru, lane);
+		const double dq = PDC_PICK6(Dr, lane);
+		a.D[6 * r + lane] = dq;
+		a.rhs[6 * r + lane] = rhs;
+		bvec[6 * r + lane] = (T)(rhs * dq);
+	}
+	// scaled block S = D B D
+#pragma unroll
+	for (int ii = 0; ii < 6; ++ii)
+#pragma unroll
+		for (int jj = 0; jj < 6; ++jj) S[ii * 6 + jj] = S[ii * 6 + jj] * (Dr[ii] * Dr[jj]);
+	// Cholesky S = L L^T with reciprocal pivots (every lane)
+	double L[36], Ld[6];
+	bool spd = true;
+#pragma unroll
+	for (int q = 0; q < 36; ++q) L[q] = 0;
+#pragma unroll
+	for (int j = 0; j < 6; ++j) {
+		double d = S[j * 6 + j];
+#pragma unroll
+		for (int k = 0; k < j; ++k) d -= L[j * 6 + k] * L[j * 6 + k];
+		spd = spd && d > 0;
+		Ld[j] = 1 / sqrt(d > 0 ? d : 1.0);
+		L[j * 6 + j] = 1 / Ld[j];
+#pragma unroll
+		for (int i = j + 1; i < 6; ++i) {
+			double t = S[i * 6 + j];
+#pragma unroll
+			for (int k = 0; k < j; ++k) t -= L[i * 6 + k] * L[j * 6 + k];
+			L[i * 6 + j] = t * Ld[j];
+		}
+	}
+	// column c = lane of S^-1 (lanes 0..5): L z = e_c, L^T x = z; not SPD:
+	// the inverse of the diagonal
+	const int c = lane < 6 ? lane : 0;
+	double e[6];
+	if (spd) {
+#pragma unroll
+		for (int i = 0; i < 6; ++i) {
+			double t = i == c ? 1.0 : 0.0;
+#pragma unroll
+			for (int k = 0; k < i; ++k) t -= L[i * 6 + k] * e[k];
+			e[i] = t * Ld[i];
+		}
+#pragma unroll
+		for (int i = 5; i >= 0; --i) {
+			double t = e[i];
+#pragma unroll
+			for (int k = i + 1; k < 6; ++k) t -= L[k * 6 + i] * e[k];
+			e[i] = t * Ld[i];
+		}
+	} else {
+#pragma unroll
+		for (int i = 0; i < 6; ++i) e[i] = i == c ? 1.0 / S[i * 6 + i] : 0.0;
+	}
+	// the preconditioner as stored
+#pragma unroll
+	for (int i = 0; i < 6; ++i) e[i] = (double)(T)e[i];
+	// lane 6 ii + jj: column jj of the inverse from lane jj
+	const int ii = lane < 36 ? lane / 6 : 0, jj = lane < 36 ? lane - 6 * (lane / 6) : 0;
+	double col[6];
+#pragma unroll
+	for (int k = 0; k < 6; ++k) col[k] = __shfl(e[k], jj, 64);
+	if (lane < 36) {
+		double acc = 0;
+#pragma unroll
+		for (int k = 0; k < 6; ++k) acc += PDC_ROW6(S, ii, k) * col[k];
+		Pval[36 * (size_t)r + lane] = (T)PDC_PICK6(col, ii);
+		Aval[36 * (size_t)adiag[r] + lane] = (T)acc;
+	}
+	// PCG start x0 and S x0, which the stored operator A P maps to A x0
+	if (lane < 6) {
+		double acc = 0;
+#pragma unroll
+		for (int k = 0; k < 6; ++k) acc += PDC_ROW6(S, lane, k) * x0[k];
+		a.xs[6 * r + lane] = PDC_PICK6(x0, lane);
+		a.ys[6 * r + lane] = acc;
 	}
 }
 
@@ -1084,43 +1103,90 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 // contacts of T_side(a) W T_side(b)^T; A_ab = D_a S D_b and A_ba = A_ab^T (the
 // Newton matrix is symmetric).  Stored are the blocks of the preconditioned
 // operator, A_ab P_b and A_ba P_a; P comes from k_asm_diag, which runs first.
-template <class T>
+// Pairs of at most kOffDirect contacts (the common case, LARGE = false): lane
+// 6 ii + jj < 36 sums entry (ii, jj) over the contacts itself, with no
+// cross-lane reduction.  Larger pairs (a launch of their own, LARGE = true, so
+// that the small pairs' kernel keeps few registers): the lanes take the
+// contacts with stride 64 and the 36 entries are reduced across the
+// wavefront.  The rows and columns for the products with P come by shuffles.
+constexpr int kOffDirect = 16;
+template <class T, bool LARGE>
 __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ Pval, T* __restrict__ Aval) {
 	PDC_TILE_RUN(a.tp);
 	(void)scn;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int ii = lane < 36 ? lane / 6 : 0, jj = lane < 36 ? lane - 6 * (lane / 6) : 0;
-	if (wid < tile.count) {
-		const int p = tile.first + wid;
-		const int ka = a.pair_adj[2 * p], kb = a.pair_adj[2 * p + 1];
-		const int blk_ab = a.adj_blk[ka], blk_ba = a.adj_blk[kb];
-		if (blk_ab < 0) return;  // a static end: no coupling block
-		const int ra = a.g2l[a.pa[p]];
-		double B[36];
-		for (int q = 0; q < 36; ++q) B[q] = 0;
-		for (int i = a.pstart[p] + lane; i < a.pstart[p + 1]; i += 64) {
+	if (wid >= tile.count) return;
+	const int p = tile.first + wid;
+	const int ka = a.pair_adj[2 * p], kb = a.pair_adj[2 * p + 1];
+	const int blk_ab = a.adj_blk[ka], blk_ba = a.adj_blk[kb];
+	if (blk_ab < 0) return;  // a static end: no coupling block
+	const int c0 = a.pstart[p], c1 = a.pstart[p + 1];
+	if ((c1 - c0 > kOffDirect) != LARGE) return;  // the other launch's pair
+	const int ra = a.g2l[a.pa[p]];
+	const int rb = a.li0[c0] == ra ? a.li1[c0] : a.li0[c0];
+	// scaling and preconditioner entries, requested before the contact loop
+	const double da = a.D[6 * ra + ii], db = a.D[6 * rb + jj];
+	double pb[6], pa[6];
+#pragma unroll
+	for (int k = 0; k < 6; ++k) {
+		pb[k] = (double)Pval[36 * (size_t)rb + 6 * k + jj];
+		pa[k] = (double)Pval[36 * (size_t)ra + 6 * k + jj];
+	}
+	double sum = 0;
+	if (!LARGE) {
+#pragma unroll 4
+		for (int i = c0; i < c1; ++i) {
 			const int j = a.li0[i] == ra ? 0 : 1;
-			add_tWt(a, i, j, 1 - j, B);
+			const double kn = a.W[i];
+			const double w00 = a.W[a.cs + i], w01 = a.W[2 * a.cs + i], w11 = a.W[3 * a.cs + i];
+			const double t0 = Tc(a, i, j, ii), t1 = Tc(a, i, j, 6 + ii), t2 = Tc(a, i, j, 12 + ii);
+			const double u0 = Tc(a, i, 1 - j, jj), u1 = Tc(a, i, 1 - j, 6 + jj), u2 = Tc(a, i, 1 - j, 12 + jj);
+			sum += (t0 * kn) * u0 + (t1 * w00 + t2 * w01) * u1 + (t1 * w01 + t2 * w11) * u2;
 		}
-		for (int q = 0; q < 36; ++q) B[q] = wave_sum(B[q]);
-		const int rb = a.li0[a.pstart[p]] == ra ? a.li1[a.pstart[p]] : a.li0[a.pstart[p]];
-		double bl = 0;  // B[lane] without a dynamically indexed array
+	} else {
+		double B[36];
 #pragma unroll
-		for (int q = 0; q < 36; ++q) bl = q == lane ? B[q] : bl;
-		// lane (ii, jj) < 36 holds A_ab[ii][jj]; rows and columns by shuffles
-		const double v = bl * (a.D[6 * ra + ii] * a.D[6 * rb + jj]);
-		double bab = 0, bba = 0;
+		for (int q = 0; q < 36; ++q) B[q] = 0;
+		for (int i = c0 + lane; i < c1; i += 64) {
+			const int j = a.li0[i] == ra ? 0 : 1, k = 1 - j;
+			const double kn = a.W[i];
+			const double w00 = a.W[a.cs + i], w01 = a.W[2 * a.cs + i], w11 = a.W[3 * a.cs + i];
+			double u0[6], u1[6], u2[6];
 #pragma unroll
-		for (int k = 0; k < 6; ++k) {
-			const double a_ik = __shfl(v, 6 * ii + k, 64);  // A_ab[ii][k]
-			const double a_ki = __shfl(v, 6 * k + ii, 64);  // A_ab[k][ii] = A_ba[ii][k]
-			bab += a_ik * (double)Pval[36 * (size_t)rb + 6 * k + jj];
-			bba += a_ki * (double)Pval[36 * (size_t)ra + 6 * k + jj];
+			for (int c = 0; c < 6; ++c) {
+				u0[c] = Tc(a, i, k, c);
+				u1[c] = Tc(a, i, k, 6 + c);
+				u2[c] = Tc(a, i, k, 12 + c);
+			}
+#pragma unroll
+			for (int row = 0; row < 6; ++row) {
+				const double t0 = Tc(a, i, j, row) * kn;
+				const double t1 = Tc(a, i, j, 6 + row), t2 = Tc(a, i, j, 12 + row);
+				const double s1 = t1 * w00 + t2 * w01, s2 = t1 * w01 + t2 * w11;
+#pragma unroll
+				for (int c = 0; c < 6; ++c) B[row * 6 + c] += t0 * u0[c] + s1 * u1[c] + s2 * u2[c];
+			}
 		}
-		if (lane < 36) {
-			Aval[36 * (size_t)blk_ab + ii * 6 + jj] = (T)bab;
-			Aval[36 * (size_t)blk_ba + ii * 6 + jj] = (T)bba;
+#pragma unroll
+		for (int q = 0; q < 36; ++q) {
+			const double t = wave_sum(B[q]);
+			sum = q == lane ? t : sum;
 		}
+	}
+	// lane (ii, jj) < 36 holds A_ab[ii][jj]
+	const double v = sum * (da * db);
+	double bab = 0, bba = 0;
+#pragma unroll
+	for (int k = 0; k < 6; ++k) {
+		const double a_ik = __shfl(v, 6 * ii + k, 64);  // A_ab[ii][k]
+		const double a_ki = __shfl(v, 6 * k + ii, 64);  // A_ab[k][ii] = A_ba[ii][k]
+		bab += a_ik * pb[k];
+		bba += a_ki * pa[k];
+	}
+	if (lane < 36) {
+		Aval[36 * (size_t)blk_ab + ii * 6 + jj] = (T)bab;
+		Aval[36 * (size_t)blk_ba + ii * 6 + jj] = (T)bba;
 	}
 }
 
@@ -1768,6 +1834,13 @@ __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restri
 	double acc[2] = {0, 0};
 	if (wid < tile.count && !a.lstat[tile.first + wid]) {
 		const int r = tile.first + wid;
+		// the body's rows first (every lane, same address), then the gathers
+		double d[6], dd[6], M[7];
+		for (int q = 0; q < 6; ++q) {
+			dd[q] = step * du[6 * r + q];
+			d[q] = (a.v[6 * r + q] + 0.99 * dd[q]) - a.vold[6 * r + q];
+		}
+		for (int q = 0; q < 7; ++q) M[q] = a.Ms[7 * r + q];
 		double add[6] = {0, 0, 0, 0, 0, 0};
 		for (int k = a.inc_ptr[r] + lane; k < a.inc_ptr[r + 1]; k += 64) {
 			const int e = a.inc[k];
@@ -1782,12 +1855,6 @@ __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restri
 #pragma unroll
 		for (int q = 0; q < 6; ++q) add[q] = wave_sum(add[q]);
 		if (lane == 0) {
-			const double* M = &a.Ms[7 * r];
-			double d[6], dd[6];
-			for (int q = 0; q < 6; ++q) {
-				dd[q] = step * du[6 * r + q];
-				d[q] = (a.v[6 * r + q] + 0.99 * dd[q]) - a.vold[6 * r + q];
-			}
 			double ru[6];
 			for (int q = 0; q < 3; ++q) ru[q] = M[0] * d[q];
 			double tt[3];
@@ -2311,10 +2378,12 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		launch(k_asm_prep, ntc, kBlock, a);
 		if (fp32) {
 			launch(k_asm_diag<float>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A32.p, ctx->P32.p, ctx->b32.p);
-			launch(k_asm_off<float>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
+			launch(k_asm_off<float, false>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
+			launch(k_asm_off<float, true>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
 		} else {
 			launch(k_asm_diag<double>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A64.p, ctx->P64.p, ctx->b64.p);
-			launch(k_asm_off<double>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
+			launch(k_asm_off<double, false>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
+			launch(k_asm_off<double, true>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
 		}
 		PDC_CHECK_LAUNCH();
 		kt_end(ctx, PDC_KT_ASSEMBLY, e0);
