@@ -92,7 +92,7 @@ void free_all(pdc_context* c) {
 	c->ap_start.release(); c->g_flag.release(); c->g2l.release(); c->l2g.release();
 	c->li0.release(); c->li1.release(); c->adj_cnt.release(); c->adj_ptr.release();
 	c->adj_cur.release(); c->adj_nb.release(); c->adj_pair.release(); c->a_cnt.release();
-	c->a_ptr.release(); c->a_col.release(); c->a_diag.release();
+	c->a_ptr.release(); c->a_diag.release();
 	c->adj_pos.release(); c->adj_blk.release(); c->adj_nb_u.release(); c->adj_pair_u.release();
 	c->adj_row_u.release(); c->pair_adj.release(); c->inc_cnt.release(); c->inc_ptr.release();
 	c->inc.release(); c->c_pid.release();
@@ -109,6 +109,7 @@ void free_all(pdc_context* c) {
 	c->r_cbeg.release(); c->r_lbeg.release(); c->r_pbeg.release(); c->r_nnzb.release();
 	c->t_c.release(); c->t_b.release(); c->t_w.release(); c->t_p.release(); c->t_pcg.release();
 	c->tb_c.release(); c->tb_b.release(); c->tb_w.release(); c->tb_p.release();
+	c->tb_r.release(); c->l_slot.release(); c->e_col.release(); c->r_amax.release();
 	c->d_ss.release(); c->d_sctl.release(); c->d_ps.release();
 	c->c_stride = 0;
 }

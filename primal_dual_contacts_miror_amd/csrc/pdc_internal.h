@@ -78,12 +78,12 @@ struct Tile {
 // workgroup reads another's record.
 struct PcgTile {
 	int scene;    // -1 for padding tiles
-	int first;    // first block row
+	int first;    // first block row (local body numbering)
 	int count;    // block rows (<= kPcgRowsPerWG)
 	int t0, W;    // the scene's tiles [t0, t0 + W)
 	int done;     // the scene's PCG has converged, or the scene is not solving
 	int it;       // iterations completed
-	int ptr[kPcgRowsPerWG + 1];  // row pointers of block rows first .. first + count
+	int pad[kPcgRowsPerWG + 1];
 };
 static_assert(sizeof(PcgTile) == 64, "one 64-byte record per PCG workgroup");
 
@@ -207,7 +207,7 @@ struct pdc_context {
 	pdc::DBuf<int> adj_cnt, adj_ptr, adj_cur, adj_nb, adj_pair, adj_pos, adj_blk;
 	pdc::DBuf<int> adj_nb_u, adj_pair_u, adj_row_u, pair_adj;
 	pdc::DBuf<int> inc_cnt, inc_ptr, inc, c_pid;
-	pdc::DBuf<int> a_cnt, a_ptr, a_col, a_diag;
+	pdc::DBuf<int> a_cnt, a_ptr, a_diag;
 	int pcg_last_iters = 0;
 	pdc::PcgGraph pcg_graph[4];                               // recorded chunks (FP32, FP64; x2 forms)
 	bool use_graphs = true;
@@ -251,6 +251,15 @@ struct pdc_context {
 	int n_tc = 0, n_tb = 0, n_tw = 0, n_tp = 0, n_tr = 0;
 	int cap_tr = 0;                                           // PCG tiles launched (fixed per scene set)
 	pdc::DBuf<pdc::PcgTile> t_pcg;                            // PCG workgroup records (cap_tr)
+	// The PCG works in slot space: slot kPcgRowsPerWG * tile + k is block row
+	// k of PCG tile `tile` (slots past a tile's count are padding).  The
+	// operator is stored ELL by slot: ell_R blocks per slot, column slots in
+	// e_col (-1: empty).
+	pdc::DBuf<int> tb_r;                                      // per scene: first PCG tile (S + 1)
+	pdc::DBuf<int> l_slot;                                    // per local body: its slot
+	pdc::DBuf<int> e_col;                                     // [slots][ell_R]
+	pdc::DBuf<int> r_amax;                                    // most blocks in a block row
+	int ell_R = 8;
 	pdc::DBuf<pdc::SceneSc> d_ss;                             // per-scene scalars
 	pdc::SceneSc* h_ss = nullptr;                             // pinned mirror
 	size_t h_ss_cap = 0;

@@ -188,15 +188,19 @@ __global__ __launch_bounds__(256) void k_inc_fill(int C, const int* __restrict__
 	}
 }
 
-// Block-CSR pattern of the Newton matrix (linsysReserve, NonSmoothForceBase.h:151-187):
+// Block pattern of the Newton matrix (linsysReserve, NonSmoothForceBase.h:151-187):
 // the diagonal block of every body plus one block per active-active pair,
-// columns ascending; adj_blk maps adjacency entries to their block (-1: none).
+// columns ascending, stored ELL by PCG slot: block q of the row of local body
+// r is entry slot(r) * R + q, whose column is the slot of the partner
+// (e_col).  adiag and adj_blk give the entries of the diagonal block and of
+// each adjacency entry's block (-1: none).
 __global__ __launch_bounds__(256) void k_a_fill(int NL, const int* __restrict__ lstat, const int* __restrict__ ptr,
-                         const int* __restrict__ nb, const int* __restrict__ aptr,
-                         int* __restrict__ acol, int* __restrict__ adiag, int* __restrict__ adj_blk) {
+                         const int* __restrict__ nb, const int* __restrict__ slot, int R,
+                         int* __restrict__ ecol, int* __restrict__ adiag, int* __restrict__ adj_blk) {
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= NL) return;
-	int o = aptr[r];
+	const int base = slot[r] * R;
+	int o = base;
 	bool diag_done = false;
 	for (int k = ptr[r]; k < ptr[r + 1]; ++k) {
 		const int c = nb[k];
@@ -206,16 +210,36 @@ __global__ __launch_bounds__(256) void k_a_fill(int NL, const int* __restrict__ 
 		}
 		if (!diag_done && c > r) {
 			adiag[r] = o;
-			acol[o++] = r;
+			ecol[o++] = slot[r];
 			diag_done = true;
 		}
 		adj_blk[k] = o;
-		acol[o++] = c;
+		ecol[o++] = slot[c];
 	}
 	if (!diag_done) {
 		adiag[r] = o;
-		acol[o] = r;
+		ecol[o] = slot[r];
 	}
+}
+
+// Most blocks in one block row (the ELL width before rounding).
+__global__ __launch_bounds__(256) void k_amax(int NL, const int* __restrict__ cnt, int* __restrict__ out) {
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	int m = r < NL ? cnt[r] : 0;
+	m = wave_max(m);
+	if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
+}
+
+// PCG slot of every local body: row k of the scene's tile t is slot
+// kPcgRowsPerWG * (first PCG tile of the scene + t) + k.
+__global__ __launch_bounds__(256) void k_row_slot(int NL, const int* __restrict__ l2g, const int* __restrict__ bscene,
+                                                const int* __restrict__ lbeg, const int* __restrict__ tbr,
+                                                int* __restrict__ slot) {
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= NL) return;
+	const int sc = bscene[l2g[r]];
+	const int off = r - lbeg[sc];
+	slot[r] = kPcgRowsPerWG * (tbr[sc] + off / kPcgRowsPerWG) + off % kPcgRowsPerWG;
 }
 
 void ensure_contact_capacity(pdc_context* ctx, size_t C) {
@@ -293,7 +317,12 @@ static void scene_ranges(pdc_context* ctx) {
 	                   ctx->b_scene.p, ctx->r_pbeg.p);
 	hipLaunchKernelGGL(k_scene_nnzb, dim3(grid_for(S + 1)), dim3(kBlock), 0, s, S, ctx->r_lbeg.p, ctx->a_ptr.p,
 	                   ctx->r_nnzb.p);
+	ctx->r_amax.ensure(1);
+	PDC_HIP(hipMemsetAsync(ctx->r_amax.p, 0, sizeof(int), s));
+	if (NL > 0) hipLaunchKernelGGL(k_amax, dim3(grid_for(NL)), dim3(kBlock), 0, s, NL, ctx->a_cnt.p, ctx->r_amax.p);
 	PDC_CHECK_LAUNCH();
+	int amax = 0;
+	PDC_HIP(hipMemcpyAsync(&amax, ctx->r_amax.p, sizeof(int), hipMemcpyDeviceToHost, s));
 	ctx->h_cbeg.resize(S + 1); ctx->h_lbeg.resize(S + 1); ctx->h_pbeg.resize(S + 1); ctx->h_nnzb.resize(S + 1);
 	PDC_HIP(hipMemcpyAsync(ctx->h_cbeg.data(), ctx->r_cbeg.p, (S + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
 	PDC_HIP(hipMemcpyAsync(ctx->h_lbeg.data(), ctx->r_lbeg.p, (S + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -301,6 +330,7 @@ static void scene_ranges(pdc_context* ctx) {
 	PDC_HIP(hipMemcpyAsync(ctx->h_nnzb.data(), ctx->r_nnzb.p, (S + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
 	PDC_HIP(hipStreamSynchronize(s));
 	ctx->NNZB = ctx->h_nnzb[S];
+	ctx->ell_R = std::max(8, (amax + 7) / 8 * 8);
 	std::vector<Tile> t;
 	std::vector<int> tb;
 	make_tiles(ctx->h_cbeg, S, kBlock, t, tb);
@@ -313,6 +343,13 @@ static void scene_ranges(pdc_context* ctx) {
 	ctx->n_tp = upload_tiles(ctx, t, tb, ctx->t_p, ctx->tb_p);
 	make_tiles(ctx->h_lbeg, S, kPcgRowsPerWG, t, tb, ctx->cap_tr);
 	if ((int)t.size() > ctx->cap_tr) throw HipError("PCG tile table exceeds its capacity", PDC_ERR_STATE);
+	ctx->tb_r.ensure(S + 1);
+	PDC_HIP(hipMemcpyAsync(ctx->tb_r.p, tb.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+	ctx->l_slot.ensure(NL + 1);
+	if (NL > 0)
+		hipLaunchKernelGGL(k_row_slot, dim3(grid_for(NL)), dim3(kBlock), 0, s, NL, ctx->l2g.p, ctx->b_scene.p,
+		                   ctx->r_lbeg.p, ctx->tb_r.p, ctx->l_slot.p);
+	PDC_CHECK_LAUNCH();
 	std::vector<PcgTile> pt(t.size());
 	for (size_t k = 0; k < t.size(); ++k) {
 		PcgTile r{};
@@ -389,10 +426,12 @@ void build_topology(pdc_context* ctx) {
 	                   ctx->adj_pos.p, ctx->inc_ptr.p, ctx->c_pid.p, ctx->inc.p);
 	PDC_CHECK_LAUNCH();
 	scene_ranges(ctx);
-	const int NNZB = ctx->NNZB;
-	ctx->a_col.ensure(NNZB);
+	// ELL pattern over every slot of the PCG tiles (empty entries -1)
+	const size_t ell = (size_t)kPcgRowsPerWG * ctx->cap_tr * ctx->ell_R;
+	ctx->e_col.ensure(ell);
+	PDC_HIP(hipMemsetAsync(ctx->e_col.p, 0xff, ell * sizeof(int), s));
 	hipLaunchKernelGGL(k_a_fill, dim3(grid_for(NL)), dim3(kBlock), 0, s, NL, ctx->l_static.p,
-	                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->a_ptr.p, ctx->a_col.p, ctx->a_diag.p,
+	                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->l_slot.p, ctx->ell_R, ctx->e_col.p, ctx->a_diag.p,
 	                   ctx->adj_blk.p);
 	PDC_CHECK_LAUNCH();
 }
@@ -432,7 +471,8 @@ struct SArgs {
 	const int* inc_ptr;   // per body: incidence offsets (NL+1)
 	const int* inc;       // incidence entries 2*contact + side
 	const int* pair_adj;  // per pair: sorted adjacency entry of each end
-	const int* adj_blk;   // per adjacency entry: Newton-matrix block or -1
+	const int* adj_blk;   // per adjacency entry: Newton-matrix block (ELL entry) or -1
+	const int* slot;      // per local body: PCG slot (index of its PCG vectors and preconditioner block)
 	// bodies
 	double* v;
 	double* vold;
@@ -1018,12 +1058,13 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 	double Dr[6];
 #pragma unroll
 	for (int q = 0; q < 6; ++q) Dr[q] = 1 / sqrt(S[q * 6 + q]);
+	const int sr = a.slot[r];
 	if (lane < 6) {
 		const double rhs = st ? 0.0 : PDC_PICK6(y, lane) - PDC_PICK6(ru, lane);
 		const double dq = PDC_PICK6(Dr, lane);
 		a.D[6 * r + lane] = dq;
 		a.rhs[6 * r + lane] = rhs;
-		bvec[6 * r + lane] = (T)(rhs * dq);
+		bvec[6 * sr + lane] = (T)(rhs * dq);
 	}
 	// scaled block S = D B D
 #pragma unroll
@@ -1086,7 +1127,7 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 		double acc = 0;
 #pragma unroll
 		for (int k = 0; k < 6; ++k) acc += PDC_ROW6(S, ii, k) * col[k];
-		Pval[36 * (size_t)r + lane] = (T)PDC_PICK6(col, ii);
+		Pval[36 * (size_t)sr + lane] = (T)PDC_PICK6(col, ii);
 		Aval[36 * (size_t)adiag[r] + lane] = (T)acc;
 	}
 	// PCG start x0 and S x0, which the stored operator A P maps to A x0
@@ -1094,8 +1135,8 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 		double acc = 0;
 #pragma unroll
 		for (int k = 0; k < 6; ++k) acc += PDC_ROW6(S, lane, k) * x0[k];
-		a.xs[6 * r + lane] = PDC_PICK6(x0, lane);
-		a.ys[6 * r + lane] = acc;
+		a.xs[6 * sr + lane] = PDC_PICK6(x0, lane);
+		a.ys[6 * sr + lane] = acc;
 	}
 }
 
@@ -1127,11 +1168,12 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ 
 	const int rb = a.li0[c0] == ra ? a.li1[c0] : a.li0[c0];
 	// scaling and preconditioner entries, requested before the contact loop
 	const double da = a.D[6 * ra + ii], db = a.D[6 * rb + jj];
+	const int sa = a.slot[ra], sb = a.slot[rb];
 	double pb[6], pa[6];
 #pragma unroll
 	for (int k = 0; k < 6; ++k) {
-		pb[k] = (double)Pval[36 * (size_t)rb + 6 * k + jj];
-		pa[k] = (double)Pval[36 * (size_t)ra + 6 * k + jj];
+		pb[k] = (double)Pval[36 * (size_t)sb + 6 * k + jj];
+		pa[k] = (double)Pval[36 * (size_t)sa + 6 * k + jj];
 	}
 	double sum = 0;
 	if (!LARGE) {
@@ -1226,19 +1268,21 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ 
 // while B and P are stored as T (FP32 on the FP32 path, which then solves the
 // FP32-rounded operator).
 //
-// Layout: one wavefront per block row; lane = 8 * row + slot for the six
-// scalar rows of the block row, the eight slot lanes of a row stride over the
-// row's blocks and are summed with three xor shuffles.
+// Layout: slot space (one block row per slot, kPcgRowsPerWG slots per
+// workgroup: slot = kPcgRowsPerWG * workgroup + wavefront), so that a
+// wavefront's row, its vectors, its preconditioner block and its ELL operator
+// row are addressed from its indices alone.  One wavefront per block row;
+// lane = 8 * row + s for the six scalar rows of the block row, the eight s
+// lanes of a row take the row's ELL entries s, s + 8, ... and are summed with
+// three xor shuffles.
 template <class T>
 struct PcgArgs {
-	const int* ptr;
-	const int* col;
-	const T* B;     // A M, block-CSR, row-major 6x6 blocks
-	const T* P;     // block-Jacobi inverse, one 6x6 block per block row
+	const int* ecol;  // [slots][R]: column slot of each ELL entry, -1 empty
+	const T* B;       // A M, ELL by slot, row-major 6x6 blocks
+	const T* P;       // block-Jacobi inverse, one 6x6 block per slot
 	const T* b;
 	double* x;
 	double* r;
-	double* u;
 	double* w[2];
 	double* z;
 	double* q;
@@ -1253,6 +1297,7 @@ struct PcgArgs {
 	const SceneSc* ss;      // per scene: `run`, the scene is still in its Newton loop
 	PcgScene* ps;           // per scene: done flag, iteration counters
 	double tol2;            // squared relative tolerance
+	int R;                  // ELL entries per slot (a multiple of 8)
 };
 
 // Element k (0 or 1) of a two-slot kernel argument, selected without a runtime
@@ -1262,38 +1307,27 @@ __device__ __forceinline__ P pick(const P (&slots)[2], int k) {
 	return k ? slots[1] : slots[0];
 }
 
-// Row `i` (0..5) of block row `br` of B times v; every lane of the row's
-// eight-lane group returns the sum.  Lanes of rows 6, 7 return 0.
-template <class T, class V>
-__device__ __forceinline__ double spmv_lane(const PcgArgs<T>& a, const V* __restrict__ v, int br, int i, int sl) {
-	double acc = 0;
-	if (i < 6) {
-		const int k0 = a.ptr[br], k1 = a.ptr[br + 1];
-		for (int k = k0 + sl; k < k1; k += 8) {
-			const T* blk = a.B + 36 * (size_t)k + 6 * i;
-			const V* vc = v + 6 * (size_t)a.col[k];
-#pragma unroll
-			for (int c = 0; c < 6; ++c) acc += (double)blk[c] * (double)vc[c];
-		}
-	}
-	acc += __shfl_xor(acc, 4, 64);
-	acc += __shfl_xor(acc, 2, 64);
-	acc += __shfl_xor(acc, 1, 64);
-	return acc;
+// The wavefront's slot.
+__device__ __forceinline__ int pcg_slot() {
+	return kPcgRowsPerWG * (int)blockIdx.x + (int)(threadIdx.x >> 6);
 }
 
-// Row `i` (0..5) of the block row whose blocks are [k0, k1) of B times v;
-// every lane of the row's eight-lane group returns the sum.
+// Row `i` (0..5) of the ELL row of slot `sr` of B times v; every lane of the
+// row's eight-lane group returns the sum.  Lanes of rows 6, 7 return 0.  An
+// empty entry's (uninitialised) block is read but not added.
 template <class T, class V>
-__device__ __forceinline__ double spmv_lane_range(const PcgArgs<T>& a, const V* __restrict__ v, int k0, int k1,
-                                                  int i, int sl) {
+__device__ __forceinline__ double spmv_ell(const PcgArgs<T>& a, const V* __restrict__ v, int sr, int i, int sl) {
 	double acc = 0;
 	if (i < 6)
-		for (int k = k0 + sl; k < k1; k += 8) {
-			const T* blk = a.B + 36 * (size_t)k + 6 * i;
-			const V* vc = v + 6 * (size_t)a.col[k];
+		for (int e = sl; e < a.R; e += 8) {
+			const size_t k = (size_t)sr * a.R + e;
+			const int c = a.ecol[k];
+			const T* blk = a.B + 36 * k + 6 * i;
+			const V* vc = v + 6 * (size_t)(c < 0 ? 0 : c);
+			double d = 0;
 #pragma unroll
-			for (int c = 0; c < 6; ++c) acc += (double)blk[c] * (double)vc[c];
+			for (int cc = 0; cc < 6; ++cc) d += (double)blk[cc] * (double)vc[cc];
+			acc += c >= 0 ? d : 0.0;
 		}
 	acc += __shfl_xor(acc, 4, 64);
 	acc += __shfl_xor(acc, 2, 64);
@@ -1341,11 +1375,10 @@ __device__ __forceinline__ void reduce_partials(const double* p0, const double* 
 	for (int k = 0; k < N; ++k) out[k] = lds[k];
 }
 
-// Start of a solve for one tile: its record's row pointers and state; the
-// scene's first tile also resets the scene's outcome and history.
+// Start of a solve for one tile: its state; the scene's first tile also
+// resets the scene's outcome and history.
 template <class T>
 __device__ __forceinline__ void pcg_tile_reset(const PcgArgs<T>& a, PcgTile* Tp, int scn) {
-	if ((int)threadIdx.x <= Tp->count) Tp->ptr[threadIdx.x] = a.ptr[Tp->first + threadIdx.x];
 	if (threadIdx.x == 0) {
 		Tp->done = 0;
 		Tp->it = 0;
@@ -1377,12 +1410,12 @@ template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_start(PcgArgs<T> a, const double* __restrict__ y) {
 	PcgTile* Tp = a.tiles + blockIdx.x;
 	if (pcg_tile_skip(a, Tp)) return;
-	const int cnt = Tp->count, first = Tp->first;
+	const int cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	if (wid < cnt) {
-		const int br = first + wid;
-		const double ay = spmv_lane(a, y, br, i, sl);
+		const int br = pcg_slot();
+		const double ay = spmv_ell(a, y, br, i, sl);
 		if (i < 6 && sl == 0) a.r[6 * br + i] = (double)a.b[6 * br + i] - ay;
 	}
 }
@@ -1392,13 +1425,13 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	__shared__ double lds[4 * 16];
 	PcgTile* Tp = a.tiles + blockIdx.x;
 	if (pcg_tile_skip(a, Tp)) return;
-	const int scn = Tp->scene, cnt = Tp->count, first = Tp->first;
+	const int scn = Tp->scene, cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[4] = {0, 0, 0, 0};
 	if (wid < cnt) {
-		const int br = first + wid;
-		const double w0 = spmv_lane(a, a.r, br, i, sl);
+		const int br = pcg_slot();
+		const double w0 = spmv_ell(a, a.r, br, i, sl);
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
 			const double rt = a.r[t], bt = a.b[t];
@@ -1426,50 +1459,61 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	pcg_tile_reset(a, Tp, scn);
 }
 
-// Operator row and row vectors of one block row as read by k_pcg_iter
-// (meaningful in the lanes with i < 6, sl == 0).  u = M r and q = M s are
-// not stored: each lane forms its entry from the block's r and s.
-struct PcgRow {
-	double n, m, w, z, s, p, x, r, u, ms;
-};
-
 // Iteration parity equals the launch slot: every chunk has an even number of
-// launches and each active launch advances the scene's counter by one.  The
-// workgroup's record (one scalar load) gives the rows, the scene's partial
-// range and the row pointers, so the partial sums, the operator rows and the
-// row vectors are all requested in the second memory round trip and the
-// gathered neighbour values in the third.  A tile whose scene has converged
-// returns after reading its record.
-template <class T>
+// launches and each active launch advances the scene's counter by one.
+// Memory round trips: (1) the workgroup's record (scalar loads) together
+// with everything addressed by the slot alone (ELL columns and blocks, the
+// preconditioner block, the row vectors); (2) the partial sums of the
+// previous launch (their range comes with the record) and the gathered
+// neighbour values (their slots come with the ELL columns).  EM is the ELL
+// width in groups of 8 entries (the loads of all of them are issued at once);
+// EM = 0 reads the width from the arguments.  A tile whose scene has
+// converged returns after its record arrives.
+template <class T, int EM>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
 	static_assert(kPcgRowsPerWG >= 4, "four partial arrays, one per wavefront");
 	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
-	// the whole record in one batch of scalar loads; the wavefront's row
-	// pointers are selected from it with the wave-uniform index (no runtime
-	// index into the private copy), and the early exit depends only on fields
-	// every wavefront shares
-	const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-	int rec[16];
 	const int* __restrict__ rp = reinterpret_cast<const int*>(Tp);
+	int rec[8];
 #pragma unroll
-	for (int k = 0; k < 16; ++k) rec[k] = rp[k];
-	const int scn = rec[0], first = rec[1], cnt = rec[2], t0 = rec[3], W = rec[4], done = rec[5], it = rec[6];
-	int k0 = rec[7], k1 = rec[8];
-#pragma unroll
-	for (int q = 1; q < kPcgRowsPerWG; ++q)
-		if (wid == q) {
-			k0 = rec[7 + q];
-			k1 = rec[8 + q];
-		}
-	if (scn < 0 || done) return;
-	const int cur = slot, nxt = slot ^ 1;
+	for (int k = 0; k < 8; ++k) rec[k] = rp[k];
+	const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 	const int lane = threadIdx.x & 63;
 	const int i = lane >> 3, sl = lane & 7;
+	const int ir = i < 6 ? i : 5;  // rows 6, 7 of the lane layout read row 5's addresses
+	const int br = kPcgRowsPerWG * (int)blockIdx.x + wid;
+	const int cur = slot, nxt = slot ^ 1;
 	const double* __restrict__ wc = pick(a.w, cur);
-	// partial sums of the previous launch: loads issued now, summed after the
-	// operator-row loads are in flight (a scene of more than 1024 tiles sums
-	// the rest afterwards)
+	// round 1: the ELL row (entries sl, sl + 8, ...), the preconditioner row
+	// and the row vectors
+	constexpr int kE = EM > 0 ? EM : 1;
+	const int R = EM > 0 ? 8 * EM : a.R;
+	int ec[kE];
+	T bv[kE][6];
+#pragma unroll
+	for (int m = 0; m < kE; ++m) {
+		const size_t k = (size_t)br * R + sl + 8 * m;
+		ec[m] = a.ecol[k];
+		const T* blk = a.B + 36 * k + 6 * ir;
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) bv[m][cc] = blk[cc];
+	}
+	double pr[6], wr[6], rrow[6], srow[6];
+#pragma unroll
+	for (int cc = 0; cc < 6; ++cc) {
+		pr[cc] = (double)a.P[36 * (size_t)br + 6 * ir + cc];
+		wr[cc] = wc[6 * br + cc];
+		rrow[cc] = a.r[6 * br + cc];
+		srow[cc] = a.s[6 * br + cc];
+	}
+	const int t = 6 * br + ir;
+	const double zo = a.z[t], po = a.p[t], xo = a.x[t];
+	const int scn = rec[0], cnt = rec[2], t0 = rec[3], W = rec[4], done = rec[5], it = rec[6];
+	if (scn < 0 || done) return;
+	// round 2: the partial sums of the previous launch (issued now, summed
+	// after the gathers are in flight; a scene of more than 1024 tiles sums
+	// the rest afterwards) and the neighbour values
 	const double* psrc = wid == 0 ? pick(a.gam, cur) : wid == 1 ? pick(a.del, cur) : wid == 2 ? pick(a.rr, cur) : a.bb;
 	psrc += t0;
 	double pv[kPartLoads];
@@ -1477,27 +1521,39 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 #pragma unroll
 		for (int k = 0; k < kPartLoads; ++k) pv[k] = psrc[min(lane + 64 * k, W - 1)];
 	}
-	const int br = first + wid;
-	PcgRow R{};
-	if (wid < cnt) {
-		R.n = spmv_lane_range(a, wc, k0, k1, i, sl);
-		if (i < 6 && sl == 0) {
-			const int t = 6 * br + i;
-			double m = 0, u = 0, ms = 0;
+	double n = 0;
+	if (i < 6) {
 #pragma unroll
-			for (int cc = 0; cc < 6; ++cc) {
-				const double pc = (double)a.P[36 * (size_t)br + 6 * i + cc];
-				m += pc * wc[6 * br + cc];
-				u += pc * a.r[6 * br + cc];
-				ms += pc * a.s[6 * br + cc];
-			}
-			R.m = m;
-			R.u = u;
-			R.ms = ms;
-			R.w = wc[t];
-			R.z = a.z[t]; R.s = a.s[t]; R.p = a.p[t];
-			R.x = a.x[t]; R.r = a.r[t];
+		for (int m = 0; m < kE; ++m) {
+			const int c = ec[m];
+			const double* vc = wc + 6 * (size_t)(c < 0 ? 0 : c);
+			double d = 0;
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) d += (double)bv[m][cc] * vc[cc];
+			n += c >= 0 ? d : 0.0;
 		}
+		if (EM == 0)
+			for (int e = sl + 8; e < R; e += 8) {
+				const size_t k = (size_t)br * R + e;
+				const int c = a.ecol[k];
+				const T* blk = a.B + 36 * k + 6 * i;
+				const double* vc = wc + 6 * (size_t)(c < 0 ? 0 : c);
+				double d = 0;
+#pragma unroll
+				for (int cc = 0; cc < 6; ++cc) d += (double)blk[cc] * vc[cc];
+				n += c >= 0 ? d : 0.0;
+			}
+	}
+	n += __shfl_xor(n, 4, 64);
+	n += __shfl_xor(n, 2, 64);
+	n += __shfl_xor(n, 1, 64);
+	// m = M w, u = M r (not stored), M s of the old s
+	double m = 0, u = 0, ms = 0;
+#pragma unroll
+	for (int cc = 0; cc < 6; ++cc) {
+		m += pr[cc] * wr[cc];
+		u += pr[cc] * rrow[cc];
+		ms += pr[cc] * srow[cc];
 	}
 	if (wid < 4) {
 		double part = 0;
@@ -1523,18 +1579,18 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	double* __restrict__ wn = pick(a.w, nxt);
 	double v[4] = {0, 0, 0, 0};
 	if (wid < cnt && i < 6 && sl == 0) {
-		const int t = 6 * br + i;
-		const double zt = R.n + beta * R.z;
-		const double qt = R.m + beta * R.ms;   // M s of the updated s
-		const double st = R.w + beta * R.s;
-		const double pt = R.u + beta * R.p;
+		const double wt = PDC_PICK6(wr, i), rt0 = PDC_PICK6(rrow, i), st0 = PDC_PICK6(srow, i);
+		const double zt = n + beta * zo;
+		const double qt = m + beta * ms;   // M s of the updated s
+		const double st = wt + beta * st0;
+		const double pt = u + beta * po;
 		a.z[t] = zt;
 		a.s[t] = st;
 		a.p[t] = pt;
-		a.x[t] = R.x + alpha * pt;
-		const double rt = R.r - alpha * st;
-		const double ut = R.u - alpha * qt;    // M r of the updated r
-		const double wnt = R.w - alpha * zt;
+		a.x[t] = xo + alpha * pt;
+		const double rt = rt0 - alpha * st;
+		const double ut = u - alpha * qt;    // M r of the updated r
+		const double wnt = wt - alpha * zt;
 		a.r[t] = rt;
 		wn[t] = wnt;
 		v[0] += rt * ut;
@@ -1572,12 +1628,12 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_init(PcgArgs<T> a) {
 	__shared__ double lds[4 * 16];
 	PcgTile* Tp = a.tiles + blockIdx.x;
 	if (pcg_tile_skip(a, Tp)) return;
-	const int scn = Tp->scene, cnt = Tp->count, first = Tp->first;
+	const int scn = Tp->scene, cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[3] = {0, 0, 0};
 	if (wid < cnt && i < 6 && sl == 0) {
-		const int br = first + wid;
+		const int br = pcg_slot();
 		const int t = 6 * br + i;
 		double zt = 0;
 #pragma unroll
@@ -1606,7 +1662,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) 
 	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
 	const int scn = Tp->scene;
 	if (scn < 0 || Tp->done) return;
-	const int it = Tp->it, t0 = Tp->t0, W = Tp->W, cnt = Tp->count, first = Tp->first;
+	const int it = Tp->it, t0 = Tp->t0, W = Tp->W, cnt = Tp->count;
 	const int cur = it & 1, nxt = cur ^ 1;
 	double red[4];
 	reduce_partials<4>(pick(a.rr, cur), a.bb, pick(a.gam, cur), pick(a.gam, nxt), t0, W, red, lds);
@@ -1623,8 +1679,8 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) 
 	const int i = lane >> 3, sl = lane & 7;
 	double v[1] = {0};
 	if (wid < cnt) {
-		const int br = first + wid;
-		const double n = spmv_lane_range(a, a.r, Tp->ptr[wid], Tp->ptr[wid + 1], i, sl);
+		const int br = pcg_slot();
+		const double n = spmv_ell(a, a.r, br, i, sl);
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
 			double yb[6];
@@ -1654,7 +1710,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) 
 	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
 	const int scn = Tp->scene;
 	if (scn < 0 || Tp->done) return;
-	const int it = Tp->it, t0 = Tp->t0, W = Tp->W, cnt = Tp->count, first = Tp->first;
+	const int it = Tp->it, t0 = Tp->t0, W = Tp->W, cnt = Tp->count;
 	const int cur = it & 1, nxt = cur ^ 1;
 	double red[2];
 	reduce_partials<2>(pick(a.gam, cur), a.del[0], nullptr, nullptr, t0, W, red, lds);
@@ -1664,7 +1720,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) 
 	const int i = lane >> 3, sl = lane & 7;
 	double v[2] = {0, 0};
 	if (wid < cnt && i < 6 && sl == 0) {
-		const int br = first + wid;
+		const int br = pcg_slot();
 		const int t = 6 * br + i;
 		double rb[6], pt = 0;
 #pragma unroll
@@ -1698,7 +1754,8 @@ __global__ __launch_bounds__(256) void k_du(SArgs a, const double* __restrict__ 
 	(void)scn;
 	if ((int)threadIdx.x < tile.count) {
 		const int r = tile.first + threadIdx.x;
-		for (int k = 0; k < 6; ++k) a.du[6 * r + k] = x[6 * r + k] * a.D[6 * r + k];
+		const int sr = a.slot[r];
+		for (int k = 0; k < 6; ++k) a.du[6 * r + k] = x[6 * sr + k] * a.D[6 * r + k];
 	}
 }
 
@@ -2093,6 +2150,7 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	a.pstart = ctx->ap_start.p; a.pa = ctx->ap_a.p; a.g2l = ctx->g2l.p;
 	a.lstat = ctx->l_static.p; a.l2g = ctx->l2g.p;
 	a.inc_ptr = ctx->inc_ptr.p; a.inc = ctx->inc.p; a.pair_adj = ctx->pair_adj.p; a.adj_blk = ctx->adj_blk.p;
+	a.slot = ctx->l_slot.p;
 	a.v = ctx->l_v.p; a.vold = ctx->l_vold.p; a.vphys = ctx->l_vphys.p; a.Ms = ctx->l_Ms.p;
 	a.Minv = ctx->l_Minv.p; a.speed = ctx->l_speed.p; a.ru = ctx->l_ru.p; a.rhs = ctx->l_rhs.p;
 	a.D = ctx->l_D.p; a.du = ctx->l_du.p; a.xs = ctx->x64.p; a.ys = ctx->y64.p;
@@ -2117,7 +2175,13 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 	for (int k = 0; k < kPcgChunk; ++k) {
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k], s));
 		if (two) hipLaunchKernelGGL(k_pcg2_a<T>, dim3(nblk), B, 0, s, pa, k & 1);
-		else hipLaunchKernelGGL(k_pcg_iter<T>, dim3(nblk), B, 0, s, pa, k & 1);
+		else switch (pa.R / 8) {
+			case 1: hipLaunchKernelGGL((k_pcg_iter<T, 1>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 2: hipLaunchKernelGGL((k_pcg_iter<T, 2>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 3: hipLaunchKernelGGL((k_pcg_iter<T, 3>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 4: hipLaunchKernelGGL((k_pcg_iter<T, 4>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			default: hipLaunchKernelGGL((k_pcg_iter<T, 0>), dim3(nblk), B, 0, s, pa, k & 1); break;
+		}
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k + 1], s));
 		if (two) hipLaunchKernelGGL(k_pcg2_b<T>, dim3(nblk), B, 0, s, pa, k & 1);
 	}
@@ -2135,21 +2199,21 @@ static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 	const int S = ctx->S;
 	// grid fixed per scene set (capacity of PCG tiles) so the recorded chunk stays valid
 	const int nblk = std::max(1, ctx->cap_tr);
-	const size_t ncap = 6 * (size_t)std::max(ctx->N, 1), pcap = (size_t)nblk;
+	const size_t ncap = 6 * (size_t)kPcgRowsPerWG * nblk, pcap = (size_t)nblk;
 	DBuf<double>& work = ctx->pcg64;
-	work.ensure(10 * ncap + 7 * pcap + 4 * (size_t)S + 64);
-	ctx->x64.ensure(ncap);
+	work.ensure(7 * ncap + 7 * pcap + 4 * (size_t)S + 64);
 	PcgArgs<T> pa;
-	pa.ptr = ctx->a_ptr.p;
-	pa.col = ctx->a_col.p;
+	std::memset(&pa, 0, sizeof(pa));  // compared bytewise with the recorded arguments
+	pa.ecol = ctx->e_col.p;
+	pa.R = ctx->ell_R;
 	pa.B = Bm.p;
 	pa.P = P.p;
 	pa.b = b.p;
 	pa.x = ctx->x64.p;
 	double* w = work.p;
-	pa.r = w; pa.u = w + ncap; pa.w[0] = w + 2 * ncap; pa.w[1] = w + 3 * ncap;
-	pa.z = w + 4 * ncap; pa.q = w + 5 * ncap; pa.s = w + 6 * ncap; pa.p = w + 7 * ncap;
-	double* parts = w + 10 * ncap;
+	pa.r = w; pa.w[0] = w + ncap; pa.w[1] = w + 2 * ncap;
+	pa.z = w + 3 * ncap; pa.q = w + 4 * ncap; pa.s = w + 5 * ncap; pa.p = w + 6 * ncap;
+	double* parts = w + 7 * ncap;
 	pa.gam[0] = parts; pa.gam[1] = parts + pcap;
 	pa.del[0] = parts + 2 * pcap; pa.del[1] = parts + 3 * pcap;
 	pa.rr[0] = parts + 4 * pcap; pa.rr[1] = parts + 5 * pcap;
@@ -2316,13 +2380,14 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	ctx->l_Ms.ensure(7 * nb); ctx->l_Minv.ensure(7 * nb); ctx->l_speed.ensure(nb);
 	ctx->l_ru.ensure(6 * nb); ctx->l_rhs.ensure(6 * nb); ctx->l_D.ensure(6 * nb);
 	ctx->l_du.ensure(6 * nb); ctx->l_gdu.ensure(6 * nb);
-	ctx->x64.ensure(6 * (size_t)std::max(ctx->N, 1)); ctx->y64.ensure(6 * (size_t)std::max(ctx->N, 1));
+	// PCG storage by slot (kPcgRowsPerWG per PCG tile), the operator ELL
+	const size_t nslot = (size_t)kPcgRowsPerWG * std::max(1, ctx->cap_tr), nell = nslot * ctx->ell_R;
+	ctx->x64.ensure(6 * nslot); ctx->y64.ensure(6 * nslot);
 	const bool fp32 = ctx->p.pcg_precision == PDC_PCG_FP32;
-	const size_t nnzb = (size_t)ctx->NNZB;
 	if (fp32) {
-		ctx->A32.ensure(36 * nnzb); ctx->P32.ensure(36 * nb); ctx->b32.ensure(6 * nb);
+		ctx->A32.ensure(36 * nell); ctx->P32.ensure(36 * nslot); ctx->b32.ensure(6 * nslot);
 	} else {
-		ctx->A64.ensure(36 * nnzb); ctx->P64.ensure(36 * nb); ctx->b64.ensure(6 * nb);
+		ctx->A64.ensure(36 * nell); ctx->P64.ensure(36 * nslot); ctx->b64.ensure(6 * nslot);
 	}
 	SArgs a0 = make_args(ctx, dt, g);
 	ctx->partials.ensure((size_t)P_COUNT * a0.pstride);
