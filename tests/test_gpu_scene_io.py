@@ -1,0 +1,99 @@
+"""A scene file baked on the device (the reference's command-line path,
+CLIContactSimulation.cpp): YAML + glTF input, fitted primitives, the baking
+loop with fixed and adaptive substeps, glTF / NPZ / .info output.  The
+trajectory of the loaded scene is checked against the oracle's FP64 one."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from primal_dual_contacts_miror_amd import scene_io as S
+
+pytestmark = pytest.mark.gpu
+
+YAML = """scene:
+  cubes: "boxes.gltf"
+  spheres: "balls.gltf"
+  static: [0]
+simulation:
+  timestep: 1e-2
+  adaptive_timestep: {adaptive}
+  substep_factor: 2.5
+  solver:
+    name: "primal_dual"
+    iterations: 100
+    tolerance: 1e-10
+    non_smooth_force:
+      name: "non_smooth_contact"
+      restitution: 0.0
+      static_coeff: 0.5
+  gravity: [0, -9.81, 0]
+baking:
+  time: {time}
+"""
+
+
+def _scene(tmp_path, adaptive=False, time=0.395):
+    cube, ico = S.unit_cube_mesh(), S.icosphere_mesh(3)
+
+    def node(scale, t):
+        M = np.eye(4)
+        M[:3, :3] = np.diag(scale)
+        M[:3, 3] = t
+        return M
+    # a static floor, three stacked unit cubes with 0.001 gaps, a ball on top
+    boxes = [node((10, 0.5, 10), (0, -0.5, 0))] + [node((0.5, 0.5, 0.5), (0, 0.501 + 1.001 * k, 0))
+                                                   for k in range(3)]
+    S.write_gltf(str(tmp_path / "boxes.gltf"), [cube] * 4, boxes)
+    S.write_gltf(str(tmp_path / "balls.gltf"), [ico], [node((0.4, 0.4, 0.4), (0.1, 3.6, 0))])
+    (tmp_path / "scene.yaml").write_text(YAML.format(adaptive="true" if adaptive else "false", time=time))
+    return str(tmp_path / "scene.yaml")
+
+
+def test_bake_matches_oracle_and_writes_outputs(require_gpu, tmp_path):
+    L = S.load_scene(_scene(tmp_path), pcg_tol=1e-13)
+    sc = L.scene
+    assert sc.n == 5
+    times = S.bake(L, str(tmp_path / "out.gltf"))
+    # frames until the time passes the baking time (SimulationSceneCLI.cpp:9-19)
+    assert len(times) == 40
+    np.testing.assert_allclose(np.diff(times), 0.01, rtol=1e-9)
+    assert times[-1] > L.baking_time and times[-2] <= L.baking_time
+    for ext in (".gltf", ".bin", ".npz", ".info"):
+        assert os.path.exists(tmp_path / ("out" + ext))
+    z = np.load(tmp_path / "out.npz")
+    assert z["time"].shape == (40,)
+    assert all(z["mesh_%d" % k].shape == (41, 7) for k in range(sc.n))
+    info = (tmp_path / "out.info").read_text()
+    assert "Total #solids: 5" in info and "Average #iterations" in info
+    # the oracle's FP64 trajectory of the same loaded scene (direct solve)
+    o = orc.OracleScene(sc, L.params)
+    o.set_linear_solver(orc.SOLVER_DIRECT)
+    for _ in range(len(times)):
+        o.step(1)
+    po, qo = o.get_state()[:2]
+    dyn = sc.is_static == 0
+    for k in np.nonzero(dyn)[0]:
+        g = z["mesh_%d" % k][-1]
+        size = float(np.max(sc.dims[k]))
+        assert np.linalg.norm(g[:3] - po[k]) < 1e-6 * max(size, 1.0), k
+        qg = g[3:]
+        ang = 2 * np.arccos(min(1.0, abs(float(np.dot(qg, qo[k])))))
+        assert ang < 1e-6, k
+    # the ball fell onto the stack and everything rests: top of the stack below 3.1
+    assert z["mesh_4"][-1][1] < 3.6 and z["mesh_4"][-1][1] > 3.0
+    nodes = S.read_gltf(str(tmp_path / "out.gltf"))
+    assert len(nodes) == sc.n
+    np.testing.assert_allclose(nodes[4].matrix[:3, 3], z["mesh_4"][-1][:3], atol=1e-12)
+
+
+def test_bake_adaptive_substeps(require_gpu, tmp_path):
+    L = S.load_scene(_scene(tmp_path, adaptive=True, time=0.045))
+    times = S.bake(L, str(tmp_path / "ad.gltf"))
+    assert len(times) == 5
+    # the substep sizes of a frame add up to one timestep (DynamicSystem.cpp:165-199)
+    np.testing.assert_allclose(np.diff(times), 0.01, rtol=1e-9)
+    info = (tmp_path / "ad.info").read_text()
+    sub = float(info.split("Average substeps per step:")[1].split()[0])
+    assert sub >= 1.0
