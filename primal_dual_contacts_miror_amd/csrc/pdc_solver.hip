@@ -1313,8 +1313,8 @@ __device__ __forceinline__ int pcg_slot() {
 }
 
 // Row `i` (0..5) of the ELL row of slot `sr` of B times v; every lane of the
-// row's eight-lane group returns the sum.  Lanes of rows 6, 7 return 0.  An
-// empty entry's (uninitialised) block is read but not added.
+// row's eight-lane group returns the sum.  Lanes of rows 6, 7 return 0.
+// Empty entries (column -1) are skipped.
 template <class T, class V>
 __device__ __forceinline__ double spmv_ell(const PcgArgs<T>& a, const V* __restrict__ v, int sr, int i, int sl) {
 	double acc = 0;
@@ -1322,12 +1322,13 @@ __device__ __forceinline__ double spmv_ell(const PcgArgs<T>& a, const V* __restr
 		for (int e = sl; e < a.R; e += 8) {
 			const size_t k = (size_t)sr * a.R + e;
 			const int c = a.ecol[k];
+			if (c < 0) continue;
 			const T* blk = a.B + 36 * k + 6 * i;
-			const V* vc = v + 6 * (size_t)(c < 0 ? 0 : c);
+			const V* vc = v + 6 * (size_t)c;
 			double d = 0;
 #pragma unroll
 			for (int cc = 0; cc < 6; ++cc) d += (double)blk[cc] * (double)vc[cc];
-			acc += c >= 0 ? d : 0.0;
+			acc += d;
 		}
 	acc += __shfl_xor(acc, 4, 64);
 	acc += __shfl_xor(acc, 2, 64);
@@ -1462,13 +1463,14 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 // Iteration parity equals the launch slot: every chunk has an even number of
 // launches and each active launch advances the scene's counter by one.
 // Memory round trips: (1) the workgroup's record (scalar loads) together
-// with everything addressed by the slot alone (ELL columns and blocks, the
+// with everything addressed by the slot alone (ELL columns, the
 // preconditioner block, the row vectors); (2) the partial sums of the
-// previous launch (their range comes with the record) and the gathered
-// neighbour values (their slots come with the ELL columns).  EM is the ELL
-// width in groups of 8 entries (the loads of all of them are issued at once);
-// EM = 0 reads the width from the arguments.  A tile whose scene has
-// converged returns after its record arrives.
+// previous launch (their range comes with the record), and the operator
+// blocks and gathered neighbour values of the occupied ELL entries (known
+// from the columns; empty entries are not read).  EM is the ELL width in
+// groups of 8 entries (the loads of all of them are issued at once); EM = 0
+// reads the width from the arguments.  A tile whose scene has converged
+// returns after its record arrives.
 template <class T, int EM>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
@@ -1485,20 +1487,13 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	const int br = kPcgRowsPerWG * (int)blockIdx.x + wid;
 	const int cur = slot, nxt = slot ^ 1;
 	const double* __restrict__ wc = pick(a.w, cur);
-	// round 1: the ELL row (entries sl, sl + 8, ...), the preconditioner row
-	// and the row vectors
+	// round 1: the ELL columns of the row (entries sl, sl + 8, ...), the
+	// preconditioner row and the row vectors
 	constexpr int kE = EM > 0 ? EM : 1;
 	const int R = EM > 0 ? 8 * EM : a.R;
 	int ec[kE];
-	T bv[kE][6];
 #pragma unroll
-	for (int m = 0; m < kE; ++m) {
-		const size_t k = (size_t)br * R + sl + 8 * m;
-		ec[m] = a.ecol[k];
-		const T* blk = a.B + 36 * k + 6 * ir;
-#pragma unroll
-		for (int cc = 0; cc < 6; ++cc) bv[m][cc] = blk[cc];
-	}
+	for (int m = 0; m < kE; ++m) ec[m] = a.ecol[(size_t)br * R + sl + 8 * m];
 	double pr[6], wr[6], rrow[6], srow[6];
 #pragma unroll
 	for (int cc = 0; cc < 6; ++cc) {
@@ -1513,7 +1508,8 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	if (scn < 0 || done) return;
 	// round 2: the partial sums of the previous launch (issued now, summed
 	// after the gathers are in flight; a scene of more than 1024 tiles sums
-	// the rest afterwards) and the neighbour values
+	// the rest afterwards), the operator blocks and the neighbour values of
+	// the row's occupied entries
 	const double* psrc = wid == 0 ? pick(a.gam, cur) : wid == 1 ? pick(a.del, cur) : wid == 2 ? pick(a.rr, cur) : a.bb;
 	psrc += t0;
 	double pv[kPartLoads];
@@ -1526,22 +1522,26 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 #pragma unroll
 		for (int m = 0; m < kE; ++m) {
 			const int c = ec[m];
-			const double* vc = wc + 6 * (size_t)(c < 0 ? 0 : c);
-			double d = 0;
+			if (c >= 0) {
+				const T* blk = a.B + 36 * ((size_t)br * R + sl + 8 * m) + 6 * i;
+				const double* vc = wc + 6 * (size_t)c;
+				double d = 0;
 #pragma unroll
-			for (int cc = 0; cc < 6; ++cc) d += (double)bv[m][cc] * vc[cc];
-			n += c >= 0 ? d : 0.0;
+				for (int cc = 0; cc < 6; ++cc) d += (double)blk[cc] * vc[cc];
+				n += d;
+			}
 		}
 		if (EM == 0)
 			for (int e = sl + 8; e < R; e += 8) {
 				const size_t k = (size_t)br * R + e;
 				const int c = a.ecol[k];
+				if (c < 0) continue;
 				const T* blk = a.B + 36 * k + 6 * i;
-				const double* vc = wc + 6 * (size_t)(c < 0 ? 0 : c);
+				const double* vc = wc + 6 * (size_t)c;
 				double d = 0;
 #pragma unroll
 				for (int cc = 0; cc < 6; ++cc) d += (double)blk[cc] * vc[cc];
-				n += c >= 0 ? d : 0.0;
+				n += d;
 			}
 	}
 	n += __shfl_xor(n, 4, 64);
