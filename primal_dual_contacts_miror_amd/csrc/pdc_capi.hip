@@ -192,6 +192,10 @@ void pdc_destroy(pdc_context* ctx) {
 		if (G.exec) (void)hipGraphExecDestroy(G.exec);
 		if (G.graph) (void)hipGraphDestroy(G.graph);
 	}
+	for (auto& G : ctx->phase_graph) {
+		if (G.exec) (void)hipGraphExecDestroy(G.exec);
+		if (G.graph) (void)hipGraphDestroy(G.graph);
+	}
 	for (auto e : ctx->kt_direct) (void)hipEventDestroy(e);
 	free_all(ctx);
 	for (auto e : ctx->kt.pool) (void)hipEventDestroy(e);

@@ -2192,12 +2192,11 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 // single-reduction recurrences stall near 1e-10 on ill-conditioned systems.
 constexpr double kSingleReductionMinTol = 1e-9;
 
-// PCG of every scene still in its Newton loop; returns whether any scene is
-// (the scenes' run flags, set by k_err_final, arrive with the PCG done flags).
+// Arguments of the PCG kernels for the current scene set (grid fixed per
+// scene set: the capacity of PCG tiles, so that a recorded chunk stays valid).
 template <class T>
-static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b) {
+static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b) {
 	const int S = ctx->S;
-	// grid fixed per scene set (capacity of PCG tiles) so the recorded chunk stays valid
 	const int nblk = std::max(1, ctx->cap_tr);
 	const size_t ncap = 6 * (size_t)kPcgRowsPerWG * nblk, pcap = (size_t)nblk;
 	DBuf<double>& work = ctx->pcg64;
@@ -2223,12 +2222,30 @@ static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 	pa.ss = ctx->d_ss.p;
 	pa.ps = ctx->d_ps.p;
 	pa.tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
+	return pa;
+}
+
+// Start of the PCG of every scene still in its Newton loop: initial residual
+// and the first partial sums.
+template <class T>
+static void pcg_begin(pdc_context* ctx, const PcgArgs<T>& pa) {
+	const int nblk = std::max(1, ctx->cap_tr);
 	hipStream_t s = ctx->stream;
-	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
 	hipLaunchKernelGGL(k_pcg_start<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa, (const double*)ctx->y64.p);
-	if (two) hipLaunchKernelGGL(k_pcg2_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
+	if (ctx->p.pcg_tol < kSingleReductionMinTol) hipLaunchKernelGGL(k_pcg2_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
 	else hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
 	PDC_CHECK_LAUNCH();
+}
+
+// The PCG iterations of every scene still in its Newton loop, after
+// pcg_begin; returns whether any scene is (the scenes' run flags, set by
+// k_err_final, arrive with the PCG done flags).  The host waits here.
+template <class T>
+static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
+	const int S = ctx->S;
+	const int nblk = std::max(1, ctx->cap_tr);
+	hipStream_t s = ctx->stream;
+	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
 	// the chunk is recorded again when a buffer moved or the grid changed
 	PcgGraph& G = ctx->pcg_graph[(sizeof(T) == 8 ? 1 : 0) + (two ? 2 : 0)];
 	static_assert(sizeof(PcgArgs<T>) <= sizeof(G.args), "recorded arguments fit");
@@ -2328,8 +2345,6 @@ static bool run_pcg(pdc_context* ctx, const SArgs& sa, DBuf<T>& Bm, DBuf<T>& P, 
 			ctx->kt.bytes += iters[sc] * (mat + (two ? 40.0 : 96.0) * n6);
 		}
 	}
-	if (ctx->n_tb > 0) hipLaunchKernelGGL(k_du, dim3(ctx->n_tb), dim3(kBlock), 0, s, sa, ctx->x64.p);
-	PDC_CHECK_LAUNCH();
 	return any;
 }
 
@@ -2353,6 +2368,31 @@ void integrate(pdc_context* ctx, double dt) {
 	                   ctx->b_static.p, ctx->b_scene.p, ctx->d_step.p, ctx->b_acc.p, ctx->b_pos.p,
 	                   ctx->b_quat.p, ctx->b_vel.p, ctx->b_ang.p, dt);
 	PDC_CHECK_LAUNCH();
+}
+
+// Records the launches `fn` makes on the context's stream as a graph and
+// makes G.exec replay them.  The launch sequence keeps its shape from solve
+// to solve (only sizes and arguments change), so the executable graph is
+// updated in place; it is instantiated again when the update is refused.
+template <class F>
+static void record_phase(pdc_context* ctx, PcgGraph& G, F&& fn) {
+	hipStream_t s = ctx->stream;
+	hipGraph_t g = nullptr;
+	PDC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+	fn();
+	PDC_HIP(hipStreamEndCapture(s, &g));
+	if (G.exec) {
+		hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+		hipGraphNode_t bad = nullptr;
+		if (hipGraphExecUpdate(G.exec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+			(void)hipGetLastError();
+			(void)hipGraphExecDestroy(G.exec);
+			G.exec = nullptr;
+		}
+	}
+	if (!G.exec) PDC_HIP(hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0));
+	if (G.graph) (void)hipGraphDestroy(G.graph);
+	G.graph = g;
 }
 
 // Per-scene scalars to the pinned host mirror (blocking).
@@ -2428,7 +2468,13 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	// for the PCG, and learns there which scenes are still iterating.
 	const double* du = ctx->l_du.p;
 	const double* gdu = ctx->l_gdu.p;
-	for (int iter = 0; iter < ctx->p.iterations; ++iter) {
+	// phase A: errors, assembly and the PCG start; phase B: the increments,
+	// the line search and the end of the iteration
+	PcgArgs<float> pa32;
+	PcgArgs<double> pa64;
+	if (fp32) pa32 = pcg_args<float>(ctx, ctx->A32, ctx->P32, ctx->b32);
+	else pa64 = pcg_args<double>(ctx, ctx->A64, ctx->P64, ctx->b64);
+	auto phase_a = [&]() {
 		hipEvent_t e0;
 		kt_begin(ctx, PDC_KT_CONTACT_TERMS, &e0);
 		launch(k_constraints, ntc, kBlock, a);
@@ -2452,10 +2498,11 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		}
 		PDC_CHECK_LAUNCH();
 		kt_end(ctx, PDC_KT_ASSEMBLY, e0);
-		// solve (the one wait of the iteration)
-		const bool any = fp32 ? run_pcg<float>(ctx, a, ctx->A32, ctx->P32, ctx->b32)
-		                      : run_pcg<double>(ctx, a, ctx->A64, ctx->P64, ctx->b64);
-		if (!any) break;  // every scene converged or reached the cap
+		if (fp32) pcg_begin<float>(ctx, pa32);
+		else pcg_begin<double>(ctx, pa64);
+	};
+	auto phase_b = [&]() {
+		launch(k_du, ntb, kBlock, a, (const double*)ctx->x64.p);
 		launch(k_increments, ntc, kBlock, a);
 		// line search: two passes of kGroup trials in the Newton direction,
 		// then the gradient-descent fallback (PrimalDualForceSolver.cpp:441-467)
@@ -2483,6 +2530,26 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		launch(k_commit_v, ntb, kBlock, a, gdu);
 		launch(k_newton_end, gS, kBlock, a, ctx->p.pcg_max_iter, ctx->p.iterations);
 		PDC_CHECK_LAUNCH();
+	};
+	// Both phases are recorded once per solve and replayed every iteration
+	// (about thirty launches each, which the host would otherwise submit one
+	// by one after every wait).  Kernel timing of a phase's own kernels needs
+	// direct launches.
+	const int kw = ctx->kt.which;
+	const bool phase_graphs = ctx->use_graphs && kw != PDC_KT_CONTACT_TERMS && kw != PDC_KT_ASSEMBLY &&
+	                          kw != PDC_KT_LINE_SEARCH;
+	if (phase_graphs) {
+		record_phase(ctx, ctx->phase_graph[0], phase_a);
+		record_phase(ctx, ctx->phase_graph[1], phase_b);
+	}
+	for (int iter = 0; iter < ctx->p.iterations; ++iter) {
+		if (phase_graphs) PDC_HIP(hipGraphLaunch(ctx->phase_graph[0].exec, s));
+		else phase_a();
+		// the one wait of the iteration
+		const bool any = fp32 ? pcg_solve<float>(ctx, pa32) : pcg_solve<double>(ctx, pa64);
+		if (!any) break;  // every scene converged or reached the cap
+		if (phase_graphs) PDC_HIP(hipGraphLaunch(ctx->phase_graph[1].exec, s));
+		else phase_b();
 	}
 	launch(k_writeback_bodies, ntb, kBlock, a, ctx->b_acc.p);
 	launch(k_writeback_contacts, ntc, kBlock, a, ctx->c_lf.p);

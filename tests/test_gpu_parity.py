@@ -268,6 +268,35 @@ def test_step_is_deterministic(require_gpu):
         np.testing.assert_array_equal(a, b)
 
 
+def test_graph_replay_equals_direct_launches(require_gpu, monkeypatch):
+    """The recorded graphs (the PCG chunks and the two phases of a Newton
+    iteration, replayed every iteration and recorded again every step) give
+    the same bits as launching every kernel directly, over steps whose
+    contact sets and sizes change, with host reads between the replays."""
+    sc = scenes.pile(60, seed=4, half=2.5, height=8.0, name="pile_60", grid_n=48, sample_spacing=0.1)
+    o = _advance_oracle(sc, 150)
+    state = o.get_state()
+    res = []
+    for direct in (False, True):
+        if direct:
+            monkeypatch.setenv("PDC_NO_GRAPHS", "1")
+        else:
+            monkeypatch.delenv("PDC_NO_GRAPHS", raising=False)
+        ds = _gpu_scene(sc)
+        ds.set_state(*state)
+        traj = []
+        for _ in range(12):
+            st = ds.step(1)
+            traj.append((st.n_contacts, st.newton_iterations, st.pcg_iterations) + tuple(x.copy() for x in ds.state()))
+        res.append(traj)
+        ds.ctx.close()
+    assert sum(t[1] for t in res[0]) > 12, "the steps should iterate"
+    for a, b in zip(res[0], res[1]):
+        assert a[:3] == b[:3]
+        for x, y in zip(a[3:], b[3:]):
+            np.testing.assert_array_equal(x, y)
+
+
 def test_settled_pile_one_step(require_gpu):
     """Per-step parity at ~30k contacts from an identical input state (the
     settled seeded pile): contact sets identical modulo ties; the FP32-PCG /
