@@ -57,7 +57,9 @@ def settled_state(sc, name, settle_steps=None, path=None):
     return st, "device-settled"
 
 
-def cpu_baseline(sc, state, budget_s, threads):
+def cpu_baseline(sc, state, budget_s, threads, first_gpu=None):
+    """CPU leg; with first_gpu (the device state one step after `state`) also
+    the precision statement of that step against the FP64 restatement."""
     from oracle import orc
     # same parameters as the device run (tolerances, PCG branch and its tolerance), so both do the same work
     o = orc.OracleScene(sc, sc.params)
@@ -66,15 +68,20 @@ def cpu_baseline(sc, state, budget_s, threads):
     t0 = time.perf_counter()
     n = 0
     contacts = 0
+    precision = None
     while True:
         st = o.step(1)
         n += 1
         contacts += st.n_contacts
         el = time.perf_counter() - t0
+        if n == 1 and first_gpu is not None:
+            t_pause = time.perf_counter()
+            precision = step_precision(sc, first_gpu, o.get_state(), state, threads)
+            t0 += time.perf_counter() - t_pause
         log("cpu baseline step %d: %d contacts, %.2fs" % (n, st.n_contacts, el))
         if el > budget_s or n >= 50:
             break
-    return {
+    return precision, {
         "value": n / el,
         "unit": "timesteps/s",
         "cores": threads,
@@ -101,6 +108,18 @@ def batched_leg(n_scenes, steps, plain_scenes, plain_steps, seed_base=1000):
     el = time.perf_counter() - t0
     contacts = [x.n_contacts for x in st]
     tail = 10
+    # the same tail of steps from the same state, with the default work
+    # shapes and with the shapes tuned per size class at this state
+    state = bs.ctx.get_state()
+    t0 = time.perf_counter()
+    for _ in range(tail):
+        bs.step(1)
+    el_default = time.perf_counter() - t0
+    bs.ctx.set_state(*state)
+    t0 = time.perf_counter()
+    shapes = bs.ctx.autotune(2)
+    el_tune = time.perf_counter() - t0
+    log("autotune (%.1fs): %s" % (el_tune, shapes.as_dict()))
     t0 = time.perf_counter()
     for _ in range(tail):
         bs.step(1)
@@ -118,6 +137,12 @@ def batched_leg(n_scenes, steps, plain_scenes, plain_steps, seed_base=1000):
         "steps": steps,
         "scene_steps_per_s": n_scenes * steps / el,
         "contacts_last_step": {"total": int(sum(contacts)), "max_per_scene": int(max(contacts))},
+        "autotune": {
+            "shapes": shapes.as_dict(),
+            "seconds": el_tune,
+            "default_shapes_scene_steps_per_s": n_scenes * tail / el_default,
+            "tuned_shapes_scene_steps_per_s": batched_tail,
+        },
         "contact_phase": {
             "batched_scene_steps_per_s": batched_tail,
             "plain_scene_steps_per_s": plain,
@@ -150,6 +175,43 @@ def served_leg(bs, rates, duration, plain_rate, seed=7):
     summ["dropped"] = len(reqs) - summ["requests"]
     log("plain %g req/s: %s" % (plain_rate, summ))
     out["plain_per_scene"] = summ
+    return out
+
+
+def _state_errors(sc, a, b):
+    dyn = sc.is_static == 0
+    size = np.max(sc.dims, axis=1)[dyn]
+    ep = np.linalg.norm(a[0][dyn] - b[0][dyn], axis=1) / size
+    ea = 2 * np.arccos(np.clip(np.abs(np.sum(a[1][dyn] * b[1][dyn], axis=1)), 0, 1))
+    gdt = float(np.linalg.norm(sc.params.gravity[:])) * sc.params.timestep
+    ev = np.linalg.norm(a[2][dyn] - b[2][dyn], axis=1) / gdt
+    return {"max_position_error_rel_body_size": float(ep.max()), "max_angle_error_rad": float(ea.max()),
+            "max_velocity_error_rel_g_dt": float(ev.max())}
+
+
+def step_precision(sc, g, o, state, threads):
+    """Errors of one device timestep (FP32 PCG inside the FP64 Newton loop)
+    against the FP64 CPU restatement from the same state: positions relative
+    to the body size, orientations in radians, velocities relative to |g| dt.
+    Next to them, the same errors between two runs of the restatement itself
+    that differ only in the inner solve's tolerance (the workload's and
+    1e-12): at the workload's Newton tolerance the step is defined only to
+    that level.  Per-contact forces are not compared: bodies resting on
+    several coplanar contacts have a nonunique force split (only the body
+    impulses are fixed)."""
+    from oracle import orc
+    q = type(sc.params).from_buffer_copy(sc.params)
+    q.pcg_tol = 1e-12
+    o2 = orc.OracleScene(sc, q)
+    o2.set_threads(threads)
+    o2.set_state(*state)
+    o2.step(1)
+    out = {"steps_from_identical_state": 1}
+    out.update(_state_errors(sc, g, o))
+    out["restatement_self_difference"] = _state_errors(sc, o, o2.get_state())
+    out["reference"] = ("FP64 CPU restatement, same parameters (Newton tolerance %g, inner PCG to %g), "
+                        "same settled state; self difference: the restatement with its inner PCG to 1e-12"
+                        % (sc.params.tolerance, sc.params.pcg_tol))
     return out
 
 
@@ -194,8 +256,11 @@ def main():
     ds = pdc.DynamicSystem(sc)
     ds.set_state(*state)
     log("warmup %d steps" % args.warmup)
-    for _ in range(args.warmup):
+    first_gpu = None
+    for k in range(args.warmup):
         st = ds.step(1)
+        if k == 0:
+            first_gpu = tuple(x.copy() for x in ds.state())
         log("warmup: %d contacts, %d Newton it, %d PCG it" % (st.n_contacts, st.newton_iterations,
                                                               st.pcg_iterations))
     ds.ctx.enable_kernel_timing(KT_NAMES[args.kernel])
@@ -259,7 +324,9 @@ def main():
         bs.ctx.close()
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(sc, state, args.cpu_budget, threads)
+        precision, out["cpu_baseline"] = cpu_baseline(sc, state, args.cpu_budget, threads, first_gpu)
+        if precision is not None:
+            out["precision"] = precision
     print(json.dumps(out), flush=True)
 
 

@@ -209,6 +209,29 @@ int pdc_set_scene_friction(pdc_context* ctx, const double* mu);
 int pdc_step_scenes(pdc_context* ctx, int32_t n_steps, int32_t n_selected,
                     const int32_t* scene_ids, pdc_solve_stats* stats);
 
+/* Work shapes, per scene size class (no counterpart in the reference, whose
+ * CPU loops have none).  A scene's class is its body count: < 64, < 256,
+ * < 1024, >= 1024.  contact_tile: contacts per workgroup of the per-contact
+ * kernels (64, 128 or 256); body_waves: bodies (and body pairs) per workgroup
+ * of the one-wavefront-per-body kernels (1, 2 or 4); pcg_chunk: launches per
+ * recorded PCG chunk (8, 16 or 32, for every class).  A scene's results
+ * depend on its own class's shapes only, so they stay the same alone and in
+ * any mix that uses the same table.  Defaults: 256, 4, 16. */
+typedef struct pdc_shapes {
+	int32_t contact_tile[4];
+	int32_t body_waves[4];
+	int32_t pcg_chunk;
+} pdc_shapes;
+int pdc_get_shapes(const pdc_context* ctx, pdc_shapes* out);
+int pdc_set_shapes(pdc_context* ctx, const pdc_shapes* shapes);
+/* Autotuner: for every size class present, times each candidate shape on
+ * the class's scenes alone (steps timesteps from the current state, which
+ * is restored afterwards) and keeps the fastest; then the PCG chunk length
+ * over all scenes.  The tuned table is kept only if all scenes together run
+ * faster with it than with the table tuning started from.  Returns the
+ * chosen table in *chosen (may be NULL). */
+int pdc_autotune(pdc_context* ctx, int32_t steps, pdc_shapes* chosen);
+
 int pdc_get_timings(const pdc_context* ctx, pdc_timings* out);
 void pdc_reset_timings(pdc_context* ctx);
 

@@ -466,6 +466,120 @@ int pdc_step_scenes(pdc_context* ctx, int32_t n_steps, int32_t n_selected, const
 	});
 }
 
+static void check_shapes(const pdc_shapes& sh) {
+	for (int c = 0; c < 4; ++c) {
+		const int ct = sh.contact_tile[c], bw = sh.body_waves[c];
+		if (ct != 64 && ct != 128 && ct != 256) throw ConfigError("contact_tile must be 64, 128 or 256");
+		if (bw != 1 && bw != 2 && bw != 4) throw ConfigError("body_waves must be 1, 2 or 4");
+	}
+	if (sh.pcg_chunk != 8 && sh.pcg_chunk != 16 && sh.pcg_chunk != 32) throw ConfigError("pcg_chunk must be 8, 16 or 32");
+}
+
+int pdc_get_shapes(const pdc_context* ctx, pdc_shapes* out) {
+	if (!ctx || !out) return PDC_ERR_INVALID_ARGUMENT;
+	*out = ctx->shapes;
+	return PDC_OK;
+}
+
+int pdc_set_shapes(pdc_context* ctx, const pdc_shapes* shapes) {
+	return guard(ctx, [&] {
+		if (!shapes) throw std::invalid_argument("pdc_set_shapes: null shapes");
+		check_shapes(*shapes);
+		ctx->shapes = *shapes;
+	});
+}
+
+// The autotuner: each trial restores the saved body state, steps the scenes
+// under test and is timed on the host (a timestep ends with a wait); the
+// solver's counters, timings and the PCG launch predictor are restored
+// afterwards, so tuning leaves no trace but the shape table.
+int pdc_autotune(pdc_context* ctx, int32_t steps, pdc_shapes* chosen) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		if (steps < 1) throw std::invalid_argument("pdc_autotune: steps must be >= 1");
+		const size_t n = (size_t)ctx->N;
+		hipStream_t s = ctx->stream;
+		pdc::DBuf<double> sp, sq, sv, sw;
+		sp.ensure(3 * n); sq.ensure(4 * n); sv.ensure(3 * n); sw.ensure(3 * n);
+		auto copy = [&](double* dst, const double* src, size_t k) {
+			PDC_HIP(hipMemcpyAsync(dst, src, k * sizeof(double), hipMemcpyDeviceToDevice, s));
+		};
+		copy(sp.p, ctx->b_pos.p, 3 * n); copy(sq.p, ctx->b_quat.p, 4 * n);
+		copy(sv.p, ctx->b_vel.p, 3 * n); copy(sw.p, ctx->b_ang.p, 3 * n);
+		const pdc_timings tm = ctx->timings;
+		const pdc_solve_stats last = ctx->last;
+		const int last_iters = ctx->pcg_last_iters;
+		const int kt_which = ctx->kt.which;
+		ctx->kt.which = PDC_KT_NONE;
+		auto restore = [&]() {
+			copy(ctx->b_pos.p, sp.p, 3 * n); copy(ctx->b_quat.p, sq.p, 4 * n);
+			copy(ctx->b_vel.p, sv.p, 3 * n); copy(ctx->b_ang.p, sw.p, 3 * n);
+			PDC_HIP(hipStreamSynchronize(s));
+			ctx->pcg_last_iters = last_iters;
+		};
+		// best of two runs: the first run after a shape change also pays for
+		// re-recording graphs and growing buffers
+		auto trial = [&](const std::vector<int>& ids) {
+			if (ids.empty()) pdc::set_stepped(ctx, 0, nullptr);
+			else pdc::set_stepped(ctx, (int)ids.size(), ids.data());
+			double tmin = 1e300;
+			for (int rep = 0; rep < 2; ++rep) {
+				restore();
+				const double t0 = now_s();
+				step_stepped_scenes(ctx, steps);
+				tmin = std::min(tmin, now_s() - t0);
+			}
+			return tmin;
+		};
+		// scenes per size class
+		std::vector<int> by_class[4];
+		for (int k = 0; k < ctx->S; ++k)
+			by_class[pdc::size_class(ctx->h_sbeg[k + 1] - ctx->h_sbeg[k])].push_back(k);
+		const pdc_shapes shapes0 = ctx->shapes;
+		pdc_shapes best = ctx->shapes;
+		const int tiles[3] = {64, 128, 256}, waves[3] = {1, 2, 4}, chunks[3] = {8, 16, 32};
+		for (int c = 0; c < 4; ++c) {
+			if (by_class[c].empty()) continue;
+			double tbest = 1e300;
+			int ct_best = best.contact_tile[c], bw_best = best.body_waves[c];
+			for (int ct : tiles)
+				for (int bw : waves) {
+					ctx->shapes = best;
+					ctx->shapes.contact_tile[c] = ct;
+					ctx->shapes.body_waves[c] = bw;
+					const double t = trial(by_class[c]);
+					if (t < tbest) { tbest = t; ct_best = ct; bw_best = bw; }
+				}
+			best.contact_tile[c] = ct_best;
+			best.body_waves[c] = bw_best;
+		}
+		double tbest = 1e300;
+		int ch_best = best.pcg_chunk;
+		for (int ch : chunks) {
+			ctx->shapes = best;
+			ctx->shapes.pcg_chunk = ch;
+			const double t = trial({});
+			if (t < tbest) { tbest = t; ch_best = ch; }
+		}
+		best.pcg_chunk = ch_best;
+		// the per-class choices are made on each class alone: keep the table
+		// only if all scenes together also run faster with it than with the
+		// table the tuning started from
+		auto timed = [&](const pdc_shapes& sh) {
+			ctx->shapes = sh;
+			return trial({});
+		};
+		if (timed(shapes0) <= timed(best)) best = shapes0;
+		ctx->shapes = best;
+		restore();
+		pdc::set_stepped(ctx, 0, nullptr);
+		ctx->timings = tm;
+		ctx->last = last;
+		ctx->kt.which = kt_which;
+		if (chosen) *chosen = best;
+	});
+}
+
 int pdc_get_timings(const pdc_context* ctx, pdc_timings* out) {
 	if (!ctx || !out) return PDC_ERR_INVALID_ARGUMENT;
 	*out = ctx->timings;

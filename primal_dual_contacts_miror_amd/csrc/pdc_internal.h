@@ -136,10 +136,14 @@ struct Scalars {
 	int pad[3];
 };
 
+// Size class of a scene of n bodies (index into pdc_shapes' per-class arrays).
+inline int size_class(int n) { return n < 64 ? 0 : n < 256 ? 1 : n < 1024 ? 2 : 3; }
+
 struct PcgGraph {
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	int nblk = 0;
+	int chunk = 0;
 	unsigned char args[512] = {};  // the kernel arguments the chunk was recorded with
 };
 
@@ -251,6 +255,7 @@ struct pdc_context {
 	pdc::DBuf<int> tb_c, tb_b, tb_w, tb_p;                    // per scene: first tile (S + 1)
 	int n_tc = 0, n_tb = 0, n_tw = 0, n_tp = 0, n_tr = 0;
 	int cap_tr = 0;                                           // PCG tiles launched (fixed per scene set)
+	pdc_shapes shapes{{256, 256, 256, 256}, {4, 4, 4, 4}, 16};  // work shapes per size class
 	pdc::DBuf<pdc::PcgTile> t_pcg;                            // PCG workgroup records (cap_tr)
 	// The PCG works in slot space: slot kPcgRowsPerWG * tile + k is block row
 	// k of PCG tile `tile` (slots past a tile's count are padding).  The

@@ -278,14 +278,16 @@ __global__ void k_scene_nnzb(int S, const int* __restrict__ lbeg, const int* __r
 	if (s == S) out[S] = aptr[lbeg[S]];
 }
 
-// Tiles of `per` consecutive elements inside each scene; padded with empty
-// tiles (scene -1) up to `pad_to`.
-static void make_tiles(const std::vector<int>& beg, int S, int per, std::vector<Tile>& t, std::vector<int>& tb,
+// Tiles of per(scene) consecutive elements inside each scene; padded with
+// empty tiles (scene -1) up to `pad_to`.
+template <class Per>
+static void make_tiles(const std::vector<int>& beg, int S, Per per_of, std::vector<Tile>& t, std::vector<int>& tb,
                        int pad_to = 0) {
 	t.clear();
 	tb.assign(S + 1, 0);
 	for (int sc = 0; sc < S; ++sc) {
 		tb[sc] = (int)t.size();
+		const int per = per_of(sc);
 		for (int e = beg[sc]; e < beg[sc + 1]; e += per) t.push_back(Tile{sc, e, std::min(per, beg[sc + 1] - e)});
 	}
 	tb[S] = (int)t.size();
@@ -333,15 +335,20 @@ static void scene_ranges(pdc_context* ctx) {
 	ctx->ell_R = std::max(8, (amax + 7) / 8 * 8);
 	std::vector<Tile> t;
 	std::vector<int> tb;
-	make_tiles(ctx->h_cbeg, S, kBlock, t, tb);
+	// work shapes of each scene's size class
+	auto cls = [&](int sc) { return size_class(ctx->h_sbeg[sc + 1] - ctx->h_sbeg[sc]); };
+	auto per_contact = [&](int sc) { return ctx->shapes.contact_tile[cls(sc)]; };
+	auto per_wave = [&](int sc) { return ctx->shapes.body_waves[cls(sc)]; };
+	auto fixed = [](int per) { return [per](int) { return per; }; };
+	make_tiles(ctx->h_cbeg, S, per_contact, t, tb);
 	ctx->n_tc = upload_tiles(ctx, t, tb, ctx->t_c, ctx->tb_c);
-	make_tiles(ctx->h_lbeg, S, kBlock, t, tb);
+	make_tiles(ctx->h_lbeg, S, fixed(kBlock), t, tb);
 	ctx->n_tb = upload_tiles(ctx, t, tb, ctx->t_b, ctx->tb_b);
-	make_tiles(ctx->h_lbeg, S, 4, t, tb);
+	make_tiles(ctx->h_lbeg, S, per_wave, t, tb);
 	ctx->n_tw = upload_tiles(ctx, t, tb, ctx->t_w, ctx->tb_w);
-	make_tiles(ctx->h_pbeg, S, 4, t, tb);
+	make_tiles(ctx->h_pbeg, S, per_wave, t, tb);
 	ctx->n_tp = upload_tiles(ctx, t, tb, ctx->t_p, ctx->tb_p);
-	make_tiles(ctx->h_lbeg, S, kPcgRowsPerWG, t, tb, ctx->cap_tr);
+	make_tiles(ctx->h_lbeg, S, fixed(kPcgRowsPerWG), t, tb, ctx->cap_tr);
 	if ((int)t.size() > ctx->cap_tr) throw HipError("PCG tile table exceeds its capacity", PDC_ERR_STATE);
 	ctx->tb_r.ensure(S + 1);
 	PDC_HIP(hipMemcpyAsync(ctx->tb_r.p, tb.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice, s));
@@ -2162,17 +2169,13 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	return a;
 }
 
-// One recorded chunk of kPcgChunk PCG iterations (launches alternating the
-// counter slots 0, 1).
-constexpr int kPcgChunk = 16;
-
-static_assert(kPcgChunk % 2 == 0, "iteration parity is the launch slot");
-
+// One recorded chunk of `chunk` PCG iterations (launches alternating the
+// counter slots 0, 1; the chunk length, pdc_shapes.pcg_chunk, is even).
 template <class T>
-static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk, bool two, hipEvent_t* ev) {
+static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk, bool two, int chunk, hipEvent_t* ev) {
 	hipStream_t s = ctx->stream;
 	const dim3 B(kPcgThreads);
-	for (int k = 0; k < kPcgChunk; ++k) {
+	for (int k = 0; k < chunk; ++k) {
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k], s));
 		if (two) hipLaunchKernelGGL(k_pcg2_a<T>, dim3(nblk), B, 0, s, pa, k & 1);
 		else switch (pa.R / 8) {
@@ -2247,17 +2250,19 @@ static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 	hipStream_t s = ctx->stream;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
 	// the chunk is recorded again when a buffer moved or the grid changed
+	const int chunk = ctx->shapes.pcg_chunk;
 	PcgGraph& G = ctx->pcg_graph[(sizeof(T) == 8 ? 1 : 0) + (two ? 2 : 0)];
 	static_assert(sizeof(PcgArgs<T>) <= sizeof(G.args), "recorded arguments fit");
-	if (ctx->use_graphs && (G.exec == nullptr || G.nblk != nblk ||
+	if (ctx->use_graphs && (G.exec == nullptr || G.nblk != nblk || G.chunk != chunk ||
 	                        std::memcmp(G.args, &pa, sizeof(pa)) != 0)) {
 		if (G.exec) { (void)hipGraphExecDestroy(G.exec); G.exec = nullptr; }
 		if (G.graph) { (void)hipGraphDestroy(G.graph); G.graph = nullptr; }
 		PDC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-		pcg_chunk_launches<T>(ctx, pa, nblk, two, nullptr);
+		pcg_chunk_launches<T>(ctx, pa, nblk, two, chunk, nullptr);
 		PDC_HIP(hipStreamEndCapture(s, &G.graph));
 		PDC_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
 		G.nblk = nblk;
+		G.chunk = chunk;
 		std::memcpy(G.args, &pa, sizeof(pa));
 	}
 	// Kernel timing of k_pcg_iter.  SPMV: launched directly between an event
@@ -2268,14 +2273,14 @@ static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 	const bool kt_spmv = ctx->kt.which == PDC_KT_SPMV;
 	const bool kt_iter = ctx->kt.which == PDC_KT_PCG;
 	const bool graph = ctx->use_graphs && !kt_spmv;
-	hipEvent_t* ev = kt_spmv ? ctx->kt_direct_ev(2 * kPcgChunk) : kt_iter ? ctx->kt_direct_ev(2) : nullptr;
+	hipEvent_t* ev = kt_spmv ? ctx->kt_direct_ev(2 * chunk) : kt_iter ? ctx->kt_direct_ev(2) : nullptr;
 	const int maxit = ctx->p.pcg_max_iter;
 	int launched = 0;
 	// The iteration count of a solve scatters widely around the previous
 	// one's: the first group covers 0.8 of it and every further group a
 	// quarter of what was launched, which balances the launches that find
 	// the solve converged (a few us each) against the host round trips.
-	auto chunks_for = [](double launches) { return std::max(1, (int)std::ceil(launches / kPcgChunk)); };
+	auto chunks_for = [&](double launches) { return std::max(1, (int)std::ceil(launches / chunk)); };
 	int want = chunks_for(0.8 * std::min(ctx->pcg_last_iters, maxit));
 	ctx->h_ps.resize(S);
 	double kt_ms = 0;
@@ -2286,8 +2291,8 @@ static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 		const int before = launched;
 		for (int c = 0; c < want && launched < maxit; ++c) {
 			if (graph) PDC_HIP(hipGraphLaunch(G.exec, s));
-			else pcg_chunk_launches<T>(ctx, pa, nblk, two, kt_spmv ? ev : nullptr);
-			launched += kPcgChunk;
+			else pcg_chunk_launches<T>(ctx, pa, nblk, two, chunk, kt_spmv ? ev : nullptr);
+			launched += chunk;
 		}
 		if (kt_iter) PDC_HIP(hipEventRecord(ev[1], s));
 		PDC_HIP(hipMemcpyAsync(ctx->h_ps.data(), ctx->d_ps.p, S * sizeof(PcgScene), hipMemcpyDeviceToHost, s));
@@ -2305,8 +2310,8 @@ static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 			int active = 0;
 			for (int sc = 0; sc < S; ++sc)
 				if (ctx->h_ps[sc].run)
-					active = std::max(active, ctx->h_ps[sc].done ? ctx->h_ps[sc].iters - before : kPcgChunk);
-			active = std::min(std::max(active, 0), kPcgChunk);
+					active = std::max(active, ctx->h_ps[sc].done ? ctx->h_ps[sc].iters - before : chunk);
+			active = std::min(std::max(active, 0), chunk);
 			for (int k = 0; k < active; ++k) {
 				float e = 0;
 				PDC_HIP(hipEventElapsedTime(&e, ev[2 * k], ev[2 * k + 1]));

@@ -72,6 +72,19 @@ class PdcTimings(ctypes.Structure):
     ]
 
 
+class PdcShapes(ctypes.Structure):
+    """Work shapes per scene size class (bodies < 64, < 256, < 1024, >= 1024)."""
+    _fields_ = [
+        ("contact_tile", ctypes.c_int32 * 4),
+        ("body_waves", ctypes.c_int32 * 4),
+        ("pcg_chunk", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {"contact_tile": list(self.contact_tile), "body_waves": list(self.body_waves),
+                "pcg_chunk": int(self.pcg_chunk)}
+
+
 def default_params(**overrides):
     """Reference code defaults, with keyword overrides (field names of pdc_params)."""
     p = PdcParams()

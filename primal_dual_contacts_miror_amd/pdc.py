@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-from .params import PdcParams, PdcSolveStats, PdcTimings
+from .params import PdcParams, PdcShapes, PdcSolveStats, PdcTimings
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libpdc.so")
@@ -37,6 +37,7 @@ EXPORTS = [
     "pdc_detect_contacts", "pdc_get_contacts", "pdc_fill_forces", "pdc_step",
     "pdc_get_timings", "pdc_reset_timings", "pdc_enable_kernel_timing", "pdc_kernel_timing",
     "pdc_set_scenes", "pdc_scene_count", "pdc_set_scene_friction", "pdc_step_scenes",
+    "pdc_get_shapes", "pdc_set_shapes", "pdc_autotune",
 ]
 
 
@@ -87,6 +88,9 @@ def load_library(path=LIB_PATH):
     L.pdc_scene_count.argtypes = [V]
     L.pdc_set_scene_friction.argtypes = [V, _D]
     L.pdc_step_scenes.argtypes = [V, ctypes.c_int32, ctypes.c_int32, _I, ctypes.POINTER(PdcSolveStats)]
+    L.pdc_get_shapes.argtypes = [V, ctypes.POINTER(PdcShapes)]
+    L.pdc_set_shapes.argtypes = [V, ctypes.POINTER(PdcShapes)]
+    L.pdc_autotune.argtypes = [V, ctypes.c_int32, ctypes.POINTER(PdcShapes)]
     L.pdc_get_timings.argtypes = [V, ctypes.POINTER(PdcTimings)]
     L.pdc_reset_timings.argtypes = [V]
     L.pdc_reset_timings.restype = None
@@ -212,6 +216,21 @@ class Context:
 
     def set_scene_friction(self, mu):
         self._check(self._L.pdc_set_scene_friction(self.h, _d(_f64(mu))))
+
+    def get_shapes(self):
+        sh = PdcShapes()
+        self._check(self._L.pdc_get_shapes(self.h, ctypes.byref(sh)))
+        return sh
+
+    def set_shapes(self, shapes):
+        self._check(self._L.pdc_set_shapes(self.h, ctypes.byref(shapes)))
+
+    def autotune(self, steps=2):
+        """Times the candidate work shapes per scene size class from the
+        current state (restored afterwards) and keeps the fastest."""
+        sh = PdcShapes()
+        self._check(self._L.pdc_autotune(self.h, int(steps), ctypes.byref(sh)))
+        return sh
 
     def step_scenes(self, n=1, scene_ids=None):
         """n timesteps of the given scenes (all when None); per-scene statistics

@@ -57,6 +57,16 @@ def test_no_device_no_fallback():
     assert L.pdc_last_error(None) == b"null context"
 
 
+def test_shapes_struct_and_null_context():
+    from primal_dual_contacts_miror_amd.params import PdcShapes
+    assert ctypes.sizeof(PdcShapes) == 9 * 4
+    L = pdc.load_library()
+    sh = PdcShapes()
+    assert L.pdc_get_shapes(None, ctypes.byref(sh)) == pdc.PDC_ERR_INVALID_ARGUMENT
+    assert L.pdc_set_shapes(None, ctypes.byref(sh)) == pdc.PDC_ERR_INVALID_ARGUMENT
+    assert L.pdc_autotune(None, 1, ctypes.byref(sh)) == pdc.PDC_ERR_INVALID_ARGUMENT
+
+
 def test_missing_library_fails_loudly(tmp_path):
     saved = pdc._lib
     try:

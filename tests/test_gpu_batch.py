@@ -132,3 +132,40 @@ def test_batch_against_oracle(require_gpu):
         print("scene %d: %d cubes, position error %.3e, angle error %.3e rad" % (k, sc.n - 1, ep, ea))
         assert ep < 1e-6 and ea < 1e-6
     bs.ctx.close()
+
+
+def test_shapes_per_size_class_and_autotune(require_gpu):
+    """Scenes of different size classes with non-default work shapes: the
+    batch still equals each scene alone under the same shape table; the
+    autotuner returns a valid table and leaves the state untouched."""
+    from primal_dual_contacts_miror_amd.params import PdcShapes
+    big = scenes.pile(90, seed=43, half=4.0, height=6.0, name="pile_90", grid_n=32)
+    big.params.gravity[1] = -40.0
+    scs = _piles([41, 42]) + [big]   # size classes 0 and 1
+    table = PdcShapes()
+    for c, (ct, bw) in enumerate(((64, 1), (128, 2), (256, 4), (256, 4))):
+        table.contact_tile[c] = ct
+        table.body_waves[c] = bw
+    table.pcg_chunk = 8
+    bs = pdc.BatchedSystem(scs)
+    bs.ctx.set_shapes(table)
+    for _ in range(40):
+        bs.step(1)
+    for k, sc in enumerate(scs):
+        ds = pdc.DynamicSystem(sc, sc.params)
+        ds.ctx.set_shapes(table)
+        for _ in range(40):
+            ds.step(1)
+        assert _same_state(bs.state(k), ds.state()), "scene %d differs under the shape table" % k
+        ds.ctx.close()
+    before = [bs.state(k) for k in range(len(scs))]
+    chosen = bs.ctx.autotune(1)
+    d = chosen.as_dict()
+    assert all(t in (64, 128, 256) for t in d["contact_tile"])
+    assert all(w in (1, 2, 4) for w in d["body_waves"])
+    assert d["pcg_chunk"] in (8, 16, 32)
+    assert bs.ctx.get_shapes().as_dict() == d
+    for k in range(len(scs)):
+        assert _same_state(bs.state(k), before[k])
+    bs.step(1)  # stepping continues with the chosen table
+    bs.ctx.close()
