@@ -1347,17 +1347,23 @@ constexpr int kPcgThreads = 64 * kPcgRowsPerWG;
 constexpr int kPartLoads = 16;              // partial values per lane and pass
 
 // Lane share of one scene's partial array (one value per tile of the previous
-// launch, W of them): the loads of a pass are unconditional (indices past W
-// clamped to the last value, which the sum then drops) so that all of them
-// are in flight together; one pass covers 1024 tiles.  The wavefront sum of
-// the returned values is the array total, in a fixed order.
+// launch, W of them): a pass issues the loads of every 64-value group that
+// holds values (a wavefront-uniform condition; lanes past W in the last group
+// read the last value, which the sum then drops), all in flight together;
+// one pass covers 1024 tiles.  Groups wholly past W are not read: every
+// workgroup reads the same array, and repeated reads of its last line would
+// all queue at one cache channel.  The wavefront sum of the returned values
+// is the array total, in a fixed order.
 __device__ __forceinline__ double partial_lane_sum(const double* __restrict__ src, int W) {
 	const int lane = threadIdx.x & 63;
 	double s = 0;
 	for (int base = 0; base < W; base += 64 * kPartLoads) {
 		double v[kPartLoads];
 #pragma unroll
-		for (int k = 0; k < kPartLoads; ++k) v[k] = src[min(base + lane + 64 * k, W - 1)];
+		for (int k = 0; k < kPartLoads; ++k) {
+			v[k] = 0;
+			if (base + 64 * k < W) v[k] = src[min(base + lane + 64 * k, W - 1)];
+		}
 #pragma unroll
 		for (int k = 0; k < kPartLoads; ++k) s += base + lane + 64 * k < W ? v[k] : 0.0;
 	}
@@ -1375,7 +1381,7 @@ __device__ __forceinline__ void reduce_partials(const double* p0, const double* 
 	const int wid = threadIdx.x >> 6;
 	if (wid < N) {
 		const double* p = wid == 0 ? p0 : wid == 1 ? p1 : wid == 2 ? p2 : p3;
-		const double t = wave_sum(partial_lane_sum(p + t0, W));
+		const double t = wave_sum_dpp(partial_lane_sum(p + t0, W));
 		if ((threadIdx.x & 63) == 0) lds[wid] = t;
 	}
 	__syncthreads();
@@ -1477,7 +1483,9 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 // from the columns; empty entries are not read).  EM is the ELL width in
 // groups of 8 entries (the loads of all of them are issued at once); EM = 0
 // reads the width from the arguments.  A tile whose scene has converged
-// returns after its record arrives.
+// returns after its record arrives.  Lane sums are DPP moves (every lane is
+// active where they are taken), and the closing workgroup sum has its own
+// LDS array, so that no wavefront writes what another may still read.
 template <class T, int EM>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
@@ -1513,6 +1521,8 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	const double zo = a.z[t], po = a.p[t], xo = a.x[t];
 	const int scn = rec[0], cnt = rec[2], t0 = rec[3], W = rec[4], done = rec[5], it = rec[6];
 	if (scn < 0 || done) return;
+	// the scene's gamma and alpha of the previous iteration
+	const double gprev = a.hist[4 * scn + 2 * nxt], aprev = a.hist[4 * scn + 2 * nxt + 1];
 	// round 2: the partial sums of the previous launch (issued now, summed
 	// after the gathers are in flight; a scene of more than 1024 tiles sums
 	// the rest afterwards), the operator blocks and the neighbour values of
@@ -1522,7 +1532,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	double pv[kPartLoads];
 	if (wid < 4) {
 #pragma unroll
-		for (int k = 0; k < kPartLoads; ++k) pv[k] = psrc[min(lane + 64 * k, W - 1)];
+		for (int k = 0; k < kPartLoads; ++k) {
+			pv[k] = 0;
+			if (64 * k < W) pv[k] = psrc[min(lane + 64 * k, W - 1)];
+		}
 	}
 	double n = 0;
 	if (i < 6) {
@@ -1551,9 +1564,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 				n += d;
 			}
 	}
-	n += __shfl_xor(n, 4, 64);
-	n += __shfl_xor(n, 2, 64);
-	n += __shfl_xor(n, 1, 64);
+	n = group8_sum(n);
 	// m = M w, u = M r (not stored), M s of the old s
 	double m = 0, u = 0, ms = 0;
 #pragma unroll
@@ -1567,7 +1578,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 #pragma unroll
 		for (int k = 0; k < kPartLoads; ++k) part += lane + 64 * k < W ? pv[k] : 0.0;
 		if (W > 64 * kPartLoads) part += partial_lane_sum(psrc + 64 * kPartLoads, W - 64 * kPartLoads);
-		part = wave_sum(part);
+		part = wave_sum_dpp(part);
 		if (lane == 0) lds[wid] = part;
 	}
 	__syncthreads();
@@ -1580,11 +1591,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 		}
 		return;
 	}
-	const double gprev = a.hist[4 * scn + 2 * nxt], aprev = a.hist[4 * scn + 2 * nxt + 1];
 	const double beta = gamma / gprev;
 	const double alpha = gamma / (delta - beta * gamma / aprev);
 	double* __restrict__ wn = pick(a.w, nxt);
-	double v[4] = {0, 0, 0, 0};
+	double v[3] = {0, 0, 0};
 	if (wid < cnt && i < 6 && sl == 0) {
 		const double wt = PDC_PICK6(wr, i), rt0 = PDC_PICK6(rrow, i), st0 = PDC_PICK6(srow, i);
 		const double zt = n + beta * zo;
@@ -1604,8 +1614,9 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 		v[1] += wnt * ut;
 		v[2] += rt * rt;
 	}
-	double out[4];
-	block_sum<4, double>(v, lds, out);
+	__shared__ double lds_b[3 * kPcgRowsPerWG];
+	double out[3];
+	block_sum_last<3, kPcgRowsPerWG>(v, lds_b, out);
 	if (threadIdx.x == 0) {
 		pick(a.gam, nxt)[blockIdx.x] = out[0];
 		pick(a.del, nxt)[blockIdx.x] = out[1];
