@@ -201,6 +201,7 @@ void pdc_destroy(pdc_context* ctx) {
 	for (auto e : ctx->kt.pool) (void)hipEventDestroy(e);
 	if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
 	if (ctx->h_ss) (void)hipHostFree(ctx->h_ss);
+	if (ctx->h_prog) (void)hipHostFree(ctx->h_prog);
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 	delete ctx;
 }

@@ -17,6 +17,8 @@
 // inside b gives normal -grad(sd_b); a sample of b inside a gives +grad(sd_a).
 #include <cmath>
 
+#include <cstring>
+
 #include "pdc_internal.h"
 #include "pdc_reduce.h"
 
@@ -190,6 +192,18 @@ void set_scenes(pdc_context* ctx, int S, const int* sbeg) {
 	ctx->h_sctl.assign(S, SceneCtl{});
 	ctx->d_ps.ensure(S);
 	PDC_HIP(hipMemsetAsync(ctx->d_ps.p, 0, S * sizeof(PcgScene), s));
+	if ((size_t)S > ctx->h_prog_cap) {
+		// no kernel of an earlier step writes it any more: every solve ends with a wait
+		PDC_HIP(hipStreamSynchronize(s));
+		if (ctx->h_prog) PDC_HIP(hipHostFree(ctx->h_prog));
+		ctx->h_prog = nullptr;
+		ctx->d_prog = nullptr;
+		ctx->h_prog_cap = 0;
+		PDC_HIP(hipHostMalloc((void**)&ctx->h_prog, S * sizeof(PcgProgress), hipHostMallocMapped | hipHostMallocCoherent));
+		PDC_HIP(hipHostGetDevicePointer((void**)&ctx->d_prog, ctx->h_prog, 0));
+		ctx->h_prog_cap = S;
+	}
+	std::memset(ctx->h_prog, 0, S * sizeof(PcgProgress));
 	if ((size_t)S > ctx->h_ss_cap) {
 		if (ctx->h_ss) PDC_HIP(hipHostFree(ctx->h_ss));
 		ctx->h_ss = nullptr;

@@ -126,6 +126,19 @@ struct PcgScene {
 	int pad;
 };
 
+// Progress of a scene's Newton loop and PCG, written by the kernels into
+// pinned host memory while the stream runs and read by the host without
+// waiting for the stream (the host sizes the PCG launches it queues from
+// it).  Each 64-bit field is written with one store, so the host never sees
+// half of one.
+struct PcgProgress {
+	long long runtag;   // (Newton iteration << 32) | run, written by k_err_final
+	long long outcome;  // (1 << 32) | iterations once the PCG converged; 0 before (reset by the host)
+	double rel2;        // |r|^2 / |b|^2 at the start of the latest active iteration
+	int it;             // PCG iterations started in the current solve
+	int pad;
+};
+
 // Detection counters (one device struct).
 struct Scalars {
 	int overflow;        // capacity overflow flag set by kernels
@@ -154,6 +167,7 @@ struct KernelTimer {
 	double bytes = 0;              // algorithmic bytes of the timed launches
 	int64_t pending_contact_slots = 0;
 	std::vector<hipEvent_t> pool;  // start/stop pairs awaiting read-back
+	std::vector<int> pair_launches;  // launches each pair spans (one per used pair)
 	size_t used = 0;
 };
 
@@ -273,6 +287,9 @@ struct pdc_context {
 	std::vector<pdc::SceneCtl> h_sctl;
 	pdc::DBuf<pdc::PcgScene> d_ps;                            // per-scene PCG control
 	std::vector<pdc::PcgScene> h_ps;
+	pdc::PcgProgress* h_prog = nullptr;                       // pinned, written by the kernels (S)
+	pdc::PcgProgress* d_prog = nullptr;                       // its device address
+	size_t h_prog_cap = 0;
 	std::vector<pdc_solve_stats> scene_stats;                 // per scene, last solve
 
 	// ---- statistics ----
@@ -292,6 +309,8 @@ void kt_begin(pdc_context* ctx, int which, hipEvent_t* ev);
 void kt_end(pdc_context* ctx, int which, hipEvent_t ev0);
 void kt_collect(pdc_context* ctx);
 void kt_end_nocollect(pdc_context* ctx, hipEvent_t ev0);
+// kt_end for a pair of events around `launches` launches of the timed kernel
+void kt_end_span(pdc_context* ctx, hipEvent_t ev0, int launches);
 
 // pdc_contacts.hip
 void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,

@@ -464,6 +464,7 @@ struct SArgs {
 	SceneSc* ss;
 	const SceneCtl* ctl;
 	PcgScene* ps;
+	PcgProgress* prog;    // pinned host memory (see PcgProgress)
 	// contacts
 	const int* li0;
 	const int* li1;
@@ -920,6 +921,7 @@ __global__ __launch_bounds__(256) void k_err_final(SArgs a, double tol) {
 		S->ls_state = LS_PENDING;
 		S->ls_step = 0;
 		a.ps[sc].run = S->run;
+		a.prog[sc].runtag = ((long long)S->it << 32) | (long long)(unsigned)S->run;
 	}
 }
 
@@ -1303,6 +1305,7 @@ struct PcgArgs {
 	PcgTile* tiles;         // one record per workgroup (kPcgRowsPerWG block rows)
 	const SceneSc* ss;      // per scene: `run`, the scene is still in its Newton loop
 	PcgScene* ps;           // per scene: done flag, iteration counters
+	PcgProgress* prog;      // per scene, pinned host memory: iterations started, |r|^2/|b|^2, outcome
 	double tol2;            // squared relative tolerance
 	int R;                  // ELL entries per slot (a multiple of 8)
 };
@@ -1587,7 +1590,11 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	if (!(rr > a.tol2 * bb)) {
 		if (threadIdx.x == 0) {
 			Tp->done = 1;
-			if (first_tile) { a.ps[scn].done = 1; a.ps[scn].iters = it; }
+			if (first_tile) {
+				a.ps[scn].done = 1;
+				a.ps[scn].iters = it;
+				a.prog[scn].outcome = (1LL << 32) | (long long)(unsigned)it;
+			}
 		}
 		return;
 	}
@@ -1625,6 +1632,8 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 		if (first_tile) {
 			a.hist[4 * scn + 2 * cur] = gamma;
 			a.hist[4 * scn + 2 * cur + 1] = alpha;
+			a.prog[scn].rel2 = rr / bb;
+			a.prog[scn].it = it + 1;
 		}
 	}
 }
@@ -1687,9 +1696,17 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) 
 	if (!(red[0] > a.tol2 * red[1])) {
 		if (threadIdx.x == 0) {
 			Tp->done = 1;
-			if ((int)blockIdx.x == t0) { a.ps[scn].done = 1; a.ps[scn].iters = it; }
+			if ((int)blockIdx.x == t0) {
+				a.ps[scn].done = 1;
+				a.ps[scn].iters = it;
+				a.prog[scn].outcome = (1LL << 32) | (long long)(unsigned)it;
+			}
 		}
 		return;
+	}
+	if (threadIdx.x == 0 && (int)blockIdx.x == t0) {
+		a.prog[scn].rel2 = red[0] / red[1];
+		a.prog[scn].it = it + 1;
 	}
 	const double beta = it > 0 ? red[2] / red[3] : 0.0;
 	double* __restrict__ y = a.p;
@@ -2163,7 +2180,7 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	a.tc = ctx->t_c.p; a.tb = ctx->t_b.p; a.tw = ctx->t_w.p; a.tp = ctx->t_p.p;
 	a.tbc = ctx->tb_c.p; a.tbb = ctx->tb_b.p; a.tbw = ctx->tb_w.p;
 	a.cbeg = ctx->r_cbeg.p; a.lbeg = ctx->r_lbeg.p;
-	a.ss = ctx->d_ss.p; a.ctl = ctx->d_sctl.p; a.ps = ctx->d_ps.p;
+	a.ss = ctx->d_ss.p; a.ctl = ctx->d_sctl.p; a.ps = ctx->d_ps.p; a.prog = ctx->d_prog;
 	a.li0 = ctx->li0.p; a.li1 = ctx->li1.p; a.T = ctx->c_T.p; a.depth = ctx->c_depth.p;
 	a.pstart = ctx->ap_start.p; a.pa = ctx->ap_a.p; a.g2l = ctx->g2l.p;
 	a.lstat = ctx->l_static.p; a.l2g = ctx->l2g.p;
@@ -2235,6 +2252,7 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 	pa.tiles = ctx->t_pcg.p;
 	pa.ss = ctx->d_ss.p;
 	pa.ps = ctx->d_ps.p;
+	pa.prog = ctx->d_prog;
 	pa.tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
 	return pa;
 }
@@ -2251,16 +2269,33 @@ static void pcg_begin(pdc_context* ctx, const PcgArgs<T>& pa) {
 	PDC_CHECK_LAUNCH();
 }
 
-// The PCG iterations of every scene still in its Newton loop, after
-// pcg_begin; returns whether any scene is (the scenes' run flags, set by
-// k_err_final, arrive with the PCG done flags).  The host waits here.
+// Algorithmic bytes of the scenes' active PCG iterations for the kernel
+// timing hooks: per iteration of a scene (of k_pcg_iter, or of k_pcg2_a for
+// the two-reduction form), each input read once and each output written
+// once, FP64 vectors:
+//   k_pcg_iter  B blocks and column indices nnzb (36 T + 4), row pointers
+//               4 (NL + 1), P blocks 36 T NL, w, z, s, p, x, r read
+//               6 x 8 n6, z, s, p, x, r, w' written 6 x 8 n6
+//   k_pcg2_a    B, column indices, row pointers as above, P 36 T NL,
+//               r, y, q read 3 x 8 n6, y, q written 2 x 8 n6
 template <class T>
-static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
-	const int S = ctx->S;
+static void pcg_account_bytes(pdc_context* ctx, const std::vector<int>& iters, bool two) {
+	const double Tb = sizeof(T);
+	for (int sc = 0; sc < ctx->S; ++sc) {
+		if (iters[sc] <= 0) continue;
+		const double NLs = ctx->h_lbeg[sc + 1] - ctx->h_lbeg[sc], n6 = 6.0 * NLs, nnzb = ctx->h_nnzb[sc];
+		const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NLs + 1) + 36.0 * Tb * NLs;
+		ctx->kt.bytes += iters[sc] * (mat + (two ? 40.0 : 96.0) * n6);
+	}
+}
+
+// The recorded chunk of PCG launches for these arguments (recorded again
+// when a buffer moved, the grid changed or the chunk length changed).
+template <class T>
+static PcgGraph& pcg_graph(pdc_context* ctx, const PcgArgs<T>& pa) {
 	const int nblk = std::max(1, ctx->cap_tr);
 	hipStream_t s = ctx->stream;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
-	// the chunk is recorded again when a buffer moved or the grid changed
 	const int chunk = ctx->shapes.pcg_chunk;
 	PcgGraph& G = ctx->pcg_graph[(sizeof(T) == 8 ? 1 : 0) + (two ? 2 : 0)];
 	static_assert(sizeof(PcgArgs<T>) <= sizeof(G.args), "recorded arguments fit");
@@ -2276,6 +2311,22 @@ static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 		G.chunk = chunk;
 		std::memcpy(G.args, &pa, sizeof(pa));
 	}
+	return G;
+}
+
+// The PCG iterations of every scene still in its Newton loop, after
+// pcg_begin; returns whether any scene is (the scenes' run flags, set by
+// k_err_final, arrive with the PCG done flags).  The host waits for the
+// stream after every group of chunks.  Used with the kernel timing hooks
+// and without graphs; pcg_solve_async below otherwise.
+template <class T>
+static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
+	const int S = ctx->S;
+	const int nblk = std::max(1, ctx->cap_tr);
+	hipStream_t s = ctx->stream;
+	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
+	const int chunk = ctx->shapes.pcg_chunk;
+	PcgGraph& G = pcg_graph<T>(ctx, pa);
 	// Kernel timing of k_pcg_iter.  SPMV: launched directly between an event
 	// pair per launch (events cannot be timed inside a replayed graph), only
 	// launches of active iterations counted.  PCG: an event pair around every
@@ -2344,24 +2395,132 @@ static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 	}
 	if (any) ctx->pcg_last_iters = last;
 	if (kt_spmv || kt_iter) {
-		// Algorithmic bytes of one active iteration of a scene (of k_pcg_iter,
-		// or of k_pcg2_a for the two-reduction form), each input read once and
-		// each output written once, FP64 vectors:
-		//   k_pcg_iter  B blocks and column indices nnzb (36 T + 4), row pointers
-		//               4 (NL + 1), P blocks 36 T NL, w, z, s, p, x, r read
-		//               6 x 8 n6, z, s, p, x, r, w' written 6 x 8 n6
-		//   k_pcg2_a    B, column indices, row pointers as above, P 36 T NL,
-		//               r, y, q read 3 x 8 n6, y, q written 2 x 8 n6
-		const double Tb = sizeof(T);
 		ctx->kt.total_ms += kt_ms;
-		for (int sc = 0; sc < S; ++sc) {
-			if (!ctx->h_ps[sc].run) continue;
-			const double NLs = ctx->h_lbeg[sc + 1] - ctx->h_lbeg[sc], n6 = 6.0 * NLs, nnzb = ctx->h_nnzb[sc];
-			const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NLs + 1) + 36.0 * Tb * NLs;
-			ctx->kt.bytes += iters[sc] * (mat + (two ? 40.0 : 96.0) * n6);
-		}
+		for (int sc = 0; sc < S; ++sc)
+			if (!ctx->h_ps[sc].run) iters[sc] = 0;
+		pcg_account_bytes<T>(ctx, iters, two);
 	}
 	return any;
+}
+
+// The PCG of one Newton iteration without waiting for the stream.  The
+// kernels write each scene's progress into pinned host memory (PcgProgress:
+// its Newton state after k_err_final, the iterations started, |r|^2/|b|^2,
+// the outcome), and the host keeps recorded chunks queued a little ahead of
+// the device's position: up to two chunks, and no more than the iterations
+// the slowest solving scene still needs (the previous solve's count at
+// first, then the residual's decay since the solve's first reading,
+// extrapolated).  The queue does not drain while a scene is solving, so
+// there is no round trip between groups, and once every running scene has
+// converged the caller queues the next phase at once, behind the few queued
+// launches that find the solve converged.  `running` (per scene, in/out):
+// the scenes still in their Newton loop; a scene leaves when its Newton
+// state for iteration `iter` says so.  Returns whether any scene still runs.
+//
+// Termination does not depend on the progress records arriving: when the
+// stream is idle and a scene has not reported an outcome, another chunk is
+// queued, and at the iteration cap the host waits for the stream, after
+// which every record is final.
+template <class T>
+static bool pcg_solve_async(pdc_context* ctx, const PcgArgs<T>& pa, int iter, std::vector<char>& running) {
+	const int S = ctx->S;
+	hipStream_t s = ctx->stream;
+	PcgGraph& G = pcg_graph<T>(ctx, pa);
+	const int chunk = G.chunk;
+	const int maxit = ctx->p.pcg_max_iter;
+	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
+	const bool kt_iter = ctx->kt.which == PDC_KT_PCG;
+	const int ahead_max = 2 * chunk, ahead_min = 4;
+	volatile PcgProgress* P = ctx->h_prog;
+	int launched = 0;
+	auto enqueue_to = [&](int target) {
+		target = std::min(target, maxit);
+		while (launched < target) {
+			PDC_HIP(hipGraphLaunch(G.exec, s));
+			launched += chunk;
+		}
+	};
+	// kernel timing: one event pair around all the solve's launches (the
+	// events add no gap between them)
+	hipEvent_t kt_e0 = nullptr;
+	if (kt_iter) kt_begin(ctx, PDC_KT_PCG, &kt_e0);
+	const int last = std::min(ctx->pcg_last_iters, maxit);
+	enqueue_to(std::max(ahead_min, std::min(last, ahead_max)));
+	std::vector<int> iters(S, -1), a_it(S, -1);
+	std::vector<double> a_rel(S, 0.0);
+	int last_pos = -1, idle_polls = 0;
+	bool synced = false;
+	for (;;) {
+		bool pending = false, any = false;
+		int pos = 0, total = 0;
+		for (int sc = 0; sc < S; ++sc) {
+			if (!running[sc]) continue;
+			const long long tag = P[sc].runtag;
+			const bool fresh = (int)(tag >> 32) == iter;
+			if (fresh && !(tag & 1)) {
+				running[sc] = 0;  // left the Newton loop at this iteration's exit test
+				continue;
+			}
+			any = true;
+			const long long oc = P[sc].outcome;
+			if (oc >> 32) {
+				iters[sc] = (int)(oc & 0xffffffffLL);
+				continue;
+			}
+			pending = true;
+			const int it = P[sc].it;
+			const double rel = P[sc].rel2;
+			pos = std::max(pos, it);
+			// iterations this solve needs in all: the previous solve's count
+			// until the residual's decay can be extrapolated
+			int need = std::max(last, it + ahead_min);
+			if (it > 0 && a_it[sc] < 0) { a_it[sc] = it; a_rel[sc] = rel; }
+			if (a_it[sc] >= 0 && it >= a_it[sc] + 8 && rel > 0 && rel < a_rel[sc]) {
+				const double rate = std::log(rel / a_rel[sc]) / (it - a_it[sc]);
+				const double goal = std::log(std::max(ctx->p.pcg_tol * ctx->p.pcg_tol, 1e-300) / rel);
+				need = it + (goal >= 0 ? 1 : (int)std::ceil(1.05 * goal / rate)) + 2;
+			}
+			total = std::max(total, need);
+		}
+		if (!any) {
+			if (kt_iter) kt_end_span(ctx, kt_e0, launched);
+			return false;
+		}
+		if (!pending) break;
+		if (launched >= maxit) {
+			if (synced) break;  // every record is final: scenes without an outcome reached the cap
+			PDC_HIP(hipStreamSynchronize(s));
+			synced = true;
+			continue;
+		}
+		// queue up to ahead_max launches past the device, but not past the
+		// estimated end (at least ahead_min past the device)
+		const int target = std::max(pos + ahead_min, std::min(pos + ahead_max, total));
+		if (launched < target) enqueue_to(target);
+		// the device's position has not moved for a while: queue more if the stream has drained
+		if (pos == last_pos) {
+			if (++idle_polls >= 256) {
+				idle_polls = 0;
+				if (hipStreamQuery(s) == hipSuccess) enqueue_to(launched + chunk);
+			}
+		} else {
+			last_pos = pos;
+			idle_polls = 0;
+		}
+		__builtin_ia32_pause();
+	}
+	int longest = 0;
+	for (int sc = 0; sc < S; ++sc) {
+		if (!running[sc]) { iters[sc] = 0; continue; }
+		if (iters[sc] < 0) iters[sc] = maxit;
+		longest = std::max(longest, iters[sc]);
+	}
+	ctx->pcg_last_iters = longest;
+	if (kt_iter) {
+		kt_end_span(ctx, kt_e0, launched);
+		pcg_account_bytes<T>(ctx, iters, two);
+	}
+	return true;
 }
 
 // Algorithmic bytes (each input read once, each output written once).
@@ -2558,11 +2717,25 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		record_phase(ctx, ctx->phase_graph[0], phase_a);
 		record_phase(ctx, ctx->phase_graph[1], phase_b);
 	}
+	// With graphs the host never waits for the stream in the loop
+	// (pcg_solve_async); without graphs, or with the per-launch timing of the
+	// PCG (which needs direct launches), it waits for each group of PCG
+	// chunks (pcg_solve).
+	const bool async = phase_graphs && (kw == PDC_KT_NONE || kw == PDC_KT_PCG);
+	std::vector<char> running(in);
 	for (int iter = 0; iter < ctx->p.iterations; ++iter) {
+		if (async)
+			for (int sc = 0; sc < S; ++sc)
+				if (running[sc]) {
+					ctx->h_prog[sc].outcome = 0;
+					ctx->h_prog[sc].it = 0;
+				}
 		if (phase_graphs) PDC_HIP(hipGraphLaunch(ctx->phase_graph[0].exec, s));
 		else phase_a();
-		// the one wait of the iteration
-		const bool any = fp32 ? pcg_solve<float>(ctx, pa32) : pcg_solve<double>(ctx, pa64);
+		bool any;
+		if (async) any = fp32 ? pcg_solve_async<float>(ctx, pa32, iter, running)
+		                      : pcg_solve_async<double>(ctx, pa64, iter, running);
+		else any = fp32 ? pcg_solve<float>(ctx, pa32) : pcg_solve<double>(ctx, pa64);
 		if (!any) break;  // every scene converged or reached the cap
 		if (phase_graphs) PDC_HIP(hipGraphLaunch(ctx->phase_graph[1].exec, s));
 		else phase_b();

@@ -80,6 +80,16 @@ void kt_end(pdc_context* ctx, int which, hipEvent_t ev0) {
 	hipEvent_t e1 = kt.pool[kt.used + 1];
 	PDC_HIP(hipEventRecord(e1, ctx->stream));
 	kt.used += 2;
+	kt.pair_launches.push_back(1);
+	if (kt.used + 2 > 1024) kt_collect(ctx);
+}
+
+void kt_end_span(pdc_context* ctx, hipEvent_t ev0, int launches) {
+	if (!ev0) return;
+	auto& kt = ctx->kt;
+	PDC_HIP(hipEventRecord(kt.pool[kt.used + 1], ctx->stream));
+	kt.used += 2;
+	kt.pair_launches.push_back(launches);
 	if (kt.used + 2 > 1024) kt_collect(ctx);
 }
 
@@ -88,6 +98,7 @@ void kt_end_nocollect(pdc_context* ctx, hipEvent_t ev0) {
 	auto& kt = ctx->kt;
 	PDC_HIP(hipEventRecord(kt.pool[kt.used + 1], ctx->stream));
 	kt.used += 2;
+	kt.pair_launches.push_back(1);
 }
 
 void kt_collect(pdc_context* ctx) {
@@ -98,9 +109,10 @@ void kt_collect(pdc_context* ctx) {
 		float ms = 0;
 		PDC_HIP(hipEventElapsedTime(&ms, kt.pool[k], kt.pool[k + 1]));
 		kt.total_ms += ms;
-		kt.launches += 1;
+		kt.launches += k / 2 < kt.pair_launches.size() ? kt.pair_launches[k / 2] : 1;
 	}
 	kt.used = 0;
+	kt.pair_launches.clear();
 }
 
 }  // namespace pdc
