@@ -61,6 +61,32 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
 	return (lane_value(0) + lane_value(16)) + (lane_value(32) + lane_value(48));
 }
 
+constexpr int kPartLoads = 16;              // partial values per lane and pass
+
+// Lane share of a partial-sum array of W values (one per workgroup tile of
+// a scene): a pass issues the loads of every 64-value group that
+// holds values (a wavefront-uniform condition; lanes past W in the last group
+// read the last value, which the sum then drops), all in flight together;
+// one pass covers 1024 tiles.  Groups wholly past W are not read: every
+// workgroup reads the same array, and repeated reads of its last line would
+// all queue at one cache channel.  The wavefront sum of the returned values
+// is the array total, in a fixed order.
+__device__ __forceinline__ double partial_lane_sum(const double* __restrict__ src, int W) {
+	const int lane = threadIdx.x & 63;
+	double s = 0;
+	for (int base = 0; base < W; base += 64 * kPartLoads) {
+		double v[kPartLoads];
+#pragma unroll
+		for (int k = 0; k < kPartLoads; ++k) {
+			v[k] = 0;
+			if (base + 64 * k < W) v[k] = src[min(base + lane + 64 * k, W - 1)];
+		}
+#pragma unroll
+		for (int k = 0; k < kPartLoads; ++k) s += base + lane + 64 * k < W ? v[k] : 0.0;
+	}
+	return s;
+}
+
 // Sums K values per thread over the workgroup; thread 0 receives the totals
 // in out[] (other threads get unspecified values).  blockDim.x must be a
 // multiple of 64 and at most 1024.  `lds` holds K * 16 values.

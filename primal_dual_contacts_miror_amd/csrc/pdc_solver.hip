@@ -24,7 +24,11 @@ namespace pdc {
 
 constexpr int kMaxWG = 1024;  // workgroups of the reducing 1-D kernels
 constexpr int kTrials = 10;   // line-search trials per gradient-descent pass (the most per pass)
-constexpr int kGroup = 5;     // line-search trials per pass in the Newton direction
+// Line-search trials per pass in the Newton direction (10 in all): a search
+// usually decides within its first two trials (about 2 per Newton iteration
+// on the settled pile), so the first pass is short and the later ones, which
+// return at once for a decided search, take the rest.
+constexpr int kNewtonPass[3] = {2, 3, 5};
 enum {
 	P_GAP = 0, P_WRF, P_RL, P_AC, P_RU, P_ND_TRACE, P_ND_CNT,
 	P_LS,                                  // 7 slots per trial: wrf2 rl2 ac2 gf gl ru2 gu
@@ -540,9 +544,17 @@ __device__ __forceinline__ void write_partials(double (&v)[K], double* part, int
 
 // Sum of partial slot `slot` over tiles [t0, t1) by the first wavefront, in a
 // fixed order; every thread of the workgroup gets it (`tmp`: an LDS scalar).
+// The wavefront's loads are all in flight together (partial_lane_sum).
 __device__ __forceinline__ double tile_sum(const double* part, int stride, int slot, int t0, int t1,
                                            double* tmp) {
-	return block_reduce_array(part + slot * (size_t)stride + t0, t1 - t0, tmp);
+	if (threadIdx.x < 64) {
+		const double v = wave_sum_dpp(partial_lane_sum(part + slot * (size_t)stride + t0, t1 - t0));
+		if (threadIdx.x == 0) *tmp = v;
+	}
+	__syncthreads();
+	const double r = *tmp;
+	__syncthreads();
+	return r;
 }
 
 // The workgroup's tile and its scene; the workgroup returns at once when the
@@ -890,14 +902,21 @@ __global__ __launch_bounds__(256) void k_body_res(SArgs a) {
 
 // err, eps and the exit test (PrimalDualForceSolver.cpp:295-320); one workgroup per scene.
 __global__ __launch_bounds__(256) void k_err_final(SArgs a, double tol) {
-	__shared__ double slot;
+	__shared__ double sums[4];
 	const int sc = blockIdx.x;
 	if (!a.ctl[sc].act || !a.ss[sc].run) return;
 	const int c0 = a.tbc[sc], c1 = a.tbc[sc + 1], w0 = a.tbw[sc], w1 = a.tbw[sc + 1];
-	const double wrf2 = tile_sum(a.part, a.pstride, P_WRF, c0, c1, &slot);
-	const double rl2 = tile_sum(a.part, a.pstride, P_RL, c0, c1, &slot);
-	const double ac2 = tile_sum(a.part, a.pstride, P_AC, c0, c1, &slot);
-	const double ru2 = tile_sum(a.part, a.pstride, P_RU, w0, w1, &slot);
+	// |wrf|^2, |rl|^2, |ac|^2 over the contact tiles and |ru|^2 over the body
+	// tiles, one wavefront each, all loads in flight together
+	{
+		const int wid = threadIdx.x >> 6;
+		const int slot = wid == 0 ? P_WRF : wid == 1 ? P_RL : wid == 2 ? P_AC : P_RU;
+		const int t0 = wid < 3 ? c0 : w0, t1 = wid < 3 ? c1 : w1;
+		const double v = wave_sum_dpp(partial_lane_sum(a.part + slot * (size_t)a.pstride + t0, t1 - t0));
+		if ((threadIdx.x & 63) == 0) sums[wid] = v;
+	}
+	__syncthreads();
+	const double wrf2 = sums[0], rl2 = sums[1], ac2 = sums[2], ru2 = sums[3];
 	if (threadIdx.x == 0) {
 		const double nC = a.cbeg[sc + 1] - a.cbeg[sc];
 		const double nR = 6.0 * (a.lbeg[sc + 1] - a.lbeg[sc]);
@@ -1347,32 +1366,6 @@ __device__ __forceinline__ double spmv_ell(const PcgArgs<T>& a, const V* __restr
 }
 
 constexpr int kPcgThreads = 64 * kPcgRowsPerWG;
-constexpr int kPartLoads = 16;              // partial values per lane and pass
-
-// Lane share of one scene's partial array (one value per tile of the previous
-// launch, W of them): a pass issues the loads of every 64-value group that
-// holds values (a wavefront-uniform condition; lanes past W in the last group
-// read the last value, which the sum then drops), all in flight together;
-// one pass covers 1024 tiles.  Groups wholly past W are not read: every
-// workgroup reads the same array, and repeated reads of its last line would
-// all queue at one cache channel.  The wavefront sum of the returned values
-// is the array total, in a fixed order.
-__device__ __forceinline__ double partial_lane_sum(const double* __restrict__ src, int W) {
-	const int lane = threadIdx.x & 63;
-	double s = 0;
-	for (int base = 0; base < W; base += 64 * kPartLoads) {
-		double v[kPartLoads];
-#pragma unroll
-		for (int k = 0; k < kPartLoads; ++k) {
-			v[k] = 0;
-			if (base + 64 * k < W) v[k] = src[min(base + lane + 64 * k, W - 1)];
-		}
-#pragma unroll
-		for (int k = 0; k < kPartLoads; ++k) s += base + lane + 64 * k < W ? v[k] : 0.0;
-	}
-	return s;
-}
-
 // Totals of N <= 4 partial arrays p0..p{N-1} over the scene's tiles
 // [t0, t0 + W): wavefront k sums array k; every thread gets all totals.
 // (Separate pointer arguments, not an array of pointers: a runtime-indexed
@@ -1833,8 +1826,9 @@ __global__ __launch_bounds__(256) void k_neg(SArgs a, const double* __restrict__
 
 // ---- (e) line search as fused reductions (PrimalDualForceSolver.cpp:562-635) --
 // The trials of one search share the direction, so they are independent:
-// one pass evaluates nt step sizes 2^-k0 ... 2^-(k0+nt-1) at once (kGroup
-// in the Newton direction, kTrials in the gradient-descent one) and
+// one pass evaluates nt step sizes 2^-k0 ... 2^-(k0+nt-1) at once (passes
+// of kNewtonPass in the Newton direction, kTrials in the gradient-descent
+// one) and
 // k_ls_final applies the reference's sequential acceptance rule to them,
 // which selects exactly the step the one-trial-at-a-time loop would.
 
@@ -1980,9 +1974,7 @@ __global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int k0, int nt, int 
 		const int slot = P_LS + q;
 		const int c = q - 7 * (q / 7);
 		const int t0 = c < 5 ? a.tbc[sc] : a.tbw[sc], t1 = c < 5 ? a.tbc[sc + 1] : a.tbw[sc + 1];
-		double v = 0;
-		for (int w = t0 + lane; w < t1; w += 64) v += a.part[slot * (size_t)a.pstride + w];
-		v = wave_sum(v);
+		const double v = wave_sum_dpp(partial_lane_sum(a.part + slot * (size_t)a.pstride + t0, t1 - t0));
 		if (lane == 0) red[q] = v;
 	}
 	__syncthreads();
@@ -2679,18 +2671,19 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	auto phase_b = [&]() {
 		launch(k_du, ntb, kBlock, a, (const double*)ctx->x64.p);
 		launch(k_increments, ntc, kBlock, a);
-		// line search: two passes of kGroup trials in the Newton direction,
+		// line search: passes of 2, 3 and 5 trials in the Newton direction,
 		// then the gradient-descent fallback (PrimalDualForceSolver.cpp:441-467)
 		// in two passes of kTrials; each pass runs for the scenes still searching
 		hipEvent_t e1;
-		for (int k0 = 0; k0 < 10; k0 += kGroup) {
+		for (int pass = 0, k0 = 0; pass < 3; k0 += kNewtonPass[pass], ++pass) {
+			const int nt = kNewtonPass[pass];
 			kt_begin(ctx, PDC_KT_LINE_SEARCH, &e1);
-			launch_trials(k_ls_contact, ntc, kGroup, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
+			launch_trials(k_ls_contact, ntc, nt, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
 			              (int)LS_PENDING);
 			kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
-			launch_trials(k_ls_body, ntw, kGroup, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
+			launch_trials(k_ls_body, ntw, nt, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
 			              (int)LS_PENDING);
-			launch(k_ls_final, S, 1024, a, k0, kGroup, 10, 0);
+			launch(k_ls_final, S, 1024, a, k0, nt, 10, 0);
 		}
 		launch(k_neg, ntb, kBlock, a, (const double*)ctx->l_ru.p, ctx->l_gdu.p);
 		for (int k0 = 0; k0 < 20; k0 += kTrials) {
