@@ -2245,6 +2245,7 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 	pa.ss = ctx->d_ss.p;
 	pa.ps = ctx->d_ps.p;
 	pa.prog = ctx->d_prog;
+	if (!pa.prog) throw HipError("PCG progress record not allocated (no scene table)", PDC_ERR_STATE);
 	pa.tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
 	return pa;
 }
