@@ -87,9 +87,21 @@ __device__ __forceinline__ double partial_lane_sum(const double* __restrict__ sr
 	return s;
 }
 
+// The wavefront sum of a block sum: DPP for double (every lane active),
+// shuffles otherwise.
+template <class T>
+__device__ __forceinline__ T wave_sum_full(T v) {
+	return wave_sum(v);
+}
+template <>
+__device__ __forceinline__ double wave_sum_full<double>(double v) {
+	return wave_sum_dpp(v);
+}
+
 // Sums K values per thread over the workgroup; thread 0 receives the totals
 // in out[] (other threads get unspecified values).  blockDim.x must be a
-// multiple of 64 and at most 1024.  `lds` holds K * 16 values.
+// multiple of 64 and at most 1024, and every lane of a calling wavefront
+// active.  `lds` holds K * 16 values.
 template <int K, class T>
 __device__ __forceinline__ void block_sum(T (&v)[K], T* lds, T (&out)[K]) {
 	const int lane = threadIdx.x & 63;
@@ -97,7 +109,7 @@ __device__ __forceinline__ void block_sum(T (&v)[K], T* lds, T (&out)[K]) {
 	const int nw = blockDim.x >> 6;
 #pragma unroll
 	for (int k = 0; k < K; ++k) {
-		T s = wave_sum(v[k]);
+		T s = wave_sum_full(v[k]);
 		if (lane == 0) lds[k * 16 + wid] = s;
 	}
 	__syncthreads();

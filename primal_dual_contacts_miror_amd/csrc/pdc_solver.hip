@@ -882,7 +882,7 @@ __global__ __launch_bounds__(256) void k_body_res(SArgs a) {
 				for (int row = 0; row < 6; ++row)
 					add[row] += Tc(a, i, j, row) * f0 + Tc(a, i, j, 6 + row) * f1 + Tc(a, i, j, 12 + row) * f2;
 			}
-			for (int q = 0; q < 6; ++q) add[q] = wave_sum(add[q]);
+			for (int q = 0; q < 6; ++q) add[q] = wave_sum_dpp(add[q]);
 			for (int q = 0; q < 3; ++q) ru[q] = M[0] * d[q];
 			double t[3];
 			sym3_mv(M + 1, d + 3, t);
@@ -1055,8 +1055,8 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 				for (int col = row; col < 6; ++col) U[u++] += a0 * tj0[col] + at0 * tj1[col] + at1 * tj2[col];
 			}
 		}
-		for (int q = 0; q < 21; ++q) U[q] = wave_sum(U[q]);
-		for (int q = 0; q < 6; ++q) y[q] = wave_sum(y[q]);
+		for (int q = 0; q < 21; ++q) U[q] = wave_sum_dpp(U[q]);
+		for (int q = 0; q < 6; ++q) y[q] = wave_sum_dpp(y[q]);
 	}
 	// the block B (every lane)
 	double S[36];
@@ -1240,7 +1240,7 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ 
 		}
 #pragma unroll
 		for (int q = 0; q < 36; ++q) {
-			const double t = wave_sum(B[q]);
+			const double t = wave_sum_dpp(B[q]);
 			sum = q == lane ? t : sum;
 		}
 	}
@@ -1939,7 +1939,7 @@ __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restri
 				add[row] += Tc(a, i, j, row) * fI[0] + Tc(a, i, j, 6 + row) * fI[1] + Tc(a, i, j, 12 + row) * fI[2];
 		}
 #pragma unroll
-		for (int q = 0; q < 6; ++q) add[q] = wave_sum(add[q]);
+		for (int q = 0; q < 6; ++q) add[q] = wave_sum_dpp(add[q]);
 		if (lane == 0) {
 			double ru[6];
 			for (int q = 0; q < 3; ++q) ru[q] = M[0] * d[q];
