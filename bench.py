@@ -231,7 +231,7 @@ def measured_traffic(kernel, workload):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pile_2000")
     ap.add_argument("--kernel", default="pcg", choices=sorted(KT_NAMES))

@@ -1561,6 +1561,9 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 			}
 	}
 	n = group8_sum(n);
+	// the previous iteration's scalars as reciprocals, formed while the
+	// partial sums are still on their way (one division after the barrier)
+	const double inv_gprev = 1.0 / gprev, inv_aprev = 1.0 / aprev;
 	// m = M w, u = M r (not stored), M s of the old s
 	double m = 0, u = 0, ms = 0;
 #pragma unroll
@@ -1591,8 +1594,8 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 		}
 		return;
 	}
-	const double beta = gamma / gprev;
-	const double alpha = gamma / (delta - beta * gamma / aprev);
+	const double beta = gamma * inv_gprev;
+	const double alpha = gamma / (delta - beta * gamma * inv_aprev);
 	double* __restrict__ wn = pick(a.w, nxt);
 	double v[3] = {0, 0, 0};
 	if (wid < cnt && i < 6 && sl == 0) {

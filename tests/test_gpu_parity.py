@@ -268,30 +268,47 @@ def test_step_is_deterministic(require_gpu):
         np.testing.assert_array_equal(a, b)
 
 
-def test_graph_replay_equals_direct_launches(require_gpu, monkeypatch):
+@pytest.mark.parametrize("variant", ["fp32", "cap", "fp64"])
+def test_graph_replay_equals_direct_launches(require_gpu, monkeypatch, variant):
     """The recorded graphs (the PCG chunks and the two phases of a Newton
-    iteration, replayed every iteration and recorded again every step) give
-    the same bits as launching every kernel directly, over steps whose
-    contact sets and sizes change, with host reads between the replays."""
+    iteration, replayed every iteration and recorded again every step) and
+    the host loop that queues PCG chunks from the kernels' progress records
+    without waiting for the stream give the same bits as launching every
+    kernel directly and waiting after each group, over steps whose contact
+    sets and sizes change.  Variants: the FP32 PCG; the FP32 PCG capped at 6
+    iterations (every solve ends at the cap, where the host loop waits for
+    the stream); the FP64 two-reduction PCG."""
     sc = scenes.pile(60, seed=4, half=2.5, height=8.0, name="pile_60", grid_n=48, sample_spacing=0.1)
     o = _advance_oracle(sc, 150)
     state = o.get_state()
+    q = type(sc.params).from_buffer_copy(sc.params)
+    steps = 12
+    if variant == "cap":
+        q.pcg_max_iter = 6
+        steps = 6
+    elif variant == "fp64":
+        q.pcg_precision = PCG_FP64
+        q.pcg_tol = 1e-12
     res = []
     for direct in (False, True):
         if direct:
             monkeypatch.setenv("PDC_NO_GRAPHS", "1")
         else:
             monkeypatch.delenv("PDC_NO_GRAPHS", raising=False)
-        ds = _gpu_scene(sc)
+        ds = _gpu_scene(sc, q)
         ds.set_state(*state)
         traj = []
-        for _ in range(12):
+        hits = 0
+        for _ in range(steps):
             st = ds.step(1)
+            hits += st.pcg_maxiter_hits
             traj.append((st.n_contacts, st.newton_iterations, st.pcg_iterations) + tuple(x.copy() for x in ds.state()))
-        res.append(traj)
+        res.append((traj, hits))
         ds.ctx.close()
-    assert sum(t[1] for t in res[0]) > 12, "the steps should iterate"
-    for a, b in zip(res[0], res[1]):
+    assert sum(t[1] for t in res[0][0]) > steps, "the steps should iterate"
+    if variant == "cap":
+        assert res[0][1] > 0 and res[0][1] == res[1][1], "solves should reach the cap"
+    for a, b in zip(res[0][0], res[1][0]):
         assert a[:3] == b[:3]
         for x, y in zip(a[3:], b[3:]):
             np.testing.assert_array_equal(x, y)
