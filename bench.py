@@ -245,7 +245,7 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--served-rates", default="200,1000,5000")
+    ap.add_argument("--served-rates", default="100,200,1000,5000")
     ap.add_argument("--served-duration", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1:
