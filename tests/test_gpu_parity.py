@@ -254,6 +254,36 @@ def test_trajectory_pyramid_fp32_inner(require_gpu):
     assert ep < 1e-6 and ea < 1e-6
 
 
+@pytest.mark.parametrize("name", ["stack_10", "pyramid_210"])
+def test_config_full_length_trajectory_fp64(require_gpu, name):
+    """The plain FP64 configurations at their full lengths: 10 stacked cubes
+    for 200 steps of 0.01 and the 210-box pyramid for 500 steps of 0.005.
+    Every step has the oracle's contact count; at the end positions agree
+    within 1e-6 of the body size and orientations within 1e-6 rad (FP64 PCG
+    to 1e-13 against the oracle's direct solve, Newton tolerance 1e-10)."""
+    sc = getattr(scenes, name)(tolerance=1e-10)
+    p = sc.params
+    p.pcg_precision = PCG_FP64
+    p.pcg_tol = 1e-13
+    o = orc.OracleScene(sc)
+    o.set_threads(16)
+    o.set_linear_solver(orc.SOLVER_DIRECT)
+    ds = _gpu_scene(sc, p)
+    worst = (0.0, 0.0)
+    for k in range(sc.steps):
+        so = o.step(1)
+        sg = ds.step(1)
+        assert so.n_contacts == sg.n_contacts, k
+        if (k + 1) % 50 == 0:
+            e = _traj_errors(sc, o.get_state(), ds.state())
+            worst = (max(worst[0], e[0]), max(worst[1], e[1]))
+            print("%s step %d: position err %.3e, angle err %.3e" % (name, k + 1, e[0], e[1]), flush=True)
+    ep, ea = _traj_errors(sc, o.get_state(), ds.state())
+    print("%s %d steps: position err %.3e (worst checkpoint %.3e), angle err %.3e (worst %.3e)"
+          % (name, sc.steps, ep, worst[0], ea, worst[1]))
+    assert ep < 1e-6 and ea < 1e-6
+
+
 def test_step_is_deterministic(require_gpu):
     sc = scenes.pile(60, seed=3, half=2.5, height=8.0, name="pile_60", grid_n=48, sample_spacing=0.1)
     o = _advance_oracle(sc, 200)
