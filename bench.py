@@ -155,12 +155,34 @@ def batched_leg(n_scenes, steps, plain_scenes, plain_steps, seed_base=1000):
     return bs, out
 
 
-def served_leg(bs, rates, duration, plain_rate, seed=7):
-    """Continuous-batching server over the batched leg's resident scenes
-    (their contact-rich state): Poisson request streams of one-step requests."""
-    srv = serve.SceneServer.over(bs)
-    n = len(bs.scenes)
-    out = {"resident_scenes": n, "duration_s": duration, "rates": []}
+def served_leg(n_scenes, settle_steps, rates, duration, plain_rate, seed_base=5000, seed=7):
+    """Continuous-batching server over resident scenes of its own: seeded
+    random piles of 10-1000 bodies (log-uniform), admitted by HBM footprint,
+    stepped together from their seeded drop for `settle_steps` steps so that
+    the requests meet contact-rich states; then Poisson request streams of
+    one-step requests at each rate, and the plain per-scene server (one
+    request per step call) at `plain_rate`."""
+    t0 = time.perf_counter()
+    scs = [scenes.random_pile(seed_base + k, n_min=10, n_max=1000) for k in range(n_scenes)]
+    srv = serve.SceneServer(scs, hbm_budget=200 * 2 ** 30)
+    n = len(srv.scenes)
+    bodies = sum(sc.n for sc in srv.scenes)
+    log("served: %d scenes built and resident (%d rejected), %d bodies, %.1f GiB, %.1fs" % (
+        n, len(srv.rejected), bodies, srv.footprint / 2 ** 30, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    st = []
+    for k in range(settle_steps):
+        st = srv.system.step(1)
+        if k % 50 == 0:
+            log("served settle step %d: %d contacts" % (k, sum(x.n_contacts for x in st)))
+    settle_s = time.perf_counter() - t0
+    out = {"workload": "%d seeded random piles of 10-1000 bodies (log-uniform), spheres + boxes, 32^3 grids, "
+                       "FP32 PCG / FP64 Newton, admitted by HBM footprint, %d steps from the seeded drop before "
+                       "the requests" % (n_scenes, settle_steps),
+           "resident_scenes": n, "rejected_scenes": len(srv.rejected), "bodies": bodies,
+           "footprint_gib": srv.footprint / 2 ** 30,
+           "contacts_resident": int(sum(x.n_contacts for x in st)) if st else 0,
+           "settle_s": settle_s, "duration_s": duration, "rates": []}
     for rate in rates:
         reqs = serve.poisson_requests(rate, duration, n, seed=seed)
         served = srv.run(reqs, batched=True, time_limit=duration * 4)
@@ -176,6 +198,7 @@ def served_leg(bs, rates, duration, plain_rate, seed=7):
     summ["dropped"] = len(reqs) - summ["requests"]
     log("plain %g req/s: %s" % (plain_rate, summ))
     out["plain_per_scene"] = summ
+    srv.close()
     return out
 
 
@@ -247,6 +270,8 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-rates", default="100,200,1000,5000")
     ap.add_argument("--served-duration", type=float, default=2.0)
+    ap.add_argument("--served-scenes", type=int, default=1024)
+    ap.add_argument("--served-settle", type=int, default=150)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -319,10 +344,11 @@ def main():
     ds.ctx.close()
     if not args.no_batched:
         bs, out["batched"] = batched_leg(args.batched_scenes, args.batched_steps, 16, 10)
+        bs.ctx.close()
         if not args.no_served:
             rates = [float(r) for r in args.served_rates.split(",") if r]
-            out["served"] = served_leg(bs, rates, args.served_duration, plain_rate=40.0)
-        bs.ctx.close()
+            out["served"] = served_leg(args.served_scenes, args.served_settle, rates, args.served_duration,
+                                       plain_rate=40.0)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         precision, out["cpu_baseline"] = cpu_baseline(sc, state, args.cpu_budget, threads, first_gpu)
