@@ -116,9 +116,9 @@ struct SceneCtl {
 	double mu;   // the scene's friction coefficient
 };
 
-// Per-scene PCG outcome for the host (written by the scene's first tile) and
-// `run`, the scene's Newton state, mirrored here for the host's one read per
-// Newton iteration.
+// Per-scene PCG outcome (written by the scene's first tile; read on the
+// device by k_newton_end, and by the host after each group of PCG chunks
+// when it waits for the stream) and `run`, the scene's Newton state.
 struct PcgScene {
 	int iters;   // iterations of the last solve
 	int done;    // converged flag

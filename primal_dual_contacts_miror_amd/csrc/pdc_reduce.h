@@ -161,20 +161,4 @@ __device__ __forceinline__ void block_sum_last(double (&v)[K], double* lds, doub
 	}
 }
 
-// Sums part[0..n) in a fixed order by the first wave; all threads of the
-// workgroup get the result (through `slot`, an LDS scalar).
-template <class T>
-__device__ __forceinline__ T block_reduce_array(const T* part, int n, T* slot) {
-	if (threadIdx.x < 64) {
-		T s = 0;
-		for (int i = threadIdx.x; i < n; i += 64) s += part[i];
-		s = wave_sum(s);
-		if (threadIdx.x == 0) *slot = s;
-	}
-	__syncthreads();
-	T r = *slot;
-	__syncthreads();
-	return r;
-}
-
 }  // namespace pdc

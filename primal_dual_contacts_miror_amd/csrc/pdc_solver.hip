@@ -2635,8 +2635,12 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	// fixed sequence of launches per iteration: each scene's decisions (exit
 	// test, line-search acceptance, gradient-descent fallback, Hu schedule,
 	// iteration cap) are taken on the device, and the launches of a scene that
-	// is in another state return at once.  The host waits once per iteration,
-	// for the PCG, and learns there which scenes are still iterating.
+	// is in another state return at once.  The host learns which scenes are
+	// still iterating, and when their PCG has converged, from the progress
+	// records the kernels write into pinned host memory (pcg_solve_async),
+	// without waiting for the stream; without graphs or with per-launch PCG
+	// timing it waits for the stream once per group of PCG chunks
+	// (pcg_solve).
 	const double* du = ctx->l_du.p;
 	const double* gdu = ctx->l_gdu.p;
 	// phase A: errors, assembly and the PCG start; phase B: the increments,
