@@ -23,12 +23,13 @@
 namespace pdc {
 
 constexpr int kMaxWG = 1024;  // workgroups of the reducing 1-D kernels
-constexpr int kTrials = 10;   // line-search trials per gradient-descent pass (the most per pass)
+constexpr int kTrials = 20;   // line-search trials of the gradient-descent search, one pass (the most per pass)
 // Line-search trials per pass in the Newton direction (10 in all): a search
 // usually decides within its first two trials (about 2 per Newton iteration
-// on the settled pile), so the first pass is short and the later ones, which
-// return at once for a decided search, take the rest.
-constexpr int kNewtonPass[3] = {2, 3, 5};
+// on the settled pile), so the first pass is short and the second, which
+// returns at once for a decided search, takes the rest.
+constexpr int kNewtonPasses = 2;
+constexpr int kNewtonPass[kNewtonPasses] = {2, 8};
 enum {
 	P_GAP = 0, P_WRF, P_RL, P_AC, P_RU, P_ND_TRACE, P_ND_CNT,
 	P_LS,                                  // 7 slots per trial: wrf2 rl2 ac2 gf gl ru2 gu
@@ -1780,13 +1781,19 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) 
 	}
 }
 
-__global__ __launch_bounds__(256) void k_du(SArgs a, const double* __restrict__ x) {
+// du from the PCG solution (undoing the Jacobi scaling), and the
+// gradient-descent direction -ru of the fallback search
+// (PrimalDualForceSolver.cpp:451), used only if that search runs.
+__global__ __launch_bounds__(256) void k_du(SArgs a, const double* __restrict__ x, double* __restrict__ gdu) {
 	PDC_TILE_RUN(a.tb);
 	(void)scn;
 	if ((int)threadIdx.x < tile.count) {
 		const int r = tile.first + threadIdx.x;
 		const int sr = a.slot[r];
-		for (int k = 0; k < 6; ++k) a.du[6 * r + k] = x[6 * sr + k] * a.D[6 * r + k];
+		for (int k = 0; k < 6; ++k) {
+			a.du[6 * r + k] = x[6 * sr + k] * a.D[6 * r + k];
+			gdu[6 * r + k] = -a.ru[6 * r + k];
+		}
 	}
 }
 
@@ -1818,22 +1825,13 @@ __global__ __launch_bounds__(256) void k_increments(SArgs a) {
 	}
 }
 
-__global__ __launch_bounds__(256) void k_neg(SArgs a, const double* __restrict__ x, double* __restrict__ y) {
-	PDC_TILE_RUN(a.tb);
-	if (a.ss[scn].ls_state != LS_GD_PENDING) return;
-	if ((int)threadIdx.x < tile.count) {
-		const int r = tile.first + threadIdx.x;
-		for (int k = 0; k < 6; ++k) y[6 * r + k] = -x[6 * r + k];
-	}
-}
-
 // ---- (e) line search as fused reductions (PrimalDualForceSolver.cpp:562-635) --
 // The trials of one search share the direction, so they are independent:
 // one pass evaluates nt step sizes 2^-k0 ... 2^-(k0+nt-1) at once (passes
-// of kNewtonPass in the Newton direction, kTrials in the gradient-descent
-// one) and
-// k_ls_final applies the reference's sequential acceptance rule to them,
-// which selects exactly the step the one-trial-at-a-time loop would.
+// of kNewtonPass trials in the Newton direction, all kTrials at once in the
+// gradient-descent one) and k_ls_final applies the reference's sequential
+// acceptance rule to them, which selects exactly the step the
+// one-trial-at-a-time loop would.
 
 // Filtered trial point of contact i at `step` (NonSmoothContactForce.cpp:281-319,
 // OnlyNormalForce.cpp:172-201): f' = f + 0.99 df_in, lambda' = lambda + 0.99 dl_in.
@@ -2677,13 +2675,14 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		else pcg_begin<double>(ctx, pa64);
 	};
 	auto phase_b = [&]() {
-		launch(k_du, ntb, kBlock, a, (const double*)ctx->x64.p);
+		launch(k_du, ntb, kBlock, a, (const double*)ctx->x64.p, ctx->l_gdu.p);
 		launch(k_increments, ntc, kBlock, a);
-		// line search: passes of 2, 3 and 5 trials in the Newton direction,
-		// then the gradient-descent fallback (PrimalDualForceSolver.cpp:441-467)
-		// in two passes of kTrials; each pass runs for the scenes still searching
+		// line search: passes of 2 and 8 trials in the Newton direction, then
+		// the gradient-descent fallback (PrimalDualForceSolver.cpp:441-467) in
+		// one pass of its 20 trials; each pass runs for the scenes still
+		// searching (a pass of a decided search returns at once)
 		hipEvent_t e1;
-		for (int pass = 0, k0 = 0; pass < 3; k0 += kNewtonPass[pass], ++pass) {
+		for (int pass = 0, k0 = 0; pass < kNewtonPasses; k0 += kNewtonPass[pass], ++pass) {
 			const int nt = kNewtonPass[pass];
 			kt_begin(ctx, PDC_KT_LINE_SEARCH, &e1);
 			launch_trials(k_ls_contact, ntc, nt, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
@@ -2693,14 +2692,11 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 			              (int)LS_PENDING);
 			launch(k_ls_final, S, 1024, a, k0, nt, 10, 0);
 		}
-		launch(k_neg, ntb, kBlock, a, (const double*)ctx->l_ru.p, ctx->l_gdu.p);
-		for (int k0 = 0; k0 < 20; k0 += kTrials) {
-			launch_trials(k_ls_contact, ntc, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p,
-			              k0, (int)LS_GD_PENDING);
-			launch_trials(k_ls_body, ntw, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, k0,
-			              (int)LS_GD_PENDING);
-			launch(k_ls_final, S, 1024, a, k0, kTrials, 20, 1);
-		}
+		launch_trials(k_ls_contact, ntc, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, 0,
+		              (int)LS_GD_PENDING);
+		launch_trials(k_ls_body, ntw, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, 0,
+		              (int)LS_GD_PENDING);
+		launch(k_ls_final, S, 1024, a, 0, kTrials, 20, 1);
 		launch(k_ls_commit, ntc, kBlock, a, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
 		       (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p);
 		launch(k_commit_v, ntb, kBlock, a, gdu);
