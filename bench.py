@@ -70,8 +70,11 @@ def cpu_baseline(sc, state, budget_s, threads, first_gpu=None):
     n = 0
     contacts = 0
     precision = None
+    step_ms = []
     while True:
+        ts = time.perf_counter()
         st = o.step(1)
+        step_ms.append(1e3 * (time.perf_counter() - ts))
         n += 1
         contacts += st.n_contacts
         el = time.perf_counter() - t0
@@ -90,7 +93,62 @@ def cpu_baseline(sc, state, budget_s, threads, first_gpu=None):
         "sample": "%d timestep(s) of the FP64 CPU restatement (OpenMP) from the same settled state, same "
                   "parameters (%d contacts/step avg; block-Jacobi PCG branch, pcg_tol %g)" % (
                       n, contacts // n, sc.params.pcg_tol),
+        "median_step_ms": float(np.median(step_ms)),
+        "p99_step_ms": float(np.percentile(step_ms, 99)),
+        "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()},
     }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def measured_copy_rate():
+    """Device-to-device copy rate of 1 GiB on this GPU (read + write bytes per
+    second), the achievable counterpart of the 8 TB/s HBM figure, through the
+    HIP runtime the library itself is linked against."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    V = ctypes.c_void_p
+    hip.hipMalloc.argtypes = [ctypes.POINTER(V), ctypes.c_size_t]
+    hip.hipFree.argtypes = [V]
+    hip.hipMemset.argtypes = [V, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpyAsync.argtypes = [V, V, ctypes.c_size_t, ctypes.c_int, V]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(V)]
+    hip.hipEventRecord.argtypes = [V, V]
+    hip.hipEventSynchronize.argtypes = [V]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), V, V]
+    hip.hipEventDestroy.argtypes = [V]
+    n = 1 << 30
+    a, b, e0, e1 = V(), V(), V(), V()
+    if hip.hipMalloc(ctypes.byref(a), n) or hip.hipMalloc(ctypes.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        hip.hipMemset(a, 0, n)
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
+        best = 0.0
+        for _ in range(6):
+            hip.hipEventRecord(e0, None)
+            hip.hipMemcpyAsync(b, a, n, 3, None)  # device to device
+            hip.hipEventRecord(e1, None)
+            hip.hipEventSynchronize(e1)
+            ms = ctypes.c_float(0)
+            hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+            if ms.value > 0:
+                best = max(best, 2.0 * n / (ms.value * 1e-3))
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return best
 
 
 def batched_leg(n_scenes, steps, plain_scenes, plain_steps, seed_base=1000):
@@ -268,7 +326,7 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--served-rates", default="100,200,1000,5000")
+    ap.add_argument("--served-rates", default="100,200,500,1000,2000,5000")
     ap.add_argument("--served-duration", type=float, default=2.0)
     ap.add_argument("--served-scenes", type=int, default=1024)
     ap.add_argument("--served-settle", type=int, default=150)
@@ -341,6 +399,10 @@ def main():
             "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
         },
     }
+    try:
+        out["roofline"]["measured_copy_GBps"] = measured_copy_rate() / 1e9
+    except Exception as exc:  # the copy is a reference figure only
+        log("copy rate not measured: %s" % exc)
     ds.ctx.close()
     if not args.no_batched:
         bs, out["batched"] = batched_leg(args.batched_scenes, args.batched_steps, 16, 10)
