@@ -1328,6 +1328,7 @@ struct PcgArgs {
 	PcgProgress* prog;      // per scene, pinned host memory: iterations started, |r|^2/|b|^2, outcome
 	double tol2;            // squared relative tolerance
 	int R;                  // ELL entries per slot (a multiple of 8)
+	int maxit;              // iteration cap of a solve
 };
 
 // Element k (0 or 1) of a two-slot kernel argument, selected without a runtime
@@ -1517,7 +1518,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	const int t = 6 * br + ir;
 	const double zo = a.z[t], po = a.p[t], xo = a.x[t];
 	const int scn = rec[0], cnt = rec[2], t0 = rec[3], W = rec[4], done = rec[5], it = rec[6];
-	if (scn < 0 || done) return;
+	if (scn < 0 || done || it >= a.maxit) return;
 	// the scene's gamma and alpha of the previous iteration
 	const double gprev = a.hist[4 * scn + 2 * nxt], aprev = a.hist[4 * scn + 2 * nxt + 1];
 	// round 2: the partial sums of the previous launch (issued now, summed
@@ -1685,7 +1686,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) 
 	(void)slot;
 	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
 	const int scn = Tp->scene;
-	if (scn < 0 || Tp->done) return;
+	if (scn < 0 || Tp->done || Tp->it >= a.maxit) return;
 	const int it = Tp->it, t0 = Tp->t0, W = Tp->W, cnt = Tp->count;
 	const int cur = it & 1, nxt = cur ^ 1;
 	double red[4];
@@ -1741,7 +1742,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) 
 	(void)slot;
 	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
 	const int scn = Tp->scene;
-	if (scn < 0 || Tp->done) return;
+	if (scn < 0 || Tp->done || Tp->it >= a.maxit) return;
 	const int it = Tp->it, t0 = Tp->t0, W = Tp->W, cnt = Tp->count;
 	const int cur = it & 1, nxt = cur ^ 1;
 	double red[2];
@@ -2248,6 +2249,7 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 	pa.prog = ctx->d_prog;
 	if (!pa.prog) throw HipError("PCG progress record not allocated (no scene table)", PDC_ERR_STATE);
 	pa.tol2 = ctx->p.pcg_tol * ctx->p.pcg_tol;
+	pa.maxit = ctx->p.pcg_max_iter;
 	return pa;
 }
 

@@ -211,6 +211,39 @@ def test_fill_forces_spring_and_restitution(require_gpu):
     _fill_parity(sc2, 200, p2, tol_rel=1e-5)
 
 
+@pytest.mark.parametrize("pcg_tol", [1e-8, 1e-13])
+def test_pcg_cap_matches_oracle_pcg(require_gpu, pcg_tol):
+    """Solves that end at the PCG iteration cap (6 iterations, far below
+    what the tolerance needs) stop at the cap exactly, as the oracle's PCG
+    branch does (it < maxit): the same inner iteration counts, cap hits and
+    Newton iterations (the Hu schedule grows by 1.3 after a capped solve)
+    and the same velocity increments within 1e-6 of |g| dt.  pcg_tol 1e-8
+    runs the single-reduction iteration, 1e-13 the two-reduction one."""
+    sc = scenes.stack_10(tolerance=1e-8)
+    p = sc.params
+    p.pcg_precision = PCG_FP64
+    p.pcg_tol = pcg_tol
+    p.pcg_max_iter = 6
+    o = _advance_oracle(sc, 30, advance_tol=1e-6)
+    pos, quat, vel, ang = o.get_state()
+    o.detect()
+    co = o.contacts()
+    o.set_linear_solver(orc.SOLVER_PCG)
+    o.set_params(p)
+    acc_o, _, st_o = o.fill_forces(co, quat, vel, ang)
+    solver = pdc.PrimalDualForceSolver(sc, p)
+    acc_g, _, st_g = solver.fill_forces(co, quat, vel, ang)
+    print("oracle: Newton %d, PCG %d, hits %d; device: Newton %d, PCG %d, hits %d"
+          % (st_o.newton_iterations, st_o.pcg_iterations, st_o.pcg_maxiter_hits,
+             st_g.newton_iterations, st_g.pcg_iterations, st_g.pcg_maxiter_hits))
+    assert st_o.pcg_maxiter_hits > 0
+    assert st_g.pcg_maxiter_hits == st_o.pcg_maxiter_hits
+    assert st_g.pcg_iterations == st_o.pcg_iterations
+    assert st_g.newton_iterations == st_o.newton_iterations
+    gdt = np.linalg.norm(sc.params.gravity[:]) * sc.params.timestep
+    assert np.max(np.abs(acc_g - acc_o)) / gdt < 1e-6
+
+
 def _traj_errors(sc, o_state, g_state):
     pos_o, q_o = o_state[0], o_state[1]
     pos_g, q_g = g_state[0], g_state[1]
