@@ -246,8 +246,9 @@ void pdc_reset_timings(pdc_context* ctx);
  * graph replay) while enabled, active iterations only; PCG = the same kernel
  * timed per replayed group of iterations, every launch counted;
  * CONTACT_TERMS = constraint + contact-residual kernels of a Newton
- * iteration; ASSEMBLY = the assembly kernels; LINE_SEARCH = the contact
- * kernel of a line-search trial; CONTACTS = the sample-test kernel. */
+ * iteration; ASSEMBLY = the assembly kernels; LINE_SEARCH = the evaluation
+ * kernel of a line-search pass in the Newton direction (its trials' contact
+ * and body parts); CONTACTS = the sample-test kernel. */
 enum { PDC_KT_NONE = 0, PDC_KT_SPMV = 1, PDC_KT_CONTACT_TERMS = 2,
        PDC_KT_ASSEMBLY = 3, PDC_KT_LINE_SEARCH = 4, PDC_KT_CONTACTS = 5,
        PDC_KT_PCG = 6 };

@@ -535,12 +535,14 @@ __device__ __forceinline__ void to_local(const SArgs& a, int i, const double* va
 }
 
 template <int K>
-__device__ __forceinline__ void write_partials(double (&v)[K], double* part, int stride, const int* slots) {
+__device__ __forceinline__ void write_partials(double (&v)[K], double* part, int stride, const int* slots,
+                                               int tix = -1) {
 	__shared__ double lds[K * 16];
 	double out[K];
 	block_sum<K, double>(v, lds, out);
+	if (tix < 0) tix = blockIdx.x;
 	if (threadIdx.x == 0)
-		for (int k = 0; k < K; ++k) part[slots[k] * (size_t)stride + blockIdx.x] = out[k];
+		for (int k = 0; k < K; ++k) part[slots[k] * (size_t)stride + tix] = out[k];
 }
 
 // Sum of partial slot `slot` over tiles [t0, t1) by the first wavefront, in a
@@ -1876,13 +1878,15 @@ __device__ __forceinline__ void trial_point(const SArgs& a, int i, const double*
 //
 // Contact part: partial |wrf'|^2, |rl'|^2, |ac'|^2, df_in.rf' and
 // sum (dl_in / lambda') rl' (fresh constraint values).
-__global__ __launch_bounds__(256) void k_ls_contact(SArgs a, const double* __restrict__ du,
-                                                    const double* __restrict__ dF,
-                                                    const double* __restrict__ dL, int k0, int want) {
-	PDC_TILE_RUN(a.tc);
-	if (a.ss[scn].ls_state != want) return;
-	const double muc = a.ss[scn].muc, mu = a.ctl[scn].mu, cm = a.ss[scn].char_mass;
+__device__ __forceinline__ void ls_contact_tile(const SArgs& a, const double* __restrict__ du,
+                                                const double* __restrict__ dF, const double* __restrict__ dL,
+                                                int k0, int want, int tix) {
+	const Tile tile = a.tc[tix];
+	if (tile.scene < 0) return;
+	const int scn = tile.scene;
+	if (!a.ss[scn].run || a.ss[scn].ls_state != want) return;
 	const int t = blockIdx.y;
+	const double muc = a.ss[scn].muc, mu = a.ctl[scn].mu, cm = a.ss[scn].char_mass;
 	const double step = ldexp(1.0, -(k0 + t));
 	double acc[5] = {0, 0, 0, 0, 0};
 	if ((int)threadIdx.x < tile.count) {
@@ -1905,19 +1909,21 @@ __global__ __launch_bounds__(256) void k_ls_contact(SArgs a, const double* __res
 	}
 	const int b = P_LS + 7 * t;
 	const int slots[5] = {b, b + 1, b + 2, b + 3, b + 4};
-	write_partials<5>(acc, a.part, a.pstride, slots);
+	write_partials<5>(acc, a.part, a.pstride, slots, tix);
 }
 
 // Body part, one wavefront per body: ru' = M (v' - vold) + sum_j T_j f';
 // partial |ru'|^2 and du_in . ru'.
-__global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restrict__ du,
-                                                 const double* __restrict__ dF,
-                                                 const double* __restrict__ dL, int k0, int want) {
-	PDC_TILE_RUN(a.tw);
-	if (a.ss[scn].ls_state != want) return;
+__device__ __forceinline__ void ls_body_tile(const SArgs& a, const double* __restrict__ du,
+                                             const double* __restrict__ dF, const double* __restrict__ dL,
+                                             int k0, int want, int tix) {
+	const Tile tile = a.tw[tix];
+	if (tile.scene < 0) return;
+	const int scn = tile.scene;
+	if (!a.ss[scn].run || a.ss[scn].ls_state != want) return;
+	const int t = blockIdx.y;
 	const double mu = a.ctl[scn].mu;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-	const int t = blockIdx.y;
 	const double step = ldexp(1.0, -(k0 + t));
 	double acc[2] = {0, 0};
 	if (wid < tile.count && !a.lstat[tile.first + wid]) {
@@ -1956,7 +1962,21 @@ __global__ __launch_bounds__(256) void k_ls_body(SArgs a, const double* __restri
 		}
 	}
 	const int slots[2] = {P_LS + 7 * t + 5, P_LS + 7 * t + 6};
-	write_partials<2>(acc, a.part, a.pstride, slots);
+	write_partials<2>(acc, a.part, a.pstride, slots, tix);
+}
+
+// One pass of trials, both parts in one launch: they evaluate the same trial
+// points independently (the body part forms its contacts' trial forces
+// itself) and write different partial slots.  Blocks [0, ntc) take the
+// contact tiles, blocks [ntc, ntc + ntw) the body tiles; blockIdx.y is the
+// trial of the pass.  (Looping over a pass's trials inside a workgroup, which
+// shares the tile's inputs and dispatches fewer workgroups, was slower at
+// every split measured: the trials of a workgroup then run one after another.)
+__global__ __launch_bounds__(256) void k_ls_eval(SArgs a, const double* __restrict__ du,
+                                                 const double* __restrict__ dF, const double* __restrict__ dL,
+                                                 int k0, int want, int ntc) {
+	if ((int)blockIdx.x < ntc) ls_contact_tile(a, du, dF, dL, k0, want, blockIdx.x);
+	else ls_body_tile(a, du, dF, dL, k0, want, blockIdx.x - ntc);
 }
 
 // Acceptance (PrimalDualForceSolver.cpp:605-631) over the nt trials of a pass
@@ -2687,17 +2707,13 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		for (int pass = 0, k0 = 0; pass < kNewtonPasses; k0 += kNewtonPass[pass], ++pass) {
 			const int nt = kNewtonPass[pass];
 			kt_begin(ctx, PDC_KT_LINE_SEARCH, &e1);
-			launch_trials(k_ls_contact, ntc, nt, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
-			              (int)LS_PENDING);
+			launch_trials(k_ls_eval, ntc + ntw, nt, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
+			              k0, (int)LS_PENDING, ntc);
 			kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
-			launch_trials(k_ls_body, ntw, nt, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p, k0,
-			              (int)LS_PENDING);
 			launch(k_ls_final, S, 1024, a, k0, nt, 10, 0);
 		}
-		launch_trials(k_ls_contact, ntc, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, 0,
-		              (int)LS_GD_PENDING);
-		launch_trials(k_ls_body, ntw, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, 0,
-		              (int)LS_GD_PENDING);
+		launch_trials(k_ls_eval, ntc + ntw, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p,
+		              0, (int)LS_GD_PENDING, ntc);
 		launch(k_ls_final, S, 1024, a, 0, kTrials, 20, 1);
 		launch(k_ls_commit, ntc, kBlock, a, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
 		       (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p);
@@ -2722,6 +2738,12 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	// chunks (pcg_solve).
 	const bool async = phase_graphs && (kw == PDC_KT_NONE || kw == PDC_KT_PCG);
 	std::vector<char> running(in);
+	// No Newton-state record of an earlier solve (nor the zeroed record of a
+	// scene never solved) may read as this solve's iteration 0: the host would
+	// take the scene for one that left its loop and stop waiting for its PCG.
+	// The stream is idle here (the previous solve ended with a wait).
+	if (async)
+		for (int sc = 0; sc < S; ++sc) ctx->h_prog[sc].runtag = -1;
 	for (int iter = 0; iter < ctx->p.iterations; ++iter) {
 		if (async)
 			for (int sc = 0; sc < S; ++sc)
