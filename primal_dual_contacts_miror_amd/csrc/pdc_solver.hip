@@ -2035,12 +2035,14 @@ __global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int k0, int nt, int 
 
 // Commit the accepted trial point of the scene's search (the regular
 // direction, or the gradient-descent one after a fallback): f, lambda in
-// place, then v (k_commit_v).
-__global__ __launch_bounds__(256) void k_ls_commit(SArgs a, const double* __restrict__ dF,
-                                                   const double* __restrict__ dL,
-                                                   const double* __restrict__ gF,
-                                                   const double* __restrict__ gL) {
-	PDC_TILE_RUN(a.tc);
+// place here, v in commit_v_tile.
+__device__ __forceinline__ void ls_commit_tile(const SArgs& a, const double* __restrict__ dF,
+                                               const double* __restrict__ dL, const double* __restrict__ gF,
+                                               const double* __restrict__ gL, int tix) {
+	const Tile tile = a.tc[tix];
+	if (tile.scene < 0) return;
+	const int scn = tile.scene;
+	if (!a.ss[scn].run) return;
 	const double step = a.ss[scn].ls_step, mu = a.ctl[scn].mu;
 	if (!(step > 0)) return;
 	const bool g = a.ss[scn].ls_state == LS_GD_DECIDED;
@@ -2060,8 +2062,11 @@ __global__ __launch_bounds__(256) void k_ls_commit(SArgs a, const double* __rest
 
 // After a gradient-descent fallback the increment becomes -ru, as in the
 // reference (:451), and so does the next PCG's start.
-__global__ __launch_bounds__(256) void k_commit_v(SArgs a, const double* __restrict__ gdu) {
-	PDC_TILE_RUN(a.tb);
+__device__ __forceinline__ void commit_v_tile(const SArgs& a, const double* __restrict__ gdu, int tix) {
+	const Tile tile = a.tb[tix];
+	if (tile.scene < 0) return;
+	const int scn = tile.scene;
+	if (!a.ss[scn].run) return;
 	const double step = a.ss[scn].ls_step;
 	const bool gd = a.ss[scn].ls_state == LS_GD_DECIDED;
 	if ((int)threadIdx.x < tile.count) {
@@ -2072,6 +2077,17 @@ __global__ __launch_bounds__(256) void k_commit_v(SArgs a, const double* __restr
 			if (gd) a.du[6 * r + k] = d;
 		}
 	}
+}
+
+// Both commits in one launch (the contact part writes f and lambda, the body
+// part v and du; neither reads what the other writes): blocks [0, ntc) take
+// the contact tiles, blocks [ntc, ntc + ntb) the body tiles.
+__global__ __launch_bounds__(256) void k_commit(SArgs a, const double* __restrict__ dF,
+                                                const double* __restrict__ dL, const double* __restrict__ gF,
+                                                const double* __restrict__ gL, const double* __restrict__ gdu,
+                                                int ntc) {
+	if ((int)blockIdx.x < ntc) ls_commit_tile(a, dF, dL, gF, gL, blockIdx.x);
+	else commit_v_tile(a, gdu, blockIdx.x - ntc);
 }
 
 // End of a Newton iteration per scene (PrimalDualForceSolver.cpp:468-490):
@@ -2715,9 +2731,8 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		launch_trials(k_ls_eval, ntc + ntw, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p,
 		              0, (int)LS_GD_PENDING, ntc);
 		launch(k_ls_final, S, 1024, a, 0, kTrials, 20, 1);
-		launch(k_ls_commit, ntc, kBlock, a, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
-		       (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p);
-		launch(k_commit_v, ntb, kBlock, a, gdu);
+		launch(k_commit, ntc + ntb, kBlock, a, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
+		       (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, gdu, ntc);
 		launch(k_newton_end, gS, kBlock, a, ctx->p.pcg_max_iter, ctx->p.iterations);
 		PDC_CHECK_LAUNCH();
 	};
