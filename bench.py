@@ -31,7 +31,7 @@ KT_NAMES = {"pcg": pdc.KT_PCG, "pcg_per_launch": pdc.KT_SPMV, "contact_terms": p
             "contacts": pdc.KT_CONTACTS}
 KT_KERNEL = {"pcg": "k_pcg_iter (one PCG iteration: block-CSR SpMV with A P, block-Jacobi apply, vector "
                     "updates, partial dot products), timed by one event pair around each solve's replayed "
-                    "16-launch graphs (the launches that find the solve converged included)",
+                    "%d-launch graphs (the launches that find the solve converged included)",
              "pcg_per_launch": "k_pcg_iter, launched without graph replay, timed per launch",
              "contact_terms": "k_constraints + k_contact_res", "contacts": "k_pair_hits"}
 
@@ -388,7 +388,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": KT_KERNEL[args.kernel],
+            "kernel": KT_KERNEL[args.kernel] % (ds.ctx.get_shapes().pcg_chunk or 32) if args.kernel == "pcg"
+            else KT_KERNEL[args.kernel],
             "achieved": achieved / 1e9,
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",

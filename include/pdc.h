@@ -214,9 +214,10 @@ int pdc_step_scenes(pdc_context* ctx, int32_t n_steps, int32_t n_selected,
  * < 1024, >= 1024.  contact_tile: contacts per workgroup of the per-contact
  * kernels (64, 128 or 256); body_waves: bodies (and body pairs) per workgroup
  * of the one-wavefront-per-body kernels (1, 2 or 4); pcg_chunk: launches per
- * recorded PCG chunk (8, 16 or 32, for every class).  A scene's results
- * depend on its own class's shapes only, so they stay the same alone and in
- * any mix that uses the same table.  Defaults: 256, 4, 16. */
+ * recorded PCG chunk (8, 16 or 32, for every class; 0: automatic, 32 for a
+ * single scene and 16 for several).  A scene's results depend on its own
+ * class's shapes only, so they stay the same alone and in any mix that uses
+ * the same table.  Defaults: 256, 4, 0. */
 typedef struct pdc_shapes {
 	int32_t contact_tile[4];
 	int32_t body_waves[4];

@@ -473,7 +473,8 @@ static void check_shapes(const pdc_shapes& sh) {
 		if (ct != 64 && ct != 128 && ct != 256) throw ConfigError("contact_tile must be 64, 128 or 256");
 		if (bw != 1 && bw != 2 && bw != 4) throw ConfigError("body_waves must be 1, 2 or 4");
 	}
-	if (sh.pcg_chunk != 8 && sh.pcg_chunk != 16 && sh.pcg_chunk != 32) throw ConfigError("pcg_chunk must be 8, 16 or 32");
+	if (sh.pcg_chunk != 0 && sh.pcg_chunk != 8 && sh.pcg_chunk != 16 && sh.pcg_chunk != 32)
+		throw ConfigError("pcg_chunk must be 0 (automatic), 8, 16 or 32");
 }
 
 int pdc_get_shapes(const pdc_context* ctx, pdc_shapes* out) {

@@ -2228,7 +2228,7 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 }
 
 // One recorded chunk of `chunk` PCG iterations (launches alternating the
-// counter slots 0, 1; the chunk length, pdc_shapes.pcg_chunk, is even).
+// counter slots 0, 1; the chunk length, pdc_context::pcg_chunk(), is even).
 template <class T>
 static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk, bool two, int chunk, hipEvent_t* ev) {
 	hipStream_t s = ctx->stream;
@@ -2328,7 +2328,7 @@ static PcgGraph& pcg_graph(pdc_context* ctx, const PcgArgs<T>& pa) {
 	const int nblk = std::max(1, ctx->cap_tr);
 	hipStream_t s = ctx->stream;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
-	const int chunk = ctx->shapes.pcg_chunk;
+	const int chunk = ctx->pcg_chunk();
 	PcgGraph& G = ctx->pcg_graph[(sizeof(T) == 8 ? 1 : 0) + (two ? 2 : 0)];
 	static_assert(sizeof(PcgArgs<T>) <= sizeof(G.args), "recorded arguments fit");
 	if (ctx->use_graphs && (G.exec == nullptr || G.nblk != nblk || G.chunk != chunk ||
@@ -2357,7 +2357,7 @@ static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 	const int nblk = std::max(1, ctx->cap_tr);
 	hipStream_t s = ctx->stream;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
-	const int chunk = ctx->shapes.pcg_chunk;
+	const int chunk = ctx->pcg_chunk();
 	PcgGraph& G = pcg_graph<T>(ctx, pa);
 	// Kernel timing of k_pcg_iter.  SPMV: launched directly between an event
 	// pair per launch (events cannot be timed inside a replayed graph), only
