@@ -228,7 +228,7 @@ struct pdc_context {
 	pdc::DBuf<int> a_cnt, a_ptr, a_diag;
 	int pcg_last_iters = 0;
 	pdc::PcgGraph pcg_graph[4];                               // recorded chunks (FP32, FP64; x2 forms)
-	pdc::PcgGraph phase_graph[2];                             // a Newton iteration's launches before / after the PCG
+	pdc::PcgGraph phase_graph[2];                             // launches before the first PCG; after a PCG and before the next
 	bool use_graphs = true;
 	std::vector<hipEvent_t> kt_direct;                        // events for direct-launch timing
 	hipEvent_t* kt_direct_ev(size_t n) {

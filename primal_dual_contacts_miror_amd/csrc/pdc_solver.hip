@@ -2736,16 +2736,24 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		launch(k_newton_end, gS, kBlock, a, ctx->p.pcg_max_iter, ctx->p.iterations);
 		PDC_CHECK_LAUNCH();
 	};
-	// Both phases are recorded once per solve and replayed every iteration
-	// (about thirty launches each, which the host would otherwise submit one
-	// by one after every wait).  Kernel timing of a phase's own kernels needs
-	// direct launches.
+	// The phases are recorded once per solve and replayed every iteration
+	// (a dozen launches and more each, which the host would otherwise submit
+	// one by one after every wait).  Kernel timing of a phase's own kernels
+	// needs direct launches.
 	const int kw = ctx->kt.which;
 	const bool phase_graphs = ctx->use_graphs && kw != PDC_KT_CONTACT_TERMS && kw != PDC_KT_ASSEMBLY &&
 	                          kw != PDC_KT_LINE_SEARCH;
+	// The host always queues phase B of one iteration and phase A of the next
+	// together, so they are one graph (a boundary between replayed graphs
+	// costs more than a launch inside one).  After the last iteration's
+	// phase B the next phase A finds every scene out of its loop and its
+	// launches return at once.
 	if (phase_graphs) {
 		record_phase(ctx, ctx->phase_graph[0], phase_a);
-		record_phase(ctx, ctx->phase_graph[1], phase_b);
+		record_phase(ctx, ctx->phase_graph[1], [&]() {
+			phase_b();
+			phase_a();
+		});
 	}
 	// With graphs the host never waits for the stream in the loop
 	// (pcg_solve_async); without graphs, or with the per-launch timing of the
@@ -2759,22 +2767,33 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	// The stream is idle here (the previous solve ended with a wait).
 	if (async)
 		for (int sc = 0; sc < S; ++sc) ctx->h_prog[sc].runtag = -1;
-	for (int iter = 0; iter < ctx->p.iterations; ++iter) {
+	// the PCG records of the iteration about to be queued
+	auto reset_records = [&]() {
 		if (async)
 			for (int sc = 0; sc < S; ++sc)
 				if (running[sc]) {
 					ctx->h_prog[sc].outcome = 0;
 					ctx->h_prog[sc].it = 0;
 				}
-		if (phase_graphs) PDC_HIP(hipGraphLaunch(ctx->phase_graph[0].exec, s));
-		else phase_a();
+	};
+	reset_records();
+	if (phase_graphs) PDC_HIP(hipGraphLaunch(ctx->phase_graph[0].exec, s));
+	else phase_a();
+	for (int iter = 0; iter < ctx->p.iterations; ++iter) {
 		bool any;
 		if (async) any = fp32 ? pcg_solve_async<float>(ctx, pa32, iter, running)
 		                      : pcg_solve_async<double>(ctx, pa64, iter, running);
 		else any = fp32 ? pcg_solve<float>(ctx, pa32) : pcg_solve<double>(ctx, pa64);
 		if (!any) break;  // every scene converged or reached the cap
-		if (phase_graphs) PDC_HIP(hipGraphLaunch(ctx->phase_graph[1].exec, s));
-		else phase_b();
+		// phase B of this iteration, then phase A of the next (no PCG record
+		// is written by phase B)
+		reset_records();
+		if (phase_graphs) {
+			PDC_HIP(hipGraphLaunch(ctx->phase_graph[1].exec, s));
+		} else {
+			phase_b();
+			if (iter + 1 < ctx->p.iterations) phase_a();
+		}
 	}
 	launch(k_writeback_bodies, ntb, kBlock, a, ctx->b_acc.p);
 	launch(k_writeback_contacts, ntc, kBlock, a, ctx->c_lf.p);
