@@ -331,16 +331,19 @@ def test_step_is_deterministic(require_gpu):
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("variant", ["fp32", "cap", "fp64"])
+@pytest.mark.parametrize("variant", ["fp32", "cap", "fp64", "newton_cap"])
 def test_graph_replay_equals_direct_launches(require_gpu, monkeypatch, variant):
-    """The recorded graphs (the PCG chunks and the two phases of a Newton
-    iteration, replayed every iteration and recorded again every step) and
-    the host loop that queues PCG chunks from the kernels' progress records
-    without waiting for the stream give the same bits as launching every
-    kernel directly and waiting after each group, over steps whose contact
-    sets and sizes change.  Variants: the FP32 PCG; the FP32 PCG capped at 6
-    iterations (every solve ends at the cap, where the host loop waits for
-    the stream); the FP64 two-reduction PCG."""
+    """The recorded graphs (the PCG chunks, phase A, and phase B of a Newton
+    iteration together with phase A of the next, replayed every iteration
+    and recorded again every step) and the host loop that queues PCG chunks
+    from the kernels' progress records without waiting for the stream give
+    the same bits as launching every kernel directly and waiting after each
+    group, over steps whose contact sets and sizes change.  Variants: the
+    FP32 PCG; the FP32 PCG capped at 6 iterations (every solve ends at the
+    cap, where the host loop waits for the stream); the FP64 two-reduction
+    PCG; the Newton loop capped at 3 iterations (solves that end at the cap
+    are followed on the graph path by one more phase A, whose launches must
+    change nothing)."""
     sc = scenes.pile(60, seed=4, half=2.5, height=8.0, name="pile_60", grid_n=48, sample_spacing=0.1)
     o = _advance_oracle(sc, 150)
     state = o.get_state()
@@ -352,6 +355,8 @@ def test_graph_replay_equals_direct_launches(require_gpu, monkeypatch, variant):
     elif variant == "fp64":
         q.pcg_precision = PCG_FP64
         q.pcg_tol = 1e-12
+    elif variant == "newton_cap":
+        q.iterations = 3
     res = []
     for direct in (False, True):
         if direct:
@@ -369,6 +374,8 @@ def test_graph_replay_equals_direct_launches(require_gpu, monkeypatch, variant):
         res.append((traj, hits))
         ds.ctx.close()
     assert sum(t[1] for t in res[0][0]) > steps, "the steps should iterate"
+    if variant == "newton_cap":
+        assert any(t[1] == 3 for t in res[0][0]), "solves should end at the Newton cap"
     if variant == "cap":
         assert res[0][1] > 0 and res[0][1] == res[1][1], "solves should reach the cap"
     for a, b in zip(res[0][0], res[1][0]):
