@@ -30,8 +30,8 @@ PEAK_HBM = 8.0e12
 KT_NAMES = {"pcg": pdc.KT_PCG, "pcg_per_launch": pdc.KT_SPMV, "contact_terms": pdc.KT_CONTACT_TERMS,
             "contacts": pdc.KT_CONTACTS}
 KT_KERNEL = {"pcg": "k_pcg_iter (one PCG iteration: block-CSR SpMV with A P, block-Jacobi apply, vector "
-                    "updates, partial dot products), timed by one event pair around each solve's replayed "
-                    "%d-launch graphs (the launches that find the solve converged included)",
+                    "updates, partial dot products), timed by one event pair around each solve's launches, "
+                    "queued %d at a time (the launches that find the solve converged included)",
              "pcg_per_launch": "k_pcg_iter, launched without graph replay, timed per launch",
              "contact_terms": "k_constraints + k_contact_res", "contacts": "k_pair_hits"}
 
@@ -388,7 +388,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": KT_KERNEL[args.kernel] % (ds.ctx.get_shapes().pcg_chunk or 32) if args.kernel == "pcg"
+            "kernel": KT_KERNEL[args.kernel] % (ds.ctx.get_shapes().pcg_chunk or 8) if args.kernel == "pcg"
             else KT_KERNEL[args.kernel],
             "achieved": achieved / 1e9,
             "peak": PEAK_HBM / 1e9,

@@ -213,9 +213,9 @@ int pdc_step_scenes(pdc_context* ctx, int32_t n_steps, int32_t n_selected,
  * CPU loops have none).  A scene's class is its body count: < 64, < 256,
  * < 1024, >= 1024.  contact_tile: contacts per workgroup of the per-contact
  * kernels (64, 128 or 256); body_waves: bodies (and body pairs) per workgroup
- * of the one-wavefront-per-body kernels (1, 2 or 4); pcg_chunk: launches per
- * recorded PCG chunk (8, 16 or 32, for every class; 0: automatic, 32 for a
- * single scene and 16 for several).  A scene's results depend on its own
+ * of the one-wavefront-per-body kernels (1, 2 or 4); pcg_chunk: PCG
+ * launches the host queues at a time (8, 16 or 32, for every class; 0:
+ * automatic, 8 for a single scene and 16 for several).  A scene's results depend on its own
  * class's shapes only, so they stay the same alone and in any mix that uses
  * the same table.  Defaults: 256, 4, 0. */
 typedef struct pdc_shapes {
@@ -245,7 +245,8 @@ void pdc_reset_timings(pdc_context* ctx);
  * the preconditioned operator, block-Jacobi apply, vector updates, partial
  * dot products), timed per launch with the iterations launched directly (no
  * graph replay) while enabled, active iterations only; PCG = the same kernel
- * timed per replayed group of iterations, every launch counted;
+ * timed by one event pair around each solve's launches, every launch
+ * counted;
  * CONTACT_TERMS = constraint + contact-residual kernels of a Newton
  * iteration; ASSEMBLY = the assembly kernels; LINE_SEARCH = the evaluation
  * kernel of a line-search pass in the Newton direction (its trials' contact

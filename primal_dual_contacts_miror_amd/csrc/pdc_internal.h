@@ -270,11 +270,10 @@ struct pdc_context {
 	int n_tc = 0, n_tb = 0, n_tw = 0, n_tp = 0, n_tr = 0;
 	int cap_tr = 0;                                           // PCG tiles launched (fixed per scene set)
 	pdc_shapes shapes{{256, 256, 256, 256}, {4, 4, 4, 4}, 0};   // work shapes per size class
-	// Launches per recorded PCG chunk: the table's, or when it says 0, 32 for
-	// one scene and 16 for several (a boundary between replayed graphs costs
-	// more than a launch inside one; with several scenes the slowest one sets
-	// the end, and longer chunks queue more launches past it).
-	int pcg_chunk() const { return shapes.pcg_chunk > 0 ? shapes.pcg_chunk : S == 1 ? 32 : 16; }
+	// PCG launches per group the host queues (and per recorded chunk where
+	// the PCG is replayed from a graph): the table's, or when it says 0, 8 for
+	// one scene and 16 for several.
+	int pcg_chunk() const { return shapes.pcg_chunk > 0 ? shapes.pcg_chunk : S == 1 ? 8 : 16; }
 	pdc::DBuf<pdc::PcgTile> t_pcg;                            // PCG workgroup records (cap_tr)
 	// The PCG works in slot space: slot kPcgRowsPerWG * tile + k is block row
 	// k of PCG tile `tile` (slots past a tile's count are padding).  The

@@ -2438,8 +2438,8 @@ static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 // The PCG of one Newton iteration without waiting for the stream.  The
 // kernels write each scene's progress into pinned host memory (PcgProgress:
 // its Newton state after k_err_final, the iterations started, |r|^2/|b|^2,
-// the outcome), and the host keeps recorded chunks queued a little ahead of
-// the device's position: up to two chunks, and no more than the iterations
+// the outcome), and the host keeps groups of launches queued a little ahead
+// of the device's position: up to two groups, and no more than the iterations
 // the slowest solving scene still needs (the previous solve's count at
 // first, then the residual's decay since the solve's first reading,
 // extrapolated).  The queue does not drain while a scene is solving, so
@@ -2450,25 +2450,29 @@ static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 // state for iteration `iter` says so.  Returns whether any scene still runs.
 //
 // Termination does not depend on the progress records arriving: when the
-// stream is idle and a scene has not reported an outcome, another chunk is
+// stream is idle and a scene has not reported an outcome, another group is
 // queued, and at the iteration cap the host waits for the stream, after
 // which every record is final.
 template <class T>
 static bool pcg_solve_async(pdc_context* ctx, const PcgArgs<T>& pa, int iter, std::vector<char>& running) {
 	const int S = ctx->S;
 	hipStream_t s = ctx->stream;
-	PcgGraph& G = pcg_graph<T>(ctx, pa);
-	const int chunk = G.chunk;
+	const int chunk = ctx->pcg_chunk();
+	const int nblk = std::max(1, ctx->cap_tr);
 	const int maxit = ctx->p.pcg_max_iter;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
 	const bool kt_iter = ctx->kt.which == PDC_KT_PCG;
 	const int ahead_max = 2 * chunk, ahead_min = 4;
 	volatile PcgProgress* P = ctx->h_prog;
 	int launched = 0;
+	// direct launches, a group of `chunk` at a time: replaying recorded
+	// chunks was slower (a boundary between replayed graphs costs more than
+	// a launch, and graphs long enough to amortise it queue more launches
+	// past the end of the solve)
 	auto enqueue_to = [&](int target) {
 		target = std::min(target, maxit);
 		while (launched < target) {
-			PDC_HIP(hipGraphLaunch(G.exec, s));
+			pcg_chunk_launches<T>(ctx, pa, nblk, two, chunk, nullptr);
 			launched += chunk;
 		}
 	};
