@@ -2740,13 +2740,19 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		launch(k_newton_end, gS, kBlock, a, ctx->p.pcg_max_iter, ctx->p.iterations);
 		PDC_CHECK_LAUNCH();
 	};
-	// The phases are recorded once per solve and replayed every iteration
-	// (a dozen launches and more each, which the host would otherwise submit
-	// one by one after every wait).  Kernel timing of a phase's own kernels
-	// needs direct launches.
+	// With graphs enabled and no per-launch timing, the host never waits for
+	// the stream in the loop (pcg_solve_async) and launches every kernel
+	// directly: replaying the phases from graphs measured slower there.
+	// With the per-launch PCG timing (direct PCG launches between events) it
+	// waits for each group of PCG launches (pcg_solve) and replays the phases,
+	// recorded once per solve, from graphs; without graphs it waits likewise
+	// and launches everything directly.  Kernel timing of a phase's own
+	// kernels needs direct launches.
 	const int kw = ctx->kt.which;
-	const bool phase_graphs = ctx->use_graphs && kw != PDC_KT_CONTACT_TERMS && kw != PDC_KT_ASSEMBLY &&
-	                          kw != PDC_KT_LINE_SEARCH;
+	const bool graphs = ctx->use_graphs && kw != PDC_KT_CONTACT_TERMS && kw != PDC_KT_ASSEMBLY &&
+	                    kw != PDC_KT_LINE_SEARCH;
+	const bool async = graphs && (kw == PDC_KT_NONE || kw == PDC_KT_PCG);
+	const bool phase_graphs = graphs && !async;
 	// The host always queues phase B of one iteration and phase A of the next
 	// together, so they are one graph (a boundary between replayed graphs
 	// costs more than a launch inside one).  After the last iteration's
@@ -2759,11 +2765,6 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 			phase_a();
 		});
 	}
-	// With graphs the host never waits for the stream in the loop
-	// (pcg_solve_async); without graphs, or with the per-launch timing of the
-	// PCG (which needs direct launches), it waits for each group of PCG
-	// chunks (pcg_solve).
-	const bool async = phase_graphs && (kw == PDC_KT_NONE || kw == PDC_KT_PCG);
 	std::vector<char> running(in);
 	// No Newton-state record of an earlier solve (nor the zeroed record of a
 	// scene never solved) may read as this solve's iteration 0: the host would

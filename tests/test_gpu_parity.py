@@ -333,17 +333,14 @@ def test_step_is_deterministic(require_gpu):
 
 @pytest.mark.parametrize("variant", ["fp32", "cap", "fp64", "newton_cap"])
 def test_graph_replay_equals_direct_launches(require_gpu, monkeypatch, variant):
-    """The recorded graphs (the PCG chunks, phase A, and phase B of a Newton
-    iteration together with phase A of the next, replayed every iteration
-    and recorded again every step) and the host loop that queues PCG chunks
-    from the kernels' progress records without waiting for the stream give
-    the same bits as launching every kernel directly and waiting after each
-    group, over steps whose contact sets and sizes change.  Variants: the
-    FP32 PCG; the FP32 PCG capped at 6 iterations (every solve ends at the
-    cap, where the host loop waits for the stream); the FP64 two-reduction
-    PCG; the Newton loop capped at 3 iterations (solves that end at the cap
-    are followed on the graph path by one more phase A, whose launches must
-    change nothing)."""
+    """The asynchronous host loop (PCG launches queued in groups from the
+    kernels' progress records without waiting for the stream, every kernel
+    launched directly) gives the same bits as the synchronous loop that
+    waits for the stream after each group of PCG launches (PDC_NO_GRAPHS),
+    over steps whose contact sets and sizes change.  Variants: the FP32 PCG;
+    the FP32 PCG capped at 6 iterations (every solve ends at the cap, where
+    the asynchronous loop waits for the stream); the FP64 two-reduction PCG;
+    the Newton loop capped at 3 iterations."""
     sc = scenes.pile(60, seed=4, half=2.5, height=8.0, name="pile_60", grid_n=48, sample_spacing=0.1)
     o = _advance_oracle(sc, 150)
     state = o.get_state()
