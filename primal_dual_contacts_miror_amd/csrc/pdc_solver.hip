@@ -2741,18 +2741,19 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		PDC_CHECK_LAUNCH();
 	};
 	// With graphs enabled and no per-launch timing, the host never waits for
-	// the stream in the loop (pcg_solve_async) and launches every kernel
-	// directly: replaying the phases from graphs measured slower there.
-	// With the per-launch PCG timing (direct PCG launches between events) it
-	// waits for each group of PCG launches (pcg_solve) and replays the phases,
-	// recorded once per solve, from graphs; without graphs it waits likewise
-	// and launches everything directly.  Kernel timing of a phase's own
-	// kernels needs direct launches.
+	// the stream in the loop (pcg_solve_async) and launches the PCG
+	// iterations directly.  The phases around the PCG, recorded once per
+	// solve, are replayed from graphs for a set of scenes; a single scene
+	// launches them directly too (replaying them measured slower there).  With
+	// the per-launch PCG timing (direct PCG launches between events) the host
+	// waits for each group of PCG launches (pcg_solve) and replays the phases;
+	// without graphs it waits likewise and launches everything directly.
+	// Kernel timing of a phase's own kernels needs direct launches.
 	const int kw = ctx->kt.which;
 	const bool graphs = ctx->use_graphs && kw != PDC_KT_CONTACT_TERMS && kw != PDC_KT_ASSEMBLY &&
 	                    kw != PDC_KT_LINE_SEARCH;
 	const bool async = graphs && (kw == PDC_KT_NONE || kw == PDC_KT_PCG);
-	const bool phase_graphs = graphs && !async;
+	const bool phase_graphs = graphs && (!async || S > 1);
 	// The host always queues phase B of one iteration and phase A of the next
 	// together, so they are one graph (a boundary between replayed graphs
 	// costs more than a launch inside one).  After the last iteration's
