@@ -1787,27 +1787,38 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) 
 // du from the PCG solution (undoing the Jacobi scaling), and the
 // gradient-descent direction -ru of the fallback search
 // (PrimalDualForceSolver.cpp:451), used only if that search runs.
-__global__ __launch_bounds__(256) void k_du(SArgs a, const double* __restrict__ x, double* __restrict__ gdu) {
-	PDC_TILE_RUN(a.tb);
-	(void)scn;
+// du of local body r: the PCG solution by slot, unscaled.
+__device__ __forceinline__ double body_du(const SArgs& a, const double* __restrict__ x, int r, int k) {
+	return x[6 * a.slot[r] + k] * a.D[6 * r + k];
+}
+
+__device__ __forceinline__ void du_tile(const SArgs& a, const double* __restrict__ x, double* __restrict__ gdu,
+                                        int tix) {
+	const Tile tile = a.tb[tix];
+	if (tile.scene < 0 || !a.ss[tile.scene].run) return;
 	if ((int)threadIdx.x < tile.count) {
 		const int r = tile.first + threadIdx.x;
-		const int sr = a.slot[r];
 		for (int k = 0; k < 6; ++k) {
-			a.du[6 * r + k] = x[6 * sr + k] * a.D[6 * r + k];
+			a.du[6 * r + k] = body_du(a, x, r, k);
 			gdu[6 * r + k] = -a.ru[6 * r + k];
 		}
 	}
 }
 
-// ---- increments (NonSmoothContactForce.cpp:254-280) -------------------------
-__global__ __launch_bounds__(256) void k_increments(SArgs a) {
-	PDC_TILE_RUN(a.tc);
-	(void)scn;
+// Per contact, from du of its two bodies (formed here from the PCG solution,
+// as du_tile forms them, so that both run in one launch).
+__device__ __forceinline__ void increments_tile(const SArgs& a, const double* __restrict__ x, int tix) {
+	const Tile tile = a.tc[tix];
+	if (tile.scene < 0 || !a.ss[tile.scene].run) return;
 	if ((int)threadIdx.x < tile.count) {
 		const int i = tile.first + threadIdx.x;
+		double du0[6], du1[6];
+		for (int k = 0; k < 6; ++k) {
+			du0[k] = body_du(a, x, a.li0[i], k);
+			du1[k] = body_du(a, x, a.li1[i], k);
+		}
 		double u[3];
-		to_local(a, i, &a.du[6 * a.li0[i]], &a.du[6 * a.li1[i]], u);
+		to_local(a, i, du0, du1, u);
 		const double l0 = a.lam[i];
 		if (a.nsc) {
 			const double l1 = a.lam[a.cs + i];
@@ -1826,6 +1837,14 @@ __global__ __launch_bounds__(256) void k_increments(SArgs a) {
 			a.dl[i] = -a.invC[i] * (-l0 * d + a.rl[i]);
 		}
 	}
+}
+
+// du and -ru per body (blocks [0, ntb)) and the force and multiplier
+// increments per contact (blocks [ntb, ntb + ntc)) in one launch.
+__global__ __launch_bounds__(256) void k_du_increments(SArgs a, const double* __restrict__ x,
+                                                       double* __restrict__ gdu, int ntb) {
+	if ((int)blockIdx.x < ntb) du_tile(a, x, gdu, blockIdx.x);
+	else increments_tile(a, x, blockIdx.x - ntb);
 }
 
 // ---- (e) line search as fused reductions (PrimalDualForceSolver.cpp:562-635) --
@@ -2717,8 +2736,7 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		else pcg_begin<double>(ctx, pa64);
 	};
 	auto phase_b = [&]() {
-		launch(k_du, ntb, kBlock, a, (const double*)ctx->x64.p, ctx->l_gdu.p);
-		launch(k_increments, ntc, kBlock, a);
+		launch(k_du_increments, ntb + ntc, kBlock, a, (const double*)ctx->x64.p, ctx->l_gdu.p, ntb);
 		// line search: passes of 2 and 8 trials in the Newton direction, then
 		// the gradient-descent fallback (PrimalDualForceSolver.cpp:441-467) in
 		// one pass of its 20 trials; each pass runs for the scenes still
