@@ -829,14 +829,17 @@ __device__ __forceinline__ void contact_terms(const SArgs& a, int i, const doubl
 }
 
 // Main-iteration residuals of the contacts: stores rf, rl; partial |wrf|^2, |rl|^2, |ac|^2.
-__global__ __launch_bounds__(256) void k_contact_res(SArgs a) {
+__device__ __forceinline__ void contact_res_tile(const SArgs& a, int tix) {
 	__shared__ double slot;
-	PDC_TILE_RUN(a.tc);
+	const Tile tile = a.tc[tix];
+	if (tile.scene < 0) return;
+	const int scn = tile.scene;
+	if (!a.ss[scn].run) return;
 	const double gapsum = tile_sum(a.part, a.pstride, P_GAP, a.tbc[scn], a.tbc[scn + 1], &slot);
 	const int Cs = a.cbeg[scn + 1] - a.cbeg[scn];
 	const double gap = -gapsum / ((double)(a.nsc ? 2 : 1) * Cs);
 	const double muc = gap * 0.1;
-	if ((int)blockIdx.x == a.tbc[scn] && threadIdx.x == 0) { a.ss[scn].gap = gap; a.ss[scn].muc = muc; }
+	if (tix == a.tbc[scn] && threadIdx.x == 0) { a.ss[scn].gap = gap; a.ss[scn].muc = muc; }
 	const double mu = a.ctl[scn].mu, cm = a.ss[scn].char_mass;
 	double acc[3] = {0, 0, 0};
 	if ((int)threadIdx.x < tile.count) {
@@ -853,15 +856,16 @@ __global__ __launch_bounds__(256) void k_contact_res(SArgs a) {
 		acc[2] += ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
 	}
 	const int slots[3] = {P_WRF, P_RL, P_AC};
-	write_partials<3>(acc, a.part, a.pstride, slots);
+	write_partials<3>(acc, a.part, a.pstride, slots, tix);
 }
+__global__ __launch_bounds__(256) void k_contact_res(SArgs a) { contact_res_tile(a, blockIdx.x); }
 
 // ru = M (v - vold) + sum_j T_j f (static rows 0)  (PrimalDualForceSolver.cpp:151-163).
 // One wavefront per body: lanes take the body's incident contacts with stride
 // 64, the six sums are reduced in a fixed butterfly order.
-__global__ __launch_bounds__(256) void k_body_res(SArgs a) {
-	PDC_TILE_RUN(a.tw);
-	(void)scn;
+__device__ __forceinline__ void body_res_tile(const SArgs& a, int tix) {
+	const Tile tile = a.tw[tix];
+	if (tile.scene < 0 || !a.ss[tile.scene].run) return;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	double acc[1] = {0};
 	const double* F = a.f;
@@ -900,7 +904,17 @@ __global__ __launch_bounds__(256) void k_body_res(SArgs a) {
 		}
 	}
 	const int slots[1] = {P_RU};
-	write_partials<1>(acc, a.part, a.pstride, slots);
+	write_partials<1>(acc, a.part, a.pstride, slots, tix);
+}
+__global__ __launch_bounds__(256) void k_body_res(SArgs a) { body_res_tile(a, blockIdx.x); }
+
+// Both residual kernels in one launch (the contact part writes rf, rl and the
+// scene's gap, the body part ru; neither reads what the other writes):
+// blocks [0, ntc) take the contact tiles, blocks [ntc, ntc + ntw) the
+// body-wavefront tiles.
+__global__ __launch_bounds__(256) void k_residuals(SArgs a, int ntc) {
+	if ((int)blockIdx.x < ntc) contact_res_tile(a, blockIdx.x);
+	else body_res_tile(a, blockIdx.x - ntc);
 }
 
 // err, eps and the exit test (PrimalDualForceSolver.cpp:295-320); one workgroup per scene.
@@ -2712,10 +2726,15 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		hipEvent_t e0;
 		kt_begin(ctx, PDC_KT_CONTACT_TERMS, &e0);
 		launch(k_constraints, ntc, kBlock, a);
-		launch(k_contact_res, ntc, kBlock, a);
-		kt_end(ctx, PDC_KT_CONTACT_TERMS, e0);
-		if (e0) ctx->kt.bytes += contact_terms_bytes(C, NL);
-		launch(k_body_res, ntw, kBlock, a);
+		if (e0) {
+			// timed on their own: the contact terms, then the body residuals
+			launch(k_contact_res, ntc, kBlock, a);
+			kt_end(ctx, PDC_KT_CONTACT_TERMS, e0);
+			ctx->kt.bytes += contact_terms_bytes(C, NL);
+			launch(k_body_res, ntw, kBlock, a);
+		} else {
+			launch(k_residuals, ntc + ntw, kBlock, a, ntc);
+		}
 		launch(k_err_final, S, 256, a, ctx->p.tolerance);
 		PDC_CHECK_LAUNCH();
 		// assembly
