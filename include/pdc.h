@@ -87,11 +87,17 @@ typedef struct pdc_solve_stats {
 	int32_t n_local_bodies;     /* bodies in contact (the local solve set) */
 	int32_t newton_iterations;
 	int32_t pcg_iterations;     /* summed over Newton iterations */
-	int32_t pcg_maxiter_hits;   /* solves that reached pcg_max_iter (Hu *= 1.3 rule) */
+	int32_t pcg_maxiter_hits;   /* solves that reached pcg_max_iter (Hu *= 1.3 rule at 2000 or
+	                               more bodies in contact, where the reference solves iteratively) */
 	int32_t gd_fallbacks;       /* line-search failures that fell back to gradient descent */
 	int32_t line_search_trials; /* trial points evaluated */
 	int32_t ties;               /* contact samples within the tie band of the threshold */
 	int32_t n_blocks;           /* 6x6 blocks of the Newton matrix */
+	int32_t nonfinite;          /* Newton iterations whose error or inner solve was not finite:
+	                               the exit test never passes on a NaN error (as with the
+	                               reference's std::max), and a solve that breaks down gives
+	                               du = 0 (as the reference's failed factorisation,
+	                               PrimalDualForceSolver.cpp:390-394) */
 	double exit_error;
 	double char_mass;
 	double char_speed;

@@ -105,7 +105,7 @@ struct SceneSc {
 	int trials;          // line-search trial points evaluated
 	int pcg_total;       // PCG iterations, summed over Newton iterations
 	int pcg_hits;        // PCG solves that reached the iteration cap
-	int pad;
+	int nonfinite;       // Newton iterations with a non-finite error or PCG result
 };
 enum { LS_PENDING = 0, LS_ACCEPTED = 1, LS_GD_PENDING = 2, LS_GD_DECIDED = 3 };
 
@@ -121,9 +121,9 @@ struct SceneCtl {
 // when it waits for the stream) and `run`, the scene's Newton state.
 struct PcgScene {
 	int iters;   // iterations of the last solve
-	int done;    // converged flag
+	int done;    // converged flag (also set when the solve broke down)
 	int run;     // the scene's Newton loop continues (copied by k_err_final)
-	int pad;
+	int bad;     // the solve met a non-finite residual: its increment is taken as zero
 };
 
 // Progress of a scene's Newton loop and PCG, written by the kernels into

@@ -13,11 +13,14 @@
 // floating-point atomics are used and every sum has a fixed order.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "pdc_internal.h"
 #include "pdc_reduce.h"
+#include "pdc_stall.h"
 
 
 namespace pdc {
@@ -917,6 +920,10 @@ __global__ __launch_bounds__(256) void k_residuals(SArgs a, int ntc) {
 	else body_res_tile(a, blockIdx.x - ntc);
 }
 
+// std::max(a, b) and std::min(a, b): (a < b) ? b : a and (b < a) ? b : a.
+__device__ __forceinline__ double ref_max(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double ref_min(double a, double b) { return (b < a) ? b : a; }
+
 // err, eps and the exit test (PrimalDualForceSolver.cpp:295-320); one workgroup per scene.
 __global__ __launch_bounds__(256) void k_err_final(SArgs a, double tol) {
 	__shared__ double sums[4];
@@ -947,11 +954,15 @@ __global__ __launch_bounds__(256) void k_err_final(SArgs a, double tol) {
 		S->sacErr = ru2 + ac2 + rl2;
 		S->eps = Hu * pow(err, 0.66667 * 0.5);
 		const double momen = sqrt(ru2) / nR;
-		double e = fmax(momen, sqrt(wrf2) / nF);
-		e = fmax(e, sqrt(rl2) / nLc);
-		e = fmax(e, S->gap);
-		e = fmin(e, fmax(momen, sqrt(ac2) / nC));
+		// std::max / std::min as the reference calls them: a NaN first
+		// argument is kept, so a non-finite error never passes the exit test
+		// (fmax / fmin would return the other, finite argument)
+		double e = ref_max(momen, sqrt(wrf2) / nF);
+		e = ref_max(e, sqrt(rl2) / nLc);
+		e = ref_max(e, S->gap);
+		e = ref_min(e, ref_max(momen, sqrt(ac2) / nC));
 		S->exitErr = e;
+		if (!isfinite(err) || !isfinite(e)) S->nonfinite += 1;
 		// the exit test; a scene that stays starts this iteration's line search afresh
 		if (e < tol) S->run = 0;
 		S->ls_state = LS_PENDING;
@@ -1413,10 +1424,27 @@ __device__ __forceinline__ void pcg_tile_reset(const PcgArgs<T>& a, PcgTile* Tp,
 		if ((int)blockIdx.x == Tp->t0) {
 			a.ps[scn].done = 0;
 			a.ps[scn].iters = 0;
+			a.ps[scn].bad = 0;
 			// history read by iteration 0: beta = gamma / inf = 0, alpha = gamma / delta
 			a.hist[4 * scn + 2] = __builtin_huge_val();
 			a.hist[4 * scn + 3] = 1.0;
 		}
+	}
+}
+
+// End of a scene's solve after `it` iterations, seen by every tile of the
+// scene (thread 0): converged, or broken down on a non-finite sum (`bad`:
+// the reference takes du = 0 when its factorisation fails,
+// PrimalDualForceSolver.cpp:390-394; the scene's first tile publishes it).
+template <class T>
+__device__ __forceinline__ void pcg_finish(const PcgArgs<T>& a, PcgTile* Tp, int scn, int it, bool bad,
+                                           bool first_tile) {
+	Tp->done = 1;
+	if (first_tile) {
+		a.ps[scn].done = 1;
+		a.ps[scn].iters = it;
+		a.ps[scn].bad = bad ? 1 : 0;
+		a.prog[scn].outcome = (1LL << 32) | (long long)(unsigned)it;
 	}
 }
 
@@ -1601,15 +1629,11 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	__syncthreads();
 	const double gamma = lds[0], delta = lds[1], rr = lds[2], bb = lds[3];
 	const bool first_tile = (int)blockIdx.x == t0;
-	if (!(rr > a.tol2 * bb)) {
-		if (threadIdx.x == 0) {
-			Tp->done = 1;
-			if (first_tile) {
-				a.ps[scn].done = 1;
-				a.ps[scn].iters = it;
-				a.prog[scn].outcome = (1LL << 32) | (long long)(unsigned)it;
-			}
-		}
+	// a non-finite sum ends the solve as a breakdown (a NaN would otherwise
+	// read as converged: every comparison with it is false)
+	const bool bad = !(isfinite(rr) && isfinite(bb) && isfinite(gamma) && isfinite(delta));
+	if (bad || !(rr > a.tol2 * bb)) {
+		if (threadIdx.x == 0) pcg_finish(a, Tp, scn, it, bad, first_tile);
 		return;
 	}
 	const double beta = gamma * inv_gprev;
@@ -1707,15 +1731,9 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) 
 	const int cur = it & 1, nxt = cur ^ 1;
 	double red[4];
 	reduce_partials<4>(pick(a.rr, cur), a.bb, pick(a.gam, cur), pick(a.gam, nxt), t0, W, red, lds);
-	if (!(red[0] > a.tol2 * red[1])) {
-		if (threadIdx.x == 0) {
-			Tp->done = 1;
-			if ((int)blockIdx.x == t0) {
-				a.ps[scn].done = 1;
-				a.ps[scn].iters = it;
-				a.prog[scn].outcome = (1LL << 32) | (long long)(unsigned)it;
-			}
-		}
+	const bool bad = !(isfinite(red[0]) && isfinite(red[1]) && isfinite(red[2]));
+	if (bad || !(red[0] > a.tol2 * red[1])) {
+		if (threadIdx.x == 0) pcg_finish(a, Tp, scn, it, bad, (int)blockIdx.x == t0);
 		return;
 	}
 	if (threadIdx.x == 0 && (int)blockIdx.x == t0) {
@@ -1806,14 +1824,17 @@ __device__ __forceinline__ double body_du(const SArgs& a, const double* __restri
 	return x[6 * a.slot[r] + k] * a.D[6 * r + k];
 }
 
+// A solve that broke down on a non-finite sum gives du = 0, as the
+// reference's failed factorisation does (PrimalDualForceSolver.cpp:390-394).
 __device__ __forceinline__ void du_tile(const SArgs& a, const double* __restrict__ x, double* __restrict__ gdu,
                                         int tix) {
 	const Tile tile = a.tb[tix];
 	if (tile.scene < 0 || !a.ss[tile.scene].run) return;
+	const bool bad = a.ps[tile.scene].bad;
 	if ((int)threadIdx.x < tile.count) {
 		const int r = tile.first + threadIdx.x;
 		for (int k = 0; k < 6; ++k) {
-			a.du[6 * r + k] = body_du(a, x, r, k);
+			a.du[6 * r + k] = bad ? 0.0 : body_du(a, x, r, k);
 			gdu[6 * r + k] = -a.ru[6 * r + k];
 		}
 	}
@@ -1824,12 +1845,13 @@ __device__ __forceinline__ void du_tile(const SArgs& a, const double* __restrict
 __device__ __forceinline__ void increments_tile(const SArgs& a, const double* __restrict__ x, int tix) {
 	const Tile tile = a.tc[tix];
 	if (tile.scene < 0 || !a.ss[tile.scene].run) return;
+	const bool bad = a.ps[tile.scene].bad;
 	if ((int)threadIdx.x < tile.count) {
 		const int i = tile.first + threadIdx.x;
 		double du0[6], du1[6];
 		for (int k = 0; k < 6; ++k) {
-			du0[k] = body_du(a, x, a.li0[i], k);
-			du1[k] = body_du(a, x, a.li1[i], k);
+			du0[k] = bad ? 0.0 : body_du(a, x, a.li0[i], k);
+			du1[k] = bad ? 0.0 : body_du(a, x, a.li1[i], k);
 		}
 		double u[3];
 		to_local(a, i, du0, du1, u);
@@ -2133,9 +2155,15 @@ __global__ void k_newton_end(SArgs a, int maxit, int iterations) {
 	const int iters = P.done ? P.iters : maxit;
 	S->pcg_total += iters;
 	if (iters == maxit) S->pcg_hits += 1;
+	if (P.done && P.bad) S->nonfinite += 1;
+	// The x1.3 rule follows a capped iterative solve, which the reference runs
+	// only for 2000 or more bodies in contact; below that it factorises, its
+	// iteration count stays 0 and a full step always gives Hu / 4
+	// (PrimalDualForceSolver.cpp:379-394, 473-479).
+	const bool ref_iterative = a.lbeg[sc + 1] - a.lbeg[sc] >= 2000;
 	double Hu = S->Hu;
 	const bool useGD = S->ls_state == LS_GD_DECIDED;
-	if (!useGD && S->ls_step == 1.0) Hu = iters == maxit ? Hu * 1.3 : Hu / 4;
+	if (!useGD && S->ls_step == 1.0) Hu = (iters == maxit && ref_iterative) ? Hu * 1.3 : Hu / 4;
 	else Hu *= 4;
 	S->Hu = fmax(Hu, 1e-6);
 	S->it += 1;
@@ -2157,6 +2185,7 @@ __global__ void k_scene_init(SArgs a) {
 	S->trials = 0;
 	S->pcg_total = 0;
 	S->pcg_hits = 0;
+	S->nonfinite = 0;
 	S->exitErr = 0;
 	a.ps[sc].run = S->run;
 }
@@ -2518,6 +2547,7 @@ static bool pcg_solve_async(pdc_context* ctx, const PcgArgs<T>& pa, int iter, st
 	std::vector<int> iters(S, -1), a_it(S, -1);
 	std::vector<double> a_rel(S, 0.0);
 	int last_pos = -1, idle_polls = 0;
+	auto moved = std::chrono::steady_clock::now();
 	bool synced = false;
 	for (;;) {
 		bool pending = false, any = false;
@@ -2566,15 +2596,29 @@ static bool pcg_solve_async(pdc_context* ctx, const PcgArgs<T>& pa, int iter, st
 		// estimated end (at least ahead_min past the device)
 		const int target = std::max(pos + ahead_min, std::min(pos + ahead_max, total));
 		if (launched < target) enqueue_to(target);
-		// the device's position has not moved for a while: queue more if the stream has drained
+		// the device's position has not moved for a while: queue more if the
+		// stream has drained; end the call if the stream failed or stayed
+		// silent past the limit
 		if (pos == last_pos) {
 			if (++idle_polls >= 256) {
 				idle_polls = 0;
-				if (hipStreamQuery(s) == hipSuccess) enqueue_to(launched + chunk);
+				const hipError_t q = hipStreamQuery(s);
+				const StreamState st = q == hipSuccess ? StreamState::Idle
+				                       : q == hipErrorNotReady ? StreamState::Busy : StreamState::Failed;
+				const double idle = std::chrono::duration<double>(std::chrono::steady_clock::now() - moved).count();
+				switch (stall_action(st, idle, kStallLimitSeconds)) {
+				case StallAction::QueueMore: enqueue_to(launched + chunk); break;
+				case StallAction::Fail:
+					PDC_HIP(q);
+					throw HipError("PCG: no progress from the device for " + std::to_string((int)idle) + " s",
+					               PDC_ERR_DEVICE);
+				case StallAction::Wait: break;
+				}
 			}
 		} else {
 			last_pos = pos;
 			idle_polls = 0;
+			moved = std::chrono::steady_clock::now();
 		}
 		__builtin_ia32_pause();
 	}
@@ -2858,6 +2902,7 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		st.char_mass = H.char_mass;
 		st.char_speed = H.char_speed;
 		st.n_blocks = ctx->h_nnzb[sc];
+		st.nonfinite = H.nonfinite;
 		itmax = std::max(itmax, H.it);
 	}
 	if (S == 1) ctx->scene_stats[0].ties = ctx->n_ties;
@@ -2880,7 +2925,8 @@ pdc_solve_stats aggregate_stats(const pdc_context* ctx) {
 		t.gd_fallbacks += st.gd_fallbacks;
 		t.line_search_trials += st.line_search_trials;
 		t.n_blocks += st.n_blocks;
-		t.exit_error = std::max(t.exit_error, st.exit_error);
+		t.nonfinite += st.nonfinite;
+		t.exit_error = (t.exit_error < st.exit_error || st.exit_error != st.exit_error) ? st.exit_error : t.exit_error;
 		if (first && st.n_contacts > 0) {
 			t.char_mass = st.char_mass;
 			t.char_speed = st.char_speed;

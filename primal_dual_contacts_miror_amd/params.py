@@ -50,6 +50,7 @@ class PdcSolveStats(ctypes.Structure):
         ("line_search_trials", ctypes.c_int32),
         ("ties", ctypes.c_int32),
         ("n_blocks", ctypes.c_int32),
+        ("nonfinite", ctypes.c_int32),
         ("exit_error", ctypes.c_double),
         ("char_mass", ctypes.c_double),
         ("char_speed", ctypes.c_double),
