@@ -72,8 +72,8 @@ typedef struct pdc_params {
 	int32_t iterations;         /* Newton iteration cap */
 	int32_t pcg_precision;      /* PDC_PCG_FP64 | PDC_PCG_FP32 */
 	double pcg_tol;             /* relative residual |r|/|b| of the scaled system; at or above
-	                               1e-9 the single-reduction PCG form is used (one launch per
-	                               iteration), below it the two-reduction form */
+	                               1e-9 the single-reduction PCG recurrences are used, below it
+	                               the standard PCG iteration (both one launch per iteration) */
 	int32_t pcg_max_iter;       /* the reference's maxCGIt = 1000 (PrimalDualForceSolver.cpp:381) */
 	int32_t grid_n;             /* SDF grid nodes per axis (64, 96, ...) */
 	double grid_pad_frac;       /* grid box = shape half-extent * (1 + grid_pad_frac) */

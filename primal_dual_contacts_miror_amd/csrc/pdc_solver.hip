@@ -1344,9 +1344,13 @@ struct PcgArgs {
 	double* q;
 	double* s;
 	double* p;
+	double* rv[2];   // standard form: r and q = A p by iteration parity
+	double* qv[2];
 	double* gam[2];  // partials, one per tile, by iteration parity
 	double* del[2];
 	double* rr[2];
+	double* e1[2];   // standard form: (q, M r) and (q, M q)
+	double* e2[2];
 	double* bb;
 	double* hist;    // per scene: gamma, alpha of iteration i at [4 s + 2 (i & 1)], [+ 1]
 	PcgTile* tiles;         // one record per workgroup (kPcgRowsPerWG block rows)
@@ -1676,143 +1680,246 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	}
 }
 
-// Two-reduction form (standard PCG), used for tolerances below the attainable
-// accuracy of the single-reduction recurrences.  It runs on the same stored
-// operator B = A M through y with p = M y, so that A p = B y:
-//   init: x = x0, r = b - A x0 (as above), y = q = 0; gamma = (r, M r), |r|^2
-//   A(i): stop when |r|^2 <= tol^2 |b|^2; beta = gamma / gamma_prev (0 at i = 0)
-//         y = r + beta y, q = B r + beta q (= A p), partial (M y, q)
-//   B(i): alpha = gamma / (p, q); x += alpha M y, r -= alpha q,
-//         partials of (r, M r) and |r|^2 for iteration i + 1
-// gamma partials are double-buffered by parity (slot i & 1 holds gamma_i, the
-// other slot gamma_{i-1} until B(i) overwrites it).  The six lanes of a block
-// row read the whole 6-vector of their block before any of them writes, which
-// a single wavefront executes in program order.
+// Standard PCG in one launch per iteration, for tolerances below the
+// attainable accuracy of the single-reduction recurrences.  Its vectors follow
+// the textbook iteration (x += alpha p, r -= alpha q with q = A p from an
+// SpMV), so that it reaches the accuracy of a standard PCG; only the scalar
+// gamma that beta needs in the same launch is formed from an exact identity.
+// It runs on the stored operator B = A M through y with p = M y, A p = B y.
+//   F(0)  (k_pcgs_init, after k_pcg_start): y0 = r0, q0 = B r0,
+//         partials of delta0 = (M y0, q0), e1 = (q0, M r0), e2 = (q0, M q0),
+//         gamma0 = (r0, M r0), |r0|^2 and |b|^2
+//   F(i), i >= 1 (k_pcgs_iter), from the totals of F(i-1):
+//         stop when |r_{i-1}|^2 <= tol^2 |b|^2 (x_{i-1} is the solution);
+//         alpha = gamma_{i-1} / delta_{i-1}
+//         x_i = x_{i-1} + alpha M y_{i-1},  r_i = r_{i-1} - alpha q_{i-1}
+//         gamma_i = (r_i, M r_i) = gamma_{i-1} - 2 alpha e1 + alpha^2 e2,
+//           beta = gamma_i / gamma_{i-1}  (M symmetric)
+//         y_i = r_i + beta y_{i-1},  q_i = B r_i + beta q_{i-1}, where a row
+//           forms B r_i from its neighbours' r_{i-1} and q_{i-1}
+//         partials of delta_i, e1, e2, and gamma_i and |r_i|^2 summed
+//           directly, for the next launch
+// alpha always takes the directly summed gamma, so the identity's rounding
+// enters only one beta and does not accumulate.  r and q (read by
+// neighbouring rows) and the partials are double-buffered by iteration
+// parity; y and x are read and written by their own row only.  The count of
+// iterations is the count of x updates, as in the textbook loop: a solve
+// that converges from its start reports 0.
 template <class T>
-__global__ __launch_bounds__(kPcgThreads) void k_pcg2_init(PcgArgs<T> a) {
-	__shared__ double lds[4 * 16];
+__global__ __launch_bounds__(kPcgThreads) void k_pcgs_init(PcgArgs<T> a) {
+	__shared__ double lds[6 * 16];
 	PcgTile* Tp = a.tiles + blockIdx.x;
 	if (pcg_tile_skip(a, Tp)) return;
 	const int scn = Tp->scene, cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
-	double v[3] = {0, 0, 0};
-	if (wid < cnt && i < 6 && sl == 0) {
+	const int ir = i < 6 ? i : 5;
+	double v[6] = {0, 0, 0, 0, 0, 0};
+	if (wid < cnt) {
 		const int br = pcg_slot();
-		const int t = 6 * br + i;
-		double zt = 0;
+		const double* __restrict__ r0 = a.rv[0];
+		const double n = spmv_ell(a, r0, br, i, sl);
+		double qa[6], ra[6], pr[6];
 #pragma unroll
-		for (int c = 0; c < 6; ++c) zt += (double)a.P[36 * (size_t)br + 6 * i + c] * a.r[6 * br + c];
-		const double rt = a.r[t], bt = a.b[t];
-		a.p[t] = 0;  // y
-		a.q[t] = 0;
-		v[0] += rt * zt;
-		v[1] += rt * rt;
-		v[2] += bt * bt;
+		for (int cc = 0; cc < 6; ++cc) {
+			qa[cc] = __shfl(n, 8 * cc, 64);  // component cc of q0, from its row group
+			ra[cc] = r0[6 * br + cc];
+			pr[cc] = (double)a.P[36 * (size_t)br + 6 * ir + cc];
+		}
+		if (i < 6 && sl == 0) {
+			const int t = 6 * br + i;
+			double mr = 0, mq = 0;
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) {
+				mr += pr[cc] * ra[cc];
+				mq += pr[cc] * qa[cc];
+			}
+			const double qt = PDC_PICK6(qa, i), rt = PDC_PICK6(ra, i), bt = a.b[t];
+			a.qv[0][t] = qt;
+			a.p[t] = rt;  // y0 = r0
+			v[0] = mr * qt;
+			v[1] = qt * mr;
+			v[2] = qt * mq;
+			v[3] = rt * mr;
+			v[4] = rt * rt;
+			v[5] = bt * bt;
+		}
 	}
-	double out[3];
-	block_sum<3, double>(v, lds, out);
+	double out[6];
+	block_sum<6, double>(v, lds, out);
 	if (threadIdx.x == 0) {
-		a.gam[0][blockIdx.x] = out[0];
-		a.rr[0][blockIdx.x] = out[1];
-		a.bb[blockIdx.x] = out[2];
+		a.del[0][blockIdx.x] = out[0];
+		a.e1[0][blockIdx.x] = out[1];
+		a.e2[0][blockIdx.x] = out[2];
+		a.gam[0][blockIdx.x] = out[3];
+		a.rr[0][blockIdx.x] = out[4];
+		a.bb[blockIdx.x] = out[5];
 	}
 	pcg_tile_reset(a, Tp, scn);
 }
 
-template <class T>
-__global__ __launch_bounds__(kPcgThreads) void k_pcg2_a(PcgArgs<T> a, int slot) {
-	__shared__ double lds[4 * 16];
-	(void)slot;
+// One iteration F(i), i >= 1, launch slot = parity of i - 1.  Memory round
+// trips as in k_pcg_iter: (1) the record, the ELL columns, the
+// preconditioner row and the row's r, q, y, x; (2) the partials of F(i-1)
+// (six arrays, one per wavefront), the operator blocks and the neighbours' r
+// and q of the occupied entries.  B r_i = B r_{i-1} - alpha B q_{i-1} is
+// summed as two sums before alpha is known.  EM as in k_pcg_iter.
+template <class T, int EM>
+__global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slot) {
+	__shared__ double lds[8];
+	static_assert(kPcgRowsPerWG >= 6, "six partial arrays, one per wavefront");
 	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
-	const int scn = Tp->scene;
-	if (scn < 0 || Tp->done || Tp->it >= a.maxit) return;
-	const int it = Tp->it, t0 = Tp->t0, W = Tp->W, cnt = Tp->count;
-	const int cur = it & 1, nxt = cur ^ 1;
-	double red[4];
-	reduce_partials<4>(pick(a.rr, cur), a.bb, pick(a.gam, cur), pick(a.gam, nxt), t0, W, red, lds);
-	const bool bad = !(isfinite(red[0]) && isfinite(red[1]) && isfinite(red[2]));
-	if (bad || !(red[0] > a.tol2 * red[1])) {
-		if (threadIdx.x == 0) pcg_finish(a, Tp, scn, it, bad, (int)blockIdx.x == t0);
+	const int* __restrict__ rp4 = reinterpret_cast<const int*>(Tp);
+	int rec[8];
+#pragma unroll
+	for (int k = 0; k < 8; ++k) rec[k] = rp4[k];
+	const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+	const int lane = threadIdx.x & 63;
+	const int i = lane >> 3, sl = lane & 7;
+	const int ir = i < 6 ? i : 5;
+	const int br = kPcgRowsPerWG * (int)blockIdx.x + wid;
+	const int cur = slot, nxt = slot ^ 1;
+	const double* __restrict__ rc = pick(a.rv, cur);
+	const double* __restrict__ qc = pick(a.qv, cur);
+	// round 1
+	constexpr int kE = EM > 0 ? EM : 1;
+	const int R = EM > 0 ? 8 * EM : a.R;
+	int ec[kE];
+#pragma unroll
+	for (int m = 0; m < kE; ++m) ec[m] = a.ecol[(size_t)br * R + sl + 8 * m];
+	double pr[6], rp[6], qp[6], yp[6];
+#pragma unroll
+	for (int cc = 0; cc < 6; ++cc) {
+		pr[cc] = (double)a.P[36 * (size_t)br + 6 * ir + cc];
+		rp[cc] = rc[6 * br + cc];
+		qp[cc] = qc[6 * br + cc];
+		yp[cc] = a.p[6 * br + cc];
+	}
+	const int t = 6 * br + ir;
+	const double xo = a.x[t];
+	const int scn = rec[0], cnt = rec[2], t0 = rec[3], W = rec[4], done = rec[5], it = rec[6];
+	if (scn < 0 || done || it >= a.maxit) return;
+	// round 2: partial sums of F(i-1), operator blocks and neighbour values
+	const double* psrc = wid == 0 ? pick(a.del, cur) : wid == 1 ? pick(a.e1, cur) : wid == 2 ? pick(a.e2, cur)
+	                   : wid == 3 ? pick(a.gam, cur) : wid == 4 ? pick(a.rr, cur) : a.bb;
+	psrc += t0;
+	double pv[kPartLoads];
+	if (wid < 6) {
+#pragma unroll
+		for (int k = 0; k < kPartLoads; ++k) {
+			pv[k] = 0;
+			if (64 * k < W) pv[k] = psrc[min(lane + 64 * k, W - 1)];
+		}
+	}
+	double nr = 0, nq = 0;  // row i of B r_{i-1} and of B q_{i-1}
+	if (i < 6) {
+#pragma unroll
+		for (int m = 0; m < kE; ++m) {
+			const int c = ec[m];
+			if (c >= 0) {
+				const T* blk = a.B + 36 * ((size_t)br * R + sl + 8 * m) + 6 * i;
+				const double* vr = rc + 6 * (size_t)c;
+				const double* vq = qc + 6 * (size_t)c;
+				double dr = 0, dq = 0;
+#pragma unroll
+				for (int cc = 0; cc < 6; ++cc) {
+					const double bv = (double)blk[cc];
+					dr += bv * vr[cc];
+					dq += bv * vq[cc];
+				}
+				nr += dr;
+				nq += dq;
+			}
+		}
+		if (EM == 0)
+			for (int e = sl + 8; e < R; e += 8) {
+				const size_t k = (size_t)br * R + e;
+				const int c = a.ecol[k];
+				if (c < 0) continue;
+				const T* blk = a.B + 36 * k + 6 * i;
+				const double* vr = rc + 6 * (size_t)c;
+				const double* vq = qc + 6 * (size_t)c;
+				double dr = 0, dq = 0;
+#pragma unroll
+				for (int cc = 0; cc < 6; ++cc) {
+					const double bv = (double)blk[cc];
+					dr += bv * vr[cc];
+					dq += bv * vq[cc];
+				}
+				nr += dr;
+				nq += dq;
+			}
+	}
+	nr = group8_sum(nr);
+	nq = group8_sum(nq);
+	// M y_{i-1} of the row (the x update)
+	double myp = 0;
+#pragma unroll
+	for (int cc = 0; cc < 6; ++cc) myp += pr[cc] * yp[cc];
+	if (wid < 6) {
+		double part = 0;
+#pragma unroll
+		for (int k = 0; k < kPartLoads; ++k) part += lane + 64 * k < W ? pv[k] : 0.0;
+		if (W > 64 * kPartLoads) part += partial_lane_sum(psrc + 64 * kPartLoads, W - 64 * kPartLoads);
+		part = wave_sum_dpp(part);
+		if (lane == 0) lds[wid] = part;
+	}
+	__syncthreads();
+	const double delta = lds[0], e1 = lds[1], e2 = lds[2], gamma = lds[3], rr = lds[4], bb = lds[5];
+	const bool first_tile = (int)blockIdx.x == t0;
+	const bool bad = !(isfinite(rr) && isfinite(bb) && isfinite(gamma) && isfinite(delta) && isfinite(e1) &&
+	                   isfinite(e2));
+	if (bad || !(rr > a.tol2 * bb)) {
+		if (threadIdx.x == 0) pcg_finish(a, Tp, scn, it, bad, first_tile);
 		return;
 	}
-	if (threadIdx.x == 0 && (int)blockIdx.x == t0) {
-		a.prog[scn].rel2 = red[0] / red[1];
-		a.prog[scn].it = it + 1;
+	const double alpha = gamma / delta;
+	const double beta = (gamma - 2.0 * alpha * e1 + alpha * alpha * e2) / gamma;
+	const double n = nr - alpha * nq;  // row i of B r_i
+	double rn[6], yn[6];
+#pragma unroll
+	for (int cc = 0; cc < 6; ++cc) {
+		rn[cc] = rp[cc] - alpha * qp[cc];
+		yn[cc] = rn[cc] + beta * yp[cc];
 	}
-	const double beta = it > 0 ? red[2] / red[3] : 0.0;
-	double* __restrict__ y = a.p;
-	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-	const int i = lane >> 3, sl = lane & 7;
-	double v[1] = {0};
-	if (wid < cnt) {
-		const int br = pcg_slot();
-		const double n = spmv_ell(a, a.r, br, i, sl);
-		if (i < 6 && sl == 0) {
-			const int t = 6 * br + i;
-			double yb[6];
+	const double qn = n + beta * PDC_PICK6(qp, ir);
+	double qa[6];
 #pragma unroll
-			for (int cc = 0; cc < 6; ++cc) yb[cc] = a.r[6 * br + cc] + beta * y[6 * br + cc];
-			double pt = 0;
-#pragma unroll
-			for (int cc = 0; cc < 6; ++cc) pt += (double)a.P[36 * (size_t)br + 6 * i + cc] * yb[cc];
-			double yt = yb[0];
-#pragma unroll
-			for (int cc = 1; cc < 6; ++cc) yt = cc == i ? yb[cc] : yt;
-			const double qt = n + beta * a.q[t];
-			y[t] = yt;
-			a.q[t] = qt;
-			v[0] += pt * qt;
-		}
-	}
-	double out[1];
-	block_sum<1, double>(v, lds, out);
-	if (threadIdx.x == 0) a.del[0][blockIdx.x] = out[0];
-}
-
-template <class T>
-__global__ __launch_bounds__(kPcgThreads) void k_pcg2_b(PcgArgs<T> a, int slot) {
-	__shared__ double lds[4 * 16];
-	(void)slot;
-	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
-	const int scn = Tp->scene;
-	if (scn < 0 || Tp->done || Tp->it >= a.maxit) return;
-	const int it = Tp->it, t0 = Tp->t0, W = Tp->W, cnt = Tp->count;
-	const int cur = it & 1, nxt = cur ^ 1;
-	double red[2];
-	reduce_partials<2>(pick(a.gam, cur), a.del[0], nullptr, nullptr, t0, W, red, lds);
-	const double alpha = red[0] / red[1];
-	const double* __restrict__ y = a.p;
-	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-	const int i = lane >> 3, sl = lane & 7;
-	double v[2] = {0, 0};
+	for (int cc = 0; cc < 6; ++cc) qa[cc] = __shfl(qn, 8 * cc, 64);
+	double v[5] = {0, 0, 0, 0, 0};
 	if (wid < cnt && i < 6 && sl == 0) {
-		const int br = pcg_slot();
-		const int t = 6 * br + i;
-		double rb[6], pt = 0;
+		double mr = 0, my = 0, mq = 0;
 #pragma unroll
 		for (int cc = 0; cc < 6; ++cc) {
-			const double pc = (double)a.P[36 * (size_t)br + 6 * i + cc];
-			pt += pc * y[6 * br + cc];
-			rb[cc] = a.r[6 * br + cc] - alpha * a.q[6 * br + cc];
+			mr += pr[cc] * rn[cc];
+			my += pr[cc] * yn[cc];
+			mq += pr[cc] * qa[cc];
 		}
-		double zt = 0;
-#pragma unroll
-		for (int cc = 0; cc < 6; ++cc) zt += (double)a.P[36 * (size_t)br + 6 * i + cc] * rb[cc];
-		double rt = rb[0];
-#pragma unroll
-		for (int cc = 1; cc < 6; ++cc) rt = cc == i ? rb[cc] : rt;
-		a.x[t] += alpha * pt;
-		a.r[t] = rt;
-		v[0] += rt * zt;
-		v[1] += rt * rt;
+		const double rt = PDC_PICK6(rn, i), yt = PDC_PICK6(yn, i);
+		a.x[t] = xo + alpha * myp;
+		pick(a.rv, nxt)[t] = rt;
+		pick(a.qv, nxt)[t] = qn;
+		a.p[t] = yt;
+		v[0] = my * qn;
+		v[1] = qn * mr;
+		v[2] = qn * mq;
+		v[3] = rt * mr;
+		v[4] = rt * rt;
 	}
-	double out[2];
-	block_sum<2, double>(v, lds, out);
+	__shared__ double lds_b[5 * kPcgRowsPerWG];
+	double out[5];
+	block_sum_last<5, kPcgRowsPerWG>(v, lds_b, out);
 	if (threadIdx.x == 0) {
-		pick(a.gam, nxt)[blockIdx.x] = out[0];
-		pick(a.rr, nxt)[blockIdx.x] = out[1];
+		pick(a.del, nxt)[blockIdx.x] = out[0];
+		pick(a.e1, nxt)[blockIdx.x] = out[1];
+		pick(a.e2, nxt)[blockIdx.x] = out[2];
+		pick(a.gam, nxt)[blockIdx.x] = out[3];
+		pick(a.rr, nxt)[blockIdx.x] = out[4];
 		Tp->it = it + 1;
+		if (first_tile) {
+			a.prog[scn].rel2 = rr / bb;
+			a.prog[scn].it = it + 1;
+		}
 	}
 }
 
@@ -2297,7 +2404,13 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 	const dim3 B(kPcgThreads);
 	for (int k = 0; k < chunk; ++k) {
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k], s));
-		if (two) hipLaunchKernelGGL(k_pcg2_a<T>, dim3(nblk), B, 0, s, pa, k & 1);
+		if (two) switch (pa.R / 8) {
+			case 1: hipLaunchKernelGGL((k_pcgs_iter<T, 1>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 2: hipLaunchKernelGGL((k_pcgs_iter<T, 2>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 3: hipLaunchKernelGGL((k_pcgs_iter<T, 3>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 4: hipLaunchKernelGGL((k_pcgs_iter<T, 4>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			default: hipLaunchKernelGGL((k_pcgs_iter<T, 0>), dim3(nblk), B, 0, s, pa, k & 1); break;
+		}
 		else switch (pa.R / 8) {
 			case 1: hipLaunchKernelGGL((k_pcg_iter<T, 1>), dim3(nblk), B, 0, s, pa, k & 1); break;
 			case 2: hipLaunchKernelGGL((k_pcg_iter<T, 2>), dim3(nblk), B, 0, s, pa, k & 1); break;
@@ -2306,12 +2419,11 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 			default: hipLaunchKernelGGL((k_pcg_iter<T, 0>), dim3(nblk), B, 0, s, pa, k & 1); break;
 		}
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k + 1], s));
-		if (two) hipLaunchKernelGGL(k_pcg2_b<T>, dim3(nblk), B, 0, s, pa, k & 1);
 	}
 	PDC_CHECK_LAUNCH();
 }
 
-// Below this relative tolerance the two-reduction form is used: the
+// Below this relative tolerance the standard form (k_pcgs_*) is used: the
 // single-reduction recurrences stall near 1e-10 on ill-conditioned systems.
 constexpr double kSingleReductionMinTol = 1e-9;
 
@@ -2323,7 +2435,7 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 	const int nblk = std::max(1, ctx->cap_tr);
 	const size_t ncap = 6 * (size_t)kPcgRowsPerWG * nblk, pcap = (size_t)nblk;
 	DBuf<double>& work = ctx->pcg64;
-	work.ensure(7 * ncap + 7 * pcap + 4 * (size_t)S + 64);
+	work.ensure(7 * ncap + 11 * pcap + 4 * (size_t)S + 64);
 	PcgArgs<T> pa;
 	std::memset(&pa, 0, sizeof(pa));  // compared bytewise with the recorded arguments
 	pa.ecol = ctx->e_col.p;
@@ -2340,7 +2452,12 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 	pa.del[0] = parts + 2 * pcap; pa.del[1] = parts + 3 * pcap;
 	pa.rr[0] = parts + 4 * pcap; pa.rr[1] = parts + 5 * pcap;
 	pa.bb = parts + 6 * pcap;
-	pa.hist = parts + 7 * pcap;
+	pa.e1[0] = parts + 7 * pcap; pa.e1[1] = parts + 8 * pcap;
+	pa.e2[0] = parts + 9 * pcap; pa.e2[1] = parts + 10 * pcap;
+	pa.hist = parts + 11 * pcap;
+	// the standard form keeps r, q by parity in the recurrence form's r, z and q, s
+	pa.rv[0] = pa.r; pa.rv[1] = pa.z;
+	pa.qv[0] = pa.q; pa.qv[1] = pa.s;
 	pa.tiles = ctx->t_pcg.p;
 	pa.ss = ctx->d_ss.p;
 	pa.ps = ctx->d_ps.p;
@@ -2358,20 +2475,20 @@ static void pcg_begin(pdc_context* ctx, const PcgArgs<T>& pa) {
 	const int nblk = std::max(1, ctx->cap_tr);
 	hipStream_t s = ctx->stream;
 	hipLaunchKernelGGL(k_pcg_start<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa, (const double*)ctx->y64.p);
-	if (ctx->p.pcg_tol < kSingleReductionMinTol) hipLaunchKernelGGL(k_pcg2_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
+	if (ctx->p.pcg_tol < kSingleReductionMinTol) hipLaunchKernelGGL(k_pcgs_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
 	else hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
 	PDC_CHECK_LAUNCH();
 }
 
 // Algorithmic bytes of the scenes' active PCG iterations for the kernel
-// timing hooks: per iteration of a scene (of k_pcg_iter, or of k_pcg2_a for
-// the two-reduction form), each input read once and each output written
-// once, FP64 vectors:
-//   k_pcg_iter  B blocks and column indices nnzb (36 T + 4), row pointers
-//               4 (NL + 1), P blocks 36 T NL, w, z, s, p, x, r read
-//               6 x 8 n6, z, s, p, x, r, w' written 6 x 8 n6
-//   k_pcg2_a    B, column indices, row pointers as above, P 36 T NL,
-//               r, y, q read 3 x 8 n6, y, q written 2 x 8 n6
+// timing hooks: per iteration of a scene (one launch of k_pcg_iter, or of
+// k_pcgs_iter for the standard form), each input read once and each output
+// written once, FP64 vectors:
+//   k_pcg_iter   B blocks and column indices nnzb (36 T + 4), row pointers
+//                4 (NL + 1), P blocks 36 T NL, w, z, s, p, x, r read
+//                6 x 8 n6, z, s, p, x, r, w' written 6 x 8 n6
+//   k_pcgs_iter  B, column indices, row pointers, P as above, r, q, y, x
+//                read 4 x 8 n6, r', q', y, x written 4 x 8 n6
 template <class T>
 static void pcg_account_bytes(pdc_context* ctx, const std::vector<int>& iters, bool two) {
 	const double Tb = sizeof(T);
@@ -2379,7 +2496,7 @@ static void pcg_account_bytes(pdc_context* ctx, const std::vector<int>& iters, b
 		if (iters[sc] <= 0) continue;
 		const double NLs = ctx->h_lbeg[sc + 1] - ctx->h_lbeg[sc], n6 = 6.0 * NLs, nnzb = ctx->h_nnzb[sc];
 		const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NLs + 1) + 36.0 * Tb * NLs;
-		ctx->kt.bytes += iters[sc] * (mat + (two ? 40.0 : 96.0) * n6);
+		ctx->kt.bytes += iters[sc] * (mat + (two ? 64.0 : 96.0) * n6);
 	}
 }
 
