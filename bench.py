@@ -2,14 +2,23 @@
 pile at ~30k contacts per step.
 
 A step is one full timestep on the device-resident scene: grid-SDF contact
-generation, the primal-dual Newton solve (FP64 outer loop, FP32 block-Jacobi
-PCG inner solve, FP64 merit line search) and integration.  The timed region
+generation, the primal-dual Newton solve (FP64 outer loop, block-Jacobi PCG
+inner solve, FP64 merit line search) and integration.  The timed region
 starts from the settled state of the seeded pile (the state after the
 workload's 1000 steps), with every input already in HBM.
 
+Settings of the inner solve:
+  tight (default, the headline)  FP64 operator, relative residual 1e-12: the
+        step is then defined to well below 1e-6 (the oracle's own step moves
+        by ~2e-9 of the body size between inner tolerances 1e-12 and 1e-13),
+        and the device matches the FP64 restatement within 1e-6
+  fast  FP32 operator, relative residual 1e-5 (reported beside the headline
+        with its error): at the pile's Newton tolerance the step is then
+        defined only to ~1e-4 of the body size
+
 Prints one JSON line.  The CPU baseline is the FP64 restatement of the same
 algorithm (oracle/), run on the host cores on a bounded sample of steps from
-the same state.
+the same state with the same parameters.
 """
 import argparse
 import json
@@ -23,17 +32,40 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from primal_dual_contacts_miror_amd import pdc, scenes, serve  # noqa: E402
+from primal_dual_contacts_miror_amd.params import PCG_FP32, PCG_FP64  # noqa: E402
 
 METRIC = "Newton timesteps/sec at 30k contacts; batched scene-steps/sec + p50/p99 latency"
 PEAK_HBM = 8.0e12
 # kernel classes timed with HIP events (include/pdc.h PDC_KT_*)
 KT_NAMES = {"pcg": pdc.KT_PCG, "pcg_per_launch": pdc.KT_SPMV, "contact_terms": pdc.KT_CONTACT_TERMS,
             "contacts": pdc.KT_CONTACTS}
-KT_KERNEL = {"pcg": "k_pcg_iter (one PCG iteration: block-CSR SpMV with A P, block-Jacobi apply, vector "
-                    "updates, partial dot products), timed by one event pair around each solve's launches, "
-                    "queued %d at a time (the launches that find the solve converged included)",
-             "pcg_per_launch": "k_pcg_iter, launched without graph replay, timed per launch",
-             "contact_terms": "k_constraints + k_contact_res", "contacts": "k_pair_hits"}
+# inner-solve settings: operator precision and relative tolerance
+SETTINGS = {"tight": (PCG_FP64, 1e-12), "fast": (PCG_FP32, 1e-5)}
+
+
+def pcg_kernel_name(p):
+    return "k_pcgs_iter" if p.pcg_tol < 1e-9 else "k_pcg_iter"
+
+
+def kernel_description(which, p, chunk):
+    name = pcg_kernel_name(p)
+    if which == "pcg":
+        return ("%s (one PCG iteration: ELL block SpMV with A P, block-Jacobi apply, vector updates, "
+                "partial dot products; %s operator), timed by one event pair around each solve's launches, "
+                "queued %d at a time (the launches that find the solve converged included)"
+                % (name, "FP64" if p.pcg_precision == PCG_FP64 else "FP32", chunk))
+    return {"pcg_per_launch": "%s, launched without graph replay, timed per launch" % name,
+            "contact_terms": "k_constraints + k_contact_res", "contacts": "k_pair_hits"}[which]
+
+
+def survey_pcg_bytes(p, nnzb, nbodies):
+    """Algorithmic bytes of one PCG iteration by the hot-path table's per-unit
+    figure: nnzb (36 e + 4) + (N + 1) 4 + N (12 e [SpMV x, y] + 48 e
+    [block-Jacobi block, its input and output] + 6 e x 13 [4 dots, 9 axpys]),
+    with e the element size (4 for the FP32 operator and vectors, 8 for
+    FP64): ~ nnzb 148 + N 552 at e = 4, nnzb 292 + N 1104 at e = 8."""
+    e = 8.0 if p.pcg_precision == PCG_FP64 else 4.0
+    return nnzb * (36 * e + 4) + (nbodies + 1) * 4 + nbodies * (12 * e + 48 * e + 6 * e * 13)
 
 
 def log(*a):
@@ -58,19 +90,24 @@ def settled_state(sc, name, settle_steps=None, path=None):
     return st, "device-settled"
 
 
-def cpu_baseline(sc, state, budget_s, threads, first_gpu=None):
-    """CPU leg; with first_gpu (the device state one step after `state`) also
-    the precision statement of that step against the FP64 restatement."""
+def cpu_baseline(sc, state, budget_s, threads, checks):
+    """CPU leg: the FP64 restatement timed from the same settled state with
+    the headline's parameters (sc.params).  It is also the reference of the
+    precision statements: `checks` lists (name, scene, parameters, start
+    state, device state one step later, reference parameters or None); each
+    gets the errors of the device step against the restatement's step at
+    the reference parameters (the same parameters when None).  A check
+    without a device state compares the restatement's own step at the
+    parameters with the reference."""
     from oracle import orc
-    # same parameters as the device run (tolerances, PCG branch and its tolerance), so both do the same work
     o = orc.OracleScene(sc, sc.params)
     o.set_threads(threads)
     o.set_state(*state)
     t0 = time.perf_counter()
     n = 0
     contacts = 0
-    precision = None
     step_ms = []
+    first = None
     while True:
         ts = time.perf_counter()
         st = o.step(1)
@@ -78,25 +115,39 @@ def cpu_baseline(sc, state, budget_s, threads, first_gpu=None):
         n += 1
         contacts += st.n_contacts
         el = time.perf_counter() - t0
-        if n == 1 and first_gpu is not None:
-            t_pause = time.perf_counter()
-            precision = step_precision(sc, first_gpu, o.get_state(), state, threads)
-            t0 += time.perf_counter() - t_pause
+        if n == 1:
+            first = tuple(x.copy() for x in o.get_state())
         log("cpu baseline step %d: %d contacts, %.2fs" % (n, st.n_contacts, el))
         if el > budget_s or n >= 50:
             break
-    return precision, {
+    baseline = {
         "value": n / el,
         "unit": "timesteps/s",
         "cores": threads,
         "kind": "port",
         "sample": "%d timestep(s) of the FP64 CPU restatement (OpenMP) from the same settled state, same "
-                  "parameters (%d contacts/step avg; block-Jacobi PCG branch, pcg_tol %g)" % (
+                  "parameters (%d contacts/step avg; inner solve: block-Jacobi PCG to %g)" % (
                       n, contacts // n, sc.params.pcg_tol),
         "median_step_ms": float(np.median(step_ms)),
         "p99_step_ms": float(np.percentile(step_ms, 99)),
         "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()},
     }
+    def restatement_step(scn, q, start):
+        if start is state and q is sc.params:
+            return first
+        r = orc.OracleScene(scn, q)
+        r.set_threads(threads)
+        r.set_state(*start)
+        r.step(1)
+        return r.get_state()
+
+    precision = {}
+    for name, scn, params, start, dev, ref_params in checks:
+        ref = restatement_step(scn, ref_params if ref_params is not None else params, start)
+        if dev is None:  # the restatement's own step at `params` against the reference
+            dev = restatement_step(scn, params, start)
+        precision.setdefault(name, []).append(state_errors(scn, dev, ref))
+    return precision, baseline, first
 
 
 def _cpu_model():
@@ -151,12 +202,15 @@ def measured_copy_rate():
     return best
 
 
-def batched_leg(n_scenes, steps, plain_scenes, plain_steps, seed_base=1000):
+def batched_leg(n_scenes, steps, plain_scenes, plain_steps, overrides, seed_base=1000):
     """Many independent seeded random piles (10-500 bodies, log-uniform) in
     one context, all stepped together from their seeded drop; then, at the
     same (contact-rich) state, the batched rate over a few steps against the
-    plain per-scene path (one scene per step call, one after another)."""
-    scs = [scenes.random_pile(seed_base + k) for k in range(n_scenes)]
+    plain per-scene path (one scene per step call, one after another).
+    Returns the leg's record and its precision checks (three scenes: the
+    largest, one of median size and the smallest, one step from the final
+    state, alone == in the batch)."""
+    scs = [scenes.random_pile(seed_base + k, **overrides) for k in range(n_scenes)]
     bs = pdc.BatchedSystem(scs)
     log("batched: %d scenes, %d bodies, %d steps" % (n_scenes, sum(s.n for s in scs), steps))
     t0 = time.perf_counter()
@@ -166,6 +220,7 @@ def batched_leg(n_scenes, steps, plain_scenes, plain_steps, seed_base=1000):
             log("batched step %d: %d contacts" % (k, sum(x.n_contacts for x in st)))
     el = time.perf_counter() - t0
     contacts = [x.n_contacts for x in st]
+    nonfinite = sum(x.nonfinite for x in st)
     tail = 10
     # the same tail of steps from the same state, with the default work
     # shapes and with the shapes tuned per size class at this state
@@ -190,12 +245,24 @@ def batched_leg(n_scenes, steps, plain_scenes, plain_steps, seed_base=1000):
     el_plain = time.perf_counter() - t0
     batched_tail = n_scenes * tail / el_tail
     plain = plain_scenes * plain_steps / el_plain
+    # precision checks: one step of three scenes from the current state
+    sizes = np.array([sc.n for sc in scs])
+    order = np.argsort(sizes)
+    picks = [int(order[-1]), int(order[len(order) // 2]), int(order[0])]
+    checks = []
+    for k in picks:
+        start = tuple(x.copy() for x in bs.state(k))
+        bs.step(1, [k])
+        checks.append(("batched", scs[k], scs[k].params, start, tuple(x.copy() for x in bs.state(k)), None))
     out = {
         "workload": "%d seeded random piles, 10-500 bodies (log-uniform), spheres + boxes, dt 0.005, 32^3 grids, "
-                    "FP32 PCG / FP64 Newton, stepped together from the seeded drop" % n_scenes,
+                    "FP64 Newton, inner PCG %s to %g, stepped together from the seeded drop" % (
+                        n_scenes, "FP64" if scs[0].params.pcg_precision == PCG_FP64 else "FP32",
+                        scs[0].params.pcg_tol),
         "steps": steps,
         "scene_steps_per_s": n_scenes * steps / el,
         "contacts_last_step": {"total": int(sum(contacts)), "max_per_scene": int(max(contacts))},
+        "nonfinite_iterations_last_step": int(nonfinite),
         "autotune": {
             "shapes": shapes.as_dict(),
             "seconds": el_tune,
@@ -209,11 +276,12 @@ def batched_leg(n_scenes, steps, plain_scenes, plain_steps, seed_base=1000):
             "sample": "%d batched steps of all scenes vs %d steps of each of %d scenes alone, same state" % (
                 tail, plain_steps, plain_scenes),
         },
+        "precision_scenes": ["%s (%d bodies)" % (scs[k].name, scs[k].n) for k in picks],
     }
-    return bs, out
+    return bs, out, checks
 
 
-def served_leg(n_scenes, settle_steps, rates, duration, plain_rate, seed_base=5000, seed=7):
+def served_leg(n_scenes, settle_steps, rates, duration, plain_rate, overrides, seed_base=5000, seed=7):
     """Continuous-batching server over resident scenes of its own: seeded
     random piles of 10-1000 bodies (log-uniform), admitted by HBM footprint,
     stepped together from their seeded drop for `settle_steps` steps so that
@@ -221,7 +289,7 @@ def served_leg(n_scenes, settle_steps, rates, duration, plain_rate, seed_base=50
     one-step requests at each rate, and the plain per-scene server (one
     request per step call) at `plain_rate`."""
     t0 = time.perf_counter()
-    scs = [scenes.random_pile(seed_base + k, n_min=10, n_max=1000) for k in range(n_scenes)]
+    scs = [scenes.random_pile(seed_base + k, n_min=10, n_max=1000, **overrides) for k in range(n_scenes)]
     srv = serve.SceneServer(scs, hbm_budget=200 * 2 ** 30)
     n = len(srv.scenes)
     bodies = sum(sc.n for sc in srv.scenes)
@@ -235,8 +303,10 @@ def served_leg(n_scenes, settle_steps, rates, duration, plain_rate, seed_base=50
             log("served settle step %d: %d contacts" % (k, sum(x.n_contacts for x in st)))
     settle_s = time.perf_counter() - t0
     out = {"workload": "%d seeded random piles of 10-1000 bodies (log-uniform), spheres + boxes, 32^3 grids, "
-                       "FP32 PCG / FP64 Newton, admitted by HBM footprint, %d steps from the seeded drop before "
-                       "the requests" % (n_scenes, settle_steps),
+                       "FP64 Newton, inner PCG %s to %g, admitted by HBM footprint, %d steps from the seeded "
+                       "drop before the requests" % (
+                           n_scenes, "FP64" if scs[0].params.pcg_precision == PCG_FP64 else "FP32",
+                           scs[0].params.pcg_tol, settle_steps),
            "resident_scenes": n, "rejected_scenes": len(srv.rejected), "bodies": bodies,
            "footprint_gib": srv.footprint / 2 ** 30,
            "contacts_resident": int(sum(x.n_contacts for x in st)) if st else 0,
@@ -256,69 +326,104 @@ def served_leg(n_scenes, settle_steps, rates, duration, plain_rate, seed_base=50
     summ["dropped"] = len(reqs) - summ["requests"]
     log("plain %g req/s: %s" % (plain_rate, summ))
     out["plain_per_scene"] = summ
+    # precision checks: one served request of three resident scenes
+    sizes = np.array([sc.n for sc in srv.scenes])
+    order = np.argsort(sizes)
+    picks = [int(order[-1]), int(order[len(order) // 2]), int(order[0])]
+    checks = []
+    for k in picks:
+        start = tuple(x.copy() for x in srv.system.state(k))
+        srv.run([serve.Request(k, 0.0)])
+        checks.append(("served", srv.scenes[k], srv.scenes[k].params, start,
+                       tuple(x.copy() for x in srv.system.state(k)), None))
+    out["precision_scenes"] = ["%s (%d bodies)" % (srv.scenes[k].name, srv.scenes[k].n) for k in picks]
     srv.close()
-    return out
+    return out, checks
 
 
-def _state_errors(sc, a, b):
+def rotation_angle(qa, qb):
+    """Angle of the rotation between unit quaternions (x, y, z, w), per row,
+    as 2 atan2(|v|, |w|) of qa^-1 qb (exact for small angles)."""
+    va, wa = qa[:, :3], qa[:, 3:4]
+    vb, wb = qb[:, :3], qb[:, 3:4]
+    v = wa * vb - wb * va - np.cross(va, vb)
+    return 2 * np.arctan2(np.linalg.norm(v, axis=1), np.abs(np.sum(qa * qb, axis=1)))
+
+
+def state_errors(sc, a, b):
+    """Maxima over the dynamic bodies: position / body size, rotation angle,
+    velocity / (|g| dt), angular velocity x body size / (|g| dt)."""
     dyn = sc.is_static == 0
     size = np.max(sc.dims, axis=1)[dyn]
-    ep = np.linalg.norm(a[0][dyn] - b[0][dyn], axis=1) / size
-    ea = 2 * np.arccos(np.clip(np.abs(np.sum(a[1][dyn] * b[1][dyn], axis=1)), 0, 1))
     gdt = float(np.linalg.norm(sc.params.gravity[:])) * sc.params.timestep
-    ev = np.linalg.norm(a[2][dyn] - b[2][dyn], axis=1) / gdt
-    return {"max_position_error_rel_body_size": float(ep.max()), "max_angle_error_rad": float(ea.max()),
-            "max_velocity_error_rel_g_dt": float(ev.max())}
+    return {"max_position_error_rel_body_size": float((np.linalg.norm(a[0][dyn] - b[0][dyn], axis=1) / size).max()),
+            "max_angle_error_rad": float(rotation_angle(a[1][dyn], b[1][dyn]).max()),
+            "max_velocity_error_rel_g_dt": float((np.linalg.norm(a[2][dyn] - b[2][dyn], axis=1) / gdt).max()),
+            "max_angular_velocity_error_rel_g_dt": float(
+                (np.linalg.norm(a[3][dyn] - b[3][dyn], axis=1) * size / gdt).max())}
 
 
-def step_precision(sc, g, o, state, threads):
-    """Errors of one device timestep (FP32 PCG inside the FP64 Newton loop)
-    against the FP64 CPU restatement from the same state: positions relative
-    to the body size, orientations in radians, velocities relative to |g| dt.
-    Next to them, the same errors between two runs of the restatement itself
-    that differ only in the inner solve's tolerance (the workload's and
-    1e-12): at the workload's Newton tolerance the step is defined only to
-    that level.  Per-contact forces are not compared: bodies resting on
-    several coplanar contacts have a nonunique force split (only the body
-    impulses are fixed)."""
-    from oracle import orc
-    q = type(sc.params).from_buffer_copy(sc.params)
-    q.pcg_tol = 1e-12
-    o2 = orc.OracleScene(sc, q)
-    o2.set_threads(threads)
-    o2.set_state(*state)
-    o2.step(1)
-    out = {"steps_from_identical_state": 1}
-    out.update(_state_errors(sc, g, o))
-    out["restatement_self_difference"] = _state_errors(sc, o, o2.get_state())
-    out["reference"] = ("FP64 CPU restatement, same parameters (Newton tolerance %g, inner PCG to %g), "
-                        "same settled state; self difference: the restatement with its inner PCG to 1e-12"
-                        % (sc.params.tolerance, sc.params.pcg_tol))
-    return out
+def worst(errs):
+    """Per-quantity maximum over a list of state_errors records."""
+    return {k: max(e[k] for e in errs) for k in errs[0]}
 
 
 def measured_traffic(kernel, workload):
-    """HBM bytes per launch of the timed kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (profiles/<kernel>_<workload>_traffic.json), or None when there is none."""
-    if kernel != "pcg":
-        return None
-    path = os.path.join(ROOT, "profiles", "pcg_iter_%s_traffic.json" % workload)
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.json" % (kernel.replace("k_", "", 1), workload))
     if not os.path.exists(path):
         return None
     with open(path) as f:
         return float(json.load(f)["traffic_bytes_per_launch"])
 
 
+def timed_run(sc, params, state, steps, warmup, kernel):
+    """Warm-up and timed timesteps of one device-resident scene from `state`.
+    Returns the record, the device state after the first warm-up step and the
+    context's stream of kernel-timing figures."""
+    ds = pdc.DynamicSystem(sc, params)
+    ds.set_state(*state)
+    first = None
+    for k in range(warmup):
+        st = ds.step(1)
+        if k == 0:
+            first = tuple(x.copy() for x in ds.state())
+        log("warmup: %d contacts, %d Newton it, %d PCG it" % (st.n_contacts, st.newton_iterations,
+                                                              st.pcg_iterations))
+    ds.ctx.enable_kernel_timing(KT_NAMES[kernel])
+    ds.ctx.reset_timings()
+    rows = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = ds.step(1)
+        rows.append((st.n_contacts, st.newton_iterations, st.pcg_iterations, st.n_blocks, st.n_local_bodies,
+                     st.nonfinite))
+    elapsed = time.perf_counter() - t0
+    ms, launches, nbytes = ds.ctx.kernel_timing()
+    chunk = ds.ctx.get_shapes().pcg_chunk or 8
+    ds.ctx.close()
+    rows = np.array(rows, dtype=np.float64)
+    log("timed %d steps in %.3fs: contacts %d..%d, Newton it %.1f, PCG it %.1f" % (
+        steps, elapsed, rows[:, 0].min(), rows[:, 0].max(), rows[:, 1].mean(), rows[:, 2].mean()))
+    survey = float(sum(r[2] * survey_pcg_bytes(params, r[3], r[4]) for r in rows))
+    return {"elapsed": elapsed, "rows": rows, "ms": ms, "launches": launches, "kernel_bytes": nbytes,
+            "survey_bytes": survey, "chunk": chunk}, first
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pile_2000")
+    ap.add_argument("--setting", default="tight", choices=sorted(SETTINGS),
+                    help="inner solve of the headline (the other one is reported beside it)")
     ap.add_argument("--kernel", default="pcg", choices=sorted(KT_NAMES))
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other inner-solve setting")
     ap.add_argument("--state", default=None, help="npz with pos, quat, vel, angvel to start from")
     ap.add_argument("--settle-steps", type=int, default=None,
                     help="settle on the device for this many steps instead of loading the settled state")
@@ -335,88 +440,113 @@ def main():
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(scene-level sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
-    sc = scenes.WORKLOADS[args.workload]()
+    prec, tol = SETTINGS[args.setting]
+    overrides = {"pcg_precision": prec, "pcg_tol": tol}
+    sc = scenes.WORKLOADS[args.workload](**overrides)
     state, origin = settled_state(sc, args.workload, args.settle_steps, args.state)
-    ds = pdc.DynamicSystem(sc)
-    ds.set_state(*state)
-    log("warmup %d steps" % args.warmup)
-    first_gpu = None
-    for k in range(args.warmup):
-        st = ds.step(1)
-        if k == 0:
-            first_gpu = tuple(x.copy() for x in ds.state())
-        log("warmup: %d contacts, %d Newton it, %d PCG it" % (st.n_contacts, st.newton_iterations,
-                                                              st.pcg_iterations))
-    ds.ctx.enable_kernel_timing(KT_NAMES[args.kernel])
-    ds.ctx.reset_timings()
-    contacts = []
-    newton = []
-    pcg_it = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        st = ds.step(1)
-        contacts.append(st.n_contacts)
-        newton.append(st.newton_iterations)
-        pcg_it.append(st.pcg_iterations)
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    ms, launches, nbytes = ds.ctx.kernel_timing()
-    log("timed %d steps in %.3fs: contacts %d..%d, Newton it %.1f, PCG it %.1f" % (
-        args.steps, elapsed, min(contacts), max(contacts), np.mean(newton), np.mean(pcg_it)))
-    achieved = nbytes / (ms * 1e-3) if ms > 0 else 0.0
+    log("headline setting %s: PCG operator %s, pcg_tol %g" % (args.setting, "FP64" if prec == PCG_FP64 else "FP32",
+                                                              tol))
+    run, first_gpu = timed_run(sc, sc.params, state, args.steps, args.warmup, args.kernel)
+    rows = run["rows"]
+    ms, launches = run["ms"], run["launches"]
+    kname = pcg_kernel_name(sc.params) if args.kernel.startswith("pcg") else None
+    bytes_used = run["survey_bytes"] if args.kernel == "pcg" else run["kernel_bytes"]
+    achieved = bytes_used / (ms * 1e-3) if ms > 0 else 0.0
     out = {
         "metric": METRIC,
-        "value": args.steps / elapsed,
+        "value": args.steps / run["elapsed"],
         "unit": "timesteps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": 1e3 * run["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (Newton, residuals, line search) + f32 (PCG inner solve)",
+        "dtype": "f64" if prec == PCG_FP64 else "f64 (Newton, residuals, line search) + f32 (PCG operator)",
         "data": "synthetic: seeded 2000-body mixed pile, %s state after %d steps" % (origin, sc.steps),
         "config": {
             "workload": "pile_2000: 1000 spheres + 1000 boxes in a 20x20x40 container, "
-                        "96^3 SDF grids, mu 0.6, dt 0.005, settled",
-            "contacts_per_step": float(np.mean(contacts)),
-            "newton_iterations_per_step": float(np.mean(newton)),
-            "pcg_iterations_per_step": float(np.mean(pcg_it)),
+                        "96^3 SDF grids, mu 0.6, dt 0.005, settled; Newton tolerance %g (%d iterations); "
+                        "inner PCG %s operator to %g" % (sc.params.tolerance, sc.params.iterations,
+                                                         "FP64" if prec == PCG_FP64 else "FP32", tol),
+            "setting": args.setting,
+            "contacts_per_step": float(rows[:, 0].mean()),
+            "newton_iterations_per_step": float(rows[:, 1].mean()),
+            "pcg_iterations_per_step": float(rows[:, 2].mean()),
+            "nonfinite_iterations": int(rows[:, 5].sum()),
             "bodies": int(sc.n),
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": KT_KERNEL[args.kernel] % (ds.ctx.get_shapes().pcg_chunk or 8) if args.kernel == "pcg"
-            else KT_KERNEL[args.kernel],
+            "kernel": kernel_description(args.kernel, sc.params, run["chunk"]),
             "achieved": achieved / 1e9,
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM,
-            "traffic": measured_traffic(args.kernel, args.workload),
+            "traffic": measured_traffic(kname, args.workload) if kname else None,
             "launches": launches,
             "avg_launch_us": 1e3 * ms / max(launches, 1),
-            "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
+            "algorithmic_bytes_per_launch": bytes_used / max(launches, 1),
+            "algorithmic_bytes": "hot-path table per-unit figure (survey_pcg_bytes) x active iterations"
+                                 if args.kernel == "pcg" else "kernel's own count",
+            "kernel_own_bytes_per_launch": run["kernel_bytes"] / max(launches, 1),
         },
     }
     try:
         out["roofline"]["measured_copy_GBps"] = measured_copy_rate() / 1e9
     except Exception as exc:  # the copy is a reference figure only
         log("copy rate not measured: %s" % exc)
-    ds.ctx.close()
+    checks = [("headline", sc, sc.params, state, first_gpu, None)]
+    if not args.no_extra:
+        other = "fast" if args.setting == "tight" else "tight"
+        oprec, otol = SETTINGS[other]
+        osc = scenes.WORKLOADS[args.workload](pcg_precision=oprec, pcg_tol=otol)
+        orun, ofirst = timed_run(osc, osc.params, state, max(10, args.steps // 2), args.warmup, "pcg")
+        orows = orun["rows"]
+        out["other_setting"] = {
+            "setting": other,
+            "inner_solve": "%s operator to %g" % ("FP64" if oprec == PCG_FP64 else "FP32", otol),
+            "timesteps_per_s": len(orows) / orun["elapsed"],
+            "ms_per_step": 1e3 * orun["elapsed"] / len(orows),
+            "pcg_iterations_per_step": float(orows[:, 2].mean()),
+            "newton_iterations_per_step": float(orows[:, 1].mean()),
+            "avg_pcg_launch_us": 1e3 * orun["ms"] / max(orun["launches"], 1),
+        }
+        # its step against the FP64 restatement at the headline's (tight) parameters, and the
+        # restatement's own step at its parameters against the same reference
+        if args.setting == "tight":
+            checks.append(("other_setting_vs_fp64", osc, osc.params, state, ofirst, sc.params))
+            checks.append(("other_setting_restatement_vs_fp64", osc, osc.params, state, None, sc.params))
+        else:
+            checks.append(("other_setting_vs_fp64", osc, osc.params, state, ofirst, None))
     if not args.no_batched:
-        bs, out["batched"] = batched_leg(args.batched_scenes, args.batched_steps, 16, 10)
+        bs, out["batched"], bchecks = batched_leg(args.batched_scenes, args.batched_steps, 16, 10, overrides)
         bs.ctx.close()
+        checks += bchecks
         if not args.no_served:
             rates = [float(r) for r in args.served_rates.split(",") if r]
-            out["served"] = served_leg(args.served_scenes, args.served_settle, rates, args.served_duration,
-                                       plain_rate=40.0)
+            out["served"], schecks = served_leg(args.served_scenes, args.served_settle, rates,
+                                                args.served_duration, 40.0, overrides)
+            checks += schecks
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        precision, out["cpu_baseline"] = cpu_baseline(sc, state, args.cpu_budget, threads, first_gpu)
-        if precision is not None:
-            out["precision"] = precision
+        precision, out["cpu_baseline"], _ = cpu_baseline(sc, state, args.cpu_budget, threads, checks)
+        out["precision"] = {
+            "reference": "FP64 CPU restatement (oracle), one step from the identical state, same parameters "
+                         "unless stated; maxima over dynamic bodies",
+            "headline": worst(precision["headline"]),
+        }
+        if "other_setting_vs_fp64" in precision:
+            out["precision"]["other_setting"] = {
+                "device_vs_fp64_restatement_at_tight_parameters": worst(precision["other_setting_vs_fp64"])}
+            if "other_setting_restatement_vs_fp64" in precision:
+                out["precision"]["other_setting"]["restatement_at_its_parameters_vs_tight"] = \
+                    worst(precision["other_setting_restatement_vs_fp64"])
+        for leg in ("batched", "served"):
+            if leg in precision:
+                out["precision"][leg] = worst(precision[leg])
     print(json.dumps(out), flush=True)
 
 
