@@ -185,6 +185,24 @@ int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts,
                     const double* vel, const double* angvel, double* out_acc,
                     double* out_local_force, pdc_solve_stats* stats);
 
+/* The same computation as pdc_fill_forces, allocating no device memory.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code that must not call hipMalloc or hipFree, such as tight serving loops
+ * or harnesses that forbid it).  Every device buffer the call needs beyond the
+ * uploaded body table comes from a fixed scratch array of 64 MiB per device
+ * inside the library, held until the process ends.  Limits: one scene in the
+ * context, at most 32768 contacts and 4096 uploaded bodies; larger requests,
+ * or a call whose buffers do not fit the scratch, return PDC_ERR_CAPACITY.
+ * Calls on one device share that scratch: calls that overlap in time on one
+ * GPU (from several threads or contexts) race.  The call's contact list is
+ * not kept in the context (pdc_get_contacts does not return it), and kernel
+ * timing is suspended for the call. */
+int pdc_fill_forces_noalloc(pdc_context* ctx, int32_t n_contacts,
+                            const int32_t* obj_ids, const double* transformation,
+                            const double* depth, const double* quat_xyzw,
+                            const double* vel, const double* angvel, double* out_acc,
+                            double* out_local_force, pdc_solve_stats* stats);
+
 /* ---- whole timestep on the device-resident state ---------------------------
  * DynamicSystem::step without adaptive substeps (DynamicSystem.cpp:130-161,
  * 207-210): contact detection, fillForces, then semi-implicit integration.

@@ -9,11 +9,47 @@
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
+#include <mutex>
 #include <numeric>
 
 #include "pdc_internal.h"
 
 using pdc::HipError;
+
+namespace pdc {
+thread_local Arena* t_arena = nullptr;
+}
+
+// ---- device scratch of the entry point that allocates nothing --------------
+// pdc_fill_forces_noalloc takes every device buffer it needs beyond the body
+// table from this array.  Each device that loads the library holds its own
+// copy until the process ends.  In host code `scratch` is only a host-side
+// name: its device address is taken once per device at context creation, with
+// that device current, and only that address is used.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int kNoAllocMaxContacts = 32768;
+constexpr int kNoAllocMaxBodies = 4096;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (taken once per device).
+static void scratch_setup(int device) {
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (device < 0 || device >= kMaxDevices)
+		throw HipError("device index beyond the scratch table", PDC_ERR_NO_DEVICE);
+	if (g_scratch_ptr[device]) return;
+	void* p = nullptr;
+	PDC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+	if (!p) throw HipError("hipGetSymbolAddress returned no address for the scratch array", PDC_ERR_DEVICE);
+	g_scratch_ptr[device] = p;
+}
+
+static void* scratch_address(int device) {
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
 
 namespace {
 
@@ -174,6 +210,7 @@ int pdc_create(int device, const pdc_params* params, pdc_context** out) {
 		hipDeviceProp_t prop;
 		PDC_HIP(hipGetDeviceProperties(&prop, device));
 		ctx->cus = prop.multiProcessorCount;
+		scratch_setup(device);
 		PDC_HIP(hipStreamSynchronize(ctx->stream));
 	});
 	if (rc != PDC_OK) {
@@ -325,11 +362,11 @@ int pdc_get_contacts(pdc_context* ctx, int32_t capacity, int32_t* obj_ids, doubl
 	});
 }
 
-int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids,
-                    const double* transformation, const double* depth, const double* quat_xyzw,
-                    const double* vel, const double* angvel, double* out_acc, double* out_local_force,
-                    pdc_solve_stats* stats) {
-	return guard(ctx, [&] {
+static void fill_forces_impl(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids,
+                             const double* transformation, const double* depth, const double* quat_xyzw,
+                             const double* vel, const double* angvel, double* out_acc, double* out_local_force,
+                             pdc_solve_stats* stats) {
+	{
 		require_bodies(ctx);
 		const int C = n_contacts;
 		const int N = ctx->N;
@@ -401,6 +438,65 @@ int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids
 		if (stats) *stats = st;
 		ctx->timings.force_process += now_s() - t0;
 		ctx->timings.steps += 1;
+	}
+}
+
+int pdc_fill_forces(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids,
+                    const double* transformation, const double* depth, const double* quat_xyzw,
+                    const double* vel, const double* angvel, double* out_acc, double* out_local_force,
+                    pdc_solve_stats* stats) {
+	return guard(ctx, [&] {
+		fill_forces_impl(ctx, n_contacts, obj_ids, transformation, depth, quat_xyzw, vel, angvel, out_acc,
+		                 out_local_force, stats);
+	});
+}
+
+int pdc_fill_forces_noalloc(pdc_context* ctx, int32_t n_contacts, const int32_t* obj_ids,
+                            const double* transformation, const double* depth, const double* quat_xyzw,
+                            const double* vel, const double* angvel, double* out_acc, double* out_local_force,
+                            pdc_solve_stats* stats) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		if (ctx->S != 1)
+			throw std::logic_error("pdc_fill_forces_noalloc: the context must hold one scene");
+		if (n_contacts > kNoAllocMaxContacts)
+			throw HipError("pdc_fill_forces_noalloc: " + std::to_string(n_contacts) + " contacts exceed the " +
+			                   std::to_string(kNoAllocMaxContacts) + " of its scratch",
+			               PDC_ERR_CAPACITY);
+		if (ctx->N > kNoAllocMaxBodies)
+			throw HipError("pdc_fill_forces_noalloc: " + std::to_string(ctx->N) + " bodies exceed the " +
+			                   std::to_string(kNoAllocMaxBodies) + " of its scratch",
+			               PDC_ERR_CAPACITY);
+		void* base = scratch_address(ctx->device);
+		if (!base) throw std::logic_error("pdc_fill_forces_noalloc: scratch address not taken for this device");
+		pdc::Arena arena;
+		arena.base = static_cast<char*>(base);
+		arena.cap = kScratchBytes;
+		// Everything this call grows comes from the arena: the contact
+		// buffers start from zero capacity, kernel timing (which creates
+		// events) is off, and afterwards every buffer, the contact stride and
+		// the timing class are put back.  The contacts of this call are not
+		// kept (they live in the scratch the next call overwrites).
+		const int saved_stride = ctx->c_stride, saved_kt = ctx->kt.which;
+		auto undo = [&]() {
+			pdc::t_arena = nullptr;
+			arena.restore();
+			ctx->c_stride = saved_stride;
+			ctx->kt.which = saved_kt;
+			ctx->C = 0;
+		};
+		ctx->c_stride = 0;
+		ctx->kt.which = PDC_KT_NONE;
+		pdc::t_arena = &arena;
+		try {
+			fill_forces_impl(ctx, n_contacts, obj_ids, transformation, depth, quat_xyzw, vel, angvel, out_acc,
+			                 out_local_force, stats);
+		} catch (...) {
+			(void)hipStreamSynchronize(ctx->stream);
+			undo();
+			throw;
+		}
+		undo();
 	});
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -38,12 +39,64 @@ inline int grid_for(int64_t n, int block = kBlock) {
 	return (int)(g < 1 ? 1 : g);
 }
 
+// Device memory handed out by a bump allocator over a fixed region (the
+// library's static scratch array, used by the entry point that allocates
+// nothing).  While an arena is active on the calling thread, every buffer
+// that grows takes its memory from the arena instead of hipMalloc, and
+// `restore` puts each such buffer back as it was before the call.
+struct Arena {
+	char* base = nullptr;
+	size_t cap = 0, used = 0;
+	struct Undo {
+		void** pp;
+		size_t* pcap;
+		void* old_p;
+		size_t old_cap;
+	};
+	std::vector<Undo> undo;
+	bool owns(const void* p) const {
+		return p != nullptr && (const char*)p >= base && (const char*)p < base + cap;
+	}
+	void note(void** pp, size_t* pcap) {
+		for (const Undo& u : undo)
+			if (u.pp == pp) return;
+		undo.push_back(Undo{pp, pcap, *pp, *pcap});
+	}
+	void* take(size_t bytes, void** pp, size_t* pcap) {
+		note(pp, pcap);
+		const size_t b = (bytes + 255) & ~(size_t)255;
+		if (used + b > cap)
+			throw HipError("the call needs more device scratch than the " + std::to_string(cap >> 20) +
+			                   " MiB of the no-allocation entry point",
+			               PDC_ERR_CAPACITY);
+		void* p = base + used;
+		used += b;
+		return p;
+	}
+	void restore() {
+		for (auto it = undo.rbegin(); it != undo.rend(); ++it) {
+			*it->pp = it->old_p;
+			*it->pcap = it->old_cap;
+		}
+		undo.clear();
+		used = 0;
+	}
+};
+extern thread_local Arena* t_arena;  // the calling thread's active arena, or null
+
 // Growable device buffer (reallocates only when the capacity grows).
 template <class T>
 struct DBuf {
 	T* p = nullptr;
 	size_t cap = 0;
 	void ensure(size_t n) {
+		if (t_arena) {
+			if (t_arena->owns(p) && n <= cap) return;
+			void* q = t_arena->take(std::max<size_t>(n, 1) * sizeof(T), reinterpret_cast<void**>(&p), &cap);
+			p = static_cast<T*>(q);
+			cap = n;
+			return;
+		}
 		if (n <= cap) return;
 		size_t nc = cap ? cap : 256;
 		while (nc < n) nc = nc + nc / 2 + 1;
@@ -53,6 +106,12 @@ struct DBuf {
 		cap = nc;
 	}
 	void release() {
+		if (t_arena) {  // the buffer is put back by the arena's restore
+			t_arena->note(reinterpret_cast<void**>(&p), &cap);
+			p = nullptr;
+			cap = 0;
+			return;
+		}
 		if (p) (void)hipFree(p);
 		p = nullptr;
 		cap = 0;

@@ -34,7 +34,7 @@ KT_NONE, KT_SPMV, KT_CONTACT_TERMS, KT_ASSEMBLY, KT_LINE_SEARCH, KT_CONTACTS, KT
 EXPORTS = [
     "pdc_default_params", "pdc_device_count", "pdc_create", "pdc_destroy", "pdc_configure",
     "pdc_last_error", "pdc_upload_bodies", "pdc_set_state", "pdc_get_state",
-    "pdc_detect_contacts", "pdc_get_contacts", "pdc_fill_forces", "pdc_step",
+    "pdc_detect_contacts", "pdc_get_contacts", "pdc_fill_forces", "pdc_fill_forces_noalloc", "pdc_step",
     "pdc_get_timings", "pdc_reset_timings", "pdc_enable_kernel_timing", "pdc_kernel_timing",
     "pdc_set_scenes", "pdc_scene_count", "pdc_set_scene_friction", "pdc_step_scenes",
     "pdc_get_shapes", "pdc_set_shapes", "pdc_autotune",
@@ -83,6 +83,7 @@ def load_library(path=LIB_PATH):
     L.pdc_get_contacts.argtypes = [V, ctypes.c_int32, _I, _D, _D, _D, _D]
     L.pdc_fill_forces.argtypes = [V, ctypes.c_int32, _I, _D, _D, _D, _D, _D, _D, _D,
                                   ctypes.POINTER(PdcSolveStats)]
+    L.pdc_fill_forces_noalloc.argtypes = L.pdc_fill_forces.argtypes
     L.pdc_step.argtypes = [V, ctypes.c_int32, ctypes.POINTER(PdcSolveStats)]
     L.pdc_set_scenes.argtypes = [V, ctypes.c_int32, _I]
     L.pdc_scene_count.argtypes = [V]
@@ -187,7 +188,9 @@ class Context:
         self._check(self._L.pdc_get_contacts(self.h, count, _i(ids), _d(pos), _d(depth), _d(T), _d(lf)))
         return ids, pos, depth, T, lf
 
-    def fill_forces(self, ids, T, depth, quat, vel, angvel):
+    def fill_forces(self, ids, T, depth, quat, vel, angvel, noalloc=False):
+        """pdc_fill_forces, or with noalloc=True pdc_fill_forces_noalloc (no
+        device allocation; the library's fixed scratch instead)."""
         ids = np.ascontiguousarray(ids, dtype=np.int32).reshape(-1, 2)
         C = ids.shape[0]
         T = _f64(T).reshape(C, 36)
@@ -195,9 +198,9 @@ class Context:
         acc = np.zeros((self.n, 6))
         lf = np.zeros((C, 3))
         st = PdcSolveStats()
-        self._check(self._L.pdc_fill_forces(self.h, C, _i(ids), _d(T), _d(depth), _d(_f64(quat)),
-                                            _d(_f64(vel)), _d(_f64(angvel)), _d(acc), _d(lf),
-                                            ctypes.byref(st)))
+        fn = self._L.pdc_fill_forces_noalloc if noalloc else self._L.pdc_fill_forces
+        self._check(fn(self.h, C, _i(ids), _d(T), _d(depth), _d(_f64(quat)), _d(_f64(vel)), _d(_f64(angvel)),
+                       _d(acc), _d(lf), ctypes.byref(st)))
         return acc, lf, st
 
     def step(self, n=1):
