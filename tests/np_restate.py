@@ -3,12 +3,16 @@ the algebra of the primal-dual method rather than from either implementation:
 
     u  = J v,   J = [-T0^T (at body a), -T1^T (at body b)]           per contact
     ru = M (v - v_old) - J^T f                        (static rows 0)
-    rf = -u + (lam0, -lam1 g) - K e0
+    rf = -u + (lam0, -lam1 g) - K e0 [- kappa mbar f_n e0]
     rl = lam o c + mu_c,  c = (-f_n, sqrt(|f_t|^2+P) - mu f_n - sqrt(P) - SH)
-    W  = diag(1/(eps + l_n), Kt^-1),  l_n = -lam0/(c0 - eps lam0),
+    W  = diag(1/(eps + l_n), Kt^-1),  l_n = -lam0/(c0 - eps lam0) [+ kappa mbar],
     Kt = eps I - lam1/(c1 - eps lam1) g g^T + lam1 H
     sf = rf + (-rl0/(c0 - eps lam0), g rl1/(c1 - eps lam1))
     A  = M + eps I + P (J^T W J) P,   rhs = P (J^T W sf - ru)        (P: active rows)
+
+The bracketed terms are the spring law (spring_k k, spring_damp d as a
+fraction of k): beta = dt k / (dt k + d k), kappa = 1 / (dt (dt k + d k)),
+K = -beta depth / (dt s); without it K = e u_n(v0) (restitution e).
 
 Used by the tests to pin the oracle's setup, residual and assembly.
 """
@@ -47,9 +51,11 @@ def radius(mu, fn):
     return np.sqrt(k * k - PERT)
 
 
-def first_system(ids, T, mass, I0, quat, vel, ang, is_static, dt, g, mu, restitution=0.0, Hu=1e-4):
+def first_system(ids, T, mass, I0, quat, vel, ang, is_static, dt, g, mu, restitution=0.0, Hu=1e-4,
+                 spring=None, depth=None):
     """ids (C,2) global; T (C,2,3,6) [contact][side][col][row].  Returns A, rhs in
-    the local numbering of first appearance, and that numbering."""
+    the local numbering of first appearance, and that numbering.  spring:
+    (spring_k, spring_damp) for the spring law (then depth (C,) is needed)."""
     C = ids.shape[0]
     order = []
     for i in range(C):
@@ -119,7 +125,16 @@ def first_system(ids, T, mass, I0, quat, vel, ang, is_static, dt, g, mu, restitu
         if act[k]:
             vold[6 * k:6 * k + 3] += g * dt
     vold = vold / s
-    K = restitution * (J @ v)[0::3]
+    if spring is not None:
+        k_spring, damp_frac = spring
+        tk, damp = dt * k_spring, damp_frac * k_spring
+        beta = tk / (tk + damp)
+        kappa = 1.0 / (dt * (tk + damp))
+        K = -beta * np.asarray(depth, dtype=float) / dt / s
+        kmb = kappa * mbar
+    else:
+        K = restitution * (J @ v)[0::3]
+        kmb = 0.0
     f = np.zeros(3 * C)
     f[0::3] = 1.0
     lam = np.full(2 * C, 1e-2)
@@ -142,7 +157,7 @@ def first_system(ids, T, mass, I0, quat, vel, ang, is_static, dt, g, mu, restitu
     u = J @ v
     rf = -u.copy()
     for i in range(C):
-        rf[3 * i] += lam[2 * i] - K[i]
+        rf[3 * i] += lam[2 * i] - K[i] - kmb * f[3 * i]
         rf[3 * i + 1:3 * i + 3] -= lam[2 * i + 1] * gr[i]
     rl = lam * c + muc
     # tangential weights and errors
@@ -166,7 +181,7 @@ def first_system(ids, T, mass, I0, quat, vel, ang, is_static, dt, g, mu, restitu
     for i in range(C):
         ic0 = 1.0 / (c[2 * i] - eps * lam[2 * i])
         ic1 = 1.0 / (c[2 * i + 1] - eps * lam[2 * i + 1])
-        ln = -lam[2 * i] * ic0
+        ln = -lam[2 * i] * ic0 + kmb
         Kt = eps * np.eye(2) - lam[2 * i + 1] * ic1 * np.outer(gr[i], gr[i]) + lam[2 * i + 1] * H[i]
         W[3 * i, 3 * i] = 1.0 / (eps + ln)
         W[3 * i + 1:3 * i + 3, 3 * i + 1:3 * i + 3] = np.linalg.inv(Kt)

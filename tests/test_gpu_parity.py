@@ -202,13 +202,13 @@ def test_fill_forces_spring_and_restitution(require_gpu):
     p = sc.params
     p.pcg_precision = PCG_FP64
     p.pcg_tol = 1e-13
-    _fill_parity(sc, 200, p, tol_rel=1e-5)
+    _fill_parity(sc, 200, p, tol_rel=1e-6)
     sc2 = scenes.pile(40, seed=5, half=2.5, height=9.0, name="pile_40", grid_n=48, sample_spacing=0.12,
                       tolerance=1e-10, restitution=0.3)
     p2 = sc2.params
     p2.pcg_precision = PCG_FP64
     p2.pcg_tol = 1e-13
-    _fill_parity(sc2, 200, p2, tol_rel=1e-5)
+    _fill_parity(sc2, 200, p2, tol_rel=1e-6)
 
 
 @pytest.mark.parametrize("pcg_tol", [1e-8, 1e-13])

@@ -260,3 +260,90 @@ def test_trace_records_iterations():
     assert tr.shape[0] == st.newton_iterations + 1
     assert tr[-1, 1] < sc.params.tolerance
     assert np.all(tr[:-1, 1] >= sc.params.tolerance)
+
+
+@pytest.mark.parametrize("law", ["restitution", "spring"])
+def test_first_system_spring_and_restitution_match_dense_restatement(law):
+    """The restitution and spring branches of the setup and the Newton system
+    (NonSmoothContactForce.cpp:26-47 K, :149-151 compliance term of rf,
+    :196-198 compliance term of l_n) against np_restate's own statement of
+    them, on the multi-body scene of the test above."""
+    rng = np.random.default_rng(7)
+    b = _Builder()
+    b.box((4.0, 0.5, 4.0), (0, -0.5, 0), static=True)
+    q1 = tuple(np.array([0.05, 0.2, -0.1, 0.97]) / np.linalg.norm([0.05, 0.2, -0.1, 0.97]))
+    b.box((0.4, 0.3, 0.35), (0.0, 0.28, 0.0), quat=q1)
+    b.sphere(0.3, (1.2, 0.29, 0.1))
+    b.box((0.3, 0.3, 0.3), (0.1, 0.85, 0.05))
+    kw = dict(restitution=0.4) if law == "restitution" else dict(spring_based_force=1, spring_k=1e4,
+                                                                 spring_damp=0.1)
+    p = default_params(timestep=0.01, friction_coeff=0.55, grid_n=48, sample_spacing=0.12, **kw)
+    sc = b.build("sys", p, 1)
+    sc.vel[1:] = rng.normal(size=(3, 3)) * 0.3
+    sc.angvel[1:] = rng.normal(size=(3, 3)) * 0.5
+    o = orc.OracleScene(sc)
+    C, _ = o.detect()
+    ct = o.contacts()
+    A, rhs, l2g = o.first_system(ct, sc.quat, sc.vel, sc.angvel)
+    extra = dict(restitution=0.4) if law == "restitution" else dict(spring=(1e4, 0.1), depth=ct.depth)
+    A2, rhs2, order, info = np_restate.first_system(
+        ct.ids, ct.T, sc.mass, sc.inertia, sc.quat, sc.vel, sc.angvel, sc.is_static, 0.01,
+        (0, -G, 0), 0.55, **extra)
+    # without the law the system differs: the comparison below pins the branch
+    A0, rhs0, _, _ = np_restate.first_system(
+        ct.ids, ct.T, sc.mass, sc.inertia, sc.quat, sc.vel, sc.angvel, sc.is_static, 0.01, (0, -G, 0), 0.55)
+    assert np.max(np.abs(rhs0 - rhs2)) > 1e-6 * np.max(np.abs(rhs2))
+    assert list(l2g) == order
+    np.testing.assert_allclose(A, A2, rtol=1e-9, atol=1e-9 * np.max(np.abs(A2)))
+    np.testing.assert_allclose(rhs, rhs2, rtol=1e-9, atol=1e-9 * np.max(np.abs(rhs2)))
+
+
+@pytest.mark.parametrize("e", [0.0, 0.3, 0.8])
+def test_restitution_closed_form(e):
+    """A cube falling flat onto a static slab at speed V with restitution e:
+    at convergence every contact has u_n + K = 0 with K = e u_n(v0)
+    (NonSmoothContactForce.cpp:42-45, :137-147), so the cube leaves the step
+    at e V upwards, without rotation, whatever gravity adds."""
+    V = 0.8
+    sc, _ = _resting_box(theta=0.0, tol=1e-12, iters=200, restitution=e)
+    sc.vel[1] = (0.0, -V, 0.0)
+    o = orc.OracleScene(sc)
+    st = o.step(1)
+    _, _, vel, ang = o.get_state()
+    print("e %.1f: contacts %d, Newton %d, v %s" % (e, st.n_contacts, st.newton_iterations, vel[1]))
+    assert st.n_contacts == 100 and st.exit_error < 1e-12
+    assert vel[1, 1] == pytest.approx(e * V, abs=1e-8)
+    assert np.max(np.abs(vel[1, [0, 2]])) < 1e-9 and np.max(np.abs(ang[1])) < 1e-8
+
+
+def test_spring_law_closed_form():
+    """A cube at rest on a slab, penetrating it by d, under the spring law
+    (NonSmoothContactForce.cpp:26-41, :149-151): with beta = dt k / (dt k + c),
+    kappa = 1 / (dt (dt k + c)), c = damp k, each of the N identical contacts
+    gives u_n - beta d / dt + kappa P = 0 for its impulse P, and
+    m (v - v0 - g dt) = N P.  Hence v = (kappa m (v0 + g dt) + N beta d / dt) /
+    (kappa m + N) and P = (beta d / dt - v) / kappa, checked against the
+    velocity after the step and each contact's local force."""
+    d, k, damp, dt = 1e-3, 1e4, 0.1, 0.01
+    sc, _ = _resting_box(theta=0.0, depth=d, tol=1e-12, iters=200, spring_based_force=1, spring_k=k,
+                         spring_damp=damp)
+    o = orc.OracleScene(sc)
+    C, _ = o.detect()
+    ct = o.contacts()
+    assert C == 100 and np.ptp(ct.depth) < 1e-12
+    depth = float(ct.depth[0])
+    acc, lf, st = o.fill_forces(ct)
+    c = damp * k
+    beta = dt * k / (dt * k + c)
+    kappa = 1.0 / (dt * (dt * k + c))
+    m = sc.mass[1]
+    v = (kappa * m * (-G * dt) + C * beta * depth / dt) / (kappa * m + C)
+    P = (beta * depth / dt - v) / kappa
+    print("spring: depth %.6e, v %.6e (closed form %.6e), P %.6e..%.6e (closed form %.6e), Newton %d"
+          % (depth, acc[1, 1], v, lf[:, 0].min(), lf[:, 0].max(), P, st.newton_iterations))
+    assert P > 0 and st.exit_error < 1e-12
+    assert acc[1, 1] == pytest.approx(v, rel=1e-7)      # acc = velocity increment from rest
+    np.testing.assert_allclose(lf[:, 0], P, rtol=1e-6)
+    # tangential components of resting contacts may balance one another (the
+    # split is not unique); their net effect on the cube is nil
+    assert np.max(np.abs(acc[1, [0, 2]])) < 1e-9 and np.max(np.abs(acc[1, 3:])) < 1e-8
