@@ -281,62 +281,67 @@ def batched_leg(n_scenes, steps, plain_scenes, plain_steps, overrides, seed_base
     return bs, out, checks
 
 
-def served_leg(n_scenes, settle_steps, rates, duration, plain_rate, overrides, seed_base=5000, seed=7):
-    """Continuous-batching server over resident scenes of its own: seeded
-    random piles of 10-1000 bodies (log-uniform), admitted by HBM footprint,
-    stepped together from their seeded drop for `settle_steps` steps so that
-    the requests meet contact-rich states; then Poisson request streams of
-    one-step requests at each rate, and the plain per-scene server (one
-    request per step call) at `plain_rate`."""
+def served_leg(n_scenes, settle_steps, rates, duration, plain_rate, overrides, seed_base=5000, seed=7,
+               shards=2, budget=200 * 2 ** 30):
+    """Continuous-batching server (serve.SceneServer: two shards, each a
+    context with its own stream and worker thread) over seeded random piles of
+    10-1000 bodies (log-uniform), admitted by HBM footprint; every resident
+    scene is stepped `settle_steps` times from its seeded drop through the
+    server so that the requests meet contact-rich states; then Poisson
+    streams of one-step requests at each rate, and the plain per-scene path
+    (one request per step call, each waited for) at `plain_rate`."""
     t0 = time.perf_counter()
     scs = [scenes.random_pile(seed_base + k, n_min=10, n_max=1000, **overrides) for k in range(n_scenes)]
-    srv = serve.SceneServer(scs, hbm_budget=200 * 2 ** 30)
-    n = len(srv.scenes)
-    bodies = sum(sc.n for sc in srv.scenes)
-    log("served: %d scenes built and resident (%d rejected), %d bodies, %.1f GiB, %.1fs" % (
-        n, len(srv.rejected), bodies, srv.footprint / 2 ** 30, time.perf_counter() - t0))
+    bodies_all = sum(sc.n + 1 for sc in scs)
+    samples_all = sum(serve.scene_samples(sc) + 6 for sc in scs)
+    srv = serve.SceneServer(scs[0].params, hbm_budget=budget, shards=shards, max_resident=4096,
+                            reserve_bodies=int(1.05 * bodies_all), reserve_samples=int(1.05 * samples_all))
+    srv.admit_all(scs)
+    res = srv.resident
+    n = len(res)
+    bodies = sum(h.scene.n for h in res)
+    log("served: %d scenes resident (%d waiting), %d bodies, footprint %.1f GiB, device %.1f GiB, %.1fs" % (
+        n, len(srv.waiting), bodies, srv.footprint() / 2 ** 30, srv.device_bytes() / 2 ** 30,
+        time.perf_counter() - t0))
     t0 = time.perf_counter()
-    st = []
     for k in range(settle_steps):
-        st = srv.system.step(1)
-        if k % 50 == 0:
-            log("served settle step %d: %d contacts" % (k, sum(x.n_contacts for x in st)))
+        srv.drain([srv.submit(h) for h in res])
+        if k % 25 == 0:
+            log("served settle step %d: %.1fs" % (k, time.perf_counter() - t0))
     settle_s = time.perf_counter() - t0
     out = {"workload": "%d seeded random piles of 10-1000 bodies (log-uniform), spheres + boxes, 32^3 grids, "
-                       "FP64 Newton, inner PCG %s to %g, admitted by HBM footprint, %d steps from the seeded "
-                       "drop before the requests" % (
+                       "FP64 Newton, inner PCG %s to %g, admitted by HBM footprint into %d shards (contexts, "
+                       "one stream and worker each), %d steps from the seeded drop before the requests" % (
                            n_scenes, "FP64" if scs[0].params.pcg_precision == PCG_FP64 else "FP32",
-                           scs[0].params.pcg_tol, settle_steps),
-           "resident_scenes": n, "rejected_scenes": len(srv.rejected), "bodies": bodies,
-           "footprint_gib": srv.footprint / 2 ** 30,
-           "contacts_resident": int(sum(x.n_contacts for x in st)) if st else 0,
-           "settle_s": settle_s, "duration_s": duration, "rates": []}
+                           scs[0].params.pcg_tol, shards, settle_steps),
+           "resident_scenes": n, "waiting_scenes": len(srv.waiting), "bodies": bodies,
+           "footprint_gib": srv.footprint() / 2 ** 30, "device_gib": srv.device_bytes() / 2 ** 30,
+           "settle_s": settle_s, "settle_scene_steps_per_s": n * settle_steps / settle_s,
+           "duration_s": duration, "rates": []}
     for rate in rates:
         reqs = serve.poisson_requests(rate, duration, n, seed=seed)
-        served = srv.run(reqs, batched=True, time_limit=duration * 4)
+        served = srv.run(reqs, time_limit=duration * 4)
         summ = serve.latency_summary(served)
         summ["rate_req_per_s"] = rate
         summ["dropped"] = len(reqs) - summ["requests"]
         log("served %g req/s: %s" % (rate, summ))
         out["rates"].append(summ)
     reqs = serve.poisson_requests(plain_rate, duration, n, seed=seed)
-    served = srv.run(reqs, batched=False, time_limit=duration * 4)
+    served = srv.run_plain(reqs, time_limit=duration * 4)
     summ = serve.latency_summary(served)
     summ["rate_req_per_s"] = plain_rate
     summ["dropped"] = len(reqs) - summ["requests"]
     log("plain %g req/s: %s" % (plain_rate, summ))
     out["plain_per_scene"] = summ
     # precision checks: one served request of three resident scenes
-    sizes = np.array([sc.n for sc in srv.scenes])
-    order = np.argsort(sizes)
-    picks = [int(order[-1]), int(order[len(order) // 2]), int(order[0])]
+    order = sorted(res, key=lambda h: h.scene.n)
+    picks = [order[-1], order[len(order) // 2], order[0]]
     checks = []
-    for k in picks:
-        start = tuple(x.copy() for x in srv.system.state(k))
-        srv.run([serve.Request(k, 0.0)])
-        checks.append(("served", srv.scenes[k], srv.scenes[k].params, start,
-                       tuple(x.copy() for x in srv.system.state(k)), None))
-    out["precision_scenes"] = ["%s (%d bodies)" % (srv.scenes[k].name, srv.scenes[k].n) for k in picks]
+    for h in picks:
+        start = tuple(x.copy() for x in srv.state(h))
+        srv.drain([srv.submit(h)])
+        checks.append(("served", h.scene, h.scene.params, start, tuple(x.copy() for x in srv.state(h)), None))
+    out["precision_scenes"] = ["%s (%d bodies)" % (h.scene.name, h.scene.n) for h in picks]
     srv.close()
     return out, checks
 
@@ -433,8 +438,8 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-rates", default="100,200,500,1000,2000,5000")
     ap.add_argument("--served-duration", type=float, default=2.0)
-    ap.add_argument("--served-scenes", type=int, default=1024)
-    ap.add_argument("--served-settle", type=int, default=150)
+    ap.add_argument("--served-scenes", type=int, default=1200)
+    ap.add_argument("--served-settle", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "

@@ -154,6 +154,40 @@ int pdc_set_state(pdc_context* ctx, const double* pos, const double* quat_xyzw,
                   const double* vel, const double* angvel);
 int pdc_get_state(pdc_context* ctx, double* pos, double* quat_xyzw,
                   double* vel, double* angvel);
+/* The same for bodies [first, first + n) only (arrays of n entries). */
+int pdc_set_state_range(pdc_context* ctx, int32_t first, int32_t n, const double* pos,
+                        const double* quat_xyzw, const double* vel, const double* angvel);
+int pdc_get_state_range(pdc_context* ctx, int32_t first, int32_t n, double* pos,
+                        double* quat_xyzw, double* vel, double* angvel);
+
+/* Resident scenes that come and go (no counterpart in the reference, which
+ * loads one scene per process, FileLoader.cpp:14-259).
+ * pdc_append_bodies adds n bodies at the end of the uploaded table as a new
+ * last scene (grids and samples built for them only, every other body and
+ * scene kept, the other scenes' friction kept); *first_body receives the
+ * index of the first.  pdc_replace_bodies overwrites bodies [first, first +
+ * n) in place (the bodies keep their scene): their surface samples must fit
+ * the sample range the replaced bodies held, and when they do not fill it
+ * the last body must emit no samples (it keeps the rest, which are never
+ * tested).  Both take the new bodies' state (a NULL array gives the identity
+ * pose or zero velocities).  A static body far from everything that emits no
+ * samples is inert: it changes no result of its scene. */
+int pdc_append_bodies(pdc_context* ctx, int32_t n, const int32_t* shape_type,
+                      const double* dims, const double* mass, const double* inertia_diag,
+                      const int32_t* is_static, const int32_t* emit_samples,
+                      const double* pos, const double* quat_xyzw, const double* vel,
+                      const double* angvel, int32_t* first_body);
+int pdc_replace_bodies(pdc_context* ctx, int32_t first, int32_t n, const int32_t* shape_type,
+                       const double* dims, const double* mass, const double* inertia_diag,
+                       const int32_t* is_static, const int32_t* emit_samples,
+                       const double* pos, const double* quat_xyzw, const double* vel,
+                       const double* angvel);
+/* Grows the capacity of the body table to `bodies` bodies and `samples`
+ * surface samples ahead of appends (so that appends do not copy the table). */
+int pdc_reserve_bodies(pdc_context* ctx, int64_t bodies, int64_t samples);
+/* Device memory held by the context's buffers, in bytes (the body table,
+ * grids, samples, contacts and solver state as they stand). */
+int pdc_device_bytes(const pdc_context* ctx, int64_t* bytes);
 
 /* ---- (a) contact generation ----------------------------------------------- */
 /* Runs broadphase, grid-SDF sample tests, compaction, the reference's
@@ -234,8 +268,8 @@ int pdc_step_scenes(pdc_context* ctx, int32_t n_steps, int32_t n_selected,
                     const int32_t* scene_ids, pdc_solve_stats* stats);
 
 /* Work shapes, per scene size class (no counterpart in the reference, whose
- * CPU loops have none).  A scene's class is its body count: < 64, < 256,
- * < 1024, >= 1024.  contact_tile: contacts per workgroup of the per-contact
+ * CPU loops have none).  A scene's class is its count of bodies that are
+ * dynamic or emit surface samples: < 64, < 256, < 1024, >= 1024.  contact_tile: contacts per workgroup of the per-contact
  * kernels (64, 128 or 256); body_waves: bodies (and body pairs) per workgroup
  * of the one-wavefront-per-body kernels (1, 2 or 4); pcg_chunk: PCG
  * launches the host queues at a time (8, 16 or 32, for every class; 0:

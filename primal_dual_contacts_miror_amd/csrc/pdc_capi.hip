@@ -18,6 +18,7 @@ using pdc::HipError;
 
 namespace pdc {
 thread_local Arena* t_arena = nullptr;
+thread_local int64_t* t_dev_bytes = nullptr;
 }
 
 // ---- device scratch of the entry point that allocates nothing --------------
@@ -81,9 +82,16 @@ void validate(const pdc_params* p) {
 	if (p->spring_based_force && !(p->spring_k > 0)) throw ConfigError("spring_k must be > 0");
 }
 
+// The buffers' byte counter of the context a call works on, for its duration.
+struct ByteCounter {
+	explicit ByteCounter(pdc_context* ctx) { pdc::t_dev_bytes = &ctx->dev_bytes; }
+	~ByteCounter() { pdc::t_dev_bytes = nullptr; }
+};
+
 template <class F>
 int guard(pdc_context* ctx, F fn) {
 	if (!ctx) return PDC_ERR_INVALID_ARGUMENT;
+	ByteCounter counter(ctx);
 	try {
 		ctx->err.clear();
 		PDC_HIP(hipSetDevice(ctx->device));
@@ -223,6 +231,7 @@ int pdc_create(int device, const pdc_params* params, pdc_context** out) {
 
 void pdc_destroy(pdc_context* ctx) {
 	if (!ctx) return;
+	ByteCounter counter(ctx);
 	(void)hipSetDevice(ctx->device);
 	if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
 	for (auto& G : ctx->pcg_graph) {
@@ -258,36 +267,119 @@ int pdc_configure(pdc_context* ctx, const pdc_params* params) {
 
 const char* pdc_last_error(const pdc_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+static void check_bodies(const char* who, int n, const int32_t* shape_type, const double* dims, const double* mass,
+                         const double* inertia_diag, const int32_t* is_static) {
+	const std::string w(who);
+	if (n < 0 || (n > 0 && (!shape_type || !dims || !mass || !inertia_diag || !is_static)))
+		throw std::invalid_argument(w + ": null array or negative count");
+	for (int i = 0; i < n; ++i) {
+		if (shape_type[i] != PDC_SHAPE_BOX && shape_type[i] != PDC_SHAPE_SPHERE)
+			throw std::invalid_argument(w + ": unknown shape type");
+		for (int a = 0; a < 3; ++a)
+			if (!(dims[3 * i + a] > 0) && !(shape_type[i] == PDC_SHAPE_SPHERE && a > 0))
+				throw std::invalid_argument(w + ": dimensions must be > 0");
+		if (!(mass[i] > 0)) throw std::invalid_argument(w + ": mass must be > 0");
+		for (int a = 0; a < 3; ++a)
+			if (!(inertia_diag[3 * i + a] > 0)) throw std::invalid_argument(w + ": inertia must be > 0");
+	}
+}
+
+// State of bodies [first, first + n); a null array leaves that part (a body
+// that is new gets the identity pose and zero velocities from the caller).
+static void write_state(pdc_context* ctx, int first, int n, const double* pos, const double* quat,
+                        const double* vel, const double* ang) {
+	hipStream_t s = ctx->stream;
+	const size_t f = (size_t)first, m = (size_t)n;
+	if (pos) PDC_HIP(hipMemcpyAsync(ctx->b_pos.p + 3 * f, pos, 3 * m * sizeof(double), hipMemcpyHostToDevice, s));
+	if (quat) PDC_HIP(hipMemcpyAsync(ctx->b_quat.p + 4 * f, quat, 4 * m * sizeof(double), hipMemcpyHostToDevice, s));
+	if (vel) PDC_HIP(hipMemcpyAsync(ctx->b_vel.p + 3 * f, vel, 3 * m * sizeof(double), hipMemcpyHostToDevice, s));
+	if (ang) PDC_HIP(hipMemcpyAsync(ctx->b_ang.p + 3 * f, ang, 3 * m * sizeof(double), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipStreamSynchronize(s));
+}
+
+static void identity_state(pdc_context* ctx, int first, int n) {
+	std::vector<double> z3(3 * (size_t)n, 0.0), q(4 * (size_t)n, 0.0);
+	for (int i = 0; i < n; ++i) q[4 * i + 3] = 1.0;
+	write_state(ctx, first, n, z3.data(), q.data(), z3.data(), z3.data());
+}
+
 int pdc_upload_bodies(pdc_context* ctx, int32_t n, const int32_t* shape_type, const double* dims,
                       const double* mass, const double* inertia_diag, const int32_t* is_static,
                       const int32_t* emit_samples) {
 	return guard(ctx, [&] {
-		if (n < 0 || (n > 0 && (!shape_type || !dims || !mass || !inertia_diag || !is_static)))
-			throw std::invalid_argument("pdc_upload_bodies: null array or negative count");
-		for (int i = 0; i < n; ++i) {
-			if (shape_type[i] != PDC_SHAPE_BOX && shape_type[i] != PDC_SHAPE_SPHERE)
-				throw std::invalid_argument("pdc_upload_bodies: unknown shape type");
-			for (int a = 0; a < 3; ++a)
-				if (!(dims[3 * i + a] > 0) && !(shape_type[i] == PDC_SHAPE_SPHERE && a > 0))
-					throw std::invalid_argument("pdc_upload_bodies: dimensions must be > 0");
-			if (!(mass[i] > 0)) throw std::invalid_argument("pdc_upload_bodies: mass must be > 0");
-			for (int a = 0; a < 3; ++a)
-				if (!(inertia_diag[3 * i + a] > 0))
-					throw std::invalid_argument("pdc_upload_bodies: inertia must be > 0");
-		}
+		check_bodies("pdc_upload_bodies", n, shape_type, dims, mass, inertia_diag, is_static);
 		pdc::upload_bodies(ctx, n, shape_type, dims, mass, inertia_diag, is_static, emit_samples);
-		// identity state until the caller sets one
-		std::vector<double> z3(3 * (size_t)n, 0.0), q(4 * (size_t)n, 0.0);
-		for (int i = 0; i < n; ++i) q[4 * i + 3] = 1.0;
-		if (n > 0) {
-			PDC_HIP(hipMemcpyAsync(ctx->b_pos.p, z3.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-			PDC_HIP(hipMemcpyAsync(ctx->b_quat.p, q.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-			PDC_HIP(hipMemcpyAsync(ctx->b_vel.p, z3.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-			PDC_HIP(hipMemcpyAsync(ctx->b_ang.p, z3.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-			PDC_HIP(hipStreamSynchronize(ctx->stream));
-		}
+		if (n > 0) identity_state(ctx, 0, n);  // until the caller sets one
 		ctx->C = 0;
 	});
+}
+
+int pdc_append_bodies(pdc_context* ctx, int32_t n, const int32_t* shape_type, const double* dims,
+                      const double* mass, const double* inertia_diag, const int32_t* is_static,
+                      const int32_t* emit_samples, const double* pos, const double* quat_xyzw,
+                      const double* vel, const double* angvel, int32_t* first_body) {
+	return guard(ctx, [&] {
+		if (n < 1) throw std::invalid_argument("pdc_append_bodies: no bodies");
+		check_bodies("pdc_append_bodies", n, shape_type, dims, mass, inertia_diag, is_static);
+		const int first = pdc::append_bodies(ctx, n, shape_type, dims, mass, inertia_diag, is_static, emit_samples);
+		identity_state(ctx, first, n);
+		write_state(ctx, first, n, pos, quat_xyzw, vel, angvel);
+		ctx->C = 0;
+		if (first_body) *first_body = first;
+	});
+}
+
+int pdc_replace_bodies(pdc_context* ctx, int32_t first, int32_t n, const int32_t* shape_type, const double* dims,
+                       const double* mass, const double* inertia_diag, const int32_t* is_static,
+                       const int32_t* emit_samples, const double* pos, const double* quat_xyzw,
+                       const double* vel, const double* angvel) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		check_bodies("pdc_replace_bodies", n, shape_type, dims, mass, inertia_diag, is_static);
+		pdc::replace_bodies(ctx, first, n, shape_type, dims, mass, inertia_diag, is_static, emit_samples);
+		identity_state(ctx, first, n);
+		write_state(ctx, first, n, pos, quat_xyzw, vel, angvel);
+		ctx->C = 0;
+	});
+}
+
+int pdc_set_state_range(pdc_context* ctx, int32_t first, int32_t n, const double* pos, const double* quat_xyzw,
+                        const double* vel, const double* angvel) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		if (first < 0 || n < 0 || first + n > ctx->N) throw std::invalid_argument("pdc_set_state_range: bad range");
+		if (n > 0) write_state(ctx, first, n, pos, quat_xyzw, vel, angvel);
+	});
+}
+
+int pdc_get_state_range(pdc_context* ctx, int32_t first, int32_t n, double* pos, double* quat_xyzw, double* vel,
+                        double* angvel) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		if (first < 0 || n < 0 || first + n > ctx->N) throw std::invalid_argument("pdc_get_state_range: bad range");
+		if (n == 0) return;
+		hipStream_t s = ctx->stream;
+		const size_t f = (size_t)first, m = (size_t)n;
+		if (pos) PDC_HIP(hipMemcpyAsync(pos, ctx->b_pos.p + 3 * f, 3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
+		if (quat_xyzw)
+			PDC_HIP(hipMemcpyAsync(quat_xyzw, ctx->b_quat.p + 4 * f, 4 * m * sizeof(double), hipMemcpyDeviceToHost, s));
+		if (vel) PDC_HIP(hipMemcpyAsync(vel, ctx->b_vel.p + 3 * f, 3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
+		if (angvel) PDC_HIP(hipMemcpyAsync(angvel, ctx->b_ang.p + 3 * f, 3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
+		PDC_HIP(hipStreamSynchronize(s));
+	});
+}
+
+int pdc_reserve_bodies(pdc_context* ctx, int64_t bodies, int64_t samples) {
+	return guard(ctx, [&] {
+		if (bodies < 0 || samples < 0) throw std::invalid_argument("pdc_reserve_bodies: negative capacity");
+		pdc::reserve_bodies(ctx, bodies, samples);
+	});
+}
+
+int pdc_device_bytes(const pdc_context* ctx, int64_t* bytes) {
+	if (!ctx || !bytes) return PDC_ERR_INVALID_ARGUMENT;
+	*bytes = ctx->dev_bytes;
+	return PDC_OK;
 }
 
 int pdc_set_state(pdc_context* ctx, const double* pos, const double* quat_xyzw, const double* vel,
@@ -632,7 +724,7 @@ int pdc_autotune(pdc_context* ctx, int32_t steps, pdc_shapes* chosen) {
 		// scenes per size class
 		std::vector<int> by_class[4];
 		for (int k = 0; k < ctx->S; ++k)
-			by_class[pdc::size_class(ctx->h_sbeg[k + 1] - ctx->h_sbeg[k])].push_back(k);
+			by_class[pdc::size_class(ctx->h_sclass[k])].push_back(k);
 		const pdc_shapes shapes0 = ctx->shapes;
 		pdc_shapes best = ctx->shapes;
 		const int tiles[3] = {64, 128, 256}, waves[3] = {1, 2, 4}, chunks[3] = {8, 16, 32};

@@ -15,9 +15,11 @@
 //      contact list (ids, position, depth, both 6x3 frames) in pair order.
 // Normal convention (MeshMeshContactGenerator.cpp:43-55): a sample of body a
 // inside b gives normal -grad(sd_b); a sample of b inside a gives +grad(sd_a).
+#include <algorithm>
+#include <climits>
 #include <cmath>
-
 #include <cstring>
+#include <string>
 
 #include "pdc_internal.h"
 #include "pdc_reduce.h"
@@ -89,6 +91,77 @@ __global__ __launch_bounds__(256) void k_build_grids(const GridDesc* __restrict_
 	}
 }
 
+// Grid box of a body: the shape's box padded by grid_pad_frac, its node block
+// at body * grid_n^3 in the pool.
+static GridDesc grid_desc(const pdc_params& p, int shape, const double* dims, int64_t body) {
+	GridDesc g{};
+	double e[3];
+	if (shape == PDC_SHAPE_SPHERE) e[0] = e[1] = e[2] = dims[0];
+	else { e[0] = dims[0]; e[1] = dims[1]; e[2] = dims[2]; }
+	const int gn = p.grid_n;
+	for (int a = 0; a < 3; ++a) {
+		const double h = e[a] * (1.0 + p.grid_pad_frac);
+		g.half[a] = h;
+		g.lo[a] = -h;
+		g.sp[a] = (2.0 * h) / (double)(gn - 1);
+	}
+	g.n = gn;
+	g.offset = body * (int64_t)gn * gn * gn;
+	return g;
+}
+
+// Flags, dimensions, masses, inertia and grid descriptors of bodies
+// [first, first + n), their grids built; the tie band narrowed to the finest
+// spacing among them (`reset`: start it afresh).
+static void write_bodies(pdc_context* ctx, int first, int n, const int* shape, const double* dims,
+                         const double* mass, const double* inertia, const int* is_static, const int* emit,
+                         bool reset) {
+	if (n <= 0) return;
+	const size_t N = (size_t)n, f = (size_t)first;
+	std::vector<int> em(N);
+	for (size_t i = 0; i < N; ++i) em[i] = emit ? (emit[i] != 0) : 1;
+	hipStream_t s = ctx->stream;
+	PDC_HIP(hipMemcpyAsync(ctx->b_shape.p + f, shape, N * sizeof(int), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_static.p + f, is_static, N * sizeof(int), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_emit.p + f, em.data(), N * sizeof(int), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_dims.p + 3 * f, dims, 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_mass.p + f, mass, N * sizeof(double), hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_I0.p + 3 * f, inertia, 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+	if (ctx->h_static.size() < f + N) ctx->h_static.resize(f + N);
+	if (ctx->h_emit.size() < f + N) ctx->h_emit.resize(f + N);
+	std::copy(is_static, is_static + N, ctx->h_static.begin() + f);
+	std::copy(em.begin(), em.end(), ctx->h_emit.begin() + f);
+	std::vector<GridDesc> gd(N);
+	double finest = 1e300;
+	for (size_t b = 0; b < N; ++b) {
+		gd[b] = grid_desc(ctx->p, shape[b], &dims[3 * b], (int64_t)(f + b));
+		for (int a = 0; a < 3; ++a) finest = std::min(finest, gd[b].sp[a]);
+	}
+	// tie band: 1e-7 of the finest grid spacing of the table
+	ctx->tie_band = (reset || ctx->tie_band <= 0) ? 1e-7 * finest : std::min(ctx->tie_band, 1e-7 * finest);
+	PDC_HIP(hipMemcpyAsync(ctx->b_grid.p + f, gd.data(), N * sizeof(GridDesc), hipMemcpyHostToDevice, s));
+	const int gn = ctx->p.grid_n;
+	const int64_t n3 = (int64_t)gn * gn * gn;
+	const int gx = (int)std::min<int64_t>((n3 + kBlock - 1) / kBlock, 4096);
+	for (size_t b0 = 0; b0 < N; b0 += 65535) {
+		const int nb = (int)std::min<size_t>(65535, N - b0);
+		hipLaunchKernelGGL(k_build_grids, dim3(gx, nb), dim3(kBlock), 0, s, ctx->b_grid.p + f + b0,
+		                   ctx->b_shape.p + f + b0, ctx->b_dims.p + 3 * (f + b0), ctx->grid_pool.p);
+		PDC_CHECK_LAUNCH();
+	}
+}
+
+// Body-frame surface samples of n bodies; counts[b] per body.
+static void samples_of(const pdc_params& p, int n, const int* shape, const double* dims, std::vector<double>& smp,
+                       std::vector<int>& counts) {
+	counts.assign(n, 0);
+	for (int b = 0; b < n; ++b) {
+		const size_t before = smp.size();
+		make_samples(shape[b], &dims[3 * b], p.sample_spacing, smp);
+		counts[b] = (int)((smp.size() - before) / 3);
+	}
+}
+
 void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,
                    const double* inertia, const int* is_static, const int* emit) {
 	ctx->N = n;
@@ -99,69 +172,130 @@ void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims
 	ctx->b_ang.ensure(3 * N); ctx->b_acc.ensure(6 * N);
 	ctx->b_R.ensure(9 * N); ctx->b_aabb.ensure(6 * N);
 	ctx->b_grid.ensure(N); ctx->b_soff.ensure(N + 1);
-	std::vector<int> em(N);
-	for (size_t i = 0; i < N; ++i) em[i] = emit ? (emit[i] != 0) : 1;
-	hipStream_t s = ctx->stream;
-	PDC_HIP(hipMemcpyAsync(ctx->b_shape.p, shape, N * sizeof(int), hipMemcpyHostToDevice, s));
-	PDC_HIP(hipMemcpyAsync(ctx->b_static.p, is_static, N * sizeof(int), hipMemcpyHostToDevice, s));
-	PDC_HIP(hipMemcpyAsync(ctx->b_emit.p, em.data(), N * sizeof(int), hipMemcpyHostToDevice, s));
-	PDC_HIP(hipMemcpyAsync(ctx->b_dims.p, dims, 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
-	PDC_HIP(hipMemcpyAsync(ctx->b_mass.p, mass, N * sizeof(double), hipMemcpyHostToDevice, s));
-	PDC_HIP(hipMemcpyAsync(ctx->b_I0.p, inertia, 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
-	ctx->h_static.assign(is_static, is_static + N);
-	// grids
-	std::vector<GridDesc> gd(N);
-	int64_t off = 0;
+	ctx->h_static.assign(N, 0);
+	ctx->h_emit.assign(N, 0);
 	const int gn = ctx->p.grid_n;
-	for (size_t b = 0; b < N; ++b) {
-		double e[3];
-		if (shape[b] == PDC_SHAPE_SPHERE) e[0] = e[1] = e[2] = dims[3 * b];
-		else { e[0] = dims[3 * b]; e[1] = dims[3 * b + 1]; e[2] = dims[3 * b + 2]; }
-		for (int a = 0; a < 3; ++a) {
-			const double h = e[a] * (1.0 + ctx->p.grid_pad_frac);
-			gd[b].half[a] = h;
-			gd[b].lo[a] = -h;
-			gd[b].sp[a] = (2.0 * h) / (double)(gn - 1);
-		}
-		gd[b].n = gn;
-		gd[b].offset = off;
-		off += (int64_t)gn * gn * gn;
-	}
-	ctx->grid_pool.ensure((size_t)off);
-	PDC_HIP(hipMemcpyAsync(ctx->b_grid.p, gd.data(), N * sizeof(GridDesc), hipMemcpyHostToDevice, s));
+	ctx->grid_pool.ensure(std::max<size_t>(N * (size_t)gn * gn * gn, 1));
+	write_bodies(ctx, 0, n, shape, dims, mass, inertia, is_static, emit, true);
+	if (n == 0) ctx->tie_band = 0.0;
 	// samples
 	std::vector<double> smp;
+	std::vector<int> cnt;
+	samples_of(ctx->p, n, shape, dims, smp, cnt);
 	std::vector<int> soff(N + 1, 0);
-	for (size_t b = 0; b < N; ++b) {
-		soff[b] = (int)(smp.size() / 3);
-		make_samples(shape[b], &dims[3 * b], ctx->p.sample_spacing, smp);
-	}
-	soff[N] = (int)(smp.size() / 3);
+	for (size_t b = 0; b < N; ++b) soff[b + 1] = soff[b] + cnt[b];
 	ctx->n_samples = soff[N];
 	ctx->h_soff = soff;
+	hipStream_t s = ctx->stream;
 	ctx->samples.ensure(std::max<size_t>(smp.size(), 3));
-	// tie band: 1e-7 of the finest grid spacing of the scene
-	double finest = 1e300;
-	for (size_t b = 0; b < N; ++b)
-		for (int a = 0; a < 3; ++a) finest = std::min(finest, gd[b].sp[a]);
-	ctx->tie_band = N ? 1e-7 * finest : 0.0;
 	if (!smp.empty())
 		PDC_HIP(hipMemcpyAsync(ctx->samples.p, smp.data(), smp.size() * sizeof(double),
 		                       hipMemcpyHostToDevice, s));
 	PDC_HIP(hipMemcpyAsync(ctx->b_soff.p, soff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-	if (N > 0) {
-		const int64_t n3 = (int64_t)gn * gn * gn;
-		int gx = (int)std::min<int64_t>((n3 + kBlock - 1) / kBlock, 4096);
-		for (size_t b0 = 0; b0 < N; b0 += 65535) {
-			const int nb = (int)std::min<size_t>(65535, N - b0);
-			hipLaunchKernelGGL(k_build_grids, dim3(gx, nb), dim3(kBlock), 0, s, ctx->b_grid.p + b0,
-			                   ctx->b_shape.p + b0, ctx->b_dims.p + 3 * b0, ctx->grid_pool.p);
-			PDC_CHECK_LAUNCH();
-		}
-	}
 	PDC_HIP(hipStreamSynchronize(s));
 	const int one[2] = {0, (int)N};
 	set_scenes(ctx, 1, one);
+}
+
+void reserve_bodies(pdc_context* ctx, int64_t bodies, int64_t samples) {
+	if (bodies > INT32_MAX || samples > INT32_MAX) throw HipError("reservation beyond 2^31", PDC_ERR_CAPACITY);
+	const size_t N0 = (size_t)ctx->N, N1 = std::max<size_t>((size_t)bodies, N0);
+	hipStream_t s = ctx->stream;
+	PDC_HIP(hipStreamSynchronize(s));
+	ctx->b_shape.grow_keep(N1, N0, s); ctx->b_static.grow_keep(N1, N0, s); ctx->b_emit.grow_keep(N1, N0, s);
+	ctx->b_dims.grow_keep(3 * N1, 3 * N0, s); ctx->b_mass.grow_keep(N1, N0, s);
+	ctx->b_I0.grow_keep(3 * N1, 3 * N0, s);
+	ctx->b_pos.grow_keep(3 * N1, 3 * N0, s); ctx->b_quat.grow_keep(4 * N1, 4 * N0, s);
+	ctx->b_vel.grow_keep(3 * N1, 3 * N0, s); ctx->b_ang.grow_keep(3 * N1, 3 * N0, s);
+	ctx->b_acc.grow_keep(6 * N1, 6 * N0, s);
+	ctx->b_R.grow_keep(9 * N1, 0, s); ctx->b_aabb.grow_keep(6 * N1, 0, s);
+	ctx->b_grid.grow_keep(N1, N0, s); ctx->b_soff.grow_keep(N1 + 1, N0 + 1, s);
+	const size_t n3 = (size_t)ctx->p.grid_n * ctx->p.grid_n * ctx->p.grid_n;
+	ctx->grid_pool.grow_keep(std::max<size_t>(N1 * n3, 1), N0 * n3, s);
+	const size_t total0 = ctx->h_soff.empty() ? 0 : (size_t)ctx->h_soff[N0];
+	ctx->samples.grow_keep(3 * std::max<size_t>((size_t)samples, std::max<size_t>(total0, 1)), 3 * total0, s);
+}
+
+int append_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,
+                  const double* inertia, const int* is_static, const int* emit) {
+	const size_t N0 = (size_t)ctx->N, N1 = N0 + (size_t)n;
+	if ((int64_t)N1 > INT32_MAX) throw HipError("body table beyond 2^31 bodies", PDC_ERR_CAPACITY);
+	hipStream_t s = ctx->stream;
+	PDC_HIP(hipStreamSynchronize(s));
+	ctx->b_shape.grow_keep(N1, N0, s); ctx->b_static.grow_keep(N1, N0, s); ctx->b_emit.grow_keep(N1, N0, s);
+	ctx->b_dims.grow_keep(3 * N1, 3 * N0, s); ctx->b_mass.grow_keep(N1, N0, s);
+	ctx->b_I0.grow_keep(3 * N1, 3 * N0, s);
+	ctx->b_pos.grow_keep(3 * N1, 3 * N0, s); ctx->b_quat.grow_keep(4 * N1, 4 * N0, s);
+	ctx->b_vel.grow_keep(3 * N1, 3 * N0, s); ctx->b_ang.grow_keep(3 * N1, 3 * N0, s);
+	ctx->b_acc.grow_keep(6 * N1, 6 * N0, s);
+	ctx->b_R.grow_keep(9 * N1, 0, s); ctx->b_aabb.grow_keep(6 * N1, 0, s);
+	ctx->b_grid.grow_keep(N1, N0, s); ctx->b_soff.grow_keep(N1 + 1, N0 + 1, s);
+	const size_t n3 = (size_t)ctx->p.grid_n * ctx->p.grid_n * ctx->p.grid_n;
+	ctx->grid_pool.grow_keep(std::max<size_t>(N1 * n3, 1), N0 * n3, s);
+	std::vector<double> smp;
+	std::vector<int> cnt;
+	samples_of(ctx->p, n, shape, dims, smp, cnt);
+	const int64_t total0 = ctx->h_soff.empty() ? 0 : ctx->h_soff[N0];
+	const int64_t total1 = total0 + (int64_t)(smp.size() / 3);
+	if (total1 > INT32_MAX) throw HipError("sample table beyond 2^31 samples", PDC_ERR_CAPACITY);
+	ctx->samples.grow_keep(std::max<size_t>(3 * (size_t)total1, 3), 3 * (size_t)total0, s);
+	ctx->h_soff.resize(N1 + 1);
+	if (N0 == 0) ctx->h_soff[0] = 0;
+	for (size_t b = 0; b < (size_t)n; ++b) ctx->h_soff[N0 + b + 1] = ctx->h_soff[N0 + b] + cnt[b];
+	ctx->n_samples = ctx->h_soff[N1];
+	if (!smp.empty())
+		PDC_HIP(hipMemcpyAsync(ctx->samples.p + 3 * total0, smp.data(), smp.size() * sizeof(double),
+		                       hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_soff.p + N0, ctx->h_soff.data() + N0, ((size_t)n + 1) * sizeof(int),
+	                       hipMemcpyHostToDevice, s));
+	ctx->N = (int)N1;
+	write_bodies(ctx, (int)N0, n, shape, dims, mass, inertia, is_static, emit, N0 == 0);
+	PDC_HIP(hipStreamSynchronize(s));
+	// the new bodies form a new last scene; the other scenes keep their friction
+	std::vector<int> sbeg = ctx->h_sbeg;
+	std::vector<double> mu = ctx->h_mu;
+	if (N0 == 0) sbeg.assign(1, 0);
+	sbeg.push_back((int)N1);
+	set_scenes(ctx, (int)sbeg.size() - 1, sbeg.data());
+	for (size_t k = 0; k < mu.size() && k + 1 < sbeg.size() && N0 > 0; ++k) ctx->h_mu[k] = mu[k];
+	return (int)N0;
+}
+
+void replace_bodies(pdc_context* ctx, int first, int n, const int* shape, const double* dims, const double* mass,
+                    const double* inertia, const int* is_static, const int* emit) {
+	if (first < 0 || n < 1 || first + n > ctx->N) throw std::invalid_argument("body range outside the table");
+	std::vector<double> smp;
+	std::vector<int> cnt;
+	samples_of(ctx->p, n, shape, dims, smp, cnt);
+	const int64_t lo = ctx->h_soff[first], hi = ctx->h_soff[first + n];
+	const int64_t m = (int64_t)(smp.size() / 3);
+	const bool last_emits = emit ? emit[n - 1] != 0 : true;
+	if (m > hi - lo)
+		throw HipError("the replacing bodies have " + std::to_string(m) + " samples, more than the " +
+		                   std::to_string(hi - lo) + " their range holds",
+		               PDC_ERR_CAPACITY);
+	if (m < hi - lo && last_emits)
+		throw std::invalid_argument("the last replacing body must emit no samples when the samples do not "
+		                            "fill the range (it keeps the rest)");
+	hipStream_t s = ctx->stream;
+	PDC_HIP(hipStreamSynchronize(s));
+	for (int b = 0; b + 1 < n; ++b) ctx->h_soff[first + b + 1] = ctx->h_soff[first + b] + cnt[b];
+	if (!smp.empty())
+		PDC_HIP(hipMemcpyAsync(ctx->samples.p + 3 * lo, smp.data(), smp.size() * sizeof(double),
+		                       hipMemcpyHostToDevice, s));
+	PDC_HIP(hipMemcpyAsync(ctx->b_soff.p + first, ctx->h_soff.data() + first, ((size_t)n + 1) * sizeof(int),
+	                       hipMemcpyHostToDevice, s));
+	write_bodies(ctx, first, n, shape, dims, mass, inertia, is_static, emit, false);
+	PDC_HIP(hipStreamSynchronize(s));
+	count_scene_classes(ctx);
+}
+
+void count_scene_classes(pdc_context* ctx) {
+	const int S = ctx->S;
+	ctx->h_sclass.assign(S, 0);
+	for (int k = 0; k < S; ++k)
+		for (int b = ctx->h_sbeg[k]; b < ctx->h_sbeg[k + 1]; ++b)
+			if (!ctx->h_static[b] || ctx->h_emit[b]) ctx->h_sclass[k] += 1;
 }
 
 void set_scenes(pdc_context* ctx, int S, const int* sbeg) {
@@ -212,6 +346,7 @@ void set_scenes(pdc_context* ctx, int S, const int* sbeg) {
 		ctx->h_ss_cap = S;
 	}
 	ctx->scene_stats.assign(S, pdc_solve_stats{});
+	count_scene_classes(ctx);
 	set_stepped(ctx, 0, nullptr);
 }
 

@@ -83,6 +83,12 @@ struct Arena {
 	}
 };
 extern thread_local Arena* t_arena;  // the calling thread's active arena, or null
+// Device bytes held by the buffers of the context the calling thread is
+// working on (set by every entry point), or null.
+extern thread_local int64_t* t_dev_bytes;
+inline void count_bytes(int64_t d) {
+	if (t_dev_bytes) *t_dev_bytes += d;
+}
 
 // Growable device buffer (reallocates only when the capacity grows).
 template <class T>
@@ -100,9 +106,35 @@ struct DBuf {
 		if (n <= cap) return;
 		size_t nc = cap ? cap : 256;
 		while (nc < n) nc = nc + nc / 2 + 1;
-		if (p) PDC_HIP(hipFree(p));
+		if (p) {
+			PDC_HIP(hipFree(p));
+			count_bytes(-(int64_t)(cap * sizeof(T)));
+		}
 		p = nullptr;
+		cap = 0;
 		PDC_HIP(hipMalloc(&p, nc * sizeof(T)));
+		count_bytes((int64_t)(nc * sizeof(T)));
+		cap = nc;
+	}
+	// Grows to at least n elements keeping the first `keep` elements (copied
+	// on `s`, which is synchronised).
+	void grow_keep(size_t n, size_t keep, hipStream_t s) {
+		if (n <= cap) return;
+		if (t_arena) throw HipError("a body table cannot grow inside the no-allocation call", PDC_ERR_STATE);
+		size_t nc = cap ? cap : 256;
+		while (nc < n) nc = nc + nc / 2 + 1;
+		T* q = nullptr;
+		PDC_HIP(hipMalloc(&q, nc * sizeof(T)));
+		count_bytes((int64_t)(nc * sizeof(T)));
+		if (p && keep) {
+			PDC_HIP(hipMemcpyAsync(q, p, std::min(keep, cap) * sizeof(T), hipMemcpyDeviceToDevice, s));
+			PDC_HIP(hipStreamSynchronize(s));
+		}
+		if (p) {
+			PDC_HIP(hipFree(p));
+			count_bytes(-(int64_t)(cap * sizeof(T)));
+		}
+		p = q;
 		cap = nc;
 	}
 	void release() {
@@ -112,7 +144,10 @@ struct DBuf {
 			cap = 0;
 			return;
 		}
-		if (p) (void)hipFree(p);
+		if (p) {
+			(void)hipFree(p);
+			count_bytes(-(int64_t)(cap * sizeof(T)));
+		}
 		p = nullptr;
 		cap = 0;
 	}
@@ -251,7 +286,8 @@ struct pdc_context {
 	pdc::DBuf<int> b_soff;                                 // N+1 sample offsets
 	pdc::DBuf<double> samples;                             // 3 per sample
 	std::vector<int> h_soff;                               // host copy of sample offsets
-	std::vector<int> h_static;
+	std::vector<int> h_static, h_emit;                     // host copies of the flags
+	int64_t dev_bytes = 0;                                 // device bytes held by the context's buffers
 
 	// ---- broadphase / pairs ----
 	pdc::DBuf<int> bp_cnt, bp_off;                          // per body a: candidates b > a
@@ -316,6 +352,8 @@ struct pdc_context {
 	// ---- scenes ----
 	int S = 1;                                                // independent scenes
 	std::vector<int> h_sbeg{0, 0};                           // per scene: first body (S + 1)
+	std::vector<int> h_sclass;                                // per scene: bodies that are dynamic or emit samples
+	                                                          // (picks the work shapes: inert padding never counts)
 	std::vector<int> h_step;                                  // per scene: stepped by the current call
 	std::vector<double> h_mu;                                 // per scene friction coefficient
 	pdc::DBuf<int> b_scene;                                   // per body: scene
@@ -378,6 +416,19 @@ void kt_end_span(pdc_context* ctx, hipEvent_t ev0, int launches);
 // pdc_contacts.hip
 void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,
                    const double* inertia, const int* is_static, const int* emit);
+// Bodies appended at the end of the table as a new last scene (grids and
+// samples built for them only); returns the index of the first.
+int append_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,
+                  const double* inertia, const int* is_static, const int* emit);
+// Bodies [first, first + n) replaced in place; their samples must fit the
+// sample range those bodies held, the last body (if it emits no samples)
+// keeping what the others leave.
+void replace_bodies(pdc_context* ctx, int first, int n, const int* shape, const double* dims,
+                    const double* mass, const double* inertia, const int* is_static, const int* emit);
+// Capacity of the body table and its samples grown ahead (contents kept).
+void reserve_bodies(pdc_context* ctx, int64_t bodies, int64_t samples);
+// Per-scene count of bodies that are dynamic or emit samples.
+void count_scene_classes(pdc_context* ctx);
 void detect_contacts(pdc_context* ctx);  // fills c_* and ctx->C
 
 // pdc_solver.hip

@@ -344,7 +344,7 @@ static void scene_ranges(pdc_context* ctx) {
 	std::vector<Tile> t;
 	std::vector<int> tb;
 	// work shapes of each scene's size class
-	auto cls = [&](int sc) { return size_class(ctx->h_sbeg[sc + 1] - ctx->h_sbeg[sc]); };
+	auto cls = [&](int sc) { return size_class(ctx->h_sclass[sc]); };
 	auto per_contact = [&](int sc) { return ctx->shapes.contact_tile[cls(sc)]; };
 	auto per_wave = [&](int sc) { return ctx->shapes.body_waves[cls(sc)]; };
 	auto fixed = [](int per) { return [per](int) { return per; }; };

@@ -38,6 +38,8 @@ EXPORTS = [
     "pdc_get_timings", "pdc_reset_timings", "pdc_enable_kernel_timing", "pdc_kernel_timing",
     "pdc_set_scenes", "pdc_scene_count", "pdc_set_scene_friction", "pdc_step_scenes",
     "pdc_get_shapes", "pdc_set_shapes", "pdc_autotune",
+    "pdc_set_state_range", "pdc_get_state_range", "pdc_append_bodies", "pdc_replace_bodies",
+    "pdc_device_bytes", "pdc_reserve_bodies",
 ]
 
 
@@ -78,6 +80,12 @@ def load_library(path=LIB_PATH):
     L.pdc_last_error.restype = ctypes.c_char_p
     L.pdc_upload_bodies.argtypes = [V, ctypes.c_int32, _I, _D, _D, _D, _I, _I]
     L.pdc_set_state.argtypes = [V, _D, _D, _D, _D]
+    L.pdc_set_state_range.argtypes = [V, ctypes.c_int32, ctypes.c_int32, _D, _D, _D, _D]
+    L.pdc_get_state_range.argtypes = [V, ctypes.c_int32, ctypes.c_int32, _D, _D, _D, _D]
+    L.pdc_append_bodies.argtypes = [V, ctypes.c_int32, _I, _D, _D, _D, _I, _I, _D, _D, _D, _D, _I]
+    L.pdc_replace_bodies.argtypes = [V, ctypes.c_int32, ctypes.c_int32, _I, _D, _D, _D, _I, _I, _D, _D, _D, _D]
+    L.pdc_device_bytes.argtypes = [V, ctypes.POINTER(ctypes.c_int64)]
+    L.pdc_reserve_bodies.argtypes = [V, ctypes.c_int64, ctypes.c_int64]
     L.pdc_get_state.argtypes = [V, _D, _D, _D, _D]
     L.pdc_detect_contacts.argtypes = [V, _I, _I]
     L.pdc_get_contacts.argtypes = [V, ctypes.c_int32, _I, _D, _D, _D, _D]
@@ -175,6 +183,50 @@ class Context:
         pos, quat, vel, ang = np.zeros((n, 3)), np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3))
         self._check(self._L.pdc_get_state(self.h, _d(pos), _d(quat), _d(vel), _d(ang)))
         return pos, quat, vel, ang
+
+    def get_state_range(self, first, n):
+        pos, quat, vel, ang = np.zeros((n, 3)), np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3))
+        self._check(self._L.pdc_get_state_range(self.h, int(first), int(n), _d(pos), _d(quat), _d(vel), _d(ang)))
+        return pos, quat, vel, ang
+
+    def set_state_range(self, first, pos, quat, vel, angvel):
+        n = int(np.asarray(pos).reshape(-1, 3).shape[0])
+        self._check(self._L.pdc_set_state_range(self.h, int(first), n, _d(_f64(pos)), _d(_f64(quat)),
+                                                _d(_f64(vel)), _d(_f64(angvel))))
+
+    @staticmethod
+    def _bodies(shape_type, dims, mass, inertia, is_static, emit, pos, quat, vel, angvel):
+        st = np.ascontiguousarray(shape_type, dtype=np.int32)
+        n = int(st.shape[0])
+        em = np.ones(n, np.int32) if emit is None else np.ascontiguousarray(emit, dtype=np.int32)
+        state = [None if x is None else _f64(x) for x in (pos, quat, vel, angvel)]
+        arrs = [st, _f64(dims), _f64(mass), _f64(inertia), np.ascontiguousarray(is_static, dtype=np.int32), em]
+        return n, arrs, state
+
+    def append_bodies(self, shape_type, dims, mass, inertia, is_static, emit=None, pos=None, quat=None,
+                      vel=None, angvel=None):
+        """Appends bodies as a new last scene; returns the index of the first."""
+        n, a, st = self._bodies(shape_type, dims, mass, inertia, is_static, emit, pos, quat, vel, angvel)
+        first = ctypes.c_int32(0)
+        self._check(self._L.pdc_append_bodies(self.h, n, _i(a[0]), _d(a[1]), _d(a[2]), _d(a[3]), _i(a[4]),
+                                              _i(a[5]), _d(st[0]), _d(st[1]), _d(st[2]), _d(st[3]),
+                                              ctypes.byref(first)))
+        self.n += n
+        return first.value
+
+    def replace_bodies(self, first, shape_type, dims, mass, inertia, is_static, emit=None, pos=None, quat=None,
+                       vel=None, angvel=None):
+        n, a, st = self._bodies(shape_type, dims, mass, inertia, is_static, emit, pos, quat, vel, angvel)
+        self._check(self._L.pdc_replace_bodies(self.h, int(first), n, _i(a[0]), _d(a[1]), _d(a[2]), _d(a[3]),
+                                               _i(a[4]), _i(a[5]), _d(st[0]), _d(st[1]), _d(st[2]), _d(st[3])))
+
+    def reserve_bodies(self, bodies, samples):
+        self._check(self._L.pdc_reserve_bodies(self.h, int(bodies), int(samples)))
+
+    def device_bytes(self):
+        b = ctypes.c_int64(0)
+        self._check(self._L.pdc_device_bytes(self.h, ctypes.byref(b)))
+        return b.value
 
     def detect_contacts(self):
         c, t = ctypes.c_int32(0), ctypes.c_int32(0)
