@@ -432,10 +432,10 @@ def main():
     ap.add_argument("--state", default=None, help="npz with pos, quat, vel, angvel to start from")
     ap.add_argument("--settle-steps", type=int, default=None,
                     help="settle on the device for this many steps instead of loading the settled state")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched and served legs")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=200)
-    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-served", action="store_true", help="skip the served leg")
     ap.add_argument("--served-rates", default="100,200,500,1000,2000,5000")
     ap.add_argument("--served-duration", type=float, default=2.0)
     ap.add_argument("--served-scenes", type=int, default=1200)
@@ -530,11 +530,11 @@ def main():
         bs, out["batched"], bchecks = batched_leg(args.batched_scenes, args.batched_steps, 16, 10, overrides)
         bs.ctx.close()
         checks += bchecks
-        if not args.no_served:
-            rates = [float(r) for r in args.served_rates.split(",") if r]
-            out["served"], schecks = served_leg(args.served_scenes, args.served_settle, rates,
-                                                args.served_duration, 40.0, overrides)
-            checks += schecks
+    if not args.no_served:
+        rates = [float(r) for r in args.served_rates.split(",") if r]
+        out["served"], schecks = served_leg(args.served_scenes, args.served_settle, rates,
+                                            args.served_duration, 40.0, overrides)
+        checks += schecks
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         precision, out["cpu_baseline"], _ = cpu_baseline(sc, state, args.cpu_budget, threads, checks)

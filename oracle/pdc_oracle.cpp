@@ -575,8 +575,19 @@ int bcsr_pcg(const BCSR& A, const double* b, double tol, int maxit, double* x, b
 	}
 	std::vector<double> r(b, b + m), z(m), p(m), q(m);
 	if (warm) {
+		/* the warm start is kept only when its residual is smaller than that
+		 * of the zero start, |b - A x0| < |b|: a start far from the solution
+		 * (the increment of a gradient-descent fallback, or of an earlier
+		 * Newton iteration) puts the relative tolerance below the rounding
+		 * level of its residual */
 		bcsr_mv(A, x, q.data());
 		for (size_t i = 0; i < m; ++i) r[i] = b[i] - q[i];
+		const double r2 = fixed_sum(m, [&](size_t i) { return r[i] * r[i]; });
+		const double b2 = fixed_sum(m, [&](size_t i) { return b[i] * b[i]; });
+		if (!(r2 < b2)) {
+			std::fill(x, x + m, 0.0);
+			r.assign(b, b + m);
+		}
 	} else {
 		std::fill(x, x + m, 0.0);
 	}

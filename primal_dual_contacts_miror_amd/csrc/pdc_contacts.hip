@@ -202,18 +202,18 @@ void reserve_bodies(pdc_context* ctx, int64_t bodies, int64_t samples) {
 	const size_t N0 = (size_t)ctx->N, N1 = std::max<size_t>((size_t)bodies, N0);
 	hipStream_t s = ctx->stream;
 	PDC_HIP(hipStreamSynchronize(s));
-	ctx->b_shape.grow_keep(N1, N0, s); ctx->b_static.grow_keep(N1, N0, s); ctx->b_emit.grow_keep(N1, N0, s);
-	ctx->b_dims.grow_keep(3 * N1, 3 * N0, s); ctx->b_mass.grow_keep(N1, N0, s);
-	ctx->b_I0.grow_keep(3 * N1, 3 * N0, s);
-	ctx->b_pos.grow_keep(3 * N1, 3 * N0, s); ctx->b_quat.grow_keep(4 * N1, 4 * N0, s);
-	ctx->b_vel.grow_keep(3 * N1, 3 * N0, s); ctx->b_ang.grow_keep(3 * N1, 3 * N0, s);
-	ctx->b_acc.grow_keep(6 * N1, 6 * N0, s);
-	ctx->b_R.grow_keep(9 * N1, 0, s); ctx->b_aabb.grow_keep(6 * N1, 0, s);
-	ctx->b_grid.grow_keep(N1, N0, s); ctx->b_soff.grow_keep(N1 + 1, N0 + 1, s);
+	ctx->b_shape.grow_keep(N1, N0, s, true); ctx->b_static.grow_keep(N1, N0, s, true); ctx->b_emit.grow_keep(N1, N0, s, true);
+	ctx->b_dims.grow_keep(3 * N1, 3 * N0, s, true); ctx->b_mass.grow_keep(N1, N0, s, true);
+	ctx->b_I0.grow_keep(3 * N1, 3 * N0, s, true);
+	ctx->b_pos.grow_keep(3 * N1, 3 * N0, s, true); ctx->b_quat.grow_keep(4 * N1, 4 * N0, s, true);
+	ctx->b_vel.grow_keep(3 * N1, 3 * N0, s, true); ctx->b_ang.grow_keep(3 * N1, 3 * N0, s, true);
+	ctx->b_acc.grow_keep(6 * N1, 6 * N0, s, true);
+	ctx->b_R.grow_keep(9 * N1, 0, s, true); ctx->b_aabb.grow_keep(6 * N1, 0, s, true);
+	ctx->b_grid.grow_keep(N1, N0, s, true); ctx->b_soff.grow_keep(N1 + 1, N0 + 1, s, true);
 	const size_t n3 = (size_t)ctx->p.grid_n * ctx->p.grid_n * ctx->p.grid_n;
-	ctx->grid_pool.grow_keep(std::max<size_t>(N1 * n3, 1), N0 * n3, s);
+	ctx->grid_pool.grow_keep(std::max<size_t>(N1 * n3, 1), N0 * n3, s, true);
 	const size_t total0 = ctx->h_soff.empty() ? 0 : (size_t)ctx->h_soff[N0];
-	ctx->samples.grow_keep(3 * std::max<size_t>((size_t)samples, std::max<size_t>(total0, 1)), 3 * total0, s);
+	ctx->samples.grow_keep(3 * std::max<size_t>((size_t)samples, std::max<size_t>(total0, 1)), 3 * total0, s, true);
 }
 
 int append_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,

@@ -116,12 +116,12 @@ struct DBuf {
 		count_bytes((int64_t)(nc * sizeof(T)));
 		cap = nc;
 	}
-	// Grows to at least n elements keeping the first `keep` elements (copied
-	// on `s`, which is synchronised).
-	void grow_keep(size_t n, size_t keep, hipStream_t s) {
+	// Grows to at least n elements (exactly n with `exact`) keeping the first
+	// `keep` elements (copied on `s`, which is synchronised).
+	void grow_keep(size_t n, size_t keep, hipStream_t s, bool exact = false) {
 		if (n <= cap) return;
 		if (t_arena) throw HipError("a body table cannot grow inside the no-allocation call", PDC_ERR_STATE);
-		size_t nc = cap ? cap : 256;
+		size_t nc = exact ? n : (cap ? cap : 256);
 		while (nc < n) nc = nc + nc / 2 + 1;
 		T* q = nullptr;
 		PDC_HIP(hipMalloc(&q, nc * sizeof(T)));

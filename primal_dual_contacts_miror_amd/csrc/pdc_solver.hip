@@ -1037,7 +1037,7 @@ __device__ __forceinline__ double sel6(int q, double v0, double v1, double v2, d
 // body's incident contacts; after the wavefront sums every lane holds the
 // block.  Every lane factors it (Cholesky), lane c < 6 solves for column c of
 // the inverse, and lane 6 i + j < 36 forms entry (i, j) of the stored blocks.
-template <class T>
+template <class T, bool PLAIN>
 __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict__ adiag, T* __restrict__ Aval,
                            T* __restrict__ Pval, T* __restrict__ bvec) {
 	PDC_TILE_RUN(a.tw);
@@ -1181,18 +1181,20 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 	for (int k = 0; k < 6; ++k) col[k] = __shfl(e[k], jj, 64);
 	if (lane < 36) {
 		double acc = 0;
+		// PLAIN: the block itself (column jj of the identity: exact); else S P
 #pragma unroll
-		for (int k = 0; k < 6; ++k) acc += PDC_ROW6(S, ii, k) * col[k];
+		for (int k = 0; k < 6; ++k) acc += PDC_ROW6(S, ii, k) * (PLAIN ? (k == jj ? 1.0 : 0.0) : col[k]);
 		Pval[36 * (size_t)sr + lane] = (T)PDC_PICK6(col, ii);
 		Aval[36 * (size_t)adiag[r] + lane] = (T)acc;
 	}
-	// PCG start x0 and S x0, which the stored operator A P maps to A x0
+	// PCG start x0, and the vector the stored operator maps to A x0: x0 itself
+	// (PLAIN) or S x0 (the operator A P)
 	if (lane < 6) {
 		double acc = 0;
 #pragma unroll
 		for (int k = 0; k < 6; ++k) acc += PDC_ROW6(S, lane, k) * x0[k];
 		a.xs[6 * sr + lane] = PDC_PICK6(x0, lane);
-		a.ys[6 * sr + lane] = acc;
+		a.ys[6 * sr + lane] = PLAIN ? PDC_PICK6(x0, lane) : acc;
 	}
 }
 
@@ -1207,7 +1209,7 @@ __global__ __launch_bounds__(256) void k_asm_diag(SArgs a, const int* __restrict
 // contacts with stride 64 and the 36 entries are reduced across the
 // wavefront.  The rows and columns for the products with P come by shuffles.
 constexpr int kOffDirect = 16;
-template <class T, bool LARGE>
+template <class T, bool LARGE, bool PLAIN>
 __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ Pval, T* __restrict__ Aval) {
 	PDC_TILE_RUN(a.tp);
 	(void)scn;
@@ -1275,12 +1277,17 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ 
 	// lane (ii, jj) < 36 holds A_ab[ii][jj]
 	const double v = sum * (da * db);
 	double bab = 0, bba = 0;
+	if (PLAIN) {
+		bab = v;
+		bba = __shfl(v, 6 * jj + ii, 64);  // A_ba[ii][jj] = A_ab[jj][ii]
+	} else {
 #pragma unroll
-	for (int k = 0; k < 6; ++k) {
-		const double a_ik = __shfl(v, 6 * ii + k, 64);  // A_ab[ii][k]
-		const double a_ki = __shfl(v, 6 * k + ii, 64);  // A_ab[k][ii] = A_ba[ii][k]
-		bab += a_ik * pb[k];
-		bba += a_ki * pa[k];
+		for (int k = 0; k < 6; ++k) {
+			const double a_ik = __shfl(v, 6 * ii + k, 64);  // A_ab[ii][k]
+			const double a_ki = __shfl(v, 6 * k + ii, 64);  // A_ab[k][ii] = A_ba[ii][k]
+			bab += a_ik * pb[k];
+			bba += a_ki * pa[k];
+		}
 	}
 	if (lane < 36) {
 		Aval[36 * (size_t)blk_ab + ii * 6 + jj] = (T)bab;
@@ -1344,12 +1351,14 @@ struct PcgArgs {
 	double* q;
 	double* s;
 	double* p;
-	double* rv[2];   // standard form: r and q = A p by iteration parity
-	double* qv[2];
+	double* zv[2];   // standard form: z = P r, w = P q and p by iteration parity
+	double* wv[2];
+	double* pv[2];
 	double* gam[2];  // partials, one per tile, by iteration parity
 	double* del[2];
 	double* rr[2];
-	double* e1[2];   // standard form: (q, M r) and (q, M q)
+	double* e1[2];   // standard form: (q, z), (r, w), (q, w)
+	double* e1b[2];
 	double* e2[2];
 	double* bb;
 	double* hist;    // per scene: gamma, alpha of iteration i at [4 s + 2 (i & 1)], [+ 1]
@@ -1464,20 +1473,49 @@ __device__ __forceinline__ bool pcg_tile_skip(const PcgArgs<T>& a, PcgTile* Tp) 
 	return false;
 }
 
-// Initial residual of both forms, r = b - A x0 = b - B y with y = M^-1 x0
-// (x0 itself is already in x).  A zero start has y = 0 and gives r = b exactly.
+// Initial residual of the single-reduction form, r = b - A x0 = b - B y with
+// y = M^-1 x0 (x0 itself is already in x), and the tile's partial sums of
+// |r|^2 (rr[1]) and |b|^2.  A zero start has y = 0 and gives r = b exactly.
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_start(PcgArgs<T> a, const double* __restrict__ y) {
+	__shared__ double lds[2 * 16];
 	PcgTile* Tp = a.tiles + blockIdx.x;
 	if (pcg_tile_skip(a, Tp)) return;
 	const int cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
+	double v[2] = {0, 0};
 	if (wid < cnt) {
 		const int br = pcg_slot();
 		const double ay = spmv_ell(a, y, br, i, sl);
-		if (i < 6 && sl == 0) a.r[6 * br + i] = (double)a.b[6 * br + i] - ay;
+		if (i < 6 && sl == 0) {
+			const double bt = (double)a.b[6 * br + i], rt = bt - ay;
+			a.r[6 * br + i] = rt;
+			v[0] = rt * rt;
+			v[1] = bt * bt;
+		}
 	}
+	double out[2];
+	block_sum<2, double>(v, lds, out);
+	if (threadIdx.x == 0) {
+		a.rr[1][blockIdx.x] = out[0];
+		a.bb[blockIdx.x] = out[1];
+	}
+}
+
+// The start a solve keeps: the warm start x0 (the previous Newton increment,
+// as the reference hands AMGCL X = du) when its residual is smaller than
+// that of a zero start, |b - A x0| < |b|; otherwise x0 = 0, r = b.  A warm
+// start far from the new solution (the increment of a gradient-descent
+// fallback, or of an earlier iteration when the Newton loop has converged
+// further) would put the relative tolerance below the rounding level of
+// |b - A x0| and the iteration could not reach it.  Every tile of a scene
+// sums the same partials in the same order and decides alike.
+template <class T>
+__device__ __forceinline__ bool pcg_zero_start(const PcgArgs<T>& a, const PcgTile* Tp, double* lds) {
+	double tot[2];
+	reduce_partials<2>(a.rr[1], a.bb, nullptr, nullptr, Tp->t0, Tp->W, tot, lds);
+	return !(tot[0] < tot[1]);
 }
 
 template <class T>
@@ -1488,16 +1526,25 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	const int scn = Tp->scene, cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
-	double v[4] = {0, 0, 0, 0};
+	const bool zero = pcg_zero_start(a, Tp, lds);
+	__syncthreads();
+	double v[3] = {0, 0, 0};
 	if (wid < cnt) {
 		const int br = pcg_slot();
-		const double w0 = spmv_ell(a, a.r, br, i, sl);
+		const double w0 = zero ? spmv_ell(a, a.b, br, i, sl) : spmv_ell(a, a.r, br, i, sl);
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
-			const double rt = a.r[t], bt = a.b[t];
+			double rrow[6];
+#pragma unroll
+			for (int c = 0; c < 6; ++c) rrow[c] = zero ? (double)a.b[6 * br + c] : a.r[6 * br + c];
+			const double rt = PDC_PICK6(rrow, i);
 			double ut = 0;
 #pragma unroll
-			for (int c = 0; c < 6; ++c) ut += (double)a.P[36 * (size_t)br + 6 * i + c] * a.r[6 * br + c];
+			for (int c = 0; c < 6; ++c) ut += (double)a.P[36 * (size_t)br + 6 * i + c] * rrow[c];
+			if (zero) {
+				a.r[t] = rt;
+				a.x[t] = 0;
+			}
 			a.w[0][t] = w0;
 			a.z[t] = 0;
 			a.s[t] = 0;
@@ -1505,16 +1552,14 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 			v[0] += rt * ut;
 			v[1] += w0 * ut;
 			v[2] += rt * rt;
-			v[3] += bt * bt;
 		}
 	}
-	double out[4];
-	block_sum<4, double>(v, lds, out);
+	double out[3];
+	block_sum<3, double>(v, lds, out);
 	if (threadIdx.x == 0) {
 		a.gam[0][blockIdx.x] = out[0];
 		a.del[0][blockIdx.x] = out[1];
 		a.rr[0][blockIdx.x] = out[2];
-		a.bb[blockIdx.x] = out[3];
 	}
 	pcg_tile_reset(a, Tp, scn);
 }
@@ -1681,93 +1726,144 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 }
 
 // Standard PCG in one launch per iteration, for tolerances below the
-// attainable accuracy of the single-reduction recurrences.  Its vectors follow
-// the textbook iteration (x += alpha p, r -= alpha q with q = A p from an
-// SpMV), so that it reaches the accuracy of a standard PCG; only the scalar
-// gamma that beta needs in the same launch is formed from an exact identity.
-// It runs on the stored operator B = A M through y with p = M y, A p = B y.
-//   F(0)  (k_pcgs_init, after k_pcg_start): y0 = r0, q0 = B r0,
-//         partials of delta0 = (M y0, q0), e1 = (q0, M r0), e2 = (q0, M q0),
-//         gamma0 = (r0, M r0), |r0|^2 and |b|^2
+// attainable accuracy of the single-reduction recurrences.  The assembly
+// stores A itself (not A P) for this form, and the iteration is the textbook
+// one, q = A p, x += alpha p, r -= alpha q, with p = z + beta p and z = P r;
+// z and w = P q are carried per row (z_i = z_{i-1} - alpha w_{i-1}), so that a
+// row forms its neighbours' p_i from their stored z, w, p with the same
+// expression, and the same bits, as they form it themselves: the q = A p of a
+// launch is the product with exactly the p that updates x.  (Folding P into
+// the stored operator, A P rounded once, separates the two by about
+// cond(diagonal block) x eps and stalls tight solves on stiff blocks.)
+//   start (k_pcgs_start): r0 = b - A x0, z0 = p0 = P r0, and P b
+//   F(0) (k_pcgs_init): the start kept (pcg_zero_start), q0 = A p0,
+//         w0 = P q0, partials of delta = (p, q),
+//         gamma = (r, z), e1 = (q, z), e1' = (r, w), e2 = (q, w), |r|^2, |b|^2
 //   F(i), i >= 1 (k_pcgs_iter), from the totals of F(i-1):
 //         stop when |r_{i-1}|^2 <= tol^2 |b|^2 (x_{i-1} is the solution);
-//         alpha = gamma_{i-1} / delta_{i-1}
-//         x_i = x_{i-1} + alpha M y_{i-1},  r_i = r_{i-1} - alpha q_{i-1}
-//         gamma_i = (r_i, M r_i) = gamma_{i-1} - 2 alpha e1 + alpha^2 e2,
-//           beta = gamma_i / gamma_{i-1}  (M symmetric)
-//         y_i = r_i + beta y_{i-1},  q_i = B r_i + beta q_{i-1}, where a row
-//           forms B r_i from its neighbours' r_{i-1} and q_{i-1}
-//         partials of delta_i, e1, e2, and gamma_i and |r_i|^2 summed
-//           directly, for the next launch
+//         alpha = gamma / delta, x_i = x_{i-1} + alpha p_{i-1},
+//         r_i = r_{i-1} - alpha q_{i-1}, z_i = z_{i-1} - alpha w_{i-1},
+//         gamma_i = (r_i, z_i) = gamma - alpha e1 - alpha e1' + alpha^2 e2,
+//         beta = gamma_i / gamma, p_i = z_i + beta p_{i-1}, q_i = A p_i,
+//         w_i = P q_i, and the partials of F(i)
 // alpha always takes the directly summed gamma, so the identity's rounding
-// enters only one beta and does not accumulate.  r and q (read by
-// neighbouring rows) and the partials are double-buffered by iteration
-// parity; y and x are read and written by their own row only.  The count of
-// iterations is the count of x updates, as in the textbook loop: a solve
-// that converges from its start reports 0.
+// enters only one beta.  z, w and p (read by neighbouring rows) and the
+// partials are double-buffered by iteration parity; r, q and x are read and
+// written by their own row only.  A launch whose sums are no longer positive
+// (the recurrence has reached its rounding floor) ends the solve at x_{i-1};
+// a non-finite sum ends it as a breakdown.  The iteration count is the count
+// of x updates, as in the textbook loop.
+template <class T>
+__global__ __launch_bounds__(kPcgThreads) void k_pcgs_start(PcgArgs<T> a, const double* __restrict__ x0) {
+	__shared__ double lds[2 * 16];
+	PcgTile* Tp = a.tiles + blockIdx.x;
+	if (pcg_tile_skip(a, Tp)) return;
+	const int cnt = Tp->count;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const int i = lane >> 3, sl = lane & 7;
+	const int ir = i < 6 ? i : 5;
+	double v[2] = {0, 0};
+	if (wid < cnt) {
+		const int br = pcg_slot();
+		const double ax = spmv_ell(a, x0, br, i, sl);
+		const double bi = (double)a.b[6 * br + ir];
+		const double ri = bi - ax;
+		double ra[6], bv[6];
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) {
+			ra[cc] = __shfl(ri, 8 * cc, 64);
+			bv[cc] = (double)a.b[6 * br + cc];
+		}
+		double zi = 0, zb = 0;
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) {
+			const double pc = (double)a.P[36 * (size_t)br + 6 * ir + cc];
+			zi += pc * ra[cc];
+			zb += pc * bv[cc];
+		}
+		if (i < 6 && sl == 0) {
+			const int t = 6 * br + i;
+			a.r[t] = ri;
+			a.zv[0][t] = zi;
+			a.pv[0][t] = zi;
+			a.zv[1][t] = zb;  // z of the zero start, P b
+			v[0] = ri * ri;
+			v[1] = bi * bi;
+		}
+	}
+	double out[2];
+	block_sum<2, double>(v, lds, out);
+	if (threadIdx.x == 0) {
+		a.rr[1][blockIdx.x] = out[0];
+		a.bb[blockIdx.x] = out[1];
+	}
+}
+
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcgs_init(PcgArgs<T> a) {
-	__shared__ double lds[6 * 16];
+	__shared__ double lds[7 * 16];
 	PcgTile* Tp = a.tiles + blockIdx.x;
 	if (pcg_tile_skip(a, Tp)) return;
 	const int scn = Tp->scene, cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	const int ir = i < 6 ? i : 5;
+	const bool zero = pcg_zero_start(a, Tp, lds);
+	__syncthreads();
 	double v[6] = {0, 0, 0, 0, 0, 0};
 	if (wid < cnt) {
 		const int br = pcg_slot();
-		const double* __restrict__ r0 = a.rv[0];
-		const double n = spmv_ell(a, r0, br, i, sl);
-		double qa[6], ra[6], pr[6];
+		// q0 = A p0 with p0 = z0 of the start kept (P b for a zero start)
+		const double n = spmv_ell(a, zero ? a.zv[1] : a.pv[0], br, i, sl);
+		double qa[6];
 #pragma unroll
-		for (int cc = 0; cc < 6; ++cc) {
-			qa[cc] = __shfl(n, 8 * cc, 64);  // component cc of q0, from its row group
-			ra[cc] = r0[6 * br + cc];
-			pr[cc] = (double)a.P[36 * (size_t)br + 6 * ir + cc];
-		}
+		for (int cc = 0; cc < 6; ++cc) qa[cc] = __shfl(n, 8 * cc, 64);
+		double wi = 0;
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) wi += (double)a.P[36 * (size_t)br + 6 * ir + cc] * qa[cc];
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
-			double mr = 0, mq = 0;
-#pragma unroll
-			for (int cc = 0; cc < 6; ++cc) {
-				mr += pr[cc] * ra[cc];
-				mq += pr[cc] * qa[cc];
+			const double rt = zero ? (double)a.b[t] : a.r[t];
+			const double zt = zero ? a.zv[1][t] : a.zv[0][t];
+			if (zero) {
+				a.r[t] = rt;
+				a.x[t] = 0;
+				a.zv[0][t] = zt;
+				a.pv[0][t] = zt;
 			}
-			const double qt = PDC_PICK6(qa, i), rt = PDC_PICK6(ra, i), bt = a.b[t];
-			a.qv[0][t] = qt;
-			a.p[t] = rt;  // y0 = r0
-			v[0] = mr * qt;
-			v[1] = qt * mr;
-			v[2] = qt * mq;
-			v[3] = rt * mr;
-			v[4] = rt * rt;
-			v[5] = bt * bt;
+			a.q[t] = n;
+			a.wv[0][t] = wi;
+			v[0] = zt * n;
+			v[1] = rt * zt;
+			v[2] = n * zt;
+			v[3] = rt * wi;
+			v[4] = n * wi;
+			v[5] = rt * rt;
 		}
 	}
 	double out[6];
 	block_sum<6, double>(v, lds, out);
 	if (threadIdx.x == 0) {
 		a.del[0][blockIdx.x] = out[0];
-		a.e1[0][blockIdx.x] = out[1];
-		a.e2[0][blockIdx.x] = out[2];
-		a.gam[0][blockIdx.x] = out[3];
-		a.rr[0][blockIdx.x] = out[4];
-		a.bb[blockIdx.x] = out[5];
+		a.gam[0][blockIdx.x] = out[1];
+		a.e1[0][blockIdx.x] = out[2];
+		a.e1b[0][blockIdx.x] = out[3];
+		a.e2[0][blockIdx.x] = out[4];
+		a.rr[0][blockIdx.x] = out[5];
 	}
 	pcg_tile_reset(a, Tp, scn);
 }
 
 // One iteration F(i), i >= 1, launch slot = parity of i - 1.  Memory round
 // trips as in k_pcg_iter: (1) the record, the ELL columns, the
-// preconditioner row and the row's r, q, y, x; (2) the partials of F(i-1)
-// (six arrays, one per wavefront), the operator blocks and the neighbours' r
-// and q of the occupied entries.  B r_i = B r_{i-1} - alpha B q_{i-1} is
-// summed as two sums before alpha is known.  EM as in k_pcg_iter.
+// preconditioner row and the row's own r, q, x, z, w, p; (2) the partials of
+// F(i-1) (seven arrays, one per wavefront), the operator blocks and the
+// neighbours' z, w, p of the occupied entries, combined into p_i after
+// alpha and beta are known.  EM as in k_pcg_iter.
 template <class T, int EM>
 __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slot) {
 	__shared__ double lds[8];
-	static_assert(kPcgRowsPerWG >= 6, "six partial arrays, one per wavefront");
+	static_assert(kPcgRowsPerWG >= 7, "seven partial arrays, one per wavefront");
 	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
 	const int* __restrict__ rp4 = reinterpret_cast<const int*>(Tp);
 	int rec[8];
@@ -1779,142 +1875,131 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slo
 	const int ir = i < 6 ? i : 5;
 	const int br = kPcgRowsPerWG * (int)blockIdx.x + wid;
 	const int cur = slot, nxt = slot ^ 1;
-	const double* __restrict__ rc = pick(a.rv, cur);
-	const double* __restrict__ qc = pick(a.qv, cur);
+	const double* __restrict__ zc = pick(a.zv, cur);
+	const double* __restrict__ wc = pick(a.wv, cur);
+	const double* __restrict__ pc = pick(a.pv, cur);
 	// round 1
 	constexpr int kE = EM > 0 ? EM : 1;
 	const int R = EM > 0 ? 8 * EM : a.R;
 	int ec[kE];
 #pragma unroll
 	for (int m = 0; m < kE; ++m) ec[m] = a.ecol[(size_t)br * R + sl + 8 * m];
-	double pr[6], rp[6], qp[6], yp[6];
+	double pr[6];
 #pragma unroll
-	for (int cc = 0; cc < 6; ++cc) {
-		pr[cc] = (double)a.P[36 * (size_t)br + 6 * ir + cc];
-		rp[cc] = rc[6 * br + cc];
-		qp[cc] = qc[6 * br + cc];
-		yp[cc] = a.p[6 * br + cc];
-	}
+	for (int cc = 0; cc < 6; ++cc) pr[cc] = (double)a.P[36 * (size_t)br + 6 * ir + cc];
 	const int t = 6 * br + ir;
-	const double xo = a.x[t];
+	const double xo = a.x[t], ro = a.r[t], qo = a.q[t], zo = zc[t], wo = wc[t], po = pc[t];
 	const int scn = rec[0], cnt = rec[2], t0 = rec[3], W = rec[4], done = rec[5], it = rec[6];
 	if (scn < 0 || done || it >= a.maxit) return;
-	// round 2: partial sums of F(i-1), operator blocks and neighbour values
-	const double* psrc = wid == 0 ? pick(a.del, cur) : wid == 1 ? pick(a.e1, cur) : wid == 2 ? pick(a.e2, cur)
-	                   : wid == 3 ? pick(a.gam, cur) : wid == 4 ? pick(a.rr, cur) : a.bb;
+	// round 2: partial sums of F(i-1), operator blocks and neighbour z, w, p
+	const double* psrc = wid == 0 ? pick(a.del, cur) : wid == 1 ? pick(a.gam, cur) : wid == 2 ? pick(a.e1, cur)
+	                   : wid == 3 ? pick(a.e1b, cur) : wid == 4 ? pick(a.e2, cur) : wid == 5 ? pick(a.rr, cur)
+	                   : a.bb;
 	psrc += t0;
-	double pv[kPartLoads];
-	if (wid < 6) {
+	double pvl[kPartLoads];
+	if (wid < 7) {
 #pragma unroll
 		for (int k = 0; k < kPartLoads; ++k) {
-			pv[k] = 0;
-			if (64 * k < W) pv[k] = psrc[min(lane + 64 * k, W - 1)];
+			pvl[k] = 0;
+			if (64 * k < W) pvl[k] = psrc[min(lane + 64 * k, W - 1)];
 		}
 	}
-	double nr = 0, nq = 0;  // row i of B r_{i-1} and of B q_{i-1}
-	if (i < 6) {
+	double blk[kE][6], gz[kE][6], gw[kE][6], gp[kE][6];
 #pragma unroll
-		for (int m = 0; m < kE; ++m) {
-			const int c = ec[m];
-			if (c >= 0) {
-				const T* blk = a.B + 36 * ((size_t)br * R + sl + 8 * m) + 6 * i;
-				const double* vr = rc + 6 * (size_t)c;
-				const double* vq = qc + 6 * (size_t)c;
-				double dr = 0, dq = 0;
+	for (int m = 0; m < kE; ++m) {
+		const int c = i < 6 ? ec[m] : -1;
+		if (c >= 0) {
+			const T* bp = a.B + 36 * ((size_t)br * R + sl + 8 * m) + 6 * i;
 #pragma unroll
-				for (int cc = 0; cc < 6; ++cc) {
-					const double bv = (double)blk[cc];
-					dr += bv * vr[cc];
-					dq += bv * vq[cc];
-				}
-				nr += dr;
-				nq += dq;
+			for (int cc = 0; cc < 6; ++cc) {
+				blk[m][cc] = (double)bp[cc];
+				gz[m][cc] = zc[6 * (size_t)c + cc];
+				gw[m][cc] = wc[6 * (size_t)c + cc];
+				gp[m][cc] = pc[6 * (size_t)c + cc];
 			}
+		} else {
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) blk[m][cc] = gz[m][cc] = gw[m][cc] = gp[m][cc] = 0.0;
 		}
-		if (EM == 0)
-			for (int e = sl + 8; e < R; e += 8) {
-				const size_t k = (size_t)br * R + e;
-				const int c = a.ecol[k];
-				if (c < 0) continue;
-				const T* blk = a.B + 36 * k + 6 * i;
-				const double* vr = rc + 6 * (size_t)c;
-				const double* vq = qc + 6 * (size_t)c;
-				double dr = 0, dq = 0;
-#pragma unroll
-				for (int cc = 0; cc < 6; ++cc) {
-					const double bv = (double)blk[cc];
-					dr += bv * vr[cc];
-					dq += bv * vq[cc];
-				}
-				nr += dr;
-				nq += dq;
-			}
 	}
-	nr = group8_sum(nr);
-	nq = group8_sum(nq);
-	// M y_{i-1} of the row (the x update)
-	double myp = 0;
-#pragma unroll
-	for (int cc = 0; cc < 6; ++cc) myp += pr[cc] * yp[cc];
-	if (wid < 6) {
+	if (wid < 7) {
 		double part = 0;
 #pragma unroll
-		for (int k = 0; k < kPartLoads; ++k) part += lane + 64 * k < W ? pv[k] : 0.0;
+		for (int k = 0; k < kPartLoads; ++k) part += lane + 64 * k < W ? pvl[k] : 0.0;
 		if (W > 64 * kPartLoads) part += partial_lane_sum(psrc + 64 * kPartLoads, W - 64 * kPartLoads);
 		part = wave_sum_dpp(part);
 		if (lane == 0) lds[wid] = part;
 	}
 	__syncthreads();
-	const double delta = lds[0], e1 = lds[1], e2 = lds[2], gamma = lds[3], rr = lds[4], bb = lds[5];
+	const double delta = lds[0], gamma = lds[1], e1 = lds[2], e1b = lds[3], e2 = lds[4], rr = lds[5], bb = lds[6];
 	const bool first_tile = (int)blockIdx.x == t0;
 	const bool bad = !(isfinite(rr) && isfinite(bb) && isfinite(gamma) && isfinite(delta) && isfinite(e1) &&
-	                   isfinite(e2));
-	if (bad || !(rr > a.tol2 * bb)) {
+	                   isfinite(e1b) && isfinite(e2));
+	// converged, at the rounding floor (sums no longer positive), or broken down
+	if (bad || !(rr > a.tol2 * bb) || !(delta > 0.0) || !(gamma > 0.0)) {
 		if (threadIdx.x == 0) pcg_finish(a, Tp, scn, it, bad, first_tile);
 		return;
 	}
 	const double alpha = gamma / delta;
-	const double beta = (gamma - 2.0 * alpha * e1 + alpha * alpha * e2) / gamma;
-	const double n = nr - alpha * nq;  // row i of B r_i
-	double rn[6], yn[6];
+	const double beta = (gamma - alpha * e1 - alpha * e1b + alpha * alpha * e2) / gamma;
+	// q_i = A p_i, each neighbour's p_i formed as it forms its own
+	double n = 0;
 #pragma unroll
-	for (int cc = 0; cc < 6; ++cc) {
-		rn[cc] = rp[cc] - alpha * qp[cc];
-		yn[cc] = rn[cc] + beta * yp[cc];
+	for (int m = 0; m < kE; ++m) {
+		double d = 0;
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) d += blk[m][cc] * ((gz[m][cc] - alpha * gw[m][cc]) + beta * gp[m][cc]);
+		n += d;
 	}
-	const double qn = n + beta * PDC_PICK6(qp, ir);
+	if (EM == 0 && i < 6)
+		for (int e = sl + 8; e < R; e += 8) {
+			const size_t k = (size_t)br * R + e;
+			const int c = a.ecol[k];
+			if (c < 0) continue;
+			const T* bp = a.B + 36 * k + 6 * i;
+			double d = 0;
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) {
+				const size_t v = 6 * (size_t)c + cc;
+				d += (double)bp[cc] * ((zc[v] - alpha * wc[v]) + beta * pc[v]);
+			}
+			n += d;
+		}
+	n = group8_sum(n);
 	double qa[6];
 #pragma unroll
-	for (int cc = 0; cc < 6; ++cc) qa[cc] = __shfl(qn, 8 * cc, 64);
-	double v[5] = {0, 0, 0, 0, 0};
+	for (int cc = 0; cc < 6; ++cc) qa[cc] = __shfl(n, 8 * cc, 64);
+	double v[6] = {0, 0, 0, 0, 0, 0};
 	if (wid < cnt && i < 6 && sl == 0) {
-		double mr = 0, my = 0, mq = 0;
+		double wn = 0;
 #pragma unroll
-		for (int cc = 0; cc < 6; ++cc) {
-			mr += pr[cc] * rn[cc];
-			my += pr[cc] * yn[cc];
-			mq += pr[cc] * qa[cc];
-		}
-		const double rt = PDC_PICK6(rn, i), yt = PDC_PICK6(yn, i);
-		a.x[t] = xo + alpha * myp;
-		pick(a.rv, nxt)[t] = rt;
-		pick(a.qv, nxt)[t] = qn;
-		a.p[t] = yt;
-		v[0] = my * qn;
-		v[1] = qn * mr;
-		v[2] = qn * mq;
-		v[3] = rt * mr;
-		v[4] = rt * rt;
+		for (int cc = 0; cc < 6; ++cc) wn += pr[cc] * qa[cc];
+		const double rn = ro - alpha * qo;
+		const double zn = zo - alpha * wo;
+		const double pn = zn + beta * po;
+		a.x[t] = xo + alpha * po;
+		a.r[t] = rn;
+		a.q[t] = n;
+		pick(a.zv, nxt)[t] = zn;
+		pick(a.wv, nxt)[t] = wn;
+		pick(a.pv, nxt)[t] = pn;
+		v[0] = pn * n;
+		v[1] = rn * zn;
+		v[2] = n * zn;
+		v[3] = rn * wn;
+		v[4] = n * wn;
+		v[5] = rn * rn;
 	}
-	__shared__ double lds_b[5 * kPcgRowsPerWG];
-	double out[5];
-	block_sum_last<5, kPcgRowsPerWG>(v, lds_b, out);
+	__shared__ double lds_b[6 * kPcgRowsPerWG];
+	double out[6];
+	block_sum_last<6, kPcgRowsPerWG>(v, lds_b, out);
 	if (threadIdx.x == 0) {
 		pick(a.del, nxt)[blockIdx.x] = out[0];
-		pick(a.e1, nxt)[blockIdx.x] = out[1];
-		pick(a.e2, nxt)[blockIdx.x] = out[2];
-		pick(a.gam, nxt)[blockIdx.x] = out[3];
-		pick(a.rr, nxt)[blockIdx.x] = out[4];
+		pick(a.gam, nxt)[blockIdx.x] = out[1];
+		pick(a.e1, nxt)[blockIdx.x] = out[2];
+		pick(a.e1b, nxt)[blockIdx.x] = out[3];
+		pick(a.e2, nxt)[blockIdx.x] = out[4];
+		pick(a.rr, nxt)[blockIdx.x] = out[5];
 		Tp->it = it + 1;
 		if (first_tile) {
 			a.prog[scn].rel2 = rr / bb;
@@ -2404,11 +2489,11 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 	const dim3 B(kPcgThreads);
 	for (int k = 0; k < chunk; ++k) {
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k], s));
+		// the standard form keeps four 6-vectors per entry in flight: above two
+		// entries per lane the generic width reads the rest after the barrier
 		if (two) switch (pa.R / 8) {
 			case 1: hipLaunchKernelGGL((k_pcgs_iter<T, 1>), dim3(nblk), B, 0, s, pa, k & 1); break;
 			case 2: hipLaunchKernelGGL((k_pcgs_iter<T, 2>), dim3(nblk), B, 0, s, pa, k & 1); break;
-			case 3: hipLaunchKernelGGL((k_pcgs_iter<T, 3>), dim3(nblk), B, 0, s, pa, k & 1); break;
-			case 4: hipLaunchKernelGGL((k_pcgs_iter<T, 4>), dim3(nblk), B, 0, s, pa, k & 1); break;
 			default: hipLaunchKernelGGL((k_pcgs_iter<T, 0>), dim3(nblk), B, 0, s, pa, k & 1); break;
 		}
 		else switch (pa.R / 8) {
@@ -2435,7 +2520,7 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 	const int nblk = std::max(1, ctx->cap_tr);
 	const size_t ncap = 6 * (size_t)kPcgRowsPerWG * nblk, pcap = (size_t)nblk;
 	DBuf<double>& work = ctx->pcg64;
-	work.ensure(7 * ncap + 11 * pcap + 4 * (size_t)S + 64);
+	work.ensure(8 * ncap + 13 * pcap + 4 * (size_t)S + 64);
 	PcgArgs<T> pa;
 	std::memset(&pa, 0, sizeof(pa));  // compared bytewise with the recorded arguments
 	pa.ecol = ctx->e_col.p;
@@ -2447,17 +2532,20 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 	double* w = work.p;
 	pa.r = w; pa.w[0] = w + ncap; pa.w[1] = w + 2 * ncap;
 	pa.z = w + 3 * ncap; pa.q = w + 4 * ncap; pa.s = w + 5 * ncap; pa.p = w + 6 * ncap;
-	double* parts = w + 7 * ncap;
+	double* parts = w + 8 * ncap;
 	pa.gam[0] = parts; pa.gam[1] = parts + pcap;
 	pa.del[0] = parts + 2 * pcap; pa.del[1] = parts + 3 * pcap;
 	pa.rr[0] = parts + 4 * pcap; pa.rr[1] = parts + 5 * pcap;
 	pa.bb = parts + 6 * pcap;
 	pa.e1[0] = parts + 7 * pcap; pa.e1[1] = parts + 8 * pcap;
 	pa.e2[0] = parts + 9 * pcap; pa.e2[1] = parts + 10 * pcap;
-	pa.hist = parts + 11 * pcap;
-	// the standard form keeps r, q by parity in the recurrence form's r, z and q, s
-	pa.rv[0] = pa.r; pa.rv[1] = pa.z;
-	pa.qv[0] = pa.q; pa.qv[1] = pa.s;
+	pa.e1b[0] = parts + 11 * pcap; pa.e1b[1] = parts + 12 * pcap;
+	pa.hist = parts + 13 * pcap;
+	// the standard form keeps r and q (own rows) in the recurrence form's r
+	// and q, and z, w, p by parity in its z, s / w[0], w[1] / p and an eighth vector
+	pa.zv[0] = pa.z; pa.zv[1] = pa.s;
+	pa.wv[0] = pa.w[0]; pa.wv[1] = pa.w[1];
+	pa.pv[0] = pa.p; pa.pv[1] = w + 7 * ncap;
 	pa.tiles = ctx->t_pcg.p;
 	pa.ss = ctx->d_ss.p;
 	pa.ps = ctx->d_ps.p;
@@ -2474,9 +2562,13 @@ template <class T>
 static void pcg_begin(pdc_context* ctx, const PcgArgs<T>& pa) {
 	const int nblk = std::max(1, ctx->cap_tr);
 	hipStream_t s = ctx->stream;
-	hipLaunchKernelGGL(k_pcg_start<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa, (const double*)ctx->y64.p);
-	if (ctx->p.pcg_tol < kSingleReductionMinTol) hipLaunchKernelGGL(k_pcgs_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
-	else hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
+	if (ctx->p.pcg_tol < kSingleReductionMinTol) {
+		hipLaunchKernelGGL(k_pcgs_start<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa, (const double*)ctx->y64.p);
+		hipLaunchKernelGGL(k_pcgs_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
+	} else {
+		hipLaunchKernelGGL(k_pcg_start<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa, (const double*)ctx->y64.p);
+		hipLaunchKernelGGL(k_pcg_init<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa);
+	}
 	PDC_CHECK_LAUNCH();
 }
 
@@ -2487,8 +2579,8 @@ static void pcg_begin(pdc_context* ctx, const PcgArgs<T>& pa) {
 //   k_pcg_iter   B blocks and column indices nnzb (36 T + 4), row pointers
 //                4 (NL + 1), P blocks 36 T NL, w, z, s, p, x, r read
 //                6 x 8 n6, z, s, p, x, r, w' written 6 x 8 n6
-//   k_pcgs_iter  B, column indices, row pointers, P as above, r, q, y, x
-//                read 4 x 8 n6, r', q', y, x written 4 x 8 n6
+//   k_pcgs_iter  A blocks, column indices, row pointers, P as above, x, r,
+//                q, z, w, p read 6 x 8 n6, written 6 x 8 n6
 template <class T>
 static void pcg_account_bytes(pdc_context* ctx, const std::vector<int>& iters, bool two) {
 	const double Tb = sizeof(T);
@@ -2496,7 +2588,7 @@ static void pcg_account_bytes(pdc_context* ctx, const std::vector<int>& iters, b
 		if (iters[sc] <= 0) continue;
 		const double NLs = ctx->h_lbeg[sc + 1] - ctx->h_lbeg[sc], n6 = 6.0 * NLs, nnzb = ctx->h_nnzb[sc];
 		const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NLs + 1) + 36.0 * Tb * NLs;
-		ctx->kt.bytes += iters[sc] * (mat + (two ? 64.0 : 96.0) * n6);
+		ctx->kt.bytes += iters[sc] * (mat + 96.0 * n6);
 	}
 }
 
@@ -2829,6 +2921,7 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	const size_t nslot = (size_t)kPcgRowsPerWG * std::max(1, ctx->cap_tr), nell = nslot * ctx->ell_R;
 	ctx->x64.ensure(6 * nslot); ctx->y64.ensure(6 * nslot);
 	const bool fp32 = ctx->p.pcg_precision == PDC_PCG_FP32;
+	const bool plain = ctx->p.pcg_tol < kSingleReductionMinTol;  // the standard form's operator
 	if (fp32) {
 		ctx->A32.ensure(36 * nell); ctx->P32.ensure(36 * nslot); ctx->b32.ensure(6 * nslot);
 	} else {
@@ -2901,14 +2994,23 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		// assembly
 		kt_begin(ctx, PDC_KT_ASSEMBLY, &e0);
 		launch(k_asm_prep, ntc, kBlock, a);
-		if (fp32) {
-			launch(k_asm_diag<float>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A32.p, ctx->P32.p, ctx->b32.p);
-			launch(k_asm_off<float, false>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
-			launch(k_asm_off<float, true>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
+		// the standard form (tight tolerances) stores A itself, the recurrences A P
+		if (fp32 && plain) {
+			launch(k_asm_diag<float, true>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A32.p, ctx->P32.p, ctx->b32.p);
+			launch(k_asm_off<float, false, true>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
+			launch(k_asm_off<float, true, true>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
+		} else if (fp32) {
+			launch(k_asm_diag<float, false>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A32.p, ctx->P32.p, ctx->b32.p);
+			launch(k_asm_off<float, false, false>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
+			launch(k_asm_off<float, true, false>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
+		} else if (plain) {
+			launch(k_asm_diag<double, true>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A64.p, ctx->P64.p, ctx->b64.p);
+			launch(k_asm_off<double, false, true>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
+			launch(k_asm_off<double, true, true>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
 		} else {
-			launch(k_asm_diag<double>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A64.p, ctx->P64.p, ctx->b64.p);
-			launch(k_asm_off<double, false>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
-			launch(k_asm_off<double, true>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
+			launch(k_asm_diag<double, false>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A64.p, ctx->P64.p, ctx->b64.p);
+			launch(k_asm_off<double, false, false>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
+			launch(k_asm_off<double, true, false>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
 		}
 		PDC_CHECK_LAUNCH();
 		kt_end(ctx, PDC_KT_ASSEMBLY, e0);

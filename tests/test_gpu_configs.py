@@ -1,0 +1,129 @@
+"""GPU: the batched and served workloads at their sizes.
+
+  batched  the benchmark's 256 seeded random piles (10-500 bodies,
+           log-uniform), stepped together 200 steps from the seeded drop at
+           the tight inner solve (FP64 operator, 1e-12): no non-finite value,
+           contacts in the contact-rich tail; 8 sampled scenes bit-identical
+           to themselves stepped alone; 3 of them one step against the FP64
+           oracle (its branch for the scene, direct below 2000 bodies as in
+           the reference): positions within 1e-6 of the body size and
+           rotation angles within 1e-6 rad; linear and angular velocities
+           within 1e-6 |g| dt when the oracle's Newton loop converges, and
+           otherwise (the loop stalls at its iteration cap, where the result
+           depends on the rounding of every inner solve) within 4x the
+           difference between the oracle's own direct and PCG branches.
+  served   a mix of 48 random piles of 10-1000 bodies plus a 1000-body pile
+           behind the server (two shards), stepped into contact through it,
+           then a Poisson stream during which 4 scenes leave and 4 others
+           arrive: sampled scenes bit-identical alone, 2 one step against the
+           oracle with the bars above.
+"""
+import numpy as np
+import pytest
+
+from oracle import orc
+from primal_dual_contacts_miror_amd import pdc, scenes, serve
+from primal_dual_contacts_miror_amd.params import PCG_FP64
+from test_gpu_parity import pile_step_errors
+
+pytestmark = pytest.mark.gpu
+TIGHT = dict(pcg_precision=PCG_FP64, pcg_tol=1e-12)
+
+
+def _alone(sc, steps):
+    ds = pdc.DynamicSystem(sc, sc.params)
+    for _ in range(steps):
+        ds.step(1)
+    st = ds.state()
+    ds.ctx.close()
+    return st
+
+
+def _same(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def _oracle_step(sc, state, solver=orc.SOLVER_REFERENCE):
+    o = orc.OracleScene(sc, sc.params)
+    o.set_threads(16)
+    o.set_state(*state)
+    o.set_linear_solver(solver)
+    st = o.step(1)
+    return o.get_state(), st
+
+
+def check_one_step(sc, start, device_state, what):
+    """The device's step from `start` against the oracle's, with the bars of
+    the module docstring."""
+    ref, so = _oracle_step(sc, start)
+    e = pile_step_errors(sc, device_state, ref)
+    msg = "%s (%d bodies, %d contacts, oracle Newton %d, exit %.2e): errors (pos, angle, vel, angvel) %s" % (
+        what, sc.n, so.n_contacts, so.newton_iterations, so.exit_error, e)
+    assert np.all(e[:2] < 1e-6), msg
+    if so.exit_error < sc.params.tolerance:
+        print(msg)
+        assert np.all(e[2:] < 1e-6), msg
+    else:
+        alt, _ = _oracle_step(sc, start, orc.SOLVER_PCG)
+        own = pile_step_errors(sc, alt, ref)
+        print(msg + "; stalled, oracle direct vs PCG %s" % own)
+        assert np.all(e[2:] <= np.maximum(1e-6, 4 * own[2:])), msg
+
+
+def test_batched_256_scenes(require_gpu):
+    scs = [scenes.random_pile(1000 + k, **TIGHT) for k in range(256)]
+    bs = pdc.BatchedSystem(scs)
+    for _ in range(200):
+        st = bs.step(1)
+    contacts = np.array([x.n_contacts for x in st])
+    print("256 scenes, %d bodies: %d contacts in the last step, max %d per scene, %d scenes in contact"
+          % (sum(s.n for s in scs), contacts.sum(), contacts.max(), np.sum(contacts > 0)))
+    assert sum(x.nonfinite for x in st) == 0
+    assert all(np.all(np.isfinite(x)) for x in bs.state())
+    assert contacts.sum() > 50000 and np.sum(contacts > 0) > 200
+    order = np.argsort([s.n for s in scs])
+    picks = [int(order[i]) for i in np.linspace(0, len(order) - 1, 8).round().astype(int)]
+    for k in picks:
+        assert _same(bs.state(k), _alone(scs[k], 200)), "scene %d (%d bodies) differs alone" % (k, scs[k].n)
+    for k in picks[-1], picks[4], picks[1]:
+        start = tuple(x.copy() for x in bs.state(k))
+        bs.step(1, [k])
+        check_one_step(scs[k], start, bs.state(k), "batched scene %d" % k)
+    bs.ctx.close()
+
+
+def test_served_mix_with_churn(require_gpu):
+    scs = [scenes.random_pile(5000 + k, n_min=10, n_max=1000, **TIGHT) for k in range(48)]
+    scs.append(scenes.random_pile(6000, n_min=1000, n_max=1000, **TIGHT))
+    newcomers = [scenes.random_pile(7000 + k, n_min=10, n_max=1000, **TIGHT) for k in range(4)]
+    srv = serve.SceneServer(scs[0].params, shards=2)
+    hs = srv.admit_all(scs)
+    assert all(h.state == "resident" for h in hs)
+    print("served mix: %d scenes, %d bodies (largest %d)" % (len(hs), sum(s.n for s in scs), max(s.n for s in scs)))
+    for _ in range(60):
+        srv.drain([srv.submit(h) for h in hs])
+    rng = np.random.default_rng(11)
+    reqs = [serve.Request(hs[int(rng.integers(len(hs)))], 0.004 * k) for k in range(400)]
+    leaving = [hs[3], hs[10], hs[20], hs[30]]
+    first = srv.run(reqs[:200])
+    for h in leaving:
+        srv.release(h)
+    new = [srv.admit(sc) for sc in newcomers]
+    reqs2 = [r for r in reqs[200:] if r.scene not in leaving]
+    reqs2 += [serve.Request(h, 0.004 * k) for k, h in enumerate(new * 10)]
+    second = srv.run(reqs2)
+    assert len(first) == 200 and len(second) == len(reqs2)
+    assert all(h.state == "left" for h in leaving) and all(h.state == "resident" for h in new)
+    rounds = {}
+    for r in first + second:
+        rounds.setdefault((r.scene.shard.index, r.done), []).append(r)
+    assert max(len(v) for v in rounds.values()) > 1
+    big = hs[-1]
+    checks = [big, hs[int(np.argmax([h.steps for h in hs[:-1]]))], new[0], new[3]]
+    for h in checks:
+        assert _same(srv.state(h), _alone(h.scene, h.steps)), "%s after %d steps" % (h.name, h.steps)
+    for h in big, checks[1]:
+        start = tuple(x.copy() for x in srv.state(h))
+        srv.drain([srv.submit(h)])
+        check_one_step(h.scene, start, srv.state(h), "served %s" % h.name)
+    srv.close()
