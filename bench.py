@@ -107,7 +107,7 @@ def cpu_baseline(sc, state, budget_s, threads, checks):
     n = 0
     contacts = 0
     step_ms = []
-    first = None
+    first = first_stats = None
     while True:
         ts = time.perf_counter()
         st = o.step(1)
@@ -117,6 +117,7 @@ def cpu_baseline(sc, state, budget_s, threads, checks):
         el = time.perf_counter() - t0
         if n == 1:
             first = tuple(x.copy() for x in o.get_state())
+            first_stats = st
         log("cpu baseline step %d: %d contacts, %.2fs" % (n, st.n_contacts, el))
         if el > budget_s or n >= 50:
             break
@@ -132,21 +133,32 @@ def cpu_baseline(sc, state, budget_s, threads, checks):
         "p99_step_ms": float(np.percentile(step_ms, 99)),
         "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()},
     }
-    def restatement_step(scn, q, start):
-        if start is state and q is sc.params:
-            return first
+    def restatement_step(scn, q, start, solver=orc.SOLVER_REFERENCE):
+        if start is state and q is sc.params and solver == orc.SOLVER_REFERENCE:
+            return first, first_stats
         r = orc.OracleScene(scn, q)
         r.set_threads(threads)
         r.set_state(*start)
-        r.step(1)
-        return r.get_state()
+        r.set_linear_solver(solver)
+        st = r.step(1)
+        return r.get_state(), st
 
     precision = {}
     for name, scn, params, start, dev, ref_params in checks:
-        ref = restatement_step(scn, ref_params if ref_params is not None else params, start)
+        q = ref_params if ref_params is not None else params
+        ref, st = restatement_step(scn, q, start)
         if dev is None:  # the restatement's own step at `params` against the reference
-            dev = restatement_step(scn, params, start)
-        precision.setdefault(name, []).append(state_errors(scn, dev, ref))
+            dev, _ = restatement_step(scn, params, start)
+        rec = state_errors(scn, dev, ref)
+        rec["bodies"] = int(scn.n)
+        rec["oracle_newton_iterations"] = int(st.newton_iterations)
+        rec["oracle_converged"] = bool(st.exit_error < q.tolerance)
+        if not rec["oracle_converged"] and st.n_local_bodies < 2000:
+            # a stalled Newton loop (iteration cap): its end depends on the
+            # rounding of every inner solve; the oracle's own two branches
+            alt, _ = restatement_step(scn, q, start, orc.SOLVER_PCG)
+            rec["oracle_direct_vs_pcg"] = state_errors(scn, alt, ref)
+        precision.setdefault(name, []).append(rec)
     return precision, baseline, first
 
 
@@ -368,9 +380,13 @@ def state_errors(sc, a, b):
                 (np.linalg.norm(a[3][dyn] - b[3][dyn], axis=1) * size / gdt).max())}
 
 
+QUANTITIES = ("max_position_error_rel_body_size", "max_angle_error_rad", "max_velocity_error_rel_g_dt",
+              "max_angular_velocity_error_rel_g_dt")
+
+
 def worst(errs):
-    """Per-quantity maximum over a list of state_errors records."""
-    return {k: max(e[k] for e in errs) for k in errs[0]}
+    """Per-quantity maximum over a list of precision records."""
+    return {k: max(e[k] for e in errs) for k in QUANTITIES}
 
 
 def measured_traffic(kernel, workload):
@@ -551,7 +567,7 @@ def main():
                     worst(precision["other_setting_restatement_vs_fp64"])
         for leg in ("batched", "served"):
             if leg in precision:
-                out["precision"][leg] = worst(precision[leg])
+                out["precision"][leg] = {"worst": worst(precision[leg]), "scenes": precision[leg]}
     print(json.dumps(out), flush=True)
 
 
