@@ -2,8 +2,10 @@
 pile at ~30k contacts per step.
 
 A step is one full timestep on the device-resident scene: grid-SDF contact
-generation, the primal-dual Newton solve (FP64 outer loop, block-Jacobi PCG
-inner solve, FP64 merit line search) and integration.  The timed region
+generation, the primal-dual Newton solve (FP64 outer loop, PCG inner solve
+preconditioned by the inverse blocks of groups of strongly coupled bodies at
+the tight setting and by the diagonal blocks at the fast one, FP64 merit line
+search) and integration.  The timed region
 starts from the settled state of the seeded pile (the state after the
 workload's 1000 steps), with every input already in HBM.
 
@@ -41,19 +43,25 @@ KT_NAMES = {"pcg": pdc.KT_PCG, "pcg_per_launch": pdc.KT_SPMV, "contact_terms": p
             "contacts": pdc.KT_CONTACTS}
 # inner-solve settings: operator precision and relative tolerance
 SETTINGS = {"tight": (PCG_FP64, 1e-12), "fast": (PCG_FP32, 1e-5)}
+STANDARD_FORM_TOL = 1e-9  # below it the library runs the standard PCG form with the body-group preconditioner
 
 
 def pcg_kernel_name(p):
-    return "k_pcgs_iter" if p.pcg_tol < 1e-9 else "k_pcg_iter"
+    return "k_pcgs_iter" if p.pcg_tol < STANDARD_FORM_TOL else "k_pcg_iter"
 
 
 def kernel_description(which, p, chunk):
     name = pcg_kernel_name(p)
     if which == "pcg":
-        return ("%s (one PCG iteration: ELL block SpMV with A P, block-Jacobi apply, vector updates, "
-                "partial dot products; %s operator), timed by one event pair around each solve's launches, "
+        if p.pcg_tol < STANDARD_FORM_TOL:
+            what = ("one standard-form PCG iteration: ELL block SpMV with A, the body groups' inverse blocks "
+                    "applied through LDS, vector updates, partial dot products")
+        else:
+            what = ("one single-reduction PCG iteration: ELL block SpMV with A P (P the inverse diagonal "
+                    "blocks), vector updates, partial dot products")
+        return ("%s (%s; %s operator), timed by one event pair around each solve's launches, "
                 "queued %d at a time (the launches that find the solve converged included)"
-                % (name, "FP64" if p.pcg_precision == PCG_FP64 else "FP32", chunk))
+                % (name, what, "FP64" if p.pcg_precision == PCG_FP64 else "FP32", chunk))
     return {"pcg_per_launch": "%s, launched without graph replay, timed per launch" % name,
             "contact_terms": "k_constraints + k_contact_res", "contacts": "k_pair_hits"}[which]
 
@@ -127,8 +135,10 @@ def cpu_baseline(sc, state, budget_s, threads, checks):
         "cores": threads,
         "kind": "port",
         "sample": "%d timestep(s) of the FP64 CPU restatement (OpenMP) from the same settled state, same "
-                  "parameters (%d contacts/step avg; inner solve: block-Jacobi PCG to %g)" % (
-                      n, contacts // n, sc.params.pcg_tol),
+                  "parameters (%d contacts/step avg; inner solve: PCG to %g, %s)" % (
+                      n, contacts // n, sc.params.pcg_tol,
+                      "body-group block preconditioner" if sc.params.pcg_tol < STANDARD_FORM_TOL
+                      else "block-Jacobi preconditioner"),
         "median_step_ms": float(np.median(step_ms)),
         "p99_step_ms": float(np.percentile(step_ms, 99)),
         "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()},

@@ -91,10 +91,18 @@ int32_t orc_first_system(orc_scene* s, int32_t C, const int32_t* ids,
  * A is n x n row-major, n a multiple of 6.  Returns 0 on success. */
 int32_t orc_block_cholesky_solve(int32_t n, const double* A, const double* b,
                                  double* x);
-/* Block-Jacobi PCG on a dense SPD matrix (for tests): from x as given when
- * `warm` is nonzero, else from zero.  Returns the iteration count. */
+/* PCG on a dense SPD matrix (for tests): from x as given when `warm` is
+ * nonzero, else from zero; preconditioned by block Jacobi when rep is NULL,
+ * else by the inverse blocks of the groups of block rows given by rep (per
+ * block row, the row holding its group's key).  Returns the iteration count. */
 int32_t orc_pcg_solve(int32_t n, const double* A, const double* b, double tol,
-                      int32_t maxit, double* x, int32_t warm);
+                      int32_t maxit, double* x, int32_t warm, const int32_t* rep);
+/* The body groups of the standard PCG form's preconditioner (per body the
+ * index of the body holding its group's key): n bodies with keys, static
+ * flags and contact counts w; np pairs (pa, pb) with pn contacts each. */
+int32_t orc_group_bodies(int32_t n, const int64_t* key, const int32_t* stat, const int64_t* w,
+                         int32_t np, const int32_t* pa, const int32_t* pb, const int32_t* pn,
+                         int32_t* rep);
 
 #ifdef __cplusplus
 }

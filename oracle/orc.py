@@ -18,6 +18,7 @@ _lib = None
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)
+_L = ctypes.POINTER(ctypes.c_int64)
 
 
 def build():
@@ -67,7 +68,10 @@ def lib():
         L.orc_block_cholesky_solve.restype = ctypes.c_int32
         L.orc_block_cholesky_solve.argtypes = [ctypes.c_int32, _D, _D, _D]
         L.orc_pcg_solve.restype = ctypes.c_int32
-        L.orc_pcg_solve.argtypes = [ctypes.c_int32, _D, _D, ctypes.c_double, ctypes.c_int32, _D, ctypes.c_int32]
+        L.orc_pcg_solve.argtypes = [ctypes.c_int32, _D, _D, ctypes.c_double, ctypes.c_int32, _D, ctypes.c_int32,
+                                    _I]
+        L.orc_group_bodies.restype = ctypes.c_int32
+        L.orc_group_bodies.argtypes = [ctypes.c_int32, _L, _I, _L, ctypes.c_int32, _I, _I, _I, _I]
         _lib = L
     return _lib
 
@@ -251,10 +255,33 @@ def block_cholesky_solve(A, b):
     return x
 
 
-def pcg_solve(A, b, tol=1e-12, maxit=1000, x0=None):
-    """Block-Jacobi PCG from x0 (zero when None); returns (x, iterations)."""
+def pcg_solve(A, b, tol=1e-12, maxit=1000, x0=None, rep=None):
+    """PCG from x0 (zero when None), preconditioned by block Jacobi, or by the
+    inverse blocks of the groups of block rows given by `rep` (per block row,
+    the row holding its group's key); returns (x, iterations)."""
     A = _f64(A)
     b = _f64(b)
     x = np.zeros_like(b) if x0 is None else _f64(x0).copy()
-    it = lib().orc_pcg_solve(A.shape[0], _d(A), _d(b), float(tol), int(maxit), _d(x), int(x0 is not None))
+    r = None if rep is None else np.ascontiguousarray(rep, dtype=np.int32)
+    it = lib().orc_pcg_solve(A.shape[0], _d(A), _d(b), float(tol), int(maxit), _d(x), int(x0 is not None),
+                             _i(r))
+    if it < 0:
+        raise ValueError("orc_pcg_solve: bad arguments")
     return x, it
+
+
+def group_bodies(key, stat, w, pairs):
+    """The body groups of the standard PCG form's preconditioner: per body the
+    index of the body holding its group's key.  pairs: (a, b, contacts)."""
+    key = np.ascontiguousarray(key, dtype=np.int64)
+    n = key.shape[0]
+    stat = np.ascontiguousarray(stat, dtype=np.int32)
+    w = np.ascontiguousarray(w, dtype=np.int64)
+    pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 3))
+    pa, pb, pn = (np.ascontiguousarray(pr[:, k]) for k in range(3))
+    rep = np.zeros(n, np.int32)
+    rc = lib().orc_group_bodies(n, key.ctypes.data_as(_L), _i(stat), w.ctypes.data_as(_L), pr.shape[0], _i(pa),
+                                _i(pb), _i(pn), _i(rep))
+    if rc != 0:
+        raise ValueError("orc_group_bodies: bad arguments")
+    return rep

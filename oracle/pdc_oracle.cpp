@@ -37,16 +37,21 @@
  * Deviations from the reference, all deliberate:
  *   - the direct branch (N < 2000) is a block-sparse Cholesky with a greedy
  *     minimum-degree ordering in place of Eigen's SimplicialLDLT; the PCG
- *     branch is block-Jacobi CG in place of AMGCL SA-AMG CG, started from
- *     zero instead of from the previous direction;
+ *     branch is CG in place of AMGCL SA-AMG CG, preconditioned by the inverse
+ *     diagonal blocks (block Jacobi) at pcg_tol >= 1e-9 and, below it, by the
+ *     inverse blocks of groups of up to eight strongly coupled bodies
+ *     (group_bodies); its start is the previous direction only when that has
+ *     a smaller residual than zero;
  *   - contact-to-body scatters are gathers (no FP atomics), so sums have a
  *     fixed order.
  */
 #include "orc.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -55,6 +60,11 @@
 #endif
 
 namespace {
+
+/* Below this PCG tolerance the device runs the standard PCG form, whose
+ * preconditioner is the group-block inverse (group_bodies); at or above it
+ * the single-reduction form with block Jacobi. */
+constexpr double kStandardFormTol = 1e-9;
 
 /* ------------------------------------------------------------------------ */
 /* small vector / matrix helpers (row-major 3x3)                             */
@@ -548,31 +558,176 @@ void bcsr_mv(const BCSR& A, const double* x, double* y) {
 	}
 }
 
-/* Block-Jacobi PCG.  Starts from x as given when `warm` (r = b - A x), else
- * from zero.  Iterates while it < maxit and |r| > tol |b| (the convergence
- * test of a textbook CG, as in AMGCL's cg, whose initial guess is the caller's
- * X: PrimalDualForceSolver.cpp:406-414).  Returns the iteration count. */
-int bcsr_pcg(const BCSR& A, const double* b, double tol, int maxit, double* x, bool warm) {
-	const int n = A.n;
-	const size_t m = 6 * (size_t)n;
-	std::vector<Blk> Pinv(n);
-	for (int r = 0; r < n; ++r) {
-		const int d = A.find(r, r);
-		Blk L;
-		/* inverse of the diagonal block through its Cholesky factor */
-		if (!chol6(A.val[d].a, L.a)) {
-			std::memset(Pinv[r].a, 0, sizeof(Pinv[r].a));
-			for (int i = 0; i < 6; ++i) Pinv[r].a[i * 6 + i] = 1.0 / A.val[d].a[i * 6 + i];
-			continue;
-		}
-		for (int c = 0; c < 6; ++c) {
-			double e[6] = {0, 0, 0, 0, 0, 0};
-			e[c] = 1;
-			fwd6(L.a, e);
-			bwd6(L.a, e);
-			for (int i = 0; i < 6; ++i) Pinv[r].a[i * 6 + c] = e[i];
+/* Inverse of a dense symmetric matrix (n x n, row-major) through its
+ * Cholesky factor; false (and `inv` untouched) when a pivot is not positive. */
+bool dense_spd_inverse(int n, const std::vector<double>& G, std::vector<double>& inv) {
+	std::vector<double> L((size_t)n * n, 0.0);
+	for (int j = 0; j < n; ++j) {
+		double d = G[(size_t)j * n + j];
+		for (int k = 0; k < j; ++k) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+		if (!(d > 0) || !std::isfinite(d)) return false;
+		const double ljj = std::sqrt(d);
+		L[(size_t)j * n + j] = ljj;
+		for (int i = j + 1; i < n; ++i) {
+			double t = G[(size_t)i * n + j];
+			for (int k = 0; k < j; ++k) t -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+			L[(size_t)i * n + j] = t / ljj;
 		}
 	}
+	inv.assign((size_t)n * n, 0.0);
+	std::vector<double> e(n);
+	for (int c = 0; c < n; ++c) {
+		for (int i = 0; i < n; ++i) {
+			double t = i == c ? 1.0 : 0.0;
+			for (int k = 0; k < i; ++k) t -= L[(size_t)i * n + k] * e[k];
+			e[i] = t / L[(size_t)i * n + i];
+		}
+		for (int i = n - 1; i >= 0; --i) {
+			double t = e[i];
+			for (int k = i + 1; k < n; ++k) t -= L[(size_t)k * n + i] * e[k];
+			e[i] = t / L[(size_t)i * n + i];
+		}
+		for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = e[i];
+	}
+	return true;
+}
+
+/* Block preconditioner of the PCG: for every group of block rows (the rows
+ * whose `rep` is the same; every row alone when `rep` is null) the inverse of
+ * the principal submatrix of A on the group's rows, or, when that is not
+ * positive definite, the inverse of its diagonal. */
+struct GroupPrec {
+	std::vector<std::vector<int>> groups;
+	std::vector<std::vector<double>> inv;
+	void build(const BCSR& A, const int* rep) {
+		const int n = A.n;
+		groups.clear();
+		std::vector<int> gi(n, -1);
+		for (int r = 0; r < n; ++r) {
+			const int k = rep ? rep[r] : r;
+			if (gi[k] < 0) { gi[k] = (int)groups.size(); groups.emplace_back(); }
+			groups[gi[k]].push_back(r);
+		}
+		inv.assign(groups.size(), {});
+#pragma omp parallel for schedule(dynamic)
+		for (int g = 0; g < (int)groups.size(); ++g) {
+			const std::vector<int>& mem = groups[g];
+			const int d = 6 * (int)mem.size();
+			std::vector<double> G((size_t)d * d, 0.0);
+			for (size_t u = 0; u < mem.size(); ++u)
+				for (size_t v = 0; v < mem.size(); ++v) {
+					const int k = A.find(mem[u], mem[v]);
+					if (k < 0) continue;
+					for (int i = 0; i < 6; ++i)
+						for (int j = 0; j < 6; ++j)
+							G[(6 * u + i) * (size_t)d + 6 * v + j] = A.val[k].a[i * 6 + j];
+				}
+			if (!dense_spd_inverse(d, G, inv[g])) {
+				inv[g].assign((size_t)d * d, 0.0);
+				for (int i = 0; i < d; ++i) inv[g][(size_t)i * d + i] = 1.0 / G[(size_t)i * d + i];
+			}
+		}
+	}
+	void apply(const std::vector<double>& in, std::vector<double>& out) const {
+#pragma omp parallel for schedule(dynamic)
+		for (int g = 0; g < (int)groups.size(); ++g) {
+			const std::vector<int>& mem = groups[g];
+			const int d = 6 * (int)mem.size();
+			for (int i = 0; i < d; ++i) {
+				double s = 0;
+				for (int j = 0; j < d; ++j) s += inv[g][(size_t)i * d + j] * in[6 * mem[j / 6] + j % 6];
+				out[6 * mem[i / 6] + i % 6] = s;
+			}
+		}
+	}
+};
+
+/* Groups of strongly coupled bodies for the preconditioner of the standard
+ * PCG form (pcg_tol < 1e-9).  This is this project's own preconditioner, in
+ * place of the reference's AMGCL smoothed-aggregation AMG
+ * (PrimalDualForceSolver.cpp:395-415), whose aggregation it resembles in
+ * grouping unknowns by coupling strength; the strength here is read off the
+ * contact list alone, so the groups are fixed for a timestep.
+ *   Strength of the coupling of two groups G and H of dynamic bodies:
+ *   n_GH / sqrt(w_G w_H), n_GH the contacts between bodies of G and bodies of
+ *   H, w the contacts of a group's bodies (static partners included).
+ *   Three levels merge groups of size 1, 2, 4 into groups of 2, 4, 8.  A
+ *   level runs up to three rounds of handshake matching: every unmerged group
+ *   of the level's size picks the unmerged neighbour group of that size with
+ *   the strongest coupling (ties: the smaller key), and two groups that pick
+ *   each other merge.  A group's key is the smallest key of its bodies.
+ *   Static bodies stay alone.
+ * Inputs per body: key (the global id), stat, w; per pair of bodies with
+ * contacts between them: (a, b, count).  Returns per body the index of the
+ * body holding its group's key. */
+std::vector<int> group_bodies(int n, const std::vector<long long>& key, const std::vector<char>& stat,
+                              const std::vector<long long>& w, const std::vector<std::array<int, 3>>& pairs) {
+	std::vector<std::vector<std::pair<int, int>>> adj(n);
+	for (const auto& pr : pairs)
+		if (!stat[pr[0]] && !stat[pr[1]] && pr[0] != pr[1]) {
+			adj[pr[0]].push_back({pr[1], pr[2]});
+			adj[pr[1]].push_back({pr[0], pr[2]});
+		}
+	std::vector<int> rep(n), size(n, 1);
+	std::vector<long long> wg(w);
+	std::vector<std::vector<int>> mem(n);
+	for (int r = 0; r < n; ++r) { rep[r] = r; mem[r] = {r}; }
+	for (int level = 0; level < 3; ++level) {
+		const int s = 1 << level;
+		auto eligible = [&](int g) { return rep[g] == g && !stat[g] && size[g] == s; };
+		for (int round = 0; round < 3; ++round) {
+			std::vector<int> best(n, -1);
+			for (int g = 0; g < n; ++g) {
+				if (!eligible(g)) continue;
+				std::map<int, long long> nGH;
+				for (int m : mem[g])
+					for (const auto& e : adj[m]) {
+						const int h = rep[e.first];
+						if (h != g && eligible(h)) nGH[h] += e.second;
+					}
+				int bh = -1;
+				long long bn = 0;
+				for (const auto& kv : nGH) {
+					const int h = kv.first;
+					const long long nh = kv.second;
+					if (bh < 0) { bh = h; bn = nh; continue; }
+					/* nh^2 / w_h against bn^2 / w_bh, exactly */
+					const unsigned __int128 lhs = (unsigned __int128)(nh * nh) * (unsigned long long)wg[bh];
+					const unsigned __int128 rhs = (unsigned __int128)(bn * bn) * (unsigned long long)wg[h];
+					if (lhs > rhs || (lhs == rhs && key[h] < key[bh])) { bh = h; bn = nh; }
+				}
+				best[g] = bh;
+			}
+			bool merged = false;
+			for (int g = 0; g < n; ++g) {
+				const int h = best[g];
+				if (h < 0 || best[h] != g || !(key[g] < key[h])) continue;
+				for (int m : mem[h]) rep[m] = g;
+				mem[g].insert(mem[g].end(), mem[h].begin(), mem[h].end());
+				mem[h].clear();
+				size[g] = 2 * s;
+				size[h] = 0;
+				wg[g] += wg[h];
+				merged = true;
+			}
+			if (!merged) break;
+		}
+	}
+	return rep;
+}
+
+/* PCG preconditioned by the inverse blocks of the groups given by `rep` (per
+ * block row, the row holding its group's key; null: every row alone, block
+ * Jacobi).  Starts from x as given when `warm` (r = b - A x), else from
+ * zero.  Iterates while it < maxit and |r| > tol |b| (the convergence test of
+ * a textbook CG, as in AMGCL's cg, whose initial guess is the caller's X:
+ * PrimalDualForceSolver.cpp:406-414).  Returns the iteration count. */
+int bcsr_pcg(const BCSR& A, const double* b, double tol, int maxit, double* x, bool warm,
+             const int* rep = nullptr) {
+	const int n = A.n;
+	const size_t m = 6 * (size_t)n;
+	GroupPrec P;
+	P.build(A, rep);
 	std::vector<double> r(b, b + m), z(m), p(m), q(m);
 	if (warm) {
 		/* the warm start is kept only when its residual is smaller than that
@@ -591,15 +746,7 @@ int bcsr_pcg(const BCSR& A, const double* b, double tol, int maxit, double* x, b
 	} else {
 		std::fill(x, x + m, 0.0);
 	}
-	auto prec = [&](const std::vector<double>& in, std::vector<double>& out) {
-#pragma omp parallel for schedule(static)
-		for (int k = 0; k < n; ++k)
-			for (int i = 0; i < 6; ++i) {
-				double s = 0;
-				for (int j = 0; j < 6; ++j) s += Pinv[k].a[i * 6 + j] * in[6 * k + j];
-				out[6 * k + i] = s;
-			}
-	};
+	auto prec = [&](const std::vector<double>& in, std::vector<double>& out) { P.apply(in, out); };
 	const double bn = std::sqrt(fixed_sum(m, [&](size_t i) { return b[i] * b[i]; }));
 	const double eps = tol * bn;
 	prec(r, z);
@@ -868,6 +1015,22 @@ struct Solver {
 		}
 	}
 	bool active(int b) const { return !stat[b]; }
+
+	/* group_bodies on this solve's contacts: keys are the global ids, w the
+	 * contact count of each body, one pair per two bodies in contact */
+	std::vector<int> contact_groups() const {
+		std::vector<long long> key(N), w(N, 0);
+		for (int b = 0; b < N; ++b) key[b] = (long long)(ob[b] - &scene->bodies[0]);
+		std::map<std::pair<int, int>, int> cnt;
+		for (const Contact& c : cs) {
+			w[c.id[0]]++;
+			w[c.id[1]]++;
+			cnt[{std::min(c.id[0], c.id[1]), std::max(c.id[0], c.id[1])}]++;
+		}
+		std::vector<std::array<int, 3>> pairs;
+		for (const auto& kv : cnt) pairs.push_back({kv.first.first, kv.first.second, kv.second});
+		return group_bodies(N, key, stat, w, pairs);
+	}
 
 	/* PrimalDualForceSolver.cpp:23-78 */
 	void nondim(double dt, const double* g) {
@@ -1354,6 +1517,9 @@ struct Solver {
 		                    (scene->solver == ORC_SOLVER_REFERENCE && N < 2000);
 		BlockCholesky chol;
 		bool analyzed = false;
+		/* the standard PCG form's groups, fixed for the timestep */
+		std::vector<int> grp;
+		if (!direct && p.pcg_tol < kStandardFormTol) grp = contact_groups();
 		scene->trace.clear();
 		Res R;
 		std::vector<double> rhs, du(nR, 0.0), df, dl, Dsc(nR), z(nR);
@@ -1411,7 +1577,8 @@ struct Solver {
 				 * descent fallback; :407-414, :451); zero in the first iteration,
 				 * where the reference's du is freshly allocated */
 				if (it > 0) z = du;
-				iters = bcsr_pcg(A, rhs.data(), p.pcg_tol, p.pcg_max_iter, z.data(), it > 0);
+				iters = bcsr_pcg(A, rhs.data(), p.pcg_tol, p.pcg_max_iter, z.data(), it > 0,
+				                 grp.empty() ? nullptr : grp.data());
 				pcgTotal += iters;
 				if (iters == p.pcg_max_iter) pcgHits++;
 			}
@@ -1750,11 +1917,30 @@ int32_t orc_block_cholesky_solve(int32_t n, const double* A, const double* b, do
 	return 0;
 }
 int32_t orc_pcg_solve(int32_t n, const double* A, const double* b, double tol, int32_t maxit,
-                      double* x, int32_t warm) {
+                      double* x, int32_t warm, const int32_t* rep) {
 	if (n % 6) return -1;
 	BCSR B;
 	dense_to_bcsr(n, A, B);
-	return bcsr_pcg(B, b, tol, maxit, x, warm != 0);
+	if (rep)
+		for (int r = 0; r < n / 6; ++r)
+			if (rep[r] < 0 || rep[r] >= n / 6) return -1;
+	return bcsr_pcg(B, b, tol, maxit, x, warm != 0, rep);
+}
+
+int32_t orc_group_bodies(int32_t n, const int64_t* key, const int32_t* stat, const int64_t* w, int32_t np,
+                         const int32_t* pa, const int32_t* pb, const int32_t* pn, int32_t* rep) {
+	if (n < 0 || np < 0) return -1;
+	std::vector<long long> k(key, key + n), ww(w, w + n);
+	std::vector<char> st(n);
+	for (int r = 0; r < n; ++r) st[r] = stat[r] ? 1 : 0;
+	std::vector<std::array<int, 3>> pairs;
+	for (int q = 0; q < np; ++q) {
+		if (pa[q] < 0 || pa[q] >= n || pb[q] < 0 || pb[q] >= n || pn[q] < 1) return -1;
+		pairs.push_back({pa[q], pb[q], pn[q]});
+	}
+	const std::vector<int> g = group_bodies(n, k, st, ww, pairs);
+	for (int r = 0; r < n; ++r) rep[r] = g[r];
+	return 0;
 }
 
 }  // extern "C"

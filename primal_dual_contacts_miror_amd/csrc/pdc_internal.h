@@ -378,6 +378,17 @@ struct pdc_context {
 	// e_col (-1: empty).
 	pdc::DBuf<int> tb_r;                                      // per scene: first PCG tile (S + 1)
 	pdc::DBuf<int> l_slot;                                    // per local body: its slot
+	// Groups of the standard PCG form's preconditioner (pcg_tol below the
+	// single-reduction form's range): per local body its group's key, per key
+	// the size, contact count and members; per scene the groups of each size
+	// class (8, 4, 2, 1 bodies); per slot log2 of its group's size; the
+	// groups' inverse blocks by slot (6 x 48 per slot: the slot's rows against
+	// the 8 slots of its tile, entries outside the group unused).
+	pdc::DBuf<int> g_rep, g_size, g_mem, g_best, g_cnt, l_gsz;
+	pdc::DBuf<long long> g_w;
+	std::vector<int> h_gcnt;
+	pdc::DBuf<float> G32;
+	pdc::DBuf<double> G64;
 	pdc::DBuf<int> e_col;                                     // [slots][ell_R]
 	pdc::DBuf<int> r_amax;                                    // most blocks in a block row
 	int ell_R = 8;

@@ -41,6 +41,13 @@ enum {
 
 inline int wg_for(int64_t n) { return std::min(grid_for(n), kMaxWG); }
 
+// Below this relative tolerance the standard form (k_pcgs_*) is used: the
+// single-reduction recurrences stall near 1e-10 on ill-conditioned systems.
+// The standard form is preconditioned by body groups (k_grp_*), the
+// single-reduction form by the diagonal blocks.
+constexpr double kSingleReductionMinTol = 1e-9;
+inline bool standard_form(const pdc_context* ctx) { return ctx->p.pcg_tol < kSingleReductionMinTol; }
+
 // ===========================================================================
 // topology
 // ===========================================================================
@@ -250,6 +257,187 @@ __global__ __launch_bounds__(256) void k_row_slot(int NL, const int* __restrict_
 	slot[r] = kPcgRowsPerWG * (tbr[sc] + off / kPcgRowsPerWG) + off % kPcgRowsPerWG;
 }
 
+// ---- groups of strongly coupled bodies --------------------------------------
+// The standard PCG form is preconditioned by the inverse blocks of groups of up
+// to kPcgRowsPerWG bodies, each group inside one PCG tile, so that a tile
+// applies its groups' blocks from its own rows within the iteration's launch.
+// (The reference's PCG branch uses AMGCL's smoothed-aggregation AMG,
+// PrimalDualForceSolver.cpp:395-415; the grouping resembles its aggregation,
+// with the strength read off the contact list, so groups are fixed for the
+// timestep.)  Strength of the coupling of groups G and H of dynamic bodies:
+// n_GH / sqrt(w_G w_H), n_GH the contacts between them, w the contacts of a
+// group's bodies (static partners included).  Three levels merge groups of
+// size 1, 2, 4 into groups of 2, 4, 8; a level runs three rounds of handshake
+// matching: every unmerged group of the level's size picks the unmerged
+// neighbour group of that size with the strongest coupling (ties: the smaller
+// key), and two groups that pick each other merge.  A group's key is its
+// smallest local index (local bodies are numbered by ascending global id, so
+// this is the smallest global id).  Static bodies stay alone.  Per body:
+// g_rep, the group's key; per key: g_size, g_w and the members g_mem (at most
+// kPcgRowsPerWG each).  A round is a propose launch and an accept launch; a
+// round without merges leaves the state as it was, so the rounds need no
+// early exit.
+constexpr int kGroupLevels = 3, kGroupRounds = 3;
+
+__global__ __launch_bounds__(256) void k_grp_init(int NL, const int* __restrict__ inc_cnt, int* __restrict__ rep,
+                                                int* __restrict__ gsize, long long* __restrict__ gw,
+                                                int* __restrict__ mem) {
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= NL) return;
+	rep[r] = r;
+	gsize[r] = 1;
+	gw[r] = inc_cnt[r];
+	mem[kPcgRowsPerWG * r] = r;
+}
+
+// Group r's pick among the neighbour groups of size s: the largest n^2 / w
+// (n the contacts between the groups, w the neighbour's contacts), compared
+// exactly; ties go to the smaller key.  The contacts to a neighbour group are
+// summed over every adjacency entry of r's members that leads to it (each
+// entry of the scan sums them again, so repeated neighbours need no
+// bookkeeping: they compare equal and change nothing).
+__global__ __launch_bounds__(256) void k_grp_propose(int NL, int s, const int* __restrict__ lstat,
+                                                   const int* __restrict__ aptr, const int* __restrict__ anb,
+                                                   const int* __restrict__ apair, const int* __restrict__ pstart,
+                                                   const int* __restrict__ rep, const int* __restrict__ gsize,
+                                                   const long long* __restrict__ gw, const int* __restrict__ mem,
+                                                   int* __restrict__ best) {
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= NL) return;
+	int bh = -1;
+	long long bn = 0;
+	if (rep[r] == r && !lstat[r] && gsize[r] == s) {
+		for (int k = 0; k < s; ++k) {
+			const int m = mem[kPcgRowsPerWG * r + k];
+			for (int q = aptr[m]; q < aptr[m + 1]; ++q) {
+				const int c = anb[q];
+				if (lstat[c]) continue;
+				const int h = rep[c];
+				if (h == r || gsize[h] != s) continue;
+				long long n = 0;
+				for (int k2 = 0; k2 < s; ++k2) {
+					const int m2 = mem[kPcgRowsPerWG * r + k2];
+					for (int q2 = aptr[m2]; q2 < aptr[m2 + 1]; ++q2) {
+						const int c2 = anb[q2];
+						if (!lstat[c2] && rep[c2] == h) {
+							const int p = apair[q2];
+							n += pstart[p + 1] - pstart[p];
+						}
+					}
+				}
+				if (bh < 0) {
+					bh = h;
+					bn = n;
+					continue;
+				}
+				const unsigned __int128 lhs = (unsigned __int128)(unsigned long long)(n * n) * (unsigned long long)gw[bh];
+				const unsigned __int128 rhs = (unsigned __int128)(unsigned long long)(bn * bn) * (unsigned long long)gw[h];
+				if (lhs > rhs || (lhs == rhs && h < bh)) {
+					bh = h;
+					bn = n;
+				}
+			}
+		}
+	}
+	best[r] = bh;
+}
+
+// Mutual picks merge into the group with the smaller key.
+__global__ __launch_bounds__(256) void k_grp_accept(int NL, int s, const int* __restrict__ best, int* __restrict__ rep,
+                                                  int* __restrict__ gsize, long long* __restrict__ gw,
+                                                  int* __restrict__ mem) {
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= NL) return;
+	const int h = best[r];
+	if (h <= r || best[h] != r) return;
+	for (int k = 0; k < s; ++k) {
+		const int m = mem[kPcgRowsPerWG * h + k];
+		mem[kPcgRowsPerWG * r + s + k] = m;
+		rep[m] = r;
+	}
+	gsize[r] = 2 * s;
+	gsize[h] = 0;
+	gw[r] += gw[h];
+}
+
+// Size class of a group: 0 for 8 bodies, 1 for 4, 2 for 2, 3 for 1.
+__device__ __forceinline__ int group_class(int size) {
+	return size >= 8 ? 0 : size >= 4 ? 1 : size >= 2 ? 2 : 3;
+}
+
+// Groups per scene and size class (one workgroup per scene).
+__global__ __launch_bounds__(256) void k_grp_count(int S, const int* __restrict__ lbeg, const int* __restrict__ rep,
+                                                 const int* __restrict__ gsize, int* __restrict__ cnt) {
+	__shared__ int c[4];
+	const int sc = blockIdx.x;
+	if (sc >= S) return;
+	if (threadIdx.x < 4) c[threadIdx.x] = 0;
+	__syncthreads();
+	for (int r = lbeg[sc] + threadIdx.x; r < lbeg[sc + 1]; r += blockDim.x)
+		if (rep[r] == r) atomicAdd(&c[group_class(gsize[r])], 1);
+	__syncthreads();
+	if (threadIdx.x < 4) cnt[4 * sc + threadIdx.x] = c[threadIdx.x];
+}
+
+// PCG slots by group: a scene's groups in order of size (8, 4, 2, 1), groups
+// of one size by key, members in merge order.  Every group then starts at a
+// multiple of its size, so none straddles a tile, and the scene's slots are
+// exactly its ceil(NL / kPcgRowsPerWG) tiles.  gsz[slot] = log2 of the
+// slot's group size.  One workgroup of 256 per scene; ranks by a workgroup
+// scan of four 16-bit counters packed in 64 bits.
+__global__ __launch_bounds__(256) void k_grp_slots(int S, const int* __restrict__ lbeg, const int* __restrict__ tbr,
+                                                 const int* __restrict__ cnt, const int* __restrict__ rep,
+                                                 const int* __restrict__ gsize, const int* __restrict__ mem,
+                                                 int* __restrict__ slot, int* __restrict__ gsz) {
+	__shared__ unsigned long long wtot[4];
+	const int sc = blockIdx.x;
+	if (sc >= S) return;
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	int off[4];
+	off[0] = 0;
+	off[1] = 8 * cnt[4 * sc];
+	off[2] = off[1] + 4 * cnt[4 * sc + 1];
+	off[3] = off[2] + 2 * cnt[4 * sc + 2];
+	int run[4] = {0, 0, 0, 0};
+	const int base = kPcgRowsPerWG * tbr[sc];
+	for (int r0 = lbeg[sc]; r0 < lbeg[sc + 1]; r0 += 256) {
+		const int r = r0 + (int)threadIdx.x;
+		int cls = -1, size = 0;
+		if (r < lbeg[sc + 1] && rep[r] == r) {
+			size = gsize[r];
+			cls = group_class(size);
+		}
+		const unsigned long long mine = cls >= 0 ? 1ull << (16 * cls) : 0ull;
+		unsigned long long x = mine;
+#pragma unroll
+		for (int o = 1; o < 64; o <<= 1) {
+			const unsigned long long y = __shfl_up(x, o, 64);
+			if (lane >= o) x += y;
+		}
+		if (lane == 63) wtot[wv] = x;
+		__syncthreads();
+		unsigned long long before = 0, total = 0;
+		for (int k = 0; k < 4; ++k) {
+			if (k < wv) before += wtot[k];
+			total += wtot[k];
+		}
+		__syncthreads();
+		const unsigned long long excl = before + x - mine;
+		if (cls >= 0) {
+			const int rank = run[cls] + (int)((excl >> (16 * cls)) & 0xffff);
+			const int first = off[cls] + size * rank;
+			int lg = 0;
+			while ((1 << lg) < size) ++lg;
+			for (int j = 0; j < size; ++j) {
+				const int m = mem[kPcgRowsPerWG * r + j];
+				slot[m] = base + first + j;
+				gsz[base + first + j] = lg;
+			}
+		}
+		for (int k = 0; k < 4; ++k) run[k] += (int)((total >> (16 * k)) & 0xffff);
+	}
+}
+
 void ensure_contact_capacity(pdc_context* ctx, size_t C) {
 	if ((int64_t)C <= ctx->c_stride) return;
 	size_t cs = std::max<size_t>(C, (size_t)ctx->c_stride + ctx->c_stride / 2);
@@ -327,6 +515,14 @@ static void scene_ranges(pdc_context* ctx) {
 	                   ctx->b_scene.p, ctx->r_pbeg.p);
 	hipLaunchKernelGGL(k_scene_nnzb, dim3(grid_for(S + 1)), dim3(kBlock), 0, s, S, ctx->r_lbeg.p, ctx->a_ptr.p,
 	                   ctx->r_nnzb.p);
+	const bool grouped = standard_form(ctx);
+	if (grouped) {
+		ctx->g_cnt.ensure(4 * (size_t)S);
+		hipLaunchKernelGGL(k_grp_count, dim3(S), dim3(256), 0, s, S, ctx->r_lbeg.p, ctx->g_rep.p, ctx->g_size.p,
+		                   ctx->g_cnt.p);
+		ctx->h_gcnt.resize(4 * (size_t)S);
+		PDC_HIP(hipMemcpyAsync(ctx->h_gcnt.data(), ctx->g_cnt.p, 4 * S * sizeof(int), hipMemcpyDeviceToHost, s));
+	}
 	ctx->r_amax.ensure(1);
 	PDC_HIP(hipMemsetAsync(ctx->r_amax.p, 0, sizeof(int), s));
 	if (NL > 0) hipLaunchKernelGGL(k_amax, dim3(grid_for(NL)), dim3(kBlock), 0, s, NL, ctx->a_cnt.p, ctx->r_amax.p);
@@ -361,9 +557,14 @@ static void scene_ranges(pdc_context* ctx) {
 	ctx->tb_r.ensure(S + 1);
 	PDC_HIP(hipMemcpyAsync(ctx->tb_r.p, tb.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice, s));
 	ctx->l_slot.ensure(NL + 1);
-	if (NL > 0)
+	if (grouped) {
+		ctx->l_gsz.ensure((size_t)kPcgRowsPerWG * std::max(1, ctx->cap_tr));
+		hipLaunchKernelGGL(k_grp_slots, dim3(S), dim3(256), 0, s, S, ctx->r_lbeg.p, ctx->tb_r.p, ctx->g_cnt.p,
+		                   ctx->g_rep.p, ctx->g_size.p, ctx->g_mem.p, ctx->l_slot.p, ctx->l_gsz.p);
+	} else if (NL > 0) {
 		hipLaunchKernelGGL(k_row_slot, dim3(grid_for(NL)), dim3(kBlock), 0, s, NL, ctx->l2g.p, ctx->b_scene.p,
 		                   ctx->r_lbeg.p, ctx->tb_r.p, ctx->l_slot.p);
+	}
 	PDC_CHECK_LAUNCH();
 	std::vector<PcgTile> pt(t.size());
 	for (size_t k = 0; k < t.size(); ++k) {
@@ -379,6 +580,28 @@ static void scene_ranges(pdc_context* ctx) {
 	ctx->n_tr = (int)t.size();
 	ctx->t_pcg.ensure(std::max<size_t>(pt.size(), 1));
 	PDC_HIP(hipMemcpyAsync(ctx->t_pcg.p, pt.data(), pt.size() * sizeof(PcgTile), hipMemcpyHostToDevice, ctx->stream));
+}
+
+// The groups of the local bodies (k_grp_init, then the levels' rounds).
+static void group_bodies(pdc_context* ctx) {
+	hipStream_t s = ctx->stream;
+	const int NL = ctx->NL;
+	const size_t nb = (size_t)NL + 1;
+	ctx->g_rep.ensure(nb); ctx->g_size.ensure(nb); ctx->g_best.ensure(nb); ctx->g_w.ensure(nb);
+	ctx->g_mem.ensure(kPcgRowsPerWG * nb);
+	if (NL == 0) return;
+	const int grid = grid_for(NL);
+	hipLaunchKernelGGL(k_grp_init, dim3(grid), dim3(kBlock), 0, s, NL, ctx->inc_cnt.p, ctx->g_rep.p, ctx->g_size.p,
+	                   ctx->g_w.p, ctx->g_mem.p);
+	for (int level = 0, size = 1; level < kGroupLevels; ++level, size *= 2)
+		for (int round = 0; round < kGroupRounds; ++round) {
+			hipLaunchKernelGGL(k_grp_propose, dim3(grid), dim3(kBlock), 0, s, NL, size, ctx->l_static.p,
+			                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->adj_pair.p, ctx->ap_start.p, ctx->g_rep.p,
+			                   ctx->g_size.p, ctx->g_w.p, ctx->g_mem.p, ctx->g_best.p);
+			hipLaunchKernelGGL(k_grp_accept, dim3(grid), dim3(kBlock), 0, s, NL, size, ctx->g_best.p, ctx->g_rep.p,
+			                   ctx->g_size.p, ctx->g_w.p, ctx->g_mem.p);
+		}
+	PDC_CHECK_LAUNCH();
 }
 
 void build_topology(pdc_context* ctx) {
@@ -432,6 +655,7 @@ void build_topology(pdc_context* ctx) {
 	                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->adj_pair.p, ctx->ap_start.p, ctx->l2g.p,
 	                   ctx->b_static.p, ctx->adj_pos.p, ctx->inc_cnt.p, ctx->a_cnt.p, ctx->l_static.p);
 	PDC_CHECK_LAUNCH();
+	if (standard_form(ctx)) group_bodies(ctx);
 	scan_exclusive(ctx, ctx->inc_cnt.p, ctx->inc_ptr.p, NL);
 	scan_exclusive(ctx, ctx->a_cnt.p, ctx->a_ptr.p, NL);
 	ctx->inc.ensure(2 * (size_t)C);
@@ -1295,6 +1519,95 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ 
 	}
 }
 
+// ---- (d) the groups' inverse blocks -------------------------------------------
+// A double read from another lane of the wavefront (a compile-time lane after
+// unrolling), by two v_readlane_b32.
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+	const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+	const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+	const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Inverse blocks of the body groups (standard PCG form) from the assembled,
+// scaled operator: one workgroup per PCG tile, one wavefront per group (the
+// wavefront of the group's first slot).  Lane i < 6 g holds row i of the
+// group's 6 g x 6 g block (the diagonal blocks and the blocks between the
+// group's bodies, found in the ELL rows), lanes past it rows of the identity;
+// the block, a principal submatrix of the symmetric positive definite Newton
+// matrix, is inverted in place by Gauss-Jordan elimination without pivoting,
+// each pivot row read from its lane.  A block with a pivot that is not
+// positive and finite gets the inverse of its diagonal instead.  Stored per
+// slot: its six rows against the tile's eight slots, entry (i, 6 m + c) at
+// G[288 slot + 48 i + 6 m + c], written and read for the group's slots only.
+constexpr int kGroupRow = 6 * kPcgRowsPerWG;             // 48 columns of a tile row
+constexpr int kGroupBlock = 6 * kGroupRow;               // 288 entries per slot
+static_assert((1 << kGroupLevels) == kPcgRowsPerWG, "the largest group fills one PCG tile");
+template <class T>
+__global__ __launch_bounds__(64 * kPcgRowsPerWG) void k_grp_inv(const PcgTile* __restrict__ tiles,
+                                                        const SceneSc* __restrict__ ss, const int* __restrict__ ecol,
+                                                        int R, const T* __restrict__ Aval,
+                                                        const int* __restrict__ gsz, T* __restrict__ G) {
+	const PcgTile* Tp = tiles + blockIdx.x;
+	const int scn = Tp->scene, cnt = Tp->count;
+	if (scn < 0 || !ss[scn].run) return;
+	const int wid = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+	if (wid >= cnt) return;
+	const int base = kPcgRowsPerWG * (int)blockIdx.x;
+	const int gs = 1 << gsz[base + wid];
+	if (wid & (gs - 1)) return;  // not the group's first slot
+	const int n = 6 * gs;
+	const int i = lane < n ? lane : 0;
+	const int sr = base + wid + i / 6, ci = i % 6;
+	double a[kGroupRow];
+#pragma unroll
+	for (int k = 0; k < kGroupRow; ++k) a[k] = (lane >= n && k == lane) ? 1.0 : 0.0;
+	if (lane < n)
+		for (int e = 0; e < R; ++e) {
+			const size_t q = (size_t)sr * R + e;
+			const int c = ecol[q];
+			const int cb = c - (base + wid);
+			if (c < 0 || cb < 0 || cb >= gs) continue;
+			const T* blk = Aval + 36 * q + 6 * ci;
+			double v[6];
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) v[cc] = (double)blk[cc];
+#pragma unroll
+			for (int m = 0; m < kPcgRowsPerWG; ++m)
+				if (m == cb)
+#pragma unroll
+					for (int cc = 0; cc < 6; ++cc) a[6 * m + cc] = v[cc];
+		}
+	double d0 = 1.0;
+#pragma unroll
+	for (int k = 0; k < kGroupRow; ++k)
+		if (k == i) d0 = a[k];
+	bool ok = true;
+	// pivot step j (uniform): the columns unrolled, j selected among them, so
+	// that the row stays in registers
+	for (int j = 0; j < n; ++j) {
+		double f = 0;
+#pragma unroll
+		for (int k = 0; k < kGroupRow; ++k) f = k == j ? a[k] : f;
+		const double pj = readlane_f64(f, j);
+		ok = ok && pj > 0.0 && isfinite(pj);
+		const double inv = 1.0 / pj;
+		const bool me = lane == j;
+		const double aj = me ? inv : -f * inv;
+#pragma unroll
+		for (int k = 0; k < kGroupRow; ++k) {
+			const double pk = readlane_f64(a[k], j) * inv;
+			a[k] = k == j ? aj : (me ? pk : a[k] - f * pk);
+		}
+	}
+	if (lane < n) {
+		T* row = G + kGroupBlock * (size_t)sr + kGroupRow * ci + 6 * wid;
+#pragma unroll
+		for (int k = 0; k < kGroupRow; ++k)
+			if (k < n) row[k] = (T)(ok ? a[k] : (k == i ? 1.0 / d0 : 0.0));
+	}
+}
+
 // ---- (d) PCG ----------------------------------------------------------------
 // Block-Jacobi preconditioned CG on the scaled block-CSR system from the
 // initial guess x0 (the previous Newton increment, zero in the first
@@ -1343,6 +1656,8 @@ struct PcgArgs {
 	const int* ecol;  // [slots][R]: column slot of each ELL entry, -1 empty
 	const T* B;       // A M, ELL by slot, row-major 6x6 blocks
 	const T* P;       // block-Jacobi inverse, one 6x6 block per slot
+	const T* G;       // standard form: the groups' inverse blocks (k_grp_inv)
+	const int* gsz;   // standard form: per slot, log2 of its group's size
 	const T* b;
 	double* x;
 	double* r;
@@ -1408,6 +1723,33 @@ __device__ __forceinline__ double spmv_ell(const PcgArgs<T>& a, const V* __restr
 }
 
 constexpr int kPcgThreads = 64 * kPcgRowsPerWG;
+
+// The standard form's preconditioner, row i of the tile row `wid` (slot br,
+// group size 2^lg): lane (i, m) holds the six entries of the row against slot
+// m of the tile when m lies in the row's group (gin), and takes their product
+// with slot m's entries of a vector the tile has placed in LDS (6 per tile
+// row); the eight lanes of the row are summed.
+template <class T>
+__device__ __forceinline__ bool group_row(const PcgArgs<T>& a, int br, int wid, int i, int m, int cnt, int lg,
+                                          double (&pg)[6]) {
+	const bool gin = i < 6 && m < cnt && (m >> lg) == (wid >> lg);
+	if (gin) {
+		const T* src = a.G + kGroupBlock * (size_t)br + kGroupRow * i + 6 * m;
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) pg[cc] = (double)src[cc];
+	} else {
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) pg[cc] = 0.0;
+	}
+	return gin;
+}
+__device__ __forceinline__ double group_apply(const double (&pg)[6], bool gin, const double* vl, int m) {
+	double t = 0;
+	if (gin)
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) t += pg[cc] * vl[6 * m + cc];
+	return group8_sum(t);
+}
 // Totals of N <= 4 partial arrays p0..p{N-1} over the scene's tiles
 // [t0, t0 + W): wavefront k sums array k; every thread gets all totals.
 // (Separate pointer arguments, not an array of pointers: a runtime-indexed
@@ -1756,40 +2098,39 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcgs_start(PcgArgs<T> a, const double* __restrict__ x0) {
 	__shared__ double lds[2 * 16];
+	__shared__ double rl[6 * kPcgRowsPerWG], bl[6 * kPcgRowsPerWG];
 	PcgTile* Tp = a.tiles + blockIdx.x;
 	if (pcg_tile_skip(a, Tp)) return;
 	const int cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	const int ir = i < 6 ? i : 5;
-	double v[2] = {0, 0};
+	const int br = pcg_slot();
+	const int lg = wid < cnt ? a.gsz[br] : 0;
+	double pg[6];
+	const bool gin = group_row(a, br, wid, i, sl, cnt, lg, pg);
+	double ri = 0, bi = 0;
 	if (wid < cnt) {
-		const int br = pcg_slot();
 		const double ax = spmv_ell(a, x0, br, i, sl);
-		const double bi = (double)a.b[6 * br + ir];
-		const double ri = bi - ax;
-		double ra[6], bv[6];
-#pragma unroll
-		for (int cc = 0; cc < 6; ++cc) {
-			ra[cc] = __shfl(ri, 8 * cc, 64);
-			bv[cc] = (double)a.b[6 * br + cc];
-		}
-		double zi = 0, zb = 0;
-#pragma unroll
-		for (int cc = 0; cc < 6; ++cc) {
-			const double pc = (double)a.P[36 * (size_t)br + 6 * ir + cc];
-			zi += pc * ra[cc];
-			zb += pc * bv[cc];
-		}
+		bi = (double)a.b[6 * br + ir];
+		ri = bi - ax;
 		if (i < 6 && sl == 0) {
-			const int t = 6 * br + i;
-			a.r[t] = ri;
-			a.zv[0][t] = zi;
-			a.pv[0][t] = zi;
-			a.zv[1][t] = zb;  // z of the zero start, P b
-			v[0] = ri * ri;
-			v[1] = bi * bi;
+			rl[6 * wid + i] = ri;
+			bl[6 * wid + i] = bi;
 		}
+	}
+	__syncthreads();
+	const double zi = group_apply(pg, gin, rl, sl);
+	const double zb = group_apply(pg, gin, bl, sl);
+	double v[2] = {0, 0};
+	if (wid < cnt && i < 6 && sl == 0) {
+		const int t = 6 * br + i;
+		a.r[t] = ri;
+		a.zv[0][t] = zi;
+		a.pv[0][t] = zi;
+		a.zv[1][t] = zb;  // z of the zero start, P b
+		v[0] = ri * ri;
+		v[1] = bi * bi;
 	}
 	double out[2];
 	block_sum<2, double>(v, lds, out);
@@ -1807,20 +2148,20 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_init(PcgArgs<T> a) {
 	const int scn = Tp->scene, cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
-	const int ir = i < 6 ? i : 5;
 	const bool zero = pcg_zero_start(a, Tp, lds);
 	__syncthreads();
+	__shared__ double ql[6 * kPcgRowsPerWG];
+	const int br = pcg_slot();
+	const int lg = wid < cnt ? a.gsz[br] : 0;
+	double pg[6];
+	const bool gin = group_row(a, br, wid, i, sl, cnt, lg, pg);
+	// q0 = A p0 with p0 = z0 of the start kept (P b for a zero start)
+	const double n = wid < cnt ? spmv_ell(a, zero ? a.zv[1] : a.pv[0], br, i, sl) : 0.0;
+	if (i < 6 && sl == 0) ql[6 * wid + i] = n;
+	__syncthreads();
+	const double wi = group_apply(pg, gin, ql, sl);
 	double v[6] = {0, 0, 0, 0, 0, 0};
 	if (wid < cnt) {
-		const int br = pcg_slot();
-		// q0 = A p0 with p0 = z0 of the start kept (P b for a zero start)
-		const double n = spmv_ell(a, zero ? a.zv[1] : a.pv[0], br, i, sl);
-		double qa[6];
-#pragma unroll
-		for (int cc = 0; cc < 6; ++cc) qa[cc] = __shfl(n, 8 * cc, 64);
-		double wi = 0;
-#pragma unroll
-		for (int cc = 0; cc < 6; ++cc) wi += (double)a.P[36 * (size_t)br + 6 * ir + cc] * qa[cc];
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
 			const double rt = zero ? (double)a.b[t] : a.r[t];
@@ -1884,14 +2225,15 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slo
 	int ec[kE];
 #pragma unroll
 	for (int m = 0; m < kE; ++m) ec[m] = a.ecol[(size_t)br * R + sl + 8 * m];
-	double pr[6];
-#pragma unroll
-	for (int cc = 0; cc < 6; ++cc) pr[cc] = (double)a.P[36 * (size_t)br + 6 * ir + cc];
+	const int lgr = a.gsz[br];
 	const int t = 6 * br + ir;
 	const double xo = a.x[t], ro = a.r[t], qo = a.q[t], zo = zc[t], wo = wc[t], po = pc[t];
 	const int scn = rec[0], cnt = rec[2], t0 = rec[3], W = rec[4], done = rec[5], it = rec[6];
 	if (scn < 0 || done || it >= a.maxit) return;
-	// round 2: partial sums of F(i-1), operator blocks and neighbour z, w, p
+	// round 2: partial sums of F(i-1), operator blocks, neighbour z, w, p and
+	// the row's preconditioner entries
+	double pg[6];
+	const bool gin = group_row(a, br, wid, i, sl, cnt, wid < cnt ? lgr : 0, pg);
 	const double* psrc = wid == 0 ? pick(a.del, cur) : wid == 1 ? pick(a.gam, cur) : wid == 2 ? pick(a.e1, cur)
 	                   : wid == 3 ? pick(a.e1b, cur) : wid == 4 ? pick(a.e2, cur) : wid == 5 ? pick(a.rr, cur)
 	                   : a.bb;
@@ -1966,14 +2308,13 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slo
 			n += d;
 		}
 	n = group8_sum(n);
-	double qa[6];
-#pragma unroll
-	for (int cc = 0; cc < 6; ++cc) qa[cc] = __shfl(n, 8 * cc, 64);
+	// w_i = P q_i over the row's group, from the tile's q_i in LDS
+	__shared__ double ql[6 * kPcgRowsPerWG];
+	if (i < 6 && sl == 0) ql[6 * wid + i] = n;
+	__syncthreads();
+	const double wn = group_apply(pg, gin, ql, sl);
 	double v[6] = {0, 0, 0, 0, 0, 0};
 	if (wid < cnt && i < 6 && sl == 0) {
-		double wn = 0;
-#pragma unroll
-		for (int cc = 0; cc < 6; ++cc) wn += pr[cc] * qa[cc];
 		const double rn = ro - alpha * qo;
 		const double zn = zo - alpha * wo;
 		const double pn = zn + beta * po;
@@ -2508,14 +2849,11 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 	PDC_CHECK_LAUNCH();
 }
 
-// Below this relative tolerance the standard form (k_pcgs_*) is used: the
-// single-reduction recurrences stall near 1e-10 on ill-conditioned systems.
-constexpr double kSingleReductionMinTol = 1e-9;
 
 // Arguments of the PCG kernels for the current scene set (grid fixed per
 // scene set: the capacity of PCG tiles, so that a recorded chunk stays valid).
 template <class T>
-static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b) {
+static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b, DBuf<T>& G) {
 	const int S = ctx->S;
 	const int nblk = std::max(1, ctx->cap_tr);
 	const size_t ncap = 6 * (size_t)kPcgRowsPerWG * nblk, pcap = (size_t)nblk;
@@ -2527,6 +2865,8 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 	pa.R = ctx->ell_R;
 	pa.B = Bm.p;
 	pa.P = P.p;
+	pa.G = G.p;
+	pa.gsz = ctx->l_gsz.p;
 	pa.b = b.p;
 	pa.x = ctx->x64.p;
 	double* w = work.p;
@@ -2579,15 +2919,21 @@ static void pcg_begin(pdc_context* ctx, const PcgArgs<T>& pa) {
 //   k_pcg_iter   B blocks and column indices nnzb (36 T + 4), row pointers
 //                4 (NL + 1), P blocks 36 T NL, w, z, s, p, x, r read
 //                6 x 8 n6, z, s, p, x, r, w' written 6 x 8 n6
-//   k_pcgs_iter  A blocks, column indices, row pointers, P as above, x, r,
-//                q, z, w, p read 6 x 8 n6, written 6 x 8 n6
+//   k_pcgs_iter  A blocks, column indices, row pointers, x, r, q, z, w, p
+//                read 6 x 8 n6, written 6 x 8 n6, and the groups' inverse
+//                blocks: 36 T g^2 per group of g bodies
 template <class T>
 static void pcg_account_bytes(pdc_context* ctx, const std::vector<int>& iters, bool two) {
 	const double Tb = sizeof(T);
 	for (int sc = 0; sc < ctx->S; ++sc) {
 		if (iters[sc] <= 0) continue;
 		const double NLs = ctx->h_lbeg[sc + 1] - ctx->h_lbeg[sc], n6 = 6.0 * NLs, nnzb = ctx->h_nnzb[sc];
-		const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NLs + 1) + 36.0 * Tb * NLs;
+		double pblocks = NLs;
+		if (two && ctx->h_gcnt.size() >= 4 * (size_t)ctx->S) {
+			const int* c = &ctx->h_gcnt[4 * (size_t)sc];
+			pblocks = 64.0 * c[0] + 16.0 * c[1] + 4.0 * c[2] + c[3];
+		}
+		const double mat = nnzb * (36.0 * Tb + 4.0) + 4.0 * (NLs + 1) + 36.0 * Tb * pblocks;
 		ctx->kt.bytes += iters[sc] * (mat + 96.0 * n6);
 	}
 }
@@ -2924,8 +3270,10 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	const bool plain = ctx->p.pcg_tol < kSingleReductionMinTol;  // the standard form's operator
 	if (fp32) {
 		ctx->A32.ensure(36 * nell); ctx->P32.ensure(36 * nslot); ctx->b32.ensure(6 * nslot);
+		if (plain) ctx->G32.ensure(kGroupBlock * nslot);
 	} else {
 		ctx->A64.ensure(36 * nell); ctx->P64.ensure(36 * nslot); ctx->b64.ensure(6 * nslot);
+		if (plain) ctx->G64.ensure(kGroupBlock * nslot);
 	}
 	SArgs a0 = make_args(ctx, dt, g);
 	ctx->partials.ensure((size_t)P_COUNT * a0.pstride);
@@ -2974,8 +3322,8 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	// the line search and the end of the iteration
 	PcgArgs<float> pa32;
 	PcgArgs<double> pa64;
-	if (fp32) pa32 = pcg_args<float>(ctx, ctx->A32, ctx->P32, ctx->b32);
-	else pa64 = pcg_args<double>(ctx, ctx->A64, ctx->P64, ctx->b64);
+	if (fp32) pa32 = pcg_args<float>(ctx, ctx->A32, ctx->P32, ctx->b32, ctx->G32);
+	else pa64 = pcg_args<double>(ctx, ctx->A64, ctx->P64, ctx->b64, ctx->G64);
 	auto phase_a = [&]() {
 		hipEvent_t e0;
 		kt_begin(ctx, PDC_KT_CONTACT_TERMS, &e0);
@@ -2995,10 +3343,14 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		kt_begin(ctx, PDC_KT_ASSEMBLY, &e0);
 		launch(k_asm_prep, ntc, kBlock, a);
 		// the standard form (tight tolerances) stores A itself, the recurrences A P
+		const int ntr = std::max(1, ctx->cap_tr);
 		if (fp32 && plain) {
 			launch(k_asm_diag<float, true>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A32.p, ctx->P32.p, ctx->b32.p);
 			launch(k_asm_off<float, false, true>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
 			launch(k_asm_off<float, true, true>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
+			launch(k_grp_inv<float>, ntr, kPcgThreads, (const PcgTile*)ctx->t_pcg.p, (const SceneSc*)ctx->d_ss.p,
+			       (const int*)ctx->e_col.p, ctx->ell_R, (const float*)ctx->A32.p, (const int*)ctx->l_gsz.p,
+			       ctx->G32.p);
 		} else if (fp32) {
 			launch(k_asm_diag<float, false>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A32.p, ctx->P32.p, ctx->b32.p);
 			launch(k_asm_off<float, false, false>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
@@ -3007,6 +3359,9 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 			launch(k_asm_diag<double, true>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A64.p, ctx->P64.p, ctx->b64.p);
 			launch(k_asm_off<double, false, true>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
 			launch(k_asm_off<double, true, true>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);
+			launch(k_grp_inv<double>, ntr, kPcgThreads, (const PcgTile*)ctx->t_pcg.p, (const SceneSc*)ctx->d_ss.p,
+			       (const int*)ctx->e_col.p, ctx->ell_R, (const double*)ctx->A64.p, (const int*)ctx->l_gsz.p,
+			       ctx->G64.p);
 		} else {
 			launch(k_asm_diag<double, false>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A64.p, ctx->P64.p, ctx->b64.p);
 			launch(k_asm_off<double, false, false>, ntp, kBlock, a, (const double*)ctx->P64.p, ctx->A64.p);

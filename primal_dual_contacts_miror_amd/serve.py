@@ -40,7 +40,7 @@ import time
 import numpy as np
 
 from . import pdc
-from .params import SHAPE_SPHERE
+from .params import PCG_FP64, SHAPE_SPHERE
 
 PAD_HALF = 1e-3      # half extent of a padding body
 PAD_FAR = 1.0e6      # its distance from everything else
@@ -67,13 +67,17 @@ def scene_samples(scene, params=None):
 
 def scene_footprint(scene, params=None):
     """Estimated HBM bytes a scene adds to a shard: per body its FP32 SDF
-    grid, state, flags and solver rows, 24 bytes per surface sample, and its
-    expected contacts (CONTACTS_PER_BODY per body, BYTES_PER_CONTACT each:
-    contact record and solver state)."""
+    grid, state, flags and solver rows (below a PCG tolerance of 1e-9 also the
+    288 entries of its group's inverse blocks and the grouping's integers),
+    24 bytes per surface sample, and its expected contacts
+    (CONTACTS_PER_BODY per body, BYTES_PER_CONTACT each: contact record and
+    solver state)."""
     p = params if params is not None else scene.params
     n = scene.n + 1
     grid = n * (p.grid_n ** 3) * 4
     per_body = 13 * 8 + 64 + 6 * 8 * 12 + 36 * 8 * 18
+    if p.pcg_tol < 1e-9:
+        per_body += 288 * (8 if p.pcg_precision == PCG_FP64 else 4) + 16 * 4
     return int(grid + 24 * scene_samples(scene, p) + n * per_body + scene.n * CONTACTS_PER_BODY * BYTES_PER_CONTACT)
 
 

@@ -191,3 +191,57 @@ def first_system(ids, T, mass, I0, quat, vel, ang, is_static, dt, g, mu, restitu
     A = M + eps * np.eye(6 * N) + (Pact[:, None] * JWJ * Pact[None, :])
     rhs = (J.T @ W @ sf - ru) * Pact
     return A, rhs, order, dict(mbar=mbar, s=s, eps=eps, err=err)
+
+
+def group_bodies(key, stat, w, pairs):
+    """The body groups of the standard PCG form's preconditioner, stated over
+    sets: groups are frozensets of body indices, coupling strengths exact
+    fractions n_GH^2 / (w_G w_H) (the square of n_GH / sqrt(w_G w_H)).
+    Levels of sizes 1, 2, 4 each run up to three rounds in which every
+    unmerged group of the level's size picks its strongest unmerged neighbour
+    group of that size (ties: smaller key) and mutual picks merge.  Returns
+    per body the index of its group's member with the smallest key."""
+    from fractions import Fraction
+    n = len(key)
+    between = {}
+    for a, b, c in pairs:
+        if a != b and not stat[a] and not stat[b]:
+            e = frozenset((a, b))
+            between[e] = between.get(e, 0) + c
+    groups = {frozenset([r]) for r in range(n)}
+    gkey = lambda G: min(key[r] for r in G)
+    wsum = lambda G: sum(w[r] for r in G)
+
+    def contacts(G, H):
+        return sum(c for e, c in between.items() if len(e & G) == 1 and len(e & H) == 1 and e <= G | H)
+
+    for size in (1, 2, 4):
+        for _ in range(3):
+            level = [G for G in groups if len(G) == size and not any(stat[r] for r in G)]
+            pick = {}
+            for G in level:
+                best = None
+                for H in level:
+                    if H is G:
+                        continue
+                    c = contacts(G, H)
+                    if c == 0:
+                        continue
+                    s = Fraction(c * c, wsum(G) * wsum(H))
+                    if best is None or s > best[0] or (s == best[0] and gkey(H) < gkey(best[1])):
+                        best = (s, H)
+                if best:
+                    pick[G] = best[1]
+            merges = [(G, H) for G, H in pick.items() if pick.get(H) is G and gkey(G) < gkey(H)]
+            for G, H in merges:
+                groups.discard(G)
+                groups.discard(H)
+                groups.add(G | H)
+            if not merges:
+                break
+    out = np.zeros(n, dtype=np.int64)
+    for G in groups:
+        holder = min(G, key=lambda r: key[r])
+        for r in G:
+            out[r] = holder
+    return out
