@@ -333,16 +333,16 @@ def test_step_is_deterministic(require_gpu):
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("variant", ["fp32", "cap", "fp64", "newton_cap"])
+@pytest.mark.parametrize("variant", ["fp32", "cap", "fp64", "fp64_cap", "newton_cap"])
 def test_graph_replay_equals_direct_launches(require_gpu, monkeypatch, variant):
     """The asynchronous host loop (PCG launches queued in groups from the
     kernels' progress records without waiting for the stream, every kernel
-    launched directly) gives the same bits as the synchronous loop that
-    waits for the stream after each group of PCG launches (PDC_NO_GRAPHS),
-    over steps whose contact sets and sizes change.  Variants: the FP32 PCG;
-    the FP32 PCG capped at 6 iterations (every solve ends at the cap, where
-    the asynchronous loop waits for the stream); the FP64 two-reduction PCG;
-    the Newton loop capped at 3 iterations."""
+    launched directly) gives the same bits as the synchronous loop that waits
+    for the stream after each group of PCG launches (PDC_NO_GRAPHS), over
+    steps whose contact sets and sizes change.  Variants: the FP32 PCG; the
+    FP32 PCG capped at 6 iterations (every solve ends at the cap, where the
+    asynchronous loop waits for the stream); the FP64 standard form; the same
+    capped at 6 iterations; the Newton loop capped at 3 iterations."""
     sc = scenes.pile(60, seed=4, half=2.5, height=8.0, name="pile_60", grid_n=48, sample_spacing=0.1)
     o = _advance_oracle(sc, 150)
     state = o.get_state()
@@ -351,9 +351,12 @@ def test_graph_replay_equals_direct_launches(require_gpu, monkeypatch, variant):
     if variant == "cap":
         q.pcg_max_iter = 6
         steps = 6
-    elif variant == "fp64":
+    elif variant in ("fp64", "fp64_cap"):
         q.pcg_precision = PCG_FP64
         q.pcg_tol = 1e-12
+        if variant == "fp64_cap":
+            q.pcg_max_iter = 6
+            steps = 6
     elif variant == "newton_cap":
         q.iterations = 3
     res = []
@@ -375,7 +378,7 @@ def test_graph_replay_equals_direct_launches(require_gpu, monkeypatch, variant):
     assert sum(t[1] for t in res[0][0]) > steps, "the steps should iterate"
     if variant == "newton_cap":
         assert any(t[1] == 3 for t in res[0][0]), "solves should end at the Newton cap"
-    if variant == "cap":
+    if variant in ("cap", "fp64_cap"):
         assert res[0][1] > 0 and res[0][1] == res[1][1], "solves should reach the cap"
     for a, b in zip(res[0][0], res[1][0]):
         assert a[:3] == b[:3]
