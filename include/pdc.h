@@ -188,6 +188,15 @@ int pdc_reserve_bodies(pdc_context* ctx, int64_t bodies, int64_t samples);
 /* Device memory held by the context's buffers, in bytes (the body table,
  * grids, samples, contacts and solver state as they stand). */
 int pdc_device_bytes(const pdc_context* ctx, int64_t* bytes);
+/* Diagnostics, no counterpart in the reference: the body groups of the last
+ * solve's preconditioner (built by solves with pcg_tol below 1e-9, whose
+ * standard PCG form is preconditioned by the inverse blocks of groups of up
+ * to 8 strongly coupled bodies).  group_key (one entry per uploaded body)
+ * receives, for every body in the last solve's contacts, the global id of
+ * its group's key body (its smallest id), and -1 for the others.
+ * PDC_ERR_STATE when no such solve has run since the last upload (or the
+ * last call was pdc_fill_forces_noalloc, which keeps nothing). */
+int pdc_get_groups(pdc_context* ctx, int32_t n_bodies, int32_t* group_key);
 
 /* ---- (a) contact generation ----------------------------------------------- */
 /* Runs broadphase, grid-SDF sample tests, compaction, the reference's

@@ -382,6 +382,26 @@ int pdc_device_bytes(const pdc_context* ctx, int64_t* bytes) {
 	return PDC_OK;
 }
 
+int pdc_get_groups(pdc_context* ctx, int32_t n_bodies, int32_t* group_key) {
+	return guard(ctx, [&] {
+		require_bodies(ctx);
+		if (!group_key || n_bodies != ctx->N)
+			throw std::invalid_argument("pdc_get_groups: group_key must hold one entry per uploaded body");
+		if (!ctx->groups_valid)
+			throw std::logic_error("pdc_get_groups: no solve below pcg_tol 1e-9 has built body groups");
+		hipStream_t s = ctx->stream;
+		const int NL = ctx->NL;
+		std::vector<int> rep(std::max(NL, 1)), l2g(std::max(NL, 1));
+		if (NL > 0) {
+			PDC_HIP(hipMemcpyAsync(rep.data(), ctx->g_rep.p, NL * sizeof(int), hipMemcpyDeviceToHost, s));
+			PDC_HIP(hipMemcpyAsync(l2g.data(), ctx->l2g.p, NL * sizeof(int), hipMemcpyDeviceToHost, s));
+		}
+		PDC_HIP(hipStreamSynchronize(s));
+		std::fill(group_key, group_key + n_bodies, -1);
+		for (int r = 0; r < NL; ++r) group_key[l2g[r]] = l2g[rep[r]];
+	});
+}
+
 int pdc_set_state(pdc_context* ctx, const double* pos, const double* quat_xyzw, const double* vel,
                   const double* angvel) {
 	return guard(ctx, [&] {
@@ -576,6 +596,7 @@ int pdc_fill_forces_noalloc(pdc_context* ctx, int32_t n_contacts, const int32_t*
 			ctx->c_stride = saved_stride;
 			ctx->kt.which = saved_kt;
 			ctx->C = 0;
+			ctx->groups_valid = false;
 		};
 		ctx->c_stride = 0;
 		ctx->kt.which = PDC_KT_NONE;

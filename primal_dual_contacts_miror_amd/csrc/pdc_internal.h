@@ -385,6 +385,7 @@ struct pdc_context {
 	// groups' inverse blocks by slot (6 x 48 per slot: the slot's rows against
 	// the 8 slots of its tile, entries outside the group unused).
 	pdc::DBuf<int> g_rep, g_size, g_mem, g_best, g_cnt, l_gsz;
+	bool groups_valid = false;                                // g_rep holds the last topology's groups
 	pdc::DBuf<long long> g_w;
 	std::vector<int> h_gcnt;
 	pdc::DBuf<float> G32;

@@ -292,54 +292,102 @@ __global__ __launch_bounds__(256) void k_grp_init(int NL, const int* __restrict_
 
 // Group r's pick among the neighbour groups of size s: the largest n^2 / w
 // (n the contacts between the groups, w the neighbour's contacts), compared
-// exactly; ties go to the smaller key.  The contacts to a neighbour group are
-// summed over every adjacency entry of r's members that leads to it (each
-// entry of the scan sums them again, so repeated neighbours need no
-// bookkeeping: they compare equal and change nothing).
+// exactly; ties go to the smaller key.  One wavefront per body (the group's
+// key holder works, the others leave): the lanes take the adjacency entries
+// of the group's members, 64 at a time; each lane sums the contacts of every
+// entry that leads to its entry's neighbour group (the entries broadcast by
+// shuffles), keeps the better of its candidates, and the wavefront reduces
+// the lanes' picks under the same order (a total order, so every lane ends
+// with the same pick).
+__device__ __forceinline__ bool grp_better(int h1, long long n1, long long w1, int h2, long long n2, long long w2) {
+	if (h1 < 0) return false;
+	if (h2 < 0) return true;
+	const unsigned __int128 l = (unsigned __int128)(unsigned long long)(n1 * n1) * (unsigned long long)w2;
+	const unsigned __int128 r = (unsigned __int128)(unsigned long long)(n2 * n2) * (unsigned long long)w1;
+	return l > r || (l == r && h1 < h2);
+}
+
 __global__ __launch_bounds__(256) void k_grp_propose(int NL, int s, const int* __restrict__ lstat,
                                                    const int* __restrict__ aptr, const int* __restrict__ anb,
                                                    const int* __restrict__ apair, const int* __restrict__ pstart,
                                                    const int* __restrict__ rep, const int* __restrict__ gsize,
                                                    const long long* __restrict__ gw, const int* __restrict__ mem,
                                                    int* __restrict__ best) {
-	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	const int r = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+	const int lane = threadIdx.x & 63;
 	if (r >= NL) return;
 	int bh = -1;
-	long long bn = 0;
+	long long bn = 0, bw = 0;
 	if (rep[r] == r && !lstat[r] && gsize[r] == s) {
-		for (int k = 0; k < s; ++k) {
-			const int m = mem[kPcgRowsPerWG * r + k];
-			for (int q = aptr[m]; q < aptr[m + 1]; ++q) {
-				const int c = anb[q];
-				if (lstat[c]) continue;
-				const int h = rep[c];
-				if (h == r || gsize[h] != s) continue;
-				long long n = 0;
-				for (int k2 = 0; k2 < s; ++k2) {
-					const int m2 = mem[kPcgRowsPerWG * r + k2];
-					for (int q2 = aptr[m2]; q2 < aptr[m2 + 1]; ++q2) {
-						const int c2 = anb[q2];
-						if (!lstat[c2] && rep[c2] == h) {
-							const int p = apair[q2];
-							n += pstart[p + 1] - pstart[p];
-						}
-					}
+		// the members' adjacency ranges, concatenated
+		int beg[kPcgRowsPerWG / 2], len[kPcgRowsPerWG / 2];
+		int E = 0;
+#pragma unroll
+		for (int k = 0; k < kPcgRowsPerWG / 2; ++k) {
+			beg[k] = 0;
+			len[k] = 0;
+			if (k < s) {
+				const int m = mem[kPcgRowsPerWG * r + k];
+				beg[k] = aptr[m];
+				len[k] = aptr[m + 1] - beg[k];
+				E += len[k];
+			}
+		}
+		// entry e of the concatenation: its neighbour group of size s (-1: none) and contacts
+		auto entry = [&](int e, int& h, long long& n) {
+			h = -1;
+			n = 0;
+			if (e >= E) return;
+			int q = -1;
+#pragma unroll
+			for (int k = 0; k < kPcgRowsPerWG / 2; ++k) {
+				if (q < 0 && e < len[k]) q = beg[k] + e;
+				else if (q < 0) e -= len[k];
+			}
+			const int c = anb[q];
+			if (lstat[c]) return;
+			const int g = rep[c];
+			if (g == r || gsize[g] != s) return;
+			const int p = apair[q];
+			h = g;
+			n = pstart[p + 1] - pstart[p];
+		};
+		for (int a0 = 0; a0 < E; a0 += 64) {
+			int ha;
+			long long na;
+			entry(a0 + lane, ha, na);
+			long long tot = 0;
+			for (int b0 = 0; b0 < E; b0 += 64) {
+				int hb = ha;
+				long long nb = na;
+				if (b0 != a0) entry(b0 + lane, hb, nb);
+				for (int l = 0; l < 64; ++l) {
+					const int hl = __shfl(hb, l, 64);
+					const long long nl = __shfl(nb, l, 64);
+					if (ha >= 0 && hl == ha) tot += nl;
 				}
-				if (bh < 0) {
-					bh = h;
-					bn = n;
-					continue;
-				}
-				const unsigned __int128 lhs = (unsigned __int128)(unsigned long long)(n * n) * (unsigned long long)gw[bh];
-				const unsigned __int128 rhs = (unsigned __int128)(unsigned long long)(bn * bn) * (unsigned long long)gw[h];
-				if (lhs > rhs || (lhs == rhs && h < bh)) {
-					bh = h;
-					bn = n;
+			}
+			if (ha >= 0) {
+				const long long wa = gw[ha];
+				if (grp_better(ha, tot, wa, bh, bn, bw)) {
+					bh = ha;
+					bn = tot;
+					bw = wa;
 				}
 			}
 		}
+#pragma unroll
+		for (int off = 32; off > 0; off >>= 1) {
+			const int oh = __shfl_xor(bh, off, 64);
+			const long long on = __shfl_xor(bn, off, 64), ow = __shfl_xor(bw, off, 64);
+			if (grp_better(oh, on, ow, bh, bn, bw)) {
+				bh = oh;
+				bn = on;
+				bw = ow;
+			}
+		}
 	}
-	best[r] = bh;
+	if (lane == 0) best[r] = bh;
 }
 
 // Mutual picks merge into the group with the smaller key.
@@ -595,7 +643,8 @@ static void group_bodies(pdc_context* ctx) {
 	                   ctx->g_w.p, ctx->g_mem.p);
 	for (int level = 0, size = 1; level < kGroupLevels; ++level, size *= 2)
 		for (int round = 0; round < kGroupRounds; ++round) {
-			hipLaunchKernelGGL(k_grp_propose, dim3(grid), dim3(kBlock), 0, s, NL, size, ctx->l_static.p,
+			hipLaunchKernelGGL(k_grp_propose, dim3(grid_for(64 * (int64_t)NL)), dim3(kBlock), 0, s, NL, size,
+			                   ctx->l_static.p,
 			                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->adj_pair.p, ctx->ap_start.p, ctx->g_rep.p,
 			                   ctx->g_size.p, ctx->g_w.p, ctx->g_mem.p, ctx->g_best.p);
 			hipLaunchKernelGGL(k_grp_accept, dim3(grid), dim3(kBlock), 0, s, NL, size, ctx->g_best.p, ctx->g_rep.p,
@@ -655,7 +704,11 @@ void build_topology(pdc_context* ctx) {
 	                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->adj_pair.p, ctx->ap_start.p, ctx->l2g.p,
 	                   ctx->b_static.p, ctx->adj_pos.p, ctx->inc_cnt.p, ctx->a_cnt.p, ctx->l_static.p);
 	PDC_CHECK_LAUNCH();
-	if (standard_form(ctx)) group_bodies(ctx);
+	ctx->groups_valid = false;
+	if (standard_form(ctx)) {
+		group_bodies(ctx);
+		ctx->groups_valid = true;
+	}
 	scan_exclusive(ctx, ctx->inc_cnt.p, ctx->inc_ptr.p, NL);
 	scan_exclusive(ctx, ctx->a_cnt.p, ctx->a_ptr.p, NL);
 	ctx->inc.ensure(2 * (size_t)C);
@@ -1520,91 +1573,174 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ 
 }
 
 // ---- (d) the groups' inverse blocks -------------------------------------------
-// A double read from another lane of the wavefront (a compile-time lane after
-// unrolling), by two v_readlane_b32.
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-	const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-	const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
-	const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
-	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+// Inverse of a symmetric positive definite 6x6 block (row-major) through its
+// Cholesky factor; false when a pivot is not positive and finite.
+__device__ __forceinline__ bool spd6_inverse(const double* S, double (&P)[36]) {
+	double L[21], rd[6];
+	bool ok = true;
+	int q = 0;
+#pragma unroll
+	for (int j = 0; j < 6; ++j) {
+#pragma unroll
+		for (int i = j; i < 6; ++i) {
+			double t = S[i * 6 + j];
+#pragma unroll
+			for (int k = 0; k < j; ++k) t -= L[i * (i + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
+			if (i == j) {
+				ok = ok && t > 0.0 && isfinite(t);
+				rd[j] = 1.0 / sqrt(t > 0.0 ? t : 1.0);
+				L[j * (j + 1) / 2 + j] = 1.0 / rd[j];
+			} else {
+				L[i * (i + 1) / 2 + j] = t * rd[j];
+			}
+			++q;
+		}
+	}
+#pragma unroll
+	for (int c = 0; c < 6; ++c) {
+		double e[6];
+#pragma unroll
+		for (int i = 0; i < 6; ++i) {
+			double t = i == c ? 1.0 : 0.0;
+#pragma unroll
+			for (int k = 0; k < i; ++k) t -= L[i * (i + 1) / 2 + k] * e[k];
+			e[i] = t * rd[i];
+		}
+#pragma unroll
+		for (int i = 5; i >= 0; --i) {
+			double t = e[i];
+#pragma unroll
+			for (int k = i + 1; k < 6; ++k) t -= L[k * (k + 1) / 2 + i] * e[k];
+			e[i] = t * rd[i];
+		}
+#pragma unroll
+		for (int i = 0; i < 6; ++i) P[i * 6 + c] = e[i];
+	}
+	return ok;
 }
 
 // Inverse blocks of the body groups (standard PCG form) from the assembled,
 // scaled operator: one workgroup per PCG tile, one wavefront per group (the
-// wavefront of the group's first slot).  Lane i < 6 g holds row i of the
-// group's 6 g x 6 g block (the diagonal blocks and the blocks between the
-// group's bodies, found in the ELL rows), lanes past it rows of the identity;
-// the block, a principal submatrix of the symmetric positive definite Newton
-// matrix, is inverted in place by Gauss-Jordan elimination without pivoting,
-// each pivot row read from its lane.  A block with a pivot that is not
-// positive and finite gets the inverse of its diagonal instead.  Stored per
-// slot: its six rows against the tile's eight slots, entry (i, 6 m + c) at
-// G[288 slot + 48 i + 6 m + c], written and read for the group's slots only.
+// wavefront of the group's first slot), lane g i + k holding block (i, k) of
+// the group's g x g matrix of 6 x 6 blocks (the diagonal blocks and the
+// blocks between the group's bodies, found in the ELL rows).  The matrix, a
+// principal submatrix of the symmetric positive definite Newton matrix, is
+// inverted in place by block Gauss-Jordan elimination without pivoting:
+// step j publishes block row and column j through LDS, every lane inverts
+// the pivot block D = G_jj (Cholesky) and updates its block: G_jj <- D^-1,
+// G_jk <- D^-1 G_jk, G_ij <- -G_ij D^-1, G_ik <- G_ik - G_ij D^-1 G_jk.  A
+// matrix with a pivot block that is not positive definite gets the inverse
+// of its diagonal instead.  Stored per slot: its six rows against the tile's
+// eight slots, entry (i, 6 m + c) at G[288 slot + 48 i + 6 m + c], written
+// and read for the group's slots only.
 constexpr int kGroupRow = 6 * kPcgRowsPerWG;             // 48 columns of a tile row
 constexpr int kGroupBlock = 6 * kGroupRow;               // 288 entries per slot
 static_assert((1 << kGroupLevels) == kPcgRowsPerWG, "the largest group fills one PCG tile");
+static_assert(kPcgRowsPerWG * kPcgRowsPerWG <= 64, "one lane per block of the largest group");
 template <class T>
 __global__ __launch_bounds__(64 * kPcgRowsPerWG) void k_grp_inv(const PcgTile* __restrict__ tiles,
                                                         const SceneSc* __restrict__ ss, const int* __restrict__ ecol,
                                                         int R, const T* __restrict__ Aval,
                                                         const int* __restrict__ gsz, T* __restrict__ G) {
+	__shared__ double pub[kPcgRowsPerWG][2 * kPcgRowsPerWG][36];  // per wavefront: column j, then row j
 	const PcgTile* Tp = tiles + blockIdx.x;
 	const int scn = Tp->scene, cnt = Tp->count;
 	if (scn < 0 || !ss[scn].run) return;
 	const int wid = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
 	if (wid >= cnt) return;
 	const int base = kPcgRowsPerWG * (int)blockIdx.x;
-	const int gs = 1 << gsz[base + wid];
+	const int lg = gsz[base + wid];
+	const int gs = 1 << lg;
 	if (wid & (gs - 1)) return;  // not the group's first slot
-	const int n = 6 * gs;
-	const int i = lane < n ? lane : 0;
-	const int sr = base + wid + i / 6, ci = i % 6;
-	double a[kGroupRow];
+	const bool act = lane < gs * gs;
+	const int bi = act ? lane >> lg : 0, bk = act ? lane & (gs - 1) : 0;
+	const int sr = base + wid + bi;  // the block's row slot
+	// block (bi, bk): the ELL entry of row sr whose column is slot base + wid + bk
+	double M[36];
 #pragma unroll
-	for (int k = 0; k < kGroupRow; ++k) a[k] = (lane >= n && k == lane) ? 1.0 : 0.0;
-	if (lane < n)
+	for (int q = 0; q < 36; ++q) M[q] = 0.0;
+	if (act)
 		for (int e = 0; e < R; ++e) {
 			const size_t q = (size_t)sr * R + e;
-			const int c = ecol[q];
-			const int cb = c - (base + wid);
-			if (c < 0 || cb < 0 || cb >= gs) continue;
-			const T* blk = Aval + 36 * q + 6 * ci;
-			double v[6];
+			if (ecol[q] != base + wid + bk) continue;
+			const T* blk = Aval + 36 * q;
 #pragma unroll
-			for (int cc = 0; cc < 6; ++cc) v[cc] = (double)blk[cc];
-#pragma unroll
-			for (int m = 0; m < kPcgRowsPerWG; ++m)
-				if (m == cb)
-#pragma unroll
-					for (int cc = 0; cc < 6; ++cc) a[6 * m + cc] = v[cc];
+			for (int t = 0; t < 36; ++t) M[t] = (double)blk[t];
+			break;
 		}
-	double d0 = 1.0;
+	double d0[6];
 #pragma unroll
-	for (int k = 0; k < kGroupRow; ++k)
-		if (k == i) d0 = a[k];
+	for (int t = 0; t < 6; ++t) d0[t] = M[7 * t];
+	double (*pc)[36] = pub[wid];
+	double (*pr)[36] = pub[wid] + kPcgRowsPerWG;
 	bool ok = true;
-	// pivot step j (uniform): the columns unrolled, j selected among them, so
-	// that the row stays in registers
-	for (int j = 0; j < n; ++j) {
-		double f = 0;
+	for (int j = 0; j < gs; ++j) {
+		if (act && bk == j)
 #pragma unroll
-		for (int k = 0; k < kGroupRow; ++k) f = k == j ? a[k] : f;
-		const double pj = readlane_f64(f, j);
-		ok = ok && pj > 0.0 && isfinite(pj);
-		const double inv = 1.0 / pj;
-		const bool me = lane == j;
-		const double aj = me ? inv : -f * inv;
+			for (int t = 0; t < 36; ++t) pc[bi][t] = M[t];
+		if (act && bi == j)
 #pragma unroll
-		for (int k = 0; k < kGroupRow; ++k) {
-			const double pk = readlane_f64(a[k], j) * inv;
-			a[k] = k == j ? aj : (me ? pk : a[k] - f * pk);
+			for (int t = 0; t < 36; ++t) pr[bk][t] = M[t];
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		double Dinv[36];
+		ok = spd6_inverse(pc[j], Dinv) && ok;
+		if (act) {
+			const double* gjk = pr[bk];  // G_jk as published (the lane's own block when bi == j)
+			const double* gij = pc[bi];
+			if (bi == j && bk == j) {
+#pragma unroll
+				for (int t = 0; t < 36; ++t) M[t] = Dinv[t];
+			} else if (bi == j) {
+				// D^-1 G_jk, row by row
+#pragma unroll
+				for (int r = 0; r < 6; ++r)
+#pragma unroll
+					for (int c = 0; c < 6; ++c) {
+						double t = 0;
+#pragma unroll
+						for (int k = 0; k < 6; ++k) t += Dinv[r * 6 + k] * gjk[k * 6 + c];
+						M[r * 6 + c] = t;
+					}
+			} else {
+				// row r of X = G_ij D^-1, then G_ik - X G_jk (or -X when k = j)
+#pragma unroll
+				for (int r = 0; r < 6; ++r) {
+					double x[6];
+#pragma unroll
+					for (int c = 0; c < 6; ++c) {
+						double t = 0;
+#pragma unroll
+						for (int k = 0; k < 6; ++k) t += gij[r * 6 + k] * Dinv[k * 6 + c];
+						x[c] = t;
+					}
+#pragma unroll
+					for (int c = 0; c < 6; ++c) {
+						if (bk == j) {
+							M[r * 6 + c] = -x[c];
+						} else {
+							double t = 0;
+#pragma unroll
+							for (int k = 0; k < 6; ++k) t += x[k] * gjk[k * 6 + c];
+							M[r * 6 + c] -= t;
+						}
+					}
+				}
+			}
 		}
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 	}
-	if (lane < n) {
-		T* row = G + kGroupBlock * (size_t)sr + kGroupRow * ci + 6 * wid;
+	if (act) {
 #pragma unroll
-		for (int k = 0; k < kGroupRow; ++k)
-			if (k < n) row[k] = (T)(ok ? a[k] : (k == i ? 1.0 / d0 : 0.0));
+		for (int r = 0; r < 6; ++r) {
+			T* row = G + kGroupBlock * (size_t)sr + kGroupRow * r + 6 * (wid + bk);
+#pragma unroll
+			for (int c = 0; c < 6; ++c)
+				row[c] = (T)(ok ? M[r * 6 + c] : (bi == bk && r == c ? 1.0 / d0[r] : 0.0));
+		}
 	}
 }
 

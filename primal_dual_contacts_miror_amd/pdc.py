@@ -39,7 +39,7 @@ EXPORTS = [
     "pdc_set_scenes", "pdc_scene_count", "pdc_set_scene_friction", "pdc_step_scenes",
     "pdc_get_shapes", "pdc_set_shapes", "pdc_autotune",
     "pdc_set_state_range", "pdc_get_state_range", "pdc_append_bodies", "pdc_replace_bodies",
-    "pdc_device_bytes", "pdc_reserve_bodies",
+    "pdc_device_bytes", "pdc_reserve_bodies", "pdc_get_groups",
 ]
 
 
@@ -85,6 +85,7 @@ def load_library(path=LIB_PATH):
     L.pdc_append_bodies.argtypes = [V, ctypes.c_int32, _I, _D, _D, _D, _I, _I, _D, _D, _D, _D, _I]
     L.pdc_replace_bodies.argtypes = [V, ctypes.c_int32, ctypes.c_int32, _I, _D, _D, _D, _I, _I, _D, _D, _D, _D]
     L.pdc_device_bytes.argtypes = [V, ctypes.POINTER(ctypes.c_int64)]
+    L.pdc_get_groups.argtypes = [V, ctypes.c_int32, _I]
     L.pdc_reserve_bodies.argtypes = [V, ctypes.c_int64, ctypes.c_int64]
     L.pdc_get_state.argtypes = [V, _D, _D, _D, _D]
     L.pdc_detect_contacts.argtypes = [V, _I, _I]
@@ -227,6 +228,13 @@ class Context:
         b = ctypes.c_int64(0)
         self._check(self._L.pdc_device_bytes(self.h, ctypes.byref(b)))
         return b.value
+
+    def groups(self):
+        """Per body, the global id of its preconditioner group's key body in
+        the last grouped solve (-1: not in its contacts)."""
+        out = np.zeros(self.n, np.int32)
+        self._check(self._L.pdc_get_groups(self.h, out.shape[0], _i(out)))
+        return out
 
     def detect_contacts(self):
         c, t = ctypes.c_int32(0), ctypes.c_int32(0)

@@ -4,19 +4,28 @@
            log-uniform), stepped together 200 steps from the seeded drop at
            the tight inner solve (FP64 operator, 1e-12): no non-finite value,
            contacts in the contact-rich tail; 8 sampled scenes bit-identical
-           to themselves stepped alone; 3 of them one step against the FP64
-           oracle (its branch for the scene, direct below 2000 bodies as in
-           the reference): positions within 1e-6 of the body size and
-           rotation angles within 1e-6 rad; linear and angular velocities
-           within 1e-6 |g| dt when the oracle's Newton loop converges, and
-           otherwise (the loop stalls at its iteration cap, where the result
-           depends on the rounding of every inner solve) within 4x the
-           difference between the oracle's own direct and PCG branches.
+           to themselves stepped alone, and each one step against the FP64
+           oracle: positions within 1e-6 of the body size and rotation angles
+           within 1e-6 rad of its reference branch (direct below 2000 bodies,
+           as in the reference); linear and angular velocities, against its
+           PCG branch (the device's linear solver: the same groups, the same
+           tolerance), within the larger of 1e-6 |g| dt and 4x what a tenfold
+           change of the inner tolerance, either way, moves the oracle's own
+           result.  Most of these solves end at the Newton cap, or after
+           gradient-descent fallbacks whose line searches amplify the inner
+           solves' differences about a millionfold: their end point is
+           defined no better than that, and the bar measures it per scene
+           instead of assuming it (the factor 4: the device's results differ
+           from the oracle's by rounding in every sum, of which the one
+           perturbed tolerance is a single direction).  (At the cap the end point also depends on
+           the linear solver itself: the oracle's direct and PCG branches can
+           differ there by 1e-4 |g| dt; that distance is reported.)
   served   a mix of 48 random piles of 10-1000 bodies plus a 1000-body pile
            behind the server (two shards), stepped into contact through it,
-           then a Poisson stream during which 4 scenes leave and 4 others
-           arrive: sampled scenes bit-identical alone, 2 one step against the
-           oracle with the bars above.
+           200 rounds, then a Poisson stream during which 4 scenes leave and
+           4 others arrive (then stepped 150 times into contact): 4 sampled
+           scenes, all in contact, bit-identical alone and one step against
+           the oracle with the bars above.
 """
 import numpy as np
 import pytest
@@ -43,8 +52,8 @@ def _same(a, b):
     return all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
-def _oracle_step(sc, state, solver=orc.SOLVER_REFERENCE):
-    o = orc.OracleScene(sc, sc.params)
+def _oracle_step(sc, state, solver=orc.SOLVER_REFERENCE, params=None):
+    o = orc.OracleScene(sc, sc.params if params is None else params)
     o.set_threads(16)
     o.set_state(*state)
     o.set_linear_solver(solver)
@@ -52,22 +61,28 @@ def _oracle_step(sc, state, solver=orc.SOLVER_REFERENCE):
     return o.get_state(), st
 
 
-def check_one_step(sc, start, device_state, what):
+def check_one_step(sc, start, device_state, device_newton, what):
     """The device's step from `start` against the oracle's, with the bars of
     the module docstring."""
     ref, so = _oracle_step(sc, start)
     e = pile_step_errors(sc, device_state, ref)
-    msg = "%s (%d bodies, %d contacts, oracle Newton %d, exit %.2e): errors (pos, angle, vel, angvel) %s" % (
-        what, sc.n, so.n_contacts, so.newton_iterations, so.exit_error, e)
+    # the oracle's PCG branch (the device's linear solver) at the inner
+    # tolerance t, and at 10 t and t / 10: what the inner accuracy moves
+    variants = []
+    for f in (1.0, 10.0, 0.1):
+        q = type(sc.params).from_buffer_copy(sc.params)
+        q.pcg_tol = f * sc.params.pcg_tol
+        variants.append(_oracle_step(sc, start, orc.SOLVER_PCG, q)[0])
+    ep = pile_step_errors(sc, device_state, variants[0])
+    sens = np.maximum(pile_step_errors(sc, variants[0], variants[1]), pile_step_errors(sc, variants[0], variants[2]))
+    bar = np.maximum(1e-6, 4 * sens[2:])
+    msg = ("%s (%d bodies, %d contacts, Newton %d, oracle %d, exit %.2e, gd %d): errors (pos, angle, vel, angvel) "
+           "against the reference branch %s, the PCG branch %s; velocity bars %s" % (
+               what, sc.n, so.n_contacts, device_newton, so.newton_iterations, so.exit_error, so.gd_fallbacks, e, ep,
+               bar))
+    print(msg)
     assert np.all(e[:2] < 1e-6), msg
-    if so.exit_error < sc.params.tolerance:
-        print(msg)
-        assert np.all(e[2:] < 1e-6), msg
-    else:
-        alt, _ = _oracle_step(sc, start, orc.SOLVER_PCG)
-        own = pile_step_errors(sc, alt, ref)
-        print(msg + "; stalled, oracle direct vs PCG %s" % own)
-        assert np.all(e[2:] <= np.maximum(1e-6, 4 * own[2:])), msg
+    assert np.all(ep[2:] <= bar), msg
 
 
 def test_batched_256_scenes(require_gpu):
@@ -85,10 +100,10 @@ def test_batched_256_scenes(require_gpu):
     picks = [int(order[i]) for i in np.linspace(0, len(order) - 1, 8).round().astype(int)]
     for k in picks:
         assert _same(bs.state(k), _alone(scs[k], 200)), "scene %d (%d bodies) differs alone" % (k, scs[k].n)
-    for k in picks[-1], picks[4], picks[1]:
+    for k in picks:
         start = tuple(x.copy() for x in bs.state(k))
-        bs.step(1, [k])
-        check_one_step(scs[k], start, bs.state(k), "batched scene %d" % k)
+        st = bs.step(1, [k])
+        check_one_step(scs[k], start, bs.state(k), st[0].newton_iterations, "batched scene %d" % k)
     bs.ctx.close()
 
 
@@ -100,7 +115,7 @@ def test_served_mix_with_churn(require_gpu):
     hs = srv.admit_all(scs)
     assert all(h.state == "resident" for h in hs)
     print("served mix: %d scenes, %d bodies (largest %d)" % (len(hs), sum(s.n for s in scs), max(s.n for s in scs)))
-    for _ in range(60):
+    for _ in range(200):
         srv.drain([srv.submit(h) for h in hs])
     rng = np.random.default_rng(11)
     reqs = [serve.Request(hs[int(rng.integers(len(hs)))], 0.004 * k) for k in range(400)]
@@ -112,6 +127,8 @@ def test_served_mix_with_churn(require_gpu):
     reqs2 = [r for r in reqs[200:] if r.scene not in leaving]
     reqs2 += [serve.Request(h, 0.004 * k) for k, h in enumerate(new * 10)]
     second = srv.run(reqs2)
+    for _ in range(150):  # the newcomers into contact
+        srv.drain([srv.submit(h) for h in new])
     assert len(first) == 200 and len(second) == len(reqs2)
     assert all(h.state == "left" for h in leaving) and all(h.state == "resident" for h in new)
     rounds = {}
@@ -122,8 +139,17 @@ def test_served_mix_with_churn(require_gpu):
     checks = [big, hs[int(np.argmax([h.steps for h in hs[:-1]]))], new[0], new[3]]
     for h in checks:
         assert _same(srv.state(h), _alone(h.scene, h.steps)), "%s after %d steps" % (h.name, h.steps)
-    for h in big, checks[1]:
+    contacts = []
+    for h in checks:
         start = tuple(x.copy() for x in srv.state(h))
         srv.drain([srv.submit(h)])
-        check_one_step(h.scene, start, srv.state(h), "served %s" % h.name)
+        # the step alone from the same state: the same bits, and its Newton count
+        ds = pdc.DynamicSystem(h.scene, h.scene.params)
+        ds.set_state(*start)
+        st = ds.step(1)
+        assert _same(ds.state(), srv.state(h)), "served %s: one step differs alone" % h.name
+        ds.ctx.close()
+        contacts.append(st.n_contacts)
+        check_one_step(h.scene, start, srv.state(h), st.newton_iterations, "served %s" % h.name)
+    assert all(c > 0 for c in contacts) and max(contacts) > 1000, contacts
     srv.close()
