@@ -3360,16 +3360,21 @@ static void record_phase(pdc_context* ctx, PcgGraph& G, F&& fn) {
 	PDC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
 	fn();
 	PDC_HIP(hipStreamEndCapture(s, &g));
+	// The graph an executable was instantiated from stays alive as long as
+	// the executable: an update takes the new capture's parameters and the
+	// capture is then discarded; only a new instantiation replaces the graph.
 	if (G.exec) {
 		hipGraphExecUpdateResult res = hipGraphExecUpdateError;
 		hipGraphNode_t bad = nullptr;
-		if (hipGraphExecUpdate(G.exec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
-			(void)hipGetLastError();
-			(void)hipGraphExecDestroy(G.exec);
-			G.exec = nullptr;
+		if (hipGraphExecUpdate(G.exec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+			(void)hipGraphDestroy(g);
+			return;
 		}
+		(void)hipGetLastError();
+		(void)hipGraphExecDestroy(G.exec);
+		G.exec = nullptr;
 	}
-	if (!G.exec) PDC_HIP(hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0));
+	PDC_HIP(hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0));
 	if (G.graph) (void)hipGraphDestroy(G.graph);
 	G.graph = g;
 }
