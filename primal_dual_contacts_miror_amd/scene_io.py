@@ -622,11 +622,12 @@ def _aabb_max_vertex_speed(loaded, pos, quat, vel, ang):
     return float(np.linalg.norm(allv - avg, axis=2).max())
 
 
-def bake(loaded, out_path, device=0, max_frames=None, log=None):
+def bake(loaded, out_path, device=0, max_frames=None, log=None, substeps_out=None):
     """Runs the scene until its baking time (SimulationSceneCLI::advanceOneFrame:
     one timestep per frame, or the adaptive substeps of DynamicSystem::step,
     then the states are saved) and writes the glTF, NPZ and .info outputs.
-    Returns the saved times."""
+    Returns the saved times; `substeps_out` (a list), if given, receives the
+    sizes of the substeps of every frame, one list per frame."""
     from . import pdc
     sc = loaded.scene
     p = loaded.params
@@ -641,6 +642,7 @@ def bake(loaded, out_path, device=0, max_frames=None, log=None):
     t0 = time.perf_counter()
     frames = 0
     while True:
+        sizes = []
         if loaded.adaptive_timestep:
             cur = 0.0
             while cur < 1.0:
@@ -649,6 +651,7 @@ def bake(loaded, out_path, device=0, max_frames=None, log=None):
                 rec = min_len / (2.0 * (vmax * dt + 1e-8) * loaded.substep_factor)
                 rec = min(1.0 - cur, rec)
                 p.timestep = rec * dt
+                sizes.append(p.timestep)
                 ds.ctx.configure(p)
                 st = ds.step(1)
                 substeps += 1
@@ -660,10 +663,13 @@ def bake(loaded, out_path, device=0, max_frames=None, log=None):
             ds.ctx.configure(p)
         else:
             st = ds.step(1)
+            sizes.append(dt)
             substeps += 1
             contacts += st.n_contacts
             contact_substeps += 1 if st.n_contacts > 0 else 0
             t += dt
+        if substeps_out is not None:
+            substeps_out.append(sizes)
         steps += 1
         pos, quat, _, _ = ds.state()
         times.append(t)

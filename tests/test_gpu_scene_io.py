@@ -19,7 +19,7 @@ YAML = """scene:
 simulation:
   timestep: 1e-2
   adaptive_timestep: {adaptive}
-  substep_factor: 2.5
+  substep_factor: {factor}
   solver:
     name: "primal_dual"
     iterations: 100
@@ -34,7 +34,7 @@ baking:
 """
 
 
-def _scene(tmp_path, adaptive=False, time=0.395):
+def _scene(tmp_path, adaptive=False, time=0.395, factor=2.5):
     cube, ico = S.unit_cube_mesh(), S.icosphere_mesh(3)
 
     def node(scale, t):
@@ -47,7 +47,8 @@ def _scene(tmp_path, adaptive=False, time=0.395):
                                                    for k in range(3)]
     S.write_gltf(str(tmp_path / "boxes.gltf"), [cube] * 4, boxes)
     S.write_gltf(str(tmp_path / "balls.gltf"), [ico], [node((0.4, 0.4, 0.4), (0.1, 3.6, 0))])
-    (tmp_path / "scene.yaml").write_text(YAML.format(adaptive="true" if adaptive else "false", time=time))
+    (tmp_path / "scene.yaml").write_text(YAML.format(adaptive="true" if adaptive else "false", time=time,
+                                                     factor=factor))
     return str(tmp_path / "scene.yaml")
 
 
@@ -89,11 +90,47 @@ def test_bake_matches_oracle_and_writes_outputs(require_gpu, tmp_path):
 
 
 def test_bake_adaptive_substeps(require_gpu, tmp_path):
-    L = S.load_scene(_scene(tmp_path, adaptive=True, time=0.045))
-    times = S.bake(L, str(tmp_path / "ad.gltf"))
-    assert len(times) == 5
-    # the substep sizes of a frame add up to one timestep (DynamicSystem.cpp:165-199)
+    """The adaptive substeps of DynamicSystem::step (DynamicSystem.cpp:165-206)
+    over the ball's fall onto the stack: the device's substep sizes add up to
+    one timestep per frame; the oracle, stepping the same substep sizes, gives
+    every saved frame within 1e-6 of the body size and 1e-6 rad; and the rule
+    applied to the oracle's own states picks the same sizes (to 1e-9)."""
+    # substep_factor 40: the ball's fall at ~3 m/s needs several substeps of the 1e-2 timestep
+    L = S.load_scene(_scene(tmp_path, adaptive=True, time=0.395, factor=40), pcg_tol=1e-13)
+    sc = L.scene
+    sizes = []
+    times = S.bake(L, str(tmp_path / "ad.gltf"), substeps_out=sizes)
+    assert len(times) == 40 and len(sizes) == 40
     np.testing.assert_allclose(np.diff(times), 0.01, rtol=1e-9)
+    for f in sizes:
+        assert sum(f) == pytest.approx(0.01, rel=1e-12)
+    n_sub = [len(f) for f in sizes]
+    print("substeps per frame:", n_sub)
+    assert max(n_sub) > 1, "the fall should need substeps"
     info = (tmp_path / "ad.info").read_text()
-    sub = float(info.split("Average substeps per step:")[1].split()[0])
-    assert sub >= 1.0
+    assert float(info.split("Average substeps per step:")[1].split()[0]) == pytest.approx(np.mean(n_sub), rel=1e-3)
+    z = np.load(tmp_path / "ad.npz")
+    o = orc.OracleScene(sc, L.params)
+    o.set_linear_solver(orc.SOLVER_DIRECT)
+    q = type(L.params).from_buffer_copy(L.params)
+    dyn = np.nonzero(sc.is_static == 0)[0]
+    min_len = min(b.min_length for b in L.bodies)
+    for frame, f in enumerate(sizes):
+        cur = 0.0
+        for h in f:
+            # the rule on the oracle's state (DynamicSystem.cpp:168-193)
+            pos, quat, vel, ang = o.get_state()
+            vmax = S._aabb_max_vertex_speed(L, pos, quat, vel, ang)
+            rec = min(1.0 - cur, min_len / (2.0 * (vmax * 0.01 + 1e-8) * L.substep_factor))
+            assert rec * 0.01 == pytest.approx(h, rel=1e-9), (frame, cur)
+            q.timestep = h
+            o.set_params(q)
+            o.step(1)
+            cur += h / 0.01
+        po, qo = o.get_state()[:2]
+        for k in dyn:
+            g = z["mesh_%d" % k][frame + 1]
+            size = float(np.max(sc.dims[k]))
+            assert np.linalg.norm(g[:3] - po[k]) < 1e-6 * max(size, 1.0), (frame, k)
+            ang_err = 2 * np.arccos(min(1.0, abs(float(np.dot(g[3:], qo[k])))))
+            assert ang_err < 1e-6, (frame, k)
