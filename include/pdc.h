@@ -40,7 +40,8 @@ enum {
 };
 
 /* ---- enums --------------------------------------------------------------- */
-enum { PDC_SHAPE_BOX = 0, PDC_SHAPE_SPHERE = 1 };
+/* Primitive shapes (src/CollisionObject/Cube.cpp, Sphere.cpp, Ellipsoid.cpp). */
+enum { PDC_SHAPE_BOX = 0, PDC_SHAPE_SPHERE = 1, PDC_SHAPE_ELLIPSOID = 2 };
 
 /* non_smooth_force.name: "non_smooth_contact" | "only_normal"
  * (PrimalDualForceSolver.cpp:664-673) */
@@ -139,7 +140,8 @@ const char* pdc_last_error(const pdc_context* ctx);
 
 /* ---- bodies (RigidObjectBase inputs: RigidObjectBase.cpp:45-119) ---------- */
 /* Uploads the body table: shape type and dimensions (box half extents, or
- * sphere radius in dims[3i]), mass, principal moments of inertia in the body
+ * sphere radius in dims[3i], or ellipsoid semi-axes), mass, principal
+ * moments of inertia in the body
  * frame, static flags, and whether the body's surface samples are tested
  * against other bodies.  Builds the FP32 SDF grid and the surface samples of
  * every body on the device. */

@@ -76,6 +76,9 @@ int32_t orc_sdf_query(const orc_scene* s, int32_t b, const double q[3],
 float orc_sdf_node(const orc_scene* s, int32_t b, int32_t i, int32_t j,
                    int32_t k);
 int32_t orc_samples(const orc_scene* s, int32_t b, int32_t cap, double* out);
+/* Analytic signed distance of the ellipsoid with semi-axes dims at n points
+ * (xyz rows). */
+void orc_ellipsoid_sdf(const double dims[3], int32_t n, const double* pts, double* out);
 
 /* Assembled Newton system of the first iteration of fillForces on the given
  * contacts/state (before Jacobi scaling): dense 6N x 6N matrix (row-major)

@@ -62,6 +62,8 @@ def lib():
                                    ctypes.c_int32]
         L.orc_samples.restype = ctypes.c_int32
         L.orc_samples.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _D]
+        L.orc_ellipsoid_sdf.restype = None
+        L.orc_ellipsoid_sdf.argtypes = [_D, ctypes.c_int32, _D, _D]
         L.orc_first_system.restype = ctypes.c_int32
         L.orc_first_system.argtypes = [ctypes.c_void_p, ctypes.c_int32, _I, _D, _D, _D, _D, _D,
                                        ctypes.c_int32, _D, _D, _I]
@@ -243,6 +245,15 @@ def normal_to_transform(pos, n, x0, x1):
     T = np.zeros(36)
     lib().orc_normal_to_transform(_d(_f64(pos)), _d(_f64(n)), _d(_f64(x0)), _d(_f64(x1)), _d(T))
     return T.reshape(2, 3, 6)
+
+
+def ellipsoid_sdf(dims, pts):
+    """Analytic signed distance to the ellipsoid with semi-axes `dims` at the
+    rows of `pts` (n x 3)."""
+    P = _f64(pts, (-1, 3))
+    out = np.zeros(len(P))
+    lib().orc_ellipsoid_sdf(_d(_f64(dims)), len(P), _d(P), _d(out))
+    return out
 
 
 def block_cholesky_solve(A, b):

@@ -237,10 +237,100 @@ struct Body {
 	std::vector<double> samples; /* 3 per sample, body frame */
 };
 
+/* Distance from y >= 0 to the boundary of the m-dimensional ellipsoid with
+ * semi-axes a[0] >= ... >= a[m-1] > 0 (Eberly, "Distance from a Point to an
+ * Ellipse, an Ellipsoid, or a Hyperellipsoid", 2013), by recursion on the
+ * dimension.  Last component positive: the closest point is
+ * x_i = r_i y_i / (s + r_i), r_i = (a_i / a_last)^2, s the root of
+ * sum_i (r_i z_i / (s + r_i))^2 = 1 (z_i = y_i / a_i) found by bisection of
+ * floating-point midpoints on [z_last - 1, |r z| - 1] (0 as the top inside);
+ * axes whose component is zero have x_i = 0 and drop out.  Last component
+ * zero: the interior closed form x_i = a_i^2 y_i / (a_i^2 - a_last^2) when it
+ * lies inside the ellipsoid, else the problem without the last axis. */
+static double ellipsoid_distance_nd(int m, const double* a, const double* y) {
+	if (m == 1) return std::fabs(y[0] - a[0]);
+	const int L = m - 1;
+	if (y[L] > 0.0) {
+		double ra[3], ry[3];
+		int k = 0;
+		for (int i = 0; i < L; ++i)
+			if (y[i] > 0.0) { ra[k] = a[i]; ry[k] = y[i]; ++k; }
+		if (k < L) {
+			ra[k] = a[L]; ry[k] = y[L];
+			return ellipsoid_distance_nd(k + 1, ra, ry);
+		}
+		double z[3] = {0, 0, 0}, r[3] = {0, 0, 0}, num[3] = {0, 0, 0};
+		double g = 0.0;
+		for (int i = 0; i < m; ++i) { z[i] = y[i] / a[i]; g += z[i] * z[i]; }
+		g -= 1.0;
+		if (g == 0.0) return 0.0;
+		for (int i = 0; i < L; ++i) { const double q = a[i] / a[L]; r[i] = q * q; num[i] = r[i] * z[i]; }
+		double lo = z[L] - 1.0, hi = 0.0;
+		if (!(g < 0.0)) {
+			double nn = 0.0;
+			for (int i = 0; i < L; ++i) nn += num[i] * num[i];
+			nn += z[L] * z[L];
+			hi = std::sqrt(nn) - 1.0;
+		}
+		double s = lo;
+		for (int it = 0; it < 1100; ++it) {
+			s = 0.5 * (lo + hi);
+			if (s == lo || s == hi) break;
+			double f = 0.0;
+			for (int i = 0; i < L; ++i) { const double t = num[i] / (s + r[i]); f += t * t; }
+			const double tl = z[L] / (s + 1.0);
+			f += tl * tl;
+			f -= 1.0;
+			if (f > 0.0) lo = s;
+			else if (f < 0.0) hi = s;
+			else break;
+		}
+		double d2 = 0.0;
+		for (int i = 0; i < L; ++i) { const double dx = r[i] * y[i] / (s + r[i]) - y[i]; d2 += dx * dx; }
+		const double dl = y[L] / (s + 1.0) - y[L];
+		d2 += dl * dl;
+		return std::sqrt(d2);
+	}
+	bool interior = true;
+	double e[3];
+	for (int i = 0; i < L; ++i) {
+		const double num = a[i] * y[i], den = a[i] * a[i] - a[L] * a[L];
+		if (!(num < den)) { interior = false; break; }
+		e[i] = num / den;
+	}
+	if (interior) {
+		double disc = 1.0;
+		for (int i = 0; i < L; ++i) disc -= e[i] * e[i];
+		if (disc > 0.0) {
+			double d2 = 0.0;
+			for (int i = 0; i < L; ++i) { const double dx = a[i] * e[i] - y[i]; d2 += dx * dx; }
+			const double xl = a[L] * std::sqrt(disc);
+			d2 += xl * xl;
+			return std::sqrt(d2);
+		}
+	}
+	return ellipsoid_distance_nd(L, a, y);
+}
+
+/* Signed distance to the ellipsoid with semi-axes dims: axes ordered by
+ * decreasing length (equal lengths in index order), negative inside. */
+double ellipsoid_sdf(const double* dims, double px, double py, double pz) {
+	const double p[3] = {std::fabs(px), std::fabs(py), std::fabs(pz)};
+	int idx[3] = {0, 1, 2};
+	std::stable_sort(idx, idx + 3, [&](int u, int v) { return dims[u] > dims[v]; });
+	double a[3], y[3];
+	for (int i = 0; i < 3; ++i) { a[i] = dims[idx[i]]; y[i] = p[idx[i]]; }
+	const double d = ellipsoid_distance_nd(3, a, y);
+	double q = 0.0;
+	for (int i = 0; i < 3; ++i) { const double z = y[i] / a[i]; q += z * z; }
+	return q < 1.0 ? -d : d;
+}
+
 double analytic_sdf(const Body& b, double px, double py, double pz) {
 	if (b.type == PDC_SHAPE_SPHERE) {
 		return std::sqrt(px * px + py * py + pz * pz) - b.dims[0];
 	}
+	if (b.type == PDC_SHAPE_ELLIPSOID) return ellipsoid_sdf(b.dims, px, py, pz);
 	const double qx = std::fabs(px) - b.dims[0];
 	const double qy = std::fabs(py) - b.dims[1];
 	const double qz = std::fabs(pz) - b.dims[2];
@@ -311,22 +401,28 @@ bool grid_query(const Body& b, const double* q, double* sd, double* grad) {
 
 /* Surface samples.  Box: on each face (axis 0..2, side -,+) a cell-centred
  * nu x nw grid, n = max(1, round(edge / spacing)).  Sphere: Fibonacci
- * lattice with max(8, round(4 pi r^2 / spacing^2)) points. */
+ * lattice with max(8, round(4 pi r^2 / spacing^2)) points.  Ellipsoid: that
+ * lattice scaled by the semi-axes. */
 void setup_samples(Body& b, double spacing) {
 	b.samples.clear();
-	if (b.type == PDC_SHAPE_SPHERE) {
-		const double r = b.dims[0];
+	if (b.type == PDC_SHAPE_SPHERE || b.type == PDC_SHAPE_ELLIPSOID) {
+		/* the ellipsoid's lattice is the unit one scaled by the semi-axes;
+		 * its count uses the area 4 pi (ab + ac + bc) / 3 (the p = 1 case of
+		 * Thomsen's approximation, exact for a sphere) */
+		const bool ell = b.type == PDC_SHAPE_ELLIPSOID;
+		const double ax = b.dims[0], ay = ell ? b.dims[1] : b.dims[0], az = ell ? b.dims[2] : b.dims[0];
 		const double PI = 3.14159265358979323846;
-		long n = std::lround(4.0 * PI * r * r / (spacing * spacing));
+		const double mean_sq = ell ? (ax * ay + ax * az + ay * az) / 3.0 : ax * ax;
+		long n = std::lround(4.0 * PI * mean_sq / (spacing * spacing));
 		if (n < 8) n = 8;
 		const double ga = PI * (3.0 - std::sqrt(5.0));
 		for (long i = 0; i < n; ++i) {
 			const double y = 1.0 - (2.0 * (double)i + 1.0) / (double)n;
 			const double rho = std::sqrt(std::max(0.0, 1.0 - y * y));
 			const double phi = (double)i * ga;
-			b.samples.push_back(r * (rho * std::cos(phi)));
-			b.samples.push_back(r * y);
-			b.samples.push_back(r * (rho * std::sin(phi)));
+			b.samples.push_back(ax * (rho * std::cos(phi)));
+			b.samples.push_back(ay * y);
+			b.samples.push_back(az * (rho * std::sin(phi)));
 		}
 		return;
 	}
@@ -1858,6 +1954,9 @@ int32_t orc_samples(const orc_scene* s, int32_t b, int32_t cap, double* out) {
 	const int n = (int)(v.size() / 3);
 	if (out) std::memcpy(out, v.data(), 3 * (size_t)std::min(n, cap) * sizeof(double));
 	return n;
+}
+void orc_ellipsoid_sdf(const double dims[3], int32_t n, const double* pts, double* out) {
+	for (int32_t i = 0; i < n; ++i) out[i] = ellipsoid_sdf(dims, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
 }
 
 int32_t orc_first_system(orc_scene* s, int32_t C, const int32_t* ids, const double* T,

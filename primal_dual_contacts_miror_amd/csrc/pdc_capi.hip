@@ -273,7 +273,8 @@ static void check_bodies(const char* who, int n, const int32_t* shape_type, cons
 	if (n < 0 || (n > 0 && (!shape_type || !dims || !mass || !inertia_diag || !is_static)))
 		throw std::invalid_argument(w + ": null array or negative count");
 	for (int i = 0; i < n; ++i) {
-		if (shape_type[i] != PDC_SHAPE_BOX && shape_type[i] != PDC_SHAPE_SPHERE)
+		if (shape_type[i] != PDC_SHAPE_BOX && shape_type[i] != PDC_SHAPE_SPHERE &&
+		    shape_type[i] != PDC_SHAPE_ELLIPSOID)
 			throw std::invalid_argument(w + ": unknown shape type");
 		for (int a = 0; a < 3; ++a)
 			if (!(dims[3 * i + a] > 0) && !(shape_type[i] == PDC_SHAPE_SPHERE && a > 0))

@@ -31,21 +31,31 @@ namespace pdc {
 //   box:    each face (axis a, side -/+) carries a cell-centred nu x nw grid,
 //           n = max(1, round(edge / spacing)), in the order axis, side, iu, iw
 //   sphere: Fibonacci lattice of max(8, round(4 pi r^2 / spacing^2)) points
+//   ellipsoid: the unit Fibonacci lattice scaled by the semi-axes (a, b, c),
+//           max(8, round(4 pi (ab + ac + bc) / 3 / spacing^2)) points (the
+//           area of the ellipsoid by the p = 1 case of Thomsen's formula,
+//           exact for a sphere)
 // ---------------------------------------------------------------------------
 static void make_samples(int type, const double* dims, double spacing, std::vector<double>& out) {
-	if (type == PDC_SHAPE_SPHERE) {
-		const double r = dims[0];
+	if (type == PDC_SHAPE_SPHERE || type == PDC_SHAPE_ELLIPSOID) {
 		const double PI = 3.14159265358979323846;
-		long n = std::lround(4.0 * PI * r * r / (spacing * spacing));
+		double sx = dims[0], sy = dims[0], sz = dims[0];
+		double area4 = sx * sx;
+		if (type == PDC_SHAPE_ELLIPSOID) {
+			sy = dims[1];
+			sz = dims[2];
+			area4 = (sx * sy + sx * sz + sy * sz) / 3.0;
+		}
+		long n = std::lround(4.0 * PI * area4 / (spacing * spacing));
 		if (n < 8) n = 8;
 		const double ga = PI * (3.0 - std::sqrt(5.0));
 		for (long i = 0; i < n; ++i) {
 			const double y = 1.0 - (2.0 * (double)i + 1.0) / (double)n;
 			const double rho = std::sqrt(std::max(0.0, 1.0 - y * y));
 			const double phi = (double)i * ga;
-			out.push_back(r * (rho * std::cos(phi)));
-			out.push_back(r * y);
-			out.push_back(r * (rho * std::sin(phi)));
+			out.push_back(sx * (rho * std::cos(phi)));
+			out.push_back(sy * y);
+			out.push_back(sz * (rho * std::sin(phi)));
 		}
 		return;
 	}

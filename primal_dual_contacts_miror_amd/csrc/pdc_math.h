@@ -152,14 +152,149 @@ PDC_HD void frame_T(const double* R0, const double* pos, const double* xj, int j
 }
 
 // ---------------------------------------------------------------------------
+// Euclidean distance from a point to an ellipsoid (D. Eberly, "Distance from
+// a Point to an Ellipse, an Ellipsoid, or a Hyperellipsoid", Geometric Tools,
+// 2013).  With semi-axes a0 >= a1 >= a2 and the point y in the first octant,
+// the closest point is x_i = r_i y_i / (s + r_i), r_i = (a_i / a2)^2, where s
+// is the root of F(s) = sum_i (r_i z_i / (s + r_i))^2 - 1, z_i = y_i / a_i;
+// F decreases on (-1, inf) when y2 > 0, and the root lies in
+// [z2 - 1, |(r0 z0, r1 z1, z2)| - 1] (or [z2 - 1, 0] inside).  Bisection over
+// floating-point midpoints ends when the midpoint equals an end; the cap is
+// above the number of halvings a double interval admits.  A zero component
+// reduces the problem to an ellipse, or (y2 = 0, point inside near the plane
+// of the two larger axes) to the closed form of the interior case.
+// ---------------------------------------------------------------------------
+constexpr int kEllipsoidBisect = 1100;
+
+PDC_HD double ellipse_root(double n0, double r0, double z1, double g) {
+	double s0 = z1 - 1.0;
+	double s1 = g < 0.0 ? 0.0 : sqrt(n0 * n0 + z1 * z1) - 1.0;
+	double s = s0;
+	for (int it = 0; it < kEllipsoidBisect; ++it) {
+		s = 0.5 * (s0 + s1);
+		if (s == s0 || s == s1) break;
+		const double t0 = n0 / (s + r0);
+		const double t1 = z1 / (s + 1.0);
+		const double f = t0 * t0 + t1 * t1 - 1.0;
+		if (f > 0.0) s0 = s;
+		else if (f < 0.0) s1 = s;
+		else break;
+	}
+	return s;
+}
+
+PDC_HD double ellipsoid_root(double n0, double r0, double n1, double r1, double z2, double g) {
+	double s0 = z2 - 1.0;
+	double s1 = g < 0.0 ? 0.0 : sqrt(n0 * n0 + n1 * n1 + z2 * z2) - 1.0;
+	double s = s0;
+	for (int it = 0; it < kEllipsoidBisect; ++it) {
+		s = 0.5 * (s0 + s1);
+		if (s == s0 || s == s1) break;
+		const double t0 = n0 / (s + r0);
+		const double t1 = n1 / (s + r1);
+		const double t2 = z2 / (s + 1.0);
+		const double f = t0 * t0 + t1 * t1 + t2 * t2 - 1.0;
+		if (f > 0.0) s0 = s;
+		else if (f < 0.0) s1 = s;
+		else break;
+	}
+	return s;
+}
+
+// Ellipse with semi-axes a0 >= a1 > 0, point (y0, y1) >= 0.
+PDC_HD double ellipse_distance(double a0, double a1, double y0, double y1) {
+	if (y1 > 0.0) {
+		if (y0 > 0.0) {
+			const double z0 = y0 / a0, z1 = y1 / a1;
+			const double g = z0 * z0 + z1 * z1 - 1.0;
+			double x0 = y0, x1 = y1;
+			if (g != 0.0) {
+				const double q0 = a0 / a1;
+				const double r0 = q0 * q0;
+				const double s = ellipse_root(r0 * z0, r0, z1, g);
+				x0 = r0 * y0 / (s + r0);
+				x1 = y1 / (s + 1.0);
+			}
+			const double d0 = x0 - y0, d1 = x1 - y1;
+			return sqrt(d0 * d0 + d1 * d1);
+		}
+		return fabs(y1 - a1);
+	}
+	const double num = a0 * y0, den = a0 * a0 - a1 * a1;
+	if (num < den) {
+		const double xd = num / den;
+		const double x0 = a0 * xd, x1 = a1 * sqrt(1.0 - xd * xd);
+		const double d0 = x0 - y0;
+		return sqrt(d0 * d0 + x1 * x1);
+	}
+	return fabs(y0 - a0);
+}
+
+// Ellipsoid with semi-axes a0 >= a1 >= a2 > 0, point y >= 0.
+PDC_HD double ellipsoid_distance(double a0, double a1, double a2, double y0, double y1, double y2) {
+	if (y2 > 0.0) {
+		if (y1 > 0.0) {
+			if (y0 > 0.0) {
+				const double z0 = y0 / a0, z1 = y1 / a1, z2 = y2 / a2;
+				const double g = z0 * z0 + z1 * z1 + z2 * z2 - 1.0;
+				double x0 = y0, x1 = y1, x2 = y2;
+				if (g != 0.0) {
+					const double q0 = a0 / a2, q1 = a1 / a2;
+					const double r0 = q0 * q0, r1 = q1 * q1;
+					const double s = ellipsoid_root(r0 * z0, r0, r1 * z1, r1, z2, g);
+					x0 = r0 * y0 / (s + r0);
+					x1 = r1 * y1 / (s + r1);
+					x2 = y2 / (s + 1.0);
+				}
+				const double d0 = x0 - y0, d1 = x1 - y1, d2 = x2 - y2;
+				return sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+			}
+			return ellipse_distance(a1, a2, y1, y2);
+		}
+		if (y0 > 0.0) return ellipse_distance(a0, a2, y0, y2);
+		return fabs(y2 - a2);
+	}
+	const double den0 = a0 * a0 - a2 * a2, den1 = a1 * a1 - a2 * a2;
+	const double num0 = a0 * y0, num1 = a1 * y1;
+	if (num0 < den0 && num1 < den1) {
+		const double e0 = num0 / den0, e1 = num1 / den1;
+		const double disc = 1.0 - e0 * e0 - e1 * e1;
+		if (disc > 0.0) {
+			const double x0 = a0 * e0, x1 = a1 * e1, x2 = a2 * sqrt(disc);
+			const double d0 = x0 - y0, d1 = x1 - y1;
+			return sqrt(d0 * d0 + d1 * d1 + x2 * x2);
+		}
+	}
+	return ellipse_distance(a0, a1, y0, y1);
+}
+
+// Signed distance to the ellipsoid with semi-axes dims (any order): the axes
+// sorted descending (ties keep their order), negative where
+// (p0/a0)^2 + (p1/a1)^2 + (p2/a2)^2 < 1.
+PDC_HD double ellipsoid_sdf(const double* dims, double px, double py, double pz) {
+	double a0 = dims[0], a1 = dims[1], a2 = dims[2];
+	double y0 = fabs(px), y1 = fabs(py), y2 = fabs(pz);
+	double t;
+	if (a0 < a1) { t = a0; a0 = a1; a1 = t; t = y0; y0 = y1; y1 = t; }
+	if (a1 < a2) { t = a1; a1 = a2; a2 = t; t = y1; y1 = y2; y2 = t; }
+	if (a0 < a1) { t = a0; a0 = a1; a1 = t; t = y0; y0 = y1; y1 = t; }
+	const double d = ellipsoid_distance(a0, a1, a2, y0, y1, y2);
+	const double z0 = y0 / a0, z1 = y1 / a1, z2 = y2 / a2;
+	return z0 * z0 + z1 * z1 + z2 * z2 < 1.0 ? -d : d;
+}
+
+// ---------------------------------------------------------------------------
 // Grid SDF of a primitive.  Grid: n^3 FP32 nodes over [-h, h] per axis,
 // h = half extent * (1 + pad), spacing 2h/(n-1); node (i,j,k) at
 // lo + (i,j,k)*spacing holds (float) of the analytic FP64 distance.
+// Types: 0 box (half extents), 1 sphere (radius in dims[0]), 2 ellipsoid
+// (semi-axes).
 // ---------------------------------------------------------------------------
 PDC_HD double analytic_sdf(int type, const double* dims, double px, double py, double pz) {
 	if (type == 1) {
 		return sqrt(px * px + py * py + pz * pz) - dims[0];
 	}
+	if (type == 2) return ellipsoid_sdf(dims, px, py, pz);
 	const double qx = fabs(px) - dims[0];
 	const double qy = fabs(py) - dims[1];
 	const double qz = fabs(pz) - dims[2];

@@ -12,6 +12,7 @@ import ctypes
 
 SHAPE_BOX = 0
 SHAPE_SPHERE = 1
+SHAPE_ELLIPSOID = 2
 NON_SMOOTH_CONTACT = 0
 ONLY_NORMAL = 1
 PCG_FP64 = 0

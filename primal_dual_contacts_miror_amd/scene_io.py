@@ -2,22 +2,24 @@
 
 Input (src/IO/FileLoader.cpp:14-259): a YAML scene file names glTF/GLB files
 per primitive kind; every node of a file that carries a mesh becomes a rigid
-body, fitted to the primitive (box or sphere) with the same inertia as the
-mesh under the node's transformation (RigidObjectBase.cpp:121-210,
-Cube.cpp:5-65, Sphere.cpp:6-37).  Bodies are numbered in load order: kinds
+body, fitted to the primitive (box, sphere or ellipsoid) with the same
+inertia as the mesh under the node's transformation
+(RigidObjectBase.cpp:121-210, Cube.cpp:5-65, Sphere.cpp:6-37,
+Ellipsoid.cpp:6-36).  Bodies are numbered in load order: kinds
 in the order meshes, convex, cubes, ellipsoids, spheres; files in list
 order; nodes in node order.  `static` and `initialize` refer to those ids.
 
 Output (src/IO/FileSaver.cpp:44-206): `<name>.gltf` (+ `.bin`) with one mesh
-`mesh_i` per body (the collision geometry: unit cube or icosphere) and one
+`mesh_i` per body (the collision geometry: unit cube or icosphere, scaled
+by the permanent transformation) and one
 node per body carrying the final total transformation; `<name>.npz` with
 `time` (T,) and per body `mesh_i` (T + 1, 7): row 0 the scale of the
 permanent transformation, then per saved frame the translation and the
 rotation quaternion (x, y, z, w); `<name>.info` with the timing summary
 (DynamicSystem.cpp:299-313, PrimalDualForceSolver.cpp:636-648).
 
-Only the primitive kinds the device pipeline supports (cubes, spheres) are
-loaded; `meshes`, `convex` and `ellipsoids` are rejected with an error.
+Only the primitive kinds the device pipeline supports (cubes, ellipsoids,
+spheres) are loaded; `meshes` and `convex` are rejected with an error.
 """
 import base64
 import json
@@ -298,21 +300,22 @@ def principal_axes(m):
 
 @dataclass
 class FittedBody:
-    kind: int                  # params.SHAPE_BOX | SHAPE_SPHERE
-    dims: np.ndarray           # box half extents / sphere radius in [0]
+    kind: int                  # params.SHAPE_BOX | SHAPE_SPHERE | SHAPE_ELLIPSOID
+    dims: np.ndarray           # box half extents / sphere radius in [0] / ellipsoid semi-axes
     volume: float
     inertia_unit: np.ndarray   # principal moments at unit density, body frame
     position: np.ndarray       # centre of mass, world
     rotation: np.ndarray       # body frame, world (3x3)
     permanent: np.ndarray      # 4x4 permanent transformation (the primitive's scale)
     mesh_trans: np.ndarray     # 4x4 input mesh in the body frame
-    min_length: float          # obbMinLength (Cube.cpp:66-70, Sphere.cpp:38-41)
+    min_length: float          # obbMinLength (Cube.cpp:66-70, Sphere.cpp:38-41, Ellipsoid.cpp:37-41)
     source: tuple = ()
 
 
 def fit_primitive(kind, positions, triangles, matrix):
     """Body of one glTF node (RigidObjectBase::preprocessMesh, then
-    Cube::initializeFCLObject or Sphere::initializeFCLObject)."""
+    Cube::initializeFCLObject, Sphere::initializeFCLObject or
+    Ellipsoid::initializeFCLObject)."""
     rot, spd = polar_decomposition(matrix[:3, :3])
     translate = matrix[:3, 3]
     v = positions @ spd.T
@@ -351,6 +354,21 @@ def fit_primitive(kind, positions, triangles, matrix):
         dims = sides / 2.0
         scale = dims.copy()
         min_len = float(sides.min())
+    elif kind == P.SHAPE_ELLIPSOID:
+        # Ellipsoid.cpp:6-36: semi-axes of the solid ellipsoid with the same
+        # principal moments, I_i = V/5 (sum of the other two squared axes);
+        # the axes stay in the order of the moments (no permutation)
+        svf = 5.0 / vol * sv
+        semi = np.sqrt(svf.sum() / 2.0 - svf)
+        r = U.copy()
+        if np.linalg.det(r) < 0:
+            r[2, :] *= -1
+        a, b, c = semi
+        V = 4.0 / 3.0 * math.pi * a * b * c
+        inertia = V / 5.0 * np.array([b * b + c * c, a * a + c * c, a * a + b * b])
+        dims = semi.copy()
+        scale = dims.copy()
+        min_len = float(semi.min())
     elif kind == P.SHAPE_SPHERE:
         svf = 5.0 / vol * sv
         semi = np.sqrt(svf.sum() / 2.0 - svf)
@@ -392,7 +410,7 @@ class LoadedScene:
     path: str = ""
 
 
-_KINDS = [("meshes", None), ("convex", None), ("cubes", P.SHAPE_BOX), ("ellipsoids", None),
+_KINDS = [("meshes", None), ("convex", None), ("cubes", P.SHAPE_BOX), ("ellipsoids", P.SHAPE_ELLIPSOID),
           ("spheres", P.SHAPE_SPHERE)]
 
 
@@ -470,7 +488,7 @@ def load_scene(path, grid_n=64, sample_spacing=None, pcg_precision=None, pcg_tol
             continue
         if kind is None:
             raise NotImplementedError("scene kind %r is not supported by the device pipeline "
-                                      "(boxes and spheres only)" % key)
+                                      "(boxes, spheres and ellipsoids only)" % key)
         files = sn[key] if isinstance(sn[key], list) else [sn[key]]
         for fn in files:
             for nd in read_gltf(os.path.join(base, str(fn))):
@@ -612,6 +630,9 @@ def _aabb_max_vertex_speed(loaded, pos, quat, vel, ang):
         R = quat_to_matrix(quat[k])
         if sc.shape_type[k] == P.SHAPE_BOX:
             half = np.abs(R) @ sc.dims[k]
+        elif sc.shape_type[k] == P.SHAPE_ELLIPSOID:
+            # the tight box of the rotated ellipsoid: |R_i . diag(a)|
+            half = np.sqrt((R * R) @ (sc.dims[k] * sc.dims[k]))
         else:
             half = np.full(3, sc.dims[k][0])
         rel = corners * half            # corner - translation (AABB centre = body centre)

@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .params import SHAPE_BOX, SHAPE_SPHERE, PCG_FP32, PCG_FP64, default_params
+from .params import SHAPE_BOX, SHAPE_ELLIPSOID, SHAPE_SPHERE, PCG_FP32, PCG_FP64, default_params
 
 
 @dataclass
@@ -71,6 +71,13 @@ class _Builder:
         m = density * 4.0 / 3.0 * math.pi * r ** 3
         i = 0.4 * m * r * r
         self.rows.append((SHAPE_SPHERE, (r, r, r), m, (i, i, i), static,
+                          (not static) if emit is None else emit, pos, quat))
+
+    def ellipsoid(self, semi, pos, quat=(0, 0, 0, 1), density=1.0, static=False, emit=None):
+        a, b_, c = semi
+        m = density * 4.0 / 3.0 * math.pi * a * b_ * c
+        inertia = (0.2 * m * (b_ * b_ + c * c), 0.2 * m * (a * a + c * c), 0.2 * m * (a * a + b_ * b_))
+        self.rows.append((SHAPE_ELLIPSOID, (a, b_, c), m, inertia, static,
                           (not static) if emit is None else emit, pos, quat))
 
     def build(self, name, params, steps, **meta):
@@ -215,6 +222,27 @@ def pile(n_bodies=2000, seed=0, half=10.0, height=40.0, sphere_r=(0.2, 0.5),
     for k, v in param_overrides.items():
         setattr(p, k, v)
     return b.build(name, p, steps=1000, n_dynamic=n_bodies, seed=seed)
+
+
+def ellipsoid_pile(n_bodies=300, seed=0, half=4.0, height=16.0, semi=(0.15, 0.5), name=None,
+                   **param_overrides):
+    """Ellipsoids with seeded uniform semi-axes in `semi`, random orientations
+    and non-overlapping bounding spheres, dropped into a static open
+    container; the solver settings of `pile`."""
+    rng = np.random.default_rng(seed)
+    axes = rng.uniform(semi[0], semi[1], (n_bodies, 3))
+    lo = np.array([-half + 0.6, 1.0, -half + 0.6])
+    hi = np.array([half - 0.6, height - 1.0, half - 0.6])
+    pos = _place(rng, axes.max(axis=1), lo, hi)
+    b = _Builder()
+    _container(b, half, half, height)
+    for e, p in zip(axes, pos):
+        b.ellipsoid(tuple(e), tuple(p), quat=_random_quat(rng))
+    p = default_params(timestep=0.005, friction_coeff=0.6, grid_n=64, sample_spacing=0.05,
+                       tolerance=1e-4, iterations=50, pcg_precision=PCG_FP32, pcg_tol=1e-5)
+    for k, v in param_overrides.items():
+        setattr(p, k, v)
+    return b.build(name or "ellipsoid_pile_%d" % n_bodies, p, steps=1000, n_dynamic=n_bodies, seed=seed)
 
 
 def pile_2000(seed=0, **kw):

@@ -40,7 +40,7 @@ import time
 import numpy as np
 
 from . import pdc
-from .params import PCG_FP64, SHAPE_SPHERE
+from .params import PCG_FP64, SHAPE_ELLIPSOID, SHAPE_SPHERE
 
 PAD_HALF = 1e-3      # half extent of a padding body
 PAD_FAR = 1.0e6      # its distance from everything else
@@ -51,11 +51,16 @@ BYTES_PER_CONTACT = 720.0
 def sample_count(shape, dims, spacing):
     """Surface samples of one body, by the library's rule (boxes: a cell-centred
     grid of max(1, round(edge / spacing)) per edge on each face; spheres: a
-    Fibonacci lattice of max(8, round(4 pi r^2 / spacing^2)) points)."""
+    Fibonacci lattice of max(8, round(4 pi r^2 / spacing^2)) points;
+    ellipsoids: that lattice with 4 pi (ab + ac + bc) / 3 in place of the
+    sphere's area)."""
     def rnd(x):  # half away from zero, as the library rounds
         return int(np.floor(x + 0.5))
     if shape == SHAPE_SPHERE:
         return max(8, rnd(4.0 * np.pi * dims[0] ** 2 / spacing ** 2))
+    if shape == SHAPE_ELLIPSOID:
+        a, b, c = dims
+        return max(8, rnd(4.0 * np.pi * ((a * b + a * c + b * c) / 3.0) / spacing ** 2))
     e = [max(1, rnd(2.0 * d / spacing)) for d in dims]
     return 2 * (e[1] * e[2] + e[2] * e[0] + e[0] * e[1])
 

@@ -5,7 +5,8 @@ test_gpu_scene_io.py.
 
 The fitting is pinned by closed forms: the volume, centre of mass and
 inertia of a box mesh, the box a scaled and rotated cube node must give, a
-sphere's radius converging under mesh refinement.  The readers are pinned by
+sphere's radius and an ellipsoid's semi-axes and axes converging under mesh
+refinement.  The readers are pinned by
 files built byte by byte here, and by the reference's own scene files where
 they are present (skipped otherwise).
 """
@@ -119,6 +120,39 @@ def test_fit_sphere_converges_with_mesh_refinement():
     assert all(e1 < e0 for e0, e1 in zip(errs, errs[1:]))
 
 
+def test_fit_ellipsoid_of_scaled_rotated_sphere_node():
+    """A unit icosphere under scale (a, b, c) and a rotation is an ellipsoid:
+    the fit converges to its semi-axes under refinement, its axes to the
+    rotated coordinate axes, with V = 4/3 pi abc and I = V/5 (b^2 + c^2, ...)
+    (Ellipsoid.cpp:6-36), and obbMinLength is the smallest semi-axis."""
+    semi = np.array([1.2, 0.5, 0.8])
+    q, R = _rot((0.3, -1.0, 0.4), 0.7)
+    errs = []
+    for sub in (2, 3, 4):
+        v, tri = S.icosphere_mesh(sub)
+        M = np.eye(4)
+        M[:3, :3] = R @ np.diag(semi)
+        M[:3, 3] = [-1.0, 0.5, 2.0]
+        fb = S.fit_primitive(P.SHAPE_ELLIPSOID, v, tri, M)
+        assert fb.kind == P.SHAPE_ELLIPSOID
+        np.testing.assert_allclose(fb.position, [-1.0, 0.5, 2.0], atol=1e-9)
+        a, b, c = fb.dims
+        assert fb.volume == pytest.approx(4 / 3 * math.pi * a * b * c, rel=1e-12)
+        np.testing.assert_allclose(fb.inertia_unit, fb.volume / 5 * np.array([b * b + c * c, a * a + c * c,
+                                                                              a * a + b * b]), rtol=1e-12)
+        assert fb.min_length == min(fb.dims)
+        assert np.linalg.det(fb.rotation) == pytest.approx(1.0, abs=1e-12)
+        # fitted axis k of the body frame is the rotated coordinate axis whose
+        # scale it recovers (up to sign)
+        for k in range(3):
+            j = int(np.argmin(np.abs(semi - fb.dims[k])))
+            assert abs(fb.rotation[:, k] @ R[:, j]) == pytest.approx(1.0, abs=1e-6)
+        errs.append(np.max(np.abs(np.sort(fb.dims) - np.sort(semi)) / np.sort(semi)))
+    assert errs[-1] < 2e-3
+    assert all(e1 < e0 for e0, e1 in zip(errs, errs[1:]))
+    np.testing.assert_allclose(fb.permanent[:3, :3], np.diag(fb.dims))
+
+
 def test_principal_axes_and_quaternions():
     rng = np.random.default_rng(0)
     for _ in range(100):
@@ -217,8 +251,9 @@ def test_gltf_write_read_round_trip(tmp_path):
         np.testing.assert_array_equal(nd.matrix, M)
 
 
-def _write_scene(tmp_path, yaml_text, cubes, spheres=None):
-    """cubes / spheres: lists of (scale(3), translation(3)) node specs, one file each."""
+def _write_scene(tmp_path, yaml_text, cubes, spheres=None, ellipsoids=None):
+    """cubes / spheres / ellipsoids: lists of (scale(3), translation(3)) node
+    specs, one file each."""
     def glb(nodes, mesh):
         v, t = mesh
         p32 = v.astype(np.float32)
@@ -237,6 +272,8 @@ def _write_scene(tmp_path, yaml_text, cubes, spheres=None):
     (tmp_path / "cubes.glb").write_bytes(glb(cubes, S.unit_cube_mesh()))
     if spheres:
         (tmp_path / "balls.glb").write_bytes(glb(spheres, S.icosphere_mesh(3)))
+    if ellipsoids:
+        (tmp_path / "eggs.glb").write_bytes(glb(ellipsoids, S.icosphere_mesh(3)))
     (tmp_path / "scene.yaml").write_text(yaml_text)
     return str(tmp_path / "scene.yaml")
 
@@ -288,6 +325,21 @@ def test_load_scene_order_static_initialize_and_params(tmp_path):
     assert L.baking_time == 2.5 and not L.adaptive_timestep and L.output_collision_geometry
     assert p.pcg_precision == P.PCG_FP64 and p.pcg_tol == 1e-10
     assert p.sample_spacing == pytest.approx(0.25 * 0.25)  # a quarter of the smallest half extent
+
+
+def test_load_scene_with_ellipsoids(tmp_path):
+    """`ellipsoids` load after `cubes` and before `spheres` (FileLoader.cpp:45-75)."""
+    y = YAML.replace('  cubes: "cubes.glb"\n', '  cubes: "cubes.glb"\n  ellipsoids: "eggs.glb"\n')
+    path = _write_scene(tmp_path, y, cubes=[((5, 0.5, 5), (0, -0.5, 0)), ((0.5, 0.5, 0.5), (0, 0.5, 0)),
+                                            ((0.25, 0.5, 0.75), (2, 0.6, 0)), ((0.5, 0.5, 0.5), (3, 0.5, 0))],
+                        spheres=[((0.4, 0.4, 0.4), (0, 2, 0))],
+                        ellipsoids=[((0.6, 0.3, 0.4), (-2, 1, 0)), ((0.2, 0.2, 0.5), (-3, 1, 0))])
+    sc = S.load_scene(path).scene
+    assert list(sc.shape_type) == [P.SHAPE_BOX] * 4 + [P.SHAPE_ELLIPSOID] * 2 + [P.SHAPE_SPHERE]
+    np.testing.assert_allclose(np.sort(sc.dims[4]), [0.3, 0.4, 0.6], rtol=3e-3)
+    np.testing.assert_allclose(sc.pos[5], [-3, 1, 0], atol=1e-9)
+    a, b, c = sc.dims[4]
+    assert sc.mass[4] == pytest.approx(4 / 3 * math.pi * a * b * c, rel=1e-12)
 
 
 def test_load_scene_defaults_and_errors(tmp_path):
