@@ -208,6 +208,18 @@ int pdc_get_groups(pdc_context* ctx, int32_t n_bodies, int32_t* group_key);
  * number of samples whose signed distance lies within the tie band. */
 int pdc_detect_contacts(pdc_context* ctx, int32_t* n_contacts, int32_t* n_ties);
 
+/* Diagnostics of the last contact detection (no counterpart in the
+ * reference): candidate pairs, raw sample hits, tie-band samples, the
+ * samples of the emitting sides summed over pairs, those of them in the
+ * sample cells kept for the partner (the ones tested), the grid nodes under
+ * each partner's box summed over pairs and sides, and pairs with more than
+ * 2048 hits (merged in global memory). */
+typedef struct pdc_detection_stats {
+	int64_t pairs, raw_hits, ties;
+	int64_t samples_all, samples_read, brick_nodes, large_pairs;
+} pdc_detection_stats;
+int pdc_get_detection_stats(const pdc_context* ctx, pdc_detection_stats* out);
+
 /* Copies the current contact list to the host in the reference's
  * ContactInformation layout (ContactInfo.h:138-144): obj_ids (2C, ordered
  * objId[0] < objId[1]), pos (3C), depth (C), transformation (36C: per contact

@@ -402,8 +402,9 @@ bool grid_query(const Body& b, const double* q, double* sd, double* grad) {
 /* Surface samples.  Box: on each face (axis 0..2, side -,+) a cell-centred
  * nu x nw grid, n = max(1, round(edge / spacing)).  Sphere: Fibonacci
  * lattice with max(8, round(4 pi r^2 / spacing^2)) points.  Ellipsoid: that
- * lattice scaled by the semi-axes. */
-void setup_samples(Body& b, double spacing) {
+ * lattice scaled by the semi-axes.  Computed in FP64, kept as FP32 values
+ * (setup_samples). */
+void surface_samples(Body& b, double spacing) {
 	b.samples.clear();
 	if (b.type == PDC_SHAPE_SPHERE || b.type == PDC_SHAPE_ELLIPSOID) {
 		/* the ellipsoid's lattice is the unit one scaled by the semi-axes;
@@ -448,6 +449,11 @@ void setup_samples(Body& b, double spacing) {
 			}
 		}
 	}
+}
+
+void setup_samples(Body& b, double spacing) {
+	surface_samples(b, spacing);
+	for (double& v : b.samples) v = (double)(float)v;
 }
 
 struct Contact {

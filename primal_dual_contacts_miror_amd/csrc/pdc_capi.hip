@@ -127,6 +127,7 @@ void free_all(pdc_context* c) {
 	c->b_mass.release(); c->b_I0.release(); c->b_pos.release(); c->b_quat.release();
 	c->b_vel.release(); c->b_ang.release(); c->b_acc.release(); c->b_R.release(); c->b_aabb.release();
 	c->b_grid.release(); c->grid_pool.release(); c->b_soff.release(); c->samples.release();
+	c->cells.release(); c->b_ncell.release(); c->stage_idx.release(); c->pr_big.release();
 	c->bp_cnt.release(); c->bp_off.release(); c->pr_a.release(); c->pr_b.release();
 	c->pr_soff.release(); c->pr_nh.release(); c->pr_nc.release(); c->pr_coff.release();
 	c->pr_tmp.release(); c->stage_raw.release(); c->stage_m.release();
@@ -428,6 +429,19 @@ int pdc_get_state(pdc_context* ctx, double* pos, double* quat_xyzw, double* vel,
 		if (angvel) PDC_HIP(hipMemcpyAsync(angvel, ctx->b_ang.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
 		PDC_HIP(hipStreamSynchronize(s));
 	});
+}
+
+int pdc_get_detection_stats(const pdc_context* ctx, pdc_detection_stats* out) {
+	if (!ctx || !out) return PDC_ERR_INVALID_ARGUMENT;
+	const pdc::DetectionStats& d = ctx->det;
+	out->pairs = d.pairs;
+	out->raw_hits = d.raw_hits;
+	out->ties = d.ties;
+	out->samples_all = d.samples_all;
+	out->samples_read = d.samples_read;
+	out->brick_nodes = d.brick_nodes;
+	out->large_pairs = d.large_pairs;
+	return PDC_OK;
 }
 
 int pdc_detect_contacts(pdc_context* ctx, int32_t* n_contacts, int32_t* n_ties) {

@@ -161,16 +161,98 @@ static void write_bodies(pdc_context* ctx, int first, int n, const int* shape, c
 	}
 }
 
-// Body-frame surface samples of n bodies; counts[b] per body.
-static void samples_of(const pdc_params& p, int n, const int* shape, const double* dims, std::vector<double>& smp,
-                       std::vector<int>& counts) {
+// Cells of one body's samples: a kc^3 grid over the box of its FP32
+// samples, kc the largest of 4, 3, 2, 1 with kc^3 <= S / 8 (so a body's cells
+// fit in the S / 8 table entries from floor(soff / 8)), none below 8 samples.
+// The samples are reordered by cell (stably) and appended to `out`; each
+// non-empty cell, in cell order, gets the box of its samples.
+static void sort_into_cells(const double* smp, int S, std::vector<float>& out, std::vector<SampleCell>& cells) {
+	std::vector<float> f(3 * (size_t)S);
+	for (size_t i = 0; i < f.size(); ++i) f[i] = (float)smp[i];
+	const int kc = S >= 512 ? 4 : S >= 216 ? 3 : S >= 64 ? 2 : S >= 8 ? 1 : 0;
+	if (kc == 0) {
+		out.insert(out.end(), f.begin(), f.end());
+		return;
+	}
+	float lo[3] = {f[0], f[1], f[2]}, hi[3] = {f[0], f[1], f[2]};
+	for (int i = 0; i < S; ++i)
+		for (int a = 0; a < 3; ++a) {
+			lo[a] = std::min(lo[a], f[3 * (size_t)i + a]);
+			hi[a] = std::max(hi[a], f[3 * (size_t)i + a]);
+		}
+	std::vector<int> cell(S);
+	const int nc = kc * kc * kc;
+	std::vector<int> cnt(nc + 1, 0);
+	for (int i = 0; i < S; ++i) {
+		int c[3];
+		for (int a = 0; a < 3; ++a) {
+			const double w = (double)hi[a] - (double)lo[a];
+			int k = w > 0.0 ? (int)(((double)f[3 * (size_t)i + a] - (double)lo[a]) / w * kc) : 0;
+			c[a] = std::min(std::max(k, 0), kc - 1);
+		}
+		cell[i] = (c[2] * kc + c[1]) * kc + c[0];
+		cnt[cell[i] + 1] += 1;
+	}
+	for (int k = 0; k < nc; ++k) cnt[k + 1] += cnt[k];
+	std::vector<int> order(S), cur(cnt.begin(), cnt.end() - 1);
+	for (int i = 0; i < S; ++i) order[cur[cell[i]]++] = i;
+	for (int k = 0; k < nc; ++k) {
+		if (cnt[k + 1] == cnt[k]) continue;
+		SampleCell sc{};
+		for (int a = 0; a < 3; ++a) { sc.lo[a] = INFINITY; sc.hi[a] = -INFINITY; }
+		for (int j = cnt[k]; j < cnt[k + 1]; ++j)
+			for (int a = 0; a < 3; ++a) {
+				const float v = f[3 * (size_t)order[j] + a];
+				sc.lo[a] = std::min(sc.lo[a], v);
+				sc.hi[a] = std::max(sc.hi[a], v);
+			}
+		sc.beg = cnt[k];
+		sc.end = cnt[k + 1];
+		cells.push_back(sc);
+	}
+	for (int j = 0; j < S; ++j)
+		for (int a = 0; a < 3; ++a) out.push_back(f[3 * (size_t)order[j] + a]);
+}
+
+// FP32 body-frame surface samples of n bodies, ordered by cell; counts[b]
+// samples and ncell[b] cells per body, the cells of all n bodies in `cells`.
+static void samples_of(const pdc_params& p, int n, const int* shape, const double* dims, std::vector<float>& smp,
+                       std::vector<int>& counts, std::vector<SampleCell>& cells, std::vector<int>& ncell) {
 	counts.assign(n, 0);
+	ncell.assign(n, 0);
+	std::vector<double> one;
 	for (int b = 0; b < n; ++b) {
-		const size_t before = smp.size();
-		make_samples(shape[b], &dims[3 * b], p.sample_spacing, smp);
-		counts[b] = (int)((smp.size() - before) / 3);
+		one.clear();
+		make_samples(shape[b], &dims[3 * b], p.sample_spacing, one);
+		const int S = (int)(one.size() / 3);
+		const size_t c0 = cells.size();
+		sort_into_cells(one.data(), S, smp, cells);
+		counts[b] = S;
+		ncell[b] = (int)(cells.size() - c0);
 	}
 }
+
+// Cells of bodies [first, first + n) to their places in the cell table (the
+// sample offsets of those bodies already set in h_soff).
+static void upload_cells(pdc_context* ctx, int first, int n, const std::vector<SampleCell>& cells,
+                         const std::vector<int>& ncell) {
+	hipStream_t s = ctx->stream;
+	size_t k = 0;
+	for (int b = 0; b < n; ++b) {
+		if (ncell[b] > 0) {
+			const size_t at = (size_t)ctx->h_soff[first + b] / kCellStride;
+			PDC_HIP(hipMemcpyAsync(ctx->cells.p + at, cells.data() + k, ncell[b] * sizeof(SampleCell),
+			                       hipMemcpyHostToDevice, s));
+		}
+		k += (size_t)ncell[b];
+	}
+	if (n > 0)
+		PDC_HIP(hipMemcpyAsync(ctx->b_ncell.p + first, ncell.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice,
+		                       s));
+	PDC_HIP(hipStreamSynchronize(s));
+}
+
+static size_t cell_slots(int64_t samples) { return (size_t)(samples / kCellStride) + 1; }
 
 void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,
                    const double* inertia, const int* is_static, const int* emit) {
@@ -181,7 +263,7 @@ void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims
 	ctx->b_pos.ensure(3 * N); ctx->b_quat.ensure(4 * N); ctx->b_vel.ensure(3 * N);
 	ctx->b_ang.ensure(3 * N); ctx->b_acc.ensure(6 * N);
 	ctx->b_R.ensure(9 * N); ctx->b_aabb.ensure(6 * N);
-	ctx->b_grid.ensure(N); ctx->b_soff.ensure(N + 1);
+	ctx->b_grid.ensure(N); ctx->b_soff.ensure(N + 1); ctx->b_ncell.ensure(N);
 	ctx->h_static.assign(N, 0);
 	ctx->h_emit.assign(N, 0);
 	const int gn = ctx->p.grid_n;
@@ -189,20 +271,25 @@ void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims
 	write_bodies(ctx, 0, n, shape, dims, mass, inertia, is_static, emit, true);
 	if (n == 0) ctx->tie_band = 0.0;
 	// samples
-	std::vector<double> smp;
-	std::vector<int> cnt;
-	samples_of(ctx->p, n, shape, dims, smp, cnt);
+	std::vector<float> smp;
+	std::vector<int> cnt, ncell;
+	std::vector<SampleCell> cells;
+	samples_of(ctx->p, n, shape, dims, smp, cnt, cells, ncell);
+	std::vector<int64_t> total(N + 1, 0);
+	for (size_t b = 0; b < N; ++b) total[b + 1] = total[b] + cnt[b];
+	if (total[N] > INT32_MAX) throw HipError("sample table beyond 2^31 samples", PDC_ERR_CAPACITY);
 	std::vector<int> soff(N + 1, 0);
-	for (size_t b = 0; b < N; ++b) soff[b + 1] = soff[b] + cnt[b];
+	for (size_t b = 0; b <= N; ++b) soff[b] = (int)total[b];
 	ctx->n_samples = soff[N];
 	ctx->h_soff = soff;
 	hipStream_t s = ctx->stream;
 	ctx->samples.ensure(std::max<size_t>(smp.size(), 3));
+	ctx->cells.ensure(cell_slots(soff[N]));
 	if (!smp.empty())
-		PDC_HIP(hipMemcpyAsync(ctx->samples.p, smp.data(), smp.size() * sizeof(double),
+		PDC_HIP(hipMemcpyAsync(ctx->samples.p, smp.data(), smp.size() * sizeof(float),
 		                       hipMemcpyHostToDevice, s));
 	PDC_HIP(hipMemcpyAsync(ctx->b_soff.p, soff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-	PDC_HIP(hipStreamSynchronize(s));
+	upload_cells(ctx, 0, n, cells, ncell);
 	const int one[2] = {0, (int)N};
 	set_scenes(ctx, 1, one);
 }
@@ -220,10 +307,13 @@ void reserve_bodies(pdc_context* ctx, int64_t bodies, int64_t samples) {
 	ctx->b_acc.grow_keep(6 * N1, 6 * N0, s, true);
 	ctx->b_R.grow_keep(9 * N1, 0, s, true); ctx->b_aabb.grow_keep(6 * N1, 0, s, true);
 	ctx->b_grid.grow_keep(N1, N0, s, true); ctx->b_soff.grow_keep(N1 + 1, N0 + 1, s, true);
+	ctx->b_ncell.grow_keep(N1, N0, s, true);
 	const size_t n3 = (size_t)ctx->p.grid_n * ctx->p.grid_n * ctx->p.grid_n;
 	ctx->grid_pool.grow_keep(std::max<size_t>(N1 * n3, 1), N0 * n3, s, true);
 	const size_t total0 = ctx->h_soff.empty() ? 0 : (size_t)ctx->h_soff[N0];
-	ctx->samples.grow_keep(3 * std::max<size_t>((size_t)samples, std::max<size_t>(total0, 1)), 3 * total0, s, true);
+	const size_t total1 = std::max<size_t>((size_t)samples, std::max<size_t>(total0, 1));
+	ctx->samples.grow_keep(3 * total1, 3 * total0, s, true);
+	ctx->cells.grow_keep(cell_slots((int64_t)total1), cell_slots((int64_t)total0), s, true);
 }
 
 int append_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,
@@ -240,24 +330,28 @@ int append_bodies(pdc_context* ctx, int n, const int* shape, const double* dims,
 	ctx->b_acc.grow_keep(6 * N1, 6 * N0, s);
 	ctx->b_R.grow_keep(9 * N1, 0, s); ctx->b_aabb.grow_keep(6 * N1, 0, s);
 	ctx->b_grid.grow_keep(N1, N0, s); ctx->b_soff.grow_keep(N1 + 1, N0 + 1, s);
+	ctx->b_ncell.grow_keep(N1, N0, s);
 	const size_t n3 = (size_t)ctx->p.grid_n * ctx->p.grid_n * ctx->p.grid_n;
 	ctx->grid_pool.grow_keep(std::max<size_t>(N1 * n3, 1), N0 * n3, s);
-	std::vector<double> smp;
-	std::vector<int> cnt;
-	samples_of(ctx->p, n, shape, dims, smp, cnt);
+	std::vector<float> smp;
+	std::vector<int> cnt, ncell;
+	std::vector<SampleCell> cells;
+	samples_of(ctx->p, n, shape, dims, smp, cnt, cells, ncell);
 	const int64_t total0 = ctx->h_soff.empty() ? 0 : ctx->h_soff[N0];
 	const int64_t total1 = total0 + (int64_t)(smp.size() / 3);
 	if (total1 > INT32_MAX) throw HipError("sample table beyond 2^31 samples", PDC_ERR_CAPACITY);
 	ctx->samples.grow_keep(std::max<size_t>(3 * (size_t)total1, 3), 3 * (size_t)total0, s);
+	ctx->cells.grow_keep(cell_slots(total1), cell_slots(total0), s);
 	ctx->h_soff.resize(N1 + 1);
 	if (N0 == 0) ctx->h_soff[0] = 0;
 	for (size_t b = 0; b < (size_t)n; ++b) ctx->h_soff[N0 + b + 1] = ctx->h_soff[N0 + b] + cnt[b];
 	ctx->n_samples = ctx->h_soff[N1];
 	if (!smp.empty())
-		PDC_HIP(hipMemcpyAsync(ctx->samples.p + 3 * total0, smp.data(), smp.size() * sizeof(double),
+		PDC_HIP(hipMemcpyAsync(ctx->samples.p + 3 * total0, smp.data(), smp.size() * sizeof(float),
 		                       hipMemcpyHostToDevice, s));
 	PDC_HIP(hipMemcpyAsync(ctx->b_soff.p + N0, ctx->h_soff.data() + N0, ((size_t)n + 1) * sizeof(int),
 	                       hipMemcpyHostToDevice, s));
+	upload_cells(ctx, (int)N0, n, cells, ncell);
 	ctx->N = (int)N1;
 	write_bodies(ctx, (int)N0, n, shape, dims, mass, inertia, is_static, emit, N0 == 0);
 	PDC_HIP(hipStreamSynchronize(s));
@@ -274,9 +368,10 @@ int append_bodies(pdc_context* ctx, int n, const int* shape, const double* dims,
 void replace_bodies(pdc_context* ctx, int first, int n, const int* shape, const double* dims, const double* mass,
                     const double* inertia, const int* is_static, const int* emit) {
 	if (first < 0 || n < 1 || first + n > ctx->N) throw std::invalid_argument("body range outside the table");
-	std::vector<double> smp;
-	std::vector<int> cnt;
-	samples_of(ctx->p, n, shape, dims, smp, cnt);
+	std::vector<float> smp;
+	std::vector<int> cnt, ncell;
+	std::vector<SampleCell> cells;
+	samples_of(ctx->p, n, shape, dims, smp, cnt, cells, ncell);
 	const int64_t lo = ctx->h_soff[first], hi = ctx->h_soff[first + n];
 	const int64_t m = (int64_t)(smp.size() / 3);
 	const bool last_emits = emit ? emit[n - 1] != 0 : true;
@@ -291,10 +386,11 @@ void replace_bodies(pdc_context* ctx, int first, int n, const int* shape, const 
 	PDC_HIP(hipStreamSynchronize(s));
 	for (int b = 0; b + 1 < n; ++b) ctx->h_soff[first + b + 1] = ctx->h_soff[first + b] + cnt[b];
 	if (!smp.empty())
-		PDC_HIP(hipMemcpyAsync(ctx->samples.p + 3 * lo, smp.data(), smp.size() * sizeof(double),
+		PDC_HIP(hipMemcpyAsync(ctx->samples.p + 3 * lo, smp.data(), smp.size() * sizeof(float),
 		                       hipMemcpyHostToDevice, s));
 	PDC_HIP(hipMemcpyAsync(ctx->b_soff.p + first, ctx->h_soff.data() + first, ((size_t)n + 1) * sizeof(int),
 	                       hipMemcpyHostToDevice, s));
+	upload_cells(ctx, first, n, cells, ncell);
 	write_bodies(ctx, first, n, shape, dims, mass, inertia, is_static, emit, false);
 	PDC_HIP(hipStreamSynchronize(s));
 	count_scene_classes(ctx);
@@ -440,9 +536,8 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_bp_write(int N, const double*
                                                           const int* __restrict__ bscene,
                                                           const int* __restrict__ sbeg,
                                                           const int* __restrict__ stepped,
-                                                          const int* __restrict__ off, int* __restrict__ pa,
-                                                          int* __restrict__ pb, const int* __restrict__ soff,
-                                                          const int* __restrict__ emit, int* __restrict__ stage_cnt) {
+                                                          const int* __restrict__ off, int cap,
+                                                          int* __restrict__ pa, int* __restrict__ pb) {
 	const int a = blockIdx.x * kBpWaves + (int)(threadIdx.x >> 6);
 	const int lane = threadIdx.x & 63;
 	if (a >= N) return;
@@ -452,49 +547,166 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_bp_write(int N, const double*
 	double A[6];
 	for (int k = 0; k < 6; ++k) A[k] = aabb[6 * (size_t)a + k];
 	const int sa = stat[a];
-	const int na = emit[a] ? soff[a + 1] - soff[a] : 0;
 	const unsigned long long below = (1ull << lane) - 1;
 	int o = off[a];
+	if (o >= cap) return;  // beyond the pair capacity: the host grows it and detects again
 	for (int base = a + 1; base < end; base += 64) {
 		const int b = base + lane;
 		const bool hit = bp_candidate(aabb, stat, A, sa, b, end);
 		const unsigned long long m = __ballot(hit);
 		if (hit) {
 			const int k = o + __popcll(m & below);
-			pa[k] = a;
-			pb[k] = b;
-			stage_cnt[k] = na + (emit[b] ? soff[b + 1] - soff[b] : 0);
+			if (k < cap) {
+				pa[k] = a;
+				pb[k] = b;
+			}
 		}
 		o += __popcll(m);
 	}
 }
 
+// ---------------------------------------------------------------------------
+// Sample pass of a pair (q's samples against g's grid).  The grid box of g
+// seen from q's body frame: centre c, axes A (A[r*3+k] = (Rq^T Rg)_rk, g's
+// axis k in q's frame), half extents h, and its q-frame box H.  A cell of q's
+// samples is kept unless one of the three axes of q or of g separates its box
+// from g's grid box by more than a margin far above the rounding of the exact
+// per-sample test, so the cull never drops a sample that test would take.
+// ---------------------------------------------------------------------------
+struct PassGeom {
+	double c[3], A[9], h[3], H[3], margin;
+};
+
+__device__ __forceinline__ void pass_geom(const double* Rq, const double* xq, const double* Rg, const double* xg,
+                                          const GridDesc& G, PassGeom& P) {
+	const double d[3] = {xg[0] - xq[0], xg[1] - xq[1], xg[2] - xq[2]};
+	for (int r = 0; r < 3; ++r) {
+		P.c[r] = Rq[0 * 3 + r] * d[0] + Rq[1 * 3 + r] * d[1] + Rq[2 * 3 + r] * d[2];
+		for (int k = 0; k < 3; ++k)
+			P.A[r * 3 + k] = Rq[0 * 3 + r] * Rg[0 * 3 + k] + Rq[1 * 3 + r] * Rg[1 * 3 + k] +
+			                 Rq[2 * 3 + r] * Rg[2 * 3 + k];
+	}
+	for (int k = 0; k < 3; ++k) P.h[k] = G.half[k];
+	for (int r = 0; r < 3; ++r)
+		P.H[r] = fabs(P.A[r * 3 + 0]) * P.h[0] + fabs(P.A[r * 3 + 1]) * P.h[1] + fabs(P.A[r * 3 + 2]) * P.h[2];
+	P.margin = 1e-6 * (1.0 + P.h[0] + P.h[1] + P.h[2] + fabs(P.c[0]) + fabs(P.c[1]) + fabs(P.c[2]));
+}
+
+__device__ __forceinline__ bool cell_overlaps(const PassGeom& P, const SampleCell& cell) {
+	double m[3], e[3], d[3];
+	for (int a = 0; a < 3; ++a) {
+		m[a] = 0.5 * ((double)cell.lo[a] + (double)cell.hi[a]);
+		e[a] = 0.5 * ((double)cell.hi[a] - (double)cell.lo[a]);
+		d[a] = P.c[a] - m[a];
+	}
+	for (int r = 0; r < 3; ++r)
+		if (fabs(d[r]) > e[r] + P.H[r] + P.margin) return false;
+	for (int k = 0; k < 3; ++k) {
+		const double pr = P.A[0 * 3 + k] * d[0] + P.A[1 * 3 + k] * d[1] + P.A[2 * 3 + k] * d[2];
+		const double rad = P.h[k] + fabs(P.A[0 * 3 + k]) * e[0] + fabs(P.A[1 * 3 + k]) * e[1] +
+		                   fabs(P.A[2 * 3 + k]) * e[2];
+		if (fabs(pr) > rad + P.margin) return false;
+	}
+	return true;
+}
+
+// Grid nodes of g under the box of q's grid (the partner's box) seen in g's
+// frame: the nodes a brick staged per pair and side would hold.
+__device__ __forceinline__ double brick_nodes(const PassGeom& P, const GridDesc& Gq, const GridDesc& Gg) {
+	double n = 1.0;
+	for (int k = 0; k < 3; ++k) {
+		// centre of q in g's frame is -A^T c; half extent sum_r |A_rk| hq_r
+		const double ck = -(P.A[0 * 3 + k] * P.c[0] + P.A[1 * 3 + k] * P.c[1] + P.A[2 * 3 + k] * P.c[2]);
+		const double hk = fabs(P.A[0 * 3 + k]) * Gq.half[0] + fabs(P.A[1 * 3 + k]) * Gq.half[1] +
+		                  fabs(P.A[2 * 3 + k]) * Gq.half[2];
+		const double lo = fmax(ck - hk, Gg.lo[k]), hi = fmin(ck + hk, -Gg.lo[k]);
+		if (!(hi >= lo)) return 0.0;
+		const double i0 = floor((lo - Gg.lo[k]) / Gg.sp[k]), i1 = ceil((hi - Gg.lo[k]) / Gg.sp[k]);
+		n *= fmin(i1, (double)(Gg.n - 1)) - fmax(i0, 0.0) + 1.0;
+	}
+	return n;
+}
+
 struct PairArgs {
 	const int* pa;
 	const int* pb;
+	const int* np;         // candidate pairs (may exceed cap)
+	int cap;               // pair capacity
 	const int* soff;       // per body sample offsets
+	const int* ncell;      // per body cells
+	const SampleCell* cells;
 	const int* emit;
-	const double* samples;
+	const float* samples;
 	const double* R;
 	const double* pos;
 	const GridDesc* grids;
 	const float* pool;
-	const int* stage_off;  // per pair
+	int* stage_cnt;        // per pair: samples the cull keeps (both sides)
+	const int* stage_off;  // per pair (P+1)
+	int64_t stage_cap;
 	double* stage_raw;     // 7 doubles per slot
 	double* stage_m;
+	int* stage_idx;        // 2 ints per slot
+	int* big;              // pairs with more than kSortCap hits
 	int* pair_nh;
 	int* pair_nc;
 	Scalars* sc;
 	double tie_band;
 };
 
-// Tests samples [s0, s0 + S) of body q against the grid of body g; appends hits.
-__device__ __forceinline__ bool sample_hit(const PairArgs& A, const double* Rq,
-                                           const double* xq, const double* Rg, const double* xg,
-                                           const GridDesc& G, int s, double sign, double* out7,
+__device__ __forceinline__ int pairs_of(const PairArgs& A) { return min(*A.np, A.cap); }
+
+// Culled sample count of one pair, one wavefront per pair: for each emitting
+// side the samples of the cells kept for the partner (all of a body without
+// cells); the sample and brick totals of the pair for the statistics.
+__global__ __launch_bounds__(64 * kBpWaves) void k_pair_cull(PairArgs A) {
+	const int lane = threadIdx.x & 63;
+	const int P = pairs_of(A);
+	for (int p = blockIdx.x * kBpWaves + (int)(threadIdx.x >> 6); p < P; p += gridDim.x * kBpWaves) {
+		const int a = A.pa[p], b = A.pb[p];
+		double Ra[9], Rb[9], xa[3], xb[3];
+		for (int k = 0; k < 9; ++k) { Ra[k] = A.R[9 * a + k]; Rb[k] = A.R[9 * b + k]; }
+		for (int k = 0; k < 3; ++k) { xa[k] = A.pos[3 * a + k]; xb[k] = A.pos[3 * b + k]; }
+		int kept = 0, all = 0;
+		double nodes = 0.0;
+		for (int pass = 0; pass < 2; ++pass) {
+			const int q = pass == 0 ? a : b, g = pass == 0 ? b : a;
+			if (!A.emit[q]) continue;
+			const int S = A.soff[q + 1] - A.soff[q];
+			all += S;
+			PassGeom G;
+			if (pass == 0) pass_geom(Ra, xa, Rb, xb, A.grids[g], G);
+			else pass_geom(Rb, xb, Ra, xa, A.grids[g], G);
+			nodes += brick_nodes(G, A.grids[q], A.grids[g]);
+			const int nc = A.ncell[q];
+			if (nc == 0) {
+				kept += S;
+				continue;
+			}
+			int c = 0;
+			if (lane < nc) {
+				const SampleCell cl = A.cells[A.soff[q] / kCellStride + lane];
+				if (cell_overlaps(G, cl)) c = cl.end - cl.beg;
+			}
+			for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+			kept += c;
+		}
+		if (lane == 0) {
+			A.stage_cnt[p] = kept;
+			atomicAdd(&A.sc->all_samples, (unsigned long long)all);
+			atomicAdd(&A.sc->read_samples, (unsigned long long)kept);
+			atomicAdd(&A.sc->brick_nodes, (unsigned long long)nodes);
+		}
+	}
+}
+
+// Tests sample s (FP32, body frame of q) against the grid of g; on a hit
+// writes position, normal and depth.
+__device__ __forceinline__ bool sample_hit(const PairArgs& A, const double* Rq, const double* xq, const double* Rg,
+                                           const double* xg, const GridDesc& G, int s, double sign, double* out7,
                                            int* tie) {
-	const double* l = &A.samples[3 * (size_t)s];
-	const double l0 = l[0], l1 = l[1], l2 = l[2];
+	const float* l = &A.samples[3 * (size_t)s];
+	const double l0 = (double)l[0], l1 = (double)l[1], l2 = (double)l[2];
 	double p[3];
 	for (int r = 0; r < 3; ++r) p[r] = Rq[r * 3 + 0] * l0 + Rq[r * 3 + 1] * l1 + Rq[r * 3 + 2] * l2 + xq[r];
 	const double d[3] = {p[0] - xg[0], p[1] - xg[1], p[2] - xg[2]};
@@ -527,171 +739,353 @@ __device__ __forceinline__ bool pos_less(double ax, double ay, double az, double
 	return az < bz;
 }
 
-// Sample pass: one 256-thread workgroup per candidate pair; hits are compacted
-// in sample order (a's samples first, then b's) into the pair's staging slots.
+// Sample pass: one 256-thread workgroup per candidate pair.  Per side, wave 0
+// culls the cells and lists the kept ones with their sample prefix; the
+// workgroup then tests the kept samples 256 at a time, compacting hits in
+// (side, cell, sample) order into the pair's staging slots.
 __global__ __launch_bounds__(256) void k_pair_hits(PairArgs A) {
 	__shared__ int wcnt[4];
 	__shared__ int s_nh, s_ties;
-	const int p = blockIdx.x;
+	__shared__ int sel_beg[64], pre[65];
+	__shared__ int s_nsel;
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-	const int a = A.pa[p], b = A.pb[p];
-	double Ra[9], Rb[9], xa[3], xb[3];
-	for (int k = 0; k < 9; ++k) { Ra[k] = A.R[9 * a + k]; Rb[k] = A.R[9 * b + k]; }
-	for (int k = 0; k < 3; ++k) { xa[k] = A.pos[3 * a + k]; xb[k] = A.pos[3 * b + k]; }
-	const GridDesc Ga = A.grids[a], Gb = A.grids[b];
-	const int base = A.stage_off[p];
-	if (tid == 0) { s_nh = 0; s_ties = 0; }
-	__syncthreads();
-	int myties = 0;
-	for (int pass = 0; pass < 2; ++pass) {
-		const int qb = pass == 0 ? a : b;
-		if (!A.emit[qb]) continue;
-		const int s0 = A.soff[qb], S = A.soff[qb + 1] - s0;
-		for (int c0 = 0; c0 < S; c0 += 256) {
-			const int s = c0 + tid;
-			double h7[7];
-			bool hit = false;
-			if (s < S) {
-				if (pass == 0) hit = sample_hit(A, Ra, xa, Rb, xb, Gb, s0 + s, -1.0, h7, &myties);
-				else hit = sample_hit(A, Rb, xb, Ra, xa, Ga, s0 + s, +1.0, h7, &myties);
+	const int P = pairs_of(A);
+	for (int p = blockIdx.x; p < P; p += gridDim.x) {
+		const int a = A.pa[p], b = A.pb[p];
+		const int base = A.stage_off[p];
+		if ((int64_t)A.stage_off[p + 1] > A.stage_cap) {
+			// staging too small: the host grows it and detects again
+			if (tid == 0) A.pair_nh[p] = 0;
+			continue;
+		}
+		double Ra[9], Rb[9], xa[3], xb[3];
+		for (int k = 0; k < 9; ++k) { Ra[k] = A.R[9 * a + k]; Rb[k] = A.R[9 * b + k]; }
+		for (int k = 0; k < 3; ++k) { xa[k] = A.pos[3 * a + k]; xb[k] = A.pos[3 * b + k]; }
+		__syncthreads();
+		if (tid == 0) { s_nh = 0; s_ties = 0; }
+		int myties = 0;
+		for (int pass = 0; pass < 2; ++pass) {
+			const int q = pass == 0 ? a : b, g = pass == 0 ? b : a;
+			if (!A.emit[q]) continue;
+			const GridDesc Gg = A.grids[g];
+			const double* Rq = pass == 0 ? Ra : Rb;
+			const double* xq = pass == 0 ? xa : xb;
+			const double* Rg = pass == 0 ? Rb : Ra;
+			const double* xg = pass == 0 ? xb : xa;
+			const int s0 = A.soff[q];
+			const int nc = A.ncell[q];
+			if (wid == 0) {
+				if (nc == 0) {
+					if (lane == 0) {
+						sel_beg[0] = 0;
+						pre[0] = 0;
+						pre[1] = A.soff[q + 1] - s0;
+						s_nsel = 1;
+					}
+				} else {
+					PassGeom G;
+					pass_geom(Rq, xq, Rg, xg, Gg, G);
+					bool keep = false;
+					SampleCell cl{};
+					if (lane < nc) {
+						cl = A.cells[s0 / kCellStride + lane];
+						keep = cell_overlaps(G, cl);
+					}
+					const unsigned long long m = __ballot(keep);
+					const int cnt = keep ? cl.end - cl.beg : 0;
+					int x = cnt;
+					for (int off = 1; off < 64; off <<= 1) {
+						const int y = __shfl_up(x, off, 64);
+						if (lane >= off) x += y;
+					}
+					if (keep) {
+						const int r = __popcll(m & ((1ull << lane) - 1));
+						sel_beg[r] = cl.beg;
+						pre[r + 1] = x;
+					}
+					if (lane == 0) {
+						pre[0] = 0;
+						s_nsel = __popcll(m);
+					}
+				}
 			}
-			const unsigned long long m = __ballot(hit);
-			const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-			const int rank = __popcll(m & lt);
-			if (lane == 0) wcnt[wid] = __popcll(m);
 			__syncthreads();
-			int before = 0;
-			for (int w = 0; w < wid; ++w) before += wcnt[w];
-			const int total = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-			const int nh0 = s_nh;
-			if (hit) {
-				double* o = &A.stage_raw[7 * (size_t)(base + nh0 + before + rank)];
-				for (int k = 0; k < 7; ++k) o[k] = h7[k];
+			const int nsel = s_nsel;
+			const int T = pre[nsel];
+			const double sign = pass == 0 ? -1.0 : 1.0;
+			for (int c0 = 0; c0 < T; c0 += 256) {
+				const int t = c0 + tid;
+				double h7[7];
+				bool hit = false;
+				if (t < T) {
+					int lo = 0, hi = nsel - 1;  // the cell j with pre[j] <= t < pre[j + 1]
+					while (lo < hi) {
+						const int mid = (lo + hi + 1) >> 1;
+						if (pre[mid] <= t) lo = mid;
+						else hi = mid - 1;
+					}
+					const int smp = s0 + sel_beg[lo] + (t - pre[lo]);
+					hit = sample_hit(A, Rq, xq, Rg, xg, Gg, smp, sign, h7, &myties);
+				}
+				const unsigned long long m = __ballot(hit);
+				const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+				const int rank = __popcll(m & lt);
+				if (lane == 0) wcnt[wid] = __popcll(m);
+				__syncthreads();
+				int before = 0;
+				for (int w = 0; w < wid; ++w) before += wcnt[w];
+				const int total = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+				const int nh0 = s_nh;
+				if (hit) {
+					double* o = &A.stage_raw[7 * (size_t)(base + nh0 + before + rank)];
+					for (int k = 0; k < 7; ++k) o[k] = h7[k];
+				}
+				__syncthreads();
+				if (tid == 0) s_nh = nh0 + total;
+				__syncthreads();
 			}
-			__syncthreads();
-			if (tid == 0) s_nh = nh0 + total;
 			__syncthreads();
 		}
-	}
-	if (myties) atomicAdd(&s_ties, myties);
-	__syncthreads();
-	if (tid == 0) {
-		const int nh = s_nh;
-		A.pair_nh[p] = nh;
-		if (s_ties) atomicAdd(&A.sc->ties, s_ties);
-		if (nh) atomicAdd(&A.sc->n_raw, nh);
-		if (nh > kSortCap) atomicOr(&A.sc->overflow, 1);
+		if (myties) atomicAdd(&s_ties, myties);
+		__syncthreads();
+		if (tid == 0) {
+			const int nh = s_nh;
+			A.pair_nh[p] = nh;
+			if (s_ties) atomicAdd(&A.sc->ties, s_ties);
+			if (nh) atomicAdd(&A.sc->n_raw, nh);
+			if (nh > kSortCap) A.big[atomicAdd(&A.sc->n_big, 1)] = p;
+		}
 	}
 }
 
-// Merge pass: one 256-thread workgroup per pair with hits; sorts the hits by
-// position (bitonic in LDS) and chain-merges them.
+// Merged record of the sorted hits [i, j) of a pair (`at(k)`: the staging
+// slot of the k-th sorted hit): the first position, the normalised sum of the
+// members' frame normals, the minimum depth (DynamicSystem.cpp:364-381).
+template <class At>
+__device__ __forceinline__ void merge_group(const double* __restrict__ raw, int i, int j, At at, double* out) {
+	const double* o = &raw[7 * (size_t)at(i)];
+	for (int k = 0; k < 7; ++k) out[k] = o[k];
+	if (j - i == 1) return;
+	double Rr[9];
+	frame_rotation(&o[3], Rr);
+	double n[3] = {Rr[0], Rr[3], Rr[6]};
+	double dep = o[6];
+	for (int k = i + 1; k < j; ++k) {
+		const double* ok = &raw[7 * (size_t)at(k)];
+		frame_rotation(&ok[3], Rr);
+		n[0] += Rr[0]; n[1] += Rr[3]; n[2] += Rr[6];
+		dep = ok[6] < dep ? ok[6] : dep;
+	}
+	const double nn = sqrt(dot3(n, n));
+	if (nn > 0) { n[0] /= nn; n[1] /= nn; n[2] /= nn; }
+	out[3] = n[0]; out[4] = n[1]; out[5] = n[2];
+	out[6] = dep;
+}
+
+// A new group starts at sorted hit i where |p_i - p_{i-1}|^2 > 1e-8
+// (DynamicSystem.cpp:356-363).
+__device__ __forceinline__ bool group_break(double x0, double y0, double z0, double x1, double y1, double z1) {
+	const double d0 = x1 - x0, d1 = y1 - y0, d2 = z1 - z0;
+	return d0 * d0 + d1 * d1 + d2 * d2 > 1e-8;
+}
+
+// Merge pass: one 256-thread workgroup per pair with 2..kSortCap hits; sorts
+// the hits by position (bitonic in LDS) and chain-merges them.
 __global__ __launch_bounds__(256) void k_pair_merge(PairArgs A) {
 	__shared__ double kx[kSortCap], ky[kSortCap], kz[kSortCap];
 	__shared__ short kidx[kSortCap];
 	__shared__ unsigned char brk[kSortCap];
 	__shared__ int gidx[kSortCap];
 	__shared__ int scan_part[256];
-	const int p = blockIdx.x;
 	const int tid = threadIdx.x;
-	const int nh = A.pair_nh[p];
-	if (nh == 0 || nh > kSortCap) {
-		if (tid == 0) A.pair_nc[p] = 0;
-		return;
-	}
-	const int base = A.stage_off[p];
-	if (nh == 1) {
-		if (tid < 7) A.stage_m[7 * (size_t)base + tid] = A.stage_raw[7 * (size_t)base + tid];
-		if (tid == 0) A.pair_nc[p] = 1;
-		return;
-	}
-	int M = 1;
-	while (M < nh) M <<= 1;
-	for (int i = tid; i < M; i += 256) {
-		if (i < nh) {
-			const double* o = &A.stage_raw[7 * (size_t)(base + i)];
-			kx[i] = o[0]; ky[i] = o[1]; kz[i] = o[2];
-		} else {
-			kx[i] = INFINITY; ky[i] = INFINITY; kz[i] = INFINITY;
+	const int P = pairs_of(A);
+	for (int p = blockIdx.x; p < P; p += gridDim.x) {
+		const int nh = A.pair_nh[p];
+		if (nh > kSortCap) continue;  // the global-memory merge takes it
+		if (nh == 0) {
+			if (tid == 0) A.pair_nc[p] = 0;
+			continue;
 		}
-		kidx[i] = (short)i;
-	}
-	__syncthreads();
-	for (int k = 2; k <= M; k <<= 1) {
-		for (int j = k >> 1; j > 0; j >>= 1) {
-			for (int i = tid; i < M; i += 256) {
-				const int l = i ^ j;
-				if (l > i) {
-					const bool up = (i & k) == 0;
-					const bool sw = up ? pos_less(kx[l], ky[l], kz[l], kx[i], ky[i], kz[i])
-					                   : pos_less(kx[i], ky[i], kz[i], kx[l], ky[l], kz[l]);
-					if (sw) {
-						double t;
-						t = kx[i]; kx[i] = kx[l]; kx[l] = t;
-						t = ky[i]; ky[i] = ky[l]; ky[l] = t;
-						t = kz[i]; kz[i] = kz[l]; kz[l] = t;
-						const short ti = kidx[i]; kidx[i] = kidx[l]; kidx[l] = ti;
+		const int base = A.stage_off[p];
+		if (nh == 1) {
+			if (tid < 7) A.stage_m[7 * (size_t)base + tid] = A.stage_raw[7 * (size_t)base + tid];
+			if (tid == 0) A.pair_nc[p] = 1;
+			continue;
+		}
+		__syncthreads();
+		int M = 1;
+		while (M < nh) M <<= 1;
+		for (int i = tid; i < M; i += 256) {
+			if (i < nh) {
+				const double* o = &A.stage_raw[7 * (size_t)(base + i)];
+				kx[i] = o[0]; ky[i] = o[1]; kz[i] = o[2];
+			} else {
+				kx[i] = INFINITY; ky[i] = INFINITY; kz[i] = INFINITY;
+			}
+			kidx[i] = (short)i;
+		}
+		__syncthreads();
+		for (int k = 2; k <= M; k <<= 1) {
+			for (int j = k >> 1; j > 0; j >>= 1) {
+				for (int i = tid; i < M; i += 256) {
+					const int l = i ^ j;
+					if (l > i) {
+						const bool up = (i & k) == 0;
+						const bool sw = up ? pos_less(kx[l], ky[l], kz[l], kx[i], ky[i], kz[i])
+						                   : pos_less(kx[i], ky[i], kz[i], kx[l], ky[l], kz[l]);
+						if (sw) {
+							double t;
+							t = kx[i]; kx[i] = kx[l]; kx[l] = t;
+							t = ky[i]; ky[i] = ky[l]; ky[l] = t;
+							t = kz[i]; kz[i] = kz[l]; kz[l] = t;
+							const short ti = kidx[i]; kidx[i] = kidx[l]; kidx[l] = ti;
+						}
 					}
+				}
+				__syncthreads();
+			}
+		}
+		for (int i = tid; i < nh; i += 256)
+			brk[i] = (i == 0 || group_break(kx[i - 1], ky[i - 1], kz[i - 1], kx[i], ky[i], kz[i])) ? 1 : 0;
+		__syncthreads();
+		{
+			int loc = 0;
+			const int b0 = tid * 8;
+			for (int k = 0; k < 8; ++k)
+				if (b0 + k < nh) loc += brk[b0 + k];
+			scan_part[tid] = loc;
+			__syncthreads();
+			if (tid == 0) {
+				int run = 0;
+				for (int t = 0; t < 256; ++t) { const int v = scan_part[t]; scan_part[t] = run; run += v; }
+			}
+			__syncthreads();
+			int run = scan_part[tid];
+			for (int k = 0; k < 8; ++k)
+				if (b0 + k < nh) { gidx[b0 + k] = run; run += brk[b0 + k]; }
+		}
+		__syncthreads();
+		const double* raw = A.stage_raw;
+		auto at = [&](int k) { return base + (int)kidx[k]; };
+		for (int i = tid; i < nh; i += 256) {
+			if (!brk[i]) continue;
+			int j = i + 1;
+			while (j < nh && !brk[j]) ++j;
+			double out[7];
+			merge_group(raw, i, j, at, out);
+			double* w = &A.stage_m[7 * (size_t)(base + gidx[i])];
+			for (int k = 0; k < 7; ++k) w[k] = out[k];
+		}
+		if (tid == 0) A.pair_nc[p] = gidx[nh - 1] + brk[nh - 1];
+		__syncthreads();
+	}
+}
+
+// Merge of a pair with more than kSortCap hits, one 256-thread workgroup per
+// such pair: a stable bottom-up merge sort of the hit indices by position in
+// global memory (the pair's two index arrays in stage_idx), then the same
+// chain merge.  For distinct positions the order equals the bitonic sort's.
+__device__ __forceinline__ bool hit_less(const double* __restrict__ raw, int base, int u, int v) {
+	const double* a = &raw[7 * (size_t)(base + u)];
+	const double* b = &raw[7 * (size_t)(base + v)];
+	return pos_less(a[0], a[1], a[2], b[0], b[1], b[2]);
+}
+
+__global__ __launch_bounds__(256) void k_pair_merge_big(PairArgs A) {
+	__shared__ int part[256];
+	__shared__ int carry;
+	const int tid = threadIdx.x;
+	const int nbig = A.sc->n_big;
+	const double* raw = A.stage_raw;
+	for (int bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
+		const int p = A.big[bi];
+		const int nh = A.pair_nh[p];
+		const int base = A.stage_off[p];
+		int* I0 = A.stage_idx + 2 * (size_t)base;
+		int* I1 = I0 + nh;
+		for (int i = tid; i < nh; i += 256) I0[i] = i;
+		__syncthreads();
+		int* src = I0;
+		int* dst = I1;
+		for (int w = 1; w < nh; w <<= 1) {
+			for (int e = tid; e < nh; e += 256) {
+				const int L = (e / (2 * w)) * (2 * w);
+				const int mid = min(L + w, nh), R = min(L + 2 * w, nh);
+				const int v = src[e];
+				int lo, hi;
+				if (e < mid) {  // left run: place after the right elements strictly less
+					lo = mid; hi = R;
+					while (lo < hi) {
+						const int m = (lo + hi) >> 1;
+						if (hit_less(raw, base, src[m], v)) lo = m + 1;
+						else hi = m;
+					}
+					dst[e + (lo - mid)] = v;
+				} else {        // right run: place after the left elements not greater
+					lo = L; hi = mid;
+					while (lo < hi) {
+						const int m = (lo + hi) >> 1;
+						if (!hit_less(raw, base, v, src[m])) lo = m + 1;
+						else hi = m;
+					}
+					dst[L + (e - mid) + (lo - L)] = v;
 				}
 			}
 			__syncthreads();
+			int* t = src; src = dst; dst = t;
 		}
-	}
-	// a group starts where |p_i - p_{i-1}|^2 > 1e-8 (DynamicSystem.cpp:356-363)
-	for (int i = tid; i < nh; i += 256) {
-		unsigned char f = 1;
-		if (i > 0) {
-			const double d0 = kx[i] - kx[i - 1], d1 = ky[i] - ky[i - 1], d2 = kz[i] - kz[i - 1];
-			f = (d0 * d0 + d1 * d1 + d2 * d2 > 1e-8) ? 1 : 0;
-		}
-		brk[i] = f;
-	}
-	__syncthreads();
-	{
-		int loc = 0;
-		const int b0 = tid * 8;
-		for (int k = 0; k < 8; ++k)
-			if (b0 + k < nh) loc += brk[b0 + k];
-		scan_part[tid] = loc;
+		// src: sorted indices; dst (free): group number of each group start
+		auto key = [&](int i, double* k3) {
+			const double* o = &raw[7 * (size_t)(base + src[i])];
+			k3[0] = o[0]; k3[1] = o[1]; k3[2] = o[2];
+		};
+		auto is_break = [&](int i) {
+			if (i == 0) return true;
+			double u[3], v[3];
+			key(i - 1, u);
+			key(i, v);
+			return group_break(u[0], u[1], u[2], v[0], v[1], v[2]);
+		};
+		if (tid == 0) carry = 0;
 		__syncthreads();
-		if (tid == 0) {
-			int run = 0;
-			for (int t = 0; t < 256; ++t) { const int v = scan_part[t]; scan_part[t] = run; run += v; }
-		}
-		__syncthreads();
-		int run = scan_part[tid];
-		for (int k = 0; k < 8; ++k)
-			if (b0 + k < nh) { gidx[b0 + k] = run; run += brk[b0 + k]; }
-	}
-	__syncthreads();
-	// group = first contact of the run; normal = normalize(sum of the members'
-	// frame normals), depth = min (DynamicSystem.cpp:364-381)
-	for (int i = tid; i < nh; i += 256) {
-		if (!brk[i]) continue;
-		int j = i + 1;
-		while (j < nh && !brk[j]) ++j;
-		const double* o = &A.stage_raw[7 * (size_t)(base + kidx[i])];
-		double out[7] = {o[0], o[1], o[2], o[3], o[4], o[5], o[6]};
-		if (j - i > 1) {
-			double Rr[9];
-			frame_rotation(&o[3], Rr);
-			double n[3] = {Rr[0], Rr[3], Rr[6]};
-			double dep = o[6];
-			for (int k = i + 1; k < j; ++k) {
-				const double* ok = &A.stage_raw[7 * (size_t)(base + kidx[k])];
-				frame_rotation(&ok[3], Rr);
-				n[0] += Rr[0]; n[1] += Rr[3]; n[2] += Rr[6];
-				dep = ok[6] < dep ? ok[6] : dep;
+		for (int c0 = 0; c0 < nh; c0 += 256) {
+			const int i = c0 + tid;
+			const int f = (i < nh && is_break(i)) ? 1 : 0;
+			part[tid] = f;
+			__syncthreads();
+			for (int off = 1; off < 256; off <<= 1) {
+				const int y = tid >= off ? part[tid - off] : 0;
+				__syncthreads();
+				part[tid] += y;
+				__syncthreads();
 			}
-			const double nn = sqrt(dot3(n, n));
-			if (nn > 0) { n[0] /= nn; n[1] /= nn; n[2] /= nn; }
-			out[3] = n[0]; out[4] = n[1]; out[5] = n[2];
-			out[6] = dep;
+			if (f) dst[i] = carry + part[tid] - 1;
+			__syncthreads();
+			if (tid == 0) carry += part[255];
+			__syncthreads();
 		}
-		double* w = &A.stage_m[7 * (size_t)(base + gidx[i])];
-		for (int k = 0; k < 7; ++k) w[k] = out[k];
+		auto at = [&](int k) { return base + src[k]; };
+		for (int i = tid; i < nh; i += 256) {
+			if (!is_break(i)) continue;
+			int j = i + 1;
+			while (j < nh && !is_break(j)) ++j;
+			double out[7];
+			merge_group(raw, i, j, at, out);
+			double* w = &A.stage_m[7 * (size_t)(base + dst[i])];
+			for (int k = 0; k < 7; ++k) w[k] = out[k];
+		}
+		if (tid == 0) A.pair_nc[p] = carry;
+		__syncthreads();
 	}
-	if (tid == 0) A.pair_nc[p] = gidx[nh - 1] + brk[nh - 1];
+}
+
+// Totals of the detection for the host's single read-back.
+__global__ void k_contact_totals(PairArgs A, const int* __restrict__ coff) {
+	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+	const int P = pairs_of(A);
+	A.sc->n_pairs = *A.np;
+	A.sc->n_stage = A.stage_off[P];
+	A.sc->n_contacts = coff[P];
 }
 
 // One workgroup per pair: final contact records in pair order.
@@ -727,12 +1121,22 @@ __global__ __launch_bounds__(256) void k_pair_write(const int* __restrict__ pa, 
 	}
 }
 
+// Contact detection with one read-back: the pair list and the staging slots
+// live in capacities kept across steps; a kernel that finds its capacity too
+// small leaves its part undone, and the host, seeing the totals, grows the
+// capacity and detects again (on a growing scene only).
 void detect_contacts(pdc_context* ctx) {
 	hipStream_t s = ctx->stream;
 	const int N = ctx->N;
 	ctx->C = 0;
+	ctx->det = DetectionStats{};
+	ctx->n_raw = 0;
+	ctx->n_ties = 0;
 	PDC_HIP(hipMemsetAsync(ctx->d_sc.p, 0, sizeof(Scalars), s));
-	if (N < 2) return;
+	if (N < 2) {
+		ctx->n_pairs = 0;
+		return;
+	}
 	hipLaunchKernelGGL(k_body_frames, dim3(grid_for(N)), dim3(kBlock), 0, s, N, ctx->b_quat.p,
 	                   ctx->b_pos.p, ctx->b_grid.p, ctx->b_R.p, ctx->b_aabb.p);
 	PDC_CHECK_LAUNCH();
@@ -743,50 +1147,82 @@ void detect_contacts(pdc_context* ctx) {
 	                   ctx->b_static.p, ctx->b_scene.p, ctx->d_sbeg.p, ctx->d_step.p, ctx->bp_cnt.p);
 	PDC_CHECK_LAUNCH();
 	scan_exclusive(ctx, ctx->bp_cnt.p, ctx->bp_off.p, N);
-	const int P = read_int(ctx, ctx->bp_off.p + N);
-	if (P == 0) return;
-	ctx->pr_a.ensure(P); ctx->pr_b.ensure(P); ctx->pr_tmp.ensure(P + 1);
-	ctx->pr_soff.ensure(P + 1); ctx->pr_nc.ensure(P + 1); ctx->pr_coff.ensure(P + 1);
-	ctx->pr_nh.ensure(P + 1);
-	hipLaunchKernelGGL(k_bp_write, dim3(bp_grid), dim3(64 * kBpWaves), 0, s, N, ctx->b_aabb.p,
-	                   ctx->b_static.p, ctx->b_scene.p, ctx->d_sbeg.p, ctx->d_step.p, ctx->bp_off.p,
-	                   ctx->pr_a.p, ctx->pr_b.p, ctx->b_soff.p, ctx->b_emit.p, ctx->pr_tmp.p);
-	PDC_CHECK_LAUNCH();
-	scan_exclusive(ctx, ctx->pr_tmp.p, ctx->pr_soff.p, P);
-	const int S = read_int(ctx, ctx->pr_soff.p + P);
-	ctx->stage_raw.ensure(7 * (size_t)std::max(S, 1));
-	ctx->stage_m.ensure(7 * (size_t)std::max(S, 1));
-	PairArgs A;
-	A.pa = ctx->pr_a.p; A.pb = ctx->pr_b.p; A.soff = ctx->b_soff.p; A.emit = ctx->b_emit.p;
-	A.samples = ctx->samples.p; A.R = ctx->b_R.p; A.pos = ctx->b_pos.p; A.grids = ctx->b_grid.p;
-	A.pool = ctx->grid_pool.p; A.stage_off = ctx->pr_soff.p; A.stage_raw = ctx->stage_raw.p;
-	A.stage_m = ctx->stage_m.p; A.pair_nc = ctx->pr_nc.p; A.sc = ctx->d_sc.p;
-	A.tie_band = ctx->tie_band;
-	A.pair_nh = ctx->pr_nh.p;
-	hipEvent_t e0;
-	kt_begin(ctx, PDC_KT_CONTACTS, &e0);
-	hipLaunchKernelGGL(k_pair_hits, dim3(P), dim3(256), 0, s, A);
-	PDC_CHECK_LAUNCH();
-	kt_end(ctx, PDC_KT_CONTACTS, e0);
-	// per tested sample: its body-frame position (24 B) and the 8 FP32 nodes of
-	// its grid cell (32 B); per hit: position, normal, depth written (56 B)
-	if (e0) ctx->kt.pending_contact_slots = S;
-	hipLaunchKernelGGL(k_pair_merge, dim3(P), dim3(256), 0, s, A);
-	PDC_CHECK_LAUNCH();
-	scan_exclusive(ctx, ctx->pr_nc.p, ctx->pr_coff.p, P);
-	sync_scalars(ctx);
-	if (ctx->h_sc->overflow)
-		throw HipError("a body pair produced more than 2048 sample hits; use a coarser sample_spacing",
-		               PDC_ERR_CAPACITY);
-	const int C = read_int(ctx, ctx->pr_coff.p + P);
+	if (ctx->pr_cap < 1) ctx->pr_cap = std::max(1024, 4 * N);
+	if (ctx->stage_cap < 1) ctx->stage_cap = std::max<int64_t>(1 << 16, 64 * (int64_t)N);
+	PairArgs A{};
+	for (int attempt = 0;; ++attempt) {
+		const int cap = ctx->pr_cap;
+		ctx->pr_a.ensure(cap); ctx->pr_b.ensure(cap); ctx->pr_tmp.ensure(cap + 1);
+		ctx->pr_soff.ensure(cap + 1); ctx->pr_nc.ensure(cap + 1); ctx->pr_coff.ensure(cap + 1);
+		ctx->pr_nh.ensure(cap + 1); ctx->pr_big.ensure(cap);
+		ctx->stage_raw.ensure(7 * (size_t)ctx->stage_cap);
+		ctx->stage_m.ensure(7 * (size_t)ctx->stage_cap);
+		ctx->stage_idx.ensure(2 * (size_t)ctx->stage_cap);
+		if (attempt > 0) PDC_HIP(hipMemsetAsync(ctx->d_sc.p, 0, sizeof(Scalars), s));
+		hipLaunchKernelGGL(k_bp_write, dim3(bp_grid), dim3(64 * kBpWaves), 0, s, N, ctx->b_aabb.p,
+		                   ctx->b_static.p, ctx->b_scene.p, ctx->d_sbeg.p, ctx->d_step.p, ctx->bp_off.p, cap,
+		                   ctx->pr_a.p, ctx->pr_b.p);
+		PDC_CHECK_LAUNCH();
+		A.pa = ctx->pr_a.p; A.pb = ctx->pr_b.p; A.np = ctx->bp_off.p + N; A.cap = cap;
+		A.soff = ctx->b_soff.p; A.ncell = ctx->b_ncell.p; A.cells = ctx->cells.p; A.emit = ctx->b_emit.p;
+		A.samples = ctx->samples.p; A.R = ctx->b_R.p; A.pos = ctx->b_pos.p; A.grids = ctx->b_grid.p;
+		A.pool = ctx->grid_pool.p; A.stage_cnt = ctx->pr_tmp.p; A.stage_off = ctx->pr_soff.p;
+		A.stage_cap = ctx->stage_cap; A.stage_raw = ctx->stage_raw.p; A.stage_m = ctx->stage_m.p;
+		A.stage_idx = ctx->stage_idx.p; A.big = ctx->pr_big.p; A.pair_nh = ctx->pr_nh.p;
+		A.pair_nc = ctx->pr_nc.p; A.sc = ctx->d_sc.p; A.tie_band = ctx->tie_band;
+		// launch width from the last detection; the kernels stride over the
+		// pairs there are, so the width only sets the parallelism
+		const int hint = std::min(cap, std::max(256, ctx->n_pairs + ctx->n_pairs / 8 + 64));
+		hipEvent_t e0;
+		kt_begin(ctx, PDC_KT_CONTACTS, &e0);
+		hipLaunchKernelGGL(k_pair_cull, dim3((hint + kBpWaves - 1) / kBpWaves), dim3(64 * kBpWaves), 0, s, A);
+		PDC_CHECK_LAUNCH();
+		scan_exclusive_dev(ctx, ctx->pr_tmp.p, ctx->pr_soff.p, cap, A.np);
+		hipLaunchKernelGGL(k_pair_hits, dim3(hint), dim3(256), 0, s, A);
+		PDC_CHECK_LAUNCH();
+		kt_end(ctx, PDC_KT_CONTACTS, e0);
+		if (e0) ctx->kt.pending_contact_slots = 1;
+		hipLaunchKernelGGL(k_pair_merge, dim3(hint), dim3(256), 0, s, A);
+		PDC_CHECK_LAUNCH();
+		hipLaunchKernelGGL(k_pair_merge_big, dim3(64), dim3(256), 0, s, A);
+		PDC_CHECK_LAUNCH();
+		scan_exclusive_dev(ctx, ctx->pr_nc.p, ctx->pr_coff.p, cap, A.np);
+		hipLaunchKernelGGL(k_contact_totals, dim3(1), dim3(64), 0, s, A, (const int*)ctx->pr_coff.p);
+		PDC_CHECK_LAUNCH();
+		sync_scalars(ctx);
+		const Scalars& h = *ctx->h_sc;
+		bool again = false;
+		if (h.n_pairs > ctx->pr_cap) {
+			ctx->pr_cap = h.n_pairs + h.n_pairs / 4 + 64;
+			again = true;
+		} else if ((int64_t)h.n_stage > ctx->stage_cap) {
+			ctx->stage_cap = (int64_t)h.n_stage + h.n_stage / 4 + 1024;
+			again = true;
+		}
+		if (!again) break;
+		ctx->kt.pending_contact_slots = 0;
+		if (attempt >= 3) throw HipError("contact detection capacities did not settle", PDC_ERR_CAPACITY);
+	}
+	const Scalars& h = *ctx->h_sc;
+	const int P = h.n_pairs;
+	const int C = h.n_contacts;
 	ctx->n_pairs = P;
-	ctx->n_raw = ctx->h_sc->n_raw;
+	ctx->n_raw = h.n_raw;
+	ctx->n_ties = h.ties;
+	ctx->det.pairs = P;
+	ctx->det.raw_hits = h.n_raw;
+	ctx->det.ties = h.ties;
+	ctx->det.samples_all = (int64_t)h.all_samples;
+	ctx->det.samples_read = (int64_t)h.read_samples;
+	ctx->det.brick_nodes = (int64_t)h.brick_nodes;
+	ctx->det.large_pairs = h.n_big;
 	if (ctx->kt.pending_contact_slots) {
-		ctx->kt.bytes += 56.0 * ctx->kt.pending_contact_slots + 56.0 * ctx->n_raw;
+		// the sample side of (a): every sample of both emitting sides of every
+		// pair in FP32, and the grid nodes under each partner's box
+		ctx->kt.bytes += 12.0 * (double)h.all_samples + 4.0 * (double)h.brick_nodes;
 		ctx->kt.pending_contact_slots = 0;
 	}
-	ctx->n_ties = ctx->h_sc->ties;
-	if (C == 0) return;
+	if (C == 0 || P == 0) return;
 	ensure_contact_capacity(ctx, C);
 	ctx->C = C;
 	hipLaunchKernelGGL(k_pair_write, dim3(P), dim3(64), 0, s, ctx->pr_a.p, ctx->pr_b.p,

@@ -17,7 +17,8 @@ namespace pdc {
 
 constexpr int kBlock = 256;            // threads per workgroup of the 1-D kernels
 constexpr int kMaxPartialBlocks = 4096;  // workgroups whose partial sums a reduction reads
-constexpr int kSortCap = 2048;         // raw sample hits one pair can sort in LDS
+constexpr int kSortCap = 2048;         // raw sample hits one pair sorts in LDS (more: in global memory)
+constexpr int kCellStride = 8;         // body b's sample cells start at floor(soff[b] / 8)
 
 struct HipError : std::runtime_error {
 	int code;
@@ -235,12 +236,30 @@ struct PcgProgress {
 
 // Detection counters (one device struct).
 struct Scalars {
-	int overflow;        // capacity overflow flag set by kernels
-	int n_pairs;         // broadphase candidate pairs
+	int n_pairs;         // broadphase candidate pairs (may exceed the pair capacity)
+	int n_stage;         // staging slots the pairs' culled samples need (may exceed its capacity)
 	int n_contacts;      // contacts after merge
 	int n_raw;           // raw sample hits
 	int ties;
-	int pad[3];
+	int n_big;           // pairs with more than kSortCap hits (merged in global memory)
+	int pad[2];
+	unsigned long long all_samples;   // samples of the emitting sides of every pair
+	unsigned long long read_samples;  // samples in the cells the cull keeps
+	unsigned long long brick_nodes;   // grid nodes under the partner's box, per pair and side
+	unsigned long long pad2;
+};
+
+// Counts of the last contact detection (pdc_get_detection_stats).
+struct DetectionStats {
+	int64_t pairs = 0, raw_hits = 0, ties = 0;
+	int64_t samples_all = 0, samples_read = 0, brick_nodes = 0, large_pairs = 0;
+};
+
+// Body-frame sample cell: the box of its FP32 samples and their range
+// [beg, end) relative to the body's first sample.
+struct SampleCell {
+	float lo[3], hi[3];
+	int beg, end;
 };
 
 // Size class of a scene of n bodies (index into pdc_shapes' per-class arrays).
@@ -284,7 +303,9 @@ struct pdc_context {
 	pdc::DBuf<pdc::GridDesc> b_grid;
 	pdc::DBuf<float> grid_pool;
 	pdc::DBuf<int> b_soff;                                 // N+1 sample offsets
-	pdc::DBuf<double> samples;                             // 3 per sample
+	pdc::DBuf<float> samples;                              // 3 per sample (FP32, body frame, by cell)
+	pdc::DBuf<pdc::SampleCell> cells;                      // body b's at floor(soff[b] / kCellStride)
+	pdc::DBuf<int> b_ncell;                                // cells per body
 	std::vector<int> h_soff;                               // host copy of sample offsets
 	std::vector<int> h_static, h_emit;                     // host copies of the flags
 	int64_t dev_bytes = 0;                                 // device bytes held by the context's buffers
@@ -298,7 +319,10 @@ struct pdc_context {
 	pdc::DBuf<int> pr_tmp;
 	pdc::DBuf<double> stage_raw;                            // 7 per slot: pos, n, depth
 	pdc::DBuf<double> stage_m;                              // 7 per slot: merged
-	int64_t stage_cap = 0;
+	pdc::DBuf<int> stage_idx;                               // 2 per slot: sort indices of large pairs
+	pdc::DBuf<int> pr_big;                                  // pairs with more than kSortCap hits
+	int pr_cap = 0;                                         // pair capacity
+	int64_t stage_cap = 0;                                  // staging slots
 
 	// ---- contacts (SoA) ----
 	int C = 0;
@@ -309,6 +333,7 @@ struct pdc_context {
 	pdc::DBuf<double> c_lf;     // [3][C]
 	int c_stride = 0;           // SoA row stride (the contact capacity)
 	int n_pairs = 0, n_raw = 0, n_ties = 0;
+	pdc::DetectionStats det;
 	double tie_band = 0;
 
 	// ---- topology (local numbering) ----
@@ -416,6 +441,8 @@ namespace pdc {
 
 // pdc_util.hip
 void scan_exclusive(pdc_context* ctx, const int* in, int* out, int n);  // out has n+1 entries
+// The same over min(cap, *n_dev) entries, the count read on the device.
+void scan_exclusive_dev(pdc_context* ctx, const int* in, int* out, int cap, const int* n_dev);
 int read_int(pdc_context* ctx, const int* dptr);
 void sync_scalars(pdc_context* ctx);  // device Scalars -> ctx->h_sc (blocking)
 void kt_begin(pdc_context* ctx, int which, hipEvent_t* ev);

@@ -4,10 +4,13 @@
 namespace pdc {
 
 // Exclusive scan of n ints by one 1024-thread workgroup; out[n] = total.
+// With n_dev, n is min(n, *n_dev) (a count known on the device only).
 __global__ __launch_bounds__(1024) void k_scan_exclusive(const int* __restrict__ in,
-                                                         int* __restrict__ out, int n) {
+                                                         int* __restrict__ out, int n,
+                                                         const int* __restrict__ n_dev) {
 	__shared__ int wsum[16];
 	__shared__ int carry;
+	if (n_dev) n = min(n, *n_dev);
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 	if (tid == 0) carry = 0;
 	__syncthreads();
@@ -42,7 +45,12 @@ __global__ __launch_bounds__(1024) void k_scan_exclusive(const int* __restrict__
 }
 
 void scan_exclusive(pdc_context* ctx, const int* in, int* out, int n) {
-	hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, ctx->stream, in, out, n);
+	hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, ctx->stream, in, out, n, (const int*)nullptr);
+	PDC_CHECK_LAUNCH();
+}
+
+void scan_exclusive_dev(pdc_context* ctx, const int* in, int* out, int cap, const int* n_dev) {
+	hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, ctx->stream, in, out, cap, n_dev);
 	PDC_CHECK_LAUNCH();
 }
 

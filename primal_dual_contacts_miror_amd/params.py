@@ -74,6 +74,15 @@ class PdcTimings(ctypes.Structure):
     ]
 
 
+class PdcDetectionStats(ctypes.Structure):
+    """Counts of the last contact detection (pdc_get_detection_stats)."""
+    _fields_ = [(name, ctypes.c_int64) for name in
+                ("pairs", "raw_hits", "ties", "samples_all", "samples_read", "brick_nodes", "large_pairs")]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class PdcShapes(ctypes.Structure):
     """Work shapes per scene size class (bodies < 64, < 256, < 1024, >= 1024)."""
     _fields_ = [

@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-from .params import PdcParams, PdcShapes, PdcSolveStats, PdcTimings
+from .params import PdcDetectionStats, PdcParams, PdcShapes, PdcSolveStats, PdcTimings
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libpdc.so")
@@ -39,7 +39,7 @@ EXPORTS = [
     "pdc_set_scenes", "pdc_scene_count", "pdc_set_scene_friction", "pdc_step_scenes",
     "pdc_get_shapes", "pdc_set_shapes", "pdc_autotune",
     "pdc_set_state_range", "pdc_get_state_range", "pdc_append_bodies", "pdc_replace_bodies",
-    "pdc_device_bytes", "pdc_reserve_bodies", "pdc_get_groups",
+    "pdc_device_bytes", "pdc_reserve_bodies", "pdc_get_groups", "pdc_get_detection_stats",
 ]
 
 
@@ -89,6 +89,7 @@ def load_library(path=LIB_PATH):
     L.pdc_reserve_bodies.argtypes = [V, ctypes.c_int64, ctypes.c_int64]
     L.pdc_get_state.argtypes = [V, _D, _D, _D, _D]
     L.pdc_detect_contacts.argtypes = [V, _I, _I]
+    L.pdc_get_detection_stats.argtypes = [V, ctypes.POINTER(PdcDetectionStats)]
     L.pdc_get_contacts.argtypes = [V, ctypes.c_int32, _I, _D, _D, _D, _D]
     L.pdc_fill_forces.argtypes = [V, ctypes.c_int32, _I, _D, _D, _D, _D, _D, _D, _D,
                                   ctypes.POINTER(PdcSolveStats)]
@@ -240,6 +241,15 @@ class Context:
         c, t = ctypes.c_int32(0), ctypes.c_int32(0)
         self._check(self._L.pdc_detect_contacts(self.h, ctypes.byref(c), ctypes.byref(t)))
         return c.value, t.value
+
+    def detection_stats(self):
+        """Counts of the last contact detection (pairs, raw hits, ties, the
+        samples of every pair's emitting sides, those tested after the cell
+        cull, grid nodes under the partners' boxes, pairs merged in global
+        memory)."""
+        st = PdcDetectionStats()
+        self._check(self._L.pdc_get_detection_stats(self.h, ctypes.byref(st)))
+        return st
 
     def get_contacts(self, count):
         ids = np.zeros((count, 2), np.int32)

@@ -74,7 +74,8 @@ def scene_footprint(scene, params=None):
     """Estimated HBM bytes a scene adds to a shard: per body its FP32 SDF
     grid, state, flags and solver rows (below a PCG tolerance of 1e-9 also the
     288 entries of its group's inverse blocks and the grouping's integers),
-    24 bytes per surface sample, and its expected contacts
+    16 bytes per surface sample (its FP32 position and its share of the
+    sample cells), and its expected contacts
     (CONTACTS_PER_BODY per body, BYTES_PER_CONTACT each: contact record and
     solver state)."""
     p = params if params is not None else scene.params
@@ -83,7 +84,7 @@ def scene_footprint(scene, params=None):
     per_body = 13 * 8 + 64 + 6 * 8 * 12 + 36 * 8 * 18
     if p.pcg_tol < 1e-9:
         per_body += 288 * (8 if p.pcg_precision == PCG_FP64 else 4) + 16 * 4
-    return int(grid + 24 * scene_samples(scene, p) + n * per_body + scene.n * CONTACTS_PER_BODY * BYTES_PER_CONTACT)
+    return int(grid + 16 * scene_samples(scene, p) + n * per_body + scene.n * CONTACTS_PER_BODY * BYTES_PER_CONTACT)
 
 
 def _padding(count, params):

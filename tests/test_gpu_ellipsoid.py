@@ -15,11 +15,14 @@ scaled Fibonacci lattice.  Bars:
                 step from the oracle's state: the same contact count and
                 velocities within 1e-6 of |g| dt.  The reference's Newton
                 loop stalls at its iteration cap on most steps with impacts
-                here (the oracle's too, with either linear solver), so a
-                free-running comparison measures how the pile amplifies
-                differences below the stalled exit error (the oracle's own
-                direct and PCG branches part by 1.5e-5 of the body size
-                within 100 steps at a cap of 300), not the device.
+                here (the oracle's too, with either linear solver), and a
+                stalled result depends on the rounding of every inner solve:
+                on those steps the bar is the larger of 1e-6 |g| dt and 4x
+                the difference between the oracle's own direct and PCG
+                branches from the same state.  A free-running comparison
+                would measure how the pile amplifies such differences (the
+                oracle's two branches part by 1.5e-5 of the body size within
+                100 steps at a cap of 300), not the device.
 """
 import math
 
@@ -109,24 +112,37 @@ def test_ellipsoid_pile_steps_follow_oracle(require_gpu):
     o = orc.OracleScene(sc)
     o.set_threads(16)
     o.set_linear_solver(orc.SOLVER_DIRECT)
+    alt = orc.OracleScene(sc)
+    alt.set_threads(16)
+    alt.set_linear_solver(orc.SOLVER_PCG)
     ds = pdc.DynamicSystem(sc, sc.params)
     gdt = np.linalg.norm(sc.params.gravity[:]) * sc.params.timestep
-    worst, seen, stalled = 0.0, 0, 0
+
+    def vel_diff(u, v):
+        return max(np.max(np.abs(u[2] - v[2])), np.max(np.abs(u[3] - v[3]))) / gdt
+
+    worst, worst_conv, seen, stalled = 0.0, 0.0, 0, 0
     for k in range(100):
-        ds.set_state(*o.get_state())
+        start = o.get_state()
+        ds.set_state(*start)
         so = o.step(1)
         sg = ds.step(1)
         assert so.n_contacts == sg.n_contacts, k
-        _, _, vo, wo = o.get_state()
-        _, _, vg, wg = ds.state()
-        err = max(np.max(np.abs(vg - vo)), np.max(np.abs(wg - wo))) / gdt
+        err = vel_diff(ds.state(), o.get_state())
         ep, _ = _traj_errors(sc, o.get_state(), ds.state())
-        worst = max(worst, err)
+        bar = 1e-6
+        if so.newton_iterations >= sc.params.iterations:
+            stalled += 1
+            alt.set_state(*start)
+            alt.step(1)
+            bar = max(1e-6, 4.0 * vel_diff(alt.get_state(), o.get_state()))
+        else:
+            worst_conv = max(worst_conv, err)
+        worst = max(worst, err / bar)
         seen = max(seen, so.n_contacts)
-        stalled += so.newton_iterations >= sc.params.iterations
-        assert err < 1e-6 and ep < 1e-6, (k, err, ep)
-    print("pile: 100 steps from the oracle's states, up to %d contacts, %d stalled solves, "
-          "worst velocity difference %.3e of |g|dt" % (seen, stalled, worst))
+        assert err < bar and ep < 1e-6, (k, err, bar, ep)
+    print("pile: 100 steps from the oracle's states, up to %d contacts, %d stalled solves; worst velocity "
+          "difference of a converged step %.3e |g|dt, worst difference / bar %.3f" % (seen, stalled, worst_conv, worst))
     assert seen > 20
     ds.ctx.close()
 

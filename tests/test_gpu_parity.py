@@ -102,6 +102,31 @@ def test_contacts_rotated_pile(require_gpu):
     assert_same_contacts(cg, co, max(ties, t2))
 
 
+def test_contacts_large_face_pair(require_gpu):
+    """A 3 x 3 box face (3600 samples at spacing 0.05) pressed 1 mm into a
+    static floor: one pair with more hits than the LDS merge holds (2048),
+    merged in global memory; the contact set equals the oracle's, and the
+    detection statistics count the pair."""
+    b = _Builder()
+    b.box((5.0, 0.5, 5.0), (0, -0.5, 0), static=True)
+    b.box((1.5, 0.2, 1.5), (0.013, 0.2 - 1e-3, -0.021))
+    b.sphere(0.3, (3.0, 0.3 - 2e-3, 3.0))
+    p = default_params(grid_n=64, sample_spacing=0.05)
+    sc = b.build("face", p, 1)
+    ds = _gpu_scene(sc)
+    cg, ties = _contacts_gpu(ds)
+    st = ds.ctx.detection_stats()
+    o = orc.OracleScene(sc)
+    n, t2 = o.detect()
+    co = o.contacts()
+    print("large face: %d contacts, gpu %d, ties %d/%d, stats %s" % (n, cg.n, ties, t2, st.as_dict()))
+    assert n > 3600 and st.large_pairs == 1 and st.pairs == 2
+    assert st.raw_hits == o.raw_count()
+    assert_same_contacts(cg, co, max(ties, t2))
+    assert st.samples_read <= st.samples_all
+    ds.ctx.close()
+
+
 def test_contacts_empty_and_far_apart(require_gpu):
     b = _Builder()
     b.sphere(0.5, (0, 10, 0))
