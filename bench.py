@@ -381,7 +381,7 @@ def state_errors(sc, a, b):
     """Maxima over the dynamic bodies: position / body size, rotation angle,
     velocity / (|g| dt), angular velocity x body size / (|g| dt)."""
     dyn = sc.is_static == 0
-    size = np.max(sc.dims, axis=1)[dyn]
+    size = scenes.body_sizes(sc)[dyn]
     gdt = float(np.linalg.norm(sc.params.gravity[:])) * sc.params.timestep
     return {"max_position_error_rel_body_size": float((np.linalg.norm(a[0][dyn] - b[0][dyn], axis=1) / size).max()),
             "max_angle_error_rad": float(rotation_angle(a[1][dyn], b[1][dyn]).max()),

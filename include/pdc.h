@@ -40,8 +40,10 @@ enum {
 };
 
 /* ---- enums --------------------------------------------------------------- */
-/* Primitive shapes (src/CollisionObject/Cube.cpp, Sphere.cpp, Ellipsoid.cpp). */
-enum { PDC_SHAPE_BOX = 0, PDC_SHAPE_SPHERE = 1, PDC_SHAPE_ELLIPSOID = 2 };
+/* Shapes: the primitives (src/CollisionObject/Cube.cpp, Sphere.cpp,
+ * Ellipsoid.cpp) and triangle meshes (Mesh.cpp, Convex.cpp; see
+ * pdc_register_mesh). */
+enum { PDC_SHAPE_BOX = 0, PDC_SHAPE_SPHERE = 1, PDC_SHAPE_ELLIPSOID = 2, PDC_SHAPE_MESH = 3 };
 
 /* non_smooth_force.name: "non_smooth_contact" | "only_normal"
  * (PrimalDualForceSolver.cpp:664-673) */
@@ -150,6 +152,25 @@ int pdc_upload_bodies(pdc_context* ctx, int32_t n_bodies,
                       const double* mass, const double* inertia_diag,
                       const int32_t* is_static, const int32_t* emit_samples);
 
+/* Triangle meshes (Mesh::initializeFCLObject src/CollisionObject/Mesh.cpp:18-110,
+ * Mesh::signedDistance :125-167, MeshMeshContactGenerator::operator()
+ * src/Contact/ContactGenerator/MeshMeshContactGenerator.cpp:7-119).
+ * Registers a triangle mesh with the context: n_vertices vertices (3 doubles
+ * each) in the body frame (centre of mass at the origin, axes along the
+ * principal axes of inertia) and n_triangles triangles (3 vertex indices
+ * each, counter-clockwise seen from outside).  *mesh_id receives its id.  A
+ * body of shape PDC_SHAPE_MESH names its mesh by dims[3i] = id (dims[3i+1]
+ * and dims[3i+2] are not read).  Its SDF grid holds the mesh's signed
+ * distance, negative behind the angle-weighted pseudonormal of the closest
+ * feature (libigl's signed_distance_pseudonormal in the reference), over the
+ * box of its vertices padded by grid_pad_frac; its surface samples are the
+ * mesh's vertices (the points the reference tests against the other mesh).
+ * Meshes stay registered for the context's life.  PDC_ERR_INVALID_ARGUMENT
+ * for an index out of range, a non-finite coordinate, or a mesh whose
+ * triangles all have zero area. */
+int pdc_register_mesh(pdc_context* ctx, int32_t n_vertices, const double* vertices,
+                      int32_t n_triangles, const int32_t* triangles, int32_t* mesh_id);
+
 /* Body state: pos (3N), quat (4N, x y z w as in the reference's NPZ output),
  * linear and angular velocity (3N each).  Kept device-resident. */
 int pdc_set_state(pdc_context* ctx, const double* pos, const double* quat_xyzw,
@@ -212,13 +233,21 @@ int pdc_detect_contacts(pdc_context* ctx, int32_t* n_contacts, int32_t* n_ties);
  * reference): candidate pairs, raw sample hits, tie-band samples, the
  * samples of the emitting sides summed over pairs, those of them in the
  * sample cells kept for the partner (the ones tested), the grid nodes under
- * each partner's box summed over pairs and sides, and pairs with more than
- * 2048 hits (merged in global memory). */
+ * each partner's box summed over pairs and sides, pairs with more than 2048
+ * hits (merged in global memory), and the tested samples that fall inside
+ * the partner's grid box (each reads 8 grid nodes). */
 typedef struct pdc_detection_stats {
 	int64_t pairs, raw_hits, ties;
 	int64_t samples_all, samples_read, brick_nodes, large_pairs;
+	int64_t samples_in_grid;
 } pdc_detection_stats;
 int pdc_get_detection_stats(const pdc_context* ctx, pdc_detection_stats* out);
+
+/* Diagnostics, no counterpart in the reference: the FP32 SDF grid of body
+ * `body` (grid_n^3 nodes, x fastest, over the body-frame box
+ * [-h, h] per axis, h = the shape's half extent times 1 + grid_pad_frac).
+ * PDC_ERR_CAPACITY when capacity < grid_n^3. */
+int pdc_get_grid_nodes(pdc_context* ctx, int32_t body, int64_t capacity, float* nodes);
 
 /* Copies the current contact list to the host in the reference's
  * ContactInformation layout (ContactInfo.h:138-144): obj_ids (2C, ordered

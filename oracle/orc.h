@@ -27,6 +27,14 @@ orc_scene* orc_create(const pdc_params* p, int32_t n, const int32_t* shape_type,
                       const double* dims, const double* mass,
                       const double* inertia_diag, const int32_t* is_static,
                       const int32_t* emit_samples);
+/* The same with triangle meshes (bodies of shape PDC_SHAPE_MESH name mesh
+ * dims[3i] = index): n_meshes meshes, mesh m with mesh_nv[m] vertices (body
+ * frame) and mesh_nt[m] triangles, concatenated in mesh_v / mesh_tri. */
+orc_scene* orc_create_meshes(const pdc_params* p, int32_t n, const int32_t* shape_type,
+                             const double* dims, const double* mass, const double* inertia_diag,
+                             const int32_t* is_static, const int32_t* emit_samples, int32_t n_meshes,
+                             const int32_t* mesh_nv, const double* mesh_v, const int32_t* mesh_nt,
+                             const int32_t* mesh_tri);
 void orc_destroy(orc_scene* s);
 void orc_set_params(orc_scene* s, const pdc_params* p);
 /* threads <= 0: the reference's rules (1 thread below 1000 unknowns, else all
@@ -79,6 +87,10 @@ int32_t orc_samples(const orc_scene* s, int32_t b, int32_t cap, double* out);
 /* Analytic signed distance of the ellipsoid with semi-axes dims at n points
  * (xyz rows). */
 void orc_ellipsoid_sdf(const double dims[3], int32_t n, const double* pts, double* out);
+/* Signed distance of n points to the triangle mesh (nv vertices, nt
+ * triangles) by the pseudonormal rule. */
+void orc_mesh_sdf(int32_t nv, const double* v, int32_t nt, const int32_t* tri, int32_t n, const double* pts,
+                  double* out);
 
 /* Assembled Newton system of the first iteration of fillForces on the given
  * contacts/state (before Jacobi scaling): dense 6N x 6N matrix (row-major)

@@ -33,6 +33,11 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         L.orc_create.restype = ctypes.c_void_p
         L.orc_create.argtypes = [ctypes.POINTER(PdcParams), ctypes.c_int32, _I, _D, _D, _D, _I, _I]
+        L.orc_create_meshes.restype = ctypes.c_void_p
+        L.orc_create_meshes.argtypes = [ctypes.POINTER(PdcParams), ctypes.c_int32, _I, _D, _D, _D, _I, _I,
+                                        ctypes.c_int32, _I, _D, _I, _I]
+        L.orc_mesh_sdf.restype = None
+        L.orc_mesh_sdf.argtypes = [ctypes.c_int32, _D, ctypes.c_int32, _I, ctypes.c_int32, _D, _D]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_set_params.argtypes = [ctypes.c_void_p, ctypes.POINTER(PdcParams)]
         L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int32]
@@ -124,8 +129,22 @@ class OracleScene:
                       np.ascontiguousarray(scene.is_static, dtype=np.int32),
                       np.ascontiguousarray(scene.emit, dtype=np.int32)]
         st, dims, mass, inert, stat, emit = self._keep
-        self.h = L.orc_create(ctypes.byref(self.params), scene.n, _i(st), _d(dims), _d(mass),
-                              _d(inert), _i(stat), _i(emit))
+        meshes = getattr(scene, "meshes", None) or []
+        if meshes:
+            nv = np.array([len(v) for v, _ in meshes], np.int32)
+            nt = np.array([len(t) for _, t in meshes], np.int32)
+            mv = _f64(np.concatenate([np.asarray(v, np.float64).reshape(-1, 3) for v, _ in meshes]))
+            mt = np.ascontiguousarray(np.concatenate([np.asarray(t).reshape(-1, 3) for _, t in meshes]),
+                                      dtype=np.int32)
+            self._keep += [nv, nt, mv, mt]
+            self.h = L.orc_create_meshes(ctypes.byref(self.params), scene.n, _i(st), _d(dims), _d(mass),
+                                         _d(inert), _i(stat), _i(emit), len(meshes), _i(nv), _d(mv), _i(nt),
+                                         _i(mt))
+        else:
+            self.h = L.orc_create(ctypes.byref(self.params), scene.n, _i(st), _d(dims), _d(mass),
+                                  _d(inert), _i(stat), _i(emit))
+        if not self.h:
+            raise ValueError("orc_create rejected the scene")
         self.n = scene.n
         self.set_state(scene.pos, scene.quat, scene.vel, scene.angvel)
 
@@ -253,6 +272,16 @@ def ellipsoid_sdf(dims, pts):
     P = _f64(pts, (-1, 3))
     out = np.zeros(len(P))
     lib().orc_ellipsoid_sdf(_d(_f64(dims)), len(P), _d(P), _d(out))
+    return out
+
+
+def mesh_sdf(vertices, triangles, pts):
+    """Signed distance of points to a triangle mesh (pseudonormal sign)."""
+    v = _f64(vertices).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+    q = _f64(pts).reshape(-1, 3)
+    out = np.zeros(q.shape[0])
+    lib().orc_mesh_sdf(v.shape[0], _d(v), t.shape[0], _i(t), q.shape[0], _d(q), _d(out))
     return out
 
 

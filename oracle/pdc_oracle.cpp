@@ -52,6 +52,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -235,7 +236,166 @@ struct Body {
 	int gn = 2;
 	double glo[3] = {0, 0, 0}, gsp[3] = {1, 1, 1}, ghalf[3] = {0, 0, 0};
 	std::vector<double> samples; /* 3 per sample, body frame */
+	const struct OMesh* mesh = nullptr; /* mesh bodies: the mesh and its node grid */
 };
+
+/* ------------------------------------------------------------------------ */
+/* Triangle meshes (Mesh.cpp:18-110 set-up, Mesh::signedDistance :125-167,    */
+/* MeshMeshContactGenerator.cpp:7-119 vertices against the other mesh).      */
+/* The signed distance of a point is its distance to the closest point of    */
+/* the mesh, negative behind the angle-weighted pseudonormal of the feature   */
+/* (face, edge, vertex) holding that point (Baerentzen and Aanaes, IEEE TVCG  */
+/* 11 (2005); libigl's signed_distance_pseudonormal in the reference).        */
+/* ------------------------------------------------------------------------ */
+struct OMesh {
+	std::vector<double> verts;   /* body frame, 3 per vertex: the surface samples */
+	std::vector<double> tri;     /* per triangle of nonzero area: a, b, c, face normal,
+	                                pseudonormals of edges ab, bc, ca and vertices a, b, c */
+	double ext[3] = {0, 0, 0};   /* half extents of the grid box before padding */
+	std::vector<float> grid;     /* gn^3 nodes */
+	int gn = 0;
+	double pad = -1.0;
+};
+
+/* Pseudonormals.  Vertices with identical coordinates are merged for the
+ * adjacency; zero-area triangles take no part in the distance.  Face: the
+ * normalised cross product (b - a) x (c - a).  Edge: the normalised sum of the
+ * normals of the faces that share it (uniform weighting, Mesh.cpp:66-68).
+ * Vertex: the normalised sum of its faces' normals, each times the face's
+ * angle at the vertex (angle weighting, Mesh.cpp:64-65).  Sums in face order. */
+void omesh_setup(OMesh& m, int nv, const double* v, int nt, const int* f) {
+	m.verts.assign(v, v + 3 * (size_t)nv);
+	std::map<std::array<double, 3>, int> seen;
+	std::vector<int> id(nv);
+	for (int i = 0; i < nv; ++i)
+		id[i] = seen.emplace(std::array<double, 3>{v[3 * i], v[3 * i + 1], v[3 * i + 2]}, i).first->second;
+	std::vector<std::array<double, 3>> face(nt), vnorm(nv, std::array<double, 3>{0.0, 0.0, 0.0});
+	std::vector<bool> live(nt, false);
+	std::map<std::pair<int, int>, std::array<double, 3>> enorm;
+	for (int t = 0; t < nt; ++t) {
+		const double* x[3];
+		for (int k = 0; k < 3; ++k) x[k] = v + 3 * f[3 * t + k];
+		double u[3], w[3], c[3];
+		for (int a = 0; a < 3; ++a) { u[a] = x[1][a] - x[0][a]; w[a] = x[2][a] - x[0][a]; }
+		cross3(u, w, c);
+		const double len = std::sqrt(dot3(c, c));
+		if (!(len > 0.0)) continue;
+		live[t] = true;
+		for (int a = 0; a < 3; ++a) face[t][a] = c[a] / len;
+		for (int k = 0; k < 3; ++k) {
+			double e1[3], e2[3];
+			for (int a = 0; a < 3; ++a) {
+				e1[a] = x[(k + 1) % 3][a] - x[k][a];
+				e2[a] = x[(k + 2) % 3][a] - x[k][a];
+			}
+			const double cs = std::min(1.0, std::max(-1.0, dot3(e1, e2) / (std::sqrt(dot3(e1, e1)) * std::sqrt(dot3(e2, e2)))));
+			const double ang = std::acos(cs);
+			for (int a = 0; a < 3; ++a) vnorm[id[f[3 * t + k]]][a] += ang * face[t][a];
+			const int p = id[f[3 * t + k]], q = id[f[3 * t + (k + 1) % 3]];
+			auto& e = enorm.emplace(std::make_pair(std::min(p, q), std::max(p, q)),
+			                        std::array<double, 3>{0.0, 0.0, 0.0}).first->second;
+			for (int a = 0; a < 3; ++a) e[a] += face[t][a];
+		}
+	}
+	auto normalise = [](std::array<double, 3>& x) {
+		const double l = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+		if (l > 0.0)
+			for (int a = 0; a < 3; ++a) x[a] /= l;
+	};
+	for (auto& x : vnorm) normalise(x);
+	for (auto& kv : enorm) normalise(kv.second);
+	m.tri.clear();
+	for (int t = 0; t < nt; ++t) {
+		if (!live[t]) continue;
+		const int c0 = id[f[3 * t]], c1 = id[f[3 * t + 1]], c2 = id[f[3 * t + 2]];
+		for (int k = 0; k < 3; ++k)
+			for (int a = 0; a < 3; ++a) m.tri.push_back(v[3 * f[3 * t + k] + a]);
+		for (int a = 0; a < 3; ++a) m.tri.push_back(face[t][a]);
+		const std::pair<int, int> eds[3] = {{std::min(c0, c1), std::max(c0, c1)},
+		                                    {std::min(c1, c2), std::max(c1, c2)},
+		                                    {std::min(c2, c0), std::max(c2, c0)}};
+		for (const auto& e : eds)
+			for (int a = 0; a < 3; ++a) m.tri.push_back(enorm.at(e)[a]);
+		for (int c : {c0, c1, c2})
+			for (int a = 0; a < 3; ++a) m.tri.push_back(vnorm[c][a]);
+	}
+	double top = 0.0;
+	for (int a = 0; a < 3; ++a) {
+		m.ext[a] = 0.0;
+		for (int i = 0; i < nv; ++i) m.ext[a] = std::max(m.ext[a], std::fabs(v[3 * i + a]));
+		top = std::max(top, m.ext[a]);
+	}
+	for (int a = 0; a < 3; ++a) m.ext[a] = std::max(m.ext[a], 1e-3 * top);
+}
+
+/* Closest point of triangle t (a, b, c at t[0..8]) to p by the Voronoi
+ * regions of its vertices, edges and face (Ericson, Real-Time Collision
+ * Detection, 2005, sec. 5.1.5); returns the squared distance and the
+ * pseudonormal of the region's feature in *n. */
+double triangle_closest(const double* t, const double* p, const double** n, double* cp) {
+	const double *A = t, *B = t + 3, *Cc = t + 6;
+	double ab[3], ac[3], ap[3], bp[3], cpp[3];
+	for (int a = 0; a < 3; ++a) {
+		ab[a] = B[a] - A[a];
+		ac[a] = Cc[a] - A[a];
+		ap[a] = p[a] - A[a];
+		bp[a] = p[a] - B[a];
+		cpp[a] = p[a] - Cc[a];
+	}
+	const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+	const double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+	const double d5 = dot3(ab, cpp), d6 = dot3(ac, cpp);
+	const double vc = d1 * d4 - d3 * d2, vb = d5 * d2 - d1 * d6, va = d3 * d6 - d5 * d4;
+	if (d1 <= 0.0 && d2 <= 0.0) {                       /* vertex a */
+		for (int a = 0; a < 3; ++a) cp[a] = A[a];
+		*n = t + 21;
+	} else if (d3 >= 0.0 && d4 <= d3) {                /* vertex b */
+		for (int a = 0; a < 3; ++a) cp[a] = B[a];
+		*n = t + 24;
+	} else if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {  /* edge ab */
+		const double s = d1 / (d1 - d3);
+		for (int a = 0; a < 3; ++a) cp[a] = A[a] + s * ab[a];
+		*n = t + 12;
+	} else if (d6 >= 0.0 && d5 <= d6) {                /* vertex c */
+		for (int a = 0; a < 3; ++a) cp[a] = Cc[a];
+		*n = t + 27;
+	} else if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {  /* edge ca */
+		const double s = d2 / (d2 - d6);
+		for (int a = 0; a < 3; ++a) cp[a] = A[a] + s * ac[a];
+		*n = t + 18;
+	} else if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {  /* edge bc */
+		const double s = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+		for (int a = 0; a < 3; ++a) cp[a] = B[a] + s * (Cc[a] - B[a]);
+		*n = t + 15;
+	} else {                                           /* face */
+		const double den = 1.0 / (va + vb + vc);
+		const double sv = vb * den, sw = vc * den;
+		for (int a = 0; a < 3; ++a) cp[a] = A[a] + ab[a] * sv + ac[a] * sw;
+		*n = t + 9;
+	}
+	double d[3];
+	for (int a = 0; a < 3; ++a) d[a] = p[a] - cp[a];
+	return dot3(d, d);
+}
+
+/* Signed distance of p to the mesh: the nearest triangle (the first of equal
+ * squared distance) gives the distance and the sign. */
+double mesh_sdf(const OMesh& m, const double* p) {
+	double best = INFINITY, sgn = 1.0;
+	const size_t nt = m.tri.size() / 30;
+	for (size_t t = 0; t < nt; ++t) {
+		const double* n;
+		double cp[3];
+		const double d2 = triangle_closest(&m.tri[30 * t], p, &n, cp);
+		if (d2 < best) {
+			double d[3];
+			for (int a = 0; a < 3; ++a) d[a] = p[a] - cp[a];
+			best = d2;
+			sgn = dot3(d, n) < 0.0 ? -1.0 : 1.0;
+		}
+	}
+	return sgn * std::sqrt(best);
+}
 
 /* Distance from y >= 0 to the boundary of the m-dimensional ellipsoid with
  * semi-axes a[0] >= ... >= a[m-1] > 0 (Eberly, "Distance from a Point to an
@@ -340,11 +500,13 @@ double analytic_sdf(const Body& b, double px, double py, double pz) {
 	return outside + inside;
 }
 
-/* Grid: gn^3 nodes over [-e(1+pad), e(1+pad)] per axis, e = half extent. */
+/* Grid: gn^3 nodes over [-e(1+pad), e(1+pad)] per axis, e = half extent
+ * (a mesh: its largest |coordinate| per axis, at least 1e-3 of the largest). */
 void setup_grid(Body& b, int n, double pad_frac) {
 	b.gn = n;
 	double e[3];
 	if (b.type == PDC_SHAPE_SPHERE) e[0] = e[1] = e[2] = b.dims[0];
+	else if (b.type == PDC_SHAPE_MESH) { e[0] = b.mesh->ext[0]; e[1] = b.mesh->ext[1]; e[2] = b.mesh->ext[2]; }
 	else { e[0] = b.dims[0]; e[1] = b.dims[1]; e[2] = b.dims[2]; }
 	for (int a = 0; a < 3; ++a) {
 		const double h = e[a] * (1.0 + pad_frac);
@@ -354,6 +516,7 @@ void setup_grid(Body& b, int n, double pad_frac) {
 	}
 }
 inline float grid_node(const Body& b, int i, int j, int k) {
+	if (b.mesh) return b.mesh->grid[((size_t)k * b.gn + j) * b.gn + i];
 	const double px = b.glo[0] + (double)i * b.gsp[0];
 	const double py = b.glo[1] + (double)j * b.gsp[1];
 	const double pz = b.glo[2] + (double)k * b.gsp[2];
@@ -406,6 +569,10 @@ bool grid_query(const Body& b, const double* q, double* sd, double* grad) {
  * (setup_samples). */
 void surface_samples(Body& b, double spacing) {
 	b.samples.clear();
+	if (b.type == PDC_SHAPE_MESH) { /* the mesh's vertices */
+		b.samples = b.mesh->verts;
+		return;
+	}
 	if (b.type == PDC_SHAPE_SPHERE || b.type == PDC_SHAPE_ELLIPSOID) {
 		/* the ellipsoid's lattice is the unit one scaled by the semi-axes;
 		 * its count uses the area 4 pi (ab + ac + bc) / 3 (the p = 1 case of
@@ -885,6 +1052,7 @@ int bcsr_pcg(const BCSR& A, const double* b, double tol, int maxit, double* x, b
 /* ------------------------------------------------------------------------ */
 struct orc_scene {
 	pdc_params p;
+	std::vector<std::unique_ptr<OMesh>> meshes;
 	int threads = 0;
 	int solver = ORC_SOLVER_REFERENCE;
 	std::vector<Body> bodies;
@@ -1825,15 +1993,57 @@ extern "C" {
 orc_scene* orc_create(const pdc_params* p, int32_t n, const int32_t* shape_type,
                       const double* dims, const double* mass, const double* inertia_diag,
                       const int32_t* is_static, const int32_t* emit_samples) {
-	if (!p || n < 0) return nullptr;
+	return orc_create_meshes(p, n, shape_type, dims, mass, inertia_diag, is_static, emit_samples, 0, nullptr,
+	                         nullptr, nullptr, nullptr);
+}
+
+orc_scene* orc_create_meshes(const pdc_params* p, int32_t n, const int32_t* shape_type,
+                             const double* dims, const double* mass, const double* inertia_diag,
+                             const int32_t* is_static, const int32_t* emit_samples, int32_t n_meshes,
+                             const int32_t* mesh_nv, const double* mesh_v, const int32_t* mesh_nt,
+                             const int32_t* mesh_tri) {
+	if (!p || n < 0 || n_meshes < 0) return nullptr;
 	orc_scene* s = new orc_scene();
 	s->p = *p;
 	std::memset(&s->last, 0, sizeof(s->last));
+	size_t ov = 0, ot = 0;
+	for (int m = 0; m < n_meshes; ++m) {
+		auto M = std::make_unique<OMesh>();
+		omesh_setup(*M, mesh_nv[m], mesh_v + 3 * ov, mesh_nt[m], mesh_tri + 3 * ot);
+		ov += (size_t)mesh_nv[m];
+		ot += (size_t)mesh_nt[m];
+		/* node grid */
+		Body tmp;
+		tmp.type = PDC_SHAPE_MESH;
+		tmp.mesh = M.get();
+		setup_grid(tmp, p->grid_n, p->grid_pad_frac);
+		const int gn = p->grid_n;
+		M->gn = gn;
+		M->pad = p->grid_pad_frac;
+		M->grid.assign((size_t)gn * gn * gn, 0.0f);
+		const int64_t n3 = (int64_t)gn * gn * gn;
+#pragma omp parallel for schedule(static)
+		for (int64_t t = 0; t < n3; ++t) {
+			const int i = (int)(t % gn), j = (int)((t / gn) % gn), k = (int)(t / ((int64_t)gn * gn));
+			const double q[3] = {tmp.glo[0] + (double)i * tmp.gsp[0], tmp.glo[1] + (double)j * tmp.gsp[1],
+			                     tmp.glo[2] + (double)k * tmp.gsp[2]};
+			M->grid[(size_t)t] = (float)mesh_sdf(*M, q);
+		}
+		s->meshes.push_back(std::move(M));
+	}
 	s->bodies.resize(n);
 	for (int i = 0; i < n; ++i) {
 		Body& b = s->bodies[i];
 		b.type = shape_type[i];
 		for (int a = 0; a < 3; ++a) b.dims[a] = dims[3 * i + a];
+		if (b.type == PDC_SHAPE_MESH) {
+			const int id = (int)dims[3 * i];
+			if (id < 0 || id >= n_meshes) {
+				delete s;
+				return nullptr;
+			}
+			b.mesh = s->meshes[id].get();
+		}
 		b.mass = mass[i];
 		for (int a = 0; a < 3; ++a) b.I0[a] = inertia_diag[3 * i + a];
 		b.stat = is_static[i] != 0;
@@ -1954,6 +2164,12 @@ int32_t orc_sdf_query(const orc_scene* s, int32_t b, const double q[3], double* 
 }
 float orc_sdf_node(const orc_scene* s, int32_t b, int32_t i, int32_t j, int32_t k) {
 	return grid_node(s->bodies[b], i, j, k);
+}
+void orc_mesh_sdf(int32_t nv, const double* v, int32_t nt, const int32_t* tri, int32_t n, const double* pts,
+                  double* out) {
+	OMesh m;
+	omesh_setup(m, nv, v, nt, tri);
+	for (int32_t i = 0; i < n; ++i) out[i] = mesh_sdf(m, pts + 3 * i);
 }
 int32_t orc_samples(const orc_scene* s, int32_t b, int32_t cap, double* out) {
 	const auto& v = s->bodies[b].samples;

@@ -127,7 +127,11 @@ void free_all(pdc_context* c) {
 	c->b_mass.release(); c->b_I0.release(); c->b_pos.release(); c->b_quat.release();
 	c->b_vel.release(); c->b_ang.release(); c->b_acc.release(); c->b_R.release(); c->b_aabb.release();
 	c->b_grid.release(); c->grid_pool.release(); c->b_soff.release(); c->samples.release();
-	c->cells.release(); c->b_ncell.release(); c->stage_idx.release(); c->pr_big.release();
+	for (auto& m : c->meshes) {
+		m->d_recs.release();
+		m->grid.release();
+	}
+	c->cells.release(); c->b_ncell.release(); c->stage_idx.release(); c->pr_big.release(); c->pr_st.release();
 	c->bp_cnt.release(); c->bp_off.release(); c->pr_a.release(); c->pr_b.release();
 	c->pr_soff.release(); c->pr_nh.release(); c->pr_nc.release(); c->pr_coff.release();
 	c->pr_tmp.release(); c->stage_raw.release(); c->stage_m.release();
@@ -268,18 +272,25 @@ int pdc_configure(pdc_context* ctx, const pdc_params* params) {
 
 const char* pdc_last_error(const pdc_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-static void check_bodies(const char* who, int n, const int32_t* shape_type, const double* dims, const double* mass,
-                         const double* inertia_diag, const int32_t* is_static) {
+static void check_bodies(const pdc_context* ctx, const char* who, int n, const int32_t* shape_type,
+                         const double* dims, const double* mass, const double* inertia_diag,
+                         const int32_t* is_static) {
 	const std::string w(who);
 	if (n < 0 || (n > 0 && (!shape_type || !dims || !mass || !inertia_diag || !is_static)))
 		throw std::invalid_argument(w + ": null array or negative count");
 	for (int i = 0; i < n; ++i) {
 		if (shape_type[i] != PDC_SHAPE_BOX && shape_type[i] != PDC_SHAPE_SPHERE &&
-		    shape_type[i] != PDC_SHAPE_ELLIPSOID)
+		    shape_type[i] != PDC_SHAPE_ELLIPSOID && shape_type[i] != PDC_SHAPE_MESH)
 			throw std::invalid_argument(w + ": unknown shape type");
-		for (int a = 0; a < 3; ++a)
-			if (!(dims[3 * i + a] > 0) && !(shape_type[i] == PDC_SHAPE_SPHERE && a > 0))
-				throw std::invalid_argument(w + ": dimensions must be > 0");
+		if (shape_type[i] == PDC_SHAPE_MESH) {
+			const double id = dims[3 * i];
+			if (!(id >= 0 && id < (double)ctx->meshes.size() && id == std::floor(id)))
+				throw std::invalid_argument(w + ": a mesh body's dims[0] must be a registered mesh id");
+		} else {
+			for (int a = 0; a < 3; ++a)
+				if (!(dims[3 * i + a] > 0) && !(shape_type[i] == PDC_SHAPE_SPHERE && a > 0))
+					throw std::invalid_argument(w + ": dimensions must be > 0");
+		}
 		if (!(mass[i] > 0)) throw std::invalid_argument(w + ": mass must be > 0");
 		for (int a = 0; a < 3; ++a)
 			if (!(inertia_diag[3 * i + a] > 0)) throw std::invalid_argument(w + ": inertia must be > 0");
@@ -305,11 +316,35 @@ static void identity_state(pdc_context* ctx, int first, int n) {
 	write_state(ctx, first, n, z3.data(), q.data(), z3.data(), z3.data());
 }
 
+int pdc_register_mesh(pdc_context* ctx, int32_t nv, const double* v, int32_t nt, const int32_t* tri,
+                      int32_t* mesh_id) {
+	return guard(ctx, [&] {
+		if (nv < 3 || nt < 1 || !v || !tri || !mesh_id)
+			throw std::invalid_argument("pdc_register_mesh: null array, or fewer than 3 vertices or 1 triangle");
+		for (int64_t i = 0; i < 3 * (int64_t)nv; ++i)
+			if (!std::isfinite(v[i])) throw std::invalid_argument("pdc_register_mesh: non-finite vertex coordinate");
+		for (int64_t i = 0; i < 3 * (int64_t)nt; ++i)
+			if (tri[i] < 0 || tri[i] >= nv) throw std::invalid_argument("pdc_register_mesh: vertex index out of range");
+		*mesh_id = pdc::register_mesh(ctx, nv, v, nt, tri);
+	});
+}
+
+int pdc_get_grid_nodes(pdc_context* ctx, int32_t body, int64_t capacity, float* nodes) {
+	return guard(ctx, [&] {
+		if (body < 0 || body >= ctx->N || !nodes) throw std::invalid_argument("pdc_get_grid_nodes: bad body or null");
+		const int64_t n3 = (int64_t)ctx->p.grid_n * ctx->p.grid_n * ctx->p.grid_n;
+		if (capacity < n3) throw pdc::HipError("pdc_get_grid_nodes: capacity below grid_n^3", PDC_ERR_CAPACITY);
+		PDC_HIP(hipMemcpyAsync(nodes, ctx->grid_pool.p + (size_t)body * (size_t)n3, (size_t)n3 * sizeof(float),
+		                       hipMemcpyDeviceToHost, ctx->stream));
+		PDC_HIP(hipStreamSynchronize(ctx->stream));
+	});
+}
+
 int pdc_upload_bodies(pdc_context* ctx, int32_t n, const int32_t* shape_type, const double* dims,
                       const double* mass, const double* inertia_diag, const int32_t* is_static,
                       const int32_t* emit_samples) {
 	return guard(ctx, [&] {
-		check_bodies("pdc_upload_bodies", n, shape_type, dims, mass, inertia_diag, is_static);
+		check_bodies(ctx, "pdc_upload_bodies", n, shape_type, dims, mass, inertia_diag, is_static);
 		pdc::upload_bodies(ctx, n, shape_type, dims, mass, inertia_diag, is_static, emit_samples);
 		if (n > 0) identity_state(ctx, 0, n);  // until the caller sets one
 		ctx->C = 0;
@@ -322,7 +357,7 @@ int pdc_append_bodies(pdc_context* ctx, int32_t n, const int32_t* shape_type, co
                       const double* vel, const double* angvel, int32_t* first_body) {
 	return guard(ctx, [&] {
 		if (n < 1) throw std::invalid_argument("pdc_append_bodies: no bodies");
-		check_bodies("pdc_append_bodies", n, shape_type, dims, mass, inertia_diag, is_static);
+		check_bodies(ctx, "pdc_append_bodies", n, shape_type, dims, mass, inertia_diag, is_static);
 		const int first = pdc::append_bodies(ctx, n, shape_type, dims, mass, inertia_diag, is_static, emit_samples);
 		identity_state(ctx, first, n);
 		write_state(ctx, first, n, pos, quat_xyzw, vel, angvel);
@@ -337,7 +372,7 @@ int pdc_replace_bodies(pdc_context* ctx, int32_t first, int32_t n, const int32_t
                        const double* vel, const double* angvel) {
 	return guard(ctx, [&] {
 		require_bodies(ctx);
-		check_bodies("pdc_replace_bodies", n, shape_type, dims, mass, inertia_diag, is_static);
+		check_bodies(ctx, "pdc_replace_bodies", n, shape_type, dims, mass, inertia_diag, is_static);
 		pdc::replace_bodies(ctx, first, n, shape_type, dims, mass, inertia_diag, is_static, emit_samples);
 		identity_state(ctx, first, n);
 		write_state(ctx, first, n, pos, quat_xyzw, vel, angvel);
@@ -438,6 +473,7 @@ int pdc_get_detection_stats(const pdc_context* ctx, pdc_detection_stats* out) {
 	out->raw_hits = d.raw_hits;
 	out->ties = d.ties;
 	out->samples_all = d.samples_all;
+	out->samples_in_grid = d.samples_in_grid;
 	out->samples_read = d.samples_read;
 	out->brick_nodes = d.brick_nodes;
 	out->large_pairs = d.large_pairs;

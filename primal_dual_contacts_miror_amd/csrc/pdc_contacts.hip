@@ -16,10 +16,13 @@
 // Normal convention (MeshMeshContactGenerator.cpp:43-55): a sample of body a
 // inside b gives normal -grad(sd_b); a sample of b inside a gives +grad(sd_a).
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <string>
+#include <utility>
 
 #include "pdc_internal.h"
 #include "pdc_reduce.h"
@@ -35,8 +38,16 @@ namespace pdc {
 //           max(8, round(4 pi (ab + ac + bc) / 3 / spacing^2)) points (the
 //           area of the ellipsoid by the p = 1 case of Thomsen's formula,
 //           exact for a sphere)
+//   mesh:   its vertices, as the reference tests a mesh's vertices against
+//           the other mesh (MeshMeshContactGenerator.cpp:27-63)
 // ---------------------------------------------------------------------------
-static void make_samples(int type, const double* dims, double spacing, std::vector<double>& out) {
+static void make_samples(const pdc_context* ctx, int type, const double* dims, double spacing,
+                         std::vector<double>& out) {
+	if (type == PDC_SHAPE_MESH) {
+		const MeshShape& m = *ctx->meshes[(size_t)dims[0]];
+		out.insert(out.end(), m.verts.begin(), m.verts.end());
+		return;
+	}
 	if (type == PDC_SHAPE_SPHERE || type == PDC_SHAPE_ELLIPSOID) {
 		const double PI = 3.14159265358979323846;
 		double sx = dims[0], sy = dims[0], sz = dims[0];
@@ -89,6 +100,7 @@ __global__ __launch_bounds__(256) void k_build_grids(const GridDesc* __restrict_
 	const int64_t n3 = (int64_t)g.n * g.n * g.n;
 	const double d[3] = {dims[3 * b], dims[3 * b + 1], dims[3 * b + 2]};
 	const int type = shape[b];
+	if (type == PDC_SHAPE_MESH) return;  // copied from the mesh's own grid
 	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n3;
 	     t += (int64_t)gridDim.x * blockDim.x) {
 		const int i = (int)(t % g.n);
@@ -101,13 +113,143 @@ __global__ __launch_bounds__(256) void k_build_grids(const GridDesc* __restrict_
 	}
 }
 
+// ---------------------------------------------------------------------------
+// Triangle meshes (Mesh::initializeFCLObject, src/CollisionObject/Mesh.cpp:18-110)
+// ---------------------------------------------------------------------------
+// Triangle records of a mesh (pdc_math.h, kTriRec doubles each).  Vertices
+// with equal coordinates are one vertex for the adjacency (glTF files repeat
+// vertices per face); triangles of zero area are left out of the distance
+// (their vertices stay samples).  Face normals are normalised cross products;
+// the pseudonormal of an edge is the normalised sum of the normals of the
+// faces sharing it (libigl's uniform edge weighting, Mesh.cpp:66-68), of a
+// vertex the normalised sum of its faces' normals weighted by the face angle
+// at the vertex (angle weighting, Mesh.cpp:64-65).  Sums run in face order.
+static void mesh_records(int nv, const double* v, int nt, const int* tri, std::vector<double>& rec) {
+	std::vector<int> canon((size_t)nv);
+	std::map<std::array<double, 3>, int> first;
+	for (int i = 0; i < nv; ++i) {
+		const std::array<double, 3> key{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+		canon[i] = first.emplace(key, i).first->second;
+	}
+	std::vector<double> fn(3 * (size_t)nt, 0.0), vn(3 * (size_t)nv, 0.0);
+	std::vector<char> ok((size_t)nt, 0);
+	std::map<std::pair<int, int>, std::array<double, 3>> en;
+	for (int f = 0; f < nt; ++f) {
+		const double* P[3] = {&v[3 * tri[3 * f]], &v[3 * tri[3 * f + 1]], &v[3 * tri[3 * f + 2]]};
+		const double e1[3] = {P[1][0] - P[0][0], P[1][1] - P[0][1], P[1][2] - P[0][2]};
+		const double e2[3] = {P[2][0] - P[0][0], P[2][1] - P[0][1], P[2][2] - P[0][2]};
+		double n[3];
+		cross3(e1, e2, n);
+		const double len = std::sqrt(dot3(n, n));
+		if (!(len > 0.0)) continue;
+		ok[f] = 1;
+		for (int k = 0; k < 3; ++k) fn[3 * (size_t)f + k] = n[k] / len;
+		const double* nf = &fn[3 * (size_t)f];
+		for (int k = 0; k < 3; ++k) {
+			const double* o = P[k];
+			const double* u = P[(k + 1) % 3];
+			const double* w = P[(k + 2) % 3];
+			const double du[3] = {u[0] - o[0], u[1] - o[1], u[2] - o[2]};
+			const double dw[3] = {w[0] - o[0], w[1] - o[1], w[2] - o[2]};
+			double cs = dot3(du, dw) / (std::sqrt(dot3(du, du)) * std::sqrt(dot3(dw, dw)));
+			cs = cs < -1.0 ? -1.0 : cs > 1.0 ? 1.0 : cs;
+			const double ang = std::acos(cs);
+			double* vk = &vn[3 * (size_t)canon[tri[3 * f + k]]];
+			for (int c = 0; c < 3; ++c) vk[c] += ang * nf[c];
+			const int i0 = canon[tri[3 * f + k]], i1 = canon[tri[3 * f + (k + 1) % 3]];
+			auto it = en.emplace(std::make_pair(std::min(i0, i1), std::max(i0, i1)),
+			                     std::array<double, 3>{0.0, 0.0, 0.0}).first;
+			for (int c = 0; c < 3; ++c) it->second[c] += nf[c];
+		}
+	}
+	auto unit = [](double* x) {
+		const double l = std::sqrt(dot3(x, x));
+		if (l > 0.0)
+			for (int c = 0; c < 3; ++c) x[c] /= l;
+	};
+	for (int i = 0; i < nv; ++i) unit(&vn[3 * (size_t)i]);
+	for (auto& kv : en) unit(kv.second.data());
+	rec.clear();
+	for (int f = 0; f < nt; ++f) {
+		if (!ok[f]) continue;
+		const int c3[3] = {canon[tri[3 * f]], canon[tri[3 * f + 1]], canon[tri[3 * f + 2]]};
+		for (int k = 0; k < 3; ++k)
+			for (int c = 0; c < 3; ++c) rec.push_back(v[3 * (size_t)tri[3 * f + k] + c]);
+		for (int c = 0; c < 3; ++c) rec.push_back(fn[3 * (size_t)f + c]);
+		for (int k = 0; k < 3; ++k) {
+			const auto& e = en.at(std::make_pair(std::min(c3[k], c3[(k + 1) % 3]), std::max(c3[k], c3[(k + 1) % 3])));
+			for (int c = 0; c < 3; ++c) rec.push_back(e[c]);
+		}
+		for (int k = 0; k < 3; ++k)
+			for (int c = 0; c < 3; ++c) rec.push_back(vn[3 * (size_t)c3[k] + c]);
+	}
+}
+
+int register_mesh(pdc_context* ctx, int nv, const double* v, int nt, const int* tri) {
+	auto m = std::make_unique<MeshShape>();
+	m->verts.assign(v, v + 3 * (size_t)nv);
+	mesh_records(nv, v, nt, tri, m->recs);
+	if (m->recs.empty()) throw std::invalid_argument("pdc_register_mesh: every triangle has zero area");
+	// grid box: the largest |coordinate| per axis, at least 1e-3 of the largest
+	// (so that a flat mesh still gets a box of nonzero thickness)
+	double emax = 0.0;
+	for (int a = 0; a < 3; ++a) {
+		for (int i = 0; i < nv; ++i) m->ext[a] = std::max(m->ext[a], std::fabs(v[3 * (size_t)i + a]));
+		emax = std::max(emax, m->ext[a]);
+	}
+	if (!(emax > 0.0)) throw std::invalid_argument("pdc_register_mesh: all vertices at the origin");
+	for (int a = 0; a < 3; ++a) m->ext[a] = std::max(m->ext[a], 1e-3 * emax);
+	ctx->meshes.push_back(std::move(m));
+	return (int)ctx->meshes.size() - 1;
+}
+
+// Half extents of a body's shape before padding.
+static void shape_extent(const pdc_context* ctx, int shape, const double* dims, double* e) {
+	if (shape == PDC_SHAPE_SPHERE) e[0] = e[1] = e[2] = dims[0];
+	else if (shape == PDC_SHAPE_MESH) {
+		const MeshShape& m = *ctx->meshes[(size_t)dims[0]];
+		e[0] = m.ext[0]; e[1] = m.ext[1]; e[2] = m.ext[2];
+	} else { e[0] = dims[0]; e[1] = dims[1]; e[2] = dims[2]; }
+}
+
+// Signed distance of every node of one mesh grid; one thread per node, the
+// triangle records staged through LDS 32 at a time.  The grid is launched
+// with exactly one thread per node rounded up to the block, so every thread
+// of a block reaches each barrier.
+constexpr int kMeshChunk = 32;
+__global__ __launch_bounds__(256) void k_mesh_grid(const double* __restrict__ recs, int nt, GridDesc g,
+                                                   float* __restrict__ out) {
+	__shared__ double st[kMeshChunk * kTriRec];
+	const int64_t n3 = (int64_t)g.n * g.n * g.n;
+	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	double p[3] = {0.0, 0.0, 0.0};
+	if (t < n3) {
+		const int i = (int)(t % g.n);
+		const int j = (int)((t / g.n) % g.n);
+		const int k = (int)(t / ((int64_t)g.n * g.n));
+		p[0] = g.lo[0] + (double)i * g.sp[0];
+		p[1] = g.lo[1] + (double)j * g.sp[1];
+		p[2] = g.lo[2] + (double)k * g.sp[2];
+	}
+	double best = INFINITY, sgn = 1.0;
+	for (int c0 = 0; c0 < nt; c0 += kMeshChunk) {
+		const int m = min(kMeshChunk, nt - c0);
+		__syncthreads();
+		for (int x = threadIdx.x; x < m * kTriRec; x += blockDim.x) st[x] = recs[(size_t)c0 * kTriRec + x];
+		__syncthreads();
+		if (t < n3)
+			for (int j = 0; j < m; ++j) mesh_sdf_step(&st[j * kTriRec], p, &best, &sgn);
+	}
+	if (t < n3) out[t] = (float)(sgn * sqrt(best));
+}
+
 // Grid box of a body: the shape's box padded by grid_pad_frac, its node block
 // at body * grid_n^3 in the pool.
-static GridDesc grid_desc(const pdc_params& p, int shape, const double* dims, int64_t body) {
+static GridDesc grid_desc(const pdc_context* ctx, int shape, const double* dims, int64_t body) {
+	const pdc_params& p = ctx->p;
 	GridDesc g{};
 	double e[3];
-	if (shape == PDC_SHAPE_SPHERE) e[0] = e[1] = e[2] = dims[0];
-	else { e[0] = dims[0]; e[1] = dims[1]; e[2] = dims[2]; }
+	shape_extent(ctx, shape, dims, e);
 	const int gn = p.grid_n;
 	for (int a = 0; a < 3; ++a) {
 		const double h = e[a] * (1.0 + p.grid_pad_frac);
@@ -118,6 +260,27 @@ static GridDesc grid_desc(const pdc_params& p, int shape, const double* dims, in
 	g.n = gn;
 	g.offset = body * (int64_t)gn * gn * gn;
 	return g;
+}
+
+// Device grid of mesh `id` at the context's grid_n and padding (built on
+// first use, rebuilt when those change).
+static const float* mesh_grid(pdc_context* ctx, int id) {
+	MeshShape& m = *ctx->meshes[(size_t)id];
+	const int gn = ctx->p.grid_n;
+	if (m.grid_n == gn && m.grid_pad == ctx->p.grid_pad_frac) return m.grid.p;
+	hipStream_t s = ctx->stream;
+	const int nt = (int)(m.recs.size() / kTriRec);
+	m.d_recs.ensure(m.recs.size());
+	PDC_HIP(hipMemcpyAsync(m.d_recs.p, m.recs.data(), m.recs.size() * sizeof(double), hipMemcpyHostToDevice, s));
+	GridDesc g = grid_desc(ctx, PDC_SHAPE_MESH, std::array<double, 3>{(double)id, 0.0, 0.0}.data(), 0);
+	const int64_t n3 = (int64_t)gn * gn * gn;
+	m.grid.ensure((size_t)n3);
+	hipLaunchKernelGGL(k_mesh_grid, dim3(grid_for(n3)), dim3(kBlock), 0, s, m.d_recs.p, nt, g, m.grid.p);
+	PDC_CHECK_LAUNCH();
+	PDC_HIP(hipStreamSynchronize(s));
+	m.grid_n = gn;
+	m.grid_pad = ctx->p.grid_pad_frac;
+	return m.grid.p;
 }
 
 // Flags, dimensions, masses, inertia and grid descriptors of bodies
@@ -144,7 +307,7 @@ static void write_bodies(pdc_context* ctx, int first, int n, const int* shape, c
 	std::vector<GridDesc> gd(N);
 	double finest = 1e300;
 	for (size_t b = 0; b < N; ++b) {
-		gd[b] = grid_desc(ctx->p, shape[b], &dims[3 * b], (int64_t)(f + b));
+		gd[b] = grid_desc(ctx, shape[b], &dims[3 * b], (int64_t)(f + b));
 		for (int a = 0; a < 3; ++a) finest = std::min(finest, gd[b].sp[a]);
 	}
 	// tie band: 1e-7 of the finest grid spacing of the table
@@ -158,6 +321,12 @@ static void write_bodies(pdc_context* ctx, int first, int n, const int* shape, c
 		hipLaunchKernelGGL(k_build_grids, dim3(gx, nb), dim3(kBlock), 0, s, ctx->b_grid.p + f + b0,
 		                   ctx->b_shape.p + f + b0, ctx->b_dims.p + 3 * (f + b0), ctx->grid_pool.p);
 		PDC_CHECK_LAUNCH();
+	}
+	for (size_t b = 0; b < N; ++b) {
+		if (shape[b] != PDC_SHAPE_MESH) continue;
+		const float* mg = mesh_grid(ctx, (int)dims[3 * b]);
+		PDC_HIP(hipMemcpyAsync(ctx->grid_pool.p + gd[b].offset, mg, (size_t)n3 * sizeof(float),
+		                       hipMemcpyDeviceToDevice, s));
 	}
 }
 
@@ -216,14 +385,14 @@ static void sort_into_cells(const double* smp, int S, std::vector<float>& out, s
 
 // FP32 body-frame surface samples of n bodies, ordered by cell; counts[b]
 // samples and ncell[b] cells per body, the cells of all n bodies in `cells`.
-static void samples_of(const pdc_params& p, int n, const int* shape, const double* dims, std::vector<float>& smp,
+static void samples_of(const pdc_context* ctx, int n, const int* shape, const double* dims, std::vector<float>& smp,
                        std::vector<int>& counts, std::vector<SampleCell>& cells, std::vector<int>& ncell) {
 	counts.assign(n, 0);
 	ncell.assign(n, 0);
 	std::vector<double> one;
 	for (int b = 0; b < n; ++b) {
 		one.clear();
-		make_samples(shape[b], &dims[3 * b], p.sample_spacing, one);
+		make_samples(ctx, shape[b], &dims[3 * b], ctx->p.sample_spacing, one);
 		const int S = (int)(one.size() / 3);
 		const size_t c0 = cells.size();
 		sort_into_cells(one.data(), S, smp, cells);
@@ -274,7 +443,7 @@ void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims
 	std::vector<float> smp;
 	std::vector<int> cnt, ncell;
 	std::vector<SampleCell> cells;
-	samples_of(ctx->p, n, shape, dims, smp, cnt, cells, ncell);
+	samples_of(ctx, n, shape, dims, smp, cnt, cells, ncell);
 	std::vector<int64_t> total(N + 1, 0);
 	for (size_t b = 0; b < N; ++b) total[b + 1] = total[b] + cnt[b];
 	if (total[N] > INT32_MAX) throw HipError("sample table beyond 2^31 samples", PDC_ERR_CAPACITY);
@@ -336,7 +505,7 @@ int append_bodies(pdc_context* ctx, int n, const int* shape, const double* dims,
 	std::vector<float> smp;
 	std::vector<int> cnt, ncell;
 	std::vector<SampleCell> cells;
-	samples_of(ctx->p, n, shape, dims, smp, cnt, cells, ncell);
+	samples_of(ctx, n, shape, dims, smp, cnt, cells, ncell);
 	const int64_t total0 = ctx->h_soff.empty() ? 0 : ctx->h_soff[N0];
 	const int64_t total1 = total0 + (int64_t)(smp.size() / 3);
 	if (total1 > INT32_MAX) throw HipError("sample table beyond 2^31 samples", PDC_ERR_CAPACITY);
@@ -371,7 +540,7 @@ void replace_bodies(pdc_context* ctx, int first, int n, const int* shape, const 
 	std::vector<float> smp;
 	std::vector<int> cnt, ncell;
 	std::vector<SampleCell> cells;
-	samples_of(ctx->p, n, shape, dims, smp, cnt, cells, ncell);
+	samples_of(ctx, n, shape, dims, smp, cnt, cells, ncell);
 	const int64_t lo = ctx->h_soff[first], hi = ctx->h_soff[first + n];
 	const int64_t m = (int64_t)(smp.size() / 3);
 	const bool last_emits = emit ? emit[n - 1] != 0 : true;
@@ -650,9 +819,16 @@ struct PairArgs {
 	int* big;              // pairs with more than kSortCap hits
 	int* pair_nh;
 	int* pair_nc;
+	int* pair_st;          // per pair kPairStats counters (summed by k_contact_totals)
 	Scalars* sc;
 	double tie_band;
 };
+
+// Per-pair detection counters: samples of the emitting sides, samples the
+// cull keeps, grid nodes under the partners' boxes, samples inside the
+// partner's grid box, tie-band samples.  Written per pair and summed by one
+// workgroup (same-address atomics from every pair serialise across the XCDs).
+enum { kStAll = 0, kStRead = 1, kStNodes = 2, kStInGrid = 3, kStTies = 4, kPairStats = 5 };
 
 __device__ __forceinline__ int pairs_of(const PairArgs& A) { return min(*A.np, A.cap); }
 
@@ -693,9 +869,10 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_pair_cull(PairArgs A) {
 		}
 		if (lane == 0) {
 			A.stage_cnt[p] = kept;
-			atomicAdd(&A.sc->all_samples, (unsigned long long)all);
-			atomicAdd(&A.sc->read_samples, (unsigned long long)kept);
-			atomicAdd(&A.sc->brick_nodes, (unsigned long long)nodes);
+			int* st = &A.pair_st[kPairStats * (size_t)p];
+			st[kStAll] = all;
+			st[kStRead] = kept;
+			st[kStNodes] = (int)fmin(nodes, 2147483647.0);
 		}
 	}
 }
@@ -704,7 +881,7 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_pair_cull(PairArgs A) {
 // writes position, normal and depth.
 __device__ __forceinline__ bool sample_hit(const PairArgs& A, const double* Rq, const double* xq, const double* Rg,
                                            const double* xg, const GridDesc& G, int s, double sign, double* out7,
-                                           int* tie) {
+                                           int* tie, int* ingrid) {
 	const float* l = &A.samples[3 * (size_t)s];
 	const double l0 = (double)l[0], l1 = (double)l[1], l2 = (double)l[2];
 	double p[3];
@@ -717,6 +894,7 @@ __device__ __forceinline__ bool sample_hit(const PairArgs& A, const double* Rq, 
 	auto fetch = [&](int i, int j, int k) -> double { return (double)pool[grid_index(n, i, j, k)]; };
 	double sd, g[3];
 	if (!grid_query(G, q, fetch, &sd, g)) return false;
+	*ingrid += 1;
 	if (fabs(sd) <= A.tie_band) *tie += 1;
 	if (!(sd < 0.0)) return false;
 	double gw[3];
@@ -745,7 +923,7 @@ __device__ __forceinline__ bool pos_less(double ax, double ay, double az, double
 // (side, cell, sample) order into the pair's staging slots.
 __global__ __launch_bounds__(256) void k_pair_hits(PairArgs A) {
 	__shared__ int wcnt[4];
-	__shared__ int s_nh, s_ties;
+	__shared__ int s_nh, s_ties, s_in;
 	__shared__ int sel_beg[64], pre[65];
 	__shared__ int s_nsel;
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -755,15 +933,19 @@ __global__ __launch_bounds__(256) void k_pair_hits(PairArgs A) {
 		const int base = A.stage_off[p];
 		if ((int64_t)A.stage_off[p + 1] > A.stage_cap) {
 			// staging too small: the host grows it and detects again
-			if (tid == 0) A.pair_nh[p] = 0;
+			if (tid == 0) {
+				A.pair_nh[p] = 0;
+				A.pair_st[kPairStats * (size_t)p + kStInGrid] = 0;
+				A.pair_st[kPairStats * (size_t)p + kStTies] = 0;
+			}
 			continue;
 		}
 		double Ra[9], Rb[9], xa[3], xb[3];
 		for (int k = 0; k < 9; ++k) { Ra[k] = A.R[9 * a + k]; Rb[k] = A.R[9 * b + k]; }
 		for (int k = 0; k < 3; ++k) { xa[k] = A.pos[3 * a + k]; xb[k] = A.pos[3 * b + k]; }
 		__syncthreads();
-		if (tid == 0) { s_nh = 0; s_ties = 0; }
-		int myties = 0;
+		if (tid == 0) { s_nh = 0; s_ties = 0; s_in = 0; }
+		int myties = 0, myin = 0;
 		for (int pass = 0; pass < 2; ++pass) {
 			const int q = pass == 0 ? a : b, g = pass == 0 ? b : a;
 			if (!A.emit[q]) continue;
@@ -825,7 +1007,7 @@ __global__ __launch_bounds__(256) void k_pair_hits(PairArgs A) {
 						else hi = mid - 1;
 					}
 					const int smp = s0 + sel_beg[lo] + (t - pre[lo]);
-					hit = sample_hit(A, Rq, xq, Rg, xg, Gg, smp, sign, h7, &myties);
+					hit = sample_hit(A, Rq, xq, Rg, xg, Gg, smp, sign, h7, &myties, &myin);
 				}
 				const unsigned long long m = __ballot(hit);
 				const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -847,12 +1029,13 @@ __global__ __launch_bounds__(256) void k_pair_hits(PairArgs A) {
 			__syncthreads();
 		}
 		if (myties) atomicAdd(&s_ties, myties);
+		if (myin) atomicAdd(&s_in, myin);
 		__syncthreads();
 		if (tid == 0) {
 			const int nh = s_nh;
 			A.pair_nh[p] = nh;
-			if (s_ties) atomicAdd(&A.sc->ties, s_ties);
-			if (nh) atomicAdd(&A.sc->n_raw, nh);
+			A.pair_st[kPairStats * (size_t)p + kStInGrid] = s_in;
+			A.pair_st[kPairStats * (size_t)p + kStTies] = s_ties;
 			if (nh > kSortCap) A.big[atomicAdd(&A.sc->n_big, 1)] = p;
 		}
 	}
@@ -1080,12 +1263,36 @@ __global__ __launch_bounds__(256) void k_pair_merge_big(PairArgs A) {
 }
 
 // Totals of the detection for the host's single read-back.
-__global__ void k_contact_totals(PairArgs A, const int* __restrict__ coff) {
-	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One workgroup: the detection totals (pair, staging and contact counts; the
+// per-pair counters and raw hits summed).
+__global__ __launch_bounds__(1024) void k_contact_totals(PairArgs A, const int* __restrict__ coff) {
+	__shared__ unsigned long long part[16][kPairStats + 1];
 	const int P = pairs_of(A);
-	A.sc->n_pairs = *A.np;
-	A.sc->n_stage = A.stage_off[P];
-	A.sc->n_contacts = coff[P];
+	unsigned long long t[kPairStats + 1] = {};
+	for (int p = threadIdx.x; p < P; p += blockDim.x) {
+		for (int k = 0; k < kPairStats; ++k) t[k] += (unsigned long long)A.pair_st[kPairStats * (size_t)p + k];
+		t[kPairStats] += (unsigned long long)A.pair_nh[p];
+	}
+	for (int k = 0; k <= kPairStats; ++k)
+		for (int off = 32; off > 0; off >>= 1) t[k] += __shfl_xor(t[k], off, 64);
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	if (lane == 0)
+		for (int k = 0; k <= kPairStats; ++k) part[wid][k] = t[k];
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		unsigned long long u[kPairStats + 1] = {};
+		for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+			for (int k = 0; k <= kPairStats; ++k) u[k] += part[w][k];
+		A.sc->n_pairs = *A.np;
+		A.sc->n_stage = A.stage_off[P];
+		A.sc->n_contacts = coff[P];
+		A.sc->all_samples = u[kStAll];
+		A.sc->read_samples = u[kStRead];
+		A.sc->brick_nodes = u[kStNodes];
+		A.sc->ingrid_samples = u[kStInGrid];
+		A.sc->ties = (int)u[kStTies];
+		A.sc->n_raw = (int)u[kPairStats];
+	}
 }
 
 // One workgroup per pair: final contact records in pair order.
@@ -1154,7 +1361,7 @@ void detect_contacts(pdc_context* ctx) {
 		const int cap = ctx->pr_cap;
 		ctx->pr_a.ensure(cap); ctx->pr_b.ensure(cap); ctx->pr_tmp.ensure(cap + 1);
 		ctx->pr_soff.ensure(cap + 1); ctx->pr_nc.ensure(cap + 1); ctx->pr_coff.ensure(cap + 1);
-		ctx->pr_nh.ensure(cap + 1); ctx->pr_big.ensure(cap);
+		ctx->pr_nh.ensure(cap + 1); ctx->pr_big.ensure(cap); ctx->pr_st.ensure(kPairStats * (size_t)cap);
 		ctx->stage_raw.ensure(7 * (size_t)ctx->stage_cap);
 		ctx->stage_m.ensure(7 * (size_t)ctx->stage_cap);
 		ctx->stage_idx.ensure(2 * (size_t)ctx->stage_cap);
@@ -1169,7 +1376,7 @@ void detect_contacts(pdc_context* ctx) {
 		A.pool = ctx->grid_pool.p; A.stage_cnt = ctx->pr_tmp.p; A.stage_off = ctx->pr_soff.p;
 		A.stage_cap = ctx->stage_cap; A.stage_raw = ctx->stage_raw.p; A.stage_m = ctx->stage_m.p;
 		A.stage_idx = ctx->stage_idx.p; A.big = ctx->pr_big.p; A.pair_nh = ctx->pr_nh.p;
-		A.pair_nc = ctx->pr_nc.p; A.sc = ctx->d_sc.p; A.tie_band = ctx->tie_band;
+		A.pair_nc = ctx->pr_nc.p; A.pair_st = ctx->pr_st.p; A.sc = ctx->d_sc.p; A.tie_band = ctx->tie_band;
 		// launch width from the last detection; the kernels stride over the
 		// pairs there are, so the width only sets the parallelism
 		const int hint = std::min(cap, std::max(256, ctx->n_pairs + ctx->n_pairs / 8 + 64));
@@ -1187,7 +1394,7 @@ void detect_contacts(pdc_context* ctx) {
 		hipLaunchKernelGGL(k_pair_merge_big, dim3(64), dim3(256), 0, s, A);
 		PDC_CHECK_LAUNCH();
 		scan_exclusive_dev(ctx, ctx->pr_nc.p, ctx->pr_coff.p, cap, A.np);
-		hipLaunchKernelGGL(k_contact_totals, dim3(1), dim3(64), 0, s, A, (const int*)ctx->pr_coff.p);
+		hipLaunchKernelGGL(k_contact_totals, dim3(1), dim3(1024), 0, s, A, (const int*)ctx->pr_coff.p);
 		PDC_CHECK_LAUNCH();
 		sync_scalars(ctx);
 		const Scalars& h = *ctx->h_sc;
@@ -1213,13 +1420,18 @@ void detect_contacts(pdc_context* ctx) {
 	ctx->det.raw_hits = h.n_raw;
 	ctx->det.ties = h.ties;
 	ctx->det.samples_all = (int64_t)h.all_samples;
+	ctx->det.samples_in_grid = (int64_t)h.ingrid_samples;
 	ctx->det.samples_read = (int64_t)h.read_samples;
 	ctx->det.brick_nodes = (int64_t)h.brick_nodes;
 	ctx->det.large_pairs = h.n_big;
 	if (ctx->kt.pending_contact_slots) {
-		// the sample side of (a): every sample of both emitting sides of every
-		// pair in FP32, and the grid nodes under each partner's box
-		ctx->kt.bytes += 12.0 * (double)h.all_samples + 4.0 * (double)h.brick_nodes;
+		// algorithmic bytes of the cull and the sample pass: per pair the two
+		// bodies' rotations, positions and grid descriptors (368 B); per kept
+		// sample its FP32 body-frame position (12 B); per sample inside the
+		// partner's grid box the 8 FP32 nodes it interpolates (32 B); per raw
+		// hit its staged record (56 B)
+		ctx->kt.bytes += 368.0 * (double)h.n_pairs + 12.0 * (double)h.read_samples +
+		                 32.0 * (double)h.ingrid_samples + 56.0 * (double)h.n_raw;
 		ctx->kt.pending_contact_slots = 0;
 	}
 	if (C == 0 || P == 0) return;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -246,13 +247,13 @@ struct Scalars {
 	unsigned long long all_samples;   // samples of the emitting sides of every pair
 	unsigned long long read_samples;  // samples in the cells the cull keeps
 	unsigned long long brick_nodes;   // grid nodes under the partner's box, per pair and side
-	unsigned long long pad2;
+	unsigned long long ingrid_samples;  // tested samples inside the partner's grid box
 };
 
 // Counts of the last contact detection (pdc_get_detection_stats).
 struct DetectionStats {
 	int64_t pairs = 0, raw_hits = 0, ties = 0;
-	int64_t samples_all = 0, samples_read = 0, brick_nodes = 0, large_pairs = 0;
+	int64_t samples_all = 0, samples_read = 0, brick_nodes = 0, large_pairs = 0, samples_in_grid = 0;
 };
 
 // Body-frame sample cell: the box of its FP32 samples and their range
@@ -260,6 +261,17 @@ struct DetectionStats {
 struct SampleCell {
 	float lo[3], hi[3];
 	int beg, end;
+};
+
+// A triangle mesh registered with pdc_register_mesh.
+struct MeshShape {
+	std::vector<double> verts;  // 3 per vertex, body frame (the surface samples)
+	std::vector<double> recs;   // kTriRec per triangle of nonzero area (the distance)
+	double ext[3] = {0, 0, 0};  // half extents of the grid box before padding
+	DBuf<double> d_recs;
+	DBuf<float> grid;           // its SDF nodes at grid_n / grid_pad (built on first use)
+	int grid_n = 0;
+	double grid_pad = -1.0;
 };
 
 // Size class of a scene of n bodies (index into pdc_shapes' per-class arrays).
@@ -306,6 +318,7 @@ struct pdc_context {
 	pdc::DBuf<float> samples;                              // 3 per sample (FP32, body frame, by cell)
 	pdc::DBuf<pdc::SampleCell> cells;                      // body b's at floor(soff[b] / kCellStride)
 	pdc::DBuf<int> b_ncell;                                // cells per body
+	std::vector<std::unique_ptr<pdc::MeshShape>> meshes;   // registered meshes, by id
 	std::vector<int> h_soff;                               // host copy of sample offsets
 	std::vector<int> h_static, h_emit;                     // host copies of the flags
 	int64_t dev_bytes = 0;                                 // device bytes held by the context's buffers
@@ -321,6 +334,7 @@ struct pdc_context {
 	pdc::DBuf<double> stage_m;                              // 7 per slot: merged
 	pdc::DBuf<int> stage_idx;                               // 2 per slot: sort indices of large pairs
 	pdc::DBuf<int> pr_big;                                  // pairs with more than kSortCap hits
+	pdc::DBuf<int> pr_st;                                   // per pair detection counters
 	int pr_cap = 0;                                         // pair capacity
 	int64_t stage_cap = 0;                                  // staging slots
 
@@ -453,6 +467,8 @@ void kt_end_nocollect(pdc_context* ctx, hipEvent_t ev0);
 void kt_end_span(pdc_context* ctx, hipEvent_t ev0, int launches);
 
 // pdc_contacts.hip
+// Registers a triangle mesh (vertices in the body frame); returns its id.
+int register_mesh(pdc_context* ctx, int nv, const double* v, int nt, const int* tri);
 void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims, const double* mass,
                    const double* inertia, const int* is_static, const int* emit);
 // Bodies appended at the end of the table as a new last scene (grids and

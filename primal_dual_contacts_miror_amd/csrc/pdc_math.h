@@ -307,6 +307,88 @@ PDC_HD double analytic_sdf(int type, const double* dims, double px, double py, d
 	return outside + inside;
 }
 
+// ---------------------------------------------------------------------------
+// Triangle meshes (Mesh::signedDistance, src/CollisionObject/Mesh.cpp:125-167,
+// which calls libigl's signed_distance_pseudonormal).  A triangle is a record
+// of kTriRec doubles: vertices a, b, c (0-8), the unit face normal (9-11), the
+// pseudonormals of edges ab, bc, ca (12-20) and of vertices a, b, c (21-29).
+// The signed distance of a point is its distance to the closest point of the
+// mesh, negative when the point lies behind the pseudonormal of the feature
+// (face, edge or vertex) holding that closest point (Baerentzen and Aanaes,
+// "Signed distance computation using the angle weighted pseudonormal", IEEE
+// TVCG 11 (2005)).  Among triangles at the same squared distance the first in
+// triangle order decides.
+// ---------------------------------------------------------------------------
+constexpr int kTriRec = 30;
+
+// Closest point of triangle t to p by the Voronoi regions of its vertices,
+// edges and face (Ericson, Real-Time Collision Detection, 2005, sec. 5.1.5).
+// Returns the squared distance; *feat: 0 face, 1-3 vertex a, b, c, 4 edge ab,
+// 5 edge bc, 6 edge ca.
+PDC_HD double tri_closest(const double* t, const double* p, double* cp, int* feat) {
+	const double* a = t;
+	const double* b = t + 3;
+	const double* c = t + 6;
+	const double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+	const double ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+	const double ap[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+	const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+	int f;
+	if (d1 <= 0.0 && d2 <= 0.0) {
+		cp[0] = a[0]; cp[1] = a[1]; cp[2] = a[2];
+		f = 1;
+	} else {
+		const double bp[3] = {p[0] - b[0], p[1] - b[1], p[2] - b[2]};
+		const double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+		const double cq[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+		const double d5 = dot3(ab, cq), d6 = dot3(ac, cq);
+		const double vc = d1 * d4 - d3 * d2;
+		const double vb = d5 * d2 - d1 * d6;
+		const double va = d3 * d6 - d5 * d4;
+		if (d3 >= 0.0 && d4 <= d3) {
+			cp[0] = b[0]; cp[1] = b[1]; cp[2] = b[2];
+			f = 2;
+		} else if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+			const double v = d1 / (d1 - d3);
+			for (int k = 0; k < 3; ++k) cp[k] = a[k] + v * ab[k];
+			f = 4;
+		} else if (d6 >= 0.0 && d5 <= d6) {
+			cp[0] = c[0]; cp[1] = c[1]; cp[2] = c[2];
+			f = 3;
+		} else if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+			const double w = d2 / (d2 - d6);
+			for (int k = 0; k < 3; ++k) cp[k] = a[k] + w * ac[k];
+			f = 6;
+		} else if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+			const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+			for (int k = 0; k < 3; ++k) cp[k] = b[k] + w * (c[k] - b[k]);
+			f = 5;
+		} else {
+			const double den = 1.0 / (va + vb + vc);
+			const double v = vb * den, w = vc * den;
+			for (int k = 0; k < 3; ++k) cp[k] = a[k] + ab[k] * v + ac[k] * w;
+			f = 0;
+		}
+	}
+	*feat = f;
+	const double d[3] = {p[0] - cp[0], p[1] - cp[1], p[2] - cp[2]};
+	return dot3(d, d);
+}
+
+// One triangle of a mesh distance query: keeps the smallest squared distance
+// (*best) and the sign of its feature's pseudonormal side (*sgn).
+PDC_HD void mesh_sdf_step(const double* t, const double* p, double* best, double* sgn) {
+	double cp[3];
+	int f;
+	const double d2 = tri_closest(t, p, cp, &f);
+	if (d2 < *best) {
+		const double* n = f == 0 ? t + 9 : f <= 3 ? t + 21 + 3 * (f - 1) : t + 12 + 3 * (f - 4);
+		const double d[3] = {p[0] - cp[0], p[1] - cp[1], p[2] - cp[2]};
+		*best = d2;
+		*sgn = dot3(d, n) < 0.0 ? -1.0 : 1.0;
+	}
+}
+
 struct GridDesc {
 	double lo[3];
 	double sp[3];

@@ -1578,7 +1578,6 @@ __global__ __launch_bounds__(256) void k_asm_off(SArgs a, const T* __restrict__ 
 __device__ __forceinline__ bool spd6_inverse(const double* S, double (&P)[36]) {
 	double L[21], rd[6];
 	bool ok = true;
-	int q = 0;
 #pragma unroll
 	for (int j = 0; j < 6; ++j) {
 #pragma unroll
@@ -1593,7 +1592,6 @@ __device__ __forceinline__ bool spd6_inverse(const double* S, double (&P)[36]) {
 			} else {
 				L[i * (i + 1) / 2 + j] = t * rd[j];
 			}
-			++q;
 		}
 	}
 #pragma unroll

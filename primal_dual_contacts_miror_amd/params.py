@@ -13,6 +13,7 @@ import ctypes
 SHAPE_BOX = 0
 SHAPE_SPHERE = 1
 SHAPE_ELLIPSOID = 2
+SHAPE_MESH = 3
 NON_SMOOTH_CONTACT = 0
 ONLY_NORMAL = 1
 PCG_FP64 = 0
@@ -77,7 +78,8 @@ class PdcTimings(ctypes.Structure):
 class PdcDetectionStats(ctypes.Structure):
     """Counts of the last contact detection (pdc_get_detection_stats)."""
     _fields_ = [(name, ctypes.c_int64) for name in
-                ("pairs", "raw_hits", "ties", "samples_all", "samples_read", "brick_nodes", "large_pairs")]
+                ("pairs", "raw_hits", "ties", "samples_all", "samples_read", "brick_nodes", "large_pairs",
+                 "samples_in_grid")]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

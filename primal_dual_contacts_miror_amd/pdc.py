@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-from .params import PdcDetectionStats, PdcParams, PdcShapes, PdcSolveStats, PdcTimings
+from .params import SHAPE_MESH, PdcDetectionStats, PdcParams, PdcShapes, PdcSolveStats, PdcTimings
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libpdc.so")
@@ -39,7 +39,8 @@ EXPORTS = [
     "pdc_set_scenes", "pdc_scene_count", "pdc_set_scene_friction", "pdc_step_scenes",
     "pdc_get_shapes", "pdc_set_shapes", "pdc_autotune",
     "pdc_set_state_range", "pdc_get_state_range", "pdc_append_bodies", "pdc_replace_bodies",
-    "pdc_device_bytes", "pdc_reserve_bodies", "pdc_get_groups", "pdc_get_detection_stats",
+    "pdc_device_bytes", "pdc_reserve_bodies", "pdc_get_groups", "pdc_get_detection_stats", "pdc_register_mesh",
+    "pdc_get_grid_nodes",
 ]
 
 
@@ -79,6 +80,8 @@ def load_library(path=LIB_PATH):
     L.pdc_last_error.argtypes = [V]
     L.pdc_last_error.restype = ctypes.c_char_p
     L.pdc_upload_bodies.argtypes = [V, ctypes.c_int32, _I, _D, _D, _D, _I, _I]
+    L.pdc_register_mesh.argtypes = [V, ctypes.c_int32, _D, ctypes.c_int32, _I, _I]
+    L.pdc_get_grid_nodes.argtypes = [V, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_float)]
     L.pdc_set_state.argtypes = [V, _D, _D, _D, _D]
     L.pdc_set_state_range.argtypes = [V, ctypes.c_int32, ctypes.c_int32, _D, _D, _D, _D]
     L.pdc_get_state_range.argtypes = [V, ctypes.c_int32, ctypes.c_int32, _D, _D, _D, _D]
@@ -165,6 +168,39 @@ class Context:
     def configure(self, params):
         self._check(self._L.pdc_configure(self.h, ctypes.byref(params)))
         self.params = params
+
+    def register_mesh(self, vertices, triangles):
+        """Registers a triangle mesh (body-frame vertices (V,3), triangles
+        (T,3)); returns its id, which mesh bodies carry in dims[:, 0]."""
+        v = _f64(vertices).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+        mid = ctypes.c_int32(-1)
+        self._check(self._L.pdc_register_mesh(self.h, v.shape[0], _d(v), t.shape[0], _i(t), ctypes.byref(mid)))
+        return mid.value
+
+    def mesh_dims(self, scene):
+        """The scene's dims with its mesh bodies' mesh indices replaced by
+        this context's ids (the scene's meshes registered on first use)."""
+        meshes = getattr(scene, "meshes", None) or []
+        dims = _f64(scene.dims).reshape(-1, 3).copy()
+        if not meshes:
+            return dims
+        reg = self.__dict__.setdefault("_mesh_ids", {})
+        key = id(meshes)
+        if key not in reg:
+            reg[key] = (meshes, [self.register_mesh(v, t) for v, t in meshes])
+        ids = reg[key][1]
+        m = np.asarray(scene.shape_type) == SHAPE_MESH
+        dims[m, 0] = [ids[int(k)] for k in dims[m, 0]]
+        return dims
+
+    def grid_nodes(self, body):
+        """The FP32 SDF grid of a body, (n, n, n) indexed [k, j, i]."""
+        n = int(self.params.grid_n)
+        out = np.zeros(n * n * n, np.float32)
+        self._check(self._L.pdc_get_grid_nodes(self.h, int(body), out.size,
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out.reshape(n, n, n)
 
     def upload_bodies(self, shape_type, dims, mass, inertia, is_static, emit=None):
         st = np.ascontiguousarray(shape_type, dtype=np.int32)
@@ -345,7 +381,7 @@ class PrimalDualForceSolver:
 
     def __init__(self, scene, params=None, device=0):
         self.ctx = Context(params if params is not None else scene.params, device)
-        self.ctx.upload_bodies(scene.shape_type, scene.dims, scene.mass, scene.inertia,
+        self.ctx.upload_bodies(scene.shape_type, self.ctx.mesh_dims(scene), scene.mass, scene.inertia,
                                scene.is_static, scene.emit)
 
     def fill_forces(self, contacts, quat, vel, angvel):
@@ -359,7 +395,7 @@ class DynamicSystem:
     def __init__(self, scene, params=None, device=0):
         self.scene = scene
         self.ctx = Context(params if params is not None else scene.params, device)
-        self.ctx.upload_bodies(scene.shape_type, scene.dims, scene.mass, scene.inertia,
+        self.ctx.upload_bodies(scene.shape_type, self.ctx.mesh_dims(scene), scene.mass, scene.inertia,
                                scene.is_static, scene.emit)
         self.ctx.set_state(scene.pos, scene.quat, scene.vel, scene.angvel)
 
@@ -389,7 +425,7 @@ class BatchedSystem:
         self.scenes = list(scenes)
         merged, self.offsets, mu = concat(self.scenes)
         self.ctx = Context(params if params is not None else merged.params, device)
-        self.ctx.upload_bodies(merged.shape_type, merged.dims, merged.mass, merged.inertia,
+        self.ctx.upload_bodies(merged.shape_type, self.ctx.mesh_dims(merged), merged.mass, merged.inertia,
                                merged.is_static, merged.emit)
         self.ctx.set_scenes(self.offsets)
         self.ctx.set_scene_friction(mu)

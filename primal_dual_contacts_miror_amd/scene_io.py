@@ -1,13 +1,16 @@
 """Scene input and output of the reference's command-line simulator.
 
 Input (src/IO/FileLoader.cpp:14-259): a YAML scene file names glTF/GLB files
-per primitive kind; every node of a file that carries a mesh becomes a rigid
-body, fitted to the primitive (box, sphere or ellipsoid) with the same
+per kind; every node of a file that carries a mesh becomes a rigid body.
+`cubes`, `ellipsoids` and `spheres` are fitted to the primitive with the same
 inertia as the mesh under the node's transformation
 (RigidObjectBase.cpp:121-210, Cube.cpp:5-65, Sphere.cpp:6-37,
-Ellipsoid.cpp:6-36).  Bodies are numbered in load order: kinds
-in the order meshes, convex, cubes, ellipsoids, spheres; files in list
-order; nodes in node order.  `static` and `initialize` refer to those ids.
+Ellipsoid.cpp:6-36); `meshes` and `convex` keep the triangle mesh itself
+(Mesh.cpp:18-110, Convex.cpp:14-39), as a device mesh whose SDF grid is its
+pseudonormal signed distance and whose surface samples are its vertices.
+Bodies are numbered in load order: kinds in the order meshes, convex, cubes,
+ellipsoids, spheres; files in list order; nodes in node order.  `static` and
+`initialize` refer to those ids.
 
 Output (src/IO/FileSaver.cpp:44-206): `<name>.gltf` (+ `.bin`) with one mesh
 `mesh_i` per body (the collision geometry: unit cube or icosphere, scaled
@@ -18,8 +21,7 @@ permanent transformation, then per saved frame the translation and the
 rotation quaternion (x, y, z, w); `<name>.info` with the timing summary
 (DynamicSystem.cpp:299-313, PrimalDualForceSolver.cpp:636-648).
 
-Only the primitive kinds the device pipeline supports (cubes, ellipsoids,
-spheres) are loaded; `meshes` and `convex` are rejected with an error.
+A mesh body's collision geometry in the output is its own mesh.
 """
 import base64
 import json
@@ -310,6 +312,7 @@ class FittedBody:
     mesh_trans: np.ndarray     # 4x4 input mesh in the body frame
     min_length: float          # obbMinLength (Cube.cpp:66-70, Sphere.cpp:38-41, Ellipsoid.cpp:37-41)
     source: tuple = ()
+    mesh_vertices: np.ndarray = None  # mesh bodies: the vertices in the body frame
 
 
 def fit_primitive(kind, positions, triangles, matrix):
@@ -381,6 +384,31 @@ def fit_primitive(kind, positions, triangles, matrix):
         dims = np.array([radius, radius, radius])
         scale = dims.copy()
         min_len = radius
+    elif kind == P.SHAPE_MESH:
+        # Mesh.cpp:18-110 (and Convex.cpp:14-39): the mesh itself, its volume
+        # and inertia tensor from the signed tetrahedra.  The body frame is
+        # the principal frame of that tensor (the reference keeps the full
+        # tensor in the node's rotation; the motion is the same), so the
+        # vertices are stored in it.
+        r = U.copy()
+        if np.linalg.det(r) < 0:
+            r[:, 2] *= -1
+        V = vol
+        inertia = np.asarray(sv, dtype=np.float64).copy()
+        dims = np.zeros(3)
+        verts = v @ r
+        # obbMinLength (Mesh.cpp:112-123): the shortest side of the box of the
+        # vertices in the principal frame of their covariance about the centre
+        # of mass (the origin of the body frame)
+        _, Q = np.linalg.eigh(verts.T @ verts / max(len(verts), 1))
+        loc = verts @ Q
+        min_len = float((loc.max(axis=0) - loc.min(axis=0)).min())
+        R = rot @ r
+        mesh_trans = np.eye(4)
+        mesh_trans[:3, :3] = r.T @ mesh_lin
+        mesh_trans[:3, 3] = r.T @ mesh_t
+        return FittedBody(kind, dims, V, inertia, position, R, mesh_trans, mesh_trans, min_len,
+                          mesh_vertices=np.ascontiguousarray(verts))
     else:
         raise ValueError("unsupported primitive kind %r" % kind)
     R = rot @ r
@@ -410,8 +438,8 @@ class LoadedScene:
     path: str = ""
 
 
-_KINDS = [("meshes", None), ("convex", None), ("cubes", P.SHAPE_BOX), ("ellipsoids", P.SHAPE_ELLIPSOID),
-          ("spheres", P.SHAPE_SPHERE)]
+_KINDS = [("meshes", P.SHAPE_MESH), ("convex", P.SHAPE_MESH), ("cubes", P.SHAPE_BOX),
+          ("ellipsoids", P.SHAPE_ELLIPSOID), ("spheres", P.SHAPE_SPHERE)]
 
 
 def _scalar(node, key, default):
@@ -482,18 +510,19 @@ def load_scene(path, grid_n=64, sample_spacing=None, pcg_precision=None, pcg_tol
         root = yaml.safe_load(f)
     sn = root.get("scene") or {}
     base = os.path.dirname(os.path.abspath(path))
-    bodies, meshes = [], []
+    bodies, meshes, body_meshes = [], [], []
     for key, kind in _KINDS:
         if key not in sn or sn[key] is None:
             continue
-        if kind is None:
-            raise NotImplementedError("scene kind %r is not supported by the device pipeline "
-                                      "(boxes, spheres and ellipsoids only)" % key)
         files = sn[key] if isinstance(sn[key], list) else [sn[key]]
         for fn in files:
             for nd in read_gltf(os.path.join(base, str(fn))):
                 fb = fit_primitive(kind, nd.positions, nd.triangles, nd.matrix)
                 fb.source = (str(fn), nd.index, nd.name)
+                if kind == P.SHAPE_MESH:
+                    # one device mesh per body: the body-frame vertices differ with the node's scaling
+                    fb.dims = np.array([float(len(body_meshes)), 0.0, 0.0])
+                    body_meshes.append((fb.mesh_vertices, np.asarray(nd.triangles, np.int32).reshape(-1, 3)))
                 bodies.append(fb)
                 meshes.append((nd.positions, nd.triangles))
     n = len(bodies)
@@ -522,7 +551,8 @@ def load_scene(path, grid_n=64, sample_spacing=None, pcg_precision=None, pcg_tol
         static[int(k)] = 1
     sim = root.get("simulation") or {}
     p = solver_params(sim)
-    dyn_len = [float(np.min(b.dims)) for b, s in zip(bodies, static) if not s]
+    dyn_len = [float(np.min(b.dims)) if b.kind != P.SHAPE_MESH else 0.5 * b.min_length
+               for b, s in zip(bodies, static) if not s]
     if sample_spacing is None:
         sample_spacing = 0.25 * min(dyn_len) if dyn_len else 0.1
     p.sample_spacing = float(sample_spacing)
@@ -545,6 +575,7 @@ def load_scene(path, grid_n=64, sample_spacing=None, pcg_precision=None, pcg_tol
         params=p,
         steps=0,
         meta={"source": path},
+        meshes=body_meshes,
     )
     bake = root.get("baking") or {}
     adaptive = _as_bool(_scalar(sim, "adaptive_timestep", False))
@@ -597,6 +628,25 @@ def icosphere_mesh(subdivisions=3):
     return np.array(verts), np.array(tri, dtype=np.int64)
 
 
+def torus_mesh(R=1.0, r=0.25, n_major=24, n_minor=12):
+    """Torus around the y axis (major radius R, tube radius r): an
+    n_major x n_minor grid of vertices, two outward triangles per cell."""
+    u = 2.0 * math.pi * np.arange(n_major) / n_major
+    w = 2.0 * math.pi * np.arange(n_minor) / n_minor
+    U, W = np.meshgrid(u, w, indexing="ij")
+    rad = R + r * np.cos(W)
+    v = np.stack([rad * np.cos(U), r * np.sin(W), rad * np.sin(U)], axis=-1).reshape(-1, 3)
+    tri = []
+    for i in range(n_major):
+        for j in range(n_minor):
+            a = i * n_minor + j
+            b = ((i + 1) % n_major) * n_minor + j
+            c = ((i + 1) % n_major) * n_minor + (j + 1) % n_minor
+            d = i * n_minor + (j + 1) % n_minor
+            tri += [(a, d, c), (a, c, b)]
+    return v, np.array(tri, dtype=np.int64)
+
+
 def _rigid(pos, quat):
     M = np.eye(4)
     M[:3, :3] = quat_to_matrix(quat)
@@ -628,6 +678,13 @@ def _aabb_max_vertex_speed(loaded, pos, quat, vel, ang):
     allv = np.zeros((n, 8, 3))
     for k in range(n):
         R = quat_to_matrix(quat[k])
+        if sc.shape_type[k] == P.SHAPE_MESH:
+            # the box of the rotated vertices, its centre off the body centre
+            wv = sc.meshes[int(sc.dims[k][0])][0] @ R.T
+            lo, hi = wv.min(axis=0), wv.max(axis=0)
+            rel = 0.5 * (lo + hi) + corners * (0.5 * (hi - lo))
+            allv[k] = vel[k] - np.cross(rel, ang[k])
+            continue
         if sc.shape_type[k] == P.SHAPE_BOX:
             half = np.abs(R) @ sc.dims[k]
         elif sc.shape_type[k] == P.SHAPE_ELLIPSOID:
@@ -719,7 +776,7 @@ def write_outputs(loaded, out_path, times, states):
     meshes, mats, trans = [], [], []
     fpos, fquat = states[-1] if states else (loaded.scene.pos, loaded.scene.quat)
     for k, b in enumerate(loaded.bodies):
-        if coll:
+        if coll and b.kind != P.SHAPE_MESH:
             meshes.append(cube if b.kind == P.SHAPE_BOX else ico)
             tm = b.permanent
         else:

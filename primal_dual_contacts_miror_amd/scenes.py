@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .params import SHAPE_BOX, SHAPE_ELLIPSOID, SHAPE_SPHERE, PCG_FP32, PCG_FP64, default_params
+from .params import SHAPE_BOX, SHAPE_ELLIPSOID, SHAPE_MESH, SHAPE_SPHERE, PCG_FP32, PCG_FP64, default_params
 
 
 @dataclass
@@ -39,6 +39,8 @@ class Scene:
     params: object = None       # PdcParams
     steps: int = 1
     meta: dict = field(default_factory=dict)
+    meshes: list = field(default_factory=list)  # (vertices (V,3) body frame, triangles (T,3)) per mesh;
+                                                # a mesh body holds its index in dims[:, 0]
 
     @property
     def n(self):
@@ -57,6 +59,7 @@ class Scene:
 class _Builder:
     def __init__(self):
         self.rows = []
+        self.meshes = []
 
     def box(self, half, pos, quat=(0, 0, 0, 1), density=1.0, static=False, emit=None):
         hx, hy, hz = half
@@ -80,6 +83,27 @@ class _Builder:
         self.rows.append((SHAPE_ELLIPSOID, (a, b_, c), m, inertia, static,
                           (not static) if emit is None else emit, pos, quat))
 
+    def mesh(self, vertices, triangles, pos, quat=(0, 0, 0, 1), density=1.0, static=False, emit=None):
+        """A triangle-mesh body.  `vertices` are in the mesh's own frame; the
+        body frame is the centre of mass and principal axes of the mesh as a
+        solid (Mesh.cpp:18-110), and `pos` / `quat` place the mesh's own frame."""
+        from .scene_io import fit_primitive, matrix_to_quat, quat_to_matrix
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+        fb = fit_primitive(SHAPE_MESH, v, t, np.eye(4))
+        R0 = quat_to_matrix(quat)
+        wpos = np.asarray(pos, np.float64) + R0 @ fb.position
+        wq = tuple(matrix_to_quat(R0 @ fb.rotation))
+        # bodies of the same mesh share one mesh entry
+        key = (v.tobytes(), t.tobytes())
+        index = self.__dict__.setdefault("_mesh_index", {})
+        if key not in index:
+            index[key] = len(self.meshes)
+            self.meshes.append((fb.mesh_vertices, t))
+        self.rows.append((SHAPE_MESH, (float(index[key]), 0.0, 0.0), density * fb.volume,
+                          tuple(density * fb.inertia_unit), static, (not static) if emit is None else emit,
+                          tuple(wpos), wq))
+
     def build(self, name, params, steps, **meta):
         n = len(self.rows)
         sc = Scene(
@@ -97,6 +121,7 @@ class _Builder:
             params=params,
             steps=steps,
             meta=dict(meta),
+            meshes=list(self.meshes),
         )
         return sc
 
@@ -245,6 +270,73 @@ def ellipsoid_pile(n_bodies=300, seed=0, half=4.0, height=16.0, semi=(0.15, 0.5)
     return b.build(name or "ellipsoid_pile_%d" % n_bodies, p, steps=1000, n_dynamic=n_bodies, seed=seed)
 
 
+def mesh_pile(n_bodies=40, seed=0, half=2.0, height=6.0, size=(0.2, 0.45), n_shapes=8, name=None,
+              **param_overrides):
+    """Triangle-mesh bodies (Mesh.cpp) with random orientations dropped into
+    a static open container: icospheres stretched along their axes, boxes
+    stored with a vertex per face corner (as glTF files hold them), dented
+    spheres (non-convex) and tori; n_shapes distinct meshes (kind and size,
+    drawn from `size`) shared among the bodies."""
+    from .scene_io import icosphere_mesh, torus_mesh, unit_cube_mesh
+    rng = np.random.default_rng(seed)
+    ico_v, ico_t = icosphere_mesh(2)
+    cube_v, cube_t = unit_cube_mesh()
+    cube_v, cube_t = cube_v[cube_t.reshape(-1)], np.arange(3 * len(cube_t)).reshape(-1, 3)
+    dent_v = ico_v.copy()
+    dent_v[np.abs(dent_v[:, 1]) < 0.35] *= 0.6
+    tor_v, tor_t = torus_mesh(1.0, 0.35, 20, 10)
+    # n_shapes distinct meshes (kind and scale), shared by the bodies
+    shape_kind = rng.integers(0, 4, n_shapes)
+    shape_scale = rng.uniform(size[0], size[1], (n_shapes, 3))
+    pick = rng.integers(0, n_shapes, n_bodies)
+    kinds, scales = shape_kind[pick], shape_scale[pick]
+    radius = []
+    meshes = []
+    for k, s in zip(kinds, scales):
+        if k == 0:
+            v, t = ico_v * s, ico_t
+        elif k == 1:
+            v, t = cube_v * (0.8 * s), cube_t
+        elif k == 2:
+            v, t = dent_v * s[0], ico_t
+        else:
+            v, t = tor_v * (s[0] / 1.35), tor_t
+        meshes.append((v, t))
+        radius.append(float(np.linalg.norm(v, axis=1).max()))
+    lo = np.array([-half + 0.6, 1.0, -half + 0.6])
+    hi = np.array([half - 0.6, height - 1.0, half - 0.6])
+    pos = _place(rng, np.array(radius), lo, hi)
+    b = _Builder()
+    _container(b, half, half, height)
+    for (v, t), p in zip(meshes, pos):
+        b.mesh(v, t, tuple(p), quat=_random_quat(rng))
+    p = default_params(timestep=0.005, friction_coeff=0.6, grid_n=48, sample_spacing=0.05,
+                       tolerance=1e-4, iterations=50, pcg_precision=PCG_FP32, pcg_tol=1e-5)
+    for k, v in param_overrides.items():
+        setattr(p, k, v)
+    return b.build(name or "mesh_pile_%d" % n_bodies, p, steps=1000, n_dynamic=n_bodies, seed=seed)
+
+
+def mesh_chain(n_links=5, **param_overrides):
+    """A chain of torus links (major radius 0.5, tube 0.12) interlocked and
+    alternating by 90 degrees, 0.02 apart, the first link static: the links
+    fall 0.02 and hang, with contacts inside the links' holes."""
+    from .scene_io import torus_mesh
+    v, t = torus_mesh(0.5, 0.12, 24, 10)
+    b = _Builder()
+    s2 = math.sqrt(0.5)
+    for k in range(n_links):
+        # the torus lies in the x-z plane: turned about x into the x-y plane,
+        # alternating with turned about z into the y-z plane
+        q = (s2, 0.0, 0.0, s2) if k % 2 == 0 else (0.0, 0.0, s2, s2)
+        b.mesh(v, t, (0.0, -0.74 * k, 0.0), quat=q, static=(k == 0))
+    p = default_params(timestep=0.005, friction_coeff=0.6, grid_n=64, sample_spacing=0.05,
+                       tolerance=1e-4, iterations=50, pcg_precision=PCG_FP64, pcg_tol=1e-12)
+    for k, val in param_overrides.items():
+        setattr(p, k, val)
+    return b.build("mesh_chain_%d" % n_links, p, steps=200, n_dynamic=n_links - 1)
+
+
 def pile_2000(seed=0, **kw):
     return pile(2000, seed=seed, **kw)
 
@@ -259,6 +351,16 @@ def random_pile(seed, n_min=10, n_max=500, **kw):
                 grid_n=kw.pop("grid_n", 32), **kw)
 
 
+def body_sizes(scene):
+    """Per body its largest half extent: a box's largest half edge, a
+    sphere's radius, an ellipsoid's largest semi-axis, a mesh's largest vertex
+    distance from its centre of mass."""
+    out = np.max(np.asarray(scene.dims, np.float64), axis=1).copy()
+    for k in np.nonzero(np.asarray(scene.shape_type) == SHAPE_MESH)[0]:
+        out[k] = float(np.linalg.norm(scene.meshes[int(scene.dims[k][0])][0], axis=1).max())
+    return out
+
+
 def concat(scenes):
     """Body tables and states of independent scenes stacked into one table.
 
@@ -270,6 +372,15 @@ def concat(scenes):
     merged = Scene(name="batch_%d" % len(scenes),
                    **{f: np.concatenate([getattr(s, f) for s in scenes]) for f in fields},
                    params=scenes[0].params, steps=scenes[0].steps)
+    # the scenes' meshes in one list, mesh bodies renumbered
+    merged.dims = merged.dims.copy()
+    at = 0
+    for s in scenes:
+        m = np.asarray(s.shape_type) == SHAPE_MESH
+        rows = at + np.nonzero(m)[0]
+        merged.dims[rows, 0] += len(merged.meshes)
+        merged.meshes.extend(s.meshes)
+        at += s.n
     offsets = np.zeros(len(scenes) + 1, dtype=np.int32)
     offsets[1:] = np.cumsum([s.n for s in scenes])
     mu = np.array([s.params.friction_coeff for s in scenes], dtype=np.float64)

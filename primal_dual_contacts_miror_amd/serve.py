@@ -40,7 +40,7 @@ import time
 import numpy as np
 
 from . import pdc
-from .params import PCG_FP64, SHAPE_ELLIPSOID, SHAPE_SPHERE
+from .params import PCG_FP64, SHAPE_ELLIPSOID, SHAPE_MESH, SHAPE_SPHERE
 
 PAD_HALF = 1e-3      # half extent of a padding body
 PAD_FAR = 1.0e6      # its distance from everything else
@@ -66,8 +66,11 @@ def sample_count(shape, dims, spacing):
 
 
 def scene_samples(scene, params=None):
+    """Surface samples of a scene's bodies (a mesh body's: its vertices)."""
     p = params if params is not None else scene.params
-    return int(sum(sample_count(int(t), scene.dims[k], p.sample_spacing) for k, t in enumerate(scene.shape_type)))
+    return int(sum(len(scene.meshes[int(scene.dims[k][0])][0]) if t == SHAPE_MESH
+                   else sample_count(int(t), scene.dims[k], p.sample_spacing)
+                   for k, t in enumerate(scene.shape_type)))
 
 
 def scene_footprint(scene, params=None):
@@ -265,8 +268,8 @@ class _Shard:
             best = self.free_segment(sc)
             if best is not None:
                 self.segments[best].owner = h  # reserved
-        fields = (sc.shape_type, sc.dims, sc.mass, sc.inertia, sc.is_static, sc.emit, sc.pos, sc.quat, sc.vel,
-                  sc.angvel)
+        fields = (sc.shape_type, self.ctx.mesh_dims(sc), sc.mass, sc.inertia, sc.is_static, sc.emit, sc.pos,
+                  sc.quat, sc.vel, sc.angvel)
         if best is not None:
             seg = self.segments[best]
             arrays = [np.concatenate([a, b]) for a, b in zip(fields, _padding(seg.cap - n, p))]

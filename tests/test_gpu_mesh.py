@@ -1,0 +1,227 @@
+"""GPU: triangle-mesh bodies (src/CollisionObject/Mesh.cpp, Convex.cpp;
+src/Contact/ContactGenerator/MeshMeshContactGenerator.cpp) on the device
+against the oracle, through the C ABI.
+
+A mesh's grid holds FP32 of its pseudonormal signed distance (the same FP64
+operations on host and device, built by k_mesh_grid), its samples are its
+vertices.  Bars:
+  grids         bit-identical to the oracle's nodes
+  contacts      identical sets modulo the tie band, on a settled pile of
+                mesh bodies (convex, non-convex, tori, boxes stored with
+                repeated vertices)
+  fillForces    FP64 PCG vs the oracle's direct solve: velocity increments
+                within 1e-6 of |g| dt
+  trajectories  a chain of interlocked torus links falling 0.02 and
+                catching (contacts inside the links' holes), 150 steps:
+                positions within 1e-6 of the body size, orientations within
+                1e-6 rad; a falling pile of meshes through its first impacts,
+                each step from the oracle's state: the same contact count,
+                velocities and positions within 1e-6 (of |g| dt, of the body
+                size) or, where the reference's Newton loop stalls, 4x the
+                spread of four oracle runs from the same state
+  scene files   a YAML scene with `meshes` and `convex` baked on the device,
+                every saved frame against the oracle within 1e-6
+"""
+import numpy as np
+import pytest
+
+from oracle import orc
+from primal_dual_contacts_miror_amd import pdc, scene_io as S, scenes
+from primal_dual_contacts_miror_amd.params import PCG_FP64
+from test_gpu_parity import _advance_oracle, _fill_parity, _traj_errors, assert_same_contacts
+
+pytestmark = pytest.mark.gpu
+
+
+def _pile(**kw):
+    return scenes.mesh_pile(40, seed=2, **kw)
+
+
+def test_mesh_grids_equal_oracle(require_gpu):
+    sc = _pile()
+    ctx = pdc.Context(sc.params)
+    ctx.upload_bodies(sc.shape_type, ctx.mesh_dims(sc), sc.mass, sc.inertia, sc.is_static, sc.emit)
+    n = sc.params.grid_n
+    seen = set()
+    for k in np.nonzero(sc.shape_type == 3)[0]:
+        m = int(sc.dims[k, 0])
+        if m in seen:
+            continue
+        seen.add(m)
+        v, t = sc.meshes[m]
+        half = np.maximum(np.abs(v).max(axis=0), 1e-3 * np.abs(v).max()) * (1.0 + sc.params.grid_pad_frac)
+        sp = 2.0 * half / (n - 1)
+        i = np.arange(n)
+        Z, Y, X = np.meshgrid(i, i, i, indexing="ij")
+        pts = np.stack([-half[0] + X * sp[0], -half[1] + Y * sp[1], -half[2] + Z * sp[2]], axis=-1).reshape(-1, 3)
+        exp = orc.mesh_sdf(v, t, pts).astype(np.float32).reshape(n, n, n)
+        got = ctx.grid_nodes(int(k))
+        print("mesh %d (%d triangles): %d of %d nodes inside" % (m, len(t), int((got < 0).sum()), got.size))
+        np.testing.assert_array_equal(got, exp)
+    assert len(seen) >= 4
+    ctx.close()
+
+
+def test_mesh_pile_contacts(require_gpu):
+    sc = _pile()
+    o = _advance_oracle(sc, 300)
+    pos, quat, vel, ang = o.get_state()
+    ds = pdc.DynamicSystem(sc)
+    ds.set_state(pos, quat, vel, ang)
+    c, ties = ds.ctx.detect_contacts()
+    ids, p, depth, T, lf = ds.ctx.get_contacts(c)
+    cg = orc.Contacts(ids, p, depth, T, lf)
+    n, t2 = o.detect()
+    co = o.contacts()
+    mm = np.all(np.asarray(sc.shape_type)[co.ids] == 3, axis=1)
+    print("mesh pile: %d contacts (%d between two meshes), device %d, ties %d/%d" % (n, int(mm.sum()), c, ties, t2))
+    assert n > 50 and mm.sum() > 10
+    assert_same_contacts(cg, co, max(ties, t2))
+    ds.ctx.close()
+
+
+def test_mesh_pile_fill_forces(require_gpu):
+    sc = _pile(tolerance=1e-10)
+    p = sc.params
+    p.pcg_precision = PCG_FP64
+    p.pcg_tol = 1e-13
+    _fill_parity(sc, 260, p)
+
+
+def test_mesh_chain_trajectory(require_gpu):
+    sc = scenes.mesh_chain(5, tolerance=1e-10, pcg_tol=1e-13)
+    o = orc.OracleScene(sc)
+    o.set_linear_solver(orc.SOLVER_DIRECT)
+    ds = pdc.DynamicSystem(sc, sc.params)
+    seen = 0
+    for k in range(150):
+        so = o.step(1)
+        sg = ds.step(1)
+        assert so.n_contacts == sg.n_contacts, k
+        seen = max(seen, so.n_contacts)
+    ep, ea = _traj_errors(sc, o.get_state(), ds.state())
+    drop = sc.pos[1:, 1] - o.get_state()[0][1:, 1]
+    print("chain 150 steps: up to %d contacts, links dropped %s, position err %.3e (of body size), angle err %.3e rad"
+          % (seen, np.array2string(drop, precision=4), ep, ea))
+    assert seen >= 4 and np.all(drop > 0.01)
+    assert ep < 1e-6 and ea < 1e-6
+    ds.ctx.close()
+
+
+def test_mesh_pile_steps_follow_oracle(require_gpu):
+    """60 steps of the falling mesh pile, each from the oracle's state.  The
+    reference's Newton loop does not reach its tolerance on this pile (it
+    stalls at the iteration cap on every step, at tolerances from 1e-6 to
+    1e-10 and caps up to 500), so each step's result is defined only to the
+    spread of the algorithm under perturbation: the bar is the larger of 1e-6
+    and 4x the largest difference among four runs of the oracle from the same
+    state (direct branch; PCG branch at the inner tolerance, at ten times it
+    and at a tenth of it).  Contact counts are equal on every step, and the
+    device takes gradient-descent fallbacks on the same steps as the oracle."""
+    sc = _pile(tolerance=1e-10)
+    sc.params.pcg_precision = PCG_FP64
+    sc.params.pcg_tol = 1e-13
+    o = _advance_oracle(sc, 120)
+    o.set_linear_solver(orc.SOLVER_DIRECT)
+    variants = [o]
+    for f in (1.0, 10.0, 0.1):
+        q = type(sc.params).from_buffer_copy(sc.params)
+        q.pcg_tol = f * sc.params.pcg_tol
+        a = orc.OracleScene(sc, q)
+        a.set_linear_solver(orc.SOLVER_PCG)
+        variants.append(a)
+    ds = pdc.DynamicSystem(sc, sc.params)
+    gdt = np.linalg.norm(sc.params.gravity[:]) * sc.params.timestep
+
+    def vel_diff(u, v):
+        return max(np.max(np.abs(u[2] - v[2])), np.max(np.abs(u[3] - v[3]))) / gdt
+
+    worst, seen, loose, gd_dev, gd_steps_same = 0.0, 0, 0, 0, 0
+    for k in range(60):
+        start = o.get_state()
+        ds.set_state(*start)
+        stats = []
+        for v in variants:
+            v.set_state(*start)
+            stats.append(v.step(1))
+        sg = ds.step(1)
+        so = stats[0]
+        assert so.n_contacts == sg.n_contacts, k
+        states = [v.get_state() for v in variants]
+        spread_v = max(vel_diff(a, b) for i, a in enumerate(states) for b in states[:i])
+        spread_p = max(_traj_errors(sc, a, b)[0] for i, a in enumerate(states) for b in states[:i])
+        bar, ep_bar = max(1e-6, 4.0 * spread_v), max(1e-6, 4.0 * spread_p)
+        err = vel_diff(ds.state(), states[0])
+        ep, _ = _traj_errors(sc, states[0], ds.state())
+        loose += bar > 1e-6
+        worst = max(worst, err / bar)
+        seen = max(seen, so.n_contacts)
+        gd_dev += sg.gd_fallbacks
+        gd_steps_same += (sg.gd_fallbacks > 0) == (so.gd_fallbacks > 0)
+        assert err < bar and ep < ep_bar, (k, err, bar, ep, ep_bar, so.newton_iterations)
+    print("mesh pile: 60 steps from the oracle's states, up to %d contacts, %d steps with the oracle's own spread "
+          "above 1e-6, worst difference / bar %.3f, %d gradient-descent fallbacks on the device, fallback steps "
+          "agree on %d of 60" % (seen, loose, worst, gd_dev, gd_steps_same))
+    assert seen > 20 and gd_dev > 0 and gd_steps_same == 60
+    ds.ctx.close()
+
+
+YAML = """scene:
+  meshes: "rings.gltf"
+  convex: "rocks.gltf"
+  cubes: "floor.gltf"
+  static: [3]
+simulation:
+  timestep: 1e-2
+  solver:
+    name: "primal_dual"
+    iterations: 100
+    tolerance: 1e-10
+    non_smooth_force:
+      name: "non_smooth_contact"
+      static_coeff: 0.5
+  gravity: [0, -9.81, 0]
+baking:
+  time: 0.295
+"""
+
+
+def test_mesh_scene_file_bake_matches_oracle(require_gpu, tmp_path):
+    tor = S.torus_mesh(1.0, 0.3, 20, 8)
+    ico = S.icosphere_mesh(2)
+    cube = S.unit_cube_mesh()
+
+    def node(scale, t, rot=np.eye(3)):
+        M = np.eye(4)
+        M[:3, :3] = rot @ np.diag(scale)
+        M[:3, 3] = t
+        return M
+    c, s = np.cos(0.4), np.sin(0.4)
+    tilt = np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+    S.write_gltf(str(tmp_path / "rings.gltf"), [tor, tor], [node((0.4, 0.4, 0.4), (0, 0.3, 0), tilt),
+                                                            node((0.3, 0.3, 0.3), (0.1, 0.9, 0.05))])
+    S.write_gltf(str(tmp_path / "rocks.gltf"), [ico], [node((0.3, 0.2, 0.25), (1.2, 0.35, 0))])
+    S.write_gltf(str(tmp_path / "floor.gltf"), [cube], [node((3, 0.5, 3), (0, -0.5, 0))])
+    (tmp_path / "scene.yaml").write_text(YAML)
+    L = S.load_scene(str(tmp_path / "scene.yaml"), grid_n=48, pcg_tol=1e-13)
+    sc = L.scene
+    assert list(sc.shape_type) == [3, 3, 3, 0] and len(sc.meshes) == 3
+    times = S.bake(L, str(tmp_path / "out.gltf"))
+    assert len(times) == 30
+    z = np.load(tmp_path / "out.npz")
+    o = orc.OracleScene(sc, L.params)
+    o.set_linear_solver(orc.SOLVER_DIRECT)
+    contacts = 0
+    for _ in range(len(times)):
+        contacts += o.step(1).n_contacts
+    po, qo = o.get_state()[:2]
+    size = scenes.body_sizes(sc)
+    worst = 0.0
+    for k in range(3):
+        # the saved frame is the node transform: body pose times the mesh transform
+        M = S._rigid(po[k], qo[k]) @ L.bodies[k].mesh_trans
+        g = z["mesh_%d" % k][-1]
+        worst = max(worst, float(np.linalg.norm(g[:3] - M[:3, 3]) / size[k]))
+    print("mesh scene: %d frames, %d contacts in all, final node translation error %.3e of the body size"
+          % (len(times), contacts, worst))
+    assert contacts > 0 and worst < 1e-6

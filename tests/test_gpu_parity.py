@@ -274,7 +274,7 @@ def test_pcg_cap_matches_oracle_pcg(require_gpu, pcg_tol):
 def _traj_errors(sc, o_state, g_state):
     pos_o, q_o = o_state[0], o_state[1]
     pos_g, q_g = g_state[0], g_state[1]
-    scale = np.max(sc.dims, axis=1)
+    scale = scenes.body_sizes(sc)
     dyn = sc.is_static == 0
     ep = np.max(np.linalg.norm(pos_g - pos_o, axis=1)[dyn] / scale[dyn])
     dots = np.clip(np.abs(np.sum(q_g * q_o, axis=1)), 0, 1)
@@ -427,7 +427,7 @@ def pile_step_errors(sc, a, b):
     rotation angle (rad), velocity / (|g| dt), angular velocity * body size /
     (|g| dt)."""
     dyn = sc.is_static == 0
-    size = np.max(sc.dims, axis=1)[dyn]
+    size = scenes.body_sizes(sc)[dyn]
     gdt = float(np.linalg.norm(sc.params.gravity[:])) * sc.params.timestep
     return np.array([
         (np.linalg.norm(a[0][dyn] - b[0][dyn], axis=1) / size).max(),
@@ -507,7 +507,7 @@ def test_settled_pile_one_step(require_gpu):
 
     def errors(a, b):
         dyn = sc.is_static == 0
-        size = np.max(sc.dims, axis=1)[dyn]
+        size = scenes.body_sizes(sc)[dyn]
         gdt = float(np.linalg.norm(sc.params.gravity[:])) * sc.params.timestep
         return np.array([
             (np.linalg.norm(a[0][dyn] - b[0][dyn], axis=1) / size).max(),

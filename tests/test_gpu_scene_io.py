@@ -9,6 +9,7 @@ import pytest
 
 from oracle import orc
 from primal_dual_contacts_miror_amd import scene_io as S
+from primal_dual_contacts_miror_amd.scenes import body_sizes
 
 pytestmark = pytest.mark.gpu
 
@@ -77,7 +78,7 @@ def test_bake_matches_oracle_and_writes_outputs(require_gpu, tmp_path):
     dyn = sc.is_static == 0
     for k in np.nonzero(dyn)[0]:
         g = z["mesh_%d" % k][-1]
-        size = float(np.max(sc.dims[k]))
+        size = float(body_sizes(sc)[k])
         assert np.linalg.norm(g[:3] - po[k]) < 1e-6 * max(size, 1.0), k
         qg = g[3:]
         ang = 2 * np.arccos(min(1.0, abs(float(np.dot(qg, qo[k])))))
@@ -130,7 +131,7 @@ def test_bake_adaptive_substeps(require_gpu, tmp_path):
         po, qo = o.get_state()[:2]
         for k in dyn:
             g = z["mesh_%d" % k][frame + 1]
-            size = float(np.max(sc.dims[k]))
+            size = float(body_sizes(sc)[k])
             assert np.linalg.norm(g[:3] - po[k]) < 1e-6 * max(size, 1.0), (frame, k)
             ang_err = 2 * np.arccos(min(1.0, abs(float(np.dot(g[3:], qo[k])))))
             assert ang_err < 1e-6, (frame, k)

@@ -342,6 +342,35 @@ def test_load_scene_with_ellipsoids(tmp_path):
     assert sc.mass[4] == pytest.approx(4 / 3 * math.pi * a * b * c, rel=1e-12)
 
 
+def test_load_scene_with_meshes(tmp_path):
+    """`meshes` and `convex` keep the triangle mesh (Mesh.cpp:18-110,
+    Convex.cpp:14-39) and load first (FileLoader.cpp:46-56): the body's volume,
+    inertia and centre are the mesh's, its body-frame vertices the node's
+    vertices moved into the principal frame, one device mesh per body."""
+    y = YAML.replace('  cubes: "cubes.glb"\n', '  cubes: "cubes.glb"\n  meshes: "balls.glb"\n  convex: "eggs.glb"\n')
+    y = y.replace('  spheres: "balls.glb"\n', '').replace("{ id: 3, velocity", "{ id: 4, velocity")
+    path = _write_scene(tmp_path, y, cubes=[((5, 0.5, 5), (0, -0.5, 0)), ((0.5, 0.5, 0.5), (0, 0.5, 0))],
+                        spheres=[((0.4, 0.4, 0.4), (0, 2, 0)), ((0.3, 0.5, 0.2), (1, 2, 0))],
+                        ellipsoids=[((0.6, 0.3, 0.4), (-2, 1, 0))])
+    L = S.load_scene(path)
+    sc = L.scene
+    assert list(sc.shape_type) == [P.SHAPE_MESH] * 3 + [P.SHAPE_BOX] * 2
+    assert list(sc.dims[:3, 0]) == [0.0, 1.0, 2.0] and len(sc.meshes) == 3
+    v, tri = S.icosphere_mesh(3)
+    for k, (scale, tr) in enumerate([((0.4, 0.4, 0.4), (0, 2, 0)), ((0.3, 0.5, 0.2), (1, 2, 0)),
+                                     ((0.6, 0.3, 0.4), (-2, 1, 0))]):
+        world = v.astype(np.float32).astype(np.float64) * np.array(scale) + np.array(tr)
+        vol, com = S.mesh_volume_com(world, tri)
+        rho = 3.0 if k == 2 else 1.0          # initialize: id 2, density 3
+        assert sc.mass[k] == pytest.approx(rho * vol, rel=1e-12)
+        np.testing.assert_allclose(sc.pos[k], com, atol=1e-12)
+        R = S.quat_to_matrix(sc.quat[k])
+        np.testing.assert_allclose(sc.meshes[k][0] @ R.T + sc.pos[k], world, atol=1e-12)
+        I = S.moment_of_inertia(world - com, tri)
+        np.testing.assert_allclose(R @ np.diag(sc.inertia[k]) @ R.T, rho * I, atol=1e-12 * rho * np.abs(I).max())
+        np.testing.assert_array_equal(sc.meshes[k][1], tri)
+
+
 def test_load_scene_defaults_and_errors(tmp_path):
     y = YAML.replace("    iterations: 30\n    tolerance: 1e-4\n", "").replace("      restitution: 0.2\n", "")
     y = y.replace('  spheres: "balls.glb"\n', "")
@@ -358,8 +387,7 @@ def test_load_scene_defaults_and_errors(tmp_path):
     p = S.load_scene(path).params
     assert p.force_model == P.ONLY_NORMAL and p.restitution == 0.5 and p.friction_coeff == 0.0
     for bad, exc in ((y.replace("id: 3,", "id: 9,"), ValueError),
-                     (y.replace('name: "primal_dual"', 'name: "other"'), ValueError),
-                     (y.replace('  cubes: "cubes.glb"', '  meshes: "cubes.glb"'), NotImplementedError)):
+                     (y.replace('name: "primal_dual"', 'name: "other"'), ValueError)):
         (tmp_path / "scene.yaml").write_text(bad)
         with pytest.raises(exc):
             S.load_scene(path)
@@ -443,5 +471,10 @@ def test_reference_scene_files():
     L = S.load_scene(os.path.join(REF_SCENES, "ball_stack", "ball_stack.yaml"))
     assert L.scene.n == 23 and L.scene.shape_type[0] == P.SHAPE_BOX and (L.scene.shape_type[1:] == P.SHAPE_SPHERE).all()
     assert L.params.restitution == 0.0 and L.params.friction_coeff == 0.6
-    with pytest.raises(NotImplementedError):
-        S.load_scene(os.path.join(REF_SCENES, "chain", "chain.yaml"))
+    # a `meshes` scene: 128 chain links, each its own triangle mesh
+    L = S.load_scene(os.path.join(REF_SCENES, "chain", "chain.yaml"))
+    sc = L.scene
+    assert sc.n == 128 and (sc.shape_type == P.SHAPE_MESH).all() and list(np.nonzero(sc.is_static)[0]) == [0]
+    assert len(sc.meshes) == 128 and all(m[0].shape == (637, 3) and m[1].shape == (1152, 3) for m in sc.meshes)
+    np.testing.assert_allclose(sc.mass, sc.mass[0], rtol=1e-6)
+    assert L.adaptive_timestep and L.substep_factor == 5.0
