@@ -40,7 +40,7 @@ METRIC = "Newton timesteps/sec at 30k contacts; batched scene-steps/sec + p50/p9
 PEAK_HBM = 8.0e12
 # kernel classes timed with HIP events (include/pdc.h PDC_KT_*)
 KT_NAMES = {"pcg": pdc.KT_PCG, "pcg_per_launch": pdc.KT_SPMV, "contact_terms": pdc.KT_CONTACT_TERMS,
-            "contacts": pdc.KT_CONTACTS}
+            "contacts": pdc.KT_CONTACTS, "assembly": pdc.KT_ASSEMBLY, "line_search": pdc.KT_LINE_SEARCH}
 # inner-solve settings: operator precision and relative tolerance
 SETTINGS = {"tight": (PCG_FP64, 1e-12), "fast": (PCG_FP32, 1e-5)}
 STANDARD_FORM_TOL = 1e-9  # below it the library runs the standard PCG form with the body-group preconditioner
@@ -74,6 +74,68 @@ def survey_pcg_bytes(p, nnzb, nbodies):
     FP64): ~ nnzb 148 + N 552 at e = 4, nnzb 292 + N 1104 at e = 8."""
     e = 8.0 if p.pcg_precision == PCG_FP64 else 4.0
     return nnzb * (36 * e + 4) + (nbodies + 1) * 4 + nbodies * (12 * e + 48 * e + 6 * e * 13)
+
+
+def subsystem_bytes(which, p, st, det):
+    """Algorithmic bytes of one timestep's work in kernel class `which`, by
+    the hot-path table's per-unit figures (e = the operator's element size):
+      (a) contacts       per pair 368 (rotations, positions, grid descriptors
+                         of both bodies) + per tested sample 12 (FP32 body-frame
+                         position) + per sample inside the partner's grid 32
+                         (its 8 FP32 nodes) + per raw hit 56 (the library's own
+                         count: the table's brick term, the nodes under the
+                         partner's box, exceeds the measured traffic 40-fold at
+                         96^3 grids, where the samples touch a thin shell)
+      (b) contact terms  per Newton iteration C 504 + N 176
+      (c) assembly       per Newton iteration C 352 + nnzb 36 e + N 192
+      (d) PCG            per iteration nnzb (36 e + 4) + (N + 1) 4 + N (12 e + 48 e + 78 e)
+      (e) line search    per trial C 440 + N 224"""
+    e = 8.0 if p.pcg_precision == PCG_FP64 else 4.0
+    C, N, nnzb = st.n_contacts, st.n_local_bodies, st.n_blocks
+    if which == "contacts":
+        return (368.0 * det.pairs + 12.0 * det.samples_read + 32.0 * det.samples_in_grid + 56.0 * det.raw_hits)
+    if which == "contact_terms":
+        return st.newton_iterations * (504.0 * C + 176.0 * N)
+    if which == "assembly":
+        return st.newton_iterations * (352.0 * C + 36.0 * e * nnzb + 192.0 * N)
+    if which == "pcg":
+        return st.pcg_iterations * survey_pcg_bytes(p, nnzb, N)
+    if which == "line_search":
+        return st.line_search_trials * (440.0 * C + 224.0 * N)
+    raise ValueError(which)
+
+
+SUBSYSTEMS = [("(a) SDF contacts", "contacts", "k_pair_cull + k_pair_hits", "one step"),
+              ("(b) per-contact terms", "contact_terms", "k_constraints + k_contact_res", "one Newton iteration"),
+              ("(c) assembly", "assembly", "k_asm_prep + k_asm_diag + 2 k_asm_off + k_grp_inv", "one Newton iteration"),
+              ("(d) PCG", "pcg", "k_pcgs_iter / k_pcg_iter", "one PCG iteration"),
+              ("(e) line search", "line_search", "k_ls_eval (all passes)", "one trial")]
+
+
+def subsystems_run(sc, params, state, steps, warmup):
+    """Per subsystem (a)-(e): `steps` timesteps from `state` with that kernel
+    class timed by HIP events on the context's stream; achieved = the
+    algorithmic bytes of the timed work / the event time."""
+    out = {}
+    for label, which, kernels, unit in SUBSYSTEMS:
+        ds = pdc.DynamicSystem(sc, params)
+        ds.set_state(*state)
+        for _ in range(warmup):
+            ds.step(1)
+        ds.ctx.enable_kernel_timing(KT_NAMES[which])
+        ds.ctx.reset_timings()
+        total = 0.0
+        for _ in range(steps):
+            st = ds.step(1)
+            total += subsystem_bytes(which, params, st, ds.ctx.detection_stats())
+        ms, launches, _ = ds.ctx.kernel_timing()
+        ds.ctx.close()
+        rate = total / (ms * 1e-3) if ms > 0 else 0.0
+        out[which] = {"subsystem": label, "kernels": kernels, "unit": unit, "ms_per_step": ms / steps,
+                      "launches_per_step": launches / steps, "algorithmic_bytes_per_step": total / steps,
+                      "achieved_GBps": rate / 1e9, "frac": rate / PEAK_HBM}
+        log("%s: %.3f ms/step, %.1f MB/step, %.0f GB/s" % (label, ms / steps, total / steps / 1e6, rate / 1e9))
+    return out
 
 
 def log(*a):
@@ -260,6 +322,19 @@ def batched_leg(n_scenes, steps, plain_scenes, plain_steps, overrides, seed_base
     for _ in range(tail):
         bs.step(1)
     el_tail = time.perf_counter() - t0
+    # the PCG kernel's roofline in the batched launches: the same tail again
+    # from the same state, every PCG launch timed by events on the context's
+    # stream; algorithmic bytes by the hot-path table's per-unit figure of
+    # each scene's system times its iterations
+    bs.ctx.set_state(*state)
+    bs.ctx.enable_kernel_timing(pdc.KT_PCG)
+    bs.ctx.reset_timings()
+    survey = 0.0
+    for _ in range(tail):
+        for x in bs.step(1):
+            survey += x.pcg_iterations * survey_pcg_bytes(scs[0].params, x.n_blocks, x.n_local_bodies)
+    pcg_ms, pcg_launches, pcg_own = bs.ctx.kernel_timing()
+    bs.ctx.enable_kernel_timing(pdc.KT_NONE)
     t0 = time.perf_counter()
     for k in range(plain_scenes):
         for _ in range(plain_steps):
@@ -299,6 +374,18 @@ def batched_leg(n_scenes, steps, plain_scenes, plain_steps, overrides, seed_base
                 tail, plain_steps, plain_scenes),
         },
         "precision_scenes": ["%s (%d bodies)" % (scs[k].name, scs[k].n) for k in picks],
+        "pcg_roofline": {
+            "kernel": pcg_kernel_name(scs[0].params) + " in the batched launches (every scene still solving "
+                      "in one launch), %d steps of the contact-rich tail" % tail,
+            "launches": pcg_launches,
+            "avg_launch_us": 1e3 * pcg_ms / max(pcg_launches, 1),
+            "algorithmic_bytes_per_launch": survey / max(pcg_launches, 1),
+            "achieved": survey / (pcg_ms * 1e-3) / 1e9 if pcg_ms > 0 else 0.0,
+            "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s",
+            "frac": survey / (pcg_ms * 1e-3) / PEAK_HBM if pcg_ms > 0 else 0.0,
+            "kernel_own_bytes_per_launch": pcg_own / max(pcg_launches, 1),
+        },
     }
     return bs, out, checks
 
@@ -455,6 +542,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other inner-solve setting")
+    ap.add_argument("--no-subsystems", action="store_true", help="skip the per-subsystem (a)-(e) rates")
     ap.add_argument("--state", default=None, help="npz with pos, quat, vel, angvel to start from")
     ap.add_argument("--settle-steps", type=int, default=None,
                     help="settle on the device for this many steps instead of loading the settled state")
@@ -529,6 +617,8 @@ def main():
         out["roofline"]["measured_copy_GBps"] = measured_copy_rate() / 1e9
     except Exception as exc:  # the copy is a reference figure only
         log("copy rate not measured: %s" % exc)
+    if not args.no_subsystems:
+        out["subsystems"] = subsystems_run(sc, sc.params, state, max(5, args.steps // 4), args.warmup)
     checks = [("headline", sc, sc.params, state, first_gpu, None)]
     if not args.no_extra:
         other = "fast" if args.setting == "tight" else "tight"

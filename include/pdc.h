@@ -351,16 +351,17 @@ void pdc_reset_timings(pdc_context* ctx);
  * pdc_kernel_timing returns the summed milliseconds, the launch count and the
  * summed algorithmic bytes (compulsory traffic: each input read once, each
  * output written once) of those launches. */
-/* Classes: SPMV = the PCG iteration kernel (k_pcg_iter: block-CSR SpMV with
- * the preconditioned operator, block-Jacobi apply, vector updates, partial
- * dot products), timed per launch with the iterations launched directly (no
+/* Classes: SPMV = the PCG iteration kernel (k_pcg_iter or k_pcgs_iter:
+ * ELL block SpMV, preconditioner apply, vector updates, partial dot
+ * products), timed per launch with the iterations launched directly (no
  * graph replay) while enabled, active iterations only; PCG = the same kernel
  * timed by one event pair around each solve's launches, every launch
- * counted;
- * CONTACT_TERMS = constraint + contact-residual kernels of a Newton
- * iteration; ASSEMBLY = the assembly kernels; LINE_SEARCH = the evaluation
- * kernel of a line-search pass in the Newton direction (its trials' contact
- * and body parts); CONTACTS = the sample-test kernel. */
+ * counted; CONTACT_TERMS = constraint + contact-residual kernels of a
+ * Newton iteration; ASSEMBLY = the assembly kernels and the group inverses;
+ * LINE_SEARCH = the evaluation kernel of every line-search pass (its trials'
+ * contact and body parts, the gradient-descent pass included); CONTACTS =
+ * the cell cull and the sample-test kernel.  Algorithmic bytes are counted
+ * by the library for SPMV, PCG, CONTACT_TERMS and CONTACTS. */
 enum { PDC_KT_NONE = 0, PDC_KT_SPMV = 1, PDC_KT_CONTACT_TERMS = 2,
        PDC_KT_ASSEMBLY = 3, PDC_KT_LINE_SEARCH = 4, PDC_KT_CONTACTS = 5,
        PDC_KT_PCG = 6 };

@@ -3526,8 +3526,10 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 			kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
 			launch(k_ls_final, S, 1024, a, k0, nt, 10, 0);
 		}
+		kt_begin(ctx, PDC_KT_LINE_SEARCH, &e1);
 		launch_trials(k_ls_eval, ntc + ntw, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p,
 		              0, (int)LS_GD_PENDING, ntc);
+		kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
 		launch(k_ls_final, S, 1024, a, 0, kTrials, 20, 1);
 		launch(k_commit, ntc + ntb, kBlock, a, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
 		       (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, gdu, ntc);

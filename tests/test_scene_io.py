@@ -478,3 +478,8 @@ def test_reference_scene_files():
     assert len(sc.meshes) == 128 and all(m[0].shape == (637, 3) and m[1].shape == (1152, 3) for m in sc.meshes)
     np.testing.assert_allclose(sc.mass, sc.mass[0], rtol=1e-6)
     assert L.adaptive_timestep and L.substep_factor == 5.0
+    # meshes and cubes together: the hourglass (static), 1701 mesh grains, a cube floor
+    L = S.load_scene(os.path.join(REF_SCENES, "hourglass", "hourglass_1k7grains.yaml"))
+    sc = L.scene
+    assert sc.n == 1703 and int(np.sum(sc.shape_type == P.SHAPE_MESH)) == 1702 and sc.shape_type[-1] == P.SHAPE_BOX
+    assert list(np.nonzero(sc.is_static)[0]) == [0, 1702]
