@@ -402,7 +402,7 @@ def served_leg(n_scenes, settle_steps, rates, duration, plain_rate, overrides, s
     t0 = time.perf_counter()
     scs = [scenes.random_pile(seed_base + k, n_min=10, n_max=1000, **overrides) for k in range(n_scenes)]
     bodies_all = sum(sc.n + 1 for sc in scs)
-    samples_all = sum(serve.scene_samples(sc) + 6 for sc in scs)
+    samples_all = sum(serve.scene_samples(sc) for sc in scs)
     srv = serve.SceneServer(scs[0].params, hbm_budget=budget, shards=shards, max_resident=4096,
                             reserve_bodies=int(1.05 * bodies_all), reserve_samples=int(1.05 * samples_all))
     srv.admit_all(scs)
@@ -552,8 +552,9 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served leg")
     ap.add_argument("--served-rates", default="100,200,500,1000,2000,5000")
     ap.add_argument("--served-duration", type=float, default=2.0)
-    ap.add_argument("--served-scenes", type=int, default=1200)
+    ap.add_argument("--served-scenes", type=int, default=4096)
     ap.add_argument("--served-settle", type=int, default=100)
+    ap.add_argument("--served-shards", type=int, default=2, help="contexts (each with its stream and worker)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -649,7 +650,7 @@ def main():
     if not args.no_served:
         rates = [float(r) for r in args.served_rates.split(",") if r]
         out["served"], schecks = served_leg(args.served_scenes, args.served_settle, rates,
-                                            args.served_duration, 40.0, overrides)
+                                            args.served_duration, 40.0, overrides, shards=args.served_shards)
         checks += schecks
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

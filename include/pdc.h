@@ -142,11 +142,11 @@ const char* pdc_last_error(const pdc_context* ctx);
 
 /* ---- bodies (RigidObjectBase inputs: RigidObjectBase.cpp:45-119) ---------- */
 /* Uploads the body table: shape type and dimensions (box half extents, or
- * sphere radius in dims[3i], or ellipsoid semi-axes), mass, principal
- * moments of inertia in the body
- * frame, static flags, and whether the body's surface samples are tested
- * against other bodies.  Builds the FP32 SDF grid and the surface samples of
- * every body on the device. */
+ * sphere radius in dims[3i], or ellipsoid semi-axes, or a mesh id), mass,
+ * principal moments of inertia in the body frame, static flags, and whether
+ * the body's surface samples are tested against other bodies (NULL: all
+ * are).  Builds the FP32 SDF grid of every body and the surface samples of
+ * every body that emits them on the device. */
 int pdc_upload_bodies(pdc_context* ctx, int32_t n_bodies,
                       const int32_t* shape_type, const double* dims,
                       const double* mass, const double* inertia_diag,

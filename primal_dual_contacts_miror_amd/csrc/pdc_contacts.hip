@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <array>
 #include <climits>
+#include <cstdint>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -385,12 +386,16 @@ static void sort_into_cells(const double* smp, int S, std::vector<float>& out, s
 
 // FP32 body-frame surface samples of n bodies, ordered by cell; counts[b]
 // samples and ncell[b] cells per body, the cells of all n bodies in `cells`.
-static void samples_of(const pdc_context* ctx, int n, const int* shape, const double* dims, std::vector<float>& smp,
-                       std::vector<int>& counts, std::vector<SampleCell>& cells, std::vector<int>& ncell) {
+// A body that emits no samples (emit[b] == 0) stores none: its samples would
+// never be tested.
+static void samples_of(const pdc_context* ctx, int n, const int* shape, const double* dims, const int* emit,
+                       std::vector<float>& smp, std::vector<int>& counts, std::vector<SampleCell>& cells,
+                       std::vector<int>& ncell) {
 	counts.assign(n, 0);
 	ncell.assign(n, 0);
 	std::vector<double> one;
 	for (int b = 0; b < n; ++b) {
+		if (emit && !emit[b]) continue;
 		one.clear();
 		make_samples(ctx, shape[b], &dims[3 * b], ctx->p.sample_spacing, one);
 		const int S = (int)(one.size() / 3);
@@ -402,18 +407,31 @@ static void samples_of(const pdc_context* ctx, int n, const int* shape, const do
 }
 
 // Cells of bodies [first, first + n) to their places in the cell table (the
-// sample offsets of those bodies already set in h_soff).
+// sample offsets of those bodies already set in h_soff), in one copy: body
+// b's cells start at floor(soff[b] / kCellStride) and end before the next
+// body's (its kc^3 <= S / 8 cells), so the span from the first body's start
+// to the last body's end holds only these bodies' cells and unused slots.
 static void upload_cells(pdc_context* ctx, int first, int n, const std::vector<SampleCell>& cells,
                          const std::vector<int>& ncell) {
 	hipStream_t s = ctx->stream;
-	size_t k = 0;
-	for (int b = 0; b < n; ++b) {
+	size_t lo = SIZE_MAX, hi = 0, k = 0;
+	for (int b = 0; b < n; ++b)
 		if (ncell[b] > 0) {
 			const size_t at = (size_t)ctx->h_soff[first + b] / kCellStride;
-			PDC_HIP(hipMemcpyAsync(ctx->cells.p + at, cells.data() + k, ncell[b] * sizeof(SampleCell),
-			                       hipMemcpyHostToDevice, s));
+			lo = std::min(lo, at);
+			hi = std::max(hi, at + (size_t)ncell[b]);
 		}
-		k += (size_t)ncell[b];
+	if (hi > lo) {
+		std::vector<SampleCell> span(hi - lo, SampleCell{});
+		for (int b = 0; b < n; ++b) {
+			if (ncell[b] > 0) {
+				const size_t at = (size_t)ctx->h_soff[first + b] / kCellStride;
+				std::copy(cells.begin() + k, cells.begin() + k + ncell[b], span.begin() + (at - lo));
+			}
+			k += (size_t)ncell[b];
+		}
+		PDC_HIP(hipMemcpyAsync(ctx->cells.p + lo, span.data(), span.size() * sizeof(SampleCell),
+		                       hipMemcpyHostToDevice, s));
 	}
 	if (n > 0)
 		PDC_HIP(hipMemcpyAsync(ctx->b_ncell.p + first, ncell.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice,
@@ -443,7 +461,7 @@ void upload_bodies(pdc_context* ctx, int n, const int* shape, const double* dims
 	std::vector<float> smp;
 	std::vector<int> cnt, ncell;
 	std::vector<SampleCell> cells;
-	samples_of(ctx, n, shape, dims, smp, cnt, cells, ncell);
+	samples_of(ctx, n, shape, dims, emit, smp, cnt, cells, ncell);
 	std::vector<int64_t> total(N + 1, 0);
 	for (size_t b = 0; b < N; ++b) total[b + 1] = total[b] + cnt[b];
 	if (total[N] > INT32_MAX) throw HipError("sample table beyond 2^31 samples", PDC_ERR_CAPACITY);
@@ -505,7 +523,7 @@ int append_bodies(pdc_context* ctx, int n, const int* shape, const double* dims,
 	std::vector<float> smp;
 	std::vector<int> cnt, ncell;
 	std::vector<SampleCell> cells;
-	samples_of(ctx, n, shape, dims, smp, cnt, cells, ncell);
+	samples_of(ctx, n, shape, dims, emit, smp, cnt, cells, ncell);
 	const int64_t total0 = ctx->h_soff.empty() ? 0 : ctx->h_soff[N0];
 	const int64_t total1 = total0 + (int64_t)(smp.size() / 3);
 	if (total1 > INT32_MAX) throw HipError("sample table beyond 2^31 samples", PDC_ERR_CAPACITY);
@@ -540,7 +558,7 @@ void replace_bodies(pdc_context* ctx, int first, int n, const int* shape, const 
 	std::vector<float> smp;
 	std::vector<int> cnt, ncell;
 	std::vector<SampleCell> cells;
-	samples_of(ctx, n, shape, dims, smp, cnt, cells, ncell);
+	samples_of(ctx, n, shape, dims, emit, smp, cnt, cells, ncell);
 	const int64_t lo = ctx->h_soff[first], hi = ctx->h_soff[first + n];
 	const int64_t m = (int64_t)(smp.size() / 3);
 	const bool last_emits = emit ? emit[n - 1] != 0 : true;
@@ -598,6 +616,7 @@ void set_scenes(pdc_context* ctx, int S, const int* sbeg) {
 	ctx->h_mu.assign(S, ctx->p.friction_coeff);
 	ctx->d_ss.ensure(S);
 	ctx->d_sctl.ensure(S);
+	ctx->d_act.ensure(S);
 	ctx->h_sctl.assign(S, SceneCtl{});
 	ctx->d_ps.ensure(S);
 	PDC_HIP(hipMemsetAsync(ctx->d_ps.p, 0, S * sizeof(PcgScene), s));

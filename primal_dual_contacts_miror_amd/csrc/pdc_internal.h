@@ -398,12 +398,15 @@ struct pdc_context {
 	pdc::DBuf<int> b_scene;                                   // per body: scene
 	pdc::DBuf<int> d_sbeg;                                    // per scene: first body (S + 1)
 	pdc::DBuf<int> d_step;                                    // per scene: stepped (broadphase, integration)
+	pdc::DBuf<int> d_act;                                     // the scenes of the current solve (ascending)
+	std::vector<int> h_act;                                   // its host copy (source of the upload)
 	pdc::DBuf<int> r_cbeg, r_lbeg, r_pbeg;                    // per scene: first contact, local body, pair
 	pdc::DBuf<int> r_nnzb;                                    // per scene: Newton-matrix blocks (+ total)
 	std::vector<int> h_cbeg, h_lbeg, h_pbeg, h_nnzb;
 	pdc::DBuf<pdc::Tile> t_c, t_b, t_w, t_p;                  // tiles: contacts, bodies, body waves, pair waves
 	pdc::DBuf<int> tb_c, tb_b, tb_w, tb_p;                    // per scene: first tile (S + 1)
 	int n_tc = 0, n_tb = 0, n_tw = 0, n_tp = 0, n_tr = 0;
+	int n_tr_act = 0;  // PCG tiles of the current solve's rows (the first n_tr_act of the table)
 	int cap_tr = 0;                                           // PCG tiles launched (fixed per scene set)
 	pdc_shapes shapes{{256, 256, 256, 256}, {4, 4, 4, 4}, 0};   // work shapes per size class
 	// PCG launches per group the host queues (and per recorded chunk where

@@ -626,6 +626,7 @@ static void scene_ranges(pdc_context* ctx) {
 		pt[k] = r;
 	}
 	ctx->n_tr = (int)t.size();
+	ctx->n_tr_act = tb[S];
 	ctx->t_pcg.ensure(std::max<size_t>(pt.size(), 1));
 	PDC_HIP(hipMemcpyAsync(ctx->t_pcg.p, pt.data(), pt.size() * sizeof(PcgTile), hipMemcpyHostToDevice, ctx->stream));
 }
@@ -718,10 +719,12 @@ void build_topology(pdc_context* ctx) {
 	                   ctx->adj_pos.p, ctx->inc_ptr.p, ctx->c_pid.p, ctx->inc.p);
 	PDC_CHECK_LAUNCH();
 	scene_ranges(ctx);
-	// ELL pattern over every slot of the PCG tiles (empty entries -1)
+	// ELL pattern over every slot of the PCG tiles (empty entries -1); the
+	// storage holds the tile capacity, the solve uses its own tiles' slots
 	const size_t ell = (size_t)kPcgRowsPerWG * ctx->cap_tr * ctx->ell_R;
 	ctx->e_col.ensure(ell);
-	PDC_HIP(hipMemsetAsync(ctx->e_col.p, 0xff, ell * sizeof(int), s));
+	const size_t ell_act = (size_t)kPcgRowsPerWG * std::max(1, ctx->n_tr_act) * ctx->ell_R;
+	PDC_HIP(hipMemsetAsync(ctx->e_col.p, 0xff, ell_act * sizeof(int), s));
 	hipLaunchKernelGGL(k_a_fill, dim3(grid_for(NL)), dim3(kBlock), 0, s, NL, ctx->l_static.p,
 	                   ctx->adj_ptr.p, ctx->adj_nb.p, ctx->l_slot.p, ctx->ell_R, ctx->e_col.p, ctx->a_diag.p,
 	                   ctx->adj_blk.p);
@@ -733,6 +736,7 @@ void build_topology(pdc_context* ctx) {
 // ===========================================================================
 struct SArgs {
 	int C, cs, NL, nsc, S;
+	const int* act;       // the scenes in this solve (ascending): workgroup k of a per-scene kernel takes act[k]
 	int pstride;          // partial-sum slot stride (>= every tile count)
 	double restitution, compliance, reduction, dt, gdt[3];
 	int spring;
@@ -948,7 +952,7 @@ __global__ __launch_bounds__(256) void k_nondim_speed(SArgs a) {
 __global__ __launch_bounds__(1024) void k_nondim_final(SArgs a, const double* __restrict__ mass, double gnorm_dt) {
 	__shared__ double slot;
 	__shared__ double lds[2 * 16];
-	const int sc = blockIdx.x;
+	const int sc = a.act[blockIdx.x];
 	if (!a.ctl[sc].act) return;
 	const double tr = tile_sum(a.part, a.pstride, P_ND_TRACE, a.tbb[sc], a.tbb[sc + 1], &slot);
 	const double cnt = tile_sum(a.part, a.pstride, P_ND_CNT, a.tbb[sc], a.tbb[sc + 1], &slot);
@@ -1204,7 +1208,7 @@ __device__ __forceinline__ double ref_min(double a, double b) { return (b < a) ?
 // err, eps and the exit test (PrimalDualForceSolver.cpp:295-320); one workgroup per scene.
 __global__ __launch_bounds__(256) void k_err_final(SArgs a, double tol) {
 	__shared__ double sums[4];
-	const int sc = blockIdx.x;
+	const int sc = a.act[blockIdx.x];
 	if (!a.ctl[sc].act || !a.ss[sc].run) return;
 	const int c0 = a.tbc[sc], c1 = a.tbc[sc + 1], w0 = a.tbw[sc], w1 = a.tbw[sc + 1];
 	// |wrf|^2, |rl|^2, |ac|^2 over the contact tiles and |ru|^2 over the body
@@ -2711,7 +2715,7 @@ __global__ __launch_bounds__(256) void k_ls_eval(SArgs a, const double* __restri
 // gradient-descent search ends in LS_GD_DECIDED with ls_step 0 on failure.
 __global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int k0, int nt, int maxLS, int gd) {
 	__shared__ double red[7 * kTrials];
-	const int sc = blockIdx.x;
+	const int sc = a.act[blockIdx.x];
 	if (!a.ctl[sc].act || !a.ss[sc].run || a.ss[sc].ls_state != (gd ? LS_GD_PENDING : LS_PENDING)) return;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	for (int q = wid; q < 7 * nt; q += 16) {
@@ -2926,6 +2930,7 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	SArgs a;
 	const pdc_params& p = ctx->p;
 	a.C = ctx->C; a.cs = ctx->c_stride; a.NL = ctx->NL; a.S = ctx->S;
+	a.act = ctx->d_act.p;
 	a.pstride = std::max(1, std::max(ctx->n_tc, std::max(ctx->n_tb, ctx->n_tw)));
 	a.nsc = p.force_model == PDC_NON_SMOOTH_CONTACT;
 	a.restitution = p.restitution;
@@ -2986,6 +2991,11 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 
 // Arguments of the PCG kernels for the current scene set (grid fixed per
 // scene set: the capacity of PCG tiles, so that a recorded chunk stays valid).
+// Workgroups of a PCG launch: the PCG tiles of the scenes in this solve (the
+// scenes of one step call), not the tile capacity of the whole scene set, so
+// that stepping a few of many resident scenes launches only their tiles.
+static int pcg_grid(const pdc_context* ctx) { return std::max(1, ctx->n_tr_act); }
+
 template <class T>
 static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b, DBuf<T>& G) {
 	const int S = ctx->S;
@@ -3034,7 +3044,7 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 // and the first partial sums.
 template <class T>
 static void pcg_begin(pdc_context* ctx, const PcgArgs<T>& pa) {
-	const int nblk = std::max(1, ctx->cap_tr);
+	const int nblk = pcg_grid(ctx);
 	hipStream_t s = ctx->stream;
 	if (ctx->p.pcg_tol < kSingleReductionMinTol) {
 		hipLaunchKernelGGL(k_pcgs_start<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa, (const double*)ctx->y64.p);
@@ -3076,7 +3086,7 @@ static void pcg_account_bytes(pdc_context* ctx, const std::vector<int>& iters, b
 // when a buffer moved, the grid changed or the chunk length changed).
 template <class T>
 static PcgGraph& pcg_graph(pdc_context* ctx, const PcgArgs<T>& pa) {
-	const int nblk = std::max(1, ctx->cap_tr);
+	const int nblk = pcg_grid(ctx);
 	hipStream_t s = ctx->stream;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
 	const int chunk = ctx->pcg_chunk();
@@ -3105,7 +3115,7 @@ static PcgGraph& pcg_graph(pdc_context* ctx, const PcgArgs<T>& pa) {
 template <class T>
 static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 	const int S = ctx->S;
-	const int nblk = std::max(1, ctx->cap_tr);
+	const int nblk = pcg_grid(ctx);
 	hipStream_t s = ctx->stream;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
 	const int chunk = ctx->pcg_chunk();
@@ -3209,7 +3219,7 @@ static bool pcg_solve_async(pdc_context* ctx, const PcgArgs<T>& pa, int iter, st
 	const int S = ctx->S;
 	hipStream_t s = ctx->stream;
 	const int chunk = ctx->pcg_chunk();
-	const int nblk = std::max(1, ctx->cap_tr);
+	const int nblk = pcg_grid(ctx);
 	const int maxit = ctx->p.pcg_max_iter;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
 	const bool kt_iter = ctx->kt.which == PDC_KT_PCG;
@@ -3422,11 +3432,18 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	std::vector<char> in(S);
 	for (int sc = 0; sc < S; ++sc) in[sc] = ctx->h_cbeg[sc + 1] > ctx->h_cbeg[sc];
 	ctx->h_sctl.resize(S);
+	std::vector<int>& act = ctx->h_act;
+	act.clear();
 	for (int sc = 0; sc < S; ++sc) {
 		ctx->h_sctl[sc].act = in[sc] ? 1 : 0;
 		ctx->h_sctl[sc].mu = ctx->h_mu[sc];
+		if (in[sc]) act.push_back(sc);
 	}
 	PDC_HIP(hipMemcpyAsync(ctx->d_sctl.p, ctx->h_sctl.data(), S * sizeof(SceneCtl), hipMemcpyHostToDevice, s));
+	// the per-scene kernels run one workgroup per scene of this solve
+	const int nact = (int)act.size();
+	if (nact > 0)
+		PDC_HIP(hipMemcpyAsync(ctx->d_act.p, act.data(), nact * sizeof(int), hipMemcpyHostToDevice, s));
 	auto launch = [&](auto kernel, int grid, int threads, auto... args) {
 		if (grid > 0) hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, s, args...);
 	};
@@ -3440,7 +3457,7 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	launch(k_setup_bodies, ntb, kBlock, a, ctx->b_vel.p, ctx->b_ang.p, ctx->b_quat.p, ctx->b_mass.p, ctx->b_I0.p);
 	launch(k_nondim_speed, ntw, kBlock, a);
 	const double gn = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-	launch(k_nondim_final, S, 1024, a, (const double*)ctx->b_mass.p, gn * dt);
+	launch(k_nondim_final, nact, 1024, a, (const double*)ctx->b_mass.p, gn * dt);
 	launch(k_scale_bodies, ntb, kBlock, a);
 	launch(k_preprocess, ntc, kBlock, a);
 	PDC_CHECK_LAUNCH();
@@ -3476,13 +3493,13 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 		} else {
 			launch(k_residuals, ntc + ntw, kBlock, a, ntc);
 		}
-		launch(k_err_final, S, 256, a, ctx->p.tolerance);
+		launch(k_err_final, nact, 256, a, ctx->p.tolerance);
 		PDC_CHECK_LAUNCH();
 		// assembly
 		kt_begin(ctx, PDC_KT_ASSEMBLY, &e0);
 		launch(k_asm_prep, ntc, kBlock, a);
 		// the standard form (tight tolerances) stores A itself, the recurrences A P
-		const int ntr = std::max(1, ctx->cap_tr);
+		const int ntr = pcg_grid(ctx);
 		if (fp32 && plain) {
 			launch(k_asm_diag<float, true>, ntw, kBlock, a, (const int*)ctx->a_diag.p, ctx->A32.p, ctx->P32.p, ctx->b32.p);
 			launch(k_asm_off<float, false, true>, ntp, kBlock, a, (const float*)ctx->P32.p, ctx->A32.p);
@@ -3524,13 +3541,13 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 			launch_trials(k_ls_eval, ntc + ntw, nt, a, du, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
 			              k0, (int)LS_PENDING, ntc);
 			kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
-			launch(k_ls_final, S, 1024, a, k0, nt, 10, 0);
+			launch(k_ls_final, nact, 1024, a, k0, nt, 10, 0);
 		}
 		kt_begin(ctx, PDC_KT_LINE_SEARCH, &e1);
 		launch_trials(k_ls_eval, ntc + ntw, kTrials, a, gdu, (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p,
 		              0, (int)LS_GD_PENDING, ntc);
 		kt_end(ctx, PDC_KT_LINE_SEARCH, e1);
-		launch(k_ls_final, S, 1024, a, 0, kTrials, 20, 1);
+		launch(k_ls_final, nact, 1024, a, 0, kTrials, 20, 1);
 		launch(k_commit, ntc + ntb, kBlock, a, (const double*)ctx->s_df.p, (const double*)ctx->s_dl.p,
 		       (const double*)ctx->s_rf.p, (const double*)ctx->s_rl.p, gdu, ntc);
 		launch(k_newton_end, gS, kBlock, a, ctx->p.pcg_max_iter, ctx->p.iterations);

@@ -66,11 +66,12 @@ def sample_count(shape, dims, spacing):
 
 
 def scene_samples(scene, params=None):
-    """Surface samples of a scene's bodies (a mesh body's: its vertices)."""
+    """Surface samples the library stores for a scene: those of the bodies
+    that emit them (a mesh body's: its vertices)."""
     p = params if params is not None else scene.params
     return int(sum(len(scene.meshes[int(scene.dims[k][0])][0]) if t == SHAPE_MESH
                    else sample_count(int(t), scene.dims[k], p.sample_spacing)
-                   for k, t in enumerate(scene.shape_type)))
+                   for k, t in enumerate(scene.shape_type) if scene.emit[k]))
 
 
 def scene_footprint(scene, params=None):
@@ -253,7 +254,7 @@ class _Shard:
         m = scene_samples(scene, self.params)
         best = None
         for k, seg in enumerate(self.segments):
-            if seg.owner is None and seg.cap >= n + 1 and seg.samples >= m + 6 * (seg.cap - n):
+            if seg.owner is None and seg.cap >= n + 1 and seg.samples >= m:
                 if best is None or (seg.cap, seg.samples) < (self.segments[best].cap, self.segments[best].samples):
                     best = k
         return best
@@ -281,7 +282,7 @@ class _Shard:
             first = self.ctx.append_bodies(*arrays[:6], pos=arrays[6], quat=arrays[7], vel=arrays[8],
                                            angvel=arrays[9])
             with self.server.lock:
-                self.segments.append(_Segment(first, n + 1, m + 6, h))
+                self.segments.append(_Segment(first, n + 1, m, h))
                 h.segment = len(self.segments) - 1
         self.ctx.set_scene_friction(np.array([
             (seg.owner.scene.params.friction_coeff if seg.owner is not None else p.friction_coeff)
