@@ -225,11 +225,16 @@ def cpu_baseline(sc, state, budget_s, threads, checks):
         rec["bodies"] = int(scn.n)
         rec["oracle_newton_iterations"] = int(st.newton_iterations)
         rec["oracle_converged"] = bool(st.exit_error < q.tolerance)
-        if not rec["oracle_converged"] and st.n_local_bodies < 2000:
-            # a stalled Newton loop (iteration cap): its end depends on the
-            # rounding of every inner solve; the oracle's own two branches
+        if st.n_local_bodies < 2000 and name != "headline":
+            # below 2000 bodies the reference factorises: the oracle's own two
+            # branches (direct, and PCG as on the device) from the same state
+            # show how far the step is defined (a stalled Newton loop, or a
+            # nearly singular contact set, leaves it sensitive to every inner
+            # solve), and the device against the branch it shares its solver with
             alt, _ = restatement_step(scn, q, start, orc.SOLVER_PCG)
             rec["oracle_direct_vs_pcg"] = state_errors(scn, alt, ref)
+            if dev is not None:
+                rec["device_vs_oracle_pcg"] = state_errors(scn, dev, alt)
         precision.setdefault(name, []).append(rec)
     return precision, baseline, first
 
