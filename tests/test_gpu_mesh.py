@@ -225,3 +225,33 @@ def test_mesh_scene_file_bake_matches_oracle(require_gpu, tmp_path):
     print("mesh scene: %d frames, %d contacts in all, final node translation error %.3e of the body size"
           % (len(times), contacts, worst))
     assert contacts > 0 and worst < 1e-6
+
+
+def test_register_mesh_and_grid_checks(require_gpu):
+    """pdc_register_mesh refuses out-of-range indices, non-finite coordinates
+    and meshes of zero-area triangles only; a mesh body must name a registered
+    mesh; pdc_get_grid_nodes refuses a buffer below grid_n^3 nodes."""
+    from primal_dual_contacts_miror_amd.params import default_params
+    ctx = pdc.Context(default_params(grid_n=16))
+    v, t = S.unit_cube_mesh()
+    v = v * 0.5
+    bad = [(v, t + 100), (np.where(np.arange(v.size).reshape(-1, 3) == 0, np.nan, v), t),
+           (np.zeros((3, 3)), np.array([[0, 1, 2]]))]
+    for vv, tt in bad:
+        with pytest.raises(pdc.PdcError) as e:
+            ctx.register_mesh(vv, tt)
+        assert e.value.code == pdc.PDC_ERR_INVALID_ARGUMENT
+    mid = ctx.register_mesh(v, t)
+    assert mid == 0 and ctx.register_mesh(v, t) == 1
+    ones = np.ones((2, 3))
+    with pytest.raises(pdc.PdcError):
+        ctx.upload_bodies([0, 3], [[1, 1, 1], [2.0, 0, 0]], [1, 1], ones, [1, 0])
+    with pytest.raises(pdc.PdcError):
+        ctx.upload_bodies([0, 3], [[1, 1, 1], [0.5, 0, 0]], [1, 1], ones, [1, 0])
+    ctx.upload_bodies([0, 3], [[1, 1, 1], [1.0, 0, 0]], [1, 1], ones, [1, 0])
+    g = ctx.grid_nodes(1)
+    assert g.shape == (16, 16, 16) and g.min() < 0 < g.max()
+    with pytest.raises(pdc.PdcError) as e:
+        ctx._check(ctx._L.pdc_get_grid_nodes(ctx.h, 1, 16 ** 3 - 1, g.ctypes.data_as(pdc.ctypes.POINTER(pdc.ctypes.c_float))))
+    assert e.value.code == pdc.PDC_ERR_CAPACITY
+    ctx.close()
