@@ -407,6 +407,8 @@ struct pdc_context {
 	pdc::DBuf<int> tb_c, tb_b, tb_w, tb_p;                    // per scene: first tile (S + 1)
 	int n_tc = 0, n_tb = 0, n_tw = 0, n_tp = 0, n_tr = 0;
 	int n_tr_act = 0;  // PCG tiles of the current solve's rows (the first n_tr_act of the table)
+	int one_t0 = 0, one_W = 0, one_scn = 0;  // a solve whose rows are one scene's: its PCG tiles
+	                                         // [one_t0, one_t0 + one_W) and its index
 	int cap_tr = 0;                                           // PCG tiles launched (fixed per scene set)
 	pdc_shapes shapes{{256, 256, 256, 256}, {4, 4, 4, 4}, 0};   // work shapes per size class
 	// PCG launches per group the host queues (and per recorded chunk where

@@ -627,6 +627,15 @@ static void scene_ranges(pdc_context* ctx) {
 	}
 	ctx->n_tr = (int)t.size();
 	ctx->n_tr_act = tb[S];
+	int solving = 0, only = 0;
+	for (int sc = 0; sc < S; ++sc)
+		if (tb[sc + 1] > tb[sc]) {
+			++solving;
+			only = sc;
+		}
+	ctx->one_t0 = solving == 1 ? tb[only] : 0;
+	ctx->one_W = solving == 1 ? tb[only + 1] - tb[only] : 0;
+	ctx->one_scn = only;
 	ctx->t_pcg.ensure(std::max<size_t>(pt.size(), 1));
 	PDC_HIP(hipMemcpyAsync(ctx->t_pcg.p, pt.data(), pt.size() * sizeof(PcgTile), hipMemcpyHostToDevice, ctx->stream));
 }
@@ -1807,6 +1816,9 @@ struct PcgArgs {
 	double* zv[2];   // standard form: z = P r, w = P q and p by iteration parity
 	double* wv[2];
 	double* pv[2];
+	int one_t0, one_W, one_scn;  // a solve of one scene: its tiles [one_t0, one_t0 + one_W) and index; else one_W = 0
+	int pstride;        // partial array k (gam 0, 1, del 2, 3, rr 4, 5, bb 6, e1 7, 8, e2 9, 10, e1b 11, 12)
+	                    // starts at gam[0] + k pstride
 	double* gam[2];  // partials, one per tile, by iteration parity
 	double* del[2];
 	double* rr[2];
@@ -2057,15 +2069,16 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 // returns after its record arrives.  Lane sums are DPP moves (every lane is
 // active where they are taken), and the closing workgroup sum has its own
 // LDS array, so that no wavefront writes what another may still read.
-template <class T, int EM>
+template <class T, int EM, bool ONE>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
 	static_assert(kPcgRowsPerWG >= 4, "four partial arrays, one per wavefront");
 	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
-	const int* __restrict__ rp = reinterpret_cast<const int*>(Tp);
-	int rec[8];
-#pragma unroll
-	for (int k = 0; k < 8; ++k) rec[k] = rp[k];
+	// the record by vector loads (see k_pcgs_iter)
+	int vz = 0;
+	asm volatile("" : "+v"(vz));
+	const int4* __restrict__ rv = reinterpret_cast<const int4*>(Tp) + vz;
+	const int4 ra = rv[0], rb = rv[1];
 	const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 	const int lane = threadIdx.x & 63;
 	const int i = lane >> 3, sl = lane & 7;
@@ -2073,13 +2086,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	const int br = kPcgRowsPerWG * (int)blockIdx.x + wid;
 	const int cur = slot, nxt = slot ^ 1;
 	const double* __restrict__ wc = pick(a.w, cur);
-	// round 1: the ELL columns of the row (entries sl, sl + 8, ...), the
-	// preconditioner row and the row vectors
+	// round 1: the preconditioner row, the row vectors and the ELL columns of
+	// the row (entries sl, sl + 8, ...)
 	constexpr int kE = EM > 0 ? EM : 1;
 	const int R = EM > 0 ? 8 * EM : a.R;
-	int ec[kE];
-#pragma unroll
-	for (int m = 0; m < kE; ++m) ec[m] = a.ecol[(size_t)br * R + sl + 8 * m];
 	double pr[6], wr[6], rrow[6], srow[6];
 #pragma unroll
 	for (int cc = 0; cc < 6; ++cc) {
@@ -2090,38 +2100,83 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	}
 	const int t = 6 * br + ir;
 	const double zo = a.z[t], po = a.p[t], xo = a.x[t];
-	const int scn = rec[0], cnt = rec[2], t0 = rec[3], W = rec[4], done = rec[5], it = rec[6];
-	if (scn < 0 || done || it >= a.maxit) return;
-	// the scene's gamma and alpha of the previous iteration
-	const double gprev = a.hist[4 * scn + 2 * nxt], aprev = a.hist[4 * scn + 2 * nxt + 1];
-	// round 2: the partial sums of the previous launch (issued now, summed
-	// after the gathers are in flight; a scene of more than 1024 tiles sums
-	// the rest afterwards), the operator blocks and the neighbour values of
-	// the row's occupied entries
-	const double* psrc = wid == 0 ? pick(a.gam, cur) : wid == 1 ? pick(a.del, cur) : wid == 2 ? pick(a.rr, cur) : a.bb;
-	psrc += t0;
-	double pv[kPartLoads];
-	if (wid < 4) {
+	int ec[kE];
 #pragma unroll
-		for (int k = 0; k < kPartLoads; ++k) {
-			pv[k] = 0;
-			if (64 * k < W) pv[k] = psrc[min(lane + 64 * k, W - 1)];
+	for (int m = 0; m < kE; ++m) ec[m] = a.ecol[(size_t)br * R + sl + 8 * m];
+	// the partial sums of the previous launch, the scene's first 256 tiles
+	// here (wavefronts 0..3: gam, del, rr of parity cur, bb; arrays 0, 2, 4
+	// (+ cur), 6 of the partial block), and the scene's gamma and alpha of the
+	// previous iteration; with the first round when the solve is one scene's,
+	// else after the record
+	constexpr unsigned kPartArray = 0x6420u;
+	constexpr int kFirst = 4;
+	const int pk = (int)((kPartArray >> (4 * min(wid, 3))) & 15u) + (wid < 3 ? cur : 0);
+	double pv[kFirst];
+	const double* psrc = nullptr;
+	int t0 = 0, W = 0;
+	double gprev = 0, aprev = 0;
+	auto load_partials = [&](int first, int count, int sc) {
+		t0 = first;
+		W = count;
+		psrc = a.gam[0] + (size_t)pk * a.pstride + t0;
+#pragma unroll
+		for (int k = 0; k < kFirst; ++k) {
+			pv[k] = 0.0;
+			if (wid < 4 && lane + 64 * k < W) pv[k] = psrc[lane + 64 * k];
+		}
+		gprev = a.hist[4 * sc + 2 * nxt];
+		aprev = a.hist[4 * sc + 2 * nxt + 1];
+	};
+	if (ONE) load_partials(a.one_t0, a.one_W, a.one_scn);
+	// every lane holds the same record
+	const int scn = __builtin_amdgcn_readfirstlane(ra.x), cnt = __builtin_amdgcn_readfirstlane(ra.z);
+	const int done = __builtin_amdgcn_readfirstlane(rb.y), it = __builtin_amdgcn_readfirstlane(rb.z);
+	if (scn < 0 || done || it >= a.maxit) return;
+	if (!ONE) load_partials(__builtin_amdgcn_readfirstlane(ra.w), __builtin_amdgcn_readfirstlane(rb.x), scn);
+	// round 2: the operator blocks and the neighbour values of the row's
+	// occupied entries; entries sl without a branch (a lane whose entry is
+	// empty, or of rows 6, 7, reads the row's first block and its own w, lines
+	// its row reads anyway, and drops the product)
+	// (all loads issued before the first product: a product between them
+	// would wait for the loads before it)
+	double bv[kE][6], vv[kE][6];
+	{
+		const bool use = i < 6 && ec[0] >= 0;
+		const int c = use ? ec[0] : br;
+		const T* blk = a.B + 36 * ((size_t)br * R + (use ? sl : 0)) + 6 * ir;
+		const double* vc = wc + 6 * (size_t)c;
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) {
+			const double b0 = (double)blk[cc], v0 = vc[cc];
+			bv[0][cc] = use ? b0 : 0.0;
+			vv[0][cc] = use ? v0 : 0.0;
+		}
+	}
+#pragma unroll
+	for (int m = 1; m < kE; ++m) {
+		const int c = i < 6 ? ec[m] : -1;
+		if (c >= 0) {
+			const T* blk = a.B + 36 * ((size_t)br * R + sl + 8 * m) + 6 * i;
+			const double* vc = wc + 6 * (size_t)c;
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) {
+				bv[m][cc] = (double)blk[cc];
+				vv[m][cc] = vc[cc];
+			}
+		} else {
+#pragma unroll
+			for (int cc = 0; cc < 6; ++cc) bv[m][cc] = vv[m][cc] = 0.0;
 		}
 	}
 	double n = 0;
+#pragma unroll
+	for (int m = 0; m < kE; ++m) {
+		double d = 0;
+#pragma unroll
+		for (int cc = 0; cc < 6; ++cc) d += bv[m][cc] * vv[m][cc];
+		n += d;
+	}
 	if (i < 6) {
-#pragma unroll
-		for (int m = 0; m < kE; ++m) {
-			const int c = ec[m];
-			if (c >= 0) {
-				const T* blk = a.B + 36 * ((size_t)br * R + sl + 8 * m) + 6 * i;
-				const double* vc = wc + 6 * (size_t)c;
-				double d = 0;
-#pragma unroll
-				for (int cc = 0; cc < 6; ++cc) d += (double)blk[cc] * vc[cc];
-				n += d;
-			}
-		}
 		if (EM == 0)
 			for (int e = sl + 8; e < R; e += 8) {
 				const size_t k = (size_t)br * R + e;
@@ -2148,10 +2203,11 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 		ms += pr[cc] * srow[cc];
 	}
 	if (wid < 4) {
-		double part = 0;
+		// (not 0 + pv[0]: see k_pcgs_iter)
+		double part = pv[0];
 #pragma unroll
-		for (int k = 0; k < kPartLoads; ++k) part += lane + 64 * k < W ? pv[k] : 0.0;
-		if (W > 64 * kPartLoads) part += partial_lane_sum(psrc + 64 * kPartLoads, W - 64 * kPartLoads);
+		for (int k = 1; k < kFirst; ++k) part += pv[k];
+		if (W > 64 * kFirst) part += partial_lane_sum(psrc + 64 * kFirst, W - 64 * kFirst);
 		part = wave_sum_dpp(part);
 		if (lane == 0) lds[wid] = part;
 	}
@@ -2334,20 +2390,25 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_init(PcgArgs<T> a) {
 }
 
 // One iteration F(i), i >= 1, launch slot = parity of i - 1.  Memory round
-// trips as in k_pcg_iter: (1) the record, the ELL columns, the
-// preconditioner row and the row's own r, q, x, z, w, p; (2) the partials of
-// F(i-1) (seven arrays, one per wavefront), the operator blocks and the
-// neighbours' z, w, p of the occupied entries, combined into p_i after
-// alpha and beta are known.  EM as in k_pcg_iter.
-template <class T, int EM>
+// trips: (1) the record, the ELL columns, the preconditioner row and the
+// row's own r, q, x, z, w, p; (2) the operator blocks and the neighbours'
+// z, w, p of the occupied entries (their addresses come with the columns),
+// combined into p_i after alpha and beta are known.  The partials of F(i-1)
+// (seven arrays, one per wavefront) load with (1) when the solve's rows are
+// one scene's (ONE: its tiles come with the arguments), else with (2) (the
+// record gives the scene's tiles).  EM as in k_pcg_iter.
+template <class T, int EM, bool ONE>
 __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slot) {
 	__shared__ double lds[8];
 	static_assert(kPcgRowsPerWG >= 7, "seven partial arrays, one per wavefront");
 	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
-	const int* __restrict__ rp4 = reinterpret_cast<const int*>(Tp);
-	int rec[8];
-#pragma unroll
-	for (int k = 0; k < 8; ++k) rec[k] = rp4[k];
+	// the record by vector loads (an offset the compiler holds in a vector
+	// register): a scalar load of it would be waited for together with the
+	// kernel arguments that address the first round
+	int vz = 0;
+	asm volatile("" : "+v"(vz));
+	const int4* __restrict__ rv = reinterpret_cast<const int4*>(Tp) + vz;
+	const int4 ra = rv[0], rb = rv[1];
 	const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 	const int lane = threadIdx.x & 63;
 	const int i = lane >> 3, sl = lane & 7;
@@ -2360,33 +2421,65 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slo
 	// round 1
 	constexpr int kE = EM > 0 ? EM : 1;
 	const int R = EM > 0 ? 8 * EM : a.R;
+	const size_t t = 6 * (size_t)br + ir;
+	const double xo = a.x[t], ro = a.r[t], qo = a.q[t], zo = zc[t], wo = wc[t], po = pc[t];
 	int ec[kE];
 #pragma unroll
 	for (int m = 0; m < kE; ++m) ec[m] = a.ecol[(size_t)br * R + sl + 8 * m];
 	const int lgr = a.gsz[br];
-	const int t = 6 * br + ir;
-	const double xo = a.x[t], ro = a.r[t], qo = a.q[t], zo = zc[t], wo = wc[t], po = pc[t];
-	const int scn = rec[0], cnt = rec[2], t0 = rec[3], W = rec[4], done = rec[5], it = rec[6];
+	// the partial sums of F(i-1), the scene's first 256 tiles here: wavefront
+	// wid < 7 sums del, gam, e1, e1b, e2, rr (parity cur) or bb, arrays 2, 0,
+	// 7, 11, 9, 4, 6 (+ cur) of the partial block.  Addressed by arithmetic
+	// and loaded under lane conditions: selecting among the argument pointers,
+	// or a uniform branch per load, compiles to code that waits for the
+	// record, or for every load, before the next load issues.
+	constexpr unsigned kPartArray = 0x649B702u;
+	constexpr int kFirst = 4;
+	const int pk = (int)((kPartArray >> (4 * min(wid, 6))) & 15u) + (wid < 6 ? cur : 0);
+	double pvl[kFirst];
+	const double* psrc = nullptr;
+	int t0 = 0, W = 0;
+	auto load_partials = [&](int first, int count) {
+		t0 = first;
+		W = count;
+		psrc = a.gam[0] + (size_t)pk * a.pstride + t0;
+#pragma unroll
+		for (int k = 0; k < kFirst; ++k) {
+			pvl[k] = 0.0;
+			if (wid < 7 && lane + 64 * k < W) pvl[k] = psrc[lane + 64 * k];
+		}
+	};
+	if (ONE) load_partials(a.one_t0, a.one_W);
+	// every lane holds the same record
+	const int scn = __builtin_amdgcn_readfirstlane(ra.x), cnt = __builtin_amdgcn_readfirstlane(ra.z);
+	const int done = __builtin_amdgcn_readfirstlane(rb.y), it = __builtin_amdgcn_readfirstlane(rb.z);
 	if (scn < 0 || done || it >= a.maxit) return;
-	// round 2: partial sums of F(i-1), operator blocks, neighbour z, w, p and
-	// the row's preconditioner entries
+	if (!ONE) load_partials(__builtin_amdgcn_readfirstlane(ra.w), __builtin_amdgcn_readfirstlane(rb.x));
+	// round 2: operator blocks, neighbour z, w, p and the row's
+	// preconditioner entries
 	double pg[6];
 	const bool gin = group_row(a, br, wid, i, sl, cnt, wid < cnt ? lgr : 0, pg);
-	const double* psrc = wid == 0 ? pick(a.del, cur) : wid == 1 ? pick(a.gam, cur) : wid == 2 ? pick(a.e1, cur)
-	                   : wid == 3 ? pick(a.e1b, cur) : wid == 4 ? pick(a.e2, cur) : wid == 5 ? pick(a.rr, cur)
-	                   : a.bb;
-	psrc += t0;
-	double pvl[kPartLoads];
-	if (wid < 7) {
+	double blk[kE][6], gz[kE][6], gw[kE][6], gp[kE][6];
+	{
+		// entries sl (about half of them occupied) without a branch: a lane
+		// whose entry is empty (or of rows 6, 7) reads the row's first block
+		// and its own z, w, p, lines its row reads anyway, and drops them, so
+		// that no wait for these loads is placed before the later entries issue
+		const bool use = i < 6 && ec[0] >= 0;
+		const int c = use ? ec[0] : br;
+		const T* bp = a.B + 36 * ((size_t)br * R + (use ? sl : 0)) + 6 * ir;
 #pragma unroll
-		for (int k = 0; k < kPartLoads; ++k) {
-			pvl[k] = 0;
-			if (64 * k < W) pvl[k] = psrc[min(lane + 64 * k, W - 1)];
+		for (int cc = 0; cc < 6; ++cc) {
+			const double b0 = (double)bp[cc], z0 = zc[6 * (size_t)c + cc], w0 = wc[6 * (size_t)c + cc],
+			             p0 = pc[6 * (size_t)c + cc];
+			blk[0][cc] = use ? b0 : 0.0;
+			gz[0][cc] = use ? z0 : 0.0;
+			gw[0][cc] = use ? w0 : 0.0;
+			gp[0][cc] = use ? p0 : 0.0;
 		}
 	}
-	double blk[kE][6], gz[kE][6], gw[kE][6], gp[kE][6];
 #pragma unroll
-	for (int m = 0; m < kE; ++m) {
+	for (int m = 1; m < kE; ++m) {
 		const int c = i < 6 ? ec[m] : -1;
 		if (c >= 0) {
 			const T* bp = a.B + 36 * ((size_t)br * R + sl + 8 * m) + 6 * i;
@@ -2403,10 +2496,12 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slo
 		}
 	}
 	if (wid < 7) {
-		double part = 0;
+		// (not 0 + pvl[0]: the compiler would form that sum in the load's
+		// branch and wait for every load issued so far there)
+		double part = pvl[0];
 #pragma unroll
-		for (int k = 0; k < kPartLoads; ++k) part += lane + 64 * k < W ? pvl[k] : 0.0;
-		if (W > 64 * kPartLoads) part += partial_lane_sum(psrc + 64 * kPartLoads, W - 64 * kPartLoads);
+		for (int k = 1; k < kFirst; ++k) part += pvl[k];
+		if (W > 64 * kFirst) part += partial_lane_sum(psrc + 64 * kFirst, W - 64 * kFirst);
 		part = wave_sum_dpp(part);
 		if (lane == 0) lds[wid] = part;
 	}
@@ -2971,17 +3066,29 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k], s));
 		// the standard form keeps four 6-vectors per entry in flight: above two
 		// entries per lane the generic width reads the rest after the barrier
-		if (two) switch (pa.R / 8) {
-			case 1: hipLaunchKernelGGL((k_pcgs_iter<T, 1>), dim3(nblk), B, 0, s, pa, k & 1); break;
-			case 2: hipLaunchKernelGGL((k_pcgs_iter<T, 2>), dim3(nblk), B, 0, s, pa, k & 1); break;
-			default: hipLaunchKernelGGL((k_pcgs_iter<T, 0>), dim3(nblk), B, 0, s, pa, k & 1); break;
+		if (two && pa.one_W > 0) switch (pa.R / 8) {
+			case 1: hipLaunchKernelGGL((k_pcgs_iter<T, 1, true>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 2: hipLaunchKernelGGL((k_pcgs_iter<T, 2, true>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			default: hipLaunchKernelGGL((k_pcgs_iter<T, 0, true>), dim3(nblk), B, 0, s, pa, k & 1); break;
+		}
+		else if (two) switch (pa.R / 8) {
+			case 1: hipLaunchKernelGGL((k_pcgs_iter<T, 1, false>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 2: hipLaunchKernelGGL((k_pcgs_iter<T, 2, false>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			default: hipLaunchKernelGGL((k_pcgs_iter<T, 0, false>), dim3(nblk), B, 0, s, pa, k & 1); break;
+		}
+		else if (pa.one_W > 0) switch (pa.R / 8) {
+			case 1: hipLaunchKernelGGL((k_pcg_iter<T, 1, true>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 2: hipLaunchKernelGGL((k_pcg_iter<T, 2, true>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 3: hipLaunchKernelGGL((k_pcg_iter<T, 3, true>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 4: hipLaunchKernelGGL((k_pcg_iter<T, 4, true>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			default: hipLaunchKernelGGL((k_pcg_iter<T, 0, true>), dim3(nblk), B, 0, s, pa, k & 1); break;
 		}
 		else switch (pa.R / 8) {
-			case 1: hipLaunchKernelGGL((k_pcg_iter<T, 1>), dim3(nblk), B, 0, s, pa, k & 1); break;
-			case 2: hipLaunchKernelGGL((k_pcg_iter<T, 2>), dim3(nblk), B, 0, s, pa, k & 1); break;
-			case 3: hipLaunchKernelGGL((k_pcg_iter<T, 3>), dim3(nblk), B, 0, s, pa, k & 1); break;
-			case 4: hipLaunchKernelGGL((k_pcg_iter<T, 4>), dim3(nblk), B, 0, s, pa, k & 1); break;
-			default: hipLaunchKernelGGL((k_pcg_iter<T, 0>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 1: hipLaunchKernelGGL((k_pcg_iter<T, 1, false>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 2: hipLaunchKernelGGL((k_pcg_iter<T, 2, false>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 3: hipLaunchKernelGGL((k_pcg_iter<T, 3, false>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			case 4: hipLaunchKernelGGL((k_pcg_iter<T, 4, false>), dim3(nblk), B, 0, s, pa, k & 1); break;
+			default: hipLaunchKernelGGL((k_pcg_iter<T, 0, false>), dim3(nblk), B, 0, s, pa, k & 1); break;
 		}
 		if (ev) PDC_HIP(hipEventRecord(ev[2 * k + 1], s));
 	}
@@ -3025,6 +3132,10 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 	pa.e2[0] = parts + 9 * pcap; pa.e2[1] = parts + 10 * pcap;
 	pa.e1b[0] = parts + 11 * pcap; pa.e1b[1] = parts + 12 * pcap;
 	pa.hist = parts + 13 * pcap;
+	pa.pstride = (int)pcap;
+	pa.one_t0 = ctx->one_t0;
+	pa.one_W = ctx->one_W;
+	pa.one_scn = ctx->one_scn;
 	// the standard form keeps r and q (own rows) in the recurrence form's r
 	// and q, and z, w, p by parity in its z, s / w[0], w[1] / p and an eighth vector
 	pa.zv[0] = pa.z; pa.zv[1] = pa.s;
