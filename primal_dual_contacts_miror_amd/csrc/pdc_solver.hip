@@ -1816,6 +1816,7 @@ struct PcgArgs {
 	double* zv[2];   // standard form: z = P r, w = P q and p by iteration parity
 	double* wv[2];
 	double* pv[2];
+	int ntiles;         // the solve's tiles (workgroups past them return)
 	int one_t0, one_W, one_scn;  // a solve of one scene: its tiles [one_t0, one_t0 + one_W) and index; else one_W = 0
 	int pstride;        // partial array k (gam 0, 1, del 2, 3, rr 4, 5, bb 6, e1 7, 8, e2 9, 10, e1b 11, 12)
 	                    // starts at gam[0] + k pstride
@@ -1843,9 +1844,19 @@ __device__ __forceinline__ P pick(const P (&slots)[2], int k) {
 	return k ? slots[1] : slots[0];
 }
 
-// The wavefront's slot.
-__device__ __forceinline__ int pcg_slot() {
-	return kPcgRowsPerWG * (int)blockIdx.x + (int)(threadIdx.x >> 6);
+// The PCG tile of this workgroup.  Workgroups are dispatched to the eight
+// XCDs in turn (workgroup b to XCD b mod 8; the grid is a multiple of 8), so
+// tile (b mod 8) (G / 8) + b / 8 gives each XCD a contiguous eighth of the
+// tiles: tiles near each other in slot order read their neighbours' rows
+// through the same L2.  Workgroups past the solve's tiles return.
+__device__ __forceinline__ int pcg_tile_id() {
+	const int b = (int)blockIdx.x, per = (int)(gridDim.x >> 3);
+	return (b & 7) * per + (b >> 3);
+}
+
+// The wavefront's slot in tile `tile`.
+__device__ __forceinline__ int pcg_slot(int tile) {
+	return kPcgRowsPerWG * tile + (int)(threadIdx.x >> 6);
 }
 
 // Row `i` (0..5) of the ELL row of slot `sr` of B times v; every lane of the
@@ -1922,11 +1933,11 @@ __device__ __forceinline__ void reduce_partials(const double* p0, const double* 
 // Start of a solve for one tile: its state; the scene's first tile also
 // resets the scene's outcome and history.
 template <class T>
-__device__ __forceinline__ void pcg_tile_reset(const PcgArgs<T>& a, PcgTile* Tp, int scn) {
+__device__ __forceinline__ void pcg_tile_reset(const PcgArgs<T>& a, PcgTile* Tp, int scn, int tile) {
 	if (threadIdx.x == 0) {
 		Tp->done = 0;
 		Tp->it = 0;
-		if ((int)blockIdx.x == Tp->t0) {
+		if (tile == Tp->t0) {
 			a.ps[scn].done = 0;
 			a.ps[scn].iters = 0;
 			a.ps[scn].bad = 0;
@@ -1971,14 +1982,16 @@ __device__ __forceinline__ bool pcg_tile_skip(const PcgArgs<T>& a, PcgTile* Tp) 
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_start(PcgArgs<T> a, const double* __restrict__ y) {
 	__shared__ double lds[2 * 16];
-	PcgTile* Tp = a.tiles + blockIdx.x;
+	const int tile = pcg_tile_id();
+	if (tile >= a.ntiles) return;
+	PcgTile* Tp = a.tiles + tile;
 	if (pcg_tile_skip(a, Tp)) return;
 	const int cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	double v[2] = {0, 0};
 	if (wid < cnt) {
-		const int br = pcg_slot();
+		const int br = pcg_slot(tile);
 		const double ay = spmv_ell(a, y, br, i, sl);
 		if (i < 6 && sl == 0) {
 			const double bt = (double)a.b[6 * br + i], rt = bt - ay;
@@ -1990,8 +2003,8 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_start(PcgArgs<T> a, const d
 	double out[2];
 	block_sum<2, double>(v, lds, out);
 	if (threadIdx.x == 0) {
-		a.rr[1][blockIdx.x] = out[0];
-		a.bb[blockIdx.x] = out[1];
+		a.rr[1][tile] = out[0];
+		a.bb[tile] = out[1];
 	}
 }
 
@@ -2013,7 +2026,9 @@ __device__ __forceinline__ bool pcg_zero_start(const PcgArgs<T>& a, const PcgTil
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	__shared__ double lds[4 * 16];
-	PcgTile* Tp = a.tiles + blockIdx.x;
+	const int tile = pcg_tile_id();
+	if (tile >= a.ntiles) return;
+	PcgTile* Tp = a.tiles + tile;
 	if (pcg_tile_skip(a, Tp)) return;
 	const int scn = Tp->scene, cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -2022,7 +2037,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	__syncthreads();
 	double v[3] = {0, 0, 0};
 	if (wid < cnt) {
-		const int br = pcg_slot();
+		const int br = pcg_slot(tile);
 		const double w0 = zero ? spmv_ell(a, a.b, br, i, sl) : spmv_ell(a, a.r, br, i, sl);
 		if (i < 6 && sl == 0) {
 			const int t = 6 * br + i;
@@ -2049,11 +2064,11 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(PcgArgs<T> a) {
 	double out[3];
 	block_sum<3, double>(v, lds, out);
 	if (threadIdx.x == 0) {
-		a.gam[0][blockIdx.x] = out[0];
-		a.del[0][blockIdx.x] = out[1];
-		a.rr[0][blockIdx.x] = out[2];
+		a.gam[0][tile] = out[0];
+		a.del[0][tile] = out[1];
+		a.rr[0][tile] = out[2];
 	}
-	pcg_tile_reset(a, Tp, scn);
+	pcg_tile_reset(a, Tp, scn, tile);
 }
 
 // Iteration parity equals the launch slot: every chunk has an even number of
@@ -2073,7 +2088,9 @@ template <class T, int EM, bool ONE>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot) {
 	__shared__ double lds[4 * 16];
 	static_assert(kPcgRowsPerWG >= 4, "four partial arrays, one per wavefront");
-	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
+	const int tile = pcg_tile_id();
+	if (tile >= a.ntiles) return;
+	PcgTile* __restrict__ Tp = a.tiles + tile;
 	// the record by vector loads (see k_pcgs_iter)
 	int vz = 0;
 	asm volatile("" : "+v"(vz));
@@ -2083,7 +2100,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	const int lane = threadIdx.x & 63;
 	const int i = lane >> 3, sl = lane & 7;
 	const int ir = i < 6 ? i : 5;  // rows 6, 7 of the lane layout read row 5's addresses
-	const int br = kPcgRowsPerWG * (int)blockIdx.x + wid;
+	const int br = kPcgRowsPerWG * tile + wid;
 	const int cur = slot, nxt = slot ^ 1;
 	const double* __restrict__ wc = pick(a.w, cur);
 	// round 1: the preconditioner row, the row vectors and the ELL columns of
@@ -2213,7 +2230,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	}
 	__syncthreads();
 	const double gamma = lds[0], delta = lds[1], rr = lds[2], bb = lds[3];
-	const bool first_tile = (int)blockIdx.x == t0;
+	const bool first_tile = tile == t0;
 	// a non-finite sum ends the solve as a breakdown (a NaN would otherwise
 	// read as converged: every comparison with it is false)
 	const bool bad = !(isfinite(rr) && isfinite(bb) && isfinite(gamma) && isfinite(delta));
@@ -2248,9 +2265,9 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_iter(PcgArgs<T> a, int slot
 	double out[3];
 	block_sum_last<3, kPcgRowsPerWG>(v, lds_b, out);
 	if (threadIdx.x == 0) {
-		pick(a.gam, nxt)[blockIdx.x] = out[0];
-		pick(a.del, nxt)[blockIdx.x] = out[1];
-		pick(a.rr, nxt)[blockIdx.x] = out[2];
+		pick(a.gam, nxt)[tile] = out[0];
+		pick(a.del, nxt)[tile] = out[1];
+		pick(a.rr, nxt)[tile] = out[2];
 		Tp->it = it + 1;
 		if (first_tile) {
 			a.hist[4 * scn + 2 * cur] = gamma;
@@ -2293,13 +2310,15 @@ template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcgs_start(PcgArgs<T> a, const double* __restrict__ x0) {
 	__shared__ double lds[2 * 16];
 	__shared__ double rl[6 * kPcgRowsPerWG], bl[6 * kPcgRowsPerWG];
-	PcgTile* Tp = a.tiles + blockIdx.x;
+	const int tile = pcg_tile_id();
+	if (tile >= a.ntiles) return;
+	PcgTile* Tp = a.tiles + tile;
 	if (pcg_tile_skip(a, Tp)) return;
 	const int cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const int i = lane >> 3, sl = lane & 7;
 	const int ir = i < 6 ? i : 5;
-	const int br = pcg_slot();
+	const int br = pcg_slot(tile);
 	const int lg = wid < cnt ? a.gsz[br] : 0;
 	double pg[6];
 	const bool gin = group_row(a, br, wid, i, sl, cnt, lg, pg);
@@ -2329,15 +2348,17 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_start(PcgArgs<T> a, const 
 	double out[2];
 	block_sum<2, double>(v, lds, out);
 	if (threadIdx.x == 0) {
-		a.rr[1][blockIdx.x] = out[0];
-		a.bb[blockIdx.x] = out[1];
+		a.rr[1][tile] = out[0];
+		a.bb[tile] = out[1];
 	}
 }
 
 template <class T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcgs_init(PcgArgs<T> a) {
 	__shared__ double lds[7 * 16];
-	PcgTile* Tp = a.tiles + blockIdx.x;
+	const int tile = pcg_tile_id();
+	if (tile >= a.ntiles) return;
+	PcgTile* Tp = a.tiles + tile;
 	if (pcg_tile_skip(a, Tp)) return;
 	const int scn = Tp->scene, cnt = Tp->count;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -2345,7 +2366,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_init(PcgArgs<T> a) {
 	const bool zero = pcg_zero_start(a, Tp, lds);
 	__syncthreads();
 	__shared__ double ql[6 * kPcgRowsPerWG];
-	const int br = pcg_slot();
+	const int br = pcg_slot(tile);
 	const int lg = wid < cnt ? a.gsz[br] : 0;
 	double pg[6];
 	const bool gin = group_row(a, br, wid, i, sl, cnt, lg, pg);
@@ -2379,14 +2400,14 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_init(PcgArgs<T> a) {
 	double out[6];
 	block_sum<6, double>(v, lds, out);
 	if (threadIdx.x == 0) {
-		a.del[0][blockIdx.x] = out[0];
-		a.gam[0][blockIdx.x] = out[1];
-		a.e1[0][blockIdx.x] = out[2];
-		a.e1b[0][blockIdx.x] = out[3];
-		a.e2[0][blockIdx.x] = out[4];
-		a.rr[0][blockIdx.x] = out[5];
+		a.del[0][tile] = out[0];
+		a.gam[0][tile] = out[1];
+		a.e1[0][tile] = out[2];
+		a.e1b[0][tile] = out[3];
+		a.e2[0][tile] = out[4];
+		a.rr[0][tile] = out[5];
 	}
-	pcg_tile_reset(a, Tp, scn);
+	pcg_tile_reset(a, Tp, scn, tile);
 }
 
 // One iteration F(i), i >= 1, launch slot = parity of i - 1.  Memory round
@@ -2401,7 +2422,9 @@ template <class T, int EM, bool ONE>
 __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slot) {
 	__shared__ double lds[8];
 	static_assert(kPcgRowsPerWG >= 7, "seven partial arrays, one per wavefront");
-	PcgTile* __restrict__ Tp = a.tiles + blockIdx.x;
+	const int tile = pcg_tile_id();
+	if (tile >= a.ntiles) return;
+	PcgTile* __restrict__ Tp = a.tiles + tile;
 	// the record by vector loads (an offset the compiler holds in a vector
 	// register): a scalar load of it would be waited for together with the
 	// kernel arguments that address the first round
@@ -2413,7 +2436,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slo
 	const int lane = threadIdx.x & 63;
 	const int i = lane >> 3, sl = lane & 7;
 	const int ir = i < 6 ? i : 5;
-	const int br = kPcgRowsPerWG * (int)blockIdx.x + wid;
+	const int br = kPcgRowsPerWG * tile + wid;
 	const int cur = slot, nxt = slot ^ 1;
 	const double* __restrict__ zc = pick(a.zv, cur);
 	const double* __restrict__ wc = pick(a.wv, cur);
@@ -2507,7 +2530,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slo
 	}
 	__syncthreads();
 	const double delta = lds[0], gamma = lds[1], e1 = lds[2], e1b = lds[3], e2 = lds[4], rr = lds[5], bb = lds[6];
-	const bool first_tile = (int)blockIdx.x == t0;
+	const bool first_tile = tile == t0;
 	const bool bad = !(isfinite(rr) && isfinite(bb) && isfinite(gamma) && isfinite(delta) && isfinite(e1) &&
 	                   isfinite(e1b) && isfinite(e2));
 	// converged, at the rounding floor (sums no longer positive), or broken down
@@ -2568,12 +2591,12 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcgs_iter(PcgArgs<T> a, int slo
 	double out[6];
 	block_sum_last<6, kPcgRowsPerWG>(v, lds_b, out);
 	if (threadIdx.x == 0) {
-		pick(a.del, nxt)[blockIdx.x] = out[0];
-		pick(a.gam, nxt)[blockIdx.x] = out[1];
-		pick(a.e1, nxt)[blockIdx.x] = out[2];
-		pick(a.e1b, nxt)[blockIdx.x] = out[3];
-		pick(a.e2, nxt)[blockIdx.x] = out[4];
-		pick(a.rr, nxt)[blockIdx.x] = out[5];
+		pick(a.del, nxt)[tile] = out[0];
+		pick(a.gam, nxt)[tile] = out[1];
+		pick(a.e1, nxt)[tile] = out[2];
+		pick(a.e1b, nxt)[tile] = out[3];
+		pick(a.e2, nxt)[tile] = out[4];
+		pick(a.rr, nxt)[tile] = out[5];
 		Tp->it = it + 1;
 		if (first_tile) {
 			a.prog[scn].rel2 = rr / bb;
@@ -3102,6 +3125,9 @@ static void pcg_chunk_launches(pdc_context* ctx, const PcgArgs<T>& pa, int nblk,
 // scenes of one step call), not the tile capacity of the whole scene set, so
 // that stepping a few of many resident scenes launches only their tiles.
 static int pcg_grid(const pdc_context* ctx) { return std::max(1, ctx->n_tr_act); }
+// Workgroups of a launch of the PCG kernels: those tiles rounded up to a
+// multiple of the eight XCDs (pcg_tile_id).
+static int pcg_launch_grid(const pdc_context* ctx) { return (pcg_grid(ctx) + 7) / 8 * 8; }
 
 template <class T>
 static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b, DBuf<T>& G) {
@@ -3133,6 +3159,7 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 	pa.e1b[0] = parts + 11 * pcap; pa.e1b[1] = parts + 12 * pcap;
 	pa.hist = parts + 13 * pcap;
 	pa.pstride = (int)pcap;
+	pa.ntiles = pcg_grid(ctx);
 	pa.one_t0 = ctx->one_t0;
 	pa.one_W = ctx->one_W;
 	pa.one_scn = ctx->one_scn;
@@ -3155,7 +3182,7 @@ static PcgArgs<T> pcg_args(pdc_context* ctx, DBuf<T>& Bm, DBuf<T>& P, DBuf<T>& b
 // and the first partial sums.
 template <class T>
 static void pcg_begin(pdc_context* ctx, const PcgArgs<T>& pa) {
-	const int nblk = pcg_grid(ctx);
+	const int nblk = pcg_launch_grid(ctx);
 	hipStream_t s = ctx->stream;
 	if (ctx->p.pcg_tol < kSingleReductionMinTol) {
 		hipLaunchKernelGGL(k_pcgs_start<T>, dim3(nblk), dim3(kPcgThreads), 0, s, pa, (const double*)ctx->y64.p);
@@ -3197,7 +3224,7 @@ static void pcg_account_bytes(pdc_context* ctx, const std::vector<int>& iters, b
 // when a buffer moved, the grid changed or the chunk length changed).
 template <class T>
 static PcgGraph& pcg_graph(pdc_context* ctx, const PcgArgs<T>& pa) {
-	const int nblk = pcg_grid(ctx);
+	const int nblk = pcg_launch_grid(ctx);
 	hipStream_t s = ctx->stream;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
 	const int chunk = ctx->pcg_chunk();
@@ -3226,7 +3253,7 @@ static PcgGraph& pcg_graph(pdc_context* ctx, const PcgArgs<T>& pa) {
 template <class T>
 static bool pcg_solve(pdc_context* ctx, const PcgArgs<T>& pa) {
 	const int S = ctx->S;
-	const int nblk = pcg_grid(ctx);
+	const int nblk = pcg_launch_grid(ctx);
 	hipStream_t s = ctx->stream;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
 	const int chunk = ctx->pcg_chunk();
@@ -3330,7 +3357,7 @@ static bool pcg_solve_async(pdc_context* ctx, const PcgArgs<T>& pa, int iter, st
 	const int S = ctx->S;
 	hipStream_t s = ctx->stream;
 	const int chunk = ctx->pcg_chunk();
-	const int nblk = pcg_grid(ctx);
+	const int nblk = pcg_launch_grid(ctx);
 	const int maxit = ctx->p.pcg_max_iter;
 	const bool two = ctx->p.pcg_tol < kSingleReductionMinTol;
 	const bool kt_iter = ctx->kt.which == PDC_KT_PCG;
