@@ -1631,15 +1631,15 @@ __device__ __forceinline__ bool spd6_inverse(const double* S, double (&P)[36]) {
 }
 
 // Inverse blocks of the body groups (standard PCG form) from the assembled,
-// scaled operator: one workgroup per PCG tile, one wavefront per group (the
-// wavefront of the group's first slot), lane g i + k holding block (i, k) of
-// the group's g x g matrix of 6 x 6 blocks (the diagonal blocks and the
-// blocks between the group's bodies, found in the ELL rows).  The matrix, a
+// scaled operator: one workgroup per PCG tile, one wavefront per slot (block
+// row i of its group's g x g matrix of 6 x 6 blocks: the diagonal blocks and
+// the blocks between the group's bodies, found in the ELL rows), lane l
+// holding elements l, l + 64, ... of the row's 36 g entries.  The matrix, a
 // principal submatrix of the symmetric positive definite Newton matrix, is
-// inverted in place by block Gauss-Jordan elimination without pivoting:
-// step j publishes block row and column j through LDS, every lane inverts
-// the pivot block D = G_jj (Cholesky) and updates its block: G_jj <- D^-1,
-// G_jk <- D^-1 G_jk, G_ij <- -G_ij D^-1, G_ik <- G_ik - G_ij D^-1 G_jk.  A
+// inverted in place by block Gauss-Jordan elimination without pivoting: step
+// j publishes block row j through LDS, every wavefront of the group inverts
+// the pivot block D = G_jj (Cholesky) and updates its row: G_jj <- D^-1,
+// G_jk <- D^-1 G_jk, G_ij <- -G_ij D^-1, G_ik <- G_ik - (G_ij D^-1) G_jk.  A
 // matrix with a pivot block that is not positive definite gets the inverse
 // of its diagonal instead.  Stored per slot: its six rows against the tile's
 // eight slots, entry (i, 6 m + c) at G[288 slot + 48 i + 6 m + c], written
@@ -1647,111 +1647,122 @@ __device__ __forceinline__ bool spd6_inverse(const double* S, double (&P)[36]) {
 constexpr int kGroupRow = 6 * kPcgRowsPerWG;             // 48 columns of a tile row
 constexpr int kGroupBlock = 6 * kGroupRow;               // 288 entries per slot
 static_assert((1 << kGroupLevels) == kPcgRowsPerWG, "the largest group fills one PCG tile");
-static_assert(kPcgRowsPerWG * kPcgRowsPerWG <= 64, "one lane per block of the largest group");
 template <class T>
 __global__ __launch_bounds__(64 * kPcgRowsPerWG) void k_grp_inv(const PcgTile* __restrict__ tiles,
                                                         const SceneSc* __restrict__ ss, const int* __restrict__ ecol,
                                                         int R, const T* __restrict__ Aval,
                                                         const int* __restrict__ gsz, T* __restrict__ G) {
-	__shared__ double pub[kPcgRowsPerWG][2 * kPcgRowsPerWG][36];  // per wavefront: column j, then row j
+	constexpr int NW = kPcgRowsPerWG;
+	constexpr int kQ = (36 * NW + 63) / 64;    // elements of a block row per lane
+	__shared__ double prow[2][NW][36];         // the pivot rows by step parity, block m at column slot m
+	__shared__ double wcol[NW][36];            // per wavefront: its block in the pivot column, G_ij
+	__shared__ double wx[NW][36];              // per wavefront: X = G_ij D^-1
+	__shared__ double wd[NW][36];              // per wavefront: D^-1 (read by element index)
 	const PcgTile* Tp = tiles + blockIdx.x;
 	const int scn = Tp->scene, cnt = Tp->count;
 	if (scn < 0 || !ss[scn].run) return;
-	const int wid = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-	if (wid >= cnt) return;
-	const int base = kPcgRowsPerWG * (int)blockIdx.x;
-	const int lg = gsz[base + wid];
-	const int gs = 1 << lg;
-	if (wid & (gs - 1)) return;  // not the group's first slot
-	const bool act = lane < gs * gs;
-	const int bi = act ? lane >> lg : 0, bk = act ? lane & (gs - 1) : 0;
-	const int sr = base + wid + bi;  // the block's row slot
-	// block (bi, bk): the ELL entry of row sr whose column is slot base + wid + bk
-	double M[36];
-#pragma unroll
-	for (int q = 0; q < 36; ++q) M[q] = 0.0;
-	if (act)
-		for (int e = 0; e < R; ++e) {
-			const size_t q = (size_t)sr * R + e;
-			if (ecol[q] != base + wid + bk) continue;
-			const T* blk = Aval + 36 * q;
-#pragma unroll
-			for (int t = 0; t < 36; ++t) M[t] = (double)blk[t];
-			break;
+	const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+	const int base = NW * (int)blockIdx.x;
+	const bool row = w < cnt;
+	// this row's group and the tile's largest group (every wavefront runs
+	// that many steps: groups of different sizes can share a tile)
+	const int lg = row ? gsz[base + w] : 0;
+	int lgmax = 0;
+	for (int k = 0; k < cnt; ++k) lgmax = max(lgmax, gsz[base + k]);
+	const int gs = 1 << lg, g0 = w & ~(gs - 1), bi = w - g0;
+	const int sr = base + w;
+	// the ELL entry of block (bi, lane) of the row, lane < gs (-1: no block):
+	// the lanes read 64 entries of the row at once and find each of the
+	// group's column slots by a ballot
+	int qb = -1;
+	if (row)
+		for (int e0 = 0; e0 < R; e0 += 64) {
+			const int v = e0 + lane < R ? ecol[(size_t)sr * R + e0 + lane] : -2;
+			for (int bk = 0; bk < gs; ++bk) {
+				const unsigned long long hit = __ballot(v == base + g0 + bk);
+				if (hit && lane == bk && qb < 0) qb = sr * R + e0 + (__ffsll((long long)hit) - 1);
+			}
 		}
-	double d0[6];
+	double M[kQ], d0[kQ];
 #pragma unroll
-	for (int t = 0; t < 6; ++t) d0[t] = M[7 * t];
-	double (*pc)[36] = pub[wid];
-	double (*pr)[36] = pub[wid] + kPcgRowsPerWG;
+	for (int q = 0; q < kQ; ++q) {
+		const int el = lane + 64 * q, bk = min(el / 36, NW - 1), rc = el % 36;
+		const int qk = __shfl(qb, bk, 64);
+		M[q] = row && el < 36 * gs && qk >= 0 ? (double)Aval[36 * (size_t)qk + rc] : 0.0;
+		d0[q] = M[q];
+	}
 	bool ok = true;
-	for (int j = 0; j < gs; ++j) {
-		if (act && bk == j)
+	for (int j = 0; j < (1 << lgmax); ++j) {
+		const bool step = row && j < gs;
+		double (*pr)[36] = prow[j & 1];
+		if (step) {
 #pragma unroll
-			for (int t = 0; t < 36; ++t) pc[bi][t] = M[t];
-		if (act && bi == j)
+			for (int q = 0; q < kQ; ++q) {
+				const int el = lane + 64 * q, bk = el / 36, rc = el % 36;
+				if (el < 36 * gs) {
+					if (bi == j) pr[g0 + bk][rc] = M[q];
+					if (bk == j) wcol[w][rc] = M[q];
+				}
+			}
+		}
+		__syncthreads();
+		if (step) {
+			{
+				double Dinv[36];
+				ok = spd6_inverse(pr[g0 + j], Dinv) && ok;
+				if (lane == 0)
 #pragma unroll
-			for (int t = 0; t < 36; ++t) pr[bk][t] = M[t];
-		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-		__builtin_amdgcn_wave_barrier();
-		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-		double Dinv[36];
-		ok = spd6_inverse(pc[j], Dinv) && ok;
-		if (act) {
-			const double* gjk = pr[bk];  // G_jk as published (the lane's own block when bi == j)
-			const double* gij = pc[bi];
-			if (bi == j && bk == j) {
+					for (int t = 0; t < 36; ++t) wd[w][t] = Dinv[t];
+			}
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+			const double* Dinv = wd[w];
+			if (bi != j) {
+				if (lane < 36) {
+					const int r = lane / 6, c = lane % 6;
+					double t = 0;
 #pragma unroll
-				for (int t = 0; t < 36; ++t) M[t] = Dinv[t];
-			} else if (bi == j) {
-				// D^-1 G_jk, row by row
+					for (int k = 0; k < 6; ++k) t += wcol[w][r * 6 + k] * Dinv[k * 6 + c];
+					wx[w][lane] = t;
+				}
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+			}
 #pragma unroll
-				for (int r = 0; r < 6; ++r)
-#pragma unroll
-					for (int c = 0; c < 6; ++c) {
-						double t = 0;
-#pragma unroll
-						for (int k = 0; k < 6; ++k) t += Dinv[r * 6 + k] * gjk[k * 6 + c];
-						M[r * 6 + c] = t;
-					}
-			} else {
-				// row r of X = G_ij D^-1, then G_ik - X G_jk (or -X when k = j)
-#pragma unroll
-				for (int r = 0; r < 6; ++r) {
-					double x[6];
-#pragma unroll
-					for (int c = 0; c < 6; ++c) {
-						double t = 0;
-#pragma unroll
-						for (int k = 0; k < 6; ++k) t += gij[r * 6 + k] * Dinv[k * 6 + c];
-						x[c] = t;
-					}
-#pragma unroll
-					for (int c = 0; c < 6; ++c) {
+			for (int q = 0; q < kQ; ++q) {
+				const int el = lane + 64 * q, bk = el / 36, rc = el % 36, r = rc / 6, c = rc % 6;
+				if (el < 36 * gs) {
+					const double* gjk = pr[g0 + bk];
+					if (bi == j) {
 						if (bk == j) {
-							M[r * 6 + c] = -x[c];
+							M[q] = Dinv[rc];
 						} else {
 							double t = 0;
 #pragma unroll
-							for (int k = 0; k < 6; ++k) t += x[k] * gjk[k * 6 + c];
-							M[r * 6 + c] -= t;
+							for (int k = 0; k < 6; ++k) t += Dinv[r * 6 + k] * gjk[k * 6 + c];
+							M[q] = t;
 						}
+					} else if (bk == j) {
+						M[q] = -wx[w][rc];
+					} else {
+						double t = 0;
+#pragma unroll
+						for (int k = 0; k < 6; ++k) t += wx[w][r * 6 + k] * gjk[k * 6 + c];
+						M[q] -= t;
 					}
 				}
 			}
 		}
-		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-		__builtin_amdgcn_wave_barrier();
-		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 	}
-	if (act) {
+	if (!row) return;
 #pragma unroll
-		for (int r = 0; r < 6; ++r) {
-			T* row = G + kGroupBlock * (size_t)sr + kGroupRow * r + 6 * (wid + bk);
-#pragma unroll
-			for (int c = 0; c < 6; ++c)
-				row[c] = (T)(ok ? M[r * 6 + c] : (bi == bk && r == c ? 1.0 / d0[r] : 0.0));
-		}
+	for (int q = 0; q < kQ; ++q) {
+		const int el = lane + 64 * q, bk = el / 36, rc = el % 36, r = rc / 6, c = rc % 6;
+		if (el < 36 * gs)
+			G[kGroupBlock * (size_t)sr + kGroupRow * r + 6 * (g0 + bk) + c] =
+			    (T)(ok ? M[q] : (bk == bi && r == c ? 1.0 / d0[q] : 0.0));
 	}
 }
 
