@@ -3379,11 +3379,15 @@ static bool pcg_solve_async(pdc_context* ctx, const PcgArgs<T>& pa, int iter, st
 	// chunks was slower (a boundary between replayed graphs costs more than
 	// a launch, and graphs long enough to amortise it queue more launches
 	// past the end of the solve)
+	// (groups of an even length, so that the launches keep alternating the
+	// counter slots; the last group of a target is cut to it, so that fewer
+	// launches run past the end of the solve)
 	auto enqueue_to = [&](int target) {
 		target = std::min(target, maxit);
 		while (launched < target) {
-			pcg_chunk_launches<T>(ctx, pa, nblk, two, chunk, nullptr);
-			launched += chunk;
+			const int n = std::min(chunk, (target - launched + 1) & ~1);
+			pcg_chunk_launches<T>(ctx, pa, nblk, two, n, nullptr);
+			launched += n;
 		}
 	};
 	// kernel timing: one event pair around all the solve's launches (the
