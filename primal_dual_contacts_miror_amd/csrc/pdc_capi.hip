@@ -154,7 +154,7 @@ void free_all(pdc_context* c) {
 	c->A32.release(); c->P32.release(); c->b32.release();
 	c->A64.release(); c->P64.release(); c->b64.release(); c->x64.release(); c->y64.release(); c->pcg64.release();
 	c->d_sc.release();
-	c->b_scene.release(); c->d_sbeg.release(); c->d_step.release(); c->d_act.release();
+	c->b_scene.release(); c->d_sbeg.release(); c->d_step.release(); c->d_act.release(); c->ls_gate.release();
 	c->r_cbeg.release(); c->r_lbeg.release(); c->r_pbeg.release(); c->r_nnzb.release();
 	c->t_c.release(); c->t_b.release(); c->t_w.release(); c->t_p.release(); c->t_pcg.release();
 	c->tb_c.release(); c->tb_b.release(); c->tb_w.release(); c->tb_p.release();

@@ -399,6 +399,7 @@ struct pdc_context {
 	pdc::DBuf<int> d_sbeg;                                    // per scene: first body (S + 1)
 	pdc::DBuf<int> d_step;                                    // per scene: stepped (broadphase, integration)
 	pdc::DBuf<int> d_act;                                     // the scenes of the current solve (ascending)
+	pdc::DBuf<int> ls_gate;                                   // per line-search tile: its scene's search state
 	std::vector<int> h_act;                                   // its host copy (source of the upload)
 	pdc::DBuf<int> r_cbeg, r_lbeg, r_pbeg;                    // per scene: first contact, local body, pair
 	pdc::DBuf<int> r_nnzb;                                    // per scene: Newton-matrix blocks (+ total)

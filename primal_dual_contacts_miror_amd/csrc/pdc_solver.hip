@@ -746,6 +746,9 @@ void build_topology(pdc_context* ctx) {
 struct SArgs {
 	int C, cs, NL, nsc, S;
 	const int* act;       // the scenes in this solve (ascending): workgroup k of a per-scene kernel takes act[k]
+	int* lsgate;          // per line-search tile (contact tiles, then body tiles): its scene's ls_state while
+	                      // the scene runs, else -1 (written by the per-scene kernels that change the state)
+	int ntc;              // contact tiles (the body tiles' gates follow them)
 	int pstride;          // partial-sum slot stride (>= every tile count)
 	double restitution, compliance, reduction, dt, gdt[3];
 	int spring;
@@ -1215,6 +1218,13 @@ __device__ __forceinline__ double ref_max(double a, double b) { return (a < b) ?
 __device__ __forceinline__ double ref_min(double a, double b) { return (b < a) ? b : a; }
 
 // err, eps and the exit test (PrimalDualForceSolver.cpp:295-320); one workgroup per scene.
+// The line-search gates of scene sc's tiles: its state while it runs, else
+// -1 (every thread of the calling workgroup takes a share).
+__device__ __forceinline__ void write_ls_gates(const SArgs& a, int sc, int g) {
+	for (int t = a.tbc[sc] + (int)threadIdx.x; t < a.tbc[sc + 1]; t += blockDim.x) a.lsgate[t] = g;
+	for (int t = a.tbw[sc] + (int)threadIdx.x; t < a.tbw[sc + 1]; t += blockDim.x) a.lsgate[a.ntc + t] = g;
+}
+
 __global__ __launch_bounds__(256) void k_err_final(SArgs a, double tol) {
 	__shared__ double sums[4];
 	const int sc = a.act[blockIdx.x];
@@ -1260,6 +1270,8 @@ __global__ __launch_bounds__(256) void k_err_final(SArgs a, double tol) {
 		a.ps[sc].run = S->run;
 		a.prog[sc].runtag = ((long long)S->it << 32) | (long long)(unsigned)S->run;
 	}
+	__syncthreads();
+	write_ls_gates(a, sc, a.ss[sc].run ? (int)LS_PENDING : -1);
 }
 
 // ---- (c) assembly ----------------------------------------------------------
@@ -2738,8 +2750,7 @@ __device__ __forceinline__ void ls_contact_tile(const SArgs& a, const double* __
                                                 int k0, int want, int tix) {
 	const Tile tile = a.tc[tix];
 	if (tile.scene < 0) return;
-	const int scn = tile.scene;
-	if (!a.ss[scn].run || a.ss[scn].ls_state != want) return;
+	const int scn = tile.scene;  // running and in state `want` (the gate)
 	const int t = blockIdx.y;
 	const double muc = a.ss[scn].muc, mu = a.ctl[scn].mu, cm = a.ss[scn].char_mass;
 	const double step = ldexp(1.0, -(k0 + t));
@@ -2774,8 +2785,7 @@ __device__ __forceinline__ void ls_body_tile(const SArgs& a, const double* __res
                                              int k0, int want, int tix) {
 	const Tile tile = a.tw[tix];
 	if (tile.scene < 0) return;
-	const int scn = tile.scene;
-	if (!a.ss[scn].run || a.ss[scn].ls_state != want) return;
+	const int scn = tile.scene;  // running and in state `want` (the gate)
 	const int t = blockIdx.y;
 	const double mu = a.ctl[scn].mu;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -2830,6 +2840,9 @@ __device__ __forceinline__ void ls_body_tile(const SArgs& a, const double* __res
 __global__ __launch_bounds__(256) void k_ls_eval(SArgs a, const double* __restrict__ du,
                                                  const double* __restrict__ dF, const double* __restrict__ dL,
                                                  int k0, int want, int ntc) {
+	// the tile's gate: a workgroup of a scene that is not in this pass's
+	// state returns after one load
+	if (a.lsgate[blockIdx.x] != want) return;
 	if ((int)blockIdx.x < ntc) ls_contact_tile(a, du, dF, dL, k0, want, blockIdx.x);
 	else ls_body_tile(a, du, dF, dL, k0, want, blockIdx.x - ntc);
 }
@@ -2886,6 +2899,8 @@ __global__ __launch_bounds__(1024) void k_ls_final(SArgs a, int k0, int nt, int 
 			}
 		}
 	}
+	__syncthreads();
+	write_ls_gates(a, sc, a.ss[sc].ls_state);
 }
 
 // Commit the accepted trial point of the scene's search (the regular
@@ -3060,6 +3075,8 @@ static SArgs make_args(pdc_context* ctx, double dt, const double* g) {
 	const pdc_params& p = ctx->p;
 	a.C = ctx->C; a.cs = ctx->c_stride; a.NL = ctx->NL; a.S = ctx->S;
 	a.act = ctx->d_act.p;
+	a.lsgate = ctx->ls_gate.p;
+	a.ntc = ctx->n_tc;
 	a.pstride = std::max(1, std::max(ctx->n_tc, std::max(ctx->n_tb, ctx->n_tw)));
 	a.nsc = p.force_model == PDC_NON_SMOOTH_CONTACT;
 	a.restitution = p.restitution;
@@ -3579,6 +3596,7 @@ void solve(pdc_context* ctx, double dt, const double* g) {
 	}
 	SArgs a0 = make_args(ctx, dt, g);
 	ctx->partials.ensure((size_t)P_COUNT * a0.pstride);
+	ctx->ls_gate.ensure((size_t)std::max(1, ctx->n_tc + ctx->n_tw));
 	SArgs a = make_args(ctx, dt, g);
 	const int ntc = ctx->n_tc, ntb = ctx->n_tb, ntw = ctx->n_tw, ntp = ctx->n_tp;
 
