@@ -271,6 +271,43 @@ def test_pcg_cap_matches_oracle_pcg(require_gpu, pcg_tol):
     assert np.max(np.abs(acc_g - acc_o)) / gdt < 1e-6
 
 
+def test_lattice_over_256_pcg_tiles_matches_oracle_pcg(require_gpu):
+    """A resting lattice of 14 x 13 x 14 spheres (2548 bodies, each pressing
+    into its neighbours and the floor by 0.01), one fillForces call at the
+    tight setting against the oracle's PCG branch (the same groups and
+    tolerance).  More than 2048 bodies in contact put more than 256 PCG tiles
+    in the one scene, so the iteration kernels sum the previous launch's
+    partials past their first 256 tiles.  Bar: the same Newton iteration
+    count, accelerations within 1e-6 |g| dt."""
+    b = _Builder()
+    b.box((8.0, 0.5, 8.0), (0.0, -0.5, 0.0), static=True)
+    r, d = 0.25, 0.49
+    for j in range(13):
+        for i in range(14):
+            for k in range(14):
+                b.sphere(r, ((i - 6.5) * d, r - 0.01 + j * d, (k - 6.5) * d))
+    p = default_params(timestep=0.005, friction_coeff=0.6, tolerance=1e-4, iterations=50, grid_n=32,
+                       sample_spacing=0.05, pcg_precision=PCG_FP64, pcg_tol=1e-12)
+    sc = b.build("lattice", p, 1)
+    o = orc.OracleScene(sc)
+    o.set_threads(16)
+    o.set_linear_solver(orc.SOLVER_PCG)
+    o.detect()
+    co = o.contacts()
+    zero = np.zeros((sc.n, 3))
+    acc_o, _, st_o = o.fill_forces(co, sc.quat, zero, zero)
+    solver = pdc.PrimalDualForceSolver(sc, p)
+    acc_g, _, st_g = solver.fill_forces(co, sc.quat, zero, zero)
+    print("lattice: %d contacts, %d bodies in contact; oracle Newton %d, PCG %d; device Newton %d, PCG %d"
+          % (co.n, st_g.n_local_bodies, st_o.newton_iterations, st_o.pcg_iterations, st_g.newton_iterations,
+             st_g.pcg_iterations))
+    assert st_g.n_local_bodies > 8 * 256
+    assert st_g.nonfinite == 0
+    assert st_g.newton_iterations == st_o.newton_iterations
+    gdt = np.linalg.norm(p.gravity[:]) * p.timestep
+    assert np.max(np.abs(acc_g - acc_o)) / gdt < 1e-6
+
+
 def _traj_errors(sc, o_state, g_state):
     pos_o, q_o = o_state[0], o_state[1]
     pos_g, q_g = g_state[0], g_state[1]
